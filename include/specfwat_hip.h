@@ -1,0 +1,177 @@
+/*
+ * specfwat_hip.h -- C ABI of libspecfwat_hip.so, the MI355X (gfx950) engine for
+ * SpecFWAT's spectral-element forward/adjoint/kernel path and its PDE kernel
+ * smoother.
+ *
+ * Calling convention (the reference's own, smooth_sem_pde.cu:298-316 and
+ * include/config.h:13-16): Fortran-callable names = lowercase + '_', every
+ * argument passed by reference, arrays in Fortran column-major order,
+ * element / point indices 1-based, logical flags as int (non-zero = true).
+ * realw is float (CUSTOM_REAL single precision).  All functions return void
+ * and abort the process with a message on error, as the reference's GPU code
+ * and exit_MPI do.
+ *
+ * Device state lives behind `long *Mesh_pointer`, SPECFEM's opaque handle
+ * that SpecFWAT stores and hands to prepare_cleanup_device
+ * (src/specfem_interface/specfem_api.f90:303).
+ *
+ * Two groups of entry points:
+ *  (1) names of the SPECFEM3D GPU C-ABI that SpecFWAT's solver path reaches
+ *      (prepare_GPU at src/specfem_interface/prepare_timerun_fwat.f90:120,
+ *      iterate_time at src/preproc/prepare_fwd.f90:364, save_adjoint_kernels
+ *      at src/preproc/prepare_fwd.f90:381).  The SPECFEM3D sources are an
+ *      un-vendored submodule (.gitmodules:1-4), so their exact argument lists
+ *      are not in the reference tree; the lists below are this library's and
+ *      carry SPECFEM's arrays under SPECFEM's names.
+ *  (2) fwat_* additions with no counterpart in the reference: the whole-run
+ *      device loop, in-HBM hand-off between the forward and adjoint runs,
+ *      mixed-precision statics and batched smoothing.
+ * smooth_pde_cuda_ keeps the reference's exact 42-argument signature.
+ */
+#ifndef SPECFWAT_HIP_H
+#define SPECFWAT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef float realw;
+
+/* ---- device selection: initialize_GPU, src/specfem_interface/initialize_simulation.f90:300 */
+void initialize_gpu_device_(int *runtime_type, int *myrank, int *ncuda_devices);
+
+/* ---- preparation: prepare_GPU, prepare_timerun_fwat.f90:120 */
+/* Mesh constants, GLL tables, Stacey faces (read_mesh_databases.f90:766-792),
+ * sources (setup_sources_receivers_fwat.f90:41-56) and receivers.
+ * h_sourcearrays (NDIM,NGLLX,NGLLY,NGLLZ,NSOURCES); h_hlagrange_rec (125,nrec). */
+void prepare_constants_device_(long *Mesh_pointer, int *h_NGLLX, int *NSPEC_AB, int *NGLOB_AB,
+                               int *NSPEC_IRREGULAR, realw *h_xix, realw *h_xiy, realw *h_xiz,
+                               realw *h_etax, realw *h_etay, realw *h_etaz, realw *h_gammax,
+                               realw *h_gammay, realw *h_gammaz, realw *h_jacobian,
+                               realw *xix_regular, realw *jacobian_regular, int *h_ibool,
+                               int *h_irregular_element_number, realw *h_hprime_xx,
+                               realw *h_hprimewgll_xx, realw *h_wgllwgll_xy, realw *h_wgllwgll_xz,
+                               realw *h_wgllwgll_yz, int *ABSORBING_CONDITIONS,
+                               int *h_num_abs_boundary_faces, int *h_abs_boundary_ispec,
+                               int *h_abs_boundary_ijk, realw *h_abs_boundary_normal,
+                               realw *h_abs_boundary_jacobian2Dw, int *NSOURCES,
+                               realw *h_sourcearrays, int *h_ispec_selected_source, int *nrec,
+                               int *h_ispec_selected_rec, realw *h_hlagrange_rec, int *NSTEP,
+                               int *SIMULATION_TYPE, int *SAVE_FORWARD);
+
+/* Mass matrices (inverted, read_mesh_databases.f90:323-333,802-804), Stacey
+ * impedances and moduli (get_model_fwat.f90:305-317), 21 c_ij when
+ * ANISOTROPY, inner/outer phases (read_mesh_databases.f90:1129-1143). */
+void prepare_fields_elastic_device_(long *Mesh_pointer, realw *rmassx, realw *rmassy, realw *rmassz,
+                                    realw *rho_vp, realw *rho_vs, realw *kappav, realw *muv,
+                                    int *ANISOTROPY, realw *c11store, realw *c12store,
+                                    realw *c13store, realw *c14store, realw *c15store,
+                                    realw *c16store, realw *c22store, realw *c23store,
+                                    realw *c24store, realw *c25store, realw *c26store,
+                                    realw *c33store, realw *c34store, realw *c35store,
+                                    realw *c36store, realw *c44store, realw *c45store,
+                                    realw *c46store, realw *c55store, realw *c56store,
+                                    realw *c66store, int *num_phase_ispec_elastic,
+                                    int *phase_ispec_inner_elastic, int *nspec_outer_elastic,
+                                    int *nspec_inner_elastic, int *COMPUTE_AND_STORE_STRAIN);
+
+/* Adjoint-run fields: b_ wavefield (read_mesh_databases.f90:1616-1622) and
+ * kernels rho/mu/kappa or cijkl (read_mesh_databases.f90:1631-1677). */
+void prepare_fields_elastic_adj_dev_(long *Mesh_pointer, int *size, int *COMPUTE_AND_STORE_STRAIN,
+                                     realw *rhostore, int *ANISOTROPIC_KL, int *APPROXIMATE_HESS_KL);
+void prepare_sim2_or_3_const_device_(long *Mesh_pointer, int *nadj_rec_local,
+                                     int *NTSTEP_BETWEEN_READ_ADJSRC);
+
+/* ---- per-step solver stages (iterate_time, prepare_fwd.f90:364) */
+/* FORWARD_OR_ADJOINT: 1 forward/adjoint field, 3 reconstructed b_ field */
+void update_displacement_cuda_(long *Mesh_pointer, realw *deltat, realw *deltatsqover2,
+                               realw *deltatover2, int *FORWARD_OR_ADJOINT);
+void compute_forces_viscoelastic_cuda_(long *Mesh_pointer, int *iphase, realw *deltat,
+                                       int *nspec_outer_elastic, int *nspec_inner_elastic,
+                                       int *COMPUTE_AND_STORE_STRAIN, int *ATTENUATION,
+                                       int *FORWARD_OR_ADJOINT);
+/* forward SAVE_FORWARD run stores record it_record; the b_ field replays it
+ * (absorb_field.bin, specfem_api.f90:359-361) */
+void compute_stacey_viscoelastic_cuda_(long *Mesh_pointer, int *iphase, int *it_record,
+                                       int *FORWARD_OR_ADJOINT);
+void compute_add_sources_viscoelastic_cuda_(long *Mesh_pointer, int *NSOURCES,
+                                            double *h_stf_pre_compute, int *FORWARD_OR_ADJOINT);
+/* h_source_adjoint (NDIM,nrec): adjoint sources of SEM/NET.STA.BX?.adj (syn_data.f90:232-251) */
+void add_sources_el_sim_type_2_or_3_(long *Mesh_pointer, int *nrec, realw *h_source_adjoint, int *it);
+void kernel_3_a_cuda_(long *Mesh_pointer, realw *deltatover2, int *FORWARD_OR_ADJOINT);
+void compute_seismograms_cuda_(long *Mesh_pointer, int *it);
+/* seismograms (NDIM,nrec,NSTEP), consumed as seismograms_d (syn_data.f90:64-66) */
+void transfer_seismograms_from_device_(long *Mesh_pointer, realw *h_seismograms);
+
+/* ---- kernels (compute_kernels / save_adjoint_kernels, prepare_fwd.f90:381) */
+void compute_kernels_elastic_cuda_(long *Mesh_pointer, realw *deltat);
+void transfer_kernels_el_to_host_(long *Mesh_pointer, realw *h_rho_kl, realw *h_mu_kl,
+                                  realw *h_kappa_kl, int *NSPEC_AB);
+void transfer_kernels_el_to_host_dp_(long *Mesh_pointer, double *h_rho_kl, double *h_mu_kl,
+                                     double *h_kappa_kl, int *NSPEC_AB);
+/* cijkl_kl (21,NGLLX,NGLLY,NGLLZ,NSPEC_AB), read as c11..c66 kernels by kernel_io.f90:100-156 */
+void transfer_kernels_ani_to_host_dp_(long *Mesh_pointer, double *h_cijkl_kl, int *NSPEC_AB);
+void transfer_kernels_hess_el_tohost_dp_(long *Mesh_pointer, double *h_hess_kl, int *NSPEC_AB);
+
+/* ---- field transfers (save_forward_arrays.bin, specfem_api.f90:312-358) */
+void transfer_fields_el_to_device_(int *size, realw *displ, realw *veloc, realw *accel, long *Mesh_pointer);
+void transfer_fields_el_from_device_(int *size, realw *displ, realw *veloc, realw *accel, long *Mesh_pointer);
+void transfer_b_fields_to_device_(int *size, realw *b_displ, realw *b_veloc, realw *b_accel, long *Mesh_pointer);
+void transfer_b_fields_from_device_(int *size, realw *b_displ, realw *b_veloc, realw *b_accel, long *Mesh_pointer);
+/* Stacey store (NDIM,NGLLSQUARE,num_abs_boundary_faces,NSTEP) <-> host */
+void transfer_stacey_store_from_device_(long *Mesh_pointer, realw *h_store);
+void transfer_stacey_store_to_device_(long *Mesh_pointer, realw *h_store);
+
+/* ---- clean-up: FinalizeSpecfem, specfem_api.f90:303 */
+void prepare_cleanup_device_(long *Mesh_pointer, int *ACOUSTIC_SIMULATION, int *ELASTIC_SIMULATION,
+                             int *NOISE_TOMOGRAPHY);
+
+/* ---- additions (no counterpart in the reference) */
+/* whole-run source time functions (NSOURCES,NSTEP) and adjoint traces (NDIM,nrec,NSTEP) */
+void fwat_set_stf_(long *Mesh_pointer, double *h_stf, int *NSTEP);
+void fwat_set_adjoint_sources_(long *Mesh_pointer, realw *h_adj, int *NSTEP);
+/* a whole SIMULATION_TYPE 1 or 3 run on the device; fused 1 = fused Newmark and
+ * two-field force pass with in-pass fp64 kernel accumulation, 0 = SPECFEM stage order */
+void fwat_iterate_time_(long *Mesh_pointer, int *SIMULATION_TYPE, realw *DT, int *fused, int *timed);
+void fwat_last_force_time_(long *Mesh_pointer, double *ms, int *launches);
+void fwat_synchronize_(long *Mesh_pointer);
+/* plain device copy of nbytes repeated reps times (counter calibration probe) */
+void fwat_copy_probe_(long *nbytes, int *reps, double *avg_ms);
+/* in-HBM replacements of save_forward_arrays.bin and absorb_field.bin */
+void fwat_copy_forward_to_backward_(long *Mesh_pointer_from, long *Mesh_pointer_to);
+void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);
+/* 0 fp32, 1 bf16, 2 fp16 with per-array scaling, for kappa/mu and the Jacobian */
+void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
+
+/* ---- PDE smoothing: smooth_pde_cuda_, src/postproc/smooth_sem_pde.cu:298-316,
+ *      called from src/postproc/smooth.f90:392-402 (exact reference signature) */
+void smooth_pde_cuda_(int *h_is_sph, int *h_nspec, int *h_nglob, int *nprocs, int *h_nstep,
+                      int *nspec_max, int *phase_ispec_inner_elastic, int *h_nspec_outer,
+                      int *h_nspec_inner, const realw *xix, const realw *xiy, const realw *xiz,
+                      const realw *etax, const realw *etay, const realw *etaz, const realw *gamx,
+                      const realw *gamy, const realw *gamz, const realw *jaco, const realw *rotate0,
+                      const realw *wgllwgll_xy, const realw *wgllwgll_xz, const realw *wgllwgll_yz,
+                      const realw *hprimeT, const realw *hprime_wgll, const int *ibool,
+                      const int *is_CPML, const realw *h_cv, const realw *h_ch, int *h_num_intfs,
+                      int *h_max_nibool, const int *my_neighbors, const int *nibool_intf,
+                      const int *ibool_intf, const realw *dat_bak, realw *dat, const realw *rvol,
+                      realw *dat_glob, realw *ddat_glob, const int *irregular_element_number,
+                      const realw *h_xix_regular, const int *h_nspec_irregular);
+/* 1 (default) = reference arithmetic on regular elements, 0 = diagonal inverse Jacobian */
+void fwat_smooth_set_faithful_(int *faithful);
+/* nfield kernels smoothed together; dat_glob (NGLOB_AB,nfield) */
+void fwat_smooth_pde_batch_(int *h_is_sph, int *h_nspec, int *h_nglob, int *h_nstep, int *h_nfield,
+                            const realw *xix, const realw *xiy, const realw *xiz, const realw *etax,
+                            const realw *etay, const realw *etaz, const realw *gamx, const realw *gamy,
+                            const realw *gamz, const realw *jaco, const realw *rotate0,
+                            const realw *wgllwgll_xy, const realw *wgllwgll_xz, const realw *wgllwgll_yz,
+                            const realw *hprimeT, const realw *hprime_wgll, const int *ibool,
+                            const realw *h_cv, const realw *h_ch, const realw *rvol, realw *dat_glob,
+                            const int *irregular_element_number, const realw *h_xix_regular,
+                            const int *h_nspec_irregular, int *timed);
+void fwat_smooth_last_time_(double *elem_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECFWAT_HIP_H */

@@ -1,0 +1,356 @@
+"""ctypes front end of the CPU oracle (liboracle_sem.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+The product package never imports this module.
+
+Function-to-reference map (file:line in the SpecFWAT tree):
+  Oracle.forward / Oracle.adjoint  SPECFEM3D elastic time loop driven by
+                                   src/preproc/prepare_fwd.f90:328-368 (solver
+                                   source absent: restated, pinned by invariants)
+  smooth_pde                       src/postproc/smooth.f90:7-437, 603-689
+  interp3                          src/shared/utils.f90:1162-1232
+  hti_cij                          src/patch/aniso.f90:55-162
+  simpson                          src/shared/utils.f90:1630-1663
+  window_taper                     src/shared/signal.f90:419-491
+  waveform_adjoint                 src/preproc/adjoint_sources/waveform_misfit.f90:15-64
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_sem.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nex", C.c_int), ("ney", C.c_int), ("nez", C.c_int),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("perturb", C.c_double),
+        ("force_irregular", C.c_int),
+        ("numbering", C.c_int),
+        ("aniso", C.c_int),
+        ("rho", _fp), ("vp", _fp), ("vs", _fp), ("cij", _fp),
+        ("dt", C.c_double),
+        ("nstep", C.c_int),
+        ("stacey", C.c_int),
+        ("nsrc", C.c_int),
+        ("src_xyz", _dp), ("src_mt", _dp), ("src_f0", _dp), ("src_tshift", _dp),
+        ("nrec", C.c_int),
+        ("rec_xyz", _dp),
+        ("nthreads", C.c_int),
+    ]
+
+
+def build(force=False):
+    """Compile the oracle with g++ (OpenMP, portable x86-64-v3 code)."""
+    src = os.path.join(_HERE, "sem_oracle.cpp")
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+        import subprocess
+        cmd = ["g++", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
+               "-o", _LIB_PATH, src]
+        subprocess.check_call(cmd)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        for n in ("orc_nglob", "orc_nfaces", "orc_nspec_irregular", "orc_ncolors"):
+            getattr(L, n).restype = C.c_int
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.orc_get_mesh.argtypes = [C.c_void_p, _ip, _dp, _ip, _dp, _dp, _dp, _ip]
+        L.orc_get_faces.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
+        L.orc_get_gll.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_lagrange.argtypes = [C.c_double, _dp, _dp]
+        L.orc_get_sources.argtypes = [C.c_void_p, _ip, _dp, _dp, _ip, _dp, _dp]
+        L.orc_get_rmass.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.orc_forward.argtypes = [C.c_void_p, _fp, C.c_int]
+        L.orc_get_final.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.orc_set_final.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        L.orc_get_stacey.argtypes = [C.c_void_p, _fp]
+        L.orc_adjoint.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _dp, _fp]
+        L.orc_energy_series.argtypes = [C.c_void_p, _fp, _dp]
+        L.orc_apply_k.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_ricker.restype = C.c_double
+        L.orc_ricker.argtypes = [C.c_double, C.c_double]
+        L.orc_locate_bissection.restype = C.c_int
+        L.orc_locate_bissection.argtypes = [_dp, C.c_int, C.c_double]
+        L.orc_interp3.restype = C.c_float
+        L.orc_interp3.argtypes = [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int, _fp,
+                                  C.c_float, C.c_float, C.c_float]
+        L.orc_hti_cij.argtypes = [C.c_float] * 5 + [_fp]
+        L.orc_simpson.restype = C.c_double
+        L.orc_simpson.argtypes = [_dp, C.c_int, C.c_double]
+        L.orc_window_taper.argtypes = [_dp, C.c_int, C.c_double, C.c_int]
+        L.orc_waveform_adjoint.restype = C.c_double
+        L.orc_waveform_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int,
+                                           C.c_double, C.c_int, _dp, _dp, _dp]
+        L.orc_smooth_pde.restype = C.c_int
+        L.orc_smooth_pde.argtypes = ([C.c_int, C.c_int, _ip, _fp, _fp, _fp, C.c_int]
+                                     + [_fp] * 10 + [_ip, C.c_float, C.c_float, C.c_int, C.c_int,
+                                                     C.c_int, _ip, _ip, _fp, C.c_float, C.c_float,
+                                                     C.c_float, C.c_int, C.c_int, C.c_int, _fp, _fp])
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def gll():
+    xi, w, hp, hpw = (np.zeros(5), np.zeros(5), np.zeros(25), np.zeros(25))
+    lib().orc_get_gll(_p(xi, C.c_double), _p(w, C.c_double), _p(hp, C.c_double), _p(hpw, C.c_double))
+    return xi, w, hp.reshape(5, 5), hpw.reshape(5, 5)
+
+
+def lagrange(x):
+    h, hd = np.zeros(5), np.zeros(5)
+    lib().orc_lagrange(float(x), _p(h, C.c_double), _p(hd, C.c_double))
+    return h, hd
+
+
+def ricker(t, f0):
+    return lib().orc_ricker(float(t), float(f0))
+
+
+class Oracle:
+    """One simulation set-up (mesh, model, source, receivers) in FP32 or FP64.
+
+    rho/vp/vs are float32 arrays [nspec*125] (local points, element-major,
+    i fastest); cij is float32 [nspec*125, 21] for the anisotropic case.
+    """
+
+    def __init__(self, nex, ney, nez, lx, ly, lz, rho, vp, vs, *, cij=None, dt=0.005, nstep=10,
+                 stacey=True, src_xyz=(), src_mt=(), src_f0=(), src_tshift=None, rec_xyz=(),
+                 perturb=0.0, force_irregular=False, numbering=1, precision=32, nthreads=0):
+        self._keep = []
+
+        def f32(a):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            self._keep.append(a)
+            return a
+
+        def f64(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self._keep.append(a)
+            return a
+
+        src_xyz = f64(np.reshape(src_xyz, (-1, 3)))
+        nsrc = src_xyz.shape[0]
+        src_mt = f64(np.reshape(src_mt, (nsrc, 6)))
+        src_f0 = f64(np.reshape(src_f0, (nsrc,)))
+        src_tshift = f64(np.zeros(nsrc) if src_tshift is None else np.reshape(src_tshift, (nsrc,)))
+        rec_xyz = f64(np.reshape(rec_xyz, (-1, 3)))
+        self.nrec = rec_xyz.shape[0]
+        self.nsrc = nsrc
+        rho, vp, vs = f32(rho), f32(vp), f32(vs)
+        cijp = f32(cij) if cij is not None else None
+        self.params = OrcParams(
+            nex, ney, nez, lx, ly, lz, perturb, int(force_irregular), numbering, int(cij is not None),
+            _p(rho, C.c_float), _p(vp, C.c_float), _p(vs, C.c_float), _p(cijp, C.c_float),
+            dt, nstep, int(stacey), nsrc, _p(src_xyz, C.c_double), _p(src_mt, C.c_double),
+            _p(src_f0, C.c_double), _p(src_tshift, C.c_double), self.nrec, _p(rec_xyz, C.c_double),
+            nthreads)
+        self.nspec = nex * ney * nez
+        self.nstep = nstep
+        self.aniso = cij is not None
+        self.h = lib().orc_create(C.byref(self.params), precision)
+        self.nglob = lib().orc_nglob(self.h)
+        self.nfaces = lib().orc_nfaces(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_destroy(self.h)
+            self.h = None
+
+    def mesh(self):
+        nloc = self.nspec * 125
+        ibool = np.zeros(nloc, np.int32)
+        xyz = np.zeros((self.nglob, 3))
+        irreg = np.zeros(self.nspec, np.int32)
+        reg = np.zeros(2)
+        jinv = np.zeros((nloc, 9))
+        jac = np.zeros(nloc)
+        color = np.zeros(self.nspec, np.int32)
+        lib().orc_get_mesh(self.h, _p(ibool, C.c_int), _p(xyz, C.c_double), _p(irreg, C.c_int),
+                           _p(reg, C.c_double), _p(jinv, C.c_double), _p(jac, C.c_double),
+                           _p(color, C.c_int))
+        return dict(ibool=ibool, xyz=xyz, irregular=irreg, xix_regular=reg[0],
+                    jacobian_regular=reg[1], jinv=jinv, jacobian=jac, color=color)
+
+    def faces(self):
+        nf = self.nfaces
+        ispec = np.zeros(nf, np.int32)
+        ijk = np.zeros((nf, 25, 3), np.int32)
+        normal = np.zeros((nf, 25, 3))
+        jw = np.zeros((nf, 25))
+        lib().orc_get_faces(self.h, _p(ispec, C.c_int), _p(ijk, C.c_int), _p(normal, C.c_double),
+                            _p(jw, C.c_double))
+        return dict(ispec=ispec, ijk=ijk, normal=normal, jacobian2Dw=jw)
+
+    def sources(self):
+        si = np.zeros(self.nsrc, np.int32)
+        sr = np.zeros((self.nsrc, 3))
+        sa = np.zeros((self.nsrc, 125, 3))
+        ri = np.zeros(self.nrec, np.int32)
+        rr = np.zeros((self.nrec, 3))
+        rh = np.zeros((self.nrec, 125))
+        lib().orc_get_sources(self.h, _p(si, C.c_int), _p(sr, C.c_double), _p(sa, C.c_double),
+                              _p(ri, C.c_int), _p(rr, C.c_double), _p(rh, C.c_double))
+        return dict(src_ispec=si, src_ref=sr, sourcearrays=sa, rec_ispec=ri, rec_ref=rr, rec_h=rh)
+
+    def rmass_inv(self):
+        out = [np.zeros(self.nglob, np.float32) for _ in range(3)]
+        lib().orc_get_rmass(self.h, *[_p(o, C.c_float) for o in out])
+        return out
+
+    def forward(self, save=True):
+        seis = np.zeros((self.nstep, self.nrec, 3), np.float32)
+        lib().orc_forward(self.h, _p(seis, C.c_float), int(save))
+        return seis
+
+    def final_state(self):
+        out = [np.zeros((self.nglob, 3), np.float32) for _ in range(3)]
+        lib().orc_get_final(self.h, *[_p(o, C.c_float) for o in out])
+        return out
+
+    def stacey_store(self):
+        buf = np.zeros((self.nstep, self.nfaces, 25, 3), np.float32)
+        lib().orc_get_stacey(self.h, _p(buf, C.c_float))
+        return buf
+
+    def adjoint(self, adj):
+        """adj: float32 [nstep, nrec, 3] adjoint source traces (forward time order)."""
+        adj = np.ascontiguousarray(adj, dtype=np.float32)
+        nloc = self.nspec * 125
+        rho = np.zeros(nloc)
+        mu = np.zeros(nloc)
+        kap = np.zeros(nloc)
+        cij = np.zeros((nloc, 21)) if self.aniso else None
+        bfin = np.zeros((3, self.nglob, 3), np.float32)
+        lib().orc_adjoint(self.h, _p(adj, C.c_float), _p(rho, C.c_double), _p(mu, C.c_double),
+                          _p(kap, C.c_double), _p(cij, C.c_double), _p(bfin, C.c_float))
+        out = dict(rho_kl=rho, b_final=bfin)
+        if self.aniso:
+            out["cijkl_kl"] = cij
+        else:
+            out["mu_kl"] = mu
+            out["kappa_kl"] = kap
+        return out
+
+    def energy_series(self, u0):
+        u0 = np.ascontiguousarray(u0, dtype=np.float32)
+        e = np.zeros(self.nstep)
+        lib().orc_energy_series(self.h, _p(u0, C.c_float), _p(e, C.c_double))
+        return e
+
+    def apply_k(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        f = np.zeros_like(u)
+        lib().orc_apply_k(self.h, _p(u, C.c_double), _p(f, C.c_double))
+        return f
+
+
+def interp3(vx, vy, vz, valin_fortran, x, y, z):
+    vx = np.ascontiguousarray(vx, np.float32)
+    vy = np.ascontiguousarray(vy, np.float32)
+    vz = np.ascontiguousarray(vz, np.float32)
+    v = np.asfortranarray(valin_fortran, np.float32).ravel(order="F")
+    return lib().orc_interp3(_p(vx, C.c_float), len(vx), _p(vy, C.c_float), len(vy),
+                             _p(vz, C.c_float), len(vz), _p(v, C.c_float), x, y, z)
+
+
+def locate_bissection(valx, x):
+    valx = np.ascontiguousarray(valx, np.float64)
+    return lib().orc_locate_bissection(_p(valx, C.c_double), len(valx), float(x))
+
+
+def hti_cij(vp, vs, rho, gcp, gsp):
+    out = np.zeros(21, np.float32)
+    lib().orc_hti_cij(vp, vs, rho, gcp, gsp, _p(out, C.c_float))
+    return out
+
+
+def simpson(y, dx):
+    y = np.ascontiguousarray(y, np.float64)
+    return lib().orc_simpson(_p(y, C.c_double), len(y), float(dx))
+
+
+def window_taper(sig, pct, itype):
+    sig = np.array(sig, np.float64)
+    lib().orc_window_taper(_p(sig, C.c_double), len(sig), float(pct), int(itype))
+    return sig
+
+
+def waveform_adjoint(dat, syn, dt, windows, taper_pct=0.05, itaper=1):
+    dat = np.ascontiguousarray(dat, np.float64)
+    syn = np.ascontiguousarray(syn, np.float64)
+    win = np.ascontiguousarray(np.reshape(windows, (-1, 2)), np.float64)
+    adj = np.zeros(len(dat))
+    mf = np.zeros(win.shape[0])
+    rs = np.zeros(win.shape[0])
+    tot = lib().orc_waveform_adjoint(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt,
+                                     _p(win, C.c_double), win.shape[0], taper_pct, itaper,
+                                     _p(adj, C.c_double), _p(mf, C.c_double), _p(rs, C.c_double))
+    return tot, adj, mf, rs
+
+
+def smooth_pde(mesh, dat_in, sigma_h, sigma_v, distance_min_glob, *, is_sph=False, faithful=True,
+               max_steps=0, is_cpml=None, phase=None):
+    """Loop-for-loop restatement of smooth_sem_pde's CPU branch.
+
+    mesh: dict with nspec, nglob, ibool (1-based, [nspec*125]), xstore/ystore/zstore,
+    xix..gammaz/jacobian ([nspec_irregular*125] each), irregular_element_number,
+    xix_regular, jacobian_regular.  Returns (dat [nspec*125], dat_glob, ntstep).
+    """
+    nspec, nglob = mesh["nspec"], mesh["nglob"]
+    keep = []
+
+    def f(a):
+        a = np.ascontiguousarray(a, np.float32)
+        keep.append(a)
+        return _p(a, C.c_float)
+
+    def i(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return _p(a, C.c_int)
+
+    if phase is None:
+        nspec_outer, nspec_inner = 0, nspec
+        num_phase = nspec
+        ph = np.zeros((2, nspec), np.int32)
+        ph[1, :] = np.arange(1, nspec + 1)
+    else:
+        nspec_outer, nspec_inner, ph = phase
+        num_phase = ph.shape[1]
+    cpml = np.zeros(nspec, np.int32) if is_cpml is None else np.asarray(is_cpml, np.int32)
+    dat = np.zeros(nspec * 125, np.float32)
+    dglob = np.zeros(nglob, np.float32)
+    nsi = int(mesh.get("nspec_irregular", 0))
+    n = lib().orc_smooth_pde(
+        nspec, nglob, i(mesh["ibool"]), f(mesh["xstore"]), f(mesh["ystore"]), f(mesh["zstore"]), nsi,
+        *[f(mesh[k]) for k in ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay",
+                               "gammaz", "jacobian")],
+        i(mesh["irregular_element_number"]), float(mesh["xix_regular"]),
+        float(mesh["jacobian_regular"]), nspec_outer, nspec_inner, num_phase, i(ph), i(cpml),
+        f(dat_in), float(sigma_h), float(sigma_v), float(distance_min_glob), int(is_sph),
+        int(faithful), int(max_steps), _p(dat, C.c_float), _p(dglob, C.c_float))
+    return dat, dglob, n

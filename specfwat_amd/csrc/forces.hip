@@ -1,0 +1,368 @@
+// Elastic internal forces, accel -= K u, for SPECFEM-style NGLL=5 hexahedra on
+// CDNA4 (gfx950).
+//
+// Replaces the solver's internal-force kernel that SpecFWAT reaches through
+// iterate_time (src/preproc/prepare_fwd.f90:364; SPECFEM3D
+// compute_forces_viscoelastic, source absent from the tree).  The arithmetic
+// is the GLL weak form:
+//   gradient     du/dxi_l  = sum_m u(m) hprime(l,m)              (3 directions x 3 components)
+//   strain/stress (isotropic kappa,mu or 21 c_ij, Voigt 1..6 = xx yy zz yz xz xy)
+//   flux         t_d = jacobian * sigma . grad(xi_d)
+//   divergence   f = wgllwgll_yz sum t_1 hprimewgll + wgllwgll_xz sum t_2 .. + wgllwgll_xy sum t_3 ..
+//   scatter      accel(iglob) -= f
+//
+// Work mapping: one element per 128-lane workgroup (two wave64s, 125 active
+// lanes, lane L = i + 5j + 25k).  The element's displacement (3x125 fp32) and
+// its nine flux fields (9x125) are staged in LDS; the 5x5 derivative tables
+// live in registers.  Workgroups are remapped so each XCD owns a contiguous
+// run of elements (shared GLL points then meet in that XCD's L2).  Assembly is
+// an fp32 atomic add to HBM (global_atomic_add_f32).
+//
+// Template switches:
+//   NF    1 = one field, 2 = adjoint + reconstructed field in the same pass
+//   ANISO 21 c_ij instead of kappa/mu
+//   KMODE 0 = forces only, 1 = also store the strain (SPECFEM-style kernel
+//         path), 2 = accumulate the mu/kappa (or c_ij) kernels in fp64 in the
+//         same pass (requires NF == 2)
+//   SP    storage precision of kappa/mu/Jacobian: fp32, bf16, fp16 (scaled)
+//   ASM   0 = fp32 atomic add into accel (SPECFEM's scatter), 1 = write the
+//         element-local forces to floc[ispec][L][3]; a per-point gather over
+//         a static CSR then assembles them in a fixed order (no atomics)
+#include "sem_device.h"
+
+namespace {
+
+struct ForceArgs {
+  const int* ibool;
+  const int* irregular;
+  const realw* jinv10;
+  const uint16_t* jinv10_lp;
+  const realw* kappa;
+  const realw* mu;
+  const uint16_t* kmu_lp;
+  const realw* cij;
+  const realw* tables;
+  const int* elem_list;
+  int elem_begin, elem_count;
+  int nblocks_padded;
+  realw xix_regular, jacobian_regular;
+  float scale_k, scale_m, scale_jinv, scale_jac;
+  const realw* u0;
+  realw* a0;
+  const realw* u1;
+  realw* a1;
+  realw* eps0;
+  realw* floc0;
+  realw* floc1;
+  double* mu_kl;
+  double* kappa_kl;
+  double* cijkl_kl;
+  double dt;
+};
+
+__device__ __forceinline__ float bf16_to_f(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
+__device__ __forceinline__ float h16_to_f(uint16_t x) {
+  _Float16 h;
+  __builtin_memcpy(&h, &x, 2);
+  return (float)h;
+}
+template <int SP>
+__device__ __forceinline__ float lp_to_f(uint16_t x) {
+  return SP == SEM_STATIC_BF16 ? bf16_to_f(x) : h16_to_f(x);
+}
+
+template <int NF, bool ANISO, int KMODE, int SP, int ASM>
+__global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
+  __shared__ realw s_tab[128];
+  __shared__ realw s_u[3][128];
+  __shared__ realw s_t[9][128];
+
+  const int t = threadIdx.x;
+  if (t < 125) s_tab[t] = A.tables[t];
+  const int lb = xcd_remap(blockIdx.x, A.nblocks_padded);
+  if (lb >= A.elem_count) return;  // block-uniform
+  const int ispec = A.elem_list ? A.elem_list[A.elem_begin + lb] : A.elem_begin + lb;
+  const bool act = t < 125;
+  const int tt = act ? t : 124;
+  const int I = tt % 5, J = (tt / 5) % 5, K = tt / 25;
+  __syncthreads();
+
+  realw hpI[5], hpJ[5], hpK[5], hwI[5], hwJ[5], hwK[5];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    hpI[l] = s_tab[I + 5 * l];
+    hpJ[l] = s_tab[J + 5 * l];
+    hpK[l] = s_tab[K + 5 * l];
+    hwI[l] = s_tab[25 + l + 5 * I];
+    hwJ[l] = s_tab[25 + l + 5 * J];
+    hwK[l] = s_tab[25 + l + 5 * K];
+  }
+  const realw fac1 = s_tab[50 + J + 5 * K];
+  const realw fac2 = s_tab[75 + I + 5 * K];
+  const realw fac3 = s_tab[100 + I + 5 * J];
+
+  const size_t Lg = (size_t)ispec * 125 + tt;
+  const int iglob = A.ibool[Lg];
+
+  // geometry
+  const int ir = A.irregular[ispec];
+  realw xix, xiy, xiz, etx, ety, etz, gax, gay, gaz, jl;
+  if (ir) {
+    const size_t base = (size_t)(ir - 1) * 1250 + tt;
+    if (SP == SEM_STATIC_FP32) {
+      const realw* G = A.jinv10 + base;
+      xix = G[0]; xiy = G[125]; xiz = G[250];
+      etx = G[375]; ety = G[500]; etz = G[625];
+      gax = G[750]; gay = G[875]; gaz = G[1000];
+      jl = G[1125];
+    } else {
+      const uint16_t* G = A.jinv10_lp + base;
+      const float sj = A.scale_jinv;
+      xix = lp_to_f<SP>(G[0]) * sj; xiy = lp_to_f<SP>(G[125]) * sj; xiz = lp_to_f<SP>(G[250]) * sj;
+      etx = lp_to_f<SP>(G[375]) * sj; ety = lp_to_f<SP>(G[500]) * sj; etz = lp_to_f<SP>(G[625]) * sj;
+      gax = lp_to_f<SP>(G[750]) * sj; gay = lp_to_f<SP>(G[875]) * sj; gaz = lp_to_f<SP>(G[1000]) * sj;
+      jl = lp_to_f<SP>(G[1125]) * A.scale_jac;
+    }
+  } else {
+    xix = A.xix_regular; ety = A.xix_regular; gaz = A.xix_regular;
+    xiy = xiz = etx = etz = gax = gay = 0.0f;
+    jl = A.jacobian_regular;
+  }
+
+  // material
+  realw kl = 0, ml = 0;
+  const realw* C = nullptr;
+  if (!ANISO) {
+    if (SP == SEM_STATIC_FP32) {
+      kl = A.kappa[Lg];
+      ml = A.mu[Lg];
+    } else {
+      const uint32_t pk = reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg];
+      kl = lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
+      ml = lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
+    }
+  } else {
+    C = A.cij + (size_t)ispec * 21 * 125 + tt;
+  }
+
+  realw ea[6];  // strain of field 0 (KMODE 2): exx eyy ezz exy exz eyz
+
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const realw* U = f == 0 ? A.u0 : A.u1;
+    realw* Acc = f == 0 ? A.a0 : A.a1;
+    realw* Floc = f == 0 ? A.floc0 : A.floc1;
+    if (act) {
+      s_u[0][t] = U[3 * (size_t)iglob];
+      s_u[1][t] = U[3 * (size_t)iglob + 1];
+      s_u[2][t] = U[3 * (size_t)iglob + 2];
+    }
+    __syncthreads();
+    realw ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, cx = 0, cy = 0, cz = 0;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      const int La = l + 5 * J + 25 * K, Lb = I + 5 * l + 25 * K, Lc = I + 5 * J + 25 * l;
+      ax += s_u[0][La] * hpI[l]; ay += s_u[1][La] * hpI[l]; az += s_u[2][La] * hpI[l];
+      bx += s_u[0][Lb] * hpJ[l]; by += s_u[1][Lb] * hpJ[l]; bz += s_u[2][Lb] * hpJ[l];
+      cx += s_u[0][Lc] * hpK[l]; cy += s_u[1][Lc] * hpK[l]; cz += s_u[2][Lc] * hpK[l];
+    }
+    const realw duxdx = xix * ax + etx * bx + gax * cx;
+    const realw duxdy = xiy * ax + ety * bx + gay * cx;
+    const realw duxdz = xiz * ax + etz * bx + gaz * cx;
+    const realw duydx = xix * ay + etx * by + gax * cy;
+    const realw duydy = xiy * ay + ety * by + gay * cy;
+    const realw duydz = xiz * ay + etz * by + gaz * cy;
+    const realw duzdx = xix * az + etx * bz + gax * cz;
+    const realw duzdy = xiy * az + ety * bz + gay * cz;
+    const realw duzdz = xiz * az + etz * bz + gaz * cz;
+    const realw exy2 = duxdy + duydx, exz2 = duzdx + duxdz, eyz2 = duzdy + duydz;
+
+    if (KMODE == 1 && act) {
+      realw* E = A.eps0 + (size_t)ispec * 750 + t;
+      E[0] = duxdx; E[125] = duydy; E[250] = duzdz;
+      E[375] = 0.5f * exy2; E[500] = 0.5f * exz2; E[625] = 0.5f * eyz2;
+    }
+    if (KMODE == 2) {
+      if (f == 0) {
+        ea[0] = duxdx; ea[1] = duydy; ea[2] = duzdz;
+        ea[3] = 0.5f * exy2; ea[4] = 0.5f * exz2; ea[5] = 0.5f * eyz2;
+      } else if (act) {
+        const realw eb0 = duxdx, eb1 = duydy, eb2 = duzdz;
+        const realw eb3 = 0.5f * exy2, eb4 = 0.5f * exz2, eb5 = 0.5f * eyz2;
+        if (!ANISO) {
+          const realw tra = ea[0] + ea[1] + ea[2], trb = eb0 + eb1 + eb2;
+          const realw t3a = tra / 3.0f, t3b = trb / 3.0f;
+          const realw prod = (ea[0] - t3a) * (eb0 - t3b) + (ea[1] - t3a) * (eb1 - t3b) +
+                             (ea[2] - t3a) * (eb2 - t3b) +
+                             2.0f * (ea[3] * eb3 + ea[4] * eb4 + ea[5] * eb5);
+          A.mu_kl[Lg] += A.dt * (double)prod;
+          A.kappa_kl[Lg] += A.dt * (double)(tra * trb);
+        } else {
+          const realw va[6] = {ea[0], ea[1], ea[2], 2.0f * ea[5], 2.0f * ea[4], 2.0f * ea[3]};
+          const realw vb[6] = {eb0, eb1, eb2, 2.0f * eb5, 2.0f * eb4, 2.0f * eb3};
+          double* Kc = A.cijkl_kl + (size_t)ispec * 21 * 125 + t;
+          int q = 0;
+#pragma unroll
+          for (int P = 0; P < 6; ++P)
+#pragma unroll
+            for (int Q = P; Q < 6; ++Q) {
+              realw pr = va[P] * vb[Q];
+              if (Q > P) pr += va[Q] * vb[P];
+              Kc[q * 125] += A.dt * (double)pr;
+              ++q;
+            }
+        }
+      }
+    }
+
+    realw sxx, syy, szz, sxy, sxz, syz;
+    if (!ANISO) {
+      const realw lp2m = kl + (4.0f / 3.0f) * ml;
+      const realw lam = lp2m - 2.0f * ml;
+      sxx = lp2m * duxdx + lam * (duydy + duzdz);
+      syy = lp2m * duydy + lam * (duxdx + duzdz);
+      szz = lp2m * duzdz + lam * (duxdx + duydy);
+      sxy = ml * exy2;
+      sxz = ml * exz2;
+      syz = ml * eyz2;
+    } else {
+      // c index (P,Q), P<=Q, upper-triangle row-major: 11 12 13 14 15 16 22 .. 66
+      const realw e1 = duxdx, e2 = duydy, e3 = duzdz, e4 = eyz2, e5 = exz2, e6 = exy2;
+      const realw c11 = C[0], c12 = C[125], c13 = C[250], c14 = C[375], c15 = C[500], c16 = C[625];
+      const realw c22 = C[750], c23 = C[875], c24 = C[1000], c25 = C[1125], c26 = C[1250];
+      const realw c33 = C[1375], c34 = C[1500], c35 = C[1625], c36 = C[1750];
+      const realw c44 = C[1875], c45 = C[2000], c46 = C[2125];
+      const realw c55 = C[2250], c56 = C[2375], c66 = C[2500];
+      sxx = c11 * e1 + c12 * e2 + c13 * e3 + c14 * e4 + c15 * e5 + c16 * e6;
+      syy = c12 * e1 + c22 * e2 + c23 * e3 + c24 * e4 + c25 * e5 + c26 * e6;
+      szz = c13 * e1 + c23 * e2 + c33 * e3 + c34 * e4 + c35 * e5 + c36 * e6;
+      syz = c14 * e1 + c24 * e2 + c34 * e3 + c44 * e4 + c45 * e5 + c46 * e6;
+      sxz = c15 * e1 + c25 * e2 + c35 * e3 + c45 * e4 + c55 * e5 + c56 * e6;
+      sxy = c16 * e1 + c26 * e2 + c36 * e3 + c46 * e4 + c56 * e5 + c66 * e6;
+    }
+    if (act) {
+      s_t[0][t] = jl * (sxx * xix + sxy * xiy + sxz * xiz);
+      s_t[1][t] = jl * (sxy * xix + syy * xiy + syz * xiz);
+      s_t[2][t] = jl * (sxz * xix + syz * xiy + szz * xiz);
+      s_t[3][t] = jl * (sxx * etx + sxy * ety + sxz * etz);
+      s_t[4][t] = jl * (sxy * etx + syy * ety + syz * etz);
+      s_t[5][t] = jl * (sxz * etx + syz * ety + szz * etz);
+      s_t[6][t] = jl * (sxx * gax + sxy * gay + sxz * gaz);
+      s_t[7][t] = jl * (sxy * gax + syy * gay + syz * gaz);
+      s_t[8][t] = jl * (sxz * gax + syz * gay + szz * gaz);
+    }
+    __syncthreads();
+    realw nx1 = 0, ny1 = 0, nz1 = 0, nx2 = 0, ny2 = 0, nz2 = 0, nx3 = 0, ny3 = 0, nz3 = 0;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      const int La = l + 5 * J + 25 * K, Lb = I + 5 * l + 25 * K, Lc = I + 5 * J + 25 * l;
+      nx1 += s_t[0][La] * hwI[l]; ny1 += s_t[1][La] * hwI[l]; nz1 += s_t[2][La] * hwI[l];
+      nx2 += s_t[3][Lb] * hwJ[l]; ny2 += s_t[4][Lb] * hwJ[l]; nz2 += s_t[5][Lb] * hwJ[l];
+      nx3 += s_t[6][Lc] * hwK[l]; ny3 += s_t[7][Lc] * hwK[l]; nz3 += s_t[8][Lc] * hwK[l];
+    }
+    if (act) {
+      const realw fx = fac1 * nx1 + fac2 * nx2 + fac3 * nx3;
+      const realw fy = fac1 * ny1 + fac2 * ny2 + fac3 * ny3;
+      const realw fz = fac1 * nz1 + fac2 * nz2 + fac3 * nz3;
+      if (ASM == 0) {
+        unsafeAtomicAdd(Acc + 3 * (size_t)iglob, -fx);
+        unsafeAtomicAdd(Acc + 3 * (size_t)iglob + 1, -fy);
+        unsafeAtomicAdd(Acc + 3 * (size_t)iglob + 2, -fz);
+      } else {
+        realw* F = Floc + 3 * Lg;
+        F[0] = -fx;
+        F[1] = -fy;
+        F[2] = -fz;
+      }
+    }
+  }
+}
+
+template <int NF, bool ANISO, int KMODE, int SP>
+void launch_t(const ForceArgs& a, hipStream_t s) {
+  const int nb = a.nblocks_padded;
+  if (a.floc0)
+    hipLaunchKernelGGL((k_forces<NF, ANISO, KMODE, SP, 1>), dim3(nb), dim3(128), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_forces<NF, ANISO, KMODE, SP, 0>), dim3(nb), dim3(128), 0, s, a);
+}
+
+}  // namespace
+
+void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin, int elem_count,
+                       realw deltat, const int* elem_list, bool gather) {
+  if (elem_count <= 0) return;
+  ForceArgs a{};
+  a.ibool = m->d_ibool;
+  a.irregular = m->d_irregular;
+  a.jinv10 = m->d_jinv10;
+  a.jinv10_lp = reinterpret_cast<const uint16_t*>(m->d_jinv10_lp);
+  a.kappa = m->d_kappa;
+  a.mu = m->d_mu;
+  a.kmu_lp = reinterpret_cast<const uint16_t*>(m->d_kmu_lp);
+  a.cij = m->d_cij;
+  a.tables = m->d_tables;
+  a.elem_list = elem_list;
+  a.elem_begin = elem_begin;
+  a.elem_count = elem_count;
+  a.nblocks_padded = xcd_pad(elem_count);
+  a.xix_regular = m->xix_regular;
+  a.jacobian_regular = m->jacobian_regular;
+  a.scale_k = m->static_scale_kappa;
+  a.scale_m = m->static_scale_mu;
+  a.scale_jinv = m->static_scale_jinv;
+  a.scale_jac = m->static_scale_jac;
+  a.dt = (double)deltat;
+  if (nfields == 2) {
+    a.u0 = m->d_displ; a.a0 = m->d_accel;
+    a.u1 = m->d_b_displ; a.a1 = m->d_b_accel;
+  } else if (nfields == 1) {
+    a.u0 = m->d_displ; a.a0 = m->d_accel;
+  } else {  // nfields == 3: the reconstructed (b_) field alone
+    a.u0 = m->d_b_displ; a.a0 = m->d_b_accel;
+    a.eps0 = m->d_b_epsilon;
+  }
+  if (nfields != 3) a.eps0 = m->d_epsilon;
+  if (gather) {
+    if (!m->d_floc) SEM_FAIL("gather assembly buffers not allocated");
+    if (elem_list || elem_begin != 0 || elem_count != m->nspec) SEM_FAIL("gather assembly needs the whole mesh in one pass");
+    a.floc0 = m->d_floc;
+    a.floc1 = nfields == 2 ? m->d_floc_b : nullptr;
+    if (nfields == 2 && !a.floc1) SEM_FAIL("second element-local force buffer not allocated");
+  }
+  if (kernel_mode == 1 && !a.eps0) SEM_FAIL("strain store requested but not allocated");
+  if (a.u0 == nullptr || a.a0 == nullptr) SEM_FAIL("force pass on unallocated fields");
+  if (m->static_precision != SEM_STATIC_FP32 && !m->d_kmu_lp) SEM_FAIL("reduced-precision statics not prepared");
+  a.mu_kl = m->d_mu_kl;
+  a.kappa_kl = m->d_kappa_kl;
+  a.cijkl_kl = m->d_cijkl_kl;
+  const int sp = m->static_precision;
+  const bool an = m->anisotropy != 0;
+  hipStream_t s = m->stream;
+  if (nfields == 2) {
+    if (kernel_mode != 2) SEM_FAIL("two-field force pass requires kernel mode 2");
+    if (!m->d_b_displ || !m->d_b_accel) SEM_FAIL("two-field force pass needs the b_ fields");
+    if (an ? !m->d_cijkl_kl : (!m->d_mu_kl || !m->d_kappa_kl))
+      SEM_FAIL("two-field force pass needs the %s kernel accumulators", an ? "cijkl" : "mu/kappa");
+    if (an) launch_t<2, true, 2, SEM_STATIC_FP32>(a, s);
+    else if (sp == SEM_STATIC_BF16) launch_t<2, false, 2, SEM_STATIC_BF16>(a, s);
+    else if (sp == SEM_STATIC_FP16) launch_t<2, false, 2, SEM_STATIC_FP16>(a, s);
+    else launch_t<2, false, 2, SEM_STATIC_FP32>(a, s);
+  } else {
+    if (kernel_mode == 2) SEM_FAIL("kernel mode 2 needs both fields");
+    if (an) {
+      if (kernel_mode == 1) launch_t<1, true, 1, SEM_STATIC_FP32>(a, s);
+      else launch_t<1, true, 0, SEM_STATIC_FP32>(a, s);
+    } else if (sp == SEM_STATIC_BF16) {
+      if (kernel_mode == 1) launch_t<1, false, 1, SEM_STATIC_BF16>(a, s);
+      else launch_t<1, false, 0, SEM_STATIC_BF16>(a, s);
+    } else if (sp == SEM_STATIC_FP16) {
+      if (kernel_mode == 1) launch_t<1, false, 1, SEM_STATIC_FP16>(a, s);
+      else launch_t<1, false, 0, SEM_STATIC_FP16>(a, s);
+    } else {
+      if (kernel_mode == 1) launch_t<1, false, 1, SEM_STATIC_FP32>(a, s);
+      else launch_t<1, false, 0, SEM_STATIC_FP32>(a, s);
+    }
+  }
+  HIP_CHECK(hipGetLastError());
+}

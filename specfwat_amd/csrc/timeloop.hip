@@ -1,0 +1,828 @@
+// Time stepping on the device: Newmark predictor/corrector, Stacey absorbing
+// boundary, point and adjoint sources, seismograms, sensitivity kernels, and
+// the whole-loop driver.
+//
+// Replaces SPECFEM3D's per-step GPU entry points that SpecFWAT's
+// iterate_time calls (prepare_fwd.f90:364; sources absent from the tree), and
+// adds fwat_iterate_time_, which runs a whole forward or adjoint run on the
+// device with no host round trip per step.
+//
+// Scheme (state n at time t_n = n dt - t0, u_0 = v_0 = a_0 = 0):
+//   predictor  u += dt v + dt^2/2 a ;  v += dt/2 a ;  a = 0
+//   forces     a -= K u ;  a -= S(v) (Stacey) ;  a += stf((it-1) dt - t0) s_src
+//   corrector  a *= 1/(M + dt/2 C) ;  v += dt/2 a
+// The reconstructed field runs the same scheme with -dt from the forward
+// run's last state, replaying the stored Stacey terms S and the source, which
+// inverts the forward steps exactly in exact arithmetic.  Adjoint step it
+// pairs with reconstructed state m = NSTEP - it; the adjoint source sample
+// NSTEP - it (0-based) is injected at adjoint step it.
+//
+// Kernels (Tromp, Tape & Liu 2005), accumulated in fp64:
+//   rho_kl   += dt  a_adj . u_rec              (per global point, expanded on transfer)
+//   mu_kl    += dt  D_adj : D_rec              (per local point)
+//   kappa_kl += dt  tr(e_adj) tr(e_rec)
+//   cijkl_kl += dt  (e_I e_J + e_J e_I)/(1 + delta_IJ)  engineering Voigt strains
+// and save_adjoint_kernels' scaling on transfer: rho_kl *= -rho,
+// mu_kl *= -2 mu, kappa_kl *= -kappa, cijkl_kl *= -1.
+#include "sem_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int nblk(size_t n, int tpb = TPB) { return (int)((n + tpb - 1) / tpb); }
+
+__global__ void k_fill_f64(double* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_fill_f32(realw* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// predictor on one field, n3 = 3*nglob scalars
+__global__ void k_predictor(size_t n3, realw* __restrict__ u, realw* __restrict__ v,
+                            realw* __restrict__ a, realw dt, realw dt2o2, realw dto2) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  realw ai = a[i], vi = v[i];
+  u[i] += dt * vi + dt2o2 * ai;
+  v[i] = vi + dto2 * ai;
+  a[i] = 0.0f;
+}
+
+// corrector on one field (SPECFEM kernel_3)
+__global__ void k_corrector(int nglob, realw* __restrict__ v, realw* __restrict__ a,
+                            const realw* __restrict__ rm, realw dto2) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nglob) return;
+  const size_t b = 3 * (size_t)g;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    realw ac = a[b + c] * rm[b + c];
+    a[b + c] = ac;
+    v[b + c] += dto2 * ac;
+  }
+}
+
+// Gather assembly fused with the Newmark update.  For every global point:
+//   f = sum of its contributions (CSR, fixed order): element-local forces,
+//       minus the Stacey face traction, plus sourcearray * stf, plus
+//       h_rec * adjoint trace
+//   a = rmass_inv * f ; v += dt/2 a            (corrector of step it)
+//   unless LAST: u += dt v + dt^2/2 a ; v += dt/2 a   (predictor of it+1)
+// TWO: the adjoint field (dt > 0) and the reconstructed field (dt < 0) are
+// updated together and the density kernel a_adj . u_rec is accumulated in fp64.
+struct AsmField {
+  const realw* floc;
+  const realw* face;   // [nfp][3] traction*jw to subtract, or null
+  const double* stf;   // [nsources] this step, or null
+  const realw* adj;    // [nrec][3] this step, or null
+  realw* u;
+  realw* v;
+  realw* a;
+  realw dt;            // signed time step
+};
+
+template <bool LAST, bool TWO>
+__global__ void k_assemble(int nglob, const int* __restrict__ off, const int* __restrict__ idx,
+                           int nloc, int nfp, int nsp, const realw* __restrict__ sarr,
+                           const realw* __restrict__ rech, const realw* __restrict__ rm,
+                           AsmField F0, AsmField F1, double* __restrict__ rho_kl,
+                           double* __restrict__ hess_kl, double dtk) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nglob) return;
+  realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
+  const int q1 = off[g + 1];
+  for (int q = off[g]; q < q1; ++q) {
+    const int c = idx[q];
+    if (c < nloc) {
+      const realw* P = F0.floc + 3 * (size_t)c;
+      f0x += P[0]; f0y += P[1]; f0z += P[2];
+      if (TWO) {
+        const realw* B = F1.floc + 3 * (size_t)c;
+        f1x += B[0]; f1y += B[1]; f1z += B[2];
+      }
+    } else if (c < nloc + nfp) {
+      const size_t p = 3 * (size_t)(c - nloc);
+      if (F0.face) { f0x -= F0.face[p]; f0y -= F0.face[p + 1]; f0z -= F0.face[p + 2]; }
+      if (TWO && F1.face) { f1x -= F1.face[p]; f1y -= F1.face[p + 1]; f1z -= F1.face[p + 2]; }
+    } else if (c < nloc + nfp + nsp) {
+      const int p = c - nloc - nfp;
+      const realw* S = sarr + 3 * (size_t)p;
+      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f0x += S[0] * w; f0y += S[1] * w; f0z += S[2] * w; }
+      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f1x += S[0] * w; f1y += S[1] * w; f1z += S[2] * w; }
+    } else {
+      const int p = c - nloc - nfp - nsp;
+      const realw h = rech[p];
+      const size_t r = 3 * (size_t)(p / 125);
+      if (F0.adj) { f0x += h * F0.adj[r]; f0y += h * F0.adj[r + 1]; f0z += h * F0.adj[r + 2]; }
+      if (TWO && F1.adj) { f1x += h * F1.adj[r]; f1y += h * F1.adj[r + 1]; f1z += h * F1.adj[r + 2]; }
+    }
+  }
+  const size_t b = 3 * (size_t)g;
+  const realw r0 = rm[b], r1 = rm[b + 1], r2 = rm[b + 2];
+  realw A[3] = {f0x * r0, f0y * r1, f0z * r2};
+  const realw d0 = F0.dt, h0 = 0.5f * F0.dt, s0 = 0.5f * F0.dt * F0.dt;
+  realw V[3], U[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { V[c] = F0.v[b + c] + h0 * A[c]; U[c] = F0.u[b + c]; }
+  if (TWO) {
+    realw BA[3] = {f1x * r0, f1y * r1, f1z * r2};
+    const realw d1 = F1.dt, h1 = 0.5f * F1.dt, s1 = 0.5f * F1.dt * F1.dt;
+    realw BV[3], BU[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { BV[c] = F1.v[b + c] + h1 * BA[c]; BU[c] = F1.u[b + c]; }
+    rho_kl[g] += dtk * (double)(A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2]);
+    if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (LAST) {
+        F1.a[b + c] = BA[c];
+        F1.v[b + c] = BV[c];
+      } else {
+        F1.u[b + c] = BU[c] + d1 * BV[c] + s1 * BA[c];
+        F1.v[b + c] = BV[c] + h1 * BA[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (LAST) {
+      F0.a[b + c] = A[c];
+      F0.v[b + c] = V[c];
+    } else {
+      F0.u[b + c] = U[c] + d0 * V[c] + s0 * A[c];
+      F0.v[b + c] = V[c] + h0 * A[c];
+    }
+  }
+}
+
+// Stacey traction of one field written per face point (no scatter):
+// out[q] = (rho_vp (v.n) n + rho_vs (v - (v.n) n)) * jacobian2Dw
+__global__ void k_stacey_face(int nfp, const int* __restrict__ figlob, const realw* __restrict__ fcoef,
+                              const realw* __restrict__ v, realw* __restrict__ out) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nfp) return;
+  const size_t g = (size_t)figlob[q];
+  const realw* C = fcoef + 8 * (size_t)q;
+  const realw nx = C[0], ny = C[1], nz = C[2], jw = C[3], rvp = C[4], rvs = C[5];
+  const realw vx = v[3 * g], vy = v[3 * g + 1], vz = v[3 * g + 2];
+  const realw vn = vx * nx + vy * ny + vz * nz;
+  out[3 * (size_t)q] = (rvp * vn * nx + rvs * (vx - vn * nx)) * jw;
+  out[3 * (size_t)q + 1] = (rvp * vn * ny + rvs * (vy - vn * ny)) * jw;
+  out[3 * (size_t)q + 2] = (rvp * vn * nz + rvs * (vz - vn * nz)) * jw;
+}
+
+// rho (and Hessian) kernel accumulation of the per-step API path
+__global__ void k_rho_kernel_glob(int nglob, const realw* __restrict__ a, const realw* __restrict__ bu,
+                                  const realw* __restrict__ ba, double* __restrict__ rho_kl,
+                                  double* __restrict__ hess_kl, double dtk) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nglob) return;
+  const size_t b = 3 * (size_t)g;
+  realw d = a[b] * bu[b] + a[b + 1] * bu[b + 1] + a[b + 2] * bu[b + 2];
+  rho_kl[g] += dtk * (double)d;
+  if (hess_kl) {
+    realw h = a[b] * ba[b] + a[b + 1] * ba[b + 1] + a[b + 2] * ba[b + 2];
+    hess_kl[g] += dtk * (double)h;
+  }
+}
+
+// mu/kappa or c_ij kernels from the two stored strain fields (SPECFEM-style
+// compute_kernels path; the fused path does this inside the force kernel).
+template <bool ANISO>
+__global__ void k_kernels_from_strain(int nspec, const realw* __restrict__ ea_all,
+                                      const realw* __restrict__ eb_all, double* __restrict__ mu_kl,
+                                      double* __restrict__ kappa_kl, double* __restrict__ cijkl_kl,
+                                      double dtk) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nspec * 125) return;
+  const size_t e = idx / 125, L = idx % 125;
+  const realw* EA = ea_all + e * 750 + L;
+  const realw* EB = eb_all + e * 750 + L;
+  realw ea[6], eb[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) { ea[c] = EA[c * 125]; eb[c] = EB[c * 125]; }
+  if (!ANISO) {
+    const realw tra = ea[0] + ea[1] + ea[2], trb = eb[0] + eb[1] + eb[2];
+    const realw t3a = tra / 3.0f, t3b = trb / 3.0f;
+    const realw prod = (ea[0] - t3a) * (eb[0] - t3b) + (ea[1] - t3a) * (eb[1] - t3b) +
+                       (ea[2] - t3a) * (eb[2] - t3b) + 2.0f * (ea[3] * eb[3] + ea[4] * eb[4] + ea[5] * eb[5]);
+    mu_kl[idx] += dtk * (double)prod;
+    kappa_kl[idx] += dtk * (double)(tra * trb);
+  } else {
+    const realw va[6] = {ea[0], ea[1], ea[2], 2.0f * ea[5], 2.0f * ea[4], 2.0f * ea[3]};
+    const realw vb[6] = {eb[0], eb[1], eb[2], 2.0f * eb[5], 2.0f * eb[4], 2.0f * eb[3]};
+    double* K = cijkl_kl + e * 21 * 125 + L;
+    int q = 0;
+#pragma unroll
+    for (int P = 0; P < 6; ++P)
+#pragma unroll
+      for (int Q = P; Q < 6; ++Q) {
+        realw pr = va[P] * vb[Q];
+        if (Q > P) pr += va[Q] * vb[P];
+        K[q * 125] += dtk * (double)pr;
+        ++q;
+      }
+  }
+}
+
+// Stacey: mode 0 absorb, 1 absorb + store, 2 replay a stored record
+__global__ void k_stacey(int nfp, const int* __restrict__ figlob, const realw* __restrict__ fcoef,
+                         const realw* __restrict__ v, realw* __restrict__ a, realw* __restrict__ rec,
+                         int mode) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nfp) return;
+  const size_t g = (size_t)figlob[q];
+  realw tx, ty, tz;
+  if (mode == 2) {
+    tx = rec[3 * (size_t)q]; ty = rec[3 * (size_t)q + 1]; tz = rec[3 * (size_t)q + 2];
+  } else {
+    const realw* C = fcoef + 8 * (size_t)q;
+    const realw nx = C[0], ny = C[1], nz = C[2], jw = C[3], rvp = C[4], rvs = C[5];
+    const realw vx = v[3 * g], vy = v[3 * g + 1], vz = v[3 * g + 2];
+    const realw vn = vx * nx + vy * ny + vz * nz;
+    tx = (rvp * vn * nx + rvs * (vx - vn * nx)) * jw;
+    ty = (rvp * vn * ny + rvs * (vy - vn * ny)) * jw;
+    tz = (rvp * vn * nz + rvs * (vz - vn * nz)) * jw;
+    if (mode == 1) { rec[3 * (size_t)q] = tx; rec[3 * (size_t)q + 1] = ty; rec[3 * (size_t)q + 2] = tz; }
+  }
+  unsafeAtomicAdd(a + 3 * g, -tx);
+  unsafeAtomicAdd(a + 3 * g + 1, -ty);
+  unsafeAtomicAdd(a + 3 * g + 2, -tz);
+}
+
+// point sources: a += stf(source) * sourcearray
+__global__ void k_add_sources(int nsrc, const int* __restrict__ siglob, const realw* __restrict__ sarr,
+                              const double* __restrict__ stf, realw* __restrict__ a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsrc * 125) return;
+  const int s = q / 125;
+  const realw f = (realw)stf[s];
+  const size_t g = (size_t)siglob[q];
+  unsafeAtomicAdd(a + 3 * g, sarr[3 * (size_t)q] * f);
+  unsafeAtomicAdd(a + 3 * g + 1, sarr[3 * (size_t)q + 1] * f);
+  unsafeAtomicAdd(a + 3 * g + 2, sarr[3 * (size_t)q + 2] * f);
+}
+
+// adjoint sources: a += h_rec * adj(sample)
+__global__ void k_add_adjoint(int nrec, const int* __restrict__ riglob, const realw* __restrict__ rh,
+                              const realw* __restrict__ adj_row, realw* __restrict__ a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrec * 125) return;
+  const int r = q / 125;
+  const realw h = rh[q];
+  const size_t g = (size_t)riglob[q];
+  unsafeAtomicAdd(a + 3 * g, h * adj_row[3 * r]);
+  unsafeAtomicAdd(a + 3 * g + 1, h * adj_row[3 * r + 1]);
+  unsafeAtomicAdd(a + 3 * g + 2, h * adj_row[3 * r + 2]);
+}
+
+// seismograms: one 128-lane block per receiver, Lagrange interpolation of u
+__global__ __launch_bounds__(128) void k_seismo(int nrec, const int* __restrict__ riglob,
+                                                const realw* __restrict__ rh, const realw* __restrict__ u,
+                                                realw* __restrict__ row) {
+  __shared__ realw s[3][128];
+  const int r = blockIdx.x, t = threadIdx.x;
+  realw x = 0, y = 0, z = 0;
+  if (t < 125) {
+    const realw h = rh[(size_t)r * 125 + t];
+    const size_t g = (size_t)riglob[(size_t)r * 125 + t];
+    x = h * u[3 * g]; y = h * u[3 * g + 1]; z = h * u[3 * g + 2];
+  }
+  s[0][t] = x; s[1][t] = y; s[2][t] = z;
+  __syncthreads();
+  for (int w = 64; w > 0; w >>= 1) {
+    if (t < w) { s[0][t] += s[0][t + w]; s[1][t] += s[1][t + w]; s[2][t] += s[2][t + w]; }
+    __syncthreads();
+  }
+  if (t == 0) { row[3 * r] = s[0][0]; row[3 * r + 1] = s[1][0]; row[3 * r + 2] = s[2][0]; }
+}
+
+__global__ void k_copy4(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_expand_glob(int nspec, const int* __restrict__ ibool, const double* __restrict__ g,
+                              double scale_sign, const realw* __restrict__ rho, double* __restrict__ out) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nspec * 125) return;
+  double v = g[ibool[idx]];
+  out[idx] = rho ? scale_sign * (double)rho[idx] * v : v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ internal stages
+
+void sem_zero_kernels(SemMesh* m) {
+  hipStream_t s = m->stream;
+  const size_t nloc = (size_t)m->nspec * 125;
+  if (m->d_rho_kl_glob) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(m->nglob)), dim3(TPB), 0, s, m->d_rho_kl_glob, (size_t)m->nglob);
+  if (m->d_hess_kl_glob) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(m->nglob)), dim3(TPB), 0, s, m->d_hess_kl_glob, (size_t)m->nglob);
+  if (m->d_mu_kl) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nloc)), dim3(TPB), 0, s, m->d_mu_kl, nloc);
+  if (m->d_kappa_kl) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nloc)), dim3(TPB), 0, s, m->d_kappa_kl, nloc);
+  if (m->d_cijkl_kl) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(nloc * 21)), dim3(TPB), 0, s, m->d_cijkl_kl, nloc * 21);
+  HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+
+void stage_predictor(SemMesh* m, bool backward, realw dt) {
+  realw* u = backward ? m->d_b_displ : m->d_displ;
+  realw* v = backward ? m->d_b_veloc : m->d_veloc;
+  realw* a = backward ? m->d_b_accel : m->d_accel;
+  const realw sdt = backward ? -dt : dt;
+  const size_t n3 = (size_t)m->nglob * 3;
+  hipLaunchKernelGGL(k_predictor, dim3(nblk(n3)), dim3(TPB), 0, m->stream, n3, u, v, a, sdt,
+                     0.5f * dt * dt, 0.5f * sdt);
+}
+
+void stage_corrector(SemMesh* m, bool backward, realw dt) {
+  realw* v = backward ? m->d_b_veloc : m->d_veloc;
+  realw* a = backward ? m->d_b_accel : m->d_accel;
+  const realw sdt = backward ? -dt : dt;
+  hipLaunchKernelGGL(k_corrector, dim3(nblk(m->nglob)), dim3(TPB), 0, m->stream, m->nglob, v, a,
+                     m->d_rmass_inv, 0.5f * sdt);
+}
+
+void stage_stacey(SemMesh* m, bool backward, int mode, size_t record) {
+  if (!m->absorbing || m->nfaces == 0) return;
+  const int nfp = m->nfaces * 25;
+  realw* v = backward ? m->d_b_veloc : m->d_veloc;
+  realw* a = backward ? m->d_b_accel : m->d_accel;
+  realw* rec = nullptr;
+  if (mode != 0) {
+    if (!m->d_stacey_store) SEM_FAIL("Stacey store requested but not allocated");
+    if (record >= m->stacey_store_steps) SEM_FAIL("Stacey record %zu outside store of %zu steps", record, m->stacey_store_steps);
+    rec = m->d_stacey_store + record * (size_t)nfp * 3;
+  }
+  hipLaunchKernelGGL(k_stacey, dim3(nblk(nfp)), dim3(TPB), 0, m->stream, nfp, m->d_face_iglob,
+                     m->d_face_coef, v, a, rec, mode);
+}
+
+void stage_sources(SemMesh* m, bool backward, const double* d_stf_row) {
+  if (m->nsources == 0) return;
+  realw* a = backward ? m->d_b_accel : m->d_accel;
+  hipLaunchKernelGGL(k_add_sources, dim3(nblk((size_t)m->nsources * 125)), dim3(TPB), 0, m->stream,
+                     m->nsources, m->d_src_iglob, m->d_sourcearrays, d_stf_row, a);
+}
+
+void stage_adjoint_sources(SemMesh* m, const realw* d_adj_row) {
+  if (m->nrec == 0) return;
+  hipLaunchKernelGGL(k_add_adjoint, dim3(nblk((size_t)m->nrec * 125)), dim3(TPB), 0, m->stream,
+                     m->nrec, m->d_rec_iglob, m->d_rec_h, d_adj_row, m->d_accel);
+}
+
+void stage_stacey_face(SemMesh* m, const realw* v, realw* out) {
+  if (!m->absorbing || m->nfaces == 0) return;
+  const int nfp = m->nfaces * 25;
+  hipLaunchKernelGGL(k_stacey_face, dim3(nblk(nfp)), dim3(TPB), 0, m->stream, nfp, m->d_face_iglob,
+                     m->d_face_coef, v, out);
+}
+
+void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const AsmField& F1, double dtk) {
+  const int nfp = m->absorbing ? m->nfaces * 25 : 0;
+  const int nloc = m->nspec * 125, nsp = m->nsources * 125;
+  const dim3 gr(nblk(m->nglob)), bl(TPB);
+  if (two) {
+    if (last)
+      hipLaunchKernelGGL((k_assemble<true, true>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, m->d_hess_kl_glob, dtk);
+    else
+      hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, m->d_hess_kl_glob, dtk);
+  } else {
+    if (last)
+      hipLaunchKernelGGL((k_assemble<true, false>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (double*)nullptr, dtk);
+    else
+      hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (double*)nullptr, dtk);
+  }
+}
+
+void stage_seismo(SemMesh* m, int it0) {
+  if (m->nrec == 0) return;
+  if (it0 < 0 || it0 >= m->nstep) SEM_FAIL("seismogram index %d outside NSTEP=%d", it0 + 1, m->nstep);
+  hipLaunchKernelGGL(k_seismo, dim3(m->nrec), dim3(128), 0, m->stream, m->nrec, m->d_rec_iglob,
+                     m->d_rec_h, m->d_displ, m->d_seismograms + (size_t)it0 * m->nrec * 3);
+}
+
+// Force launches bracketed by HIP events on the simulation stream; the
+// elapsed times are summed after the run, so timing adds no host sync.
+struct ForceTimer {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  void begin(size_t n) {
+    for (auto e : ev) HIP_CHECK(hipEventDestroy(e));
+    ev.assign(2 * n, nullptr);
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    used = 0;
+  }
+  void mark(hipStream_t s) {
+    if (used >= ev.size()) SEM_FAIL("force timer overflow");
+    HIP_CHECK(hipEventRecord(ev[used++], s));
+  }
+  double total_ms() {
+    double t = 0;
+    for (size_t q = 0; q + 1 < used; q += 2) {
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      t += ms;
+    }
+    return t;
+  }
+  ~ForceTimer() {
+    for (auto e : ev) hipEventDestroy(e);
+  }
+};
+
+void stage_forces_timed(SemMesh* m, int nfields, int kmode, realw dt, ForceTimer* tm, bool gather) {
+  if (tm) tm->mark(m->stream);
+  sem_launch_forces(m, nfields, kmode, 0, m->nspec, dt, nullptr, gather);
+  if (tm) {
+    tm->mark(m->stream);
+    m->last_force_launches += 1;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+
+extern "C" {
+
+// FORWARD_OR_ADJOINT: 1 = forward/adjoint field, 3 = reconstructed (b_) field
+void update_displacement_cuda_(long* Mesh_pointer, realw* deltat, realw* deltatsqover2,
+                               realw* deltatover2, int* FORWARD_OR_ADJOINT) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  (void)deltatsqover2; (void)deltatover2;
+  stage_predictor(m, *FORWARD_OR_ADJOINT == 3, *deltat);
+  HIP_CHECK(hipGetLastError());
+}
+
+// Internal forces of one inner/outer phase (iphase 1 = outer, 2 = inner).
+// COMPUTE_AND_STORE_STRAIN stores the strain for compute_kernels_elastic_cuda_.
+void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* deltat,
+                                       int* nspec_outer_elastic, int* nspec_inner_elastic,
+                                       int* COMPUTE_AND_STORE_STRAIN, int* ATTENUATION,
+                                       int* FORWARD_OR_ADJOINT) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*ATTENUATION) SEM_FAIL("ATTENUATION is not supported on this path (SpecFWAT disables it for adjoint runs)");
+  const int nfields = *FORWARD_OR_ADJOINT == 3 ? 3 : 1;
+  const int kmode = *COMPUTE_AND_STORE_STRAIN ? 1 : 0;
+  if (kmode && (nfields == 1 ? !m->d_epsilon : !m->d_b_epsilon)) SEM_FAIL("strain store not allocated");
+  // iphase 1 = outer (halo) elements, 2 = inner elements, from
+  // phase_ispec_inner_elastic; without phase lists every element is inner.
+  if (m->d_phase_ispec && m->num_phase_ispec > 0) {
+    if (*nspec_outer_elastic != m->nspec_outer || *nspec_inner_elastic != m->nspec_inner)
+      SEM_FAIL("phase sizes differ from prepare_fields_elastic_device_");
+    const int* list = m->d_phase_ispec + (*iphase == 1 ? 0 : m->num_phase_ispec);
+    const int count = *iphase == 1 ? m->nspec_outer : m->nspec_inner;
+    sem_launch_forces(m, nfields, kmode, 0, count, *deltat, list, false);
+    return;
+  }
+  if (*iphase != 2) return;
+  sem_launch_forces(m, nfields, kmode, 0, m->nspec, *deltat, nullptr, false);
+}
+
+// Stacey: the forward run of a SAVE_FORWARD simulation stores record it
+// (1-based); the reconstructed field replays it.
+void compute_stacey_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, int* it_record,
+                                       int* FORWARD_OR_ADJOINT) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*iphase != 1) return;
+  const bool back = *FORWARD_OR_ADJOINT == 3;
+  int mode = back ? 2 : ((m->save_forward && m->simulation_type == 1) ? 1 : 0);
+  stage_stacey(m, back, mode, (size_t)std::max(*it_record - 1, 0));
+  HIP_CHECK(hipGetLastError());
+}
+
+// h_stf_pre_compute (NSOURCES): source time function values of this step
+void compute_add_sources_viscoelastic_cuda_(long* Mesh_pointer, int* NSOURCES, double* h_stf_pre_compute,
+                                            int* FORWARD_OR_ADJOINT) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*NSOURCES != m->nsources) SEM_FAIL("NSOURCES mismatch");
+  if (m->nsources == 0) return;
+  HIP_CHECK(hipMemcpyAsync(m->d_stf_step, h_stf_pre_compute, sizeof(double) * m->nsources, hipMemcpyHostToDevice, m->stream));
+  stage_sources(m, *FORWARD_OR_ADJOINT == 3, m->d_stf_step);
+  HIP_CHECK(hipGetLastError());
+}
+
+// h_source_adjoint (NDIM, nadj_rec_local): adjoint source samples of this step
+void add_sources_el_sim_type_2_or_3_(long* Mesh_pointer, int* nrec, realw* h_source_adjoint, int* it) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  (void)it;
+  if (*nrec != m->nrec) SEM_FAIL("nrec mismatch");
+  if (m->nrec == 0) return;
+  HIP_CHECK(hipMemcpyAsync(m->d_adj_step, h_source_adjoint, sizeof(realw) * 3 * m->nrec, hipMemcpyHostToDevice, m->stream));
+  stage_adjoint_sources(m, m->d_adj_step);
+  HIP_CHECK(hipGetLastError());
+}
+
+// kernel_3: accel *= rmass_inv, veloc += deltatover2 * accel
+void kernel_3_a_cuda_(long* Mesh_pointer, realw* deltatover2, int* FORWARD_OR_ADJOINT) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  const bool back = *FORWARD_OR_ADJOINT == 3;
+  stage_corrector(m, back, 2.0f * std::fabs(*deltatover2));
+  HIP_CHECK(hipGetLastError());
+}
+
+// record seismograms of step it (1-based) into the device buffer
+void compute_seismograms_cuda_(long* Mesh_pointer, int* it) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  stage_seismo(m, *it - 1);
+  HIP_CHECK(hipGetLastError());
+}
+
+// seismograms (NDIM, nrec, NSTEP)
+void transfer_seismograms_from_device_(long* Mesh_pointer, realw* h_seismograms) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (m->nrec == 0) return;
+  HIP_CHECK(hipMemcpyAsync(h_seismograms, m->d_seismograms, sizeof(realw) * (size_t)m->nstep * m->nrec * 3,
+                           hipMemcpyDeviceToHost, m->stream));
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
+// Kernel accumulation of one adjoint step from the stored strains
+// (COMPUTE_AND_STORE_STRAIN path) and the current accel / b_displ.
+void compute_kernels_elastic_cuda_(long* Mesh_pointer, realw* deltat) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (!m->d_rho_kl_glob) SEM_FAIL("kernels not prepared (prepare_fields_elastic_adj_dev_)");
+  if (!m->d_epsilon || !m->d_b_epsilon) SEM_FAIL("compute_kernels_elastic_cuda_ needs COMPUTE_AND_STORE_STRAIN");
+  const double dtk = (double)*deltat;
+  hipLaunchKernelGGL(k_rho_kernel_glob, dim3(nblk(m->nglob)), dim3(TPB), 0, m->stream, m->nglob,
+                     m->d_accel, m->d_b_displ, m->d_b_accel, m->d_rho_kl_glob, m->d_hess_kl_glob, dtk);
+  const size_t nloc = (size_t)m->nspec * 125;
+  if (m->anisotropic_kl)
+    hipLaunchKernelGGL(k_kernels_from_strain<true>, dim3(nblk(nloc)), dim3(TPB), 0, m->stream, m->nspec,
+                       m->d_epsilon, m->d_b_epsilon, nullptr, nullptr, m->d_cijkl_kl, dtk);
+  else
+    hipLaunchKernelGGL(k_kernels_from_strain<false>, dim3(nblk(nloc)), dim3(TPB), 0, m->stream, m->nspec,
+                       m->d_epsilon, m->d_b_epsilon, m->d_mu_kl, m->d_kappa_kl, nullptr, dtk);
+  HIP_CHECK(hipGetLastError());
+}
+
+// Scaled kernels (save_adjoint_kernels): rho_kl = -rho K_rho, mu_kl = -2 mu K_mu,
+// kappa_kl = -kappa K_kappa, each (NGLLX,NGLLY,NGLLZ,NSPEC_AB) fp64.
+void transfer_kernels_el_to_host_dp_(long* Mesh_pointer, double* h_rho_kl, double* h_mu_kl,
+                                     double* h_kappa_kl, int* NSPEC_AB) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*NSPEC_AB != m->nspec) SEM_FAIL("NSPEC_AB mismatch");
+  const size_t nloc = (size_t)m->nspec * 125;
+  hipStream_t s = m->stream;
+  double* tmp = dev_alloc<double>(nloc);
+  hipLaunchKernelGGL(k_expand_glob, dim3(nblk(nloc)), dim3(TPB), 0, s, m->nspec, m->d_ibool,
+                     m->d_rho_kl_glob, -1.0, m->d_rho, tmp);
+  HIP_CHECK(hipMemcpyAsync(h_rho_kl, tmp, nloc * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  dev_free(tmp);
+  if (!m->anisotropic_kl) {
+    std::vector<float> kap(nloc), mu(nloc);
+    HIP_CHECK(hipMemcpyAsync(h_mu_kl, m->d_mu_kl, nloc * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_kappa_kl, m->d_kappa_kl, nloc * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(kap.data(), m->d_kappa, nloc * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(mu.data(), m->d_mu, nloc * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (size_t L = 0; L < nloc; ++L) {
+      h_mu_kl[L] *= -2.0 * (double)mu[L];
+      h_kappa_kl[L] *= -(double)kap[L];
+    }
+  }
+}
+
+// fp32 variant with SPECFEM's realw output type
+void transfer_kernels_el_to_host_(long* Mesh_pointer, realw* h_rho_kl, realw* h_mu_kl,
+                                  realw* h_kappa_kl, int* NSPEC_AB) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  const size_t nloc = (size_t)m->nspec * 125;
+  std::vector<double> r(nloc), mu(nloc), k(nloc);
+  transfer_kernels_el_to_host_dp_(Mesh_pointer, r.data(), mu.data(), k.data(), NSPEC_AB);
+  for (size_t L = 0; L < nloc; ++L) {
+    h_rho_kl[L] = (realw)r[L];
+    if (!m->anisotropic_kl) { h_mu_kl[L] = (realw)mu[L]; h_kappa_kl[L] = (realw)k[L]; }
+  }
+}
+
+// cijkl_kl (21,NGLLX,NGLLY,NGLLZ,NSPEC_AB) fp64, scaled by -1
+void transfer_kernels_ani_to_host_dp_(long* Mesh_pointer, double* h_cijkl_kl, int* NSPEC_AB) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*NSPEC_AB != m->nspec) SEM_FAIL("NSPEC_AB mismatch");
+  if (!m->d_cijkl_kl) SEM_FAIL("anisotropic kernels not prepared");
+  const size_t nloc = (size_t)m->nspec * 125;
+  std::vector<double> tmp(nloc * 21);
+  HIP_CHECK(hipMemcpyAsync(tmp.data(), m->d_cijkl_kl, tmp.size() * 8, hipMemcpyDeviceToHost, m->stream));
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+  for (size_t e = 0; e < (size_t)m->nspec; ++e)
+    for (int q = 0; q < 21; ++q)
+      for (int L = 0; L < 125; ++L) h_cijkl_kl[((e * 125) + L) * 21 + q] = -tmp[(e * 21 + q) * 125 + L];
+}
+
+// approximate Hessian (sum of dt * a_adj . a_rec), (NGLLX,NGLLY,NGLLZ,NSPEC_AB)
+void transfer_kernels_hess_el_tohost_dp_(long* Mesh_pointer, double* h_hess_kl, int* NSPEC_AB) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*NSPEC_AB != m->nspec) SEM_FAIL("NSPEC_AB mismatch");
+  if (!m->d_hess_kl_glob) SEM_FAIL("Hessian not prepared (APPROXIMATE_HESS_KL)");
+  const size_t nloc = (size_t)m->nspec * 125;
+  double* tmp = dev_alloc<double>(nloc);
+  hipLaunchKernelGGL(k_expand_glob, dim3(nblk(nloc)), dim3(TPB), 0, m->stream, m->nspec, m->d_ibool,
+                     m->d_hess_kl_glob, 1.0, (const realw*)nullptr, tmp);
+  HIP_CHECK(hipMemcpyAsync(h_hess_kl, tmp, nloc * 8, hipMemcpyDeviceToHost, m->stream));
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+  dev_free(tmp);
+}
+
+// Upload the whole-run source time functions (NSOURCES, NSTEP) and adjoint
+// source traces (NDIM, nrec, NSTEP, forward time order) for fwat_iterate_time_.
+void fwat_set_stf_(long* Mesh_pointer, double* h_stf, int* NSTEP) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*NSTEP != m->nstep) SEM_FAIL("NSTEP mismatch");
+  if (m->nsources == 0) return;
+  dev_free(m->d_stf);
+  m->d_stf = dev_upload(h_stf, (size_t)m->nstep * m->nsources, m->stream);
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
+void fwat_set_adjoint_sources_(long* Mesh_pointer, realw* h_adj, int* NSTEP) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*NSTEP != m->nstep) SEM_FAIL("NSTEP mismatch");
+  if (m->nrec == 0) return;
+  dev_free(m->d_adj);
+  m->d_adj = dev_upload(h_adj, (size_t)m->nstep * m->nrec * 3, m->stream);
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
+// Copy the forward run's final state (displ, veloc, accel) of handle `from`
+// into the reconstructed field of handle `to` (both on this device): the
+// in-HBM replacement of save_forward_arrays.bin (specfem_api.f90:312-358).
+void fwat_copy_forward_to_backward_(long* Mesh_pointer_from, long* Mesh_pointer_to) {
+  SemMesh* a = sem_mesh(Mesh_pointer_from);
+  SemMesh* b = sem_mesh(Mesh_pointer_to);
+  if (a->nglob != b->nglob || !b->d_b_displ) SEM_FAIL("fwat_copy_forward_to_backward_: incompatible handles");
+  const size_t n = (size_t)a->nglob * 3 * sizeof(realw);
+  HIP_CHECK(hipStreamSynchronize(a->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_b_displ, a->d_displ, n, hipMemcpyDeviceToDevice, b->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_b_veloc, a->d_veloc, n, hipMemcpyDeviceToDevice, b->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_b_accel, a->d_accel, n, hipMemcpyDeviceToDevice, b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+}
+
+// Whole run on the device.  SIMULATION_TYPE 1: forward steps 1..NSTEP with
+// seismograms (and the Stacey store when SAVE_FORWARD).  SIMULATION_TYPE 3:
+// adjoint + reconstruction + kernels.  fused = 1 uses the fused Newmark and
+// the two-field force pass with in-pass kernel accumulation; fused = 0 runs
+// the SPECFEM-style stage sequence (separate predictor/corrector, strain
+// stores, compute_kernels).  timed = 1 brackets every force launch with HIP
+// events on the simulation stream (fwat_last_force_time_ reports the sum).
+void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int* fused, int* timed) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  const realw dt = *DT;
+  const int N = m->nstep;
+  const bool fz = *fused != 0;
+  m->last_force_ms = 0.0;
+  m->last_force_launches = 0;
+  ForceTimer timer;
+  ForceTimer* tm = nullptr;
+  if (*timed) {
+    timer.begin((size_t)N * 2);
+    tm = &timer;
+  }
+  if (m->nsources > 0 && !m->d_stf) SEM_FAIL("fwat_set_stf_ not called");
+  hipStream_t s = m->stream;
+  const realw dt2o2 = 0.5f * dt * dt, dto2 = 0.5f * dt;
+  if (*SIMULATION_TYPE == 1) {
+    const int smode = (m->save_forward && m->d_stacey_store) ? 1 : 0;
+    if (fz) {
+      const int nfp = m->nfaces * 25;
+      stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
+      for (int it = 1; it <= N; ++it) {
+        stage_forces_timed(m, 1, 0, dt, tm, true);
+        realw* face = nullptr;
+        if (m->absorbing && m->nfaces) {
+          face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
+          stage_stacey_face(m, m->d_veloc, face);
+        }
+        stage_seismo(m, it - 1);
+        AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
+                    nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
+        stage_assemble(m, it == N, false, F0, F0, 0.0);
+      }
+    } else {
+      for (int it = 1; it <= N; ++it) {
+        stage_predictor(m, false, dt);
+        stage_forces_timed(m, 1, 0, dt, tm, false);
+        stage_stacey(m, false, smode, (size_t)(it - 1));
+        if (m->nsources) stage_sources(m, false, m->d_stf + (size_t)(it - 1) * m->nsources);
+        stage_corrector(m, false, dt);
+        stage_seismo(m, it - 1);
+      }
+    }
+  } else if (*SIMULATION_TYPE == 3) {
+    if (!m->d_b_displ) SEM_FAIL("adjoint fields not prepared");
+    if (m->absorbing && m->nfaces && !m->d_stacey_store) SEM_FAIL("adjoint run needs the forward Stacey store");
+    if (m->absorbing && m->nfaces && m->stacey_store_steps < (size_t)N)
+      SEM_FAIL("Stacey store holds %zu steps, adjoint run needs %d", m->stacey_store_steps, N);
+    if (m->nrec && !m->d_adj) SEM_FAIL("fwat_set_adjoint_sources_ not called");
+    if (!fz && (!m->d_epsilon || !m->d_b_epsilon)) SEM_FAIL("unfused adjoint run needs COMPUTE_AND_STORE_STRAIN");
+    const double dtk = (double)dt;
+    if (fz) {
+      const int nfp = m->nfaces * 25;
+      stage_predictor(m, true, dt);   // reconstructed field: first step back from the last state
+      stage_predictor(m, false, dt);  // adjoint field (from rest)
+      for (int it = 1; it <= N; ++it) {
+        const int ms = N - it;
+        stage_forces_timed(m, 2, 2, dt, tm, true);
+        realw* face_a = nullptr;
+        if (m->absorbing && m->nfaces) {
+          face_a = m->d_face_scratch;
+          stage_stacey_face(m, m->d_veloc, face_a);
+        }
+        const realw* face_b = (ms >= 1 && m->absorbing && m->nfaces) ? m->d_stacey_store + (size_t)(ms - 1) * nfp * 3 : nullptr;
+        AsmField F0{m->d_floc, face_a, nullptr, m->nrec ? m->d_adj + (size_t)ms * m->nrec * 3 : nullptr,
+                    m->d_displ, m->d_veloc, m->d_accel, dt};
+        AsmField F1{m->d_floc_b, face_b, (ms >= 1 && m->nsources) ? m->d_stf + (size_t)(ms - 1) * m->nsources : nullptr,
+                    nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
+        stage_assemble(m, it == N, true, F0, F1, dtk);
+      }
+    } else {
+      for (int it = 1; it <= N; ++it) {
+        const int ms = N - it;
+        stage_predictor(m, true, dt);
+        stage_predictor(m, false, dt);
+        if (tm) tm->mark(s);
+        sem_launch_forces(m, 3, 1, 0, m->nspec, dt, nullptr, false);
+        sem_launch_forces(m, 1, 1, 0, m->nspec, dt, nullptr, false);
+        if (tm) {
+          tm->mark(s);
+          m->last_force_launches += 2;
+        }
+        if (ms >= 1) {
+          stage_stacey(m, true, 2, (size_t)(ms - 1));
+          if (m->nsources) stage_sources(m, true, m->d_stf + (size_t)(ms - 1) * m->nsources);
+        }
+        stage_stacey(m, false, 0, 0);
+        if (m->nrec) stage_adjoint_sources(m, m->d_adj + (size_t)ms * m->nrec * 3);
+        stage_corrector(m, true, dt);
+        stage_corrector(m, false, dt);
+        compute_kernels_elastic_cuda_(Mesh_pointer, DT);
+      }
+    }
+  } else {
+    SEM_FAIL("SIMULATION_TYPE %d not supported", *SIMULATION_TYPE);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (tm) m->last_force_ms = tm->total_ms();
+}
+
+// Sum of the force-kernel durations of the last fwat_iterate_time_ call (ms)
+// and the number of timed launches.
+void fwat_last_force_time_(long* Mesh_pointer, double* ms, int* launches) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  *ms = m->last_force_ms;
+  *launches = m->last_force_launches;
+}
+
+// Plain device copy of nbytes (float4 loads/stores), repeated reps times;
+// avg_ms is the mean duration.  A reference kernel of known traffic for
+// scaling hardware byte counters.
+void fwat_copy_probe_(long* nbytes, int* reps, double* avg_ms) {
+  const size_t n4 = (size_t)*nbytes / 16;
+  if (n4 == 0 || *reps <= 0) SEM_FAIL("fwat_copy_probe_: empty probe");
+  float4 *a = nullptr, *b = nullptr;
+  HIP_CHECK(hipMalloc((void**)&a, n4 * 16));
+  HIP_CHECK(hipMalloc((void**)&b, n4 * 16));
+  HIP_CHECK(hipMemset(a, 0, n4 * 16));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  const int blocks = (int)std::min<size_t>((n4 + 255) / 256, 1u << 20);
+  hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+  HIP_CHECK(hipEventRecord(e0, 0));
+  for (int r = 0; r < *reps; ++r) hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+  HIP_CHECK(hipEventRecord(e1, 0));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / *reps;
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipFree(a));
+  HIP_CHECK(hipFree(b));
+}
+
+void fwat_synchronize_(long* Mesh_pointer) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""Host driver of the GPU solver: the run sequence of SpecFWAT's PrepareFWD.
+
+Mirrors src/preproc/prepare_fwd.f90:
+  run_simulation(run_opt)   :328-368  InitSpecfem -> setup_sources_receivers ->
+                                      restore_rmass -> prepare_timerun (prepare_GPU) ->
+                                      iterate_time -> FinalizeSpecfem
+  simulation()              :297-326  forward (SIMULATION_TYPE 1, SAVE_FORWARD) ->
+                                      measure_adj -> adjoint (SIMULATION_TYPE 3) ->
+                                      postproc_adjoint (save_adjoint_kernels)
+All device work goes through the C ABI of libspecfwat_hip.so; the forward
+state and the Stacey store are handed to the adjoint run inside HBM instead
+of through save_forward_arrays.bin / absorb_field.bin.
+"""
+import numpy as np
+
+from . import _lib
+from .model import moduli, CIJ_NAMES
+
+F32 = np.float32
+
+
+def mass_matrices(mesh, rho, rho_vp, rho_vs, dt, stacey=True):
+    """Inverted rmassx/y/z (float32): lumped mass plus the dt/2 Stacey term."""
+    w = mesh.local_weights() * rho.astype(np.float64)
+    rm = np.bincount(mesh.ibool0.ravel(), weights=w.ravel(), minlength=mesh.nglob)
+    rmx, rmy, rmz = rm.copy(), rm.copy(), rm.copy()
+    if stacey and mesh.num_abs_boundary_faces:
+        e = mesh.abs_boundary_ispec - 1
+        ijk = mesh.abs_boundary_ijk - 1
+        L = ijk[..., 0] + 5 * ijk[..., 1] + 25 * ijk[..., 2]
+        g = mesh.ibool0[e[:, None], L]
+        rvp = rho_vp[e[:, None], L].astype(np.float64)
+        rvs = rho_vs[e[:, None], L].astype(np.float64)
+        n = mesh.abs_boundary_normal.astype(np.float64)
+        jw = mesh.abs_boundary_jacobian2Dw.astype(np.float64)
+        dto2 = 0.5 * dt
+        vn = dto2 * n.sum(axis=-1)
+        for comp, arr in enumerate((rmx, rmy, rmz)):
+            t = rvp * vn * n[..., comp] + rvs * (dto2 - vn * n[..., comp])
+            np.add.at(arr, g.ravel(), (t * jw).ravel())
+    return (1.0 / rmx).astype(F32), (1.0 / rmy).astype(F32), (1.0 / rmz).astype(F32)
+
+
+class ElasticRun:
+    """One solver run (SIMULATION_TYPE 1 or 3) held on the GPU."""
+
+    def __init__(self, mesh, rho, vp, vs, sources, receivers, dt, nstep, simulation_type=1,
+                 save_forward=False, cij=None, stacey=True, compute_and_store_strain=False,
+                 approximate_hess=False, static_precision=0):
+        self.mesh, self.dt, self.nstep = mesh, float(dt), int(nstep)
+        self.simulation_type = simulation_type
+        self.aniso = cij is not None
+        self.nrec = receivers.nrec if receivers is not None else 0
+        self.nsrc = sources.nsources if sources is not None else 0
+        rho = np.ascontiguousarray(rho, F32).reshape(mesh.nspec, 125)
+        vp = np.ascontiguousarray(vp, F32).reshape(mesh.nspec, 125)
+        vs = np.ascontiguousarray(vs, F32).reshape(mesh.nspec, 125)
+        kappa, mu, rho_vp, rho_vs = moduli(rho, vp, vs)
+        self.rho, self.kappa, self.mu = rho, kappa, mu
+        rmx, rmy, rmz = mass_matrices(mesh, rho, rho_vp, rho_vs, self.dt, stacey)
+        self.h = _lib.HandleBox()
+        g = mesh.gll
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)
+        f32 = lambda a: np.ascontiguousarray(a, F32)
+        src_arr = f32(sources.sourcearrays) if self.nsrc else np.zeros((1, 125, 3), F32)
+        src_isp = i32(sources.ispec) if self.nsrc else np.ones(1, np.int32)
+        rec_isp = i32(receivers.ispec) if self.nrec else np.ones(1, np.int32)
+        rec_h = f32(receivers.hlagrange) if self.nrec else np.zeros((1, 125), F32)
+        nf = mesh.num_abs_boundary_faces if stacey else 0
+        _lib.call("prepare_constants_device_", self.h, 5, mesh.nspec, mesh.nglob, mesh.nspec_irregular,
+                  mesh.xix, mesh.xiy, mesh.xiz, mesh.etax, mesh.etay, mesh.etaz, mesh.gammax,
+                  mesh.gammay, mesh.gammaz, mesh.jacobian, float(mesh.xix_regular),
+                  float(mesh.jacobian_regular), i32(mesh.ibool), i32(mesh.irregular_element_number),
+                  g.hprime_xx, g.hprimewgll_xx, g.wgllwgll_xy, g.wgllwgll_xz, g.wgllwgll_yz,
+                  int(stacey), nf, i32(mesh.abs_boundary_ispec), i32(mesh.abs_boundary_ijk),
+                  f32(mesh.abs_boundary_normal), f32(mesh.abs_boundary_jacobian2Dw), self.nsrc,
+                  src_arr, src_isp, self.nrec, rec_isp, rec_h, self.nstep, simulation_type,
+                  int(save_forward))
+        if self.aniso:
+            cij = np.ascontiguousarray(cij, F32).reshape(mesh.nspec, 125, 21)
+            cs = [np.ascontiguousarray(cij[..., q]) for q in range(21)]
+        else:
+            cs = [None] * 21
+        _lib.call("prepare_fields_elastic_device_", self.h, rmx, rmy, rmz, rho_vp, rho_vs, kappa, mu,
+                  int(self.aniso), *cs, mesh.num_phase_ispec, i32(mesh.phase_ispec_inner),
+                  mesh.nspec_outer, mesh.nspec_inner, int(compute_and_store_strain))
+        if simulation_type == 3:
+            _lib.call("prepare_fields_elastic_adj_dev_", self.h, 3 * mesh.nglob,
+                      int(compute_and_store_strain), rho, int(self.aniso), int(approximate_hess))
+            _lib.call("prepare_sim2_or_3_const_device_", self.h, self.nrec, self.nstep)
+        if static_precision:
+            _lib.call("fwat_set_static_precision_", self.h, int(static_precision))
+        if self.nsrc:
+            self.stf = sources.stf(self.nstep, self.dt)
+            _lib.call("fwat_set_stf_", self.h, self.stf, self.nstep)
+
+    # ------------------------------------------------------------ runs
+    def set_adjoint_sources(self, adj):
+        adj = np.ascontiguousarray(adj, F32).reshape(self.nstep, self.nrec, 3)
+        _lib.call("fwat_set_adjoint_sources_", self.h, adj, self.nstep)
+
+    def iterate_time(self, fused=True, timed=False):
+        _lib.call("fwat_iterate_time_", self.h, self.simulation_type, self.dt, int(fused), int(timed))
+
+    def force_time(self):
+        ms = np.zeros(1)
+        n = np.zeros(1, np.int32)
+        _lib.call("fwat_last_force_time_", self.h, ms, n)
+        return float(ms[0]), int(n[0])
+
+    def seismograms(self):
+        out = np.zeros((self.nstep, self.nrec, 3), F32)
+        if self.nrec:
+            _lib.call("transfer_seismograms_from_device_", self.h, out)
+        return out
+
+    def fields(self):
+        n3 = 3 * self.mesh.nglob
+        u, v, a = (np.zeros((self.mesh.nglob, 3), F32) for _ in range(3))
+        _lib.call("transfer_fields_el_from_device_", n3, u, v, a, self.h)
+        return u, v, a
+
+    def b_fields(self):
+        n3 = 3 * self.mesh.nglob
+        u, v, a = (np.zeros((self.mesh.nglob, 3), F32) for _ in range(3))
+        _lib.call("transfer_b_fields_from_device_", n3, u, v, a, self.h)
+        return u, v, a
+
+    def take_forward(self, fwd):
+        """In-HBM hand-off: forward final state -> b_ fields, Stacey store moved."""
+        _lib.call("fwat_copy_forward_to_backward_", fwd.h, self.h)
+        if self.mesh.num_abs_boundary_faces:
+            _lib.call("fwat_move_stacey_store_", fwd.h, self.h)
+
+    def kernels(self):
+        """save_adjoint_kernels-scaled kernels, float64 [nspec,125] (or [nspec,125,21])."""
+        nloc = (self.mesh.nspec, 125)
+        rho_kl = np.zeros(nloc)
+        if self.aniso:
+            c = np.zeros(nloc + (21,))
+            mu = np.zeros(nloc)
+            kap = np.zeros(nloc)
+            _lib.call("transfer_kernels_el_to_host_dp_", self.h, rho_kl, mu, kap, self.mesh.nspec)
+            _lib.call("transfer_kernels_ani_to_host_dp_", self.h, c, self.mesh.nspec)
+            return {"rho_kl": rho_kl, "cijkl_kl": c}
+        mu = np.zeros(nloc)
+        kap = np.zeros(nloc)
+        _lib.call("transfer_kernels_el_to_host_dp_", self.h, rho_kl, mu, kap, self.mesh.nspec)
+        return {"rho_kl": rho_kl, "mu_kl": mu, "kappa_kl": kap}
+
+    def close(self):
+        if self.h:
+            _lib.call("prepare_cleanup_device_", self.h, 0, 1, 0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def derived_kernels(k, rho, kappa, mu):
+    """alpha/beta/rho' kernels from the scaled rho/mu/kappa kernels
+    (names of kernel_io.f90:123-133; Tromp, Tape & Liu 2005)."""
+    kappa = kappa.astype(np.float64)
+    mu = mu.astype(np.float64)
+    r = 4.0 * mu / (3.0 * kappa)
+    return {"alpha_kl": 2.0 * (1.0 + r) * k["kappa_kl"],
+            "beta_kl": 2.0 * (k["mu_kl"] - r * k["kappa_kl"]),
+            "rhop_kl": k["rho_kl"] + k["kappa_kl"] + k["mu_kl"]}
+
+
+def event_gradient(mesh, model, obs, sources, receivers, dt, nstep, *, cij=None, fused=True,
+                   timed=False, windows=None, static_precision=0, approximate_hess=False):
+    """PrepareFWD%simulation for one event: forward (SAVE_FORWARD), waveform
+    misfit + adjoint source, adjoint run with reconstruction, kernels."""
+    from .misfit import event_adjoint_sources
+    rho, vp, vs = model
+    fwd = ElasticRun(mesh, rho, vp, vs, sources, receivers, dt, nstep, 1, True, cij=cij,
+                     static_precision=static_precision)
+    fwd.iterate_time(fused=fused, timed=timed)
+    syn = fwd.seismograms()
+    misfit, adj = event_adjoint_sources(obs, syn, dt, windows)
+    adjr = ElasticRun(mesh, rho, vp, vs, sources, receivers, dt, nstep, 3, False, cij=cij,
+                      compute_and_store_strain=not fused, static_precision=static_precision,
+                      approximate_hess=approximate_hess)
+    adjr.take_forward(fwd)
+    fwd.close()
+    adjr.set_adjoint_sources(adj)
+    adjr.iterate_time(fused=fused, timed=timed)
+    k = adjr.kernels()
+    adjr.close()
+    return misfit, syn, k

@@ -1,0 +1,236 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars (from the north star): seismograms within 1e-5 relative L2, kernels
+within 1e-4 relative L2 of the oracle on identical seeded inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from specfwat_amd import _lib
+from specfwat_amd.mesh import BoxMesh
+from specfwat_amd.model import hti_model
+from specfwat_amd.sources import Receivers, Sources
+from specfwat_amd.solver import ElasticRun
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+KERN_TOL = 1e-4
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_case(nex=4, ney=4, nez=3, h=2500.0, seed=0, perturb=0.0, force_irregular=False,
+              nstep=150, dt=0.02, aniso=False, nrec=5):
+    print(f"case nex={nex} ney={ney} nez={nez} perturb={perturb} irr={force_irregular} "
+          f"aniso={aniso} nstep={nstep} seed={seed}", flush=True)
+    mesh = BoxMesh(nex, ney, nez, nex * h, ney * h, nez * h, perturb=perturb,
+                   force_irregular=force_irregular)
+    rng = np.random.default_rng(seed)
+    f = (1 + 0.03 * rng.standard_normal(mesh.nglob))[mesh.ibool0]
+    rho = (2700 * f).astype(np.float32)
+    vp = (6000 * f).astype(np.float32)
+    vs = (3500 * f).astype(np.float32)
+    cij = hti_model(mesh, rho, vp, vs, seed=seed + 5, block=h * 2) if aniso else None
+    sxyz = [[nex * h * 0.43, ney * h * 0.57, -nez * h * 0.45]]
+    mt = [list(1e15 * rng.standard_normal(6))]
+    f0 = [0.8]
+    rxyz = np.stack([rng.uniform(0.1, 0.9, nrec) * nex * h, rng.uniform(0.1, 0.9, nrec) * ney * h,
+                     np.zeros(nrec)], axis=1)
+    return dict(mesh=mesh, rho=rho, vp=vp, vs=vs, cij=cij, sxyz=sxyz, mt=mt, f0=f0, rxyz=rxyz,
+                dt=dt, nstep=nstep, perturb=perturb, force_irregular=force_irregular)
+
+
+def oracle_of(c, precision):
+    m = c["mesh"]
+    return orc.Oracle(m.nex, m.ney, m.nez, m.lx, m.ly, m.lz, c["rho"].ravel(), c["vp"].ravel(),
+                      c["vs"].ravel(), cij=None if c["cij"] is None else c["cij"].reshape(-1, 21),
+                      dt=c["dt"], nstep=c["nstep"], src_xyz=c["sxyz"], src_mt=c["mt"],
+                      src_f0=c["f0"], rec_xyz=c["rxyz"], perturb=c["perturb"],
+                      force_irregular=c["force_irregular"], precision=precision)
+
+
+def gpu_forward(c, save=True, fused=True, static_precision=0):
+    m = c["mesh"]
+    src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+    rec = Receivers(m, c["rxyz"])
+    run = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, save,
+                     cij=c["cij"], static_precision=static_precision)
+    run.iterate_time(fused=fused)
+    return run, src, rec
+
+
+def gpu_gradient(c, adj, fused=True, static_precision=0, hess=False):
+    fwd, src, rec = gpu_forward(c, True, fused, static_precision)
+    syn = fwd.seismograms()
+    m = c["mesh"]
+    a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False,
+                   cij=c["cij"], compute_and_store_strain=not fused,
+                   static_precision=static_precision, approximate_hess=hess)
+    a.take_forward(fwd)
+    fwd.close()
+    a.set_adjoint_sources(adj)
+    a.iterate_time(fused=fused)
+    k = a.kernels()
+    bfin = a.b_fields()
+    a.close()
+    return syn, k, bfin
+
+
+@pytest.mark.parametrize("kind", ["regular", "irregular", "forced_irregular"])
+def test_forward_seismograms_match_oracle(kind):
+    c = make_case(perturb=0.12 if kind == "irregular" else 0.0,
+                  force_irregular=kind == "forced_irregular", seed=1)
+    run, _, _ = gpu_forward(c, save=False)
+    syn = run.seismograms()
+    run.close()
+    o32 = oracle_of(c, 32).forward(save=False)
+    o64 = oracle_of(c, 64).forward(save=False)
+    e32, e64 = rel(syn, o32), rel(syn, o64)
+    print(f"{kind}: rel L2 vs FP32 oracle {e32:.2e}, vs FP64 oracle {e64:.2e}")
+    assert np.abs(o64).max() > 0
+    assert e32 < SEIS_TOL and e64 < SEIS_TOL
+
+
+def test_unfused_stage_order_matches_fused():
+    c = make_case(seed=2)
+    r1, _, _ = gpu_forward(c, save=False, fused=True)
+    r2, _, _ = gpu_forward(c, save=False, fused=False)
+    s1, s2 = r1.seismograms(), r2.seismograms()
+    r1.close(); r2.close()
+    assert rel(s2, s1) < 1e-6
+
+
+def test_adjoint_kernels_match_oracle():
+    c = make_case(seed=3)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)  # waveform-difference-like forcing
+    syn, k, bfin = gpu_gradient(c, adj)
+    ko = o.adjoint(adj)
+    assert rel(syn, obs) < SEIS_TOL
+    for n in ("rho_kl", "mu_kl", "kappa_kl"):
+        e = rel(k[n], ko[n])
+        print(f"{n}: rel L2 {e:.2e}")
+        assert e < KERN_TOL, n
+    # reconstruction returns to the zero initial state
+    u_end = np.abs(bfin[0]).max()
+    u_scale = np.abs(o.final_state()[0]).max()
+    assert u_end < 1e-3 * u_scale + 1e-30
+
+
+def test_unfused_adjoint_matches_oracle():
+    c = make_case(seed=4, nstep=100)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = obs.astype(np.float32)
+    _, k, _ = gpu_gradient(c, adj, fused=False)
+    ko = o.adjoint(adj)
+    for n in ("rho_kl", "mu_kl", "kappa_kl"):
+        assert rel(k[n], ko[n]) < KERN_TOL, n
+
+
+def test_irregular_adjoint_matches_oracle():
+    c = make_case(seed=5, perturb=0.1, nstep=100)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = obs.astype(np.float32)
+    syn, k, _ = gpu_gradient(c, adj)
+    ko = o.adjoint(adj)
+    assert rel(syn, obs) < SEIS_TOL
+    for n in ("rho_kl", "mu_kl", "kappa_kl"):
+        assert rel(k[n], ko[n]) < KERN_TOL, n
+
+
+def test_anisotropic_forward_and_cijkl_kernels():
+    c = make_case(seed=6, aniso=True, nstep=100)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = obs.astype(np.float32)
+    syn, k, _ = gpu_gradient(c, adj)
+    ko = o.adjoint(adj)
+    assert rel(syn, obs) < SEIS_TOL
+    assert rel(k["rho_kl"], ko["rho_kl"]) < KERN_TOL
+    assert rel(k["cijkl_kl"].reshape(-1, 21), ko["cijkl_kl"]) < KERN_TOL
+
+
+def test_per_step_c_abi_matches_whole_loop():
+    """Drive the forward run step by step through SPECFEM's per-step entry
+    points (update_displacement, compute_forces, Stacey, sources, kernel_3,
+    seismograms) and compare with fwat_iterate_time_."""
+    c = make_case(seed=7, nstep=60)
+    ref, src, rec = gpu_forward(c, save=False)
+    s_ref = ref.seismograms()
+    ref.close()
+    m = c["mesh"]
+    run = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, False)
+    dt = c["dt"]
+    stf = src.stf(c["nstep"], dt)
+    for it in range(1, c["nstep"] + 1):
+        _lib.call("update_displacement_cuda_", run.h, dt, 0.5 * dt * dt, 0.5 * dt, 1)
+        for iphase in (1, 2):
+            _lib.call("compute_forces_viscoelastic_cuda_", run.h, iphase, dt, m.nspec_outer,
+                      m.nspec_inner, 0, 0, 1)
+            _lib.call("compute_stacey_viscoelastic_cuda_", run.h, iphase, it, 1)
+            if iphase == 1:
+                _lib.call("compute_add_sources_viscoelastic_cuda_", run.h, 1,
+                          np.ascontiguousarray(stf[it - 1]), 1)
+        _lib.call("kernel_3_a_cuda_", run.h, 0.5 * dt, 1)
+        _lib.call("compute_seismograms_cuda_", run.h, it)
+    s = run.seismograms()
+    run.close()
+    assert rel(s, s_ref) < 1e-6
+
+
+def test_mixed_precision_statics_error_bound():
+    c = make_case(seed=8, nstep=100, perturb=0.1)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = obs.astype(np.float32)
+    ko = o.adjoint(adj)
+    for sp in (1, 2):
+        syn, k, _ = gpu_gradient(c, adj, static_precision=sp)
+        es = rel(syn, obs)
+        ek = max(rel(k[n], ko[n]) for n in ("rho_kl", "mu_kl", "kappa_kl"))
+        print(f"static precision {sp}: seismogram err {es:.2e}, kernel err {ek:.2e}")
+        assert es < 1e-2 and ek < 1e-2
+
+
+def test_smoother_matches_oracle_restatement():
+    """smooth_pde_cuda_ against the loop restatement of smooth.f90.  The
+    reference arithmetic on regular elements (xix_regular in all nine
+    slots) is unstable on an all-regular mesh, so that case is compared over
+    its first steps only; the diagonal mode and irregular meshes run the
+    full ntstep."""
+    from specfwat_amd.smooth import smooth_sem_pde
+    for faithful, perturb, steps in ((True, 0.0, 5), (False, 0.0, None), (True, 0.1, None)):
+        m = BoxMesh(6, 6, 4, 6000.0, 6000.0, 4000.0, perturb=perturb)
+        rng = np.random.default_rng(11)
+        dat_in = rng.standard_normal((m.nspec, 125)).astype(np.float32)
+        sh, sv = 800.0, 400.0
+        dat, dglob, nt = smooth_sem_pde(m, dat_in, sh, sv, faithful=faithful, nstep=steps)
+        do, dgo, nto = orc.smooth_pde(m.smooth_arrays(), dat_in.ravel(), sh, sv, m.distance_min_glob(),
+                                      faithful=faithful, max_steps=steps or 0)
+        if steps is None:
+            assert nt == nto
+        assert np.isfinite(dgo).all()
+        e = rel(dglob, dgo)
+        print(f"smoother faithful={faithful} perturb={perturb}: steps={nt} rel L2 {e:.2e}")
+        assert e < 1e-5
+        assert np.array_equal(dat.ravel(), dglob[m.ibool0.ravel()])
+
+
+def test_smoother_batch_matches_single():
+    from specfwat_amd.smooth import smooth_batch, smooth_sem_pde
+    m = BoxMesh(6, 6, 4, 6000.0, 6000.0, 4000.0)
+    rng = np.random.default_rng(12)
+    fields = rng.standard_normal((3, m.nspec, 125)).astype(np.float32)
+    out, nt, _ = smooth_batch(m, fields, 800.0, 400.0, faithful=False)
+    for q in range(3):
+        _, d1, _ = smooth_sem_pde(m, fields[q], 800.0, 400.0, faithful=False)
+        assert rel(out[q], d1) < 1e-6
