@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device copy of BYTES (counter calibration) and exit")
     return p.parse_args()
 
 
@@ -62,6 +64,12 @@ def main():
               "(the event-parallel multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     from specfwat_amd import _lib
+    if a.copy_probe:
+        ms = np.zeros(1)
+        _lib.call("fwat_copy_probe_", a.copy_probe, 5, ms)
+        print(json.dumps({"copy_probe_bytes": a.copy_probe, "avg_ms": float(ms[0]),
+                          "GBps_read_plus_write": 2 * a.copy_probe / (ms[0] * 1e-3) / 1e9}), flush=True)
+        return
     from specfwat_amd.mesh import BoxMesh
     from specfwat_amd.model import perturbed_model
     from specfwat_amd.sources import Receivers, Sources, surface_grid
@@ -126,6 +134,7 @@ def main():
     wall = te - ts
     fwd_ms, fwd_n = f.force_time()
     adj_ms, adj_n = ad.force_time()
+    f_asm, a_asm = f.assembly_time(), ad.assembly_time()
     k = ad.kernels()
     f.close(); ad.close()
     if not all(np.isfinite(v).all() for v in k.values()):
@@ -134,27 +143,44 @@ def main():
 
     nglob, nloc = mesh.nglob, mesh.nspec * 125
     nirr = mesh.nspec_irregular * 125
+    nfp = mesh.num_abs_boundary_faces * 25
+    ncontrib = nloc + nfp + 125 * (src.nsources + rec.nrec)
     value = nglob * K / wall
-    # algorithmic bytes per launch of the dominant kernel (two-field adjoint force pass):
-    # statics ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local point,
-    # fp64 mu/kappa accumulators read+write 32 per local point,
-    # two fields: displacement gather 12 and accel read-modify-write 24 per unique point
+    fwd_asm_ms, fwd_asm_n = f_asm
+    adj_asm_ms, adj_asm_n = a_asm
+    # Algorithmic bytes per launch (each array touched once, in bytes):
     if fused:
-        adj_bytes = 44.0 * nloc + 40.0 * nirr + 72.0 * nglob
-        adj_avg_ms = adj_ms / max(adj_n, 1)
-        kname = "k_forces<2,iso,fused-kernels>"
+        # two-field element pass: ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local
+        # point, fp64 mu/kappa accumulators read+write 32, element-local forces written
+        # for both fields 24; displacement gather of both fields 12 + 12 per unique point
+        kinds = {
+            "k_forces<2,iso,fused-kernels>": (68.0 * nloc + 40.0 * nirr + 24.0 * nglob, adj_ms, adj_n),
+            # two-field assembly: CSR offsets 4 per point + indices 4 per contribution,
+            # element-local forces of both fields 24 per local point, Stacey terms of both
+            # fields 24 per face point, rmass_inv 12, u,v of both fields read 48 + write 48,
+            # fp64 rho kernel read+write 16 per unique point
+            "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp
+                                      + 12.0 * nglob + 96.0 * nglob + 16.0 * nglob, adj_asm_ms, adj_asm_n),
+        }
+        fwd_bytes = 24.0 * nloc + 40.0 * nirr + 12.0 * nglob
+        fwd_asm_bytes = 4.0 * nglob + 4.0 * ncontrib + 12.0 * nloc + 12.0 * nfp + 12.0 * nglob + 48.0 * nglob
     else:
-        adj_bytes = 12.0 * nloc + 24.0 * nloc + 40.0 * nirr + 36.0 * nglob
-        adj_avg_ms = adj_ms / max(adj_n, 1) / 2.0
-        kname = "k_forces<1,iso,strain-store>"
-    fwd_bytes = 12.0 * nloc + 40.0 * nirr + 36.0 * nglob
-    fwd_avg_ms = fwd_ms / max(fwd_n, 1)
-    achieved = adj_bytes / (adj_avg_ms * 1e-3) / 1e9
+        # SPECFEM stage order: one field per launch, atomic scatter, strain store
+        kinds = {"k_forces<1,iso,strain-store,atomic>": (12.0 * nloc + 24.0 * nloc + 40.0 * nirr + 36.0 * nglob,
+                                                         adj_ms, adj_n)}
+        fwd_bytes = 12.0 * nloc + 40.0 * nirr + 36.0 * nglob
+        fwd_asm_bytes = 0.0
+    kname = max(kinds, key=lambda k: kinds[k][1])
+    kbytes, kms, kn = kinds[kname]
+    avg_ms = kms / max(kn, 1)
+    achieved = kbytes / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tf_path = os.path.join(HERE, "profiles", "force_adjoint_c2_traffic.json")
+    tf_path = os.path.join(HERE, "profiles", "c2_fwd_adj_hbm_traffic.md")
     if os.path.exists(tf_path) and fused and a.config == "c2" and a.static_precision == 0:
-        with open(tf_path) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+        for line in open(tf_path):
+            if line.startswith("hbm_bytes_per_launch " + kname):
+                traffic = float(line.split("=")[1])
+    fwd_avg_ms = fwd_ms / max(fwd_n, 1)
     result = {
         "metric": "GLL-pt-steps/s/event fwd+adj",
         "value": value,
@@ -175,12 +201,16 @@ def main():
                    "path": "fused" if fused else "SPECFEM stage order (strain stores)"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": adj_bytes, "avg_launch_ms": adj_avg_ms,
-                     "launches": adj_n},
+                     "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
         "breakdown": {"wall_s": wall, "forward_s": tf - ts, "adjoint_s": te - tf,
+                      "kernels": {k: {"avg_ms": v[1] / max(v[2], 1),
+                                      "GBps": v[0] / (v[1] / max(v[2], 1) * 1e-3) / 1e9 if v[1] > 0 else None}
+                                  for k, v in kinds.items()},
                       "fwd_force_avg_ms": fwd_avg_ms,
                       "fwd_force_GBps": fwd_bytes / (fwd_avg_ms * 1e-3) / 1e9 if fwd_avg_ms > 0 else None,
-                      "adj_force_avg_ms": adj_avg_ms},
+                      "fwd_assembly_avg_ms": fwd_asm_ms / max(fwd_asm_n, 1),
+                      "fwd_assembly_GBps": (fwd_asm_bytes / (fwd_asm_ms / fwd_asm_n * 1e-3) / 1e9
+                                            if fwd_asm_n and fwd_asm_ms > 0 else None)},
     }
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, sxyz, mt, rec, a.cpu_steps)

@@ -134,6 +134,7 @@ void fwat_set_adjoint_sources_(long *Mesh_pointer, realw *h_adj, int *NSTEP);
  * two-field force pass with in-pass fp64 kernel accumulation, 0 = SPECFEM stage order */
 void fwat_iterate_time_(long *Mesh_pointer, int *SIMULATION_TYPE, realw *DT, int *fused, int *timed);
 void fwat_last_force_time_(long *Mesh_pointer, double *ms, int *launches);
+void fwat_last_assembly_time_(long *Mesh_pointer, double *ms, int *launches);
 void fwat_synchronize_(long *Mesh_pointer);
 /* plain device copy of nbytes repeated reps times (counter calibration probe) */
 void fwat_copy_probe_(long *nbytes, int *reps, double *avg_ms);

@@ -825,7 +825,13 @@ struct Sim : SimBase {
                double* cijkl_kl, float* b_final) override {
     size_t n3 = (size_t)m.nglob * 3, nloc = (size_t)m.nspec * NG3;
     size_t rec_sz = (size_t)m.nfaces * NG2 * 3;
+    // Reconstructed field: adjoint step it pairs with forward state
+    // m = N + 1 - it (the discrete adjoint of the central-difference scheme
+    // pairs lambda_n with u_n).  Step 1 uses the saved last state itself:
+    // its half-step velocity v_N + dt/2 a_N is set here so the corrector of
+    // step 1 returns v_N; steps it >= 2 take one step back from m + 1.
     bu = u; bv = v; ba = a;
+    for (size_t q = 0; q < n3; ++q) bv[q] += (T)0.5 * (T)p.dt * ba[q];
     au.assign(n3, 0); av.assign(n3, 0); aa.assign(n3, 0);
     std::vector<double> krho(nloc, 0.0), kmu(nloc, 0.0), kkap(nloc, 0.0);
     std::vector<double> kc(p.aniso ? nloc * 21 : 0, 0.0);
@@ -833,14 +839,15 @@ struct Sim : SimBase {
     const int N = p.nstep;
     std::vector<T> sa((size_t)NG3 * 6), sb((size_t)NG3 * 6);
     for (int it = 1; it <= N; ++it) {
-      const int mstate = N - it;
-      // backward reconstruction of forward state mstate
-      predictor(bu, bv, ba, -dt);
-      forces(bu.data(), ba.data());
-      if (mstate >= 1) {
-        if (p.stacey) stacey(bv, ba, 2, &store[rec_sz * (mstate - 1)]);
-        add_sources(ba, mstate);
+      const int mstate = N + 1 - it;
+      if (it >= 2) {
+        predictor(bu, bv, ba, -dt);
+      } else {
+        std::fill(ba.begin(), ba.end(), (T)0);
       }
+      forces(bu.data(), ba.data());
+      if (p.stacey) stacey(bv, ba, 2, &store[rec_sz * (mstate - 1)]);
+      add_sources(ba, mstate);
       corrector(bv, ba, -dt);
       // adjoint field, forward in reversed time
       predictor(au, av, aa, dt);

@@ -48,6 +48,7 @@ SIGNATURES = {
     "fwat_set_adjoint_sources_": "LFi",
     "fwat_iterate_time_": "Lifii",
     "fwat_last_force_time_": "LDI",
+    "fwat_last_assembly_time_": "LDI",
     "fwat_synchronize_": "L",
     "fwat_copy_probe_": "liD",
     "fwat_copy_forward_to_backward_": "LL",

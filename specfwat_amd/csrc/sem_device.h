@@ -146,6 +146,8 @@ struct SemMesh {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   double last_force_ms = 0.0;
   int last_force_launches = 0;
+  double last_asm_ms = 0.0;
+  int last_asm_launches = 0;
 };
 
 inline SemMesh* sem_mesh(long* Mesh_pointer) {

@@ -14,8 +14,11 @@
 // The reconstructed field runs the same scheme with -dt from the forward
 // run's last state, replaying the stored Stacey terms S and the source, which
 // inverts the forward steps exactly in exact arithmetic.  Adjoint step it
-// pairs with reconstructed state m = NSTEP - it; the adjoint source sample
-// NSTEP - it (0-based) is injected at adjoint step it.
+// pairs with reconstructed state m = NSTEP + 1 - it (the pairing of the
+// discrete adjoint of the central-difference scheme, lambda_n with u_n):
+// step 1 uses the saved last state, each later step goes one step back and
+// replays Stacey record m and the source of forward step m.  The adjoint
+// source sample NSTEP - it (0-based) is injected at adjoint step it.
 //
 // Kernels (Tromp, Tape & Liu 2005), accumulated in fp64:
 //   rho_kl   += dt  a_adj . u_rec              (per global point, expanded on transfer)
@@ -178,6 +181,14 @@ __global__ void k_stacey_face(int nfp, const int* __restrict__ figlob, const rea
   out[3 * (size_t)q] = (rvp * vn * nx + rvs * (vx - vn * nx)) * jw;
   out[3 * (size_t)q + 1] = (rvp * vn * ny + rvs * (vy - vn * ny)) * jw;
   out[3 * (size_t)q + 2] = (rvp * vn * nz + rvs * (vz - vn * nz)) * jw;
+}
+
+// Start of a reconstruction: the saved last state (u_N, v_N, a_N) becomes
+// the state of adjoint step 1; its velocity is set to v_N + dt/2 a_N so that
+// the corrector of step 1 returns v_N.
+__global__ void k_b_init(size_t n3, realw* __restrict__ v, const realw* __restrict__ a, realw dto2) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) v[i] += dto2 * a[i];
 }
 
 // rho (and Hessian) kernel accumulation of the per-step API path
@@ -690,11 +701,16 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   const bool fz = *fused != 0;
   m->last_force_ms = 0.0;
   m->last_force_launches = 0;
-  ForceTimer timer;
+  m->last_asm_ms = 0.0;
+  m->last_asm_launches = 0;
+  ForceTimer timer, atimer;
   ForceTimer* tm = nullptr;
+  ForceTimer* ta = nullptr;
   if (*timed) {
     timer.begin((size_t)N * 2);
+    atimer.begin((size_t)N);
     tm = &timer;
+    ta = &atimer;
   }
   if (m->nsources > 0 && !m->d_stf) SEM_FAIL("fwat_set_stf_ not called");
   hipStream_t s = m->stream;
@@ -714,7 +730,9 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         stage_seismo(m, it - 1);
         AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
                     nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
+        if (ta) ta->mark(s);
         stage_assemble(m, it == N, false, F0, F0, 0.0);
+        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
       }
     } else {
       for (int it = 1; it <= N; ++it) {
@@ -734,29 +752,33 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
     if (m->nrec && !m->d_adj) SEM_FAIL("fwat_set_adjoint_sources_ not called");
     if (!fz && (!m->d_epsilon || !m->d_b_epsilon)) SEM_FAIL("unfused adjoint run needs COMPUTE_AND_STORE_STRAIN");
     const double dtk = (double)dt;
+    const size_t n3 = (size_t)m->nglob * 3;
+    hipLaunchKernelGGL(k_b_init, dim3(nblk(n3)), dim3(TPB), 0, s, n3, m->d_b_veloc, m->d_b_accel, 0.5f * dt);
     if (fz) {
       const int nfp = m->nfaces * 25;
-      stage_predictor(m, true, dt);   // reconstructed field: first step back from the last state
       stage_predictor(m, false, dt);  // adjoint field (from rest)
       for (int it = 1; it <= N; ++it) {
-        const int ms = N - it;
+        const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
         stage_forces_timed(m, 2, 2, dt, tm, true);
         realw* face_a = nullptr;
         if (m->absorbing && m->nfaces) {
           face_a = m->d_face_scratch;
           stage_stacey_face(m, m->d_veloc, face_a);
         }
-        const realw* face_b = (ms >= 1 && m->absorbing && m->nfaces) ? m->d_stacey_store + (size_t)(ms - 1) * nfp * 3 : nullptr;
-        AsmField F0{m->d_floc, face_a, nullptr, m->nrec ? m->d_adj + (size_t)ms * m->nrec * 3 : nullptr,
+        const realw* face_b = (m->absorbing && m->nfaces) ? m->d_stacey_store + (size_t)rec0 * nfp * 3 : nullptr;
+        AsmField F0{m->d_floc, face_a, nullptr, m->nrec ? m->d_adj + (size_t)rec0 * m->nrec * 3 : nullptr,
                     m->d_displ, m->d_veloc, m->d_accel, dt};
-        AsmField F1{m->d_floc_b, face_b, (ms >= 1 && m->nsources) ? m->d_stf + (size_t)(ms - 1) * m->nsources : nullptr,
+        AsmField F1{m->d_floc_b, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
                     nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
+        if (ta) ta->mark(s);
         stage_assemble(m, it == N, true, F0, F1, dtk);
+        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
       }
     } else {
       for (int it = 1; it <= N; ++it) {
-        const int ms = N - it;
-        stage_predictor(m, true, dt);
+        const int rec0 = N - it;
+        if (it >= 2) stage_predictor(m, true, dt);
+        else hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n3)), dim3(TPB), 0, s, m->d_b_accel, n3);
         stage_predictor(m, false, dt);
         if (tm) tm->mark(s);
         sem_launch_forces(m, 3, 1, 0, m->nspec, dt, nullptr, false);
@@ -765,12 +787,10 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
           tm->mark(s);
           m->last_force_launches += 2;
         }
-        if (ms >= 1) {
-          stage_stacey(m, true, 2, (size_t)(ms - 1));
-          if (m->nsources) stage_sources(m, true, m->d_stf + (size_t)(ms - 1) * m->nsources);
-        }
+        stage_stacey(m, true, 2, (size_t)rec0);
+        if (m->nsources) stage_sources(m, true, m->d_stf + (size_t)rec0 * m->nsources);
         stage_stacey(m, false, 0, 0);
-        if (m->nrec) stage_adjoint_sources(m, m->d_adj + (size_t)ms * m->nrec * 3);
+        if (m->nrec) stage_adjoint_sources(m, m->d_adj + (size_t)rec0 * m->nrec * 3);
         stage_corrector(m, true, dt);
         stage_corrector(m, false, dt);
         compute_kernels_elastic_cuda_(Mesh_pointer, DT);
@@ -782,6 +802,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
   if (tm) m->last_force_ms = tm->total_ms();
+  if (ta) m->last_asm_ms = ta->total_ms();
 }
 
 // Sum of the force-kernel durations of the last fwat_iterate_time_ call (ms)
@@ -818,6 +839,14 @@ void fwat_copy_probe_(long* nbytes, int* reps, double* avg_ms) {
   HIP_CHECK(hipEventDestroy(e1));
   HIP_CHECK(hipFree(a));
   HIP_CHECK(hipFree(b));
+}
+
+// Summed durations (ms) and launch counts of the assembly passes of the
+// last timed fwat_iterate_time_ call.
+void fwat_last_assembly_time_(long* Mesh_pointer, double* ms, int* launches) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  *ms = m->last_asm_ms;
+  *launches = m->last_asm_launches;
 }
 
 void fwat_synchronize_(long* Mesh_pointer) {

@@ -44,6 +44,7 @@ class BoxMesh:
         self.nex, self.ney, self.nez = nex, ney, nez
         self.lx, self.ly, self.lz = float(lx), float(ly), float(lz)
         self.nspec = nex * ney * nez
+        self.perturb = float(perturb)
         self.gll = GLL()
         hx, hy, hz = self.lx / nex, self.ly / ney, self.lz / nez
         self.hx, self.hy, self.hz = hx, hy, hz
@@ -233,10 +234,25 @@ class BoxMesh:
     def locate(self, x, y, z):
         """First element (ascending ispec) containing the point and its
         reference coordinates (Newton on the trilinear map)."""
+        if not self.perturb:
+            # undeformed box: per-axis lowest element whose interval holds the
+            # coordinate (the lowest ispec), reference coordinate in closed form
+            out, ref = [], []
+            for v, lo, h, n in ((x, 0.0, self.hx, self.nex), (y, 0.0, self.hy, self.ney),
+                                (z, -self.lz, self.hz, self.nez)):
+                t = (v - lo) / h
+                if t < -1e-9 or t > n + 1e-9:
+                    raise ValueError(f"point ({x}, {y}, {z}) is outside the mesh")
+                e = min(max(int(np.ceil(t - 1e-9)) - 1, 0), n - 1)
+                out.append(e)
+                ref.append(min(max(2.0 * (t - e) - 1.0, -1.0), 1.0))
+            return int(out[0] + self.nex * (out[1] + self.ney * out[2])), np.array(ref)
         X = self.elem_anchors
-        lo, hi = X.min(axis=1), X.max(axis=1)
+        if getattr(self, "_bbox", None) is None:
+            lo, hi = X.min(axis=1), X.max(axis=1)
+            self._bbox = (lo, hi, 1e-9 * (hi - lo))
+        lo, hi, tol = self._bbox
         p = np.array([x, y, z], float)
-        tol = 1e-9 * (hi - lo)
         cand = np.nonzero(np.all((p >= lo - tol) & (p <= hi + tol), axis=1))[0]
         for e in cand:
             r = np.zeros(3)

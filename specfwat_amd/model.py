@@ -49,8 +49,19 @@ def locate_bissection(valx, x):
     return ind
 
 
-def interp3(valx, valy, valz, valin, x, y, z):
+def interp3(valx, valy, valz, valin, x, y, z, chunk=1 << 20):
     """interp3_0_cr: float32 trilinear interpolation; valin[ix, iy, iz]."""
+    x = np.asarray(x, F32); y = np.asarray(y, F32); z = np.asarray(z, F32)
+    if x.size > chunk:
+        out = np.empty(x.shape, F32)
+        for c0 in range(0, x.size, chunk):
+            sl = slice(c0, c0 + chunk)
+            out.ravel()[sl] = _interp3(valx, valy, valz, valin, x.ravel()[sl], y.ravel()[sl], z.ravel()[sl])
+        return out
+    return _interp3(valx, valy, valz, valin, x, y, z)
+
+
+def _interp3(valx, valy, valz, valin, x, y, z):
     valx = np.asarray(valx, F32); valy = np.asarray(valy, F32); valz = np.asarray(valz, F32)
     x = np.asarray(x, F32); y = np.asarray(y, F32); z = np.asarray(z, F32)
     nx, ny, nz = len(valx), len(valy), len(valz)

@@ -108,6 +108,12 @@ class ElasticRun:
         _lib.call("fwat_last_force_time_", self.h, ms, n)
         return float(ms[0]), int(n[0])
 
+    def assembly_time(self):
+        ms = np.zeros(1)
+        n = np.zeros(1, np.int32)
+        _lib.call("fwat_last_assembly_time_", self.h, ms, n)
+        return float(ms[0]), int(n[0])
+
     def seismograms(self):
         out = np.zeros((self.nstep, self.nrec, 3), F32)
         if self.nrec:
