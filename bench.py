@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
+    p.add_argument("--no-tune", action="store_true", help="skip the element-pass launch-shape autotuner")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     return p.parse_args()
@@ -91,6 +92,13 @@ def main():
     dt = w["dt"]
     log(f"model/source/receivers ready ({time.time() - t0:.1f}s)")
     fused = not a.unfused
+    tuning = {"regular_path": 1, "blocks_per_cu_fwd": 0, "blocks_per_cu_adj": 8}
+    if fused and not a.no_tune:
+        from specfwat_amd.tune import tune_element_passes
+        tuning, table = tune_element_passes(mesh, smooth_model, src, rec, dt,
+                                            static_precision=a.static_precision)
+        log("autotuner (regular_path, blocks/CU, fwd ms, adj ms): "
+            + ", ".join(f"({r},{b},{x:.3f},{y:.3f})" for r, b, x, y in table) + f" -> {tuning}")
 
     def observed(nstep):
         r = ElasticRun(mesh, *true_model, src, rec, dt, nstep, 1, False)
@@ -104,6 +112,8 @@ def main():
                        static_precision=a.static_precision)
         ad = ElasticRun(mesh, *smooth_model, src, rec, dt, nstep, 3, False,
                         compute_and_store_strain=not fused, static_precision=a.static_precision)
+        f.set_tuning(**tuning)
+        ad.set_tuning(**tuning)
         return f, ad
 
     def event(f, ad, obs, timed):
@@ -149,12 +159,13 @@ def main():
     fwd_asm_ms, fwd_asm_n = f_asm
     adj_asm_ms, adj_asm_n = a_asm
     # Algorithmic bytes per launch (each array touched once, in bytes):
+    kmu = 8.0 if a.static_precision == 0 else 4.0  # kappa,mu bytes per local point
     if fused:
         # two-field element pass: ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local
         # point, fp64 mu/kappa accumulators read+write 32, element-local forces written
         # for both fields 24; displacement gather of both fields 12 + 12 per unique point
         kinds = {
-            "k_forces<2,iso,fused-kernels>": (68.0 * nloc + 40.0 * nirr + 24.0 * nglob, adj_ms, adj_n),
+            "k_forces<2,iso,fused-kernels>": ((60.0 + kmu) * nloc + 40.0 * nirr + 24.0 * nglob, adj_ms, adj_n),
             # two-field assembly: CSR offsets 4 per point + indices 4 per contribution,
             # element-local forces of both fields 24 per local point, Stacey terms of both
             # fields 24 per face point, rmass_inv 12, u,v of both fields read 48 + write 48,
@@ -162,7 +173,7 @@ def main():
             "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp
                                       + 12.0 * nglob + 96.0 * nglob + 16.0 * nglob, adj_asm_ms, adj_asm_n),
         }
-        fwd_bytes = 24.0 * nloc + 40.0 * nirr + 12.0 * nglob
+        fwd_bytes = (16.0 + kmu) * nloc + 40.0 * nirr + 12.0 * nglob
         fwd_asm_bytes = 4.0 * nglob + 4.0 * ncontrib + 12.0 * nloc + 12.0 * nfp + 12.0 * nglob + 48.0 * nglob
     else:
         # SPECFEM stage order: one field per launch, atomic scatter, strain store
@@ -198,7 +209,8 @@ def main():
                                 f"{w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, NGLL=5, "
                                 f"{rec.nrec} receivers, dt={dt}, NSTEP={K}, fwd + waveform misfit + "
                                 "adjoint/reconstruction + rho/mu/kappa kernels (fp64 accum)"),
-                   "path": "fused" if fused else "SPECFEM stage order (strain stores)"},
+                   "path": "fused" if fused else "SPECFEM stage order (strain stores)",
+                   "element_pass_tuning": tuning if fused else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},

@@ -141,6 +141,9 @@ void fwat_copy_probe_(long *nbytes, int *reps, double *avg_ms);
 /* in-HBM replacements of save_forward_arrays.bin and absorb_field.bin */
 void fwat_copy_forward_to_backward_(long *Mesh_pointer_from, long *Mesh_pointer_to);
 void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);
+/* element-pass launch shape (autotuner knobs): regular_path 1 = folded-table kernel on
+ * all-regular meshes; blocks_per_cu_* = persistent workgroups per CU, 0 = one per element */
+void fwat_set_tuning_(long *Mesh_pointer, int *regular_path, int *blocks_per_cu_fwd, int *blocks_per_cu_adj);
 /* 0 fp32, 1 bf16, 2 fp16 with per-array scaling, for kappa/mu and the Jacobian */
 void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 

@@ -54,6 +54,7 @@ SIGNATURES = {
     "fwat_copy_forward_to_backward_": "LL",
     "fwat_move_stacey_store_": "LL",
     "fwat_set_static_precision_": "Li",
+    "fwat_set_tuning_": "Liii",
     "smooth_pde_cuda_": "iiiiiiIii" + "F" * 10 + "F" + "FFFFF" + "I" + "I" + "ff" + "iiIII" + "FFFFF" + "Ifi",
     "fwat_smooth_set_faithful_": "i",
     "fwat_smooth_pde_batch_": "iiiii" + "F" * 10 + "F" + "FFFFF" + "I" + "ff" + "FF" + "Ifii",

@@ -30,6 +30,8 @@
 //         a static CSR then assembles them in a fixed order (no atomics)
 #include "sem_device.h"
 
+#include <algorithm>
+
 namespace {
 
 struct ForceArgs {
@@ -278,6 +280,183 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
   }
 }
 
+// Regular-mesh element pass (every element an axis-aligned cube of the
+// mesh size, NSPEC_IRREGULAR = 0), gather-assembly output.  The inverse
+// Jacobian is xix_regular * I, so:
+//   * the derivative tables carry xix_regular: the contraction yields
+//     du/dx directly;
+//   * the divergence tables carry -jacobian_regular * xix_regular *
+//     wgllwgll(..) of the output point, and LDS holds the six stress
+//     components only (the flux of direction d is row d of sigma);
+//   * the three directions accumulate into one register per component.
+// Workgroups are persistent: each loops over its share of an XCD-contiguous
+// run of elements, so the table set-up is paid once per workgroup.
+template <int NF, bool ANISO, int KMODE, int SP>
+__global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
+  __shared__ realw s_tab[128];
+  __shared__ realw s_u[3][128];
+  __shared__ realw s_s[6][128];
+  const int t = threadIdx.x;
+  if (t < 125) s_tab[t] = A.tables[t];
+  const bool act = t < 125;
+  const int tt = act ? t : 124;
+  const int I = tt % 5, J = (tt / 5) % 5, K = tt / 25;
+  __syncthreads();
+  const realw xr = A.xix_regular;
+  const realw jx = A.jacobian_regular * A.xix_regular;
+  realw hpI[5], hpJ[5], hpK[5], hwI[5], hwJ[5], hwK[5];
+  {
+    const realw f1 = -jx * s_tab[50 + J + 5 * K];
+    const realw f2 = -jx * s_tab[75 + I + 5 * K];
+    const realw f3 = -jx * s_tab[100 + I + 5 * J];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      hpI[l] = s_tab[I + 5 * l] * xr;
+      hpJ[l] = s_tab[J + 5 * l] * xr;
+      hpK[l] = s_tab[K + 5 * l] * xr;
+      hwI[l] = s_tab[25 + l + 5 * I] * f1;
+      hwJ[l] = s_tab[25 + l + 5 * J] * f2;
+      hwK[l] = s_tab[25 + l + 5 * K] * f3;
+    }
+  }
+  const int La0 = 5 * J + 25 * K, Lb0 = I + 25 * K, Lc0 = I + 5 * J;
+  // XCD-aware persistent schedule: XCD x = blockIdx % 8 owns elements
+  // [x*chunk, (x+1)*chunk); its workgroups stride through that run together.
+  const int nblk = gridDim.x;
+  const int x = blockIdx.x & 7, per_xcd = nblk >> 3, slot = blockIdx.x >> 3;
+  const int chunk = (A.elem_count + 7) >> 3;
+  const int e_end = min(A.elem_count, (x + 1) * chunk);
+  for (int e = x * chunk + slot; e < e_end; e += per_xcd) {
+    const size_t Lg = (size_t)e * 125 + tt;
+    const int iglob = A.ibool[Lg];
+    realw kl = 0, ml = 0;
+    const realw* C = nullptr;
+    if (!ANISO) {
+      if (SP == SEM_STATIC_FP32) {
+        kl = A.kappa[Lg];
+        ml = A.mu[Lg];
+      } else {
+        const uint32_t pk = reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg];
+        kl = lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
+        ml = lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
+      }
+    } else {
+      C = A.cij + (size_t)e * 21 * 125 + tt;
+    }
+    realw ea[6];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const realw* U = f == 0 ? A.u0 : A.u1;
+      realw* Floc = f == 0 ? A.floc0 : A.floc1;
+      __syncthreads();  // previous element / field done with s_u and s_s
+      if (act) {
+        s_u[0][t] = U[3 * (size_t)iglob];
+        s_u[1][t] = U[3 * (size_t)iglob + 1];
+        s_u[2][t] = U[3 * (size_t)iglob + 2];
+      }
+      __syncthreads();
+      realw duxdx = 0, duydx = 0, duzdx = 0, duxdy = 0, duydy = 0, duzdy = 0, duxdz = 0, duydz = 0, duzdz = 0;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        const int La = La0 + l, Lb = Lb0 + 5 * l, Lc = Lc0 + 25 * l;
+        duxdx += s_u[0][La] * hpI[l]; duydx += s_u[1][La] * hpI[l]; duzdx += s_u[2][La] * hpI[l];
+        duxdy += s_u[0][Lb] * hpJ[l]; duydy += s_u[1][Lb] * hpJ[l]; duzdy += s_u[2][Lb] * hpJ[l];
+        duxdz += s_u[0][Lc] * hpK[l]; duydz += s_u[1][Lc] * hpK[l]; duzdz += s_u[2][Lc] * hpK[l];
+      }
+      const realw exy2 = duxdy + duydx, exz2 = duzdx + duxdz, eyz2 = duzdy + duydz;
+      if (KMODE == 2) {
+        if (f == 0) {
+          ea[0] = duxdx; ea[1] = duydy; ea[2] = duzdz;
+          ea[3] = 0.5f * exy2; ea[4] = 0.5f * exz2; ea[5] = 0.5f * eyz2;
+        } else if (act) {
+          const realw eb0 = duxdx, eb1 = duydy, eb2 = duzdz;
+          const realw eb3 = 0.5f * exy2, eb4 = 0.5f * exz2, eb5 = 0.5f * eyz2;
+          if (!ANISO) {
+            const realw tra = ea[0] + ea[1] + ea[2], trb = eb0 + eb1 + eb2;
+            const realw t3a = tra / 3.0f, t3b = trb / 3.0f;
+            const realw prod = (ea[0] - t3a) * (eb0 - t3b) + (ea[1] - t3a) * (eb1 - t3b) +
+                               (ea[2] - t3a) * (eb2 - t3b) +
+                               2.0f * (ea[3] * eb3 + ea[4] * eb4 + ea[5] * eb5);
+            A.mu_kl[Lg] += A.dt * (double)prod;
+            A.kappa_kl[Lg] += A.dt * (double)(tra * trb);
+          } else {
+            const realw va[6] = {ea[0], ea[1], ea[2], 2.0f * ea[5], 2.0f * ea[4], 2.0f * ea[3]};
+            const realw vb[6] = {eb0, eb1, eb2, 2.0f * eb5, 2.0f * eb4, 2.0f * eb3};
+            double* Kc = A.cijkl_kl + (size_t)e * 21 * 125 + t;
+            int q = 0;
+#pragma unroll
+            for (int P = 0; P < 6; ++P)
+#pragma unroll
+              for (int Q = P; Q < 6; ++Q) {
+                realw pr = va[P] * vb[Q];
+                if (Q > P) pr += va[Q] * vb[P];
+                Kc[q * 125] += A.dt * (double)pr;
+                ++q;
+              }
+          }
+        }
+      }
+      realw sxx, syy, szz, sxy, sxz, syz;
+      if (!ANISO) {
+        const realw lam = kl - (2.0f / 3.0f) * ml;
+        const realw m2 = 2.0f * ml;
+        const realw ltr = lam * (duxdx + duydy + duzdz);
+        sxx = ltr + m2 * duxdx;
+        syy = ltr + m2 * duydy;
+        szz = ltr + m2 * duzdz;
+        sxy = ml * exy2;
+        sxz = ml * exz2;
+        syz = ml * eyz2;
+      } else {
+        const realw e1 = duxdx, e2 = duydy, e3 = duzdz, e4 = eyz2, e5 = exz2, e6 = exy2;
+        const realw c11 = C[0], c12 = C[125], c13 = C[250], c14 = C[375], c15 = C[500], c16 = C[625];
+        const realw c22 = C[750], c23 = C[875], c24 = C[1000], c25 = C[1125], c26 = C[1250];
+        const realw c33 = C[1375], c34 = C[1500], c35 = C[1625], c36 = C[1750];
+        const realw c44 = C[1875], c45 = C[2000], c46 = C[2125];
+        const realw c55 = C[2250], c56 = C[2375], c66 = C[2500];
+        sxx = c11 * e1 + c12 * e2 + c13 * e3 + c14 * e4 + c15 * e5 + c16 * e6;
+        syy = c12 * e1 + c22 * e2 + c23 * e3 + c24 * e4 + c25 * e5 + c26 * e6;
+        szz = c13 * e1 + c23 * e2 + c33 * e3 + c34 * e4 + c35 * e5 + c36 * e6;
+        syz = c14 * e1 + c24 * e2 + c34 * e3 + c44 * e4 + c45 * e5 + c46 * e6;
+        sxz = c15 * e1 + c25 * e2 + c35 * e3 + c45 * e4 + c55 * e5 + c56 * e6;
+        sxy = c16 * e1 + c26 * e2 + c36 * e3 + c46 * e4 + c56 * e5 + c66 * e6;
+      }
+      if (act) {
+        s_s[0][t] = sxx; s_s[1][t] = syy; s_s[2][t] = szz;
+        s_s[3][t] = sxy; s_s[4][t] = sxz; s_s[5][t] = syz;
+      }
+      __syncthreads();
+      realw fx = 0, fy = 0, fz = 0;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        const int La = La0 + l, Lb = Lb0 + 5 * l, Lc = Lc0 + 25 * l;
+        // direction x: row x of sigma at (l,J,K); y: row y at (I,l,K); z: row z at (I,J,l)
+        fx += s_s[0][La] * hwI[l]; fy += s_s[3][La] * hwI[l]; fz += s_s[4][La] * hwI[l];
+        fx += s_s[3][Lb] * hwJ[l]; fy += s_s[1][Lb] * hwJ[l]; fz += s_s[5][Lb] * hwJ[l];
+        fx += s_s[4][Lc] * hwK[l]; fy += s_s[5][Lc] * hwK[l]; fz += s_s[2][Lc] * hwK[l];
+      }
+      if (act) {
+        realw* F = Floc + 3 * Lg;
+        F[0] = fx;
+        F[1] = fy;
+        F[2] = fz;
+      }
+    }
+  }
+}
+
+template <int NF, bool ANISO, int KMODE, int SP>
+void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu) {
+  int dev = 0, ncu = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  // a multiple of 8 (one share per XCD); blocks_per_cu = 0: one workgroup per element
+  const int all = (a.elem_count + 7) & ~7;
+  int nb = blocks_per_cu > 0 ? std::min(all, (ncu * blocks_per_cu + 7) & ~7) : all;
+  nb = std::max(8, nb);
+  hipLaunchKernelGGL((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+}
+
 template <int NF, bool ANISO, int KMODE, int SP>
 void launch_t(const ForceArgs& a, hipStream_t s) {
   const int nb = a.nblocks_padded;
@@ -339,6 +518,23 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   const int sp = m->static_precision;
   const bool an = m->anisotropy != 0;
   hipStream_t s = m->stream;
+  if (gather && m->nspec_irregular == 0 && m->regular_fast_path && kernel_mode != 1) {
+    if (nfields == 2) {
+      if (an ? !m->d_cijkl_kl : (!m->d_mu_kl || !m->d_kappa_kl))
+        SEM_FAIL("two-field force pass needs the %s kernel accumulators", an ? "cijkl" : "mu/kappa");
+      if (an) launch_reg<2, true, 2, SEM_STATIC_FP32>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+      else if (sp == SEM_STATIC_BF16) launch_reg<2, false, 2, SEM_STATIC_BF16>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+      else if (sp == SEM_STATIC_FP16) launch_reg<2, false, 2, SEM_STATIC_FP16>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+      else launch_reg<2, false, 2, SEM_STATIC_FP32>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+    } else {
+      if (an) launch_reg<1, true, 0, SEM_STATIC_FP32>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+      else if (sp == SEM_STATIC_BF16) launch_reg<1, false, 0, SEM_STATIC_BF16>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+      else if (sp == SEM_STATIC_FP16) launch_reg<1, false, 0, SEM_STATIC_FP16>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+      else launch_reg<1, false, 0, SEM_STATIC_FP32>(a, s, nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd);
+    }
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   if (nfields == 2) {
     if (kernel_mode != 2) SEM_FAIL("two-field force pass requires kernel mode 2");
     if (!m->d_b_displ || !m->d_b_accel) SEM_FAIL("two-field force pass needs the b_ fields");

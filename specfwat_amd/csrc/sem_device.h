@@ -67,6 +67,9 @@ struct SemMesh {
   int approximate_hess_kl = 0;
   int compute_and_store_strain = 0;
   int static_precision = SEM_STATIC_FP32;
+  int regular_fast_path = 1;  // folded-table element pass on all-regular meshes
+  int blocks_per_cu_fwd = 0;  // element-pass workgroups per CU (0: one per element)
+  int blocks_per_cu_adj = 8;
   // reduced-precision statics hold value/scale (FP16 needs it for range)
   float static_scale_kappa = 1.0f, static_scale_mu = 1.0f;
   float static_scale_jinv = 1.0f, static_scale_jac = 1.0f;

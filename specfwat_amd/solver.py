@@ -94,6 +94,9 @@ class ElasticRun:
             self.stf = sources.stf(self.nstep, self.dt)
             _lib.call("fwat_set_stf_", self.h, self.stf, self.nstep)
 
+    def set_tuning(self, regular_path=1, blocks_per_cu_fwd=0, blocks_per_cu_adj=8):
+        _lib.call("fwat_set_tuning_", self.h, int(regular_path), int(blocks_per_cu_fwd), int(blocks_per_cu_adj))
+
     # ------------------------------------------------------------ runs
     def set_adjoint_sources(self, adj):
         adj = np.ascontiguousarray(adj, F32).reshape(self.nstep, self.nrec, 3)

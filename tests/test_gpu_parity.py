@@ -234,3 +234,36 @@ def test_smoother_batch_matches_single():
     for q in range(3):
         _, d1, _ = smooth_sem_pde(m, fields[q], 800.0, 400.0, faithful=False)
         assert rel(out[q], d1) < 1e-6
+
+
+def test_element_pass_launch_shapes_agree():
+    """General kernel, folded-table kernel with one workgroup per element and
+    persistent workgroups give the same seismograms and kernels."""
+    c = make_case(seed=9, nstep=80)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = obs.astype(np.float32)
+    ko = o.adjoint(adj)
+    m = c["mesh"]
+    results = []
+    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8)):
+        src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+        rec = Receivers(m, c["rxyz"])
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
+        f.set_tuning(reg, bpc, bpc)
+        f.iterate_time()
+        syn = f.seismograms()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
+        a.set_tuning(reg, bpc, bpc)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj)
+        a.iterate_time()
+        k = a.kernels()
+        a.close()
+        assert rel(syn, obs) < SEIS_TOL
+        for n in ("rho_kl", "mu_kl", "kappa_kl"):
+            assert rel(k[n], ko[n]) < KERN_TOL, (reg, bpc, n)
+        results.append(syn)
+    for s in results[1:]:
+        assert rel(s, results[0]) < 1e-6
