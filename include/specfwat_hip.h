@@ -147,6 +147,33 @@ void fwat_set_tuning_(long *Mesh_pointer, int *regular_path, int *blocks_per_cu_
 /* 0 fp32, 1 bf16, 2 fp16 with per-array scaling, for kappa/mu and the Jacobian */
 void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 
+/* ---- multi-event batches: E = 1, 2, 4 or 8 events advanced together on one regular
+ *      isotropic mesh.  Replaces SpecFWAT's serial event loop
+ *      (src/preproc/fwat_fwd_measure_adj.f90:43-51, one PrepareFWD%simulation per event,
+ *      src/preproc/prepare_fwd.f90:297-326); per-event arithmetic is fwat_iterate_time_'s.
+ *      The mesh handle holds the model, receivers and Stacey faces (its own sources are
+ *      not used).  Events may have different NSTEP: forward runs stop at NSTEP_e, adjoint
+ *      runs are aligned at the end. */
+/* nstep(nevents); h_sourcearrays(NDIM,NGLLX,NGLLY,NGLLZ,nevents), one source per event;
+ * h_ispec_selected_source(nevents); h_stf(NSTEP_max,nevents) */
+void fwat_batch_create_(long *Mesh_pointer, long *Batch_pointer, int *nevents, int *nstep, realw *DT,
+                        realw *h_sourcearrays, int *h_ispec_selected_source, double *h_stf);
+void fwat_batch_set_tuning_(long *Batch_pointer, int *blocks_per_cu);
+/* forward runs with SAVE_FORWARD (per-event Stacey stores kept in HBM) */
+void fwat_batch_forward_(long *Batch_pointer, int *timed);
+/* h_adj(NDIM,nrec,NSTEP_max,nevents), forward time order; event e uses its first NSTEP_e rows */
+void fwat_batch_set_adjoint_sources_(long *Batch_pointer, realw *h_adj);
+/* adjoint + reconstruction + rho/mu/kappa kernels of every event */
+void fwat_batch_adjoint_(long *Batch_pointer, int *timed);
+/* h_seis(NDIM,nrec,NSTEP_max,nevents) */
+void fwat_batch_seismograms_(long *Batch_pointer, realw *h_seis);
+/* save_adjoint_kernels-scaled kernels, (NGLLX,NGLLY,NGLLZ,NSPEC_AB,nevents) each, fp64;
+ * h_rhostore(NGLLX,NGLLY,NGLLZ,NSPEC_AB) scales rho_kl */
+void fwat_batch_kernels_(long *Batch_pointer, realw *h_rhostore, double *h_rho_kl, double *h_mu_kl,
+                         double *h_kappa_kl);
+void fwat_batch_last_time_(long *Batch_pointer, double *elem_ms, int *elem_n, double *asm_ms, int *asm_n);
+void fwat_batch_destroy_(long *Batch_pointer);
+
 /* ---- PDE smoothing: smooth_pde_cuda_, src/postproc/smooth_sem_pde.cu:298-316,
  *      called from src/postproc/smooth.f90:392-402 (exact reference signature) */
 void smooth_pde_cuda_(int *h_is_sph, int *h_nspec, int *h_nglob, int *nprocs, int *h_nstep,

@@ -55,6 +55,15 @@ SIGNATURES = {
     "fwat_move_stacey_store_": "LL",
     "fwat_set_static_precision_": "Li",
     "fwat_set_tuning_": "Liii",
+    "fwat_batch_create_": "LLiIfFID",
+    "fwat_batch_set_tuning_": "Li",
+    "fwat_batch_forward_": "Li",
+    "fwat_batch_set_adjoint_sources_": "LF",
+    "fwat_batch_adjoint_": "Li",
+    "fwat_batch_seismograms_": "LF",
+    "fwat_batch_kernels_": "LFDDD",
+    "fwat_batch_last_time_": "LDIDI",
+    "fwat_batch_destroy_": "L",
     "smooth_pde_cuda_": "iiiiiiIii" + "F" * 10 + "F" + "FFFFF" + "I" + "I" + "ff" + "iiIII" + "FFFFF" + "Ifi",
     "fwat_smooth_set_faithful_": "i",
     "fwat_smooth_pde_batch_": "iiiii" + "F" * 10 + "F" + "FFFFF" + "I" + "ff" + "FF" + "Ifii",

@@ -190,3 +190,20 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libspecfwat_hip.so")
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_event_batch_refuses_unsupported_shapes():
+    """Batch-size and mesh checks happen on the host before any device call."""
+    from specfwat_amd.batch import EventBatch
+    h = 2500.0
+    m = BoxMesh(2, 2, 2, 2 * h, 2 * h, 2 * h)
+    nloc = (m.nspec, 125)
+    rho, vp, vs = (np.full(nloc, v, np.float32) for v in (2700, 6000, 3500))
+    src = Sources(m, [[h, h, -h]], [[1e15, 0, 0, 0, 0, 0]], [1.0])
+    with pytest.raises(ValueError, match="batch size"):
+        EventBatch(m, rho, vp, vs, [src] * 3, None, 0.01, [10, 10, 10])
+    with pytest.raises(ValueError, match="one NSTEP"):
+        EventBatch(m, rho, vp, vs, [src] * 2, None, 0.01, [10])
+    mi = BoxMesh(2, 2, 2, 2 * h, 2 * h, 2 * h, force_irregular=True)
+    with pytest.raises(ValueError, match="regular"):
+        EventBatch(mi, rho, vp, vs, [src] * 2, None, 0.01, [10, 10])
