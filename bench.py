@@ -37,7 +37,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10000)
     p.add_argument("--warmup", type=int, default=100)
-    p.add_argument("--config", choices=["c2", "c1"], default="c2")
+    p.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2",
+                   help="c2: one production-size event (default); c1: the small event; "
+                        "c3: c1 with bf16 / fp16 statics against the FP64 oracle; "
+                        "c4: 32 uneven events batched vs one at a time on the 64x64x32 mesh; "
+                        "c5: HTI anisotropic event with 21 c_ij kernels + smoothing of all 21")
+    p.add_argument("--c4-events", type=int, default=32)
+    p.add_argument("--c4-nstep-scale", type=float, default=0.25,
+                   help="NSTEP of the c4 events is U[4000,12000] times this factor")
+    p.add_argument("--events-per-batch", type=int, default=0, help="0: autotuned")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-steps", type=int, default=4)
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
@@ -55,6 +63,8 @@ def log(msg):
 def workload(cfg):
     if cfg == "c2":
         return dict(nex=96, ney=96, nez=48, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.004, nrx=20)
+    if cfg == "c4":
+        return dict(nex=64, ney=64, nez=32, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.005, nrx=20)
     return dict(nex=16, ney=16, nez=8, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.005, nrx=4)
 
 
@@ -71,6 +81,12 @@ def main():
         print(json.dumps({"copy_probe_bytes": a.copy_probe, "avg_ms": float(ms[0]),
                           "GBps_read_plus_write": 2 * a.copy_probe / (ms[0] * 1e-3) / 1e9}), flush=True)
         return
+    if a.config == "c4":
+        return batched(a)
+    if a.config == "c3":
+        return precision(a)
+    if a.config == "c5":
+        return aniso_smooth(a)
     from specfwat_amd.mesh import BoxMesh
     from specfwat_amd.model import perturbed_model
     from specfwat_amd.sources import Receivers, Sources, surface_grid
@@ -229,7 +245,383 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps):
+def batched(a):
+    """Many uneven events on one mesh: the batched path (events grouped by
+    NSTEP, events per launch and workgroups per CU autotuned) against the
+    plain single-event path on the same events, with the per-event
+    differences between the two stated."""
+    from specfwat_amd import _lib
+    from specfwat_amd.batch import EventBatch, plan_batches, tune_batch
+    from specfwat_amd.mesh import BoxMesh
+    from specfwat_amd.misfit import event_adjoint_sources
+    from specfwat_amd.model import perturbed_model
+    from specfwat_amd.solver import ElasticRun
+    from specfwat_amd.sources import Receivers, Sources, surface_grid
+    from specfwat_amd.tune import tune_element_passes
+
+    w = workload("c4")
+    t0 = time.time()
+    mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
+    true_model = perturbed_model(mesh, seed=2024)
+    smooth_model = perturbed_model(mesh, smooth=True)
+    rec = Receivers(mesh, surface_grid(mesh, w["nrx"], w["nrx"]))
+    dt = w["dt"]
+    rng = np.random.Generator(np.random.Philox(31))
+    srcs, nsteps = [], []
+    for _ in range(a.c4_events):
+        sxyz = [[rng.uniform(0.0, 1.0) * w["lx"], rng.uniform(0.0, 1.0) * w["ly"], -rng.uniform(2000.0, 18000.0)]]
+        srcs.append(Sources(mesh, sxyz, [list(1e15 * rng.standard_normal(6))], [rng.uniform(0.5, 1.5)]))
+        nsteps.append(max(2, int(round(rng.uniform(4000.0, 12000.0) * a.c4_nstep_scale))))
+    log(f"c4: mesh {mesh.nspec} el, {mesh.nglob} pts; {len(srcs)} events, NSTEP {min(nsteps)}..{max(nsteps)} "
+        f"({time.time() - t0:.1f}s)")
+    sp = a.static_precision
+    if a.events_per_batch:
+        E, bpc, ttable = a.events_per_batch, 8, []
+    else:
+        (E, bpc), ttable = tune_batch(mesh, smooth_model, srcs, rec, dt, static_precision=sp)
+        log("batch tuner (events, blocks/CU, fwd ms/event-step, adj ms/event-step): "
+            + ", ".join(f"({e},{b},{x:.3f},{y:.3f})" for e, b, x, y in ttable) + f" -> E={E} bpc={bpc}")
+    groups = plan_batches(nsteps, E)
+
+    # observed data: batched forward runs in the true model (untimed)
+    obs = [None] * len(srcs)
+    for g in groups:
+        b = EventBatch(mesh, *true_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g])
+        b.forward()
+        s = b.seismograms()
+        for k, i in enumerate(g):
+            obs[i] = s[k, :nsteps[i]].copy()
+        b.close()
+    log(f"observed data ready ({time.time() - t0:.1f}s)")
+
+    def run_group(g, keep):
+        b = EventBatch(mesh, *smooth_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g],
+                       static_precision=sp)
+        b.set_tuning(bpc)
+        _lib.call("fwat_synchronize_", b.run.h)
+        ts = time.time()
+        b.forward(timed=True)
+        fwd_t = b.pass_times()
+        syn = b.seismograms()
+        adj = np.zeros_like(syn)
+        for k, i in enumerate(g):
+            _, adj[k, :nsteps[i]] = event_adjoint_sources(obs[i], syn[k, :nsteps[i]], dt)
+        b.set_adjoint_sources(adj)
+        b.adjoint(timed=True)
+        _lib.call("fwat_synchronize_", b.run.h)
+        wall = time.time() - ts
+        adj_t = b.pass_times()
+        kern = b.kernels() if keep else None
+        b.close()
+        return wall, syn, kern, fwd_t, adj_t
+
+    bwall, bsyn, bkern, times = 0.0, {}, {}, {}
+    check = set(groups[0][:2])  # the two longest events: kernels compared with the plain path
+    for g in groups:
+        wall, syn, kern, ft, at = run_group(g, any(i in check for i in g))
+        bwall += wall
+        for k, i in enumerate(g):
+            bsyn[i] = syn[k, :nsteps[i]]
+            if kern is not None and i in check:
+                bkern[i] = {q: v[k] for q, v in kern.items()}
+        times.setdefault(len(g), (ft, at))
+    total = mesh.nglob * float(sum(nsteps))
+    bvalue = total / bwall
+    log(f"batched: {bwall:.2f}s over {len(groups)} batches -> {bvalue:.4e} GLL-pt-steps/s/event")
+
+    # plain path: the same events one at a time (fused single-event path, tuned shapes)
+    tuning, _ = tune_element_passes(mesh, smooth_model, srcs[0], rec, dt, static_precision=sp)
+    pwall, perr, kerr = 0.0, 0.0, {}
+    for i, src in enumerate(srcs):
+        f = ElasticRun(mesh, *smooth_model, src, rec, dt, nsteps[i], 1, True, static_precision=sp)
+        ad = ElasticRun(mesh, *smooth_model, src, rec, dt, nsteps[i], 3, False, static_precision=sp)
+        f.set_tuning(**tuning)
+        ad.set_tuning(**tuning)
+        _lib.call("fwat_synchronize_", f.h)
+        ts = time.time()
+        f.iterate_time()
+        syn = f.seismograms()
+        _, adj = event_adjoint_sources(obs[i], syn, dt)
+        ad.set_adjoint_sources(adj)
+        ad.take_forward(f)
+        ad.iterate_time()
+        _lib.call("fwat_synchronize_", ad.h)
+        pwall += time.time() - ts
+        f.close()
+        if np.linalg.norm(syn) > 0:
+            perr = max(perr, float(np.linalg.norm(bsyn[i] - syn) / np.linalg.norm(syn)))
+        if i in bkern:
+            k = ad.kernels()
+            for q in k:
+                kerr[q] = max(kerr.get(q, 0.0),
+                              float(np.linalg.norm(bkern[i][q] - k[q]) / np.linalg.norm(k[q])))
+        ad.close()
+    pvalue = total / pwall
+    log(f"plain: {pwall:.2f}s -> {pvalue:.4e} GLL-pt-steps/s/event")
+
+    # roofline of the largest batch's dominant pass (algorithmic bytes per launch)
+    ftimes, atimes = times[max(times)]
+    e = max(times)
+    nglob, nloc = mesh.nglob, mesh.nspec * 125
+    nfp = mesh.num_abs_boundary_faces * 25
+    ncontrib = nloc + nfp + 125 * (e + rec.nrec)
+    kmu = 8.0 if sp == 0 else 4.0
+    kinds = {
+        # statics once (ibool 4 + kappa,mu) per local point; per event: both fields gathered
+        # 24 per unique point, forces of both fields written 24 and fp64 mu/kappa read+write 32
+        # per local point
+        f"k_batch_elem<{e},2>": ((4.0 + kmu) * nloc + e * (24.0 * nglob + 56.0 * nloc), atimes[0], atimes[1]),
+        # CSR 4 per point + 4 per contribution, rmass_inv 12 per point once; per event: forces
+        # of both fields 24 per local point, Stacey terms 24 per face point, u,v of both fields
+        # read + written 96 and the fp64 rho kernel read+written 16 per unique point
+        f"k_batch_asm<{e},two-field>": (16.0 * nglob + 4.0 * ncontrib
+                                        + e * (24.0 * nloc + 24.0 * nfp + 112.0 * nglob), atimes[2], atimes[3]),
+    }
+    kname = max(kinds, key=lambda k: kinds[k][1])
+    kbytes, kms, kn = kinds[kname]
+    avg_ms = kms / max(kn, 1)
+    achieved = kbytes / (avg_ms * 1e-3) / 1e9
+    result = {
+        "metric": "GLL-pt-steps/s/event fwd+adj",
+        "value": bvalue,
+        "unit": "GLL-pt-steps/s",
+        "n_gpus": 1,
+        "steps": int(sum(nsteps)),
+        "warmup": 0,
+        "ms_per_step": bwall / sum(nsteps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if sp == 0 else ("f32+bf16-statics" if sp == 1 else "f32+fp16-statics"),
+        "data": "synthetic",
+        "config": {"workload": (f"C4 batched: isotropic box {w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, "
+                                f"{len(srcs)} events (depth U[2,18] km, random MT, f0 U[0.5,1.5] Hz), "
+                                f"NSTEP U[4000,12000] x {a.c4_nstep_scale} = {min(nsteps)}..{max(nsteps)}, "
+                                f"{rec.nrec} receivers, dt={dt}; fwd + misfit + adjoint/reconstruction + "
+                                "rho/mu/kappa kernels per event"),
+                   "events_per_batch": E, "blocks_per_cu": bpc, "batches": [len(g) for g in groups],
+                   "tuner": [list(r) for r in ttable]},
+        "batched_vs_plain": {"plain_value": pvalue, "speedup": bvalue / pvalue,
+                             "max_seismogram_rel_l2": perr, "kernel_rel_l2": kerr,
+                             "kernel_events_checked": sorted(check)},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
+        "breakdown": {"batched_s": bwall, "plain_s": pwall,
+                      "kernels": {k: {"avg_ms": v[1] / max(v[2], 1),
+                                      "GBps": v[0] / (v[1] / max(v[2], 1) * 1e-3) / 1e9 if v[1] > 0 else None}
+                                  for k, v in kinds.items()},
+                      "fwd_elem_avg_ms": ftimes[0] / max(ftimes[1], 1),
+                      "fwd_asm_avg_ms": ftimes[2] / max(ftimes[3], 1)},
+    }
+    if not a.no_cpu:
+        s0 = srcs[0]
+        result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, s0.xyz.tolist(), s0.mt.tolist(), rec,
+                                              a.cpu_steps, f0=float(s0.f0[0]), label="C4 event")
+    print(json.dumps(result), flush=True)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else None
+
+
+def precision(a):
+    """The small event with the static arrays (kappa/mu, and the inverse
+    Jacobian of irregular elements) in fp32, bf16 and fp16-with-scaling, each
+    against an FP64 oracle run of the same pipeline (forward, waveform misfit
+    and adjoint source from its own synthetics, adjoint + kernels), on the
+    regular mesh and on the forced-irregular (general hexahedron) variant."""
+    from oracle import oracle as orc
+    from specfwat_amd import _lib
+    from specfwat_amd.mesh import BoxMesh
+    from specfwat_amd.misfit import event_adjoint_sources
+    from specfwat_amd.model import perturbed_model
+    from specfwat_amd.solver import ElasticRun
+    from specfwat_amd.sources import Receivers, Sources, surface_grid
+
+    w = workload("c1")
+    K = a.steps if a.steps != 10000 else 2000
+    dt = w["dt"]
+    rows, timing = {}, {}
+    for firr in (False, True):
+        mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], force_irregular=firr)
+        true_model = perturbed_model(mesh, seed=2024)
+        smooth_model = perturbed_model(mesh, smooth=True)
+        rng = np.random.Generator(np.random.Philox(7))
+        sxyz = [[rng.uniform(0.3, 0.7) * w["lx"], rng.uniform(0.3, 0.7) * w["ly"], -10000.0]]
+        mt = [list(1e15 * rng.standard_normal(6))]
+        src = Sources(mesh, sxyz, mt, [1.0])
+        rxyz = surface_grid(mesh, w["nrx"], w["nrx"])
+        rec = Receivers(mesh, rxyz)
+        r = ElasticRun(mesh, *true_model, src, rec, dt, K, 1, False)
+        r.iterate_time()
+        obs = r.seismograms()
+        r.close()
+        t0 = time.time()
+        o = orc.Oracle(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], *[m.ravel() for m in smooth_model],
+                       dt=dt, nstep=K, src_xyz=sxyz, src_mt=mt, src_f0=[1.0], rec_xyz=rxyz,
+                       force_irregular=firr, precision=64)
+        s64 = o.forward(save=True)
+        _, adj64 = event_adjoint_sources(obs, s64, dt)
+        k64 = o.adjoint(adj64)
+        log(f"c3 {'forced-irregular' if firr else 'regular'}: FP64 oracle {time.time() - t0:.1f}s")
+        for sp, name in ((0, "fp32"), (1, "bf16"), (2, "fp16-scaled")):
+            f = ElasticRun(mesh, *smooth_model, src, rec, dt, K, 1, True, static_precision=sp)
+            ad = ElasticRun(mesh, *smooth_model, src, rec, dt, K, 3, False, static_precision=sp)
+            _lib.call("fwat_synchronize_", f.h)
+            ts = time.time()
+            f.iterate_time()
+            syn = f.seismograms()
+            _, adj = event_adjoint_sources(obs, syn, dt)
+            ad.set_adjoint_sources(adj)
+            ad.take_forward(f)
+            ad.iterate_time()
+            _lib.call("fwat_synchronize_", ad.h)
+            wall = time.time() - ts
+            k = ad.kernels()
+            f.close(); ad.close()
+            key = f"{'irregular' if firr else 'regular'}/{name}"
+            rows[key] = {"seismograms": rel_l2(syn, s64),
+                         **{q: rel_l2(k[q].ravel(), k64[q]) for q in ("rho_kl", "mu_kl", "kappa_kl")}}
+            timing[key] = mesh.nglob * K / wall
+            log(f"c3 {key}: {rows[key]}  {timing[key]:.3e} GLL-pt-steps/s")
+    best = "regular/bf16"
+    result = {
+        "metric": "GLL-pt-steps/s/event fwd+adj",
+        "value": timing[best],
+        "unit": "GLL-pt-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": 1e3 * mesh.nglob / timing[best],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+bf16-statics", "data": "synthetic",
+        "config": {"workload": (f"C3: C1 event (16x16x8 el, 139,425 GLL pts, NSTEP={K}, dt={dt}, 16 receivers) "
+                                "with static arrays in bf16 / fp16-with-scaling, regular and forced-irregular "
+                                "meshes; errors = relative L2 against an FP64 oracle run of the same pipeline")},
+        "precision_errors_vs_fp64_oracle": rows,
+        "rates": timing,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def aniso_smooth(a):
+    """HTI anisotropic event on the 64x64x32 mesh with full 21-component c_ij
+    kernels in fp64, then Gaussian PDE smoothing (sigma_h 5 km, sigma_v 2 km)
+    of all 21 kernels in one device run sharing the geometry reads."""
+    from specfwat_amd import _lib
+    from specfwat_amd.mesh import BoxMesh
+    from specfwat_amd.misfit import event_adjoint_sources
+    from specfwat_amd.model import hti_model, perturbed_model
+    from specfwat_amd.smooth import smooth_batch
+    from specfwat_amd.solver import ElasticRun
+    from specfwat_amd.sources import Receivers, Sources, surface_grid
+    from specfwat_amd.tune import tune_element_passes
+
+    w = workload("c4")
+    K = a.steps if a.steps != 10000 else 8000
+    dt = w["dt"]
+    t0 = time.time()
+    mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
+    true_model = perturbed_model(mesh, seed=2024)
+    smooth_model = perturbed_model(mesh, smooth=True)
+    cij_true = hti_model(mesh, *true_model, seed=77)
+    cij_smooth = hti_model(mesh, *smooth_model, seed=78, max_strength=0.0)
+    rng = np.random.Generator(np.random.Philox(5))
+    sxyz = [[rng.uniform(0.2, 0.8) * w["lx"], rng.uniform(0.2, 0.8) * w["ly"], -rng.uniform(2000.0, 18000.0)]]
+    src = Sources(mesh, sxyz, [list(1e15 * rng.standard_normal(6))], [rng.uniform(0.5, 1.5)])
+    rec = Receivers(mesh, surface_grid(mesh, w["nrx"], w["nrx"]))
+    log(f"c5: mesh {mesh.nspec} el, {mesh.nglob} pts, HTI c_ij ready ({time.time() - t0:.1f}s)")
+    tuning, _ = tune_element_passes(mesh, smooth_model, src, rec, dt)
+    r = ElasticRun(mesh, *true_model, src, rec, dt, K, 1, False, cij=cij_true)
+    r.set_tuning(**tuning)
+    r.iterate_time()
+    obs = r.seismograms()
+    r.close()
+    log(f"observed data ({K} steps, HTI true model) ({time.time() - t0:.1f}s)")
+    f = ElasticRun(mesh, *smooth_model, src, rec, dt, K, 1, True, cij=cij_smooth)
+    ad = ElasticRun(mesh, *smooth_model, src, rec, dt, K, 3, False, cij=cij_smooth)
+    f.set_tuning(**tuning)
+    ad.set_tuning(**tuning)
+    _lib.call("fwat_synchronize_", f.h)
+    ts = time.time()
+    f.iterate_time(timed=True)
+    syn = f.seismograms()
+    misfit, adj = event_adjoint_sources(obs, syn, dt)
+    ad.set_adjoint_sources(adj)
+    ad.take_forward(f)
+    ad.iterate_time(timed=True)
+    _lib.call("fwat_synchronize_", ad.h)
+    wall = time.time() - ts
+    fwd_ms, fwd_n = f.force_time()
+    adj_ms, adj_n = ad.force_time()
+    a_asm = ad.assembly_time()
+    k = ad.kernels()
+    f.close(); ad.close()
+    value = mesh.nglob * K / wall
+    log(f"aniso event: {wall:.2f}s -> {value:.4e} GLL-pt-steps/s, misfit {misfit:.4e}")
+
+    # smoothing of the 21 c_ij kernels (diagonal map on the all-regular mesh)
+    fields = np.ascontiguousarray(np.moveaxis(k["cijkl_kl"], -1, 0), np.float32)  # [21][nspec][125]
+    ts = time.time()
+    sm, ntstep, sm_elem_ms, sm_upd_ms = smooth_batch(mesh, fields, 5000.0, 2000.0, faithful=False, timed=True)
+    sm_wall = time.time() - ts
+    if not np.isfinite(sm).all():
+        raise RuntimeError("non-finite smoothed kernels")
+    log(f"smoothing 21 kernels: ntstep {ntstep}, {sm_wall:.2f}s (element {sm_elem_ms / 1e3:.2f}s, "
+        f"gather/update {sm_upd_ms / 1e3:.2f}s)")
+
+    nglob, nloc = mesh.nglob, mesh.nspec * 125
+    nfp = mesh.num_abs_boundary_faces * 25
+    ncontrib = nloc + nfp + 125 * (src.nsources + rec.nrec)
+    # adjoint element pass, anisotropic: ibool 4 + 21 c_ij 84 per local point, fp64 c_ij kernels
+    # read+write 336, element-local forces of both fields 24; gathers 24 per unique point
+    kinds = {
+        "k_forces<2,aniso,fused-kernels>": (448.0 * nloc + 24.0 * nglob, adj_ms, adj_n),
+        "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp + 124.0 * nglob,
+                                  a_asm[0], a_asm[1]),
+    }
+    kname = max(kinds, key=lambda q: kinds[q][1])
+    kbytes, kms, kn = kinds[kname]
+    avg_ms = kms / max(kn, 1)
+    achieved = kbytes / (avg_ms * 1e-3) / 1e9
+    # smoothing bytes per diffusion step (21 fields padded to 24 in HBM): element pass ibool 4 +
+    # jacobian 4 per local point, gather of 24 values per unique point 96, element-local results
+    # 96 per local point; gather pass CSR 4 per point + 4 per local point, results 96 per local
+    # point, rvol 4 and dat read+write 2 x 84 per unique point
+    sm_elem_b = 8.0 * nloc + 96.0 * nglob + 96.0 * nloc
+    sm_upd_b = 4.0 * nglob + 4.0 * nloc + 96.0 * nloc + 4.0 * nglob + 168.0 * nglob
+    result = {
+        "metric": "GLL-pt-steps/s/event fwd+adj",
+        "value": value,
+        "unit": "GLL-pt-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"C5: HTI anisotropic box {w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, "
+                                f"21 c_ij (strength U[0,4%], azimuth per 10 km block), one event, NSTEP={K}, "
+                                f"dt={dt}, {rec.nrec} receivers, 21 c_ij kernels in fp64; then PDE smoothing of "
+                                "all 21 kernels, sigma_h 5 km, sigma_v 2 km (diagonal regular-element map)"),
+                   "element_pass_tuning": tuning},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
+        "smoothing": {"fields": 21, "ntstep": ntstep, "wall_s": sm_wall,
+                      "GLL-pt-steps/s_per_field": nglob * ntstep * 21 / sm_wall,
+                      "elem_avg_ms": sm_elem_ms / ntstep, "update_avg_ms": sm_upd_ms / ntstep,
+                      "elem_GBps": sm_elem_b / (sm_elem_ms / ntstep * 1e-3) / 1e9,
+                      "update_GBps": sm_upd_b / (sm_upd_ms / ntstep * 1e-3) / 1e9},
+        "breakdown": {"wall_s": wall, "fwd_force_avg_ms": fwd_ms / max(fwd_n, 1),
+                      "kernels": {q: {"avg_ms": v[1] / max(v[2], 1),
+                                      "GBps": v[0] / (v[1] / max(v[2], 1) * 1e-3) / 1e9 if v[1] > 0 else None}
+                                  for q, v in kinds.items()}},
+    }
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event"):
     """The oracle (C++/OpenMP FP32 restatement) on the host cores, bounded
     sample: the same mesh/model/source/receivers, nsteps forward steps with
     the Stacey store then nsteps adjoint+reconstruction+kernel steps."""
@@ -241,7 +633,7 @@ def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps):
     rxyz = surface_grid(mesh, w["nrx"], w["nrx"])
     t0 = time.time()
     o = orc.Oracle(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], rho.ravel(), vp.ravel(),
-                   vs.ravel(), dt=w["dt"], nstep=nsteps, src_xyz=sxyz, src_mt=mt, src_f0=[1.0],
+                   vs.ravel(), dt=w["dt"], nstep=nsteps, src_xyz=sxyz, src_mt=mt, src_f0=[f0],
                    rec_xyz=rxyz, precision=32, nthreads=threads)
     log(f"cpu baseline set-up {time.time() - t0:.1f}s ({threads} threads)")
     ts = time.time()
@@ -251,7 +643,7 @@ def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps):
     log(f"cpu baseline {nsteps} fwd+adj steps: {wall:.2f}s")
     return {"value": mesh.nglob * nsteps / wall, "unit": "GLL-pt-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"same C2 event, {nsteps} forward + {nsteps} adjoint/reconstruction/kernel steps, "
+            "sample": f"same {label}, {nsteps} forward + {nsteps} adjoint/reconstruction/kernel steps, "
                       f"FP32 oracle (C++/OpenMP, colour-ordered assembly), {wall:.1f}s"}
 
 

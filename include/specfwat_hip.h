@@ -200,7 +200,8 @@ void fwat_smooth_pde_batch_(int *h_is_sph, int *h_nspec, int *h_nglob, int *h_ns
                             const realw *h_cv, const realw *h_ch, const realw *rvol, realw *dat_glob,
                             const int *irregular_element_number, const realw *h_xix_regular,
                             const int *h_nspec_irregular, int *timed);
-void fwat_smooth_last_time_(double *elem_ms);
+/* element-pass and gather/update time of the last timed fwat_smooth_pde_batch_ (ms) */
+void fwat_smooth_last_time_(double *elem_ms, double *update_ms);
 
 #ifdef __cplusplus
 }

@@ -67,7 +67,7 @@ SIGNATURES = {
     "smooth_pde_cuda_": "iiiiiiIii" + "F" * 10 + "F" + "FFFFF" + "I" + "I" + "ff" + "iiIII" + "FFFFF" + "Ifi",
     "fwat_smooth_set_faithful_": "i",
     "fwat_smooth_pde_batch_": "iiiii" + "F" * 10 + "F" + "FFFFF" + "I" + "ff" + "FF" + "Ifii",
-    "fwat_smooth_last_time_": "D",
+    "fwat_smooth_last_time_": "DD",
 }
 
 _lib = None

@@ -116,3 +116,57 @@ def batch_gradient(mesh, model, obs, sources, receivers, dt, nsteps, windows=Non
     finally:
         b.close()
     return misfit, syn, k
+
+
+def plan_batches(nsteps, events_per_batch=8):
+    """Event groups for batched runs: events sorted by NSTEP (longest first) so
+    that a batch's events have similar NSTEP (idle slots cost a full step),
+    cut into batches of `events_per_batch`; a remainder is split into the
+    supported sizes (1, 2, 4, 8)."""
+    if events_per_batch not in BATCH_SIZES:
+        raise ValueError(f"events per batch {events_per_batch} not in {BATCH_SIZES}")
+    order = [int(k) for k in np.argsort(-np.asarray(nsteps), kind="stable")]
+    groups, i = [], 0
+    while i < len(order):
+        e = max(s for s in BATCH_SIZES if s <= min(len(order) - i, events_per_batch))
+        groups.append(order[i:i + e])
+        i += e
+    return groups
+
+
+def batch_device_bytes(mesh, nevents, nmax):
+    """HBM held by one EventBatch (fields, element-local forces, kernels,
+    Stacey rows, CSR), besides the mesh handle."""
+    nloc, ng = mesh.nspec * 125, mesh.nglob
+    nfp = mesh.num_abs_boundary_faces * 25
+    e = nevents
+    return (6 * ng * e * 12 + 2 * nloc * e * 12 + (2 * nloc + ng) * e * 8
+            + (nmax + 1) * nfp * e * 12 + 4 * (ng + nloc + nfp) + 8 * e * nmax)
+
+
+def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_cu=(4, 8, 16), nstep=12,
+               static_precision=0, hbm_budget=None):
+    """Events per launch and persistent workgroups per CU of the batched
+    element pass, from short timed runs on the real mesh.  The figure of
+    merit is the element + assembly time per event-step of a forward and an
+    adjoint step.  Returns ((events, blocks_per_cu), table)."""
+    table = []
+    for e in sizes:
+        if e > len(sources):
+            continue
+        if hbm_budget is not None and batch_device_bytes(mesh, e, nstep) > hbm_budget:
+            continue
+        b = EventBatch(mesh, *model, sources[:e], receivers, dt, [nstep] * e,
+                       static_precision=static_precision)
+        b.forward()  # warm-up; fills the Stacey rows the adjoint replays
+        b.set_adjoint_sources(np.zeros((e, nstep, b.nrec, 3), F32))
+        for bpc in blocks_per_cu:
+            b.set_tuning(bpc)
+            b.forward(timed=True)
+            fe, fn, fa, _ = b.pass_times()
+            b.adjoint(timed=True)
+            ae, an, aa, _ = b.pass_times()
+            table.append((e, bpc, (fe + fa) / (fn * e), (ae + aa) / (an * e)))
+        b.close()
+    best = min(table, key=lambda r: r[2] + r[3])
+    return (best[0], best[1]), table

@@ -42,6 +42,8 @@ struct ForceArgs {
   const realw* kappa;
   const realw* mu;
   const uint16_t* kmu_lp;
+  const float* kmu_base;     // [nspec][2] per-element means (reduced-precision statics)
+  const float* jinv10_base;  // [nspec_irr][10]
   const realw* cij;
   const realw* tables;
   const int* elem_list;
@@ -118,12 +120,16 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
       gax = G[750]; gay = G[875]; gaz = G[1000];
       jl = G[1125];
     } else {
+      // per-element fp32 mean + reduced-precision deviation
       const uint16_t* G = A.jinv10_lp + base;
+      const float* B = A.jinv10_base + (size_t)(ir - 1) * 10;
       const float sj = A.scale_jinv;
-      xix = lp_to_f<SP>(G[0]) * sj; xiy = lp_to_f<SP>(G[125]) * sj; xiz = lp_to_f<SP>(G[250]) * sj;
-      etx = lp_to_f<SP>(G[375]) * sj; ety = lp_to_f<SP>(G[500]) * sj; etz = lp_to_f<SP>(G[625]) * sj;
-      gax = lp_to_f<SP>(G[750]) * sj; gay = lp_to_f<SP>(G[875]) * sj; gaz = lp_to_f<SP>(G[1000]) * sj;
-      jl = lp_to_f<SP>(G[1125]) * A.scale_jac;
+      xix = B[0] + lp_to_f<SP>(G[0]) * sj; xiy = B[1] + lp_to_f<SP>(G[125]) * sj;
+      xiz = B[2] + lp_to_f<SP>(G[250]) * sj; etx = B[3] + lp_to_f<SP>(G[375]) * sj;
+      ety = B[4] + lp_to_f<SP>(G[500]) * sj; etz = B[5] + lp_to_f<SP>(G[625]) * sj;
+      gax = B[6] + lp_to_f<SP>(G[750]) * sj; gay = B[7] + lp_to_f<SP>(G[875]) * sj;
+      gaz = B[8] + lp_to_f<SP>(G[1000]) * sj;
+      jl = B[9] + lp_to_f<SP>(G[1125]) * A.scale_jac;
     }
   } else {
     xix = A.xix_regular; ety = A.xix_regular; gaz = A.xix_regular;
@@ -140,8 +146,9 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
       ml = A.mu[Lg];
     } else {
       const uint32_t pk = reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg];
-      kl = lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
-      ml = lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
+      const float2 kb = reinterpret_cast<const float2*>(A.kmu_base)[ispec];
+      kl = kb.x + lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
+      ml = kb.y + lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
     }
   } else {
     C = A.cij + (size_t)ispec * 21 * 125 + tt;
@@ -337,8 +344,9 @@ __global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
         ml = A.mu[Lg];
       } else {
         const uint32_t pk = reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg];
-        kl = lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
-        ml = lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
+        const float2 kb = reinterpret_cast<const float2*>(A.kmu_base)[e];
+        kl = kb.x + lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
+        ml = kb.y + lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
       }
     } else {
       C = A.cij + (size_t)e * 21 * 125 + tt;
@@ -479,6 +487,8 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   a.kappa = m->d_kappa;
   a.mu = m->d_mu;
   a.kmu_lp = reinterpret_cast<const uint16_t*>(m->d_kmu_lp);
+  a.kmu_base = m->d_kmu_base;
+  a.jinv10_base = m->d_jinv10_base;
   a.cij = m->d_cij;
   a.tables = m->d_tables;
   a.elem_list = elem_list;

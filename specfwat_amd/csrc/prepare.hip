@@ -348,45 +348,81 @@ void fwat_set_static_precision_(long* Mesh_pointer, int* precision) {
   if (sp != SEM_STATIC_FP32 && m->anisotropy)
     SEM_FAIL("reduced-precision statics are implemented for the isotropic path only");
   m->static_precision = sp;
-  if (m->d_kmu_lp) { HIP_CHECK(hipFree(m->d_kmu_lp)); m->d_kmu_lp = nullptr; }
-  if (m->d_jinv10_lp) { HIP_CHECK(hipFree(m->d_jinv10_lp)); m->d_jinv10_lp = nullptr; }
+  dev_free(m->d_kmu_lp);
+  dev_free(m->d_kmu_base);
+  dev_free(m->d_jinv10_lp);
+  dev_free(m->d_jinv10_base);
   if (sp == SEM_STATIC_FP32) return;
+  // Encoding: value = fp32 mean over the element's 125 points + deviation
+  // stored in bf16 (or fp16 scaled by a per-array factor that maps the
+  // largest deviation to 1024).  A homogeneous element is exact; otherwise
+  // the error is the deviation's rounding, not the value's.
+  const auto encode = [sp](const float* v, size_t nelem, int ncomp, std::vector<float>& base,
+                           std::vector<uint16_t>& dev, float& scale) {
+    // v: [nelem][ncomp][125]
+    base.assign(nelem * ncomp, 0.0f);
+    std::vector<float> d(nelem * ncomp * 125);
+    float dmax = 0.0f;
+    for (size_t e = 0; e < nelem; ++e)
+      for (int c = 0; c < ncomp; ++c) {
+        const float* x = v + (e * ncomp + c) * 125;
+        double sum = 0.0;
+        for (int L = 0; L < 125; ++L) sum += x[L];
+        const float b = (float)(sum / 125.0);
+        base[e * ncomp + c] = b;
+        for (int L = 0; L < 125; ++L) {
+          const float r = x[L] - b;
+          d[(e * ncomp + c) * 125 + L] = r;
+          dmax = std::max(dmax, std::fabs(r));
+        }
+      }
+    scale = (sp == SEM_STATIC_FP16 && dmax > 0.0f) ? dmax / 1024.0f : 1.0f;
+    dev.resize(d.size());
+    for (size_t q = 0; q < d.size(); ++q) dev[q] = f2lp(d[q] / scale, sp);
+  };
   const size_t nloc = (size_t)m->nspec * 125;
-  std::vector<realw> k(nloc), mu(nloc);
-  HIP_CHECK(hipMemcpyAsync(k.data(), m->d_kappa, nloc * 4, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(mu.data(), m->d_mu, nloc * 4, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  float kmax = 0, mmax = 0;
-  for (size_t L = 0; L < nloc; ++L) { kmax = std::max(kmax, std::fabs(k[L])); mmax = std::max(mmax, std::fabs(mu[L])); }
-  m->static_scale_kappa = (sp == SEM_STATIC_FP16 && kmax > 0) ? kmax / 1024.0f : 1.0f;
-  m->static_scale_mu = (sp == SEM_STATIC_FP16 && mmax > 0) ? mmax / 1024.0f : 1.0f;
-  std::vector<uint32_t> pk(nloc);
-  for (size_t L = 0; L < nloc; ++L)
-    pk[L] = (uint32_t)f2lp(k[L] / m->static_scale_kappa, sp) |
-            ((uint32_t)f2lp(mu[L] / m->static_scale_mu, sp) << 16);
-  m->d_kmu_lp = dev_upload(pk.data(), nloc, s);
+  {
+    std::vector<float> k(nloc), mu(nloc);
+    HIP_CHECK(hipMemcpyAsync(k.data(), m->d_kappa, nloc * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(mu.data(), m->d_mu, nloc * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<float> kb, mb;
+    std::vector<uint16_t> kd, md;
+    encode(k.data(), m->nspec, 1, kb, kd, m->static_scale_kappa);
+    encode(mu.data(), m->nspec, 1, mb, md, m->static_scale_mu);
+    std::vector<uint32_t> pk(nloc);
+    for (size_t L = 0; L < nloc; ++L) pk[L] = (uint32_t)kd[L] | ((uint32_t)md[L] << 16);
+    std::vector<float> base(2 * (size_t)m->nspec);
+    for (int e = 0; e < m->nspec; ++e) { base[2 * e] = kb[e]; base[2 * e + 1] = mb[e]; }
+    m->d_kmu_lp = dev_upload(pk.data(), nloc, s);
+    m->d_kmu_base = dev_upload(base.data(), base.size(), s);
+  }
   if (m->nspec_irregular > 0) {
+    // jinv10 [nirr][10][125]: the nine inverse-Jacobian terms share one fp16
+    // scale, the jacobian its own
     const size_t n = (size_t)m->nspec_irregular * 1250;
-    std::vector<realw> j(n);
+    std::vector<float> j(n);
     HIP_CHECK(hipMemcpyAsync(j.data(), m->d_jinv10, n * 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    float jmax = 0, gmax = 0;
-    for (size_t e = 0; e < (size_t)m->nspec_irregular; ++e)
-      for (int c = 0; c < 10; ++c)
-        for (int L = 0; L < 125; ++L) {
-          float v = std::fabs(j[(e * 10 + c) * 125 + L]);
-          if (c < 9) gmax = std::max(gmax, v); else jmax = std::max(jmax, v);
-        }
-    m->static_scale_jinv = (sp == SEM_STATIC_FP16 && gmax > 0) ? gmax / 1024.0f : 1.0f;
-    m->static_scale_jac = (sp == SEM_STATIC_FP16 && jmax > 0) ? jmax / 1024.0f : 1.0f;
+    std::vector<float> g9((size_t)m->nspec_irregular * 1125), jac((size_t)m->nspec_irregular * 125);
+    for (size_t e = 0; e < (size_t)m->nspec_irregular; ++e) {
+      std::memcpy(&g9[e * 1125], &j[e * 1250], 1125 * 4);
+      std::memcpy(&jac[e * 125], &j[e * 1250 + 1125], 125 * 4);
+    }
+    std::vector<float> gb, jb;
+    std::vector<uint16_t> gd, jd;
+    encode(g9.data(), m->nspec_irregular, 9, gb, gd, m->static_scale_jinv);
+    encode(jac.data(), m->nspec_irregular, 1, jb, jd, m->static_scale_jac);
     std::vector<uint16_t> lp(n);
-    for (size_t e = 0; e < (size_t)m->nspec_irregular; ++e)
-      for (int c = 0; c < 10; ++c)
-        for (int L = 0; L < 125; ++L) {
-          size_t q = (e * 10 + c) * 125 + L;
-          lp[q] = f2lp(j[q] / (c < 9 ? m->static_scale_jinv : m->static_scale_jac), sp);
-        }
+    std::vector<float> base((size_t)m->nspec_irregular * 10);
+    for (size_t e = 0; e < (size_t)m->nspec_irregular; ++e) {
+      std::memcpy(&lp[e * 1250], &gd[e * 1125], 1125 * 2);
+      std::memcpy(&lp[e * 1250 + 1125], &jd[e * 125], 125 * 2);
+      for (int c = 0; c < 9; ++c) base[e * 10 + c] = gb[e * 9 + c];
+      base[e * 10 + 9] = jb[e];
+    }
     m->d_jinv10_lp = dev_upload(lp.data(), n, s);
+    m->d_jinv10_base = dev_upload(base.data(), base.size(), s);
   }
   HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -472,8 +508,10 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   if (!m) return;
   HIP_CHECK(hipStreamSynchronize(m->stream));
   dev_free(m->d_ibool); dev_free(m->d_irregular); dev_free(m->d_jinv10); dev_free(m->d_tables);
-  if (m->d_jinv10_lp) HIP_CHECK(hipFree(m->d_jinv10_lp));
-  if (m->d_kmu_lp) HIP_CHECK(hipFree(m->d_kmu_lp));
+  dev_free(m->d_jinv10_lp);
+  dev_free(m->d_jinv10_base);
+  dev_free(m->d_kmu_lp);
+  dev_free(m->d_kmu_base);
   dev_free(m->d_kappa); dev_free(m->d_mu); dev_free(m->d_rho); dev_free(m->d_cij); dev_free(m->d_rmass_inv);
   dev_free(m->d_phase_ispec);
   dev_free(m->d_displ); dev_free(m->d_veloc); dev_free(m->d_accel);

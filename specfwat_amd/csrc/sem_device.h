@@ -81,14 +81,16 @@ struct SemMesh {
   int* d_ibool = nullptr;
   int* d_irregular = nullptr;  // [nspec], 0 regular else 1-based
   realw* d_jinv10 = nullptr;
-  void* d_jinv10_lp = nullptr;  // bf16/fp16 copy for reduced-precision statics
+  void* d_jinv10_lp = nullptr;  // bf16/fp16 deviations from the per-element means [nspec_irr][10][125]
+  float* d_jinv10_base = nullptr;  // fp32 per-element means [nspec_irr][10]
   realw* d_tables = nullptr;    // hprime[25], hprimewgll[25], wgllwgll_yz[25], _xz[25], _xy[25]
   realw h_tables[125];
 
   // materials
   realw* d_kappa = nullptr;
   realw* d_mu = nullptr;
-  void* d_kmu_lp = nullptr;  // packed (kappa,mu) pairs in bf16/fp16 [nspec][125][2]
+  void* d_kmu_lp = nullptr;     // packed (kappa,mu) deviations in bf16/fp16 [nspec][125][2]
+  float* d_kmu_base = nullptr;  // fp32 per-element means of kappa, mu [nspec][2]
   realw* d_rho = nullptr;
   realw* d_cij = nullptr;
   realw* d_rmass_inv = nullptr;  // [nglob][3]

@@ -80,7 +80,8 @@ def smooth_sem_pde(mesh, dat_in, sigma_h, sigma_v, *, is_sph=False, faithful=Tru
 def smooth_batch(mesh, fields, sigma_h, sigma_v, *, is_sph=False, faithful=True,
                  distance_min_glob=None, nstep=None, timed=False):
     """Smooth several element-local fields ([nfield, nspec, 125]) in one device
-    run sharing the geometry reads.  Returns (dat_glob [nfield, nglob], ntstep, elem_ms)."""
+    run sharing the geometry reads.  Returns (dat_glob [nfield, nglob], ntstep, elem_ms,
+    update_ms); the times are filled when timed."""
     if distance_min_glob is None:
         distance_min_glob = mesh.distance_min_glob()
     cv, ch, ntstep = smoothing_constants(sigma_h, sigma_v, distance_min_glob)
@@ -102,6 +103,6 @@ def smooth_batch(mesh, fields, sigma_h, sigma_v, *, is_sph=False, faithful=True,
               g.hprime_xxT, g.hprimewgll_xx, i32(mesh.ibool), float(cv), float(ch), rvol, dglob,
               i32(mesh.irregular_element_number), float(mesh.xix_regular), mesh.nspec_irregular,
               int(timed))
-    ms = np.zeros(1)
-    _lib.call("fwat_smooth_last_time_", ms)
-    return dglob, ntstep, float(ms[0])
+    ems, ums = np.zeros(1), np.zeros(1)
+    _lib.call("fwat_smooth_last_time_", ems, ums)
+    return dglob, ntstep, float(ems[0]), float(ums[0])

@@ -26,6 +26,7 @@ def ricker(t, f0):
 class Sources:
     def __init__(self, mesh, xyz, mt, f0, tshift=None):
         xyz = np.reshape(np.asarray(xyz, float), (-1, 3))
+        self.xyz = xyz
         self.nsources = xyz.shape[0]
         self.mt = np.reshape(np.asarray(mt, float), (self.nsources, 6))  # Mxx Myy Mzz Mxy Mxz Myz
         self.f0 = np.reshape(np.asarray(f0, float), (self.nsources,))

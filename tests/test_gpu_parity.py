@@ -230,7 +230,7 @@ def test_smoother_batch_matches_single():
     m = BoxMesh(6, 6, 4, 6000.0, 6000.0, 4000.0)
     rng = np.random.default_rng(12)
     fields = rng.standard_normal((3, m.nspec, 125)).astype(np.float32)
-    out, nt, _ = smooth_batch(m, fields, 800.0, 400.0, faithful=False)
+    out, nt, _, _ = smooth_batch(m, fields, 800.0, 400.0, faithful=False)
     for q in range(3):
         _, d1, _ = smooth_sem_pde(m, fields[q], 800.0, 400.0, faithful=False)
         assert rel(out[q], d1) < 1e-6
