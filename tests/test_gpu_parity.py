@@ -187,8 +187,13 @@ def test_per_step_c_abi_matches_whole_loop():
     assert rel(s, s_ref) < 1e-6
 
 
-def test_mixed_precision_statics_error_bound():
-    c = make_case(seed=8, nstep=100, perturb=0.1)
+@pytest.mark.parametrize("perturb", [0.0, 0.1])
+def test_mixed_precision_statics_error_bound(perturb):
+    """bf16 / fp16-scaled statics (per-element fp32 mean + reduced-precision
+    deviation) against the FP64 oracle, on a model with 3 % point-to-point
+    noise (the worst case for the deviation encoding): the mixed-precision
+    bar is 1e-3 for seismograms and kernels."""
+    c = make_case(seed=8, nstep=100, perturb=perturb)
     o = oracle_of(c, 64)
     obs = o.forward(save=True)
     adj = obs.astype(np.float32)
@@ -197,8 +202,8 @@ def test_mixed_precision_statics_error_bound():
         syn, k, _ = gpu_gradient(c, adj, static_precision=sp)
         es = rel(syn, obs)
         ek = max(rel(k[n], ko[n]) for n in ("rho_kl", "mu_kl", "kappa_kl"))
-        print(f"static precision {sp}: seismogram err {es:.2e}, kernel err {ek:.2e}")
-        assert es < 1e-2 and ek < 1e-2
+        print(f"perturb {perturb} static precision {sp}: seismogram err {es:.2e}, kernel err {ek:.2e}")
+        assert es < 1e-3 and ek < 1e-3
 
 
 def test_smoother_matches_oracle_restatement():
