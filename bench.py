@@ -161,6 +161,8 @@ def main():
     fwd_ms, fwd_n = f.force_time()
     adj_ms, adj_n = ad.force_time()
     f_asm, a_asm = f.assembly_time(), ad.assembly_time()
+    brick = tuning.get("brick") if fused else None
+    binfo = ad.brick_info() if brick else None
     k = ad.kernels()
     f.close(); ad.close()
     if not all(np.isfinite(v).all() for v in k.values()):
@@ -176,7 +178,26 @@ def main():
     adj_asm_ms, adj_asm_n = a_asm
     # Algorithmic bytes per launch (each array touched once, in bytes):
     kmu = 8.0 if a.static_precision == 0 else 4.0  # kappa,mu bytes per local point
-    if fused:
+    if brick:
+        nh, nslot, nch = binfo
+        ninner = nglob - nh
+        bname = "x".join(str(v) for v in brick)
+        kinds = {
+            # brick pass, two fields: ibool 4 + kappa,mu per local point, fp64 mu/kappa read+write
+            # 32 per local point, displacement gathers 12 + 12 per unique point; brick-inner
+            # points: rmass_inv 12, u,v of both fields read + written 96, fp64 rho kernel 16;
+            # partial sums of both fields written 24 per slot
+            f"k_brick<{bname},two-field>": ((4.0 + kmu) * nloc + 32.0 * nloc + 24.0 * nglob
+                                            + 124.0 * ninner + 24.0 * nslot, adj_ms, adj_n),
+            # halo pass, two fields: point list + CSR offsets 8 per halo point, 4 per contribution,
+            # partial sums 24 per slot, Stacey terms 24 per face point, rmass_inv 12, u,v of both
+            # fields 96 and the rho kernel 16 per halo point
+            "k_halo<two-field>": (8.0 * nh + 4.0 * nch + 24.0 * nslot + 24.0 * nfp + 124.0 * nh,
+                                  adj_asm_ms, adj_asm_n),
+        }
+        fwd_bytes = (4.0 + kmu) * nloc + 12.0 * nglob + 60.0 * ninner + 12.0 * nslot
+        fwd_asm_bytes = 8.0 * nh + 4.0 * nch + 12.0 * nslot + 12.0 * nfp + 60.0 * nh
+    elif fused:
         # two-field element pass: ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local
         # point, fp64 mu/kappa accumulators read+write 32, element-local forces written
         # for both fields 24; displacement gather of both fields 12 + 12 per unique point
@@ -225,7 +246,8 @@ def main():
                                 f"{w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, NGLL=5, "
                                 f"{rec.nrec} receivers, dt={dt}, NSTEP={K}, fwd + waveform misfit + "
                                 "adjoint/reconstruction + rho/mu/kappa kernels (fp64 accum)"),
-                   "path": "fused" if fused else "SPECFEM stage order (strain stores)",
+                   "path": (f"brick-fused {'x'.join(map(str, brick))}" if brick else "fused")
+                           if fused else "SPECFEM stage order (strain stores)",
                    "element_pass_tuning": tuning if fused else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

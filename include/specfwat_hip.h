@@ -144,6 +144,22 @@ void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);
 /* element-pass launch shape (autotuner knobs): regular_path 1 = folded-table kernel on
  * all-regular meshes; blocks_per_cu_* = persistent workgroups per CU, 0 = one per element */
 void fwat_set_tuning_(long *Mesh_pointer, int *regular_path, int *blocks_per_cu_fwd, int *blocks_per_cu_adj);
+/* Element lattice of the mesh: element (ix,iy,iz) = ix + nex*(iy + ney*iz).  Checked against
+ * ibool (shared faces, point multiplicities); enables fwat_set_brick_.  No counterpart in the
+ * reference (SPECFEM meshes carry no lattice; this box mesher's do). */
+void fwat_set_lattice_(long *Mesh_pointer, int *nex, int *ney, int *nez);
+/* Brick-fused element + assembly pass of the whole-run loop on all-regular isotropic lattice
+ * meshes: bricks of bx x by x bz elements (2x2x2, 4x2x2, 4x4x2; bx = 0: off), blocks_per_cu
+ * persistent workgroups per CU (0: one per brick).  Replaces compute_forces + assembly of
+ * iterate_time (prepare_fwd.f90:364) for that mesh class; results within rounding of the
+ * gather path. */
+void fwat_set_brick_(long *Mesh_pointer, int *bx, int *by, int *bz, int *blocks_per_cu);
+/* brick decomposition sizes: halo points, partial-sum slots per field and step, halo CSR length */
+void fwat_brick_info_(long *Mesh_pointer, int *nhalo, long *nslots, long *ncontrib);
+/* Steps per fp32 partial-sum block of the element-pass kernels (mu/kappa, c_ij) in the fused
+ * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
+ * fp64).  The rho kernel is always accumulated per step in fp64. */
+void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
 /* 0 fp32, 1 bf16, 2 fp16 with per-array scaling, for kappa/mu and the Jacobian */
 void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 

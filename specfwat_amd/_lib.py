@@ -55,6 +55,10 @@ SIGNATURES = {
     "fwat_move_stacey_store_": "LL",
     "fwat_set_static_precision_": "Li",
     "fwat_set_tuning_": "Liii",
+    "fwat_set_lattice_": "Liii",
+    "fwat_set_brick_": "Liiii",
+    "fwat_brick_info_": "Lill",
+    "fwat_set_kernel_block_": "Li",
     "fwat_batch_create_": "LLiIfFID",
     "fwat_batch_set_tuning_": "Li",
     "fwat_batch_forward_": "Li",

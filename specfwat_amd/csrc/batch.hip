@@ -51,6 +51,9 @@ struct BatchElemArgs {
   realw* floc1;
   double* mu_kl;     // [E][nloc]
   double* kappa_kl;
+  float* mu_part;    // [E][nloc] fp32 partial sums (null: straight into fp64)
+  float* kappa_part;
+  int kflush;        // this step closes a partial-sum block
   double dt;
   int active_mask;  // bit e set: event e advances this step
 };
@@ -196,8 +199,8 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
             const realw prod = (ea0 - t3a) * (eb0 - t3b) + (ea1 - t3a) * (eb1 - t3b) + (ea2 - t3a) * (eb2 - t3b) +
                                2.0f * (ea3 * eb3 + ea4 * eb4 + ea5 * eb5);
             const size_t kq = (size_t)ev * nloc + Lg;
-            A.mu_kl[kq] += A.dt * (double)prod;
-            A.kappa_kl[kq] += A.dt * (double)(tra * trb);
+            kernel_acc(A.mu_kl, A.mu_part, kq, prod, A.dt, A.kflush != 0);
+            kernel_acc(A.kappa_kl, A.kappa_part, kq, tra * trb, A.dt, A.kflush != 0);
           }
         }
         const realw ltr = lam * (duxdx + duydy + duzdz);
@@ -411,6 +414,8 @@ struct SemBatch {
   realw* seis = nullptr;                              // [E][nmax][nrec][3]
   realw* adj = nullptr;                               // [E][nmax][nrec][3]
   double *mu_kl = nullptr, *kappa_kl = nullptr;       // [E][nloc]
+  float *mu_part = nullptr, *kappa_part = nullptr;    // [E][nloc] fp32 partial sums
+  int kflush = 1;
   double* rho_kl = nullptr;                           // [nglob][E]
   double elem_ms = 0, asm_ms = 0;
   int elem_n = 0, asm_n = 0;
@@ -435,6 +440,7 @@ void launch_elem(SemBatch* B, int nf, const int* active) {
   A.scale_k = m->static_scale_kappa; A.scale_m = m->static_scale_mu;
   A.u0 = B->u; A.u1 = B->bu; A.floc0 = B->floc0; A.floc1 = B->floc1;
   A.mu_kl = B->mu_kl; A.kappa_kl = B->kappa_kl; A.dt = (double)B->dt;
+  A.mu_part = B->mu_part; A.kappa_part = B->kappa_part; A.kflush = B->kflush;
   A.active_mask = 0;
   for (int e = 0; e < E; ++e) A.active_mask |= active[e] ? 1 << e : 0;
   int dev = 0, ncu = 0;
@@ -681,6 +687,16 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_kl, (size_t)E * nloc);
   hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_kl, (size_t)E * nloc);
   hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * ng)), dim3(TPB), 0, s, B->rho_kl, (size_t)E * ng);
+  // fp32 partial sums of mu/kappa over blocks of the mesh handle's kernel_block steps
+  const int kb = m->kernel_block;
+  dev_free(B->mu_part);
+  dev_free(B->kappa_part);
+  if (kb > 1) {
+    B->mu_part = dev_alloc<float>((size_t)E * nloc);
+    B->kappa_part = dev_alloc<float>((size_t)E * nloc);
+    hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_part, (size_t)E * nloc);
+    hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_part, (size_t)E * nloc);
+  }
   HIP_CHECK(hipGetLastError());
   B->elem_ms = B->asm_ms = 0;
   B->elem_n = B->asm_n = 0;
@@ -688,6 +704,7 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   int active[EMAX] = {0};
   for (int it = 1; it <= N; ++it) {
     for (int e = 0; e < E; ++e) active[e] = it > N - B->nstep[e];
+    B->kflush = kb <= 1 || it % kb == 0 || it == N;  // every event is active at it == N
     if (tm) tm->mark(it, 0, s);
     elem_pass(B, 2, active);
     if (tm) tm->mark(it, 1, s);
@@ -760,6 +777,7 @@ void fwat_batch_destroy_(long* Batch_pointer) {
   dev_free(B->floc0); dev_free(B->floc1); dev_free(B->face_scratch); dev_free(B->store);
   dev_free(B->seis); dev_free(B->adj);
   dev_free(B->mu_kl); dev_free(B->kappa_kl); dev_free(B->rho_kl);
+  dev_free(B->mu_part); dev_free(B->kappa_part);
   delete B;
   *Batch_pointer = 0;
 }

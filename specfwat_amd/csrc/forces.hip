@@ -61,6 +61,10 @@ struct ForceArgs {
   double* mu_kl;
   double* kappa_kl;
   double* cijkl_kl;
+  float* mu_part;     // fp32 partial sums of the kernels (null: straight into fp64)
+  float* kappa_part;
+  float* cij_part;
+  int kflush;         // this step closes a partial-sum block
   double dt;
 };
 
@@ -204,12 +208,12 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
           const realw prod = (ea[0] - t3a) * (eb0 - t3b) + (ea[1] - t3a) * (eb1 - t3b) +
                              (ea[2] - t3a) * (eb2 - t3b) +
                              2.0f * (ea[3] * eb3 + ea[4] * eb4 + ea[5] * eb5);
-          A.mu_kl[Lg] += A.dt * (double)prod;
-          A.kappa_kl[Lg] += A.dt * (double)(tra * trb);
+          kernel_acc(A.mu_kl, A.mu_part, Lg, prod, A.dt, A.kflush != 0);
+          kernel_acc(A.kappa_kl, A.kappa_part, Lg, tra * trb, A.dt, A.kflush != 0);
         } else {
           const realw va[6] = {ea[0], ea[1], ea[2], 2.0f * ea[5], 2.0f * ea[4], 2.0f * ea[3]};
           const realw vb[6] = {eb0, eb1, eb2, 2.0f * eb5, 2.0f * eb4, 2.0f * eb3};
-          double* Kc = A.cijkl_kl + (size_t)ispec * 21 * 125 + t;
+          const size_t k0 = (size_t)ispec * 21 * 125 + t;
           int q = 0;
 #pragma unroll
           for (int P = 0; P < 6; ++P)
@@ -217,7 +221,7 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
             for (int Q = P; Q < 6; ++Q) {
               realw pr = va[P] * vb[Q];
               if (Q > P) pr += va[Q] * vb[P];
-              Kc[q * 125] += A.dt * (double)pr;
+              kernel_acc(A.cijkl_kl, A.cij_part, k0 + (size_t)q * 125, pr, A.dt, A.kflush != 0);
               ++q;
             }
         }
@@ -385,12 +389,12 @@ __global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
             const realw prod = (ea[0] - t3a) * (eb0 - t3b) + (ea[1] - t3a) * (eb1 - t3b) +
                                (ea[2] - t3a) * (eb2 - t3b) +
                                2.0f * (ea[3] * eb3 + ea[4] * eb4 + ea[5] * eb5);
-            A.mu_kl[Lg] += A.dt * (double)prod;
-            A.kappa_kl[Lg] += A.dt * (double)(tra * trb);
+            kernel_acc(A.mu_kl, A.mu_part, Lg, prod, A.dt, A.kflush != 0);
+            kernel_acc(A.kappa_kl, A.kappa_part, Lg, tra * trb, A.dt, A.kflush != 0);
           } else {
             const realw va[6] = {ea[0], ea[1], ea[2], 2.0f * ea[5], 2.0f * ea[4], 2.0f * ea[3]};
             const realw vb[6] = {eb0, eb1, eb2, 2.0f * eb5, 2.0f * eb4, 2.0f * eb3};
-            double* Kc = A.cijkl_kl + (size_t)e * 21 * 125 + t;
+            const size_t k0 = (size_t)e * 21 * 125 + t;
             int q = 0;
 #pragma unroll
             for (int P = 0; P < 6; ++P)
@@ -398,7 +402,7 @@ __global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
               for (int Q = P; Q < 6; ++Q) {
                 realw pr = va[P] * vb[Q];
                 if (Q > P) pr += va[Q] * vb[P];
-                Kc[q * 125] += A.dt * (double)pr;
+                kernel_acc(A.cijkl_kl, A.cij_part, k0 + (size_t)q * 125, pr, A.dt, A.kflush != 0);
                 ++q;
               }
           }
@@ -525,6 +529,14 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   a.mu_kl = m->d_mu_kl;
   a.kappa_kl = m->d_kappa_kl;
   a.cijkl_kl = m->d_cijkl_kl;
+  if (kernel_mode == 2 && m->kernel_block > 1) {
+    a.mu_part = m->d_mu_part;
+    a.kappa_part = m->d_kappa_part;
+    a.cij_part = m->d_cij_part;
+    if (m->anisotropic_kl ? !a.cij_part : (!a.mu_part || !a.kappa_part))
+      SEM_FAIL("kernel partial sums not allocated (fwat_iterate_time_ allocates them)");
+  }
+  a.kflush = m->kernel_flush;
   const int sp = m->static_precision;
   const bool an = m->anisotropy != 0;
   hipStream_t s = m->stream;

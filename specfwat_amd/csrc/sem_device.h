@@ -130,7 +130,14 @@ struct SemMesh {
   realw* d_epsilon = nullptr;    // [nspec][6][125] exx eyy ezz exy exz eyz
   realw* d_b_epsilon = nullptr;
 
-  // kernels (FP64 accumulators)
+  // kernels (FP64 accumulators).  The element-pass kernels (mu/kappa or the 21
+  // c_ij) reach them through fp32 partial sums over blocks of kernel_block steps
+  // (kernel_acc): part += x each step, K += dt * part at the block's last step.
+  int kernel_block = 16;  // 1: every step straight into fp64
+  int kernel_flush = 1;   // set per step by the time loop: this step closes a block
+  float* d_mu_part = nullptr;
+  float* d_kappa_part = nullptr;
+  float* d_cij_part = nullptr;  // [nspec][21][125]
   double* d_rho_kl_glob = nullptr;
   double* d_mu_kl = nullptr;
   double* d_kappa_kl = nullptr;
@@ -146,6 +153,20 @@ struct SemMesh {
   int* d_asm_off = nullptr;     // [nglob+1]
   int* d_asm_idx = nullptr;     // [ncontrib]
   size_t n_contrib = 0;
+
+  // brick-fused pass on regular lattice meshes (element + assembly in one
+  // kernel for brick-interior points; see brick.hip)
+  int lat_nex = 0, lat_ney = 0, lat_nez = 0;  // validated element lattice, 0 = none
+  int brick_x = 0, brick_y = 0, brick_z = 0;  // 0 = brick pass off
+  int brick_bpc = 0;                          // persistent workgroups per CU (0: one per brick)
+  int nbricks = 0;
+  uint8_t* d_brick_special = nullptr;  // [nbricks] 1: every point goes through the halo pass
+  realw* d_brick_part0 = nullptr;      // [nbricks][npts][3] partial sums of brick-boundary points
+  realw* d_brick_part1 = nullptr;      // second field of the adjoint pass
+  int nhalo = 0;
+  int* d_halo_g = nullptr;    // [nhalo] points assembled by the halo pass
+  int* d_halo_off = nullptr;  // [nhalo+1]
+  int* d_halo_idx = nullptr;  // slots b*npts+bl, then nslot + (face, source, receiver contributions)
 
   // timing (filled by the whole-loop driver)
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -192,7 +213,89 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks_padded) {
 
 inline int xcd_pad(int n) { return (n + 7) & ~7; }
 
+// One field's contributions to the Newmark update of a point (assembly passes).
+struct AsmField {
+  const realw* floc;
+  const realw* face;   // [nfp][3] traction*jw to subtract, or null
+  const double* stf;   // [nsources] this step, or null
+  const realw* adj;    // [nrec][3] this step, or null
+  realw* u;
+  realw* v;
+  realw* a;
+  realw dt;            // signed time step
+};
+
+// Newmark finish of point g once its force sums are complete: a = f * rmass_inv,
+// corrector v += dt/2 a of this step and, unless last, the predictor
+// u += dt v + dt^2/2 a, v += dt/2 a of the next one (last: a and v stored).
+// TWO: the second field too, and the fp64 density kernel a0 . u1 (and the
+// approximate Hessian a0 . a1) accumulated.  Every assembly pass calls this,
+// so they share one expression order.
+template <bool TWO>
+__device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict__ rm, realw f0x, realw f0y,
+                                              realw f0z, realw f1x, realw f1y, realw f1z, const AsmField& F0,
+                                              const AsmField& F1, bool last, double* __restrict__ rho_kl,
+                                              double* __restrict__ hess_kl, double dtk) {
+  const size_t b = 3 * g;
+  const realw r0 = rm[b], r1 = rm[b + 1], r2 = rm[b + 2];
+  const realw A[3] = {f0x * r0, f0y * r1, f0z * r2};
+  const realw d0 = F0.dt, h0 = 0.5f * F0.dt, s0 = 0.5f * F0.dt * F0.dt;
+  realw V[3], U[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { V[c] = F0.v[b + c] + h0 * A[c]; U[c] = F0.u[b + c]; }
+  if (TWO) {
+    const realw BA[3] = {f1x * r0, f1y * r1, f1z * r2};
+    const realw d1 = F1.dt, h1 = 0.5f * F1.dt, s1 = 0.5f * F1.dt * F1.dt;
+    realw BV[3], BU[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { BV[c] = F1.v[b + c] + h1 * BA[c]; BU[c] = F1.u[b + c]; }
+    rho_kl[g] += dtk * (double)(A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2]);
+    if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (last) {
+        F1.a[b + c] = BA[c];
+        F1.v[b + c] = BV[c];
+      } else {
+        F1.u[b + c] = BU[c] + d1 * BV[c] + s1 * BA[c];
+        F1.v[b + c] = BV[c] + h1 * BA[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (last) {
+      F0.a[b + c] = A[c];
+      F0.v[b + c] = V[c];
+    } else {
+      F0.u[b + c] = U[c] + d0 * V[c] + s0 * A[c];
+      F0.v[b + c] = V[c] + h0 * A[c];
+    }
+  }
+}
+
+// Time integration of one element-pass kernel density x (this step's
+// integrand, fp32): with partial sums, part += x and, at the block's last
+// step, K += dt * (part + x), part = 0; without (part == null), K += dt * x.
+__device__ __forceinline__ void kernel_acc(double* __restrict__ K, float* __restrict__ part, size_t i, float x,
+                                           double dt, bool flush) {
+  if (!part) {
+    K[i] += dt * (double)x;
+  } else if (flush) {
+    K[i] += dt * (double)(part[i] + x);
+    part[i] = 0.0f;
+  } else {
+    part[i] += x;
+  }
+}
+
 // internal launchers shared between the C-ABI files
 void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin, int elem_count,
                        realw deltat, const int* elem_list, bool gather);
 void sem_zero_kernels(SemMesh* m);
+// brick-fused pass (brick.hip): true when the mesh handle runs it
+bool sem_brick_active(const SemMesh* m, int nfields);
+// elements + Newmark of brick-interior points (field 0, and field 1 when nfields = 2)
+void sem_launch_brick(SemMesh* m, int nfields, const AsmField& F0, const AsmField& F1, bool last, double dtk);
+// Newmark of the halo points: brick partial sums + Stacey, source and receiver terms
+void sem_launch_halo(SemMesh* m, int nfields, const AsmField& F0, const AsmField& F1, bool last, double dtk);

@@ -82,16 +82,6 @@ __global__ void k_corrector(int nglob, realw* __restrict__ v, realw* __restrict_
 //   unless LAST: u += dt v + dt^2/2 a ; v += dt/2 a   (predictor of it+1)
 // TWO: the adjoint field (dt > 0) and the reconstructed field (dt < 0) are
 // updated together and the density kernel a_adj . u_rec is accumulated in fp64.
-struct AsmField {
-  const realw* floc;
-  const realw* face;   // [nfp][3] traction*jw to subtract, or null
-  const double* stf;   // [nsources] this step, or null
-  const realw* adj;    // [nrec][3] this step, or null
-  realw* u;
-  realw* v;
-  realw* a;
-  realw dt;            // signed time step
-};
 
 template <bool LAST, bool TWO>
 __global__ void k_assemble(int nglob, const int* __restrict__ off, const int* __restrict__ idx,
@@ -717,7 +707,26 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   const realw dt2o2 = 0.5f * dt * dt, dto2 = 0.5f * dt;
   if (*SIMULATION_TYPE == 1) {
     const int smode = (m->save_forward && m->d_stacey_store) ? 1 : 0;
-    if (fz) {
+    if (fz && sem_brick_active(m, 1)) {
+      // brick pass: elements + Newmark of brick-inner points; halo pass for the rest
+      const int nfp = m->nfaces * 25;
+      stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
+      for (int it = 1; it <= N; ++it) {
+        realw* face = nullptr;
+        if (m->absorbing && m->nfaces)
+          face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
+        AsmField F0{nullptr, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
+                    nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
+        if (tm) tm->mark(s);
+        sem_launch_brick(m, 1, F0, F0, it == N, 0.0);
+        if (tm) { tm->mark(s); m->last_force_launches += 1; }
+        if (face) stage_stacey_face(m, m->d_veloc, face);
+        stage_seismo(m, it - 1);
+        if (ta) ta->mark(s);
+        sem_launch_halo(m, 1, F0, F0, it == N, 0.0);
+        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+      }
+    } else if (fz) {
       const int nfp = m->nfaces * 25;
       stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
       for (int it = 1; it <= N; ++it) {
@@ -754,11 +763,48 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
     const double dtk = (double)dt;
     const size_t n3 = (size_t)m->nglob * 3;
     hipLaunchKernelGGL(k_b_init, dim3(nblk(n3)), dim3(TPB), 0, s, n3, m->d_b_veloc, m->d_b_accel, 0.5f * dt);
-    if (fz) {
+    const int kb = m->kernel_block;
+    if (fz && kb > 1) {
+      // fp32 partial sums of the element-pass kernels, zero at the start of the run
+      const size_t nloc = (size_t)m->nspec * 125;
+      if (m->anisotropic_kl) {
+        if (!m->d_cij_part) m->d_cij_part = dev_alloc<float>(nloc * 21);
+        hipLaunchKernelGGL(k_fill_f32, dim3(nblk(nloc * 21)), dim3(TPB), 0, s, m->d_cij_part, nloc * 21);
+      } else {
+        if (!m->d_mu_part) m->d_mu_part = dev_alloc<float>(nloc);
+        if (!m->d_kappa_part) m->d_kappa_part = dev_alloc<float>(nloc);
+        hipLaunchKernelGGL(k_fill_f32, dim3(nblk(nloc)), dim3(TPB), 0, s, m->d_mu_part, nloc);
+        hipLaunchKernelGGL(k_fill_f32, dim3(nblk(nloc)), dim3(TPB), 0, s, m->d_kappa_part, nloc);
+      }
+    }
+    // a block closes every kb steps and at the last step
+    const auto closes_block = [kb, N](int it) { return kb <= 1 || it % kb == 0 || it == N; };
+    if (fz && sem_brick_active(m, 2) && !m->anisotropic_kl) {
       const int nfp = m->nfaces * 25;
       stage_predictor(m, false, dt);  // adjoint field (from rest)
       for (int it = 1; it <= N; ++it) {
         const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
+        realw* face_a = (m->absorbing && m->nfaces) ? m->d_face_scratch : nullptr;
+        const realw* face_b = (m->absorbing && m->nfaces) ? m->d_stacey_store + (size_t)rec0 * nfp * 3 : nullptr;
+        AsmField F0{nullptr, face_a, nullptr, m->nrec ? m->d_adj + (size_t)rec0 * m->nrec * 3 : nullptr,
+                    m->d_displ, m->d_veloc, m->d_accel, dt};
+        AsmField F1{nullptr, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
+                    nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
+        m->kernel_flush = closes_block(it);
+        if (tm) tm->mark(s);
+        sem_launch_brick(m, 2, F0, F1, it == N, dtk);
+        if (tm) { tm->mark(s); m->last_force_launches += 1; }
+        if (face_a) stage_stacey_face(m, m->d_veloc, face_a);
+        if (ta) ta->mark(s);
+        sem_launch_halo(m, 2, F0, F1, it == N, dtk);
+        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+      }
+    } else if (fz) {
+      const int nfp = m->nfaces * 25;
+      stage_predictor(m, false, dt);  // adjoint field (from rest)
+      for (int it = 1; it <= N; ++it) {
+        const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
+        m->kernel_flush = closes_block(it);
         stage_forces_timed(m, 2, 2, dt, tm, true);
         realw* face_a = nullptr;
         if (m->absorbing && m->nfaces) {
@@ -799,6 +845,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   } else {
     SEM_FAIL("SIMULATION_TYPE %d not supported", *SIMULATION_TYPE);
   }
+  m->kernel_flush = 1;
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
   if (tm) m->last_force_ms = tm->total_ms();
@@ -852,6 +899,15 @@ void fwat_set_tuning_(long* Mesh_pointer, int* regular_path, int* blocks_per_cu_
   m->regular_fast_path = *regular_path != 0;
   m->blocks_per_cu_fwd = *blocks_per_cu_fwd;
   m->blocks_per_cu_adj = *blocks_per_cu_adj;
+}
+
+// Steps per fp32 partial-sum block of the element-pass kernels (mu/kappa or
+// c_ij) in the fused adjoint loop: 1 accumulates every step straight into the
+// fp64 arrays; n > 1 sums n steps in fp32 before adding them into fp64.
+void fwat_set_kernel_block_(long* Mesh_pointer, int* nsteps) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*nsteps < 1 || *nsteps > 256) SEM_FAIL("fwat_set_kernel_block_: steps per block must be in 1..256");
+  m->kernel_block = *nsteps;
 }
 
 // Summed durations (ms) and launch counts of the assembly passes of the

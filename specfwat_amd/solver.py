@@ -90,12 +90,35 @@ class ElasticRun:
             _lib.call("prepare_sim2_or_3_const_device_", self.h, self.nrec, self.nstep)
         if static_precision:
             _lib.call("fwat_set_static_precision_", self.h, int(static_precision))
+        # a box mesh of regular elements is an element lattice: declare it (checked on the
+        # device side) so that the brick-fused pass can be switched on
+        self.lattice = (mesh.nspec_irregular == 0 and hasattr(mesh, "nex")
+                        and mesh.nex * mesh.ney * mesh.nez == mesh.nspec)
+        if self.lattice:
+            _lib.call("fwat_set_lattice_", self.h, mesh.nex, mesh.ney, mesh.nez)
         if self.nsrc:
             self.stf = sources.stf(self.nstep, self.dt)
             _lib.call("fwat_set_stf_", self.h, self.stf, self.nstep)
 
-    def set_tuning(self, regular_path=1, blocks_per_cu_fwd=0, blocks_per_cu_adj=8):
+    def set_tuning(self, regular_path=1, blocks_per_cu_fwd=0, blocks_per_cu_adj=8, brick=(0, 0, 0),
+                   brick_blocks_per_cu=0):
+        """Element-pass launch shape; brick=(bx, by, bz) switches the brick-fused
+        element + assembly pass on (lattice meshes of regular elements only)."""
         _lib.call("fwat_set_tuning_", self.h, int(regular_path), int(blocks_per_cu_fwd), int(blocks_per_cu_adj))
+        if brick[0] and not self.lattice:
+            raise ValueError("the brick pass needs a lattice mesh of regular elements")
+        if brick[0] or self.lattice:
+            _lib.call("fwat_set_brick_", self.h, int(brick[0]), int(brick[1]), int(brick[2]),
+                      int(brick_blocks_per_cu))
+
+    def set_kernel_block(self, nsteps):
+        """Steps per fp32 partial-sum block of the mu/kappa (c_ij) kernels; 1 = fp64 every step."""
+        _lib.call("fwat_set_kernel_block_", self.h, int(nsteps))
+
+    def brick_info(self):
+        """(halo points, partial-sum slots per field and step, halo CSR length) of the brick pass."""
+        nh, ns, nc = _lib.call("fwat_brick_info_", self.h, 0, 0, 0)
+        return int(nh), int(ns), int(nc)
 
     # ------------------------------------------------------------ runs
     def set_adjoint_sources(self, adj):

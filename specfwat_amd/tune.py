@@ -2,20 +2,24 @@
 
 Times short forward and adjoint runs on the real mesh for a set of
 candidate shapes (general kernel, folded-table kernel with one workgroup
-per element, persistent workgroups per CU) and returns the fastest; the
-caller applies it with ElasticRun.set_tuning.  The measurement is the
-live HIP-event time of the element passes on the simulation stream.
+per element, persistent workgroups per CU) and returns the fastest; on
+lattice meshes it then compares the brick-fused element + assembly pass
+(brick shapes 2x2x2, 4x2x2, 4x4x2) against the best gather shapes.  The
+caller applies the result with ElasticRun.set_tuning.  The measurement is
+the live HIP-event time of the passes on the simulation stream.
 """
 import numpy as np
 
 from .solver import ElasticRun
 
 CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32)]
+BRICKS = [(2, 2, 2), (4, 2, 2), (4, 4, 2)]
 
 
 def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidates=CANDIDATES,
-                        static_precision=0):
-    """Returns ({'regular_path', 'blocks_per_cu_fwd', 'blocks_per_cu_adj'}, table)."""
+                        static_precision=0, bricks=BRICKS):
+    """Returns ({'regular_path', 'blocks_per_cu_fwd', 'blocks_per_cu_adj'[, 'brick']}, table);
+    the dict is ElasticRun.set_tuning's keyword arguments."""
     rho, vp, vs = model
     f = ElasticRun(mesh, rho, vp, vs, sources, receivers, dt, nstep, 1, True,
                    static_precision=static_precision)
@@ -33,12 +37,37 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
         a.iterate_time(timed=True)
         ams, an = a.force_time()
         table.append((reg, bpc, fms / max(fn, 1), ams / max(an, 1)))
-    f.close()
-    a.close()
     best_f = min(table, key=lambda r: r[2])
     best_a = min(table, key=lambda r: r[3])
     # one regular_path switch for both passes: keep the better total
     reg = 1 if (best_f[0] + best_a[0]) >= 1 else 0
     bf = min((r for r in table if r[0] == reg), key=lambda r: r[2])[1]
     ba = min((r for r in table if r[0] == reg), key=lambda r: r[3])[1]
-    return {"regular_path": reg, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba}, table
+    best = {"regular_path": reg, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba}
+    # brick-fused pass (lattice meshes): element + assembly time per step against the
+    # element + assembly time of the best gather shapes
+    if bricks and f.lattice:
+        def step_ms(**kw):
+            f.set_tuning(**kw)
+            a.set_tuning(**kw)
+            f.iterate_time(timed=True)
+            fe, fa = f.force_time(), f.assembly_time()
+            a.iterate_time(timed=True)
+            ae, aa = a.force_time(), a.assembly_time()
+            return (fe[0] + fa[0]) / max(fe[1], 1), (ae[0] + aa[0]) / max(ae[1], 1)
+        base_f, base_a = step_ms(**best)
+        table.append(("gather", best["blocks_per_cu_fwd"], base_f, base_a))
+        top = (base_f + base_a, None)
+        for br in bricks:
+            m = mesh
+            if m.nex % br[0] or m.ney % br[1] or m.nez % br[2]:
+                continue
+            tf, ta = step_ms(**best, brick=br)
+            table.append(("brick%dx%dx%d" % br, 0, tf, ta))
+            if tf + ta < top[0]:
+                top = (tf + ta, br)
+        if top[1] is not None:
+            best = dict(best, brick=top[1])
+    f.close()
+    a.close()
+    return best, table

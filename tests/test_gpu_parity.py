@@ -272,3 +272,36 @@ def test_element_pass_launch_shapes_agree():
         results.append(syn)
     for s in results[1:]:
         assert rel(s, results[0]) < 1e-6
+
+
+@pytest.mark.parametrize("aniso", [False, True])
+def test_kernel_partial_sum_blocks_match_fp64_every_step(aniso):
+    """fp32 partial sums over blocks of steps (16, and 7 which does not divide
+    NSTEP) give the kernels of straight fp64 accumulation, within the bar vs
+    the FP64 oracle."""
+    c = make_case(seed=17, nstep=90, aniso=aniso)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)
+    ko = o.adjoint(adj)
+    m = c["mesh"]
+    out = {}
+    for kb in (1, 16, 7):
+        src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+        rec = Receivers(m, c["rxyz"])
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True, cij=c["cij"])
+        f.iterate_time()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False, cij=c["cij"])
+        a.set_kernel_block(kb)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj)
+        a.iterate_time()
+        out[kb] = a.kernels()
+        a.close()
+    names = ("rho_kl", "cijkl_kl") if aniso else ("rho_kl", "mu_kl", "kappa_kl")
+    for kb in (16, 7):
+        for n in names:
+            e1, eo = rel(out[kb][n], out[1][n]), rel(out[kb][n], ko[n])
+            print(f"aniso={aniso} block {kb} {n}: vs fp64-every-step {e1:.2e}, vs FP64 oracle {eo:.2e}")
+            assert e1 < 1e-5 and eo < KERN_TOL, (kb, n)
