@@ -35,7 +35,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10000)
+    p.add_argument("--steps", type=int, default=None,
+                   help="NSTEP of the timed event (default: the configuration's, 10000 for c2, 2000 for c1)")
     p.add_argument("--warmup", type=int, default=100)
     p.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2",
                    help="c2: one production-size event (default); c1: the small event; "
@@ -47,10 +48,14 @@ def parse():
                    help="NSTEP of the c4 events is U[4000,12000] times this factor")
     p.add_argument("--events-per-batch", type=int, default=0, help="0: autotuned")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-steps", type=int, default=4)
+    p.add_argument("--cpu-steps", type=int, default=None,
+                   help="steps of the CPU baseline sample (default: 4 for c2/c4, the whole run for c1)")
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--no-tune", action="store_true", help="skip the element-pass launch-shape autotuner")
+    p.add_argument("--tuning", default="", metavar="R,F,A",
+                   help="fixed launch shape instead of the autotuner: regular_path, forward and adjoint "
+                        "workgroups per CU (e.g. 1,16,8), as the autotuner picks them")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     return p.parse_args()
@@ -62,10 +67,10 @@ def log(msg):
 
 def workload(cfg):
     if cfg == "c2":
-        return dict(nex=96, ney=96, nez=48, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.004, nrx=20)
+        return dict(nex=96, ney=96, nez=48, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.004, nrx=20, nstep=10000)
     if cfg == "c4":
-        return dict(nex=64, ney=64, nez=32, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.005, nrx=20)
-    return dict(nex=16, ney=16, nez=8, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.005, nrx=4)
+        return dict(nex=64, ney=64, nez=32, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.005, nrx=20, nstep=8000)
+    return dict(nex=16, ney=16, nez=8, lx=40000.0, ly=40000.0, lz=20000.0, dt=0.005, nrx=4, nstep=2000)
 
 
 def main():
@@ -94,7 +99,8 @@ def main():
     from specfwat_amd.misfit import event_adjoint_sources
 
     w = workload(a.config)
-    K, W = a.steps, a.warmup
+    K, W = a.steps or w["nstep"], a.warmup
+    cpu_steps = a.cpu_steps or (K if a.config == "c1" else 4)
     t0 = time.time()
     mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
     log(f"mesh {mesh.nspec} elements, {mesh.nglob} points ({time.time() - t0:.1f}s)")
@@ -109,7 +115,10 @@ def main():
     log(f"model/source/receivers ready ({time.time() - t0:.1f}s)")
     fused = not a.unfused
     tuning = {"regular_path": 1, "blocks_per_cu_fwd": 0, "blocks_per_cu_adj": 8}
-    if fused and not a.no_tune:
+    if a.tuning:
+        r, bf, ba = (int(v) for v in a.tuning.split(","))
+        tuning = {"regular_path": r, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba}
+    elif fused and not a.no_tune:
         from specfwat_amd.tune import tune_element_passes
         tuning, table = tune_element_passes(mesh, smooth_model, src, rec, dt,
                                             static_precision=a.static_precision)
@@ -118,6 +127,7 @@ def main():
 
     def observed(nstep):
         r = ElasticRun(mesh, *true_model, src, rec, dt, nstep, 1, False)
+        r.set_tuning(**tuning)
         r.iterate_time()
         obs = r.seismograms()
         r.close()
@@ -263,7 +273,8 @@ def main():
                                             if fwd_asm_n and fwd_asm_ms > 0 else None)},
     }
     if not a.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, sxyz, mt, rec, a.cpu_steps)
+        result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, sxyz, mt, rec, cpu_steps,
+                                              label=("C1 event" if a.config == "c1" else "C2 event"))
     print(json.dumps(result), flush=True)
 
 
@@ -439,7 +450,7 @@ def batched(a):
     if not a.no_cpu:
         s0 = srcs[0]
         result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, s0.xyz.tolist(), s0.mt.tolist(), rec,
-                                              a.cpu_steps, f0=float(s0.f0[0]), label="C4 event")
+                                              a.cpu_steps or 4, f0=float(s0.f0[0]), label="C4 event")
     print(json.dumps(result), flush=True)
 
 
@@ -465,7 +476,7 @@ def precision(a):
     from specfwat_amd.sources import Receivers, Sources, surface_grid
 
     w = workload("c1")
-    K = a.steps if a.steps != 10000 else 2000
+    K = a.steps or 2000
     dt = w["dt"]
     rows, timing = {}, {}
     for firr, kind in ((False, "starting"), (False, "perturbed"), (True, "starting"), (True, "perturbed")):
@@ -547,7 +558,7 @@ def aniso_smooth(a):
     from specfwat_amd.tune import tune_element_passes
 
     w = workload("c4")
-    K = a.steps if a.steps != 10000 else 8000
+    K = a.steps or 8000
     dt = w["dt"]
     t0 = time.time()
     mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])

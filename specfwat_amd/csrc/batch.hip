@@ -249,6 +249,8 @@ struct BatchAsmArgs {
   const realw* floc1;
   realw *u0, *v0, *a0, *u1, *v1, *a1;  // [nglob][E][3]
   double* rho_kl;      // [nglob][E]
+  float* rho_part;     // [nglob][E] fp32 partial sums (null: straight into fp64)
+  int kflush;
   const int* nstep;    // [E]
   const realw* face0;  // [nfp][E][3] traction of field 0 (forward: store row it-1; adjoint: scratch)
   const realw* face1;  // [nfp][E][3] replayed store row NSTEP_max - it (adjoint)
@@ -316,7 +318,7 @@ __global__ void k_batch_asm(BatchAsmArgs A) {
     realw BV[3], BU[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) { BV[c] = A.v1[o + c] - h0 * BA[c]; BU[c] = A.u1[o + c]; }
-    A.rho_kl[q] += A.dtk * (double)(Ac[0] * BU[0] + Ac[1] * BU[1] + Ac[2] * BU[2]);
+    kernel_acc(A.rho_kl, A.rho_part, q, Ac[0] * BU[0] + Ac[1] * BU[1] + Ac[2] * BU[2], A.dtk, A.kflush != 0);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if (last) { A.a1[o + c] = BA[c]; A.v1[o + c] = BV[c]; }
@@ -415,6 +417,7 @@ struct SemBatch {
   realw* adj = nullptr;                               // [E][nmax][nrec][3]
   double *mu_kl = nullptr, *kappa_kl = nullptr;       // [E][nloc]
   float *mu_part = nullptr, *kappa_part = nullptr;    // [E][nloc] fp32 partial sums
+  float* rho_part = nullptr;                          // [nglob][E]
   int kflush = 1;
   double* rho_kl = nullptr;                           // [nglob][E]
   double elem_ms = 0, asm_ms = 0;
@@ -482,6 +485,7 @@ void asm_pass(SemBatch* B, bool two, int it) {
   A.floc0 = B->floc0; A.floc1 = B->floc1;
   A.u0 = B->u; A.v0 = B->v; A.a0 = B->a; A.u1 = B->bu; A.v1 = B->bv; A.a1 = B->ba;
   A.rho_kl = B->rho_kl; A.nstep = B->d_nstep; A.stf = B->d_stf; A.adj = B->adj;
+  A.rho_part = B->rho_part; A.kflush = B->kflush;
   A.dt = B->dt; A.dtk = (double)B->dt;
   const size_t row = B->nfp * B->E * 3;
   if (two) {
@@ -691,11 +695,14 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   const int kb = m->kernel_block;
   dev_free(B->mu_part);
   dev_free(B->kappa_part);
+  dev_free(B->rho_part);
   if (kb > 1) {
     B->mu_part = dev_alloc<float>((size_t)E * nloc);
     B->kappa_part = dev_alloc<float>((size_t)E * nloc);
+    B->rho_part = dev_alloc<float>((size_t)E * ng);
     hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_part, (size_t)E * nloc);
     hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_part, (size_t)E * nloc);
+    hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * ng)), dim3(TPB), 0, s, B->rho_part, (size_t)E * ng);
   }
   HIP_CHECK(hipGetLastError());
   B->elem_ms = B->asm_ms = 0;
@@ -777,7 +784,7 @@ void fwat_batch_destroy_(long* Batch_pointer) {
   dev_free(B->floc0); dev_free(B->floc1); dev_free(B->face_scratch); dev_free(B->store);
   dev_free(B->seis); dev_free(B->adj);
   dev_free(B->mu_kl); dev_free(B->kappa_kl); dev_free(B->rho_kl);
-  dev_free(B->mu_part); dev_free(B->kappa_part);
+  dev_free(B->mu_part); dev_free(B->kappa_part); dev_free(B->rho_part);
   delete B;
   *Batch_pointer = 0;
 }

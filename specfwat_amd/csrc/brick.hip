@@ -49,8 +49,9 @@ struct BrickArgs {
   double* kappa_kl;
   double* rho_kl;
   double* hess_kl;
-  float* mu_part;     // fp32 partial sums of mu/kappa (null: straight into fp64)
+  float* mu_part;     // fp32 partial sums of mu/kappa/rho (null: straight into fp64)
   float* kappa_part;
+  float* rho_part;
   int kflush;
   double dt;   // kernel time weight of the element pass (mu/kappa)
   double dtk;  // kernel time weight of the point update (rho, Hessian)
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(128) void k_brick(BrickArgs A) {
       const realw f1z = NF == 2 ? s_acc[NF - 1][2][q] : 0.0f;
       if (inner) {
         newmark_point<NF == 2>((size_t)s_g[q], A.rm, f0x, f0y, f0z, f1x, f1y, f1z, A.F0, A.F1, A.last != 0,
-                               A.rho_kl, A.hess_kl, A.dtk);
+                               A.rho_kl, A.rho_part, A.kflush != 0, A.hess_kl, A.dtk);
       } else {
         const size_t o = 3 * ((size_t)b * NPTS + q);
         A.part0[o] = f0x; A.part0[o + 1] = f0y; A.part0[o + 2] = f0z;
@@ -213,7 +214,8 @@ __global__ void k_halo(int nhalo, const int* __restrict__ halo_g, const int* __r
                        const int* __restrict__ idx, int nslot, int nfp, int nsp, const realw* __restrict__ sarr,
                        const realw* __restrict__ rech, const realw* __restrict__ rm, const realw* __restrict__ part0,
                        const realw* __restrict__ part1, AsmField F0, AsmField F1, int last,
-                       double* __restrict__ rho_kl, double* __restrict__ hess_kl, double dtk) {
+                       double* __restrict__ rho_kl, float* __restrict__ rho_part, int flush,
+                       double* __restrict__ hess_kl, double dtk) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nhalo) return;
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
@@ -244,7 +246,8 @@ __global__ void k_halo(int nhalo, const int* __restrict__ halo_g, const int* __r
       if (TWO && F1.adj) { f1x += hh * F1.adj[r]; f1y += hh * F1.adj[r + 1]; f1z += hh * F1.adj[r + 2]; }
     }
   }
-  newmark_point<TWO>((size_t)halo_g[h], rm, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last != 0, rho_kl, hess_kl, dtk);
+  newmark_point<TWO>((size_t)halo_g[h], rm, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last != 0, rho_kl, rho_part,
+                     flush != 0, hess_kl, dtk);
 }
 
 void brick_free(SemMesh* m) {
@@ -318,7 +321,8 @@ void sem_launch_brick(SemMesh* m, int nfields, const AsmField& F0, const AsmFiel
   if (nfields == 2 && m->kernel_block > 1) {
     a.mu_part = m->d_mu_part;
     a.kappa_part = m->d_kappa_part;
-    if (!a.mu_part || !a.kappa_part) SEM_FAIL("kernel partial sums not allocated");
+    a.rho_part = m->d_rho_part;
+    if (!a.mu_part || !a.kappa_part || !a.rho_part) SEM_FAIL("kernel partial sums not allocated");
   }
   a.kflush = m->kernel_flush;
   a.dt = (double)F0.dt;
@@ -342,11 +346,12 @@ void sem_launch_halo(SemMesh* m, int nfields, const AsmField& F0, const AsmField
   if (nfields == 2)
     hipLaunchKernelGGL(k_halo<true>, gr, bl, 0, m->stream, m->nhalo, m->d_halo_g, m->d_halo_off, m->d_halo_idx, nslot,
                        nfp, nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, m->d_brick_part0, m->d_brick_part1,
-                       F0, F1, last ? 1 : 0, m->d_rho_kl_glob, m->d_hess_kl_glob, dtk);
+                       F0, F1, last ? 1 : 0, m->d_rho_kl_glob, m->kernel_block > 1 ? m->d_rho_part : nullptr,
+                       m->kernel_flush, m->d_hess_kl_glob, dtk);
   else
     hipLaunchKernelGGL(k_halo<false>, gr, bl, 0, m->stream, m->nhalo, m->d_halo_g, m->d_halo_off, m->d_halo_idx, nslot,
                        nfp, nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, m->d_brick_part0, m->d_brick_part1,
-                       F0, F1, last ? 1 : 0, (double*)nullptr, (double*)nullptr, dtk);
+                       F0, F1, last ? 1 : 0, (double*)nullptr, (float*)nullptr, 1, (double*)nullptr, dtk);
   HIP_CHECK(hipGetLastError());
 }
 

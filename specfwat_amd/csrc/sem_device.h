@@ -138,6 +138,7 @@ struct SemMesh {
   float* d_mu_part = nullptr;
   float* d_kappa_part = nullptr;
   float* d_cij_part = nullptr;  // [nspec][21][125]
+  float* d_rho_part = nullptr;  // [nglob] partial sums of the density kernel
   double* d_rho_kl_glob = nullptr;
   double* d_mu_kl = nullptr;
   double* d_kappa_kl = nullptr;
@@ -213,6 +214,21 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks_padded) {
 
 inline int xcd_pad(int n) { return (n + 7) & ~7; }
 
+// Time integration of one kernel density x (this step's integrand, fp32):
+// with partial sums, part += x and, at the block's last step,
+// K += dt * (part + x), part = 0; without (part == null), K += dt * x.
+__device__ __forceinline__ void kernel_acc(double* __restrict__ K, float* __restrict__ part, size_t i, float x,
+                                           double dt, bool flush) {
+  if (!part) {
+    K[i] += dt * (double)x;
+  } else if (flush) {
+    K[i] += dt * (double)(part[i] + x);
+    part[i] = 0.0f;
+  } else {
+    part[i] += x;
+  }
+}
+
 // One field's contributions to the Newmark update of a point (assembly passes).
 struct AsmField {
   const realw* floc;
@@ -228,13 +244,15 @@ struct AsmField {
 // Newmark finish of point g once its force sums are complete: a = f * rmass_inv,
 // corrector v += dt/2 a of this step and, unless last, the predictor
 // u += dt v + dt^2/2 a, v += dt/2 a of the next one (last: a and v stored).
-// TWO: the second field too, and the fp64 density kernel a0 . u1 (and the
-// approximate Hessian a0 . a1) accumulated.  Every assembly pass calls this,
-// so they share one expression order.
+// TWO: the second field too, and the density kernel a0 . u1 (fp64, through
+// fp32 block partial sums when rho_part is given) and the approximate Hessian
+// a0 . a1 (fp64) accumulated.  Every assembly pass calls this, so they share
+// one expression order.
 template <bool TWO>
 __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict__ rm, realw f0x, realw f0y,
                                               realw f0z, realw f1x, realw f1y, realw f1z, const AsmField& F0,
                                               const AsmField& F1, bool last, double* __restrict__ rho_kl,
+                                              float* __restrict__ rho_part, bool flush,
                                               double* __restrict__ hess_kl, double dtk) {
   const size_t b = 3 * g;
   const realw r0 = rm[b], r1 = rm[b + 1], r2 = rm[b + 2];
@@ -249,7 +267,7 @@ __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict_
     realw BV[3], BU[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) { BV[c] = F1.v[b + c] + h1 * BA[c]; BU[c] = F1.u[b + c]; }
-    rho_kl[g] += dtk * (double)(A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2]);
+    kernel_acc(rho_kl, rho_part, g, A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2], dtk, flush);
     if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -271,21 +289,6 @@ __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict_
       F0.u[b + c] = U[c] + d0 * V[c] + s0 * A[c];
       F0.v[b + c] = V[c] + h0 * A[c];
     }
-  }
-}
-
-// Time integration of one element-pass kernel density x (this step's
-// integrand, fp32): with partial sums, part += x and, at the block's last
-// step, K += dt * (part + x), part = 0; without (part == null), K += dt * x.
-__device__ __forceinline__ void kernel_acc(double* __restrict__ K, float* __restrict__ part, size_t i, float x,
-                                           double dt, bool flush) {
-  if (!part) {
-    K[i] += dt * (double)x;
-  } else if (flush) {
-    K[i] += dt * (double)(part[i] + x);
-    part[i] = 0.0f;
-  } else {
-    part[i] += x;
   }
 }
 

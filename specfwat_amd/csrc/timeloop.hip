@@ -88,7 +88,7 @@ __global__ void k_assemble(int nglob, const int* __restrict__ off, const int* __
                            int nloc, int nfp, int nsp, const realw* __restrict__ sarr,
                            const realw* __restrict__ rech, const realw* __restrict__ rm,
                            AsmField F0, AsmField F1, double* __restrict__ rho_kl,
-                           double* __restrict__ hess_kl, double dtk) {
+                           float* __restrict__ rho_part, int flush, double* __restrict__ hess_kl, double dtk) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nglob) return;
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
@@ -119,42 +119,8 @@ __global__ void k_assemble(int nglob, const int* __restrict__ off, const int* __
       if (TWO && F1.adj) { f1x += h * F1.adj[r]; f1y += h * F1.adj[r + 1]; f1z += h * F1.adj[r + 2]; }
     }
   }
-  const size_t b = 3 * (size_t)g;
-  const realw r0 = rm[b], r1 = rm[b + 1], r2 = rm[b + 2];
-  realw A[3] = {f0x * r0, f0y * r1, f0z * r2};
-  const realw d0 = F0.dt, h0 = 0.5f * F0.dt, s0 = 0.5f * F0.dt * F0.dt;
-  realw V[3], U[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) { V[c] = F0.v[b + c] + h0 * A[c]; U[c] = F0.u[b + c]; }
-  if (TWO) {
-    realw BA[3] = {f1x * r0, f1y * r1, f1z * r2};
-    const realw d1 = F1.dt, h1 = 0.5f * F1.dt, s1 = 0.5f * F1.dt * F1.dt;
-    realw BV[3], BU[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { BV[c] = F1.v[b + c] + h1 * BA[c]; BU[c] = F1.u[b + c]; }
-    rho_kl[g] += dtk * (double)(A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2]);
-    if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (LAST) {
-        F1.a[b + c] = BA[c];
-        F1.v[b + c] = BV[c];
-      } else {
-        F1.u[b + c] = BU[c] + d1 * BV[c] + s1 * BA[c];
-        F1.v[b + c] = BV[c] + h1 * BA[c];
-      }
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (LAST) {
-      F0.a[b + c] = A[c];
-      F0.v[b + c] = V[c];
-    } else {
-      F0.u[b + c] = U[c] + d0 * V[c] + s0 * A[c];
-      F0.v[b + c] = V[c] + h0 * A[c];
-    }
-  }
+  newmark_point<TWO>((size_t)g, rm, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, LAST, rho_kl, rho_part, flush != 0,
+                     hess_kl, dtk);
 }
 
 // Stacey traction of one field written per face point (no scatter):
@@ -392,22 +358,27 @@ void stage_stacey_face(SemMesh* m, const realw* v, realw* out) {
 
 void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const AsmField& F1, double dtk) {
   const int nfp = m->absorbing ? m->nfaces * 25 : 0;
+  float* rho_part = m->kernel_block > 1 ? m->d_rho_part : nullptr;
   const int nloc = m->nspec * 125, nsp = m->nsources * 125;
   const dim3 gr(nblk(m->nglob)), bl(TPB);
   if (two) {
     if (last)
       hipLaunchKernelGGL((k_assemble<true, true>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
-                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, m->d_hess_kl_glob, dtk);
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, rho_part, m->kernel_flush,
+                         m->d_hess_kl_glob, dtk);
     else
       hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
-                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, m->d_hess_kl_glob, dtk);
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, rho_part, m->kernel_flush,
+                         m->d_hess_kl_glob, dtk);
   } else {
     if (last)
       hipLaunchKernelGGL((k_assemble<true, false>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
-                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (double*)nullptr, dtk);
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (float*)nullptr, 1,
+                         (double*)nullptr, dtk);
     else
       hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
-                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (double*)nullptr, dtk);
+                         nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (float*)nullptr, 1,
+                         (double*)nullptr, dtk);
   }
 }
 
@@ -770,7 +741,10 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       if (m->anisotropic_kl) {
         if (!m->d_cij_part) m->d_cij_part = dev_alloc<float>(nloc * 21);
         hipLaunchKernelGGL(k_fill_f32, dim3(nblk(nloc * 21)), dim3(TPB), 0, s, m->d_cij_part, nloc * 21);
-      } else {
+      }
+      if (!m->d_rho_part) m->d_rho_part = dev_alloc<float>((size_t)m->nglob);
+      hipLaunchKernelGGL(k_fill_f32, dim3(nblk((size_t)m->nglob)), dim3(TPB), 0, s, m->d_rho_part, (size_t)m->nglob);
+      if (!m->anisotropic_kl) {
         if (!m->d_mu_part) m->d_mu_part = dev_alloc<float>(nloc);
         if (!m->d_kappa_part) m->d_kappa_part = dev_alloc<float>(nloc);
         hipLaunchKernelGGL(k_fill_f32, dim3(nblk(nloc)), dim3(TPB), 0, s, m->d_mu_part, nloc);
