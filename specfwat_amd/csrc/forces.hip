@@ -241,6 +241,7 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
     } else {
       // c index (P,Q), P<=Q, upper-triangle row-major: 11 12 13 14 15 16 22 .. 66
       const realw e1 = duxdx, e2 = duydy, e3 = duzdz, e4 = eyz2, e5 = exz2, e6 = exy2;
+      asm volatile("" ::: "memory");  // reload c_ij per field rather than keep 21 registers live
       const realw c11 = C[0], c12 = C[125], c13 = C[250], c14 = C[375], c15 = C[500], c16 = C[625];
       const realw c22 = C[750], c23 = C[875], c24 = C[1000], c25 = C[1125], c26 = C[1250];
       const realw c33 = C[1375], c34 = C[1500], c35 = C[1625], c36 = C[1750];
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
 // Workgroups are persistent: each loops over its share of an XCD-contiguous
 // run of elements, so the table set-up is paid once per workgroup.
 template <int NF, bool ANISO, int KMODE, int SP>
-__global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
+__device__ __forceinline__ void forces_reg_body(const ForceArgs& A) {
   __shared__ realw s_tab[128];
   __shared__ realw s_u[3][128];
   __shared__ realw s_s[6][128];
@@ -421,6 +422,9 @@ __global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
         syz = ml * eyz2;
       } else {
         const realw e1 = duxdx, e2 = duydy, e3 = duzdz, e4 = eyz2, e5 = exz2, e6 = exy2;
+        // load the 21 c_ij here for each field (cache hits the second time) instead of
+        // keeping them live across both fields: 144 -> fewer VGPRs, more waves per SIMD
+        asm volatile("" ::: "memory");
         const realw c11 = C[0], c12 = C[125], c13 = C[250], c14 = C[375], c15 = C[500], c16 = C[625];
         const realw c22 = C[750], c23 = C[875], c24 = C[1000], c25 = C[1125], c26 = C[1250];
         const realw c33 = C[1375], c34 = C[1500], c35 = C[1625], c36 = C[1750];
@@ -457,6 +461,19 @@ __global__ __launch_bounds__(128) void k_forces_reg(ForceArgs A) {
   }
 }
 
+// Kernel entries of the regular-mesh pass.  The isotropic one asks for five
+// waves per SIMD (fits in 96 VGPRs without spills); the anisotropic one keeps
+// the compiler's allocation (96 would spill the c_ij products).
+template <int NF, bool ANISO, int KMODE, int SP>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k_forces_reg(ForceArgs A) {
+  forces_reg_body<NF, ANISO, KMODE, SP>(A);
+}
+
+template <int NF, bool ANISO, int KMODE, int SP>
+__global__ __launch_bounds__(128) void k_forces_reg_aniso(ForceArgs A) {
+  forces_reg_body<NF, ANISO, KMODE, SP>(A);
+}
+
 template <int NF, bool ANISO, int KMODE, int SP>
 void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu) {
   int dev = 0, ncu = 0;
@@ -466,7 +483,8 @@ void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu) {
   const int all = (a.elem_count + 7) & ~7;
   int nb = blocks_per_cu > 0 ? std::min(all, (ncu * blocks_per_cu + 7) & ~7) : all;
   nb = std::max(8, nb);
-  hipLaunchKernelGGL((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  if constexpr (ANISO) hipLaunchKernelGGL((k_forces_reg_aniso<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else hipLaunchKernelGGL((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
 }
 
 template <int NF, bool ANISO, int KMODE, int SP>
