@@ -228,13 +228,19 @@ def main():
                                                          adj_ms, adj_n)}
         fwd_bytes = 12.0 * nloc + 40.0 * nirr + 36.0 * nglob
         fwd_asm_bytes = 0.0
+    # element slabs: every pass is split into nsl launches over equal element ranges
+    nsl = -(-mesh.nspec // tuning["slab_elems"]) if (fused and tuning.get("slab_elems")) else 1
+    if nsl > 1:
+        kinds = {k: (v[0] / nsl, v[1], v[2]) for k, v in kinds.items()}
+        fwd_bytes /= nsl
+        fwd_asm_bytes /= nsl
     kname = max(kinds, key=lambda k: kinds[k][1])
     kbytes, kms, kn = kinds[kname]
     avg_ms = kms / max(kn, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9
     traffic = None
     tf_path = os.path.join(HERE, "profiles", "c2_fwd_adj_hbm_traffic.md")
-    if os.path.exists(tf_path) and fused and a.config == "c2" and a.static_precision == 0:
+    if os.path.exists(tf_path) and fused and a.config == "c2" and a.static_precision == 0 and nsl == 1:
         for line in open(tf_path):
             if line.startswith("hbm_bytes_per_launch " + kname):
                 traffic = float(line.split("=")[1])
@@ -256,7 +262,8 @@ def main():
                                 f"{w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, NGLL=5, "
                                 f"{rec.nrec} receivers, dt={dt}, NSTEP={K}, fwd + waveform misfit + "
                                 "adjoint/reconstruction + rho/mu/kappa kernels (fp64 accum)"),
-                   "path": (f"brick-fused {'x'.join(map(str, brick))}" if brick else "fused")
+                   "path": (f"brick-fused {'x'.join(map(str, brick))}" if brick else
+                            (f"fused, {nsl} element slabs" if nsl > 1 else "fused"))
                            if fused else "SPECFEM stage order (strain stores)",
                    "element_pass_tuning": tuning if fused else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,

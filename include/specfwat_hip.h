@@ -156,6 +156,12 @@ void fwat_set_lattice_(long *Mesh_pointer, int *nex, int *ney, int *nez);
 void fwat_set_brick_(long *Mesh_pointer, int *bx, int *by, int *bz, int *blocks_per_cu);
 /* brick decomposition sizes: halo points, partial-sum slots per field and step, halo CSR length */
 void fwat_brick_info_(long *Mesh_pointer, int *nhalo, long *nslots, long *ncontrib);
+/* Element slabs of the whole-run loop: element pass + assembly run slab after slab over
+ * contiguous ranges of elems_per_slab elements (0: one pass over the mesh); each point is
+ * assembled right after the slab holding its last contributing element, so the element-local
+ * forces are read back while in the memory-side cache.  Results are bitwise those of the
+ * single pass (same per-point summation order).  No counterpart in the reference. */
+void fwat_set_slabs_(long *Mesh_pointer, int *elems_per_slab);
 /* Steps per fp32 partial-sum block of the element-pass kernels (mu/kappa, c_ij) in the fused
  * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
  * fp64).  The rho kernel is always accumulated per step in fp64. */

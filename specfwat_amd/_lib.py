@@ -59,6 +59,7 @@ SIGNATURES = {
     "fwat_set_brick_": "Liiii",
     "fwat_brick_info_": "Lill",
     "fwat_set_kernel_block_": "Li",
+    "fwat_set_slabs_": "Li",
     "fwat_batch_create_": "LLiIfFID",
     "fwat_batch_set_tuning_": "Li",
     "fwat_batch_forward_": "Li",

@@ -338,7 +338,8 @@ __device__ __forceinline__ void forces_reg_body(const ForceArgs& A) {
   const int x = blockIdx.x & 7, per_xcd = nblk >> 3, slot = blockIdx.x >> 3;
   const int chunk = (A.elem_count + 7) >> 3;
   const int e_end = min(A.elem_count, (x + 1) * chunk);
-  for (int e = x * chunk + slot; e < e_end; e += per_xcd) {
+  for (int le = x * chunk + slot; le < e_end; le += per_xcd) {
+    const int e = A.elem_begin + le;  // elements [elem_begin, elem_begin + elem_count)
     const size_t Lg = (size_t)e * 125 + tt;
     const int iglob = A.ibool[Lg];
     realw kl = 0, ml = 0;
@@ -536,7 +537,8 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   if (nfields != 3) a.eps0 = m->d_epsilon;
   if (gather) {
     if (!m->d_floc) SEM_FAIL("gather assembly buffers not allocated");
-    if (elem_list || elem_begin != 0 || elem_count != m->nspec) SEM_FAIL("gather assembly needs the whole mesh in one pass");
+    if (elem_list) SEM_FAIL("gather assembly takes a contiguous element range");
+    if (elem_begin < 0 || elem_begin + elem_count > m->nspec) SEM_FAIL("element range outside the mesh");
     a.floc0 = m->d_floc;
     a.floc1 = nfields == 2 ? m->d_floc_b : nullptr;
     if (nfields == 2 && !a.floc1) SEM_FAIL("second element-local force buffer not allocated");

@@ -515,6 +515,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   dev_free(m->d_brick_special); dev_free(m->d_brick_part0); dev_free(m->d_brick_part1);
   dev_free(m->d_halo_g); dev_free(m->d_halo_off); dev_free(m->d_halo_idx);
   dev_free(m->d_mu_part); dev_free(m->d_kappa_part); dev_free(m->d_cij_part); dev_free(m->d_rho_part);
+  dev_free(m->d_slab_g);
   dev_free(m->d_kappa); dev_free(m->d_mu); dev_free(m->d_rho); dev_free(m->d_cij); dev_free(m->d_rmass_inv);
   dev_free(m->d_phase_ispec);
   dev_free(m->d_displ); dev_free(m->d_veloc); dev_free(m->d_accel);

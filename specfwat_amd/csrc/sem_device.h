@@ -155,6 +155,15 @@ struct SemMesh {
   int* d_asm_idx = nullptr;     // [ncontrib]
   size_t n_contrib = 0;
 
+  // element slabs: the fused loops run element pass + assembly slab after slab
+  // (contiguous element ranges of slab_elems elements); a point is assembled
+  // right after the slab holding its last contributing element, while that
+  // slab's element-local forces are still in the memory-side cache
+  int slab_elems = 0;                // 0: one pass over the whole mesh
+  int nslabs = 0;
+  std::vector<int> h_slab_goff;      // [nslabs+1] offsets into d_slab_g
+  int* d_slab_g = nullptr;           // points of each slab, ascending
+
   // brick-fused pass on regular lattice meshes (element + assembly in one
   // kernel for brick-interior points; see brick.hip)
   int lat_nex = 0, lat_ney = 0, lat_nez = 0;  // validated element lattice, 0 = none

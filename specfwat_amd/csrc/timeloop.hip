@@ -83,14 +83,16 @@ __global__ void k_corrector(int nglob, realw* __restrict__ v, realw* __restrict_
 // TWO: the adjoint field (dt > 0) and the reconstructed field (dt < 0) are
 // updated together and the density kernel a_adj . u_rec is accumulated in fp64.
 
+// glist: the points of this launch (one slab's), or null for points 0..npts-1.
 template <bool LAST, bool TWO>
-__global__ void k_assemble(int nglob, const int* __restrict__ off, const int* __restrict__ idx,
-                           int nloc, int nfp, int nsp, const realw* __restrict__ sarr,
+__global__ void k_assemble(int npts, const int* __restrict__ glist, const int* __restrict__ off,
+                           const int* __restrict__ idx, int nloc, int nfp, int nsp, const realw* __restrict__ sarr,
                            const realw* __restrict__ rech, const realw* __restrict__ rm,
                            AsmField F0, AsmField F1, double* __restrict__ rho_kl,
                            float* __restrict__ rho_part, int flush, double* __restrict__ hess_kl, double dtk) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nglob) return;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= npts) return;
+  const int g = glist ? glist[h] : h;
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
   const int q1 = off[g + 1];
   for (int q = off[g]; q < q1; ++q) {
@@ -356,27 +358,32 @@ void stage_stacey_face(SemMesh* m, const realw* v, realw* out) {
                      m->d_face_coef, v, out);
 }
 
-void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const AsmField& F1, double dtk) {
+// Assembly + Newmark of points 0..nglob-1, or of slab `slab`'s point list.
+void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const AsmField& F1, double dtk,
+                    int slab = -1) {
   const int nfp = m->absorbing ? m->nfaces * 25 : 0;
   float* rho_part = m->kernel_block > 1 ? m->d_rho_part : nullptr;
   const int nloc = m->nspec * 125, nsp = m->nsources * 125;
-  const dim3 gr(nblk(m->nglob)), bl(TPB);
+  const int* glist = slab >= 0 ? m->d_slab_g + m->h_slab_goff[slab] : nullptr;
+  const int npts = slab >= 0 ? m->h_slab_goff[slab + 1] - m->h_slab_goff[slab] : m->nglob;
+  if (npts == 0) return;
+  const dim3 gr(nblk(npts)), bl(TPB);
   if (two) {
     if (last)
-      hipLaunchKernelGGL((k_assemble<true, true>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      hipLaunchKernelGGL((k_assemble<true, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, rho_part, m->kernel_flush,
                          m->d_hess_kl_glob, dtk);
     else
-      hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, rho_part, m->kernel_flush,
                          m->d_hess_kl_glob, dtk);
   } else {
     if (last)
-      hipLaunchKernelGGL((k_assemble<true, false>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      hipLaunchKernelGGL((k_assemble<true, false>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (float*)nullptr, 1,
                          (double*)nullptr, dtk);
     else
-      hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, m->stream, m->nglob, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (float*)nullptr, 1,
                          (double*)nullptr, dtk);
   }
@@ -388,6 +395,10 @@ void stage_seismo(SemMesh* m, int it0) {
   hipLaunchKernelGGL(k_seismo, dim3(m->nrec), dim3(128), 0, m->stream, m->nrec, m->d_rec_iglob,
                      m->d_rec_h, m->d_displ, m->d_seismograms + (size_t)it0 * m->nrec * 3);
 }
+
+// With element slabs a step has 2 x nslabs launches; timing samples every
+// timer_stride(N)-th step (about 256 steps) instead of all of them.
+inline int timer_stride(int nstep) { return std::max(1, nstep / 256); }
 
 // Force launches bracketed by HIP events on the simulation stream; the
 // elapsed times are summed after the run, so timing adds no host sync.
@@ -668,8 +679,10 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   ForceTimer* tm = nullptr;
   ForceTimer* ta = nullptr;
   if (*timed) {
-    timer.begin((size_t)N * 2);
-    atimer.begin((size_t)N);
+    const size_t per_step = m->nslabs > 0 ? (size_t)m->nslabs : 2;  // timed launches per step
+    const size_t steps = m->nslabs > 0 ? (size_t)(N + timer_stride(N) - 1) / timer_stride(N) : (size_t)N;
+    timer.begin(steps * per_step);
+    atimer.begin(steps * std::max<size_t>(per_step, 1));
     tm = &timer;
     ta = &atimer;
   }
@@ -696,6 +709,32 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         if (ta) ta->mark(s);
         sem_launch_halo(m, 1, F0, F0, it == N, 0.0);
         if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+      }
+    } else if (fz && m->nslabs > 0) {
+      // slab order: Stacey terms and seismograms read the predicted state only,
+      // so they come first; then element pass + assembly slab after slab
+      const int nfp = m->nfaces * 25;
+      const int ts = timer_stride(N);
+      stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
+      for (int it = 1; it <= N; ++it) {
+        realw* face = nullptr;
+        if (m->absorbing && m->nfaces) {
+          face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
+          stage_stacey_face(m, m->d_veloc, face);
+        }
+        stage_seismo(m, it - 1);
+        AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
+                    nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
+        const bool sample = (it - 1) % ts == 0;
+        for (int sl = 0; sl < m->nslabs; ++sl) {
+          const int e0 = sl * m->slab_elems, ne = std::min(m->slab_elems, m->nspec - e0);
+          if (tm && sample) tm->mark(s);
+          sem_launch_forces(m, 1, 0, e0, ne, dt, nullptr, true);
+          if (tm && sample) { tm->mark(s); m->last_force_launches += 1; }
+          if (ta && sample) ta->mark(s);
+          stage_assemble(m, it == N, false, F0, F0, 0.0, sl);
+          if (ta && sample) { ta->mark(s); m->last_asm_launches += 1; }
+        }
       }
     } else if (fz) {
       const int nfp = m->nfaces * 25;
@@ -772,6 +811,34 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         if (ta) ta->mark(s);
         sem_launch_halo(m, 2, F0, F1, it == N, dtk);
         if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+      }
+    } else if (fz && m->nslabs > 0) {
+      const int nfp = m->nfaces * 25;
+      const int ts = timer_stride(N);
+      stage_predictor(m, false, dt);  // adjoint field (from rest)
+      for (int it = 1; it <= N; ++it) {
+        const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
+        m->kernel_flush = closes_block(it);
+        realw* face_a = nullptr;
+        if (m->absorbing && m->nfaces) {
+          face_a = m->d_face_scratch;
+          stage_stacey_face(m, m->d_veloc, face_a);
+        }
+        const realw* face_b = (m->absorbing && m->nfaces) ? m->d_stacey_store + (size_t)rec0 * nfp * 3 : nullptr;
+        AsmField F0{m->d_floc, face_a, nullptr, m->nrec ? m->d_adj + (size_t)rec0 * m->nrec * 3 : nullptr,
+                    m->d_displ, m->d_veloc, m->d_accel, dt};
+        AsmField F1{m->d_floc_b, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
+                    nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
+        const bool sample = (it - 1) % ts == 0;
+        for (int sl = 0; sl < m->nslabs; ++sl) {
+          const int e0 = sl * m->slab_elems, ne = std::min(m->slab_elems, m->nspec - e0);
+          if (tm && sample) tm->mark(s);
+          sem_launch_forces(m, 2, 2, e0, ne, dt, nullptr, true);
+          if (tm && sample) { tm->mark(s); m->last_force_launches += 1; }
+          if (ta && sample) ta->mark(s);
+          stage_assemble(m, it == N, true, F0, F1, dtk, sl);
+          if (ta && sample) { ta->mark(s); m->last_asm_launches += 1; }
+        }
       }
     } else if (fz) {
       const int nfp = m->nfaces * 25;
@@ -873,6 +940,43 @@ void fwat_set_tuning_(long* Mesh_pointer, int* regular_path, int* blocks_per_cu_
   m->regular_fast_path = *regular_path != 0;
   m->blocks_per_cu_fwd = *blocks_per_cu_fwd;
   m->blocks_per_cu_adj = *blocks_per_cu_adj;
+}
+
+// Element slabs of the fused loops: contiguous ranges of elems_per_slab
+// elements (0: off).  Each point is assembled right after the slab holding
+// its last contributing element, so the slab's element-local forces are read
+// back while still in the memory-side cache.  Valid for any mesh; worth it
+// when a point's elements are close in element order (lattice-ordered
+// meshes).  Irregular meshes, anisotropy and the per-step entry points are
+// unaffected by the ordering.
+void fwat_set_slabs_(long* Mesh_pointer, int* elems_per_slab) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  dev_free(m->d_slab_g);
+  m->h_slab_goff.clear();
+  m->nslabs = 0;
+  m->slab_elems = 0;
+  const int C = *elems_per_slab;
+  if (C <= 0 || C >= m->nspec) return;
+  const size_t nloc = (size_t)m->nspec * 125;
+  std::vector<int> ib(nloc);
+  HIP_CHECK(hipMemcpyAsync(ib.data(), m->d_ibool, nloc * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+  std::vector<int> last(m->nglob, -1);
+  for (size_t L = 0; L < nloc; ++L) last[ib[L]] = std::max(last[ib[L]], (int)(L / 125));
+  const int ns = (m->nspec + C - 1) / C;
+  std::vector<int> off(ns + 1, 0), pts(m->nglob);
+  for (int g = 0; g < m->nglob; ++g) {
+    if (last[g] < 0) SEM_FAIL("fwat_set_slabs_: point %d belongs to no element", g + 1);
+    off[last[g] / C + 1]++;
+  }
+  for (int q = 0; q < ns; ++q) off[q + 1] += off[q];
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int g = 0; g < m->nglob; ++g) pts[fill[last[g] / C]++] = g;  // ascending within a slab
+  m->d_slab_g = dev_upload(pts.data(), pts.size(), m->stream);
+  m->h_slab_goff = off;
+  m->nslabs = ns;
+  m->slab_elems = C;
+  HIP_CHECK(hipStreamSynchronize(m->stream));
 }
 
 // Steps per fp32 partial-sum block of the element-pass kernels (mu/kappa or

@@ -101,10 +101,12 @@ class ElasticRun:
             _lib.call("fwat_set_stf_", self.h, self.stf, self.nstep)
 
     def set_tuning(self, regular_path=1, blocks_per_cu_fwd=0, blocks_per_cu_adj=8, brick=(0, 0, 0),
-                   brick_blocks_per_cu=0):
+                   brick_blocks_per_cu=0, slab_elems=0):
         """Element-pass launch shape; brick=(bx, by, bz) switches the brick-fused
-        element + assembly pass on (lattice meshes of regular elements only)."""
+        element + assembly pass on (lattice meshes of regular elements only);
+        slab_elems > 0 runs element pass + assembly slab after slab."""
         _lib.call("fwat_set_tuning_", self.h, int(regular_path), int(blocks_per_cu_fwd), int(blocks_per_cu_adj))
+        _lib.call("fwat_set_slabs_", self.h, int(slab_elems))
         if brick[0] and not self.lattice:
             raise ValueError("the brick pass needs a lattice mesh of regular elements")
         if brick[0] or self.lattice:

@@ -14,10 +14,11 @@ from .solver import ElasticRun
 
 CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32)]
 BRICKS = [(2, 2, 2), (4, 2, 2), (4, 4, 2)]
+SLABS = [6, 12, 24]  # slabs per mesh
 
 
 def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidates=CANDIDATES,
-                        static_precision=0, bricks=BRICKS):
+                        static_precision=0, bricks=BRICKS, slabs=SLABS):
     """Returns ({'regular_path', 'blocks_per_cu_fwd', 'blocks_per_cu_adj'[, 'brick']}, table);
     the dict is ElasticRun.set_tuning's keyword arguments."""
     rho, vp, vs = model
@@ -68,6 +69,28 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
                 top = (tf + ta, br)
         if top[1] is not None:
             best = dict(best, brick=top[1])
+    # element slabs (any mesh): element pass + assembly slab after slab
+    if slabs and "brick" not in best:
+        def step_ms_s(nsl, **kw):
+            # element + assembly time per step; with slabs every step has nsl launches of each
+            f.set_tuning(**kw)
+            a.set_tuning(**kw)
+            f.iterate_time(timed=True)
+            fe, fa = f.force_time(), f.assembly_time()
+            a.iterate_time(timed=True)
+            ae, aa = a.force_time(), a.assembly_time()
+            return ((fe[0] + fa[0]) * nsl / max(fe[1], 1), (ae[0] + aa[0]) * nsl / max(ae[1], 1))
+        base = step_ms_s(1, **best)
+        table.append(("whole-mesh", 0, base[0], base[1]))
+        top = (base[0] + base[1], 0)
+        for nsl in slabs:
+            c = -(-mesh.nspec // nsl)
+            tf, ta = step_ms_s(-(-mesh.nspec // c), **best, slab_elems=c)
+            table.append((f"slabs{nsl}", c, tf, ta))
+            if tf + ta < top[0]:
+                top = (tf + ta, c)
+        if top[1]:
+            best = dict(best, slab_elems=top[1])
     f.close()
     a.close()
     return best, table
