@@ -48,6 +48,7 @@ class ElasticRun:
                  save_forward=False, cij=None, stacey=True, compute_and_store_strain=False,
                  approximate_hess=False, static_precision=0):
         self.mesh, self.dt, self.nstep = mesh, float(dt), int(nstep)
+        self.stacey = bool(stacey)
         self.simulation_type = simulation_type
         self.aniso = cij is not None
         self.nrec = receivers.nrec if receivers is not None else 0
@@ -163,7 +164,7 @@ class ElasticRun:
     def take_forward(self, fwd):
         """In-HBM hand-off: forward final state -> b_ fields, Stacey store moved."""
         _lib.call("fwat_copy_forward_to_backward_", fwd.h, self.h)
-        if self.mesh.num_abs_boundary_faces:
+        if self.stacey and self.mesh.num_abs_boundary_faces:
             _lib.call("fwat_move_stacey_store_", fwd.h, self.h)
 
     def kernels(self):
