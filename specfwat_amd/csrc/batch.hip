@@ -277,7 +277,26 @@ __global__ void k_batch_asm(BatchAsmArgs A) {
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
   const int nsp = E * 125;
   const int q1 = A.off[g + 1];
-  for (int k = A.off[g]; k < q1; ++k) {
+  int k = A.off[g];
+  // pairs of element-local contributions: both rows loaded before either is
+  // added (two loads in flight per lane), same summation order
+  for (; k + 1 < q1; k += 2) {
+    const int c0 = A.idx[k], c1 = A.idx[k + 1];
+    if (c1 >= A.nloc) break;  // c0 < c1
+    const realw* P0 = A.floc0 + ((size_t)c0 * E + ev) * 3;
+    const realw* P1 = A.floc0 + ((size_t)c1 * E + ev) * 3;
+    const realw a0 = P0[0], a1 = P0[1], a2 = P0[2], b0 = P1[0], b1 = P1[1], b2 = P1[2];
+    if (TWO) {
+      const realw* B0 = A.floc1 + ((size_t)c0 * E + ev) * 3;
+      const realw* B1 = A.floc1 + ((size_t)c1 * E + ev) * 3;
+      const realw d0 = B0[0], d1 = B0[1], d2 = B0[2], e0 = B1[0], e1 = B1[1], e2 = B1[2];
+      f1x += d0; f1y += d1; f1z += d2;
+      f1x += e0; f1y += e1; f1z += e2;
+    }
+    f0x += a0; f0y += a1; f0z += a2;
+    f0x += b0; f0y += b1; f0z += b2;
+  }
+  for (; k < q1; ++k) {
     const int c = A.idx[k];
     if (c < A.nloc) {
       const realw* P = A.floc0 + ((size_t)c * E + ev) * 3;

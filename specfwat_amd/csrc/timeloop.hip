@@ -95,7 +95,26 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
   const int g = glist ? glist[h] : h;
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
   const int q1 = off[g + 1];
-  for (int q = off[g]; q < q1; ++q) {
+  int q = off[g];
+  // pairs of element-local contributions (most of them): both rows are loaded
+  // before either is added, two loads in flight per lane; same summation order
+  for (; q + 1 < q1; q += 2) {
+    const int c0 = idx[q], c1 = idx[q + 1];
+    if (c1 >= nloc) break;  // c0 < c1: both element-local below this line
+    const realw* P0 = F0.floc + 3 * (size_t)c0;
+    const realw* P1 = F0.floc + 3 * (size_t)c1;
+    const realw a0 = P0[0], a1 = P0[1], a2 = P0[2], b0 = P1[0], b1 = P1[1], b2 = P1[2];
+    if (TWO) {
+      const realw* B0 = F1.floc + 3 * (size_t)c0;
+      const realw* B1 = F1.floc + 3 * (size_t)c1;
+      const realw d0 = B0[0], d1 = B0[1], d2 = B0[2], e0 = B1[0], e1 = B1[1], e2 = B1[2];
+      f1x += d0; f1y += d1; f1z += d2;
+      f1x += e0; f1y += e1; f1z += e2;
+    }
+    f0x += a0; f0y += a1; f0z += a2;
+    f0x += b0; f0y += b1; f0z += b2;
+  }
+  for (; q < q1; ++q) {
     const int c = idx[q];
     if (c < nloc) {
       const realw* P = F0.floc + 3 * (size_t)c;
