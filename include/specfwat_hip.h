@@ -225,6 +225,63 @@ void fwat_smooth_pde_batch_(int *h_is_sph, int *h_nspec, int *h_nglob, int *h_ns
 /* element-pass and gather/update time of the last timed fwat_smooth_pde_batch_ (ms) */
 void fwat_smooth_last_time_(double *elem_ms, double *update_ms);
 
+/* ---- kernel post-processing (xfwat_post_proc, src/postproc/fwat_post_proc.f90:22-47).
+ *      A post handle holds one data type's fp64 kernel sums and Hessian sum in HBM
+ *      (PostFlow, src/postproc/post_processing.f90:20-32).  model_type 1: alpha, beta, rhop
+ *      (KERNEL_ISO, src/shared/config.f90:35); 2: alpha, beta, rhop, gcp, gsp
+ *      (KERNEL_AZI_ANI, config.f90:36).  use_rho_scaling leaves rhop out of the sums
+ *      (post_processing.f90:131). */
+void fwat_post_create_(long *Post_pointer, int *nspec, int *model_type, int *use_rho_scaling);
+/* model of kernel_cijkl2hti (src/postproc/kernel_io.f90:107-118), each (NGLLX,NGLLY,NGLLZ,NSPEC) */
+void fwat_post_set_model_(long *Post_pointer, realw *c44store, realw *c45store, realw *c55store,
+                          realw *rho_vp, realw *rho_vs, realw *rhostore);
+/* sum_kernel's per-event step (post_processing.f90:108-149) fed straight from a finished
+ * adjoint run's device kernels: save_adjoint_kernels scaling + alpha/beta/rhop, or
+ * kernel_cijkl2hti (kernel_io.f90:100-156); with precond_type 0/1 also the Hessian sum of
+ * apply_precond (post_processing.f90:171-183; 1 = absolute values) */
+void fwat_post_add_event_(long *Post_pointer, long *Mesh_pointer, int *precond_type);
+/* the same from one event's EKERNEL files as read by read_event_kernel (kernel_io.f90:76-98):
+ * h_ker (NGLLX,NGLLY,NGLLZ,NSPEC,3) alpha, beta, rhop or (...,11) rho, c11, c12, c13, c22, c23,
+ * c33, c44, c45, c55, c66; h_hess (NGLLX,NGLLY,NGLLZ,NSPEC) or NULL */
+void fwat_post_add_event_files_(long *Post_pointer, realw *h_ker, realw *h_hess, int *precond_type);
+/* h_ker (NGLLX,NGLLY,NGLLZ,NSPEC,nker) fp64 */
+void fwat_post_get_kernels_(long *Post_pointer, double *h_ker);
+void fwat_post_get_hess_(long *Post_pointer, double *h_hess);
+/* replaces the Hessian sum (run_mode 2 reads a stored one, post_processing.f90:184-186) */
+void fwat_post_set_hess_(long *Post_pointer, double *h_hess);
+/* invert_hess (post_processing.f90:344-377) on the Hessian sum, in place; precond_type 0
+ * takes |sum| first (:181) */
+void fwat_post_invert_hess_(long *Post_pointer, int *precond_type);
+/* kernels *= inverted Hessian (apply_precond, post_processing.f90:189-191) */
+void fwat_post_apply_precond_(long *Post_pointer);
+void fwat_post_info_(long *Post_pointer, int *nker, int *nevents, int *nhess);
+void fwat_post_destroy_(long *Post_pointer);
+
+/* ---- gll2grid (src/grid/model_grid_data.f90:192-211): point location and interpolation
+ *      coefficients of compute_interpolation_coeff_FD_SEM
+ *      (src/shared/projection_on_FD_grid_mod_fwat.f90:186-506, locate_point_in_element
+ *      :935-1125) for the grid ox + (i-1) hx (create_grid, model_grid_data.f90:18-53);
+ *      ibool 1-based (NGLLX,NGLLY,NGLLZ,NSPEC); returns nb_fd_point */
+void fwat_gll2grid_prepare_(long *Grid_pointer, int *nspec, int *nglob, int *ibool, realw *xstore,
+                            realw *ystore, realw *zstore, realw *ox, realw *oy, realw *oz, realw *hx,
+                            realw *hy, realw *hz, int *nx, int *ny, int *nz, int *num_iter,
+                            long *nb_fd_point);
+/* Project_model_SEM2FD_grid (projection_on_FD_grid_mod_fwat.f90:512-596) of nfield fields:
+ * gll_data (NGLLX,NGLLY,NGLLZ,NSPEC,nfield) -> grid_data (nx,ny,nz,nfield) */
+void fwat_gll2grid_(long *Grid_pointer, int *nfield, realw *gll_data, realw *grid_data);
+/* owner(nx,ny,nz): 1-based element of each grid point (0 = none); ref(3,nx,ny,nz) xi,eta,gamma */
+void fwat_gll2grid_owner_(long *Grid_pointer, int *owner, double *ref);
+void fwat_gll2grid_last_time_(long *Grid_pointer, double *ms);
+void fwat_gll2grid_destroy_(long *Grid_pointer);
+/* taper_kernel_grid (post_processing.f90:296-333): taper_cls over the mesh box with spacing
+ * (distance_max_glob + distance_min_glob)/2 (src/postproc/taper_3D_mod.f90:16-141), sampled at
+ * every grid point; data (nx,ny,nz,nfield) is multiplied in place; mask (nx,ny,nz) or NULL */
+void fwat_taper_grid_(realw *x_min_glob, realw *x_max_glob, realw *y_min_glob, realw *y_max_glob,
+                      realw *z_min_glob, realw *z_max_glob, realw *distance_min_glob,
+                      realw *distance_max_glob, realw *h_suppress, realw *h_buffer, realw *v_suppress,
+                      realw *v_buffer, realw *ox, realw *oy, realw *oz, realw *hx, realw *hy, realw *hz,
+                      int *nx, int *ny, int *nz, int *nfield, realw *data, realw *mask);
+
 #ifdef __cplusplus
 }
 #endif

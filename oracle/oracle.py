@@ -10,6 +10,12 @@ Function-to-reference map (file:line in the SpecFWAT tree):
                                    source absent: restated, pinned by invariants)
   smooth_pde                       src/postproc/smooth.f90:7-437, 603-689
   interp3                          src/shared/utils.f90:1162-1232
+  cijkl2hti                        src/postproc/kernel_io.f90:100-156
+  invert_hess                      src/postproc/post_processing.f90:344-377
+  gll2grid_coeff / gll2grid_project
+                                   src/shared/projection_on_FD_grid_mod_fwat.f90:186-596, 935-1125
+  taper_grid                       src/postproc/post_processing.f90:296-333,
+                                   src/postproc/taper_3D_mod.f90:16-141
   hti_cij                          src/patch/aniso.f90:55-162
   simpson                          src/shared/utils.f90:1630-1663
   window_taper                     src/shared/signal.f90:419-491
@@ -51,11 +57,12 @@ class OrcParams(C.Structure):
 
 def build(force=False):
     """Compile the oracle with g++ (OpenMP, portable x86-64-v3 code)."""
-    src = os.path.join(_HERE, "sem_oracle.cpp")
-    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+    srcs = [os.path.join(_HERE, "sem_oracle.cpp"), os.path.join(_HERE, "post_oracle.cpp")]
+    if force or not os.path.exists(_LIB_PATH) or any(os.path.getmtime(_LIB_PATH) < os.path.getmtime(s)
+                                                      for s in srcs):
         import subprocess
         cmd = ["g++", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
-               "-o", _LIB_PATH, src]
+               "-o", _LIB_PATH] + srcs
         subprocess.check_call(cmd)
     return _LIB_PATH
 
@@ -104,6 +111,14 @@ def lib():
                                      + [_fp] * 10 + [_ip, C.c_float, C.c_float, C.c_int, C.c_int,
                                                      C.c_int, _ip, _ip, _fp, C.c_float, C.c_float,
                                                      C.c_float, C.c_int, C.c_int, C.c_int, _fp, _fp])
+        L.orc_cijkl2hti.argtypes = [C.c_long] + [_fp] * 8
+        L.orc_invert_hess.argtypes = [C.c_long, _fp]
+        L.orc_gll2grid_coeff.restype = C.c_long
+        L.orc_gll2grid_coeff.argtypes = ([C.c_int, _ip, _fp, _fp, _fp] + [C.c_float] * 6 + [C.c_int] * 4
+                                         + [_ip, _dp])
+        L.orc_gll2grid_project.argtypes = [C.c_long, _ip, _dp, _fp, _fp]
+        L.orc_taper_grid.restype = C.c_int
+        L.orc_taper_grid.argtypes = [C.c_float] * 11 + [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int, _fp]
         _lib = L
     return _lib
 
@@ -354,3 +369,57 @@ def smooth_pde(mesh, dat_in, sigma_h, sigma_v, distance_min_glob, *, is_sph=Fals
         f(dat_in), float(sigma_h), float(sigma_v), float(distance_min_glob), int(is_sph),
         int(faithful), int(max_steps), _p(dat, C.c_float), _p(dglob, C.c_float))
     return dat, dglob, n
+
+
+def cijkl2hti(c44, c45, c55, rho_vp, rho_vs, rho, k11):
+    """One event's HTI kernels [5][n] (alpha, beta, rho, gcp, gsp) from the
+    rho + 10 c_ij kernels k11 [11][n] (rho, c11, c12, c13, c22, c23, c33, c44,
+    c45, c55, c66); single precision like the reference."""
+    f = [np.ascontiguousarray(a, np.float32).ravel() for a in (c44, c45, c55, rho_vp, rho_vs, rho)]
+    n = f[0].size
+    k = np.ascontiguousarray(k11, np.float32).reshape(11, n)
+    out = np.zeros((5, n), np.float32)
+    lib().orc_cijkl2hti(n, *[_p(a, C.c_float) for a in f], _p(k, C.c_float), _p(out, C.c_float))
+    return out
+
+
+def invert_hess(h):
+    h = np.array(h, np.float32).ravel()
+    lib().orc_invert_hess(h.size, _p(h, C.c_float))
+    return h
+
+
+def gll2grid_coeff(ibool, xstore, ystore, zstore, grid, num_iter=5):
+    """(owner [npt] 1-based or 0, ref [npt,3], nb_fd_point); grid = (ox, oy, oz,
+    hx, hy, hz, nx, ny, nz); points in Fortran (nx, ny, nz) order."""
+    ox, oy, oz, hx, hy, hz, nx, ny, nz = grid
+    ib = np.ascontiguousarray(ibool, np.int32).ravel()
+    xs, ys, zs = (np.ascontiguousarray(a, np.float32) for a in (xstore, ystore, zstore))
+    npt = nx * ny * nz
+    owner = np.zeros(npt, np.int32)
+    ref = np.zeros((npt, 3), np.float64)
+    nf = lib().orc_gll2grid_coeff(ib.size // 125, _p(ib, C.c_int), _p(xs, C.c_float), _p(ys, C.c_float),
+                                  _p(zs, C.c_float), ox, oy, oz, hx, hy, hz, nx, ny, nz, num_iter,
+                                  _p(owner, C.c_int), _p(ref, C.c_double))
+    return owner, ref, int(nf)
+
+
+def gll2grid_project(owner, ref, model):
+    model = np.ascontiguousarray(model, np.float32).ravel()
+    out = np.zeros(owner.size, np.float32)
+    lib().orc_gll2grid_project(owner.size, _p(owner, C.c_int), _p(ref, C.c_double), _p(model, C.c_float),
+                               _p(out, C.c_float))
+    return out
+
+
+def taper_grid(box, dh, h_suppress, h_buffer, v_suppress, v_buffer, gx, gy, gz):
+    """Taper mask at the grid points, Fortran (nx, ny, nz) order; box =
+    (xmin, xmax, ymin, ymax, zmin, zmax)."""
+    gx, gy, gz = (np.ascontiguousarray(a, np.float32) for a in (gx, gy, gz))
+    out = np.zeros(gx.size * gy.size * gz.size, np.float32)
+    rc = lib().orc_taper_grid(*[float(v) for v in box], float(dh), float(h_suppress), float(h_buffer),
+                              float(v_suppress), float(v_buffer), _p(gx, C.c_float), gx.size,
+                              _p(gy, C.c_float), gy.size, _p(gz, C.c_float), gz.size, _p(out, C.c_float))
+    if rc != 0:
+        raise ValueError("taper ranges do not fit the taper grid")
+    return out

@@ -73,6 +73,23 @@ SIGNATURES = {
     "fwat_smooth_set_faithful_": "i",
     "fwat_smooth_pde_batch_": "iiiii" + "F" * 10 + "F" + "FFFFF" + "I" + "ff" + "FF" + "Ifii",
     "fwat_smooth_last_time_": "DD",
+    "fwat_post_create_": "Liii",
+    "fwat_post_set_model_": "LFFFFFF",
+    "fwat_post_add_event_": "LLi",
+    "fwat_post_add_event_files_": "LFFi",
+    "fwat_post_get_kernels_": "LD",
+    "fwat_post_get_hess_": "LD",
+    "fwat_post_set_hess_": "LD",
+    "fwat_post_invert_hess_": "Li",
+    "fwat_post_apply_precond_": "L",
+    "fwat_post_info_": "Liii",
+    "fwat_post_destroy_": "L",
+    "fwat_gll2grid_prepare_": "LiiIFFF" + "f" * 6 + "iiii" + "l",
+    "fwat_gll2grid_": "LiFF",
+    "fwat_gll2grid_owner_": "LID",
+    "fwat_gll2grid_last_time_": "Ld",
+    "fwat_gll2grid_destroy_": "L",
+    "fwat_taper_grid_": "f" * 18 + "iiii" + "FF",
 }
 
 _lib = None

@@ -444,7 +444,7 @@ struct ForceTimer {
     return t;
   }
   ~ForceTimer() {
-    for (auto e : ev) hipEventDestroy(e);
+    for (auto e : ev) (void)hipEventDestroy(e);
   }
 };
 

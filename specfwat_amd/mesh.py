@@ -210,6 +210,18 @@ class BoxMesh:
             best = min(best, d.min())
         return float(best)
 
+    def distance_max_glob(self):
+        """Maximum distance between neighbouring GLL points of an element
+        (check_mesh_distances' distance_max_glob, used by taper_kernel_grid)."""
+        i = np.arange(125)
+        x = self.xyz[self.ibool0]
+        best = 0.0
+        for stride, cond in ((1, i % 5 < 4), (5, (i // 5) % 5 < 4), (25, i // 25 < 4)):
+            src = i[cond]
+            d = np.linalg.norm(x[:, src + stride] - x[:, src], axis=-1)
+            best = max(best, d.max())
+        return float(best)
+
     def local_weights(self):
         """jacobian * w_i w_j w_k per local point (float64, [nspec,125])."""
         w = self.gll.w

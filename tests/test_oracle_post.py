@@ -1,0 +1,204 @@
+"""Pins of the post-processing oracle (oracle/post_oracle.cpp) by mathematics
+the reference fixes, plus the host side of the kernel files.
+
+  cijkl2hti      the chain rule: each HTI kernel is sum_ij K_ij dc_ij/dp for the
+                 HTI parameterisation of aniso.f90 (itself pinned bit-exact in
+                 test_oracle_leaves.py); the derivatives come from central
+                 differences of that parameterisation, exact here because every
+                 c_ij is quadratic in vp, vs and linear in rho, gcp, gsp
+  invert_hess    hand-computed cases (threshold branch, zero Hessian)
+  gll2grid       exact interpolation of fields the degree-4 Lagrange basis holds
+                 exactly (polynomials on a box, linear fields on a deformed mesh
+                 evaluated at the owner element's clamped location), the
+                 first-element-claims rule, points outside the mesh
+  taper_grid     the Hann half-window values at the taper-grid nodes
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+from specfwat_amd.mesh import BoxMesh, _shape  # noqa: E402
+from specfwat_amd import postproc  # noqa: E402
+
+VOIGT = ["11", "12", "13", "14", "15", "16", "22", "23", "24", "25", "26", "33", "34", "35", "36",
+         "44", "45", "46", "55", "56", "66"]
+HTI10 = ["11", "12", "13", "22", "23", "33", "44", "45", "55", "66"]
+
+
+def _hti_points(n, seed):
+    rng = np.random.default_rng(seed)
+    vp = rng.uniform(5000, 7000, n)
+    vs = vp / rng.uniform(1.6, 1.9, n)
+    rho = rng.uniform(2400, 3100, n)
+    gcp = rng.uniform(-0.08, 0.08, n)
+    gsp = rng.uniform(-0.08, 0.08, n)
+    return vp, vs, rho, gcp, gsp
+
+
+def test_cijkl2hti_is_the_chain_rule_of_the_hti_parameterisation():
+    n = 96
+    vp, vs, rho, gcp, gsp = _hti_points(n, 3)
+    c = np.array([O.hti_cij(*p) for p in zip(vp, vs, rho, gcp, gsp)], np.float64)  # (n, 21)
+    rng = np.random.default_rng(4)
+    K = rng.standard_normal((n, 21))  # every c_ij kernel, also the ones HTI leaves constant
+    Krho = rng.standard_normal(n)
+    # central differences (exact for quadratic / linear dependence)
+    def dc(which, h, rel):
+        args_p = [vp.copy(), vs.copy(), rho.copy(), gcp.copy(), gsp.copy()]
+        args_m = [vp.copy(), vs.copy(), rho.copy(), gcp.copy(), gsp.copy()]
+        if rel:
+            args_p[which] *= 1 + h
+            args_m[which] *= 1 - h
+            den = 2 * h
+        else:
+            args_p[which] += h
+            args_m[which] -= h
+            den = 2 * h
+        cp = np.array([O.hti_cij(*p) for p in zip(*args_p)], np.float64)
+        cm = np.array([O.hti_cij(*p) for p in zip(*args_m)], np.float64)
+        return (cp - cm) / den
+    J = [dc(0, 0.1, True), dc(1, 0.1, True), dc(2, 0.1, True), dc(3, 0.05, False), dc(4, 0.05, False)]
+    expect = [np.sum(K * J[0], 1), np.sum(K * J[1], 1), rho * Krho + np.sum(K * J[2], 1),
+              np.sum(K * J[3], 1), np.sum(K * J[4], 1)]
+    q = {v: i for i, v in enumerate(VOIGT)}
+    k11 = np.stack([Krho] + [K[:, q[v]] for v in HTI10])
+    got = O.cijkl2hti(c[:, q["44"]], c[:, q["45"]], c[:, q["55"]], rho * vp, rho * vs, rho, k11)
+    for i in range(5):
+        err = np.linalg.norm(got[i] - expect[i]) / np.linalg.norm(expect[i])
+        assert err < 2e-5, (postproc.KERNEL_AZI_ANI[i], err)
+    # the c_ij HTI leaves constant carry no sensitivity to any parameter
+    other = [q[v] for v in VOIGT if v not in HTI10]
+    assert all(np.abs(Jp[:, other]).max() == 0.0 for Jp in J)
+
+
+def test_invert_hess_hand_cases():
+    h = O.invert_hess([0.0, 1e-4, 0.5, 1.0])
+    np.testing.assert_allclose(h, [1.0, 1.0, 0.002, 0.001], rtol=1e-6)
+    h = O.invert_hess([-4.0, 2.0, 0.0])  # normalised: -1, .5, 0 -> -1, 2, 1000 -> / 1000
+    np.testing.assert_allclose(h, [-0.001, 0.002, 1.0], rtol=1e-6)
+    assert np.all(O.invert_hess(np.zeros(7)) == 1.0)
+    np.testing.assert_allclose(O.invert_hess(np.full(5, 3.5)), 1.0, rtol=1e-7)
+
+
+def _grid_points(grid):
+    ox, oy, oz, hx, hy, hz, nx, ny, nz = grid
+    f = np.float32
+    gx = f(ox) + np.arange(nx, dtype=f) * f(hx)
+    gy = f(oy) + np.arange(ny, dtype=f) * f(hy)
+    gz = f(oz) + np.arange(nz, dtype=f) * f(hz)
+    X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+    return np.stack([X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")], 1).astype(np.float64)
+
+
+def _model(mesh, f):
+    xyz = np.stack([mesh.xstore, mesh.ystore, mesh.zstore], 1).astype(np.float64)
+    return f(xyz[mesh.ibool0]).astype(np.float32)
+
+
+def test_gll2grid_is_exact_for_polynomials_on_a_box():
+    m = BoxMesh(4, 3, 2, 4000.0, 3000.0, 2000.0)
+    grid = (0.0, 0.0, -2000.0, 250.0, 250.0, 250.0, 17, 13, 9)
+    owner, ref, nf = O.gll2grid_coeff(m.ibool, m.xstore, m.ystore, m.zstore, grid)
+    assert nf == owner.size and np.all(owner > 0)
+    f = lambda q: (1.0 + (q[..., 0] / 4000) ** 4 - 2 * (q[..., 1] / 3000) ** 3 * (q[..., 2] / 2000)
+                   + (q[..., 0] / 4000) * (q[..., 1] / 3000) ** 2 * (q[..., 2] / 2000) ** 4)
+    out = O.gll2grid_project(owner, ref, _model(m, f))
+    np.testing.assert_allclose(out, f(_grid_points(grid)), atol=2e-6, rtol=0)
+    # points on faces shared by elements belong to the lowest element index
+    P = _grid_points(grid)
+    ex = np.minimum(np.floor(P[:, 0] / 1000 + 1e-9), 3).astype(int)
+    ey = np.minimum(np.floor(P[:, 1] / 1000 + 1e-9), 2).astype(int)
+    ez = np.minimum(np.floor((P[:, 2] + 2000) / 1000 + 1e-9), 1).astype(int)
+    low = lambda e, v, h: np.where((np.abs(v / h - np.round(v / h)) < 1e-9) & (e > 0) & (np.round(v / h) == e),
+                                   e - 1, e)
+    ex, ey, ez = low(ex, P[:, 0], 1000.0), low(ey, P[:, 1], 1000.0), low(ez, P[:, 2] + 2000, 1000.0)
+    np.testing.assert_array_equal(owner - 1, ex + 4 * (ey + 3 * ez))
+
+
+def test_gll2grid_deformed_mesh_takes_the_first_element_whose_index_box_covers_the_point():
+    m = BoxMesh(4, 3, 2, 4000.0, 3000.0, 2000.0, perturb=0.15)
+    grid = (0.0, 0.0, -2000.0, 250.0, 250.0, 250.0, 17, 13, 9)
+    owner, ref, nf = O.gll2grid_coeff(m.ibool, m.xstore, m.ystore, m.zstore, grid)
+    P = _grid_points(grid)
+    assert nf == owner.size
+    # the owner is the lowest element whose single-precision index box holds the point
+    x = np.stack([m.xstore, m.ystore, m.zstore], 1).astype(np.float32)[m.ibool0]  # (nspec,125,3)
+    lo, hi = x.min(1), x.max(1)
+    o = np.array([0.0, 0.0, -2000.0], np.float32)
+    h = np.float32(250.0)
+    imin = (np.float32(1) + (lo - o) / h).astype(np.int64)
+    imax = (np.float32(1) + (hi - o) / h).astype(np.int64)
+    idx = np.stack(np.unravel_index(np.arange(owner.size), (17, 13, 9), order="F"), 1) + 1
+    for p in range(owner.size):
+        inbox = np.all((idx[p] >= imin) & (idx[p] <= imax), axis=1)
+        assert owner[p] - 1 == int(np.argmax(inbox)), p
+    # the value is the field at the owner's clamped location; inside points map back exactly
+    f = lambda q: 3.0 + q[..., 0] / 4000 - 2 * q[..., 1] / 3000 + 0.5 * q[..., 2] / 2000
+    out = O.gll2grid_project(owner, ref, _model(m, f))
+    c8 = [0, 4, 20, 24, 100, 104, 120, 124]
+    xyz = np.stack([m.xstore, m.ystore, m.zstore], 1).astype(np.float64)
+    mapped = np.array([_shape(ref[p, :1], ref[p, 1:2], ref[p, 2:3])[0][0] @ xyz[m.ibool0[owner[p] - 1, c8]]
+                       for p in range(owner.size)])
+    np.testing.assert_allclose(out, f(mapped), atol=2e-6, rtol=0)
+    inside = np.all(np.abs(ref) < 1 - 1e-9, axis=1)
+    assert inside.sum() > owner.size // 3
+    np.testing.assert_allclose(mapped[inside], P[inside], atol=1e-6)
+
+
+def test_gll2grid_points_outside_the_mesh_stay_zero():
+    m = BoxMesh(2, 2, 2, 2000.0, 2000.0, 2000.0)
+    grid = (-500.0, 0.0, -2000.0, 500.0, 500.0, 500.0, 7, 5, 5)  # x from -500 to 2500
+    owner, ref, nf = O.gll2grid_coeff(m.ibool, m.xstore, m.ystore, m.zstore, grid)
+    P = _grid_points(grid)
+    outside = (P[:, 0] < 0) | (P[:, 0] > 2000)
+    assert np.all(owner[outside] == 0) and np.all(owner[~outside] > 0) and nf == (~outside).sum()
+    out = O.gll2grid_project(owner, ref, _model(m, lambda q: 1.0 + q[..., 0]))
+    assert np.all(out[outside] == 0.0)
+
+
+def test_taper_grid_hann_windows_at_the_nodes():
+    dh = 100.0
+    gx = np.arange(41, dtype=np.float32) * 100
+    gy = np.arange(31, dtype=np.float32) * 100
+    gz = -2000 + np.arange(21, dtype=np.float32) * 100
+    mask = O.taper_grid((0, 4000, 0, 3000, -2000, 0), dh, 200.0, 600.0, 300.0, 500.0, gx, gy, gz)
+    mask = mask.reshape(21, 31, 41)
+    win = lambda w: 0.5 - 0.5 * np.cos(2 * np.pi * (w - 1) / 12.0)  # nn = 2*6+1
+    xt = np.zeros(41)
+    xt[2:8] = win(np.arange(1, 7))
+    xt[8:33] = 1.0
+    xt[33:39] = win(np.arange(8, 14))
+    yt = np.zeros(31)
+    yt[2:8] = win(np.arange(1, 7))
+    yt[8:23] = 1.0
+    yt[23:29] = win(np.arange(8, 14))
+    zt = np.zeros(21)
+    zt[3:8] = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(5) / 10.0)  # nn = 2*5+1
+    zt[8:] = 1.0
+    expect = zt[:, None, None] * yt[None, :, None] * xt[None, None, :]
+    np.testing.assert_allclose(mask, expect, atol=2e-6)
+    # outside the mesh box the mask is 0
+    out = O.taper_grid((0, 4000, 0, 3000, -2000, 0), dh, 200.0, 600.0, 300.0, 500.0,
+                       np.array([-50.0, 2000.0], np.float32), np.array([1500.0], np.float32),
+                       np.array([-1000.0, 10.0], np.float32))
+    np.testing.assert_array_equal(out.reshape(2, 1, 2), [[[0.0, 1.0]], [[0.0, 0.0]]])
+
+
+def test_kernel_files_are_fortran_sequential_records(tmp_path):
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((6, 125)).astype(np.float32)
+    postproc.write_kernel(str(tmp_path), "beta_kernel", a, rank=3)
+    fn = tmp_path / "proc000003_beta_kernel.bin"
+    raw = fn.read_bytes()
+    assert len(raw) == 6 * 125 * 4 + 8
+    assert np.frombuffer(raw[:4], "<i4")[0] == 6 * 125 * 4 == np.frombuffer(raw[-4:], "<i4")[0]
+    np.testing.assert_array_equal(np.frombuffer(raw[4:-4], "<f4").reshape(6, 125), a)
+    np.testing.assert_array_equal(postproc.read_kernel(str(tmp_path), "beta_kernel", 6, rank=3), a)
+    with pytest.raises(ValueError):
+        postproc.read_kernel(str(tmp_path), "beta_kernel", 5, rank=3)
