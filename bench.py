@@ -38,11 +38,14 @@ def parse():
     p.add_argument("--steps", type=int, default=None,
                    help="NSTEP of the timed event (default: the configuration's, 10000 for c2, 2000 for c1)")
     p.add_argument("--warmup", type=int, default=100)
-    p.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2",
+    p.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5", "post"], default="c2",
                    help="c2: one production-size event (default); c1: the small event; "
                         "c3: c1 with bf16 / fp16 statics against the FP64 oracle; "
                         "c4: 32 uneven events batched vs one at a time on the 64x64x32 mesh; "
-                        "c5: HTI anisotropic event with 21 c_ij kernels + smoothing of all 21")
+                        "c5: HTI anisotropic event with 21 c_ij kernels + smoothing of all 21; "
+                        "post: kernel post-processing chain (event sums, preconditioner, smoothing, "
+                        "grid projection, taper) on the 64x64x32 mesh")
+    p.add_argument("--post-events", type=int, default=32, help="events summed by the post leg")
     p.add_argument("--c4-events", type=int, default=32)
     p.add_argument("--c4-nstep-scale", type=float, default=0.25,
                    help="NSTEP of the c4 events is U[4000,12000] times this factor")
@@ -90,6 +93,8 @@ def main():
         return batched(a)
     if a.config == "c3":
         return precision(a)
+    if a.config == "post":
+        return post_chain(a)
     if a.config == "c5":
         return aniso_smooth(a)
     from specfwat_amd.mesh import BoxMesh
@@ -664,6 +669,158 @@ def aniso_smooth(a):
                                   for q, v in kinds.items()}},
     }
     print(json.dumps(result), flush=True)
+
+
+def post_chain(a):
+    """xfwat_post_proc on the 64x64x32 mesh: E events' isotropic kernels summed
+    in fp64 straight from a finished adjoint run in HBM (the run's kernels are
+    added E times: the pass costs the same whatever the values), then four events
+    through the EKERNEL-file path, the Hessian preconditioner, PDE smoothing of
+    the three kernels, projection onto a 250 m inversion grid and the taper."""
+    from specfwat_amd import postproc
+    from specfwat_amd.mesh import BoxMesh
+    from specfwat_amd.model import perturbed_model
+    from specfwat_amd.solver import ElasticRun
+    from specfwat_amd.sources import Receivers, Sources, surface_grid
+
+    w = workload("c4")
+    E = a.post_events
+    t0 = time.time()
+    mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
+    model = perturbed_model(mesh, smooth=True)
+    rng = np.random.Generator(np.random.Philox(9))
+    src = Sources(mesh, [[0.4 * w["lx"], 0.55 * w["ly"], -8000.0]], [list(1e15 * rng.standard_normal(6))], [1.0])
+    rec = Receivers(mesh, surface_grid(mesh, w["nrx"], w["nrx"]))
+    nstep = 200
+    f = ElasticRun(mesh, *model, src, rec, w["dt"], nstep, 1, True)
+    f.iterate_time()
+    syn = f.seismograms()
+    ad = ElasticRun(mesh, *model, src, rec, w["dt"], nstep, 3, False, approximate_hess=True)
+    ad.take_forward(f)
+    f.close()
+    ad.set_adjoint_sources(np.ascontiguousarray(syn[::-1], np.float32))
+    ad.iterate_time()
+    log(f"post: mesh {mesh.nspec} el, adjoint run with Hessian ready ({time.time() - t0:.1f}s)")
+    nloc, nglob = mesh.nspec * 125, mesh.nglob
+    stages = {}
+    pf = postproc.PostFlow(mesh, 1, precond_type=1)
+    pf.add_event(ad)  # warm-up
+    pf.close()
+    pf = postproc.PostFlow(mesh, 1, precond_type=1)
+    dev_ms = 0.0
+    ts = time.time()
+    for _ in range(E):
+        pf.add_event(ad)
+        dev_ms += pf.last_time_ms()
+    stages["sum_device_runs"] = {"events": E, "wall_s": time.time() - ts, "device_ms_per_event": dev_ms / E}
+    ad.close()
+    # EKERNEL-file path: single-precision arrays in host memory, four events
+    files = [{n: rng.standard_normal((mesh.nspec, 125), dtype=np.float32) for n in postproc.KERNEL_ISO}
+             for _ in range(4)]
+    hfiles = [np.abs(rng.standard_normal((mesh.nspec, 125), dtype=np.float32)) for _ in range(4)]
+    pf2 = postproc.PostFlow(mesh, 1, precond_type=1)
+    ts = time.time()
+    fms = 0.0
+    for e, h in zip(files, hfiles):
+        pf2.add_event_files(e, h)
+        fms += pf2.last_time_ms()
+    stages["sum_event_files"] = {"events": 4, "wall_s": time.time() - ts, "device_ms_per_event": fms / 4,
+                                 "note": "host-to-device copy of 4 x 66 MB per event dominates the wall time"}
+    pf2.close()
+    ts = time.time()
+    pf.invert_hess()
+    inv_ms = pf.last_time_ms()
+    kp = pf.apply_precond()
+    stages["precond"] = {"wall_s": time.time() - ts, "invert_device_ms": inv_ms, "apply_device_ms": pf.last_time_ms()}
+    grid = postproc.RegularGrid((0.0, 0.0, -w["lz"]), (250.0, 250.0, 250.0),
+                                (int(w["lx"] / 250) + 1, int(w["ly"] / 250) + 1, int(w["lz"] / 250) + 1))
+    ts = time.time()
+    gp = postproc.GridProjection(mesh, grid)
+    stages["gll2grid_prepare"] = {"wall_s": time.time() - ts, "device_ms": gp.last_time_ms(),
+                                  "grid": [grid.nx, grid.ny, grid.nz], "nb_fd_point": gp.nb_fd_point}
+    ts = time.time()
+    smoothed = pf.pde_smooth(5000.0, 2000.0, gp, kernels=kp, faithful=False, timed=True)
+    nt, sm_e, sm_u = pf.last_smooth
+    stages["pde_smooth_and_project"] = {"wall_s": time.time() - ts, "ntstep": nt,
+                                        "smooth_device_ms": sm_e + sm_u, "project_device_ms": gp.last_time_ms()}
+    pf.mesh_extent()
+    ts = time.time()
+    tapered = pf.taper_kernel_grid(smoothed, grid, (2000.0, 5000.0, 2000.0, 4000.0))
+    stages["taper"] = {"wall_s": time.time() - ts}
+    if not all(np.isfinite(v).all() for v in tapered.values()):
+        raise RuntimeError("non-finite gradient grid")
+    gp.close()
+    pf.close()
+    log("post stages: " + ", ".join(f"{k} {v['wall_s']:.3f}s" for k, v in stages.items()))
+    per = stages["sum_device_runs"]["device_ms_per_event"]
+    # per local point: ibool 4, mu/kappa kernels 16, rho/kappa/mu statics 12, three fp64 sums
+    # read+write 48, Hessian sum read+write 16; per unique point the rho and Hessian
+    # accumulators 16
+    sum_bytes = 96.0 * nloc + 16.0 * nglob
+    achieved = sum_bytes / (per * 1e-3) / 1e9
+    value = nloc * E / (stages["sum_device_runs"]["wall_s"])
+    result = {
+        "metric": "kernel-point-events/s (post-processing event sums)",
+        "value": value, "unit": "kernel-point-events/s",
+        "n_gpus": 1, "steps": E, "warmup": 1, "ms_per_step": stages["sum_device_runs"]["wall_s"] / E * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"post-processing on the {w['nex']}x{w['ney']}x{w['nez']} el mesh ({nloc} kernel "
+                                f"points, {nglob} GLL pts): {E} isotropic events summed from HBM, precondition, "
+                                "smooth 3 kernels (sigma 5/2 km), project onto a 250 m grid, taper")},
+        "roofline": {"bound": "hbm", "kernel": "k_post_add_iso_run", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": sum_bytes, "avg_launch_ms": per, "launches": E},
+        "stages": stages,
+    }
+    if not a.no_cpu:
+        result["cpu_baseline"] = post_cpu_baseline(mesh, grid, files, hfiles)
+    print(json.dumps(result), flush=True)
+
+
+def post_cpu_baseline(mesh, grid, files, hfiles):
+    """The oracle's post-processing stages on the host (single thread, single
+    precision like the reference): four events' sums, invert_hess and the
+    preconditioner, gll2grid location + projection of the three kernels, taper.
+    Smoothing is not part of the sample."""
+    from oracle import oracle as orc
+    from specfwat_amd import postproc
+    # mesh distances and box: read_mesh_databases_for_init's set-up, not a timed stage
+    dh = np.float32(np.float32(mesh.distance_max_glob()) + np.float32(mesh.distance_min_glob())) / np.float32(2)
+    box = (mesh.xstore.min(), mesh.xstore.max(), mesh.ystore.min(), mesh.ystore.max(), mesh.zstore.min(),
+           mesh.zstore.max())
+    gx, gy, gz = grid.axes()
+    ts = time.time()
+    tot = {n: np.zeros((mesh.nspec, 125), np.float32) for n in postproc.KERNEL_ISO}
+    hs = np.zeros((mesh.nspec, 125), np.float32)
+    for e, h in zip(files, hfiles):
+        for n in postproc.KERNEL_ISO:
+            tot[n] += e[n]
+        hs += np.abs(h)
+    inv = orc.invert_hess(hs).reshape(mesh.nspec, 125)
+    for n in tot:
+        tot[n] *= inv
+    t_sum = time.time() - ts
+    t1 = time.time()
+    owner, ref, _ = orc.gll2grid_coeff(mesh.ibool, mesh.xstore, mesh.ystore, mesh.zstore,
+                                       (grid.ox, grid.oy, grid.oz, grid.hx, grid.hy, grid.hz, grid.nx, grid.ny,
+                                        grid.nz))
+    t_loc = time.time() - t1
+    t1 = time.time()
+    g = [orc.gll2grid_project(owner, ref, tot[n]) for n in tot]
+    t_proj = time.time() - t1
+    t1 = time.time()
+    mask = orc.taper_grid(box, dh, 2000.0, 5000.0, 2000.0, 4000.0, gx, gy, gz)
+    for q in range(len(g)):
+        g[q] = g[q] * mask
+    t_tap = time.time() - t1
+    wall = time.time() - ts
+    log(f"post cpu baseline: sums+precond {t_sum:.2f}s, locate {t_loc:.2f}s, project {t_proj:.2f}s, "
+        f"taper {t_tap:.2f}s")
+    return {"value": mesh.nspec * 125 * len(files) / t_sum, "unit": "kernel-point-events/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"4 events' file kernels summed + preconditioned in single precision ({t_sum:.2f}s), "
+                       f"gll2grid location {t_loc:.2f}s and projection of 3 kernels {t_proj:.2f}s, taper "
+                       f"{t_tap:.2f}s; oracle (C++) and numpy, one thread; smoothing not sampled")}
 
 
 def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event"):

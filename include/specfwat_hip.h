@@ -255,6 +255,8 @@ void fwat_post_invert_hess_(long *Post_pointer, int *precond_type);
 /* kernels *= inverted Hessian (apply_precond, post_processing.f90:189-191) */
 void fwat_post_apply_precond_(long *Post_pointer);
 void fwat_post_info_(long *Post_pointer, int *nker, int *nevents, int *nhess);
+/* device time (ms) of the last add / invert / apply call */
+void fwat_post_last_time_(long *Post_pointer, double *ms);
 void fwat_post_destroy_(long *Post_pointer);
 
 /* ---- gll2grid (src/grid/model_grid_data.f90:192-211): point location and interpolation

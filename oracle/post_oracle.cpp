@@ -28,8 +28,9 @@
 #include <cstring>
 #include <vector>
 
-extern "C" float orc_interp3(const float* vx, int nx, const float* vy, int ny, const float* vz, int nz,
-                             const float* valin, float x, float y, float z);
+extern "C" float orc_interp3_pre(const double* dx, const float* vx, int nx, const double* dy, const float* vy,
+                                 int ny, const double* dz, const float* vz, int nz, const float* valin, float x,
+                                 float y, float z);
 
 namespace {
 
@@ -299,13 +300,15 @@ int orc_taper_grid(float xmin, float xmax, float ymin, float ymax, float zmin, f
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) tap[(size_t)i + (size_t)nx * (j + (size_t)ny * k)] = xt[i] * yt[j] * zt[k];
+  const std::vector<double> dxc(xc.begin(), xc.end()), dyc(yc.begin(), yc.end()), dzc(zc.begin(), zc.end());
   for (int k = 0; k < gnz; ++k)
     for (int j = 0; j < gny; ++j)
       for (int i = 0; i < gnx; ++i) {
         const float x = gx[i], y = gy[j], z = gz[k];
         float v = 0.0f;
         if (!(x < xc[0] || x > xc[nx - 1] || y < yc[0] || y > yc[ny - 1] || z < zc[0] || z > zc[nz - 1]))
-          v = orc_interp3(xc.data(), nx, yc.data(), ny, zc.data(), nz, tap.data(), x, y, z);
+          v = orc_interp3_pre(dxc.data(), xc.data(), nx, dyc.data(), yc.data(), ny, dzc.data(), zc.data(), nz,
+                              tap.data(), x, y, z);
         mask[(size_t)i + (size_t)gnx * (j + (size_t)gny * k)] = v;
       }
   return 0;

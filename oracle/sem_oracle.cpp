@@ -1180,13 +1180,22 @@ int orc_locate_bissection(const double* valx, int n, double x) {
   return jl;
 }
 
+float orc_interp3_pre(const double* dx, const float* vx, int nx, const double* dy, const float* vy, int ny,
+                      const double* dz, const float* vz, int nz, const float* valin, float x, float y, float z);
+
 // interp3_0_cr, src/shared/utils.f90:1162-1232; valin is Fortran (nx,ny,nz)
 float orc_interp3(const float* vx, int nx, const float* vy, int ny, const float* vz, int nz,
                   const float* valin, float x, float y, float z) {
   std::vector<double> dx(vx, vx + nx), dy(vy, vy + ny), dz(vz, vz + nz);
-  int indx = orc_locate_bissection(dx.data(), nx, (double)x);
-  int indy = orc_locate_bissection(dy.data(), ny, (double)y);
-  int indz = orc_locate_bissection(dz.data(), nz, (double)z);
+  return orc_interp3_pre(dx.data(), vx, nx, dy.data(), vy, ny, dz.data(), vz, nz, valin, x, y, z);
+}
+
+// the same with the double copies of the axes made once by the caller
+float orc_interp3_pre(const double* dx, const float* vx, int nx, const double* dy, const float* vy, int ny,
+                      const double* dz, const float* vz, int nz, const float* valin, float x, float y, float z) {
+  int indx = orc_locate_bissection(dx, nx, (double)x);
+  int indy = orc_locate_bissection(dy, ny, (double)y);
+  int indz = orc_locate_bissection(dz, nz, (double)z);
   const int m = 2;
   int kx = std::min(std::max(indx - (m - 1) / 2, 1), nx + 1 - m);
   int ky = std::min(std::max(indy - (m - 1) / 2, 1), ny + 1 - m);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "fwat_post_invert_hess_": "Li",
     "fwat_post_apply_precond_": "L",
     "fwat_post_info_": "Liii",
+    "fwat_post_last_time_": "Ld",
     "fwat_post_destroy_": "L",
     "fwat_gll2grid_prepare_": "LiiIFFF" + "f" * 6 + "iiii" + "l",
     "fwat_gll2grid_": "LiFF",

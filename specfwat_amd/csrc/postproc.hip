@@ -40,7 +40,20 @@ struct PostState {
   size_t stage_n = 0;
   unsigned long long* d_max = nullptr;
   int nevents = 0, nhess = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;  // device time of the last add / invert / apply call
 };
+
+void post_begin(PostState* p) { HIP_CHECK(hipEventRecord(p->ev0, p->stream)); }
+
+void post_end(PostState* p) {
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(p->ev1, p->stream));
+  HIP_CHECK(hipStreamSynchronize(p->stream));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  p->last_ms = ms;
+}
 
 PostState* post_state(long* pp) {
   PostState* p = reinterpret_cast<PostState*>(*pp);
@@ -527,6 +540,8 @@ void fwat_post_create_(long* pp, int* nspec, int* model_type, int* use_rho_scali
   p->nker = *model_type == POST_ISO ? 3 : 5;
   p->use_rho_scaling = *use_rho_scaling != 0;
   HIP_CHECK(hipStreamCreate(&p->stream));
+  HIP_CHECK(hipEventCreate(&p->ev0));
+  HIP_CHECK(hipEventCreate(&p->ev1));
   p->d_ker = dev_alloc<double>(p->nker * p->nloc);
   p->d_hess = dev_alloc<double>(p->nloc);
   p->d_max = dev_alloc<unsigned long long>(1);
@@ -558,6 +573,7 @@ void fwat_post_add_event_(long* pp, long* Mesh_pointer, int* precond_type) {
   if ((*precond_type == 0 || *precond_type == 1) && !hg)
     SEM_FAIL("fwat_post_add_event_: PRECOND_TYPE %d needs the run's Hessian (APPROXIMATE_HESS_KL)", *precond_type);
   const int habs = *precond_type == 1;
+  post_begin(p);
   if (p->model_type == POST_ISO) {
     if (m->anisotropic_kl) SEM_FAIL("fwat_post_add_event_: isotropic sums from an anisotropic-kernel run");
     hipLaunchKernelGGL(k_post_add_iso_run, dim3(pblk(p->nloc)), dim3(PTPB), 0, p->stream, p->nloc, m->d_ibool,
@@ -569,8 +585,7 @@ void fwat_post_add_event_(long* pp, long* Mesh_pointer, int* precond_type) {
     hipLaunchKernelGGL(k_post_add_hti_run, dim3(pblk(p->nloc)), dim3(PTPB), 0, p->stream, p->nloc, m->d_ibool,
                        m->d_rho_kl_glob, m->d_rho, m->d_cijkl_kl, p->d_model, hg, habs, p->d_ker, p->d_hess);
   }
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(p->stream));
+  post_end(p);
   p->nevents++;
   if (hg) p->nhess++;
 }
@@ -590,12 +605,12 @@ void fwat_post_add_event_files_(long* pp, realw* ker, realw* hess, int* precond_
   if (with_h)
     HIP_CHECK(hipMemcpyAsync(p->d_stage + (size_t)nin * p->nloc, hess, p->nloc * sizeof(float),
                              hipMemcpyHostToDevice, p->stream));
+  post_begin(p);
   hipLaunchKernelGGL(k_post_add_files, dim3(pblk(p->nloc)), dim3(PTPB), 0, p->stream, p->nloc, p->model_type,
                      p->use_rho_scaling ? 0 : 1, p->d_stage, p->d_model,
                      with_h ? p->d_stage + (size_t)nin * p->nloc : nullptr, *precond_type == 1 ? 1 : 0, p->d_ker,
                      p->d_hess);
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(p->stream));
+  post_end(p);
   p->nevents++;
   if (with_h) p->nhess++;
 }
@@ -620,22 +635,22 @@ void fwat_post_set_hess_(long* pp, double* h_hess) {
 
 void fwat_post_invert_hess_(long* pp, int* precond_type) {
   PostState* p = post_state(pp);
+  post_begin(p);
   device_maxabs(p, p->d_hess, p->nloc);
   hipLaunchKernelGGL(k_hess_invert, dim3(pblk(p->nloc)), dim3(PTPB), 0, p->stream, p->nloc, p->d_hess, p->d_max,
                      *precond_type == 0 ? 1 : 0);
   device_maxabs(p, p->d_hess, p->nloc);
   hipLaunchKernelGGL(k_scale_by_inv_max, dim3(pblk(p->nloc)), dim3(PTPB), 0, p->stream, p->nloc, p->d_hess,
                      p->d_max);
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(p->stream));
+  post_end(p);
 }
 
 void fwat_post_apply_precond_(long* pp) {
   PostState* p = post_state(pp);
+  post_begin(p);
   hipLaunchKernelGGL(k_apply_precond, dim3(pblk(p->nloc)), dim3(PTPB), 0, p->stream, p->nloc, p->nker, p->d_ker,
                      p->d_hess);
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(p->stream));
+  post_end(p);
 }
 
 void fwat_post_info_(long* pp, int* nker, int* nevents, int* nhess) {
@@ -645,10 +660,14 @@ void fwat_post_info_(long* pp, int* nker, int* nevents, int* nhess) {
   *nhess = p->nhess;
 }
 
+void fwat_post_last_time_(long* pp, double* ms) { *ms = post_state(pp)->last_ms; }
+
 void fwat_post_destroy_(long* pp) {
   if (!*pp) return;
   PostState* p = post_state(pp);
   HIP_CHECK(hipStreamSynchronize(p->stream));
+  HIP_CHECK(hipEventDestroy(p->ev0));
+  HIP_CHECK(hipEventDestroy(p->ev1));
   dev_free(p->d_ker); dev_free(p->d_hess); dev_free(p->d_model); dev_free(p->d_stage); dev_free(p->d_max);
   HIP_CHECK(hipStreamDestroy(p->stream));
   delete p;

@@ -181,10 +181,16 @@ class PostFlow:
                  for k in ("c44", "c45", "c55", "rho_vp", "rho_vs", "rho")]
             _lib.call("fwat_post_set_model_", self.h, *m)
         self.inverted = False
+        self.last_smooth = (0, 0.0, 0.0)
+        self._extent = None
 
     @property
     def nevents(self):
         return _lib.call("fwat_post_info_", self.h, 0, 0, 0)[1]
+
+    def last_time_ms(self):
+        """Device time of the last add / invert / apply call."""
+        return float(_lib.call("fwat_post_last_time_", self.h, 0.0)[0])
 
     def add_event(self, run):
         """Add a finished adjoint run's kernels (and Hessian) straight from HBM."""
@@ -249,23 +255,28 @@ class PostFlow:
         k = self.sum_kernel() if kernels is None else kernels
         names = [n for n in self.names if not (self.use_rho_scaling and n == "rhop")]
         fields = np.stack([np.asarray(k[n], F32) for n in names])
-        dglob, _, _, _ = smooth_batch(self.mesh, fields, sigma_h, sigma_v, **smooth_kw)
+        dglob, nt, elem_ms, upd_ms = smooth_batch(self.mesh, fields, sigma_h, sigma_v, **smooth_kw)
+        self.last_smooth = (nt, elem_ms, upd_ms)
         grid = projection(dglob[:, self.mesh.ibool0])
         out = {n: grid[q] for q, n in enumerate(names)}
         if self.use_rho_scaling:
             out["rhop"] = (out["beta"] * F32(RHO_SCALING_FAC)).astype(F32)
         return out
 
-    def taper_kernel_grid(self, grids, grid, taper, distance_min_glob=None, distance_max_glob=None):
-        mesh = self.mesh
-        if distance_min_glob is None:
-            distance_min_glob = mesh.distance_min_glob()
-        if distance_max_glob is None:
-            distance_max_glob = mesh.distance_max_glob()
-        box = (mesh.xstore.min(), mesh.xstore.max(), mesh.ystore.min(), mesh.ystore.max(), mesh.zstore.min(),
-               mesh.zstore.max())
+    def mesh_extent(self):
+        """(box, distance_min_glob, distance_max_glob) of check_mesh_distances, computed once
+        (read_mesh_databases_for_init, kernel_io.f90:15-74)."""
+        if self._extent is None:
+            m = self.mesh
+            box = (m.xstore.min(), m.xstore.max(), m.ystore.min(), m.ystore.max(), m.zstore.min(), m.zstore.max())
+            self._extent = (box, m.distance_min_glob(), m.distance_max_glob())
+        return self._extent
+
+    def taper_kernel_grid(self, grids, grid, taper):
+        """taper = (TAPER_H_SUPPRESS, TAPER_H_BUFFER, TAPER_V_SUPPRESS, TAPER_V_BUFFER)."""
+        box, dmin, dmax = self.mesh_extent()
         names = list(grids)
-        data = taper_grid(box, distance_min_glob, distance_max_glob, taper, grid, np.stack([grids[n] for n in names]))
+        data = taper_grid(box, dmin, dmax, taper, grid, np.stack([grids[n] for n in names]))
         return dict(zip(names, data))
 
     def write_gradient_grid(self, grids, grid, filename):
