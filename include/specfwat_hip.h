@@ -164,7 +164,7 @@ void fwat_brick_info_(long *Mesh_pointer, int *nhalo, long *nslots, long *ncontr
 void fwat_set_slabs_(long *Mesh_pointer, int *elems_per_slab);
 /* Steps per fp32 partial-sum block of the element-pass kernels (mu/kappa, c_ij) in the fused
  * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
- * fp64).  The rho kernel is always accumulated per step in fp64. */
+ * fp64).  The rho kernel (summed in the assembly pass) uses the same blocks. */
 void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
 /* 0 fp32, 1 bf16, 2 fp16 with per-array scaling, for kappa/mu and the Jacobian */
 void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
