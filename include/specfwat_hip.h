@@ -284,6 +284,29 @@ void fwat_taper_grid_(realw *x_min_glob, realw *x_max_glob, realw *y_min_glob, r
                       realw *v_buffer, realw *ox, realw *oy, realw *oz, realw *hx, realw *hy, realw *hz,
                       int *nx, int *ny, int *nz, int *nfield, realw *data, realw *mask);
 
+/* ---- grid optimiser (xfwat_optimize, src/optimization/optimize_grid.f90).  Grids are
+ *      (nx,ny,nz,nkernel) single precision, n = nx*ny*nz*nkernel, npg = nx*ny*nz.  Every inner
+ *      product is an fp64 sum in a fixed order (the reference's inner_product,
+ *      src/shared/utils.f90:1527-1534, is a single-precision sum). */
+/* get_SD_direction (optimize_grid.f90:229-241): direction = -gradient*hess / max|.| */
+void fwat_opt_sd_direction_(long *n, realw *gradient, realw *hess, realw *direction);
+/* get_lbfgs_direction (optimize_grid.f90:243-339): grads and models (nx,ny,nz,nkernel,nstore+1)
+ * hold iterations iter_store .. iter_current as read_gradient_grid / read_model_grid return them
+ * (models linear; the first three parameters enter as logs, opt_io.f90:79-81); hess the inverse
+ * Hessian grid; returns the normalised direction and its angle to -gradient in degrees */
+void fwat_opt_lbfgs_direction_(long *n, long *npg, int *nstore, realw *grads, realw *models, realw *hess,
+                               realw *dx, realw *dy, realw *dz, realw *direction, double *angle);
+/* get_CG_direction (optimize_grid.f90:341-389): g1 current, g0 previous gradient; alpha returned */
+void fwat_opt_cg_direction_(long *n, long *npg, realw *g1, realw *g0, realw *hess, realw *direction,
+                            double *alpha);
+/* model_update + alpha_scaling (optimize_grid.f90:176-203, 391-404): model_type 1 or 2;
+ * update_iso / update_ani select the alternating-inversion halves of model type 2;
+ * vpvs_ratio_range(2) = VPVS_RATIO_RANGE; model (nx,ny,nz,nkernel) updated in place */
+void fwat_opt_model_update_(long *npg, int *nkernel, int *model_type, int *update_iso, int *update_ani,
+                            realw *step_len, realw *vpvs_ratio_range, realw *model, realw *direction);
+/* device time (ms) of the last fwat_opt_* call, uploads excluded */
+void fwat_opt_last_time_(double *ms);
+
 #ifdef __cplusplus
 }
 #endif

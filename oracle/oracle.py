@@ -16,6 +16,9 @@ Function-to-reference map (file:line in the SpecFWAT tree):
                                    src/shared/projection_on_FD_grid_mod_fwat.f90:186-596, 935-1125
   taper_grid                       src/postproc/post_processing.f90:296-333,
                                    src/postproc/taper_3D_mod.f90:16-141
+  sd / lbfgs / cg_direction        src/optimization/optimize_grid.f90:229-389,
+                                   src/shared/utils.f90:1527-1534
+  model_update                     src/optimization/optimize_grid.f90:176-203, 391-404
   hti_cij                          src/patch/aniso.f90:55-162
   simpson                          src/shared/utils.f90:1630-1663
   window_taper                     src/shared/signal.f90:419-491
@@ -57,7 +60,7 @@ class OrcParams(C.Structure):
 
 def build(force=False):
     """Compile the oracle with g++ (OpenMP, portable x86-64-v3 code)."""
-    srcs = [os.path.join(_HERE, "sem_oracle.cpp"), os.path.join(_HERE, "post_oracle.cpp")]
+    srcs = [os.path.join(_HERE, n) for n in ("sem_oracle.cpp", "post_oracle.cpp", "opt_oracle.cpp")]
     if force or not os.path.exists(_LIB_PATH) or any(os.path.getmtime(_LIB_PATH) < os.path.getmtime(s)
                                                       for s in srcs):
         import subprocess
@@ -119,6 +122,12 @@ def lib():
         L.orc_gll2grid_project.argtypes = [C.c_long, _ip, _dp, _fp, _fp]
         L.orc_taper_grid.restype = C.c_int
         L.orc_taper_grid.argtypes = [C.c_float] * 11 + [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int, _fp]
+        L.orc_sd_direction.argtypes = [C.c_long, _fp, _fp, _fp, C.c_int]
+        L.orc_lbfgs_direction.argtypes = [C.c_long, C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
+                                          C.c_float, _fp, _dp, C.c_int]
+        L.orc_cg_direction.argtypes = [C.c_long, C.c_long, _fp, _fp, _fp, _fp, _dp, C.c_int]
+        L.orc_model_update.argtypes = [C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                       C.c_float, _fp, _fp]
         _lib = L
     return _lib
 
@@ -423,3 +432,43 @@ def taper_grid(box, dh, h_suppress, h_buffer, v_suppress, v_buffer, gx, gy, gz):
     if rc != 0:
         raise ValueError("taper ranges do not fit the taper grid")
     return out
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def sd_direction(grad, hess, dbl=False):
+    g, h = _f32(grad).ravel(), _f32(hess).ravel()
+    out = np.zeros(g.size, np.float32)
+    lib().orc_sd_direction(g.size, _p(g, C.c_float), _p(h, C.c_float), _p(out, C.c_float), int(dbl))
+    return out
+
+
+def lbfgs_direction(grads, models, hess, npg, dx=1.0, dy=1.0, dz=1.0, dbl=False):
+    """grads, models: [nstore+1, n] (iterations iter_store .. iter_current);
+    returns (direction [n], angle in degrees)."""
+    g, m, h = _f32(grads), _f32(models), _f32(hess).ravel()
+    n = h.size
+    out = np.zeros(n, np.float32)
+    ang = np.zeros(1)
+    lib().orc_lbfgs_direction(n, npg, g.shape[0] - 1, _p(g, C.c_float), _p(m, C.c_float), _p(h, C.c_float),
+                              dx, dy, dz, _p(out, C.c_float), _p(ang, C.c_double), int(dbl))
+    return out, float(ang[0])
+
+
+def cg_direction(g1, g0, hess, npg, dbl=False):
+    a, b, h = _f32(g1).ravel(), _f32(g0).ravel(), _f32(hess).ravel()
+    out = np.zeros(a.size, np.float32)
+    alpha = np.zeros(1)
+    lib().orc_cg_direction(a.size, npg, _p(a, C.c_float), _p(b, C.c_float), _p(h, C.c_float),
+                           _p(out, C.c_float), _p(alpha, C.c_double), int(dbl))
+    return out, float(alpha[0])
+
+
+def model_update(model, direction, npg, model_type, step, vpvs_range, update_iso=True, update_ani=True):
+    m = np.array(model, np.float32).ravel()
+    d = _f32(direction).ravel()
+    lib().orc_model_update(npg, m.size // npg, model_type, int(update_iso), int(update_ani), step,
+                           vpvs_range[0], vpvs_range[1], _p(m, C.c_float), _p(d, C.c_float))
+    return m

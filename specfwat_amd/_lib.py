@@ -91,6 +91,11 @@ SIGNATURES = {
     "fwat_gll2grid_last_time_": "Ld",
     "fwat_gll2grid_destroy_": "L",
     "fwat_taper_grid_": "f" * 18 + "iiii" + "FF",
+    "fwat_opt_sd_direction_": "lFFF",
+    "fwat_opt_lbfgs_direction_": "lliFFFfffFd",
+    "fwat_opt_cg_direction_": "llFFFFd",
+    "fwat_opt_model_update_": "liiiifFFF",
+    "fwat_opt_last_time_": "d",
 }
 
 _lib = None
