@@ -677,7 +677,7 @@ def post_chain(a):
     added E times: the pass costs the same whatever the values), then four events
     through the EKERNEL-file path, the Hessian preconditioner, PDE smoothing of
     the three kernels, projection onto a 250 m inversion grid and the taper."""
-    from specfwat_amd import postproc
+    from specfwat_amd import optimize, postproc
     from specfwat_amd.mesh import BoxMesh
     from specfwat_amd.model import perturbed_model
     from specfwat_amd.solver import ElasticRun
@@ -751,6 +751,18 @@ def post_chain(a):
         raise RuntimeError("non-finite gradient grid")
     gp.close()
     pf.close()
+    # optimiser: L-BFGS direction from 5 stored iterations of the gradient grid, model update
+    grads, models = _opt_history(tapered, grid, 5)
+    hgrid = np.ones_like(grads[0])
+    ts = time.time()
+    direction, angle = optimize.get_lbfgs_direction(grads, models, hgrid, spacing=(grid.hx, grid.hy, grid.hz))
+    lb_ms = optimize.last_time_ms()
+    new_model = optimize.model_update(models[-1], direction, 1, 0.02, (1.5, 2.0))
+    stages["lbfgs_direction_and_update"] = {"wall_s": time.time() - ts, "lbfgs_device_ms": lb_ms,
+                                            "update_device_ms": optimize.last_time_ms(), "nstore": 5,
+                                            "angle_deg": angle}
+    if not np.isfinite(new_model).all():
+        raise RuntimeError("non-finite updated model")
     log("post stages: " + ", ".join(f"{k} {v['wall_s']:.3f}s" for k, v in stages.items()))
     per = stages["sum_device_runs"]["device_ms_per_event"]
     # per local point: ibool 4, mu/kappa kernels 16, rho/kappa/mu statics 12, three fp64 sums
@@ -773,15 +785,29 @@ def post_chain(a):
         "stages": stages,
     }
     if not a.no_cpu:
-        result["cpu_baseline"] = post_cpu_baseline(mesh, grid, files, hfiles)
+        result["cpu_baseline"] = post_cpu_baseline(mesh, grid, files, hfiles, (grads, models, hgrid))
     print(json.dumps(result), flush=True)
 
 
-def post_cpu_baseline(mesh, grid, files, hfiles):
+def _opt_history(gradient, grid, nstore):
+    """A synthetic optimisation history on the inversion grid: nstore + 1
+    models (vp, vs, rho) and gradients, the last gradient the tapered one."""
+    rng = np.random.Generator(np.random.Philox(11))
+    g = np.stack([gradient[n] for n in ("alpha", "beta", "rhop")]).astype(np.float32)
+    scale = np.abs(g).max() or 1.0
+    base = np.stack([np.full(g.shape[1:], v, np.float32) for v in (6000.0, 3500.0, 2700.0)])
+    models, grads = [base], [g / scale]
+    for _ in range(nstore):
+        models.append((models[-1] * np.exp(rng.normal(0, 0.01, g.shape))).astype(np.float32))
+        grads.append((grads[-1] + rng.uniform(0.5, 2.0, g.shape) * np.log(models[-1] / models[-2])).astype(np.float32))
+    return grads, models
+
+
+def post_cpu_baseline(mesh, grid, files, hfiles, opt_hist):
     """The oracle's post-processing stages on the host (single thread, single
     precision like the reference): four events' sums, invert_hess and the
-    preconditioner, gll2grid location + projection of the three kernels, taper.
-    Smoothing is not part of the sample."""
+    preconditioner, gll2grid location + projection of the three kernels, taper,
+    and the L-BFGS direction.  Smoothing is not part of the sample."""
     from oracle import oracle as orc
     from specfwat_amd import postproc
     # mesh distances and box: read_mesh_databases_for_init's set-up, not a timed stage
@@ -813,14 +839,20 @@ def post_cpu_baseline(mesh, grid, files, hfiles):
     for q in range(len(g)):
         g[q] = g[q] * mask
     t_tap = time.time() - t1
+    t1 = time.time()
+    G, M, H = opt_hist
+    orc.lbfgs_direction(np.stack(G).reshape(len(G), -1), np.stack(M).reshape(len(M), -1), H, grid.npts,
+                        grid.hx, grid.hy, grid.hz, dbl=False)
+    t_opt = time.time() - t1
     wall = time.time() - ts
     log(f"post cpu baseline: sums+precond {t_sum:.2f}s, locate {t_loc:.2f}s, project {t_proj:.2f}s, "
-        f"taper {t_tap:.2f}s")
+        f"taper {t_tap:.2f}s, lbfgs {t_opt:.2f}s ({wall:.1f}s)")
     return {"value": mesh.nspec * 125 * len(files) / t_sum, "unit": "kernel-point-events/s", "cores": 1,
             "kind": "port",
             "sample": (f"4 events' file kernels summed + preconditioned in single precision ({t_sum:.2f}s), "
                        f"gll2grid location {t_loc:.2f}s and projection of 3 kernels {t_proj:.2f}s, taper "
-                       f"{t_tap:.2f}s; oracle (C++) and numpy, one thread; smoothing not sampled")}
+                       f"{t_tap:.2f}s, L-BFGS direction from 5 stored iterations {t_opt:.2f}s; oracle (C++) "
+                       "and numpy, one thread; smoothing not sampled")}
 
 
 def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event"):
