@@ -12,6 +12,9 @@ the reference fixes, plus the host side of the kernel files.
                  evaluated at the owner element's clamped location), the
                  first-element-claims rule, points outside the mesh
   taper_grid     the Hann half-window values at the taper-grid nodes
+  kernel files   a model file of the reference's toy example (tests/golden/,
+                 copied by tests/golden/make_fixtures.py), read and re-written
+                 byte for byte
 """
 import os
 import sys
@@ -202,3 +205,15 @@ def test_kernel_files_are_fortran_sequential_records(tmp_path):
     np.testing.assert_array_equal(postproc.read_kernel(str(tmp_path), "beta_kernel", 6, rank=3), a)
     with pytest.raises(ValueError):
         postproc.read_kernel(str(tmp_path), "beta_kernel", 5, rank=3)
+
+
+def test_kernel_file_io_reproduces_a_reference_model_file(tmp_path):
+    """proc000000_vp.bin of the reference's toy example (written by SPECFEM's
+    save routines): read as (5,5,5,1152) and written back byte for byte."""
+    gold = os.path.join(ROOT, "tests", "golden")
+    raw = open(os.path.join(gold, "proc000000_vp.bin"), "rb").read()
+    vp = postproc.read_kernel(gold, "vp", 1152)
+    assert vp.shape == (1152, 125) and vp.dtype == np.float32
+    assert 6374.0 < vp.min() < 6376.0 and 7615.0 < vp.max() < 7617.0
+    postproc.write_kernel(str(tmp_path), "vp", vp)
+    assert (tmp_path / "proc000000_vp.bin").read_bytes() == raw
