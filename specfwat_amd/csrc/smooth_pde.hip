@@ -233,7 +233,9 @@ __global__ void k_to_field_major(int nglob, int nfield, int nfp, const realw* __
 
 int g_smooth_faithful = 1;
 
-int padded_fields(int nf) { return nf == 1 ? 1 : ((nf + 3) / 4) * 4; }
+// fields per point row in HBM: 1-3 as they are (rows of 4, 8 or 12 bytes),
+// more padded to a multiple of 4 for 16-byte vector access
+int padded_fields(int nf) { return nf <= 3 ? nf : ((nf + 3) / 4) * 4; }
 
 struct SmoothPlan {
   int nspec = 0, nglob = 0, nirr = 0, npml = 0, nfield = 0, nfp = 1;
@@ -305,6 +307,8 @@ void run_steps(SmoothPlan& p, int nstep, realw cv, realw ch, realw xix_regular, 
   p.elem_ms = p.update_ms = 0;
   switch (p.nfp) {
     case 1: launch_steps<1>(p, a, nstep, timed); break;
+    case 2: launch_steps<2>(p, a, nstep, timed); break;
+    case 3: launch_steps<3>(p, a, nstep, timed); break;
     case 4: launch_steps<4>(p, a, nstep, timed); break;
     case 8: launch_steps<8>(p, a, nstep, timed); break;
     case 12: launch_steps<12>(p, a, nstep, timed); break;

@@ -244,6 +244,7 @@ class PostFlow:
         """sum_precond: the inverse Hessian, smoothed and projected onto the
         grid, normalised by its maximum |value|."""
         inv = self.invert_hess() if not self.inverted else self.hessian()
+        smooth_kw.setdefault("distance_min_glob", self.mesh_extent()[1])
         dglob, _, _, _ = smooth_batch(self.mesh, inv[None].astype(F32), sigma_h, sigma_v, **smooth_kw)
         gm = projection(dglob[:, self.mesh.ibool0])[0]
         m = np.abs(gm).max()
@@ -255,6 +256,7 @@ class PostFlow:
         k = self.sum_kernel() if kernels is None else kernels
         names = [n for n in self.names if not (self.use_rho_scaling and n == "rhop")]
         fields = np.stack([np.asarray(k[n], F32) for n in names])
+        smooth_kw.setdefault("distance_min_glob", self.mesh_extent()[1])
         dglob, nt, elem_ms, upd_ms = smooth_batch(self.mesh, fields, sigma_h, sigma_v, **smooth_kw)
         self.last_smooth = (nt, elem_ms, upd_ms)
         grid = projection(dglob[:, self.mesh.ibool0])

@@ -230,13 +230,15 @@ def test_smoother_matches_oracle_restatement():
         assert np.array_equal(dat.ravel(), dglob[m.ibool0.ravel()])
 
 
-def test_smoother_batch_matches_single():
+@pytest.mark.parametrize("nfield", [2, 3, 5])
+def test_smoother_batch_matches_single(nfield):
+    """Point rows of 2 and 3 fields unpadded, 5 padded to 8."""
     from specfwat_amd.smooth import smooth_batch, smooth_sem_pde
     m = BoxMesh(6, 6, 4, 6000.0, 6000.0, 4000.0)
     rng = np.random.default_rng(12)
-    fields = rng.standard_normal((3, m.nspec, 125)).astype(np.float32)
+    fields = rng.standard_normal((nfield, m.nspec, 125)).astype(np.float32)
     out, nt, _, _ = smooth_batch(m, fields, 800.0, 400.0, faithful=False)
-    for q in range(3):
+    for q in range(nfield):
         _, d1, _ = smooth_sem_pde(m, fields[q], 800.0, 400.0, faithful=False)
         assert rel(out[q], d1) < 1e-6
 
