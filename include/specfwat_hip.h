@@ -307,6 +307,20 @@ void fwat_opt_model_update_(long *npg, int *nkernel, int *model_type, int *updat
 /* device time (ms) of the last fwat_opt_* call, uploads excluded */
 void fwat_opt_last_time_(double *ms);
 
+/* ---- waveform misfit and adjoint sources of every trace of an event in one device pass:
+ *      the per-receiver loop of leq_data.f90:148 over WaveformMisfit%calc_adjoint_source
+ *      (src/preproc/adjoint_sources/waveform_misfit.f90:15-64; simpson, utils.f90:1630-1663;
+ *      window_taper, signal.f90:419-491) and filter_adj (adjoint_source.f90:92-114).
+ *      obs, syn, adj: (NDIM,nrec,NSTEP) = ntrace traces sample-major; windows(2,nwin) in seconds;
+ *      sos(6,nsect) = b0 b1 b2 1 a1 a2 of the bandpass sections (bandpass_dp's Butterworth,
+ *      signal.f90:7-36), nsect = 0 for no filter; misfits(ntrace) = per-trace misfit summed over
+ *      windows; residuals(nwin,ntrace) or NULL */
+void fwat_waveform_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
+                                  double *windows, double *taper_percentage, int *itaper_type, int *nsect,
+                                  realw *sos, double *misfits, double *residuals, realw *adj);
+/* device time (ms) of the last fwat_waveform_adjoint_batch_, copies excluded */
+void fwat_waveform_adjoint_last_time_(double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,8 @@ Function-to-reference map (file:line in the SpecFWAT tree):
                                    src/shared/projection_on_FD_grid_mod_fwat.f90:186-596, 935-1125
   taper_grid                       src/postproc/post_processing.f90:296-333,
                                    src/postproc/taper_3D_mod.f90:16-141
+  bandpass_design / _apply,        src/shared/signal.f90:7-36 (SAC xapiir, Butterworth bandpass),
+  filter_adj                       src/preproc/adjoint_sources/adjoint_source.f90:92-114
   sd / lbfgs / cg_direction        src/optimization/optimize_grid.f90:229-389,
                                    src/shared/utils.f90:1527-1534
   model_update                     src/optimization/optimize_grid.f90:176-203, 391-404
@@ -60,7 +62,8 @@ class OrcParams(C.Structure):
 
 def build(force=False):
     """Compile the oracle with g++ (OpenMP, portable x86-64-v3 code)."""
-    srcs = [os.path.join(_HERE, n) for n in ("sem_oracle.cpp", "post_oracle.cpp", "opt_oracle.cpp")]
+    srcs = [os.path.join(_HERE, n) for n in ("sem_oracle.cpp", "post_oracle.cpp", "opt_oracle.cpp",
+                                              "measure_oracle.cpp")]
     if force or not os.path.exists(_LIB_PATH) or any(os.path.getmtime(_LIB_PATH) < os.path.getmtime(s)
                                                       for s in srcs):
         import subprocess
@@ -122,6 +125,13 @@ def lib():
         L.orc_gll2grid_project.argtypes = [C.c_long, _ip, _dp, _fp, _fp]
         L.orc_taper_grid.restype = C.c_int
         L.orc_taper_grid.argtypes = [C.c_float] * 11 + [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int, _fp]
+        L.orc_bandpass_design.restype = C.c_int
+        L.orc_bandpass_design.argtypes = [C.c_double, C.c_float, C.c_float, _fp, _fp]
+        L.orc_bandpass_design_d.restype = C.c_int
+        L.orc_bandpass_design_d.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.orc_bandpass_apply.argtypes = [_fp, C.c_long, _fp, _fp, C.c_int, C.c_int]
+        L.orc_filter_adj.argtypes = [_dp, C.c_int, C.c_double, C.c_float, C.c_float, _dp, C.c_int, C.c_double,
+                                     C.c_int]
         L.orc_sd_direction.argtypes = [C.c_long, _fp, _fp, _fp, C.c_int]
         L.orc_lbfgs_direction.argtypes = [C.c_long, C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
                                           C.c_float, _fp, _dp, C.c_int]
@@ -472,3 +482,35 @@ def model_update(model, direction, npg, model_type, step, vpvs_range, update_iso
     lib().orc_model_update(npg, m.size // npg, model_type, int(update_iso), int(update_ani), step,
                            vpvs_range[0], vpvs_range[1], _p(m, C.c_float), _p(d, C.c_float))
     return m
+
+
+def bandpass_design(dt, f1, f2):
+    """(sn [2,3], sd [2,3]) float32 second-order sections."""
+    sn = np.zeros(6, np.float32)
+    sd = np.zeros(6, np.float32)
+    ns = lib().orc_bandpass_design(dt, f1, f2, _p(sn, C.c_float), _p(sd, C.c_float))
+    return sn.reshape(-1, 3)[:ns], sd.reshape(-1, 3)[:ns]
+
+
+def bandpass_design_d(dt, f1, f2):
+    sn = np.zeros(6)
+    sd = np.zeros(6)
+    ns = lib().orc_bandpass_design_d(dt, f1, f2, _p(sn, C.c_double), _p(sd, C.c_double))
+    return sn.reshape(-1, 3)[:ns], sd.reshape(-1, 3)[:ns]
+
+
+def bandpass_apply(x, sn, sd, zero_phase=True):
+    x = np.array(x, np.float32)
+    sn = np.ascontiguousarray(sn, np.float32)
+    sd = np.ascontiguousarray(sd, np.float32)
+    lib().orc_bandpass_apply(_p(x, C.c_float), x.size, _p(sn, C.c_float), _p(sd, C.c_float), sn.shape[0],
+                             int(zero_phase))
+    return x
+
+
+def filter_adj(adj, dt, f1, f2, windows, taper_pct=0.05, itaper=1):
+    a = np.array(adj, np.float64)
+    win = np.ascontiguousarray(np.reshape(windows, (-1, 2)), np.float64)
+    lib().orc_filter_adj(_p(a, C.c_double), a.size, dt, f1, f2, _p(win, C.c_double), win.shape[0], taper_pct,
+                         itaper)
+    return a

@@ -96,6 +96,8 @@ SIGNATURES = {
     "fwat_opt_cg_direction_": "llFFFFd",
     "fwat_opt_model_update_": "liiiifFFF",
     "fwat_opt_last_time_": "d",
+    "fwat_waveform_adjoint_batch_": "iidFFiDdiiFDDF",
+    "fwat_waveform_adjoint_last_time_": "d",
 }
 
 _lib = None

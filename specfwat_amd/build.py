@@ -3,7 +3,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["prepare.hip", "forces.hip", "timeloop.hip", "brick.hip", "smooth_pde.hip", "batch.hip", "postproc.hip", "optimize.hip"]
+SOURCES = ["prepare.hip", "forces.hip", "timeloop.hip", "brick.hip", "smooth_pde.hip", "batch.hip", "postproc.hip", "optimize.hip", "measure.hip"]
 OUT = os.path.join(HERE, "libspecfwat_hip.so")
 
 HIPCC_FLAGS = [

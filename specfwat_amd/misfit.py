@@ -1,11 +1,19 @@
-"""Waveform-difference misfit and adjoint source in double precision.
+"""Waveform-difference misfit and adjoint sources.
 
-Restates WaveformMisfit%calc_adjoint_source
-(src/preproc/adjoint_sources/waveform_misfit.f90:15-64) with its helpers
-get_window_info / window_taper (src/shared/signal.f90:419-508) and simpson
-(src/shared/utils.f90:1630-1663).  Host FP64 as in the reference.
+WaveformMisfit%calc_adjoint_source (src/preproc/adjoint_sources/waveform_misfit.f90:
+15-64) with get_window_info / window_taper (src/shared/signal.f90:419-508) and
+simpson (src/shared/utils.f90:1630-1663), then filter_adj
+(src/preproc/adjoint_sources/adjoint_source.f90:92-114: bandpass_dp's zero-phase
+Butterworth bandpass, signal.f90:7-36, and a taper over the windows' span).
+
+event_adjoint_sources measures every trace of an event in one device pass
+(fwat_waveform_adjoint_batch_; double-precision misfits as the reference,
+single-precision filter as SAC's xapiir).  The single-trace functions below are
+the host statement of the same arithmetic.
 """
 import numpy as np
+
+from . import _lib
 
 
 def simpson(y, dx):
@@ -82,18 +90,63 @@ def waveform_adjoint_source(dat, syn, dt, windows, taper_percentage=0.05, itaper
     return total, adj, misfits, residuals
 
 
-def event_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, itaper_type=1):
-    """obs/syn (NSTEP, nrec, 3) -> (misfit, adjoint traces float32 (NSTEP, nrec, 3)).
-    Default window: the whole trace."""
-    nstep, nrec, _ = syn.shape
+IORD = 2  # config.f90:76
+
+
+def bandpass_sos(dt, f1, f2):
+    """Second-order sections [nsect, 6] (b0 b1 b2 1 a1 a2, single precision) of
+    the filter bandpass_dp asks of SAC's xapiir: Butterworth prototype of
+    order IORD, band edges prewarped to w = tan(pi f dt), lowpass-to-bandpass
+    s -> (s^2 + w0^2) / (B s), bilinear s = (1 - 1/z) / (1 + 1/z); one section
+    per bandpass pole pair.  Coefficients are formed in double."""
+    wl, wh = np.tan(np.pi * float(f1) * dt), np.tan(np.pi * float(f2) * dt)
+    if not 0.0 < wl < wh:
+        raise ValueError("bandpass edges must satisfy 0 < f1 < f2 < Nyquist")
+    w0sq, B = wl * wh, wh - wl
+    rows = []
+    for k in range(IORD // 2):
+        p = np.exp(1j * np.pi * (2 * k + IORD + 1) / (2 * IORD))  # upper prototype pole
+        disc = np.sqrt(p * p * B * B - 4.0 * w0sq + 0j)
+        for q in ((p * B + disc) / 2.0, (p * B - disc) / 2.0):
+            a1, a0 = -2.0 * q.real, abs(q) ** 2
+            D = 1.0 + a1 + a0
+            rows.append([B / D, 0.0, -B / D, 1.0, (2.0 * a0 - 2.0) / D, (1.0 - a1 + a0) / D])
+    return np.array(rows, np.float32)
+
+
+def event_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, itaper_type=1,
+                          min_period=None, max_period=None):
+    """obs/syn (NSTEP, nrec, 3) -> (misfit, adjoint traces float32 (NSTEP, nrec, 3)),
+    every trace in one device pass.  Default window: the whole trace.  With
+    min_period and max_period the adjoint sources go through filter_adj (band
+    1/max_period .. 1/min_period, the reference's cfg%min_period / max_period)."""
+    nstep = syn.shape[0]
     if windows is None:
         windows = [[0.0, nstep * dt]]
-    adj = np.zeros(syn.shape, np.float32)
+    win = np.ascontiguousarray(np.reshape(np.asarray(windows, np.float64), (-1, 2)))
+    o = np.ascontiguousarray(obs, np.float32)
+    s = np.ascontiguousarray(syn, np.float32)
+    if o.shape != s.shape:
+        raise ValueError("observed and synthetic traces differ in shape")
+    ntrace = s.size // nstep
+    for w0, w1 in win:
+        nb, ne, _ = get_window_info((w0, w1), dt)
+        if nb < 1 or ne > nstep:
+            raise ValueError(f"measurement window [{w0}, {w1}] s lies outside the {nstep}-sample trace")
+    if (min_period is None) != (max_period is None):
+        raise ValueError("give both min_period and max_period, or neither")
+    sos = (bandpass_sos(dt, np.float32(1.0 / max_period), np.float32(1.0 / min_period))
+           if min_period is not None else np.zeros((0, 6), np.float32))
+    adj = np.zeros(s.shape, np.float32)
+    mf = np.zeros(ntrace)
+    _lib.call("fwat_waveform_adjoint_batch_", nstep, ntrace, float(dt), o, s, win.shape[0], win,
+              float(taper_percentage), int(itaper_type), sos.shape[0], sos if sos.size else np.zeros(6, np.float32),
+              mf, None, adj)
     total = 0.0
-    for r in range(nrec):
-        for c in range(3):
-            mf, a, _, _ = waveform_adjoint_source(obs[:, r, c], syn[:, r, c], dt, windows,
-                                                  taper_percentage, itaper_type)
-            total += mf
-            adj[:, r, c] = a
+    for v in mf:  # trace order, as the reference's receiver/component loop
+        total += v
     return total, adj
+
+
+def last_time_ms():
+    return float(_lib.call("fwat_waveform_adjoint_last_time_", 0.0)[0])

@@ -207,3 +207,12 @@ def test_event_batch_refuses_unsupported_shapes():
     mi = BoxMesh(2, 2, 2, 2 * h, 2 * h, 2 * h, force_irregular=True)
     with pytest.raises(ValueError, match="regular"):
         EventBatch(mi, rho, vp, vs, [src] * 2, None, 0.01, [10, 10])
+
+
+def test_measurement_rejects_windows_outside_the_trace():
+    from specfwat_amd.misfit import event_adjoint_sources
+    obs = np.zeros((100, 2, 3), np.float32)
+    with pytest.raises(ValueError):
+        event_adjoint_sources(obs, obs, 0.01, [[0.5, 2.0]])
+    with pytest.raises(ValueError):
+        event_adjoint_sources(obs, obs, 0.01, None, min_period=1.0)
