@@ -318,8 +318,16 @@ void fwat_opt_last_time_(double *ms);
 void fwat_waveform_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
                                   double *windows, double *taper_percentage, int *itaper_type, int *nsect,
                                   realw *sos, double *misfits, double *residuals, realw *adj);
-/* device time (ms) of the last fwat_waveform_adjoint_batch_, copies excluded */
+/* device time (ms) of the last fwat_waveform_adjoint_batch_ or fwat_cc_adjoint_batch_, copies excluded */
 void fwat_waveform_adjoint_last_time_(double *ms);
+/* cross-correlation traveltime (imeas 11) / amplitude (12) measurement of every trace: CCTTMisfit%
+ * calc_adjoint_source (src/preproc/adjoint_sources/cc_tt_misfit.f90:17-146 with utils/cc.f90:10-160)
+ * and filter_adj; cfg(7) = dt_sigma_min, dlna_sigma_min, TSHIFT_MIN, TSHIFT_MAX, DLNA_MIN, DLNA_MAX,
+ * CC_MIN (adj_config.f90:18-23); meas(8,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
+ * selected, ishift (or NULL) */
+void fwat_cc_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
+                            double *windows, double *taper_percentage, int *itaper_type, int *imeas,
+                            double *cfg, int *nsect, realw *sos, double *misfits, double *meas, realw *adj);
 
 #ifdef __cplusplus
 }

@@ -20,12 +20,46 @@
 //   orc_filter_adj        filter_adj, src/preproc/adjoint_sources/adjoint_source.f90:
 //                         92-114: bandpass 1/max_period .. 1/min_period, then taper
 //                         over [min window start, max window end], zero outside
+//   orc_cc_adjoint        CCTTMisfit%calc_adjoint_source, src/preproc/adjoint_sources/
+//                         cc_tt_misfit.f90:17-85 with cross_correlate (utils/cc.f90:
+//                         10-160: calc_cc_shift, calc_cc_error, cc_correction,
+//                         xcorr_shift over mycorrelation_dp / myconvolution_dp,
+//                         signal.f90:146-220, cc_max_coef, calc_cc_adjsrc), gradient
+//                         (utils.f90:1613-1628) and cc_measure_select (cc_tt_misfit.f90:
+//                         126-146); double precision, loops in the reference's order
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
 
 extern "C" void orc_window_taper(double* sig, int npts, double pct, int itype);
+extern "C" double orc_simpson(const double* y, int n, double dx);
+
+namespace {
+
+// gradient (utils.f90:1613-1628): central differences, one-sided at the ends
+std::vector<double> gradient(const std::vector<double>& y, double dx) {
+  const int n = (int)y.size();
+  std::vector<double> g(n, 0.0);
+  for (int i = 1; i < n - 1; ++i) g[i] = (y[i + 1] - y[i - 1]) / (2.0 * dx);
+  if (n >= 2) { g[0] = (y[1] - y[0]) / dx; g[n - 1] = (y[n - 1] - y[n - 2]) / dx; }
+  return g;
+}
+
+// xcorr_shift: full correlation of d with s, first maximum, lag = index - n
+int xcorr_shift(const std::vector<double>& d, const std::vector<double>& s) {
+  const int n = (int)d.size();
+  std::vector<double> cc(2 * n - 1, 0.0);
+  // myconvolution_dp(d, flip(s)): conv(i1 + i2 - 1) += d(i1) * s(n - i2 + 1), i1 outer
+  for (int i1 = 0; i1 < n; ++i1)
+    for (int i2 = 0; i2 < n; ++i2) cc[i1 + i2] = cc[i1 + i2] + d[i1] * s[n - 1 - i2];
+  int best = 0;
+  for (int m = 1; m < 2 * n - 1; ++m)
+    if (cc[m] > cc[best]) best = m;
+  return best + 1 - n;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -95,6 +129,86 @@ void orc_filter_adj(double* adj, int n, double dt, float f1, float f2, const dou
   orc_window_taper(tw.data(), nl, taper_pct, itaper);
   for (int i = 0; i < n; ++i) adj[i] = 0.0;
   for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = tw[i];
+}
+
+// Cross-correlation traveltime (imeas 11) or amplitude (12) measurement of one
+// trace: cfg = {dt_sigma_min, dlna_sigma_min, tshift_min, tshift_max, dlna_min,
+// dlna_max, cc_min}; per window out: misfit, residual (tshift or dlna), error,
+// cc_max, tshift, dlna, selected.  Returns the total misfit.
+double orc_cc_adjoint(const double* dat, const double* syn, int n, double dt, const double* windows, int nwin,
+                      double taper_pct, int itaper, int imeas, const double* cfg, double* adj, double* misfits,
+                      double* residuals, double* errors, double* ccmax, double* tshift_o, double* dlna_o,
+                      int* selected) {
+  double total = 0.0;
+  for (int i = 0; i < n; ++i) adj[i] = 0.0;
+  for (int w = 0; w < nwin; ++w) {
+    const int nb = (int)(windows[2 * w] / dt) + 1, ne = (int)(windows[2 * w + 1] / dt);
+    const int nl = ne - nb + 1;
+    std::vector<double> sv(syn + nb - 1, syn + ne), dv(dat + nb - 1, dat + ne);
+    orc_window_taper(sv.data(), nl, taper_pct, itaper);
+    orc_window_taper(dv.data(), nl, taper_pct, itaper);
+    // calc_cc_shift
+    const int ishift = xcorr_shift(dv, sv);
+    const double tshift = ishift * dt;
+    double sd2 = 0.0, ss2 = 0.0;
+    for (int i = 0; i < nl; ++i) sd2 = sd2 + dv[i] * dv[i];
+    for (int i = 0; i < nl; ++i) ss2 = ss2 + sv[i] * sv[i];
+    const double dlna = 0.5 * std::log(sd2 / ss2);
+    // calc_cc_error with cc_correction
+    std::vector<double> scc(nl, 0.0), sccd(nl, 0.0);
+    for (int i = 0; i < nl; ++i) {
+      const int j = i - ishift;
+      if (j >= 0 && j < nl) { scc[i] = sv[j]; sccd[i] = std::exp(dlna) * sv[j]; }
+    }
+    const std::vector<double> vel = gradient(sccd, dt);
+    double num = 0.0, den = 0.0, den2 = 0.0;
+    for (int i = 0; i < nl; ++i) num = num + (dv[i] - sccd[i]) * (dv[i] - sccd[i]);
+    for (int i = 0; i < nl; ++i) den = den + vel[i] * vel[i];
+    for (int i = 0; i < nl; ++i) den2 = den2 + scc[i] * scc[i];
+    double sig_dt = std::sqrt(num / den), sig_dlna = std::sqrt(num / den2);
+    if (sig_dt < cfg[0] || std::isnan(sig_dt)) sig_dt = cfg[0];
+    if (sig_dlna < cfg[1] || std::isnan(sig_dlna)) sig_dlna = cfg[1];
+    // cc_max_coef (same shift; d's edges zeroed where the shifted s is empty)
+    std::vector<double> dsh(dv);
+    if (ishift > 0) for (int i = 0; i < ishift && i < nl; ++i) dsh[i] = 0.0;
+    else if (ishift < 0) for (int i = nl + ishift; i < nl; ++i) if (i >= 0) dsh[i] = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = 0; i < nl; ++i) a = a + dsh[i] * scc[i];
+    for (int i = 0; i < nl; ++i) b = b + dsh[i] * dsh[i];
+    for (int i = 0; i < nl; ++i) c = c + scc[i] * scc[i];
+    const double cmax = a / std::sqrt(b * c);
+    ccmax[w] = cmax;
+    tshift_o[w] = tshift;
+    dlna_o[w] = dlna;
+    const bool sel = !(cmax < cfg[6] || tshift < cfg[2] || tshift > cfg[3] || dlna < cfg[4] || dlna > cfg[5]);
+    selected[w] = sel;
+    residuals[w] = imeas == 11 ? tshift : dlna;
+    errors[w] = imeas == 11 ? sig_dt : sig_dlna;
+    misfits[w] = 0.0;
+    if (!sel) continue;
+    // calc_cc_adjsrc
+    const double mp = 0.5 * (tshift / sig_dt) * (tshift / sig_dt);
+    const double mq = 0.5 * (dlna / sig_dlna) * (dlna / sig_dlna);
+    std::vector<double> out(nl);
+    if (imeas == 11) {
+      const std::vector<double> dsdt = gradient(sv, dt);
+      std::vector<double> sq(nl);
+      for (int i = 0; i < nl; ++i) sq[i] = dsdt[i] * dsdt[i];
+      const double nn = orc_simpson(sq.data(), nl, dt);
+      for (int i = 0; i < nl; ++i) out[i] = dsdt[i] * tshift / nn / (sig_dt * sig_dt);
+    } else {
+      std::vector<double> sq(nl);
+      for (int i = 0; i < nl; ++i) sq[i] = sv[i] * sv[i];
+      const double nn = orc_simpson(sq.data(), nl, dt);
+      for (int i = 0; i < nl; ++i) out[i] = -1.0 * sv[i] * dlna / nn / (sig_dlna * sig_dlna);
+    }
+    orc_window_taper(out.data(), nl, taper_pct, itaper);
+    const double mf = imeas == 11 ? mp : mq;
+    misfits[w] = mf;
+    total = total + mf;
+    for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = out[i];
+  }
+  return total;
 }
 
 }  // extern "C"

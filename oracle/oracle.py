@@ -18,6 +18,7 @@ Function-to-reference map (file:line in the SpecFWAT tree):
                                    src/postproc/taper_3D_mod.f90:16-141
   bandpass_design / _apply,        src/shared/signal.f90:7-36 (SAC xapiir, Butterworth bandpass),
   filter_adj                       src/preproc/adjoint_sources/adjoint_source.f90:92-114
+  cc_adjoint                       src/preproc/adjoint_sources/cc_tt_misfit.f90:17-146, utils/cc.f90
   sd / lbfgs / cg_direction        src/optimization/optimize_grid.f90:229-389,
                                    src/shared/utils.f90:1527-1534
   model_update                     src/optimization/optimize_grid.f90:176-203, 391-404
@@ -132,6 +133,9 @@ def lib():
         L.orc_bandpass_apply.argtypes = [_fp, C.c_long, _fp, _fp, C.c_int, C.c_int]
         L.orc_filter_adj.argtypes = [_dp, C.c_int, C.c_double, C.c_float, C.c_float, _dp, C.c_int, C.c_double,
                                      C.c_int]
+        L.orc_cc_adjoint.restype = C.c_double
+        L.orc_cc_adjoint.argtypes = ([_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                      _dp] + [_dp] * 7 + [_ip])
         L.orc_sd_direction.argtypes = [C.c_long, _fp, _fp, _fp, C.c_int]
         L.orc_lbfgs_direction.argtypes = [C.c_long, C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
                                           C.c_float, _fp, _dp, C.c_int]
@@ -514,3 +518,28 @@ def filter_adj(adj, dt, f1, f2, windows, taper_pct=0.05, itaper=1):
     lib().orc_filter_adj(_p(a, C.c_double), a.size, dt, f1, f2, _p(win, C.c_double), win.shape[0], taper_pct,
                          itaper)
     return a
+
+
+CC_DEFAULTS = dict(dt_sigma_min=1.0, dlna_sigma_min=0.5, tshift_min=-5.0, tshift_max=5.0, dlna_min=-1.5,
+                   dlna_max=1.5, cc_min=0.7)  # adj_config.f90:18-23
+
+
+def cc_adjoint(dat, syn, dt, windows, imeas=11, taper_pct=0.05, itaper=1, **cfg):
+    """One trace's CC traveltime (11) or amplitude (12) measurement: (total,
+    adj, per-window dict of misfit, residual, error, cc_max, tshift, dlna, selected)."""
+    c = dict(CC_DEFAULTS, **cfg)
+    cv = np.array([c["dt_sigma_min"], c["dlna_sigma_min"], c["tshift_min"], c["tshift_max"], c["dlna_min"],
+                   c["dlna_max"], c["cc_min"]], np.float64)
+    dat = np.ascontiguousarray(dat, np.float64)
+    syn = np.ascontiguousarray(syn, np.float64)
+    win = np.ascontiguousarray(np.reshape(windows, (-1, 2)), np.float64)
+    nw = win.shape[0]
+    adj = np.zeros(len(dat))
+    o = {k: np.zeros(nw) for k in ("misfit", "residual", "error", "cc_max", "tshift", "dlna")}
+    sel = np.zeros(nw, np.int32)
+    tot = lib().orc_cc_adjoint(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt, _p(win, C.c_double), nw,
+                               taper_pct, itaper, imeas, _p(cv, C.c_double), _p(adj, C.c_double),
+                               *[_p(o[k], C.c_double) for k in ("misfit", "residual", "error", "cc_max", "tshift",
+                                                                 "dlna")], _p(sel, C.c_int))
+    o["selected"] = sel.astype(bool)
+    return tot, adj, o

@@ -98,6 +98,7 @@ SIGNATURES = {
     "fwat_opt_last_time_": "d",
     "fwat_waveform_adjoint_batch_": "iidFFiDdiiFDDF",
     "fwat_waveform_adjoint_last_time_": "d",
+    "fwat_cc_adjoint_batch_": "iidFFiDdiiDiFDDF",
 }
 
 _lib = None

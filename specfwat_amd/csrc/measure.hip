@@ -12,6 +12,7 @@
 // designs (zero phase: forward, then reversed, direct form I as SAC's APPLY)
 // and tapered over [first window start, last window end].  The per-index taper
 // weights are the same for every trace and come precomputed in double.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -103,6 +104,164 @@ __global__ void k_window_final(int nstep, int ntrace, int nb, int nl, const doub
   if (i >= (size_t)nstep * ntrace) return;
   const int k = (int)(i / ntrace) - nb;
   x[i] = (k >= 0 && k < nl) ? (float)((double)x[i] * tw[k]) : 0.0f;
+}
+// ---- cross-correlation family (cc_tt_misfit.f90, utils/cc.f90)
+
+// tapered window of every trace, transposed to trace-major rows [ntrace][nl]
+// through 64 x 64 LDS tiles (reads and writes both coalesced)
+__global__ void __launch_bounds__(256) k_cc_rows(int ntrace, int nb, int nl, const double* __restrict__ tw,
+                                                 const float* __restrict__ obs, const float* __restrict__ syn,
+                                                 double* __restrict__ D, double* __restrict__ S) {
+  __shared__ double sd[64][65], ss[64][65];
+  const int t0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int k = k0 + r, t = t0 + (threadIdx.x & 63);
+    if (k < nl && t < ntrace) {
+      const size_t a = (size_t)(nb + k) * ntrace + t;
+      sd[r][threadIdx.x & 63] = (double)obs[a] * tw[k];
+      ss[r][threadIdx.x & 63] = (double)syn[a] * tw[k];
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int t = t0 + r, k = k0 + (threadIdx.x & 63);
+    if (k < nl && t < ntrace) {
+      D[(size_t)t * nl + k] = sd[threadIdx.x & 63][r];
+      S[(size_t)t * nl + k] = ss[threadIdx.x & 63][r];
+    }
+  }
+}
+
+// xcorr_shift: one workgroup per trace, lanes over the 2 nl - 1 lags of the
+// full correlation; each sum runs over i1 in increasing order as in
+// myconvolution_dp; the first maximum wins (maxloc)
+__global__ void __launch_bounds__(256) k_cc_shift(int ntrace, int nl, const double* __restrict__ D,
+                                                  const double* __restrict__ S, int* __restrict__ ishift) {
+  const int t = blockIdx.x;
+  if (t >= ntrace) return;
+  const double* d = D + (size_t)t * nl;
+  const double* s = S + (size_t)t * nl;
+  double best = -INFINITY;
+  int bm = 0x7fffffff;
+  for (int m = threadIdx.x; m < 2 * nl - 1; m += blockDim.x) {
+    const int lo = m - (nl - 1) > 0 ? m - (nl - 1) : 0, hi = m < nl - 1 ? m : nl - 1;
+    double c = 0.0;
+    for (int i1 = lo; i1 <= hi; ++i1) c = c + d[i1] * s[nl - 1 - (m - i1)];
+    if (c > best) { best = c; bm = m; }
+  }
+  __shared__ double sv[256];
+  __shared__ int sm[256];
+  sv[threadIdx.x] = best;
+  sm[threadIdx.x] = bm;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const double v2 = sv[threadIdx.x + o];
+      const int m2 = sm[threadIdx.x + o];
+      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && m2 < sm[threadIdx.x])) {
+        sv[threadIdx.x] = v2;
+        sm[threadIdx.x] = m2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ishift[t] = sm[0] + 1 - nl;
+}
+
+struct CcCfg {
+  double dt_sigma_min, dlna_sigma_min, tshift_min, tshift_max, dlna_min, dlna_max, cc_min;
+};
+
+// per trace: dlna, calc_cc_error (cc_correction + gradient), cc_max_coef,
+// cc_measure_select and calc_cc_adjsrc; window samples read on the fly from
+// the sample-major traces; out[8] per (window, trace): misfit, residual,
+// error, cc_max, tshift, dlna, selected, ishift
+__global__ void __launch_bounds__(MTPB) k_cc_finish(int ntrace, double dt, int nb, int nl, int imeas, CcCfg cfg,
+                                                    const double* __restrict__ tw, const float* __restrict__ obs,
+                                                    const float* __restrict__ syn, const int* __restrict__ ishift_in,
+                                                    float* __restrict__ adj, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrace) return;
+  const size_t T = (size_t)ntrace;
+  auto sv = [&](int k) { return (double)syn[(size_t)(nb + k) * T + t] * tw[k]; };
+  auto dv = [&](int k) { return (double)obs[(size_t)(nb + k) * T + t] * tw[k]; };
+  const int ish = ishift_in[t];
+  const double tshift = ish * dt;
+  double sd2 = 0.0, ss2 = 0.0;
+  for (int i = 0; i < nl; ++i) { const double x = dv(i); sd2 = sd2 + x * x; }
+  for (int i = 0; i < nl; ++i) { const double x = sv(i); ss2 = ss2 + x * x; }
+  const double dlna = 0.5 * log(sd2 / ss2);
+  const double ea = exp(dlna);
+  auto scc = [&](int i) { const int j = i - ish; return (j >= 0 && j < nl) ? sv(j) : 0.0; };
+  auto sccd = [&](int i) { const int j = i - ish; return (j >= 0 && j < nl) ? ea * sv(j) : 0.0; };
+  auto vel = [&](int i) {
+    if (nl < 2) return 0.0;
+    if (i == 0) return (sccd(1) - sccd(0)) / dt;
+    if (i == nl - 1) return (sccd(nl - 1) - sccd(nl - 2)) / dt;
+    return (sccd(i + 1) - sccd(i - 1)) / (2.0 * dt);
+  };
+  double num = 0.0, den = 0.0, den2 = 0.0;
+  for (int i = 0; i < nl; ++i) { const double r = dv(i) - sccd(i); num = num + r * r; }
+  for (int i = 0; i < nl; ++i) { const double v = vel(i); den = den + v * v; }
+  for (int i = 0; i < nl; ++i) { const double c = scc(i); den2 = den2 + c * c; }
+  double sig_dt = sqrt(num / den), sig_dlna = sqrt(num / den2);
+  if (sig_dt < cfg.dt_sigma_min || isnan(sig_dt)) sig_dt = cfg.dt_sigma_min;
+  if (sig_dlna < cfg.dlna_sigma_min || isnan(sig_dlna)) sig_dlna = cfg.dlna_sigma_min;
+  // cc_max_coef: d with the edge the shifted synthetic leaves empty zeroed
+  auto dsh = [&](int i) {
+    if (ish > 0 && i < ish) return 0.0;
+    if (ish < 0 && i >= nl + ish) return 0.0;
+    return dv(i);
+  };
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = 0; i < nl; ++i) a = a + dsh(i) * scc(i);
+  for (int i = 0; i < nl; ++i) { const double x = dsh(i); b = b + x * x; }
+  for (int i = 0; i < nl; ++i) { const double x = scc(i); c = c + x * x; }
+  const double cmax = a / sqrt(b * c);
+  const bool sel = !(cmax < cfg.cc_min || tshift < cfg.tshift_min || tshift > cfg.tshift_max ||
+                     dlna < cfg.dlna_min || dlna > cfg.dlna_max);
+  double* o = out + 8 * (size_t)t;
+  o[1] = imeas == 11 ? tshift : dlna;
+  o[2] = imeas == 11 ? sig_dt : sig_dlna;
+  o[3] = cmax;
+  o[4] = tshift;
+  o[5] = dlna;
+  o[6] = sel ? 1.0 : 0.0;
+  o[7] = (double)ish;
+  o[0] = 0.0;
+  if (!sel) return;
+  // calc_cc_adjsrc on the tapered synthetic, then the window taper
+  auto dsdt = [&](int i) {
+    if (nl < 2) return 0.0;
+    if (i == 0) return (sv(1) - sv(0)) / dt;
+    if (i == nl - 1) return (sv(nl - 1) - sv(nl - 2)) / dt;
+    return (sv(i + 1) - sv(i - 1)) / (2.0 * dt);
+  };
+  auto yq = [&](int i) { const double x = imeas == 11 ? dsdt(i) : sv(i); return x * x; };
+  double nn = 0.0;  // simpson (utils.f90:1630-1663)
+  if (nl >= 2) {
+    if (nl % 2 == 0) {
+      double sum = yq(0) + yq(nl - 2);
+      for (int k = 1; k <= nl - 3; k += 2) sum = sum + 4.0 * yq(k) + 2.0 * yq(k + 1);
+      nn = (dt / 3.0) * sum + 0.5 * dt * (yq(nl - 2) + yq(nl - 1));
+    } else {
+      double sum = yq(0) + yq(nl - 1);
+      for (int k = 1; k <= nl - 2; k += 2) sum = sum + 4.0 * yq(k);
+      for (int k = 2; k <= nl - 3; k += 2) sum = sum + 2.0 * yq(k);
+      nn = (dt / 3.0) * sum;
+    }
+  }
+  for (int i = 0; i < nl; ++i) {
+    const double v = imeas == 11 ? dsdt(i) * tshift / nn / (sig_dt * sig_dt)
+                                 : -1.0 * sv(i) * dlna / nn / (sig_dlna * sig_dlna);
+    adj[(size_t)(nb + i) * T + t] = (float)(v * tw[i]);
+  }
+  o[0] = imeas == 11 ? 0.5 * (tshift / sig_dt) * (tshift / sig_dt) : 0.5 * (dlna / sig_dlna) * (dlna / sig_dlna);
+}
+
+__global__ void k_fill_f32m(size_t n, float* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0f;
 }
 #pragma clang fp contract(fast)
 
@@ -215,5 +374,94 @@ void fwat_waveform_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* ob
 }
 
 void fwat_waveform_adjoint_last_time_(double* ms) { *ms = g_meas_ms; }
+
+// cc_tt_misfit.f90 CCTTMisfit%calc_adjoint_source for every trace: imeas 11
+// (traveltime) or 12 (amplitude); cfg(7) = dt_sigma_min, dlna_sigma_min,
+// TSHIFT_MIN, TSHIFT_MAX, DLNA_MIN, DLNA_MAX, CC_MIN; meas(8, nwin, ntrace) =
+// misfit, residual, error, cc_max, tshift, dlna, selected, ishift; misfits
+// (ntrace) summed over windows; sos / nsect as for the waveform measurement
+void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw* syn, int* nwin, double* windows,
+                            double* taper_percentage, int* itaper_type, int* imeas, double* cfg, int* nsect,
+                            realw* sos, double* misfits, double* meas, realw* adj) {
+  const int N = *nstep, T = *ntrace, nw = *nwin;
+  if (N <= 0 || T <= 0) SEM_FAIL("fwat_cc_adjoint_batch_: sizes must be positive");
+  if (nw < 1 || nw > MAX_WIN) SEM_FAIL("fwat_cc_adjoint_batch_: 1..%d windows", MAX_WIN);
+  if (*imeas != 11 && *imeas != 12) SEM_FAIL("fwat_cc_adjoint_batch_: imeas must be 11 (CC-TT) or 12 (CC-DLNA)");
+  if (*nsect < 0 || *nsect > 8) SEM_FAIL("fwat_cc_adjoint_batch_: 0..8 filter sections");
+  if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
+  const CcCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6]};
+  const size_t n = (size_t)N * T;
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  float* d_obs = dev_upload(obs, n, s);
+  float* d_syn = dev_upload(syn, n, s);
+  float* d_adj = dev_alloc<float>(n);
+  int* d_ish = dev_alloc<int>(T);
+  double* d_out = dev_alloc<double>(8 * (size_t)nw * T);
+  int maxnl = 0;
+  std::vector<int> nbs(nw), nls(nw);
+  std::vector<std::vector<double>> tws(nw);
+  double wmin = windows[0], wmax = windows[0];
+  for (int w = 0; w < nw; ++w) {
+    window_range(windows[2 * w], windows[2 * w + 1], *dt, N, nbs[w], nls[w]);
+    taper_weights(nls[w], *taper_percentage, *itaper_type, tws[w]);
+    maxnl = std::max(maxnl, nls[w]);
+    wmin = std::min(wmin, std::min(windows[2 * w], windows[2 * w + 1]));
+    wmax = std::max(wmax, std::max(windows[2 * w], windows[2 * w + 1]));
+  }
+  double* d_D = dev_alloc<double>((size_t)T * maxnl);
+  double* d_S = dev_alloc<double>((size_t)T * maxnl);
+  double* d_tw = dev_alloc<double>(maxnl);
+  HIP_CHECK(hipEventRecord(e0, s));
+  hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_adj);
+  const unsigned nbk = (unsigned)((T + MTPB - 1) / MTPB);
+  for (int w = 0; w < nw; ++w) {
+    HIP_CHECK(hipMemcpyAsync(d_tw, tws[w].data(), nls[w] * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cc_rows, dim3((unsigned)((T + 63) / 64), (unsigned)((nls[w] + 63) / 64)), dim3(256), 0, s, T,
+                       nbs[w], nls[w], d_tw, d_obs, d_syn, d_D, d_S);
+    hipLaunchKernelGGL(k_cc_shift, dim3((unsigned)T), dim3(256), 0, s, T, nls[w], d_D, d_S, d_ish);
+    hipLaunchKernelGGL(k_cc_finish, dim3(nbk), dim3(MTPB), 0, s, T, *dt, nbs[w], nls[w], *imeas, C, d_tw, d_obs, d_syn,
+                       d_ish, d_adj, d_out + 8 * (size_t)w * T);
+  }
+  std::vector<double> fw;
+  double* d_fw = nullptr;
+  float* d_sos = nullptr;
+  if (*nsect > 0) {
+    int fnb = 0, fnl = 0;
+    window_range(wmin, wmax, *dt, N, fnb, fnl);
+    taper_weights(fnl, *taper_percentage, *itaper_type, fw);
+    d_fw = dev_upload(fw.data(), fw.size(), s);
+    d_sos = dev_upload(sos, 6 * (size_t)*nsect, s);
+    hipLaunchKernelGGL(k_sos_filter, dim3(nbk), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
+                       d_adj);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(e1, s));
+  std::vector<double> out(8 * (size_t)nw * T);
+  HIP_CHECK(hipMemcpyAsync(adj, d_adj, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  g_meas_ms = ms;
+  for (int t = 0; t < T; ++t) {
+    double m = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      const double* o = &out[8 * ((size_t)w * T + t)];
+      m += o[0];
+      if (meas) for (int q = 0; q < 8; ++q) meas[8 * ((size_t)t * nw + w) + q] = o[q];
+    }
+    misfits[t] = m;
+  }
+  dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_ish); dev_free(d_out); dev_free(d_D); dev_free(d_S);
+  dev_free(d_tw); dev_free(d_fw); dev_free(d_sos);
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipStreamDestroy(s));
+}
 
 }  // extern "C"

@@ -120,23 +120,7 @@ def event_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, ita
     every trace in one device pass.  Default window: the whole trace.  With
     min_period and max_period the adjoint sources go through filter_adj (band
     1/max_period .. 1/min_period, the reference's cfg%min_period / max_period)."""
-    nstep = syn.shape[0]
-    if windows is None:
-        windows = [[0.0, nstep * dt]]
-    win = np.ascontiguousarray(np.reshape(np.asarray(windows, np.float64), (-1, 2)))
-    o = np.ascontiguousarray(obs, np.float32)
-    s = np.ascontiguousarray(syn, np.float32)
-    if o.shape != s.shape:
-        raise ValueError("observed and synthetic traces differ in shape")
-    ntrace = s.size // nstep
-    for w0, w1 in win:
-        nb, ne, _ = get_window_info((w0, w1), dt)
-        if nb < 1 or ne > nstep:
-            raise ValueError(f"measurement window [{w0}, {w1}] s lies outside the {nstep}-sample trace")
-    if (min_period is None) != (max_period is None):
-        raise ValueError("give both min_period and max_period, or neither")
-    sos = (bandpass_sos(dt, np.float32(1.0 / max_period), np.float32(1.0 / min_period))
-           if min_period is not None else np.zeros((0, 6), np.float32))
+    nstep, ntrace, win, o, s, sos = _event_inputs(obs, syn, dt, windows, min_period, max_period)
     adj = np.zeros(s.shape, np.float32)
     mf = np.zeros(ntrace)
     _lib.call("fwat_waveform_adjoint_batch_", nstep, ntrace, float(dt), o, s, win.shape[0], win,
@@ -146,6 +130,59 @@ def event_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, ita
     for v in mf:  # trace order, as the reference's receiver/component loop
         total += v
     return total, adj
+
+
+IMEAS_CC_TT, IMEAS_CC_DLNA = 11, 12  # adj_config
+CC_DEFAULTS = dict(dt_sigma_min=1.0, dlna_sigma_min=0.5, tshift_min=-5.0, tshift_max=5.0, dlna_min=-1.5,
+                   dlna_max=1.5, cc_min=0.7)  # ADJConfig defaults, adj_config.f90:18-23
+
+
+def _event_inputs(obs, syn, dt, windows, min_period, max_period):
+    nstep = syn.shape[0]
+    if windows is None:
+        windows = [[0.0, nstep * dt]]
+    win = np.ascontiguousarray(np.reshape(np.asarray(windows, np.float64), (-1, 2)))
+    o = np.ascontiguousarray(obs, np.float32)
+    s = np.ascontiguousarray(syn, np.float32)
+    if o.shape != s.shape:
+        raise ValueError("observed and synthetic traces differ in shape")
+    for w0, w1 in win:
+        nb, ne, _ = get_window_info((w0, w1), dt)
+        if nb < 1 or ne > nstep:
+            raise ValueError(f"measurement window [{w0}, {w1}] s lies outside the {nstep}-sample trace")
+    if (min_period is None) != (max_period is None):
+        raise ValueError("give both min_period and max_period, or neither")
+    sos = (bandpass_sos(dt, np.float32(1.0 / max_period), np.float32(1.0 / min_period))
+           if min_period is not None else np.zeros((0, 6), np.float32))
+    return nstep, s.size // nstep, win, o, s, sos
+
+
+def event_cc_adjoint_sources(obs, syn, dt, windows=None, imeas=IMEAS_CC_TT, taper_percentage=0.05,
+                             itaper_type=1, min_period=None, max_period=None, **cfg):
+    """Cross-correlation traveltime (imeas 11) or amplitude (12) measurement of
+    every trace (CCTTMisfit%calc_adjoint_source) in one device pass.  Returns
+    (misfit, adjoint traces float32 like syn, per-window measurements
+    [nrec*3, nwin, 8]: misfit, residual, error, cc_max, tshift, dlna, selected,
+    ishift).  cfg overrides CC_DEFAULTS."""
+    nstep, ntrace, win, o, s, sos = _event_inputs(obs, syn, dt, windows, min_period, max_period)
+    unknown = set(cfg) - set(CC_DEFAULTS)
+    if unknown:
+        raise KeyError(f"unknown CC settings {sorted(unknown)}")
+    c = dict(CC_DEFAULTS, **cfg)
+    cv = np.array([c[k] for k in ("dt_sigma_min", "dlna_sigma_min", "tshift_min", "tshift_max", "dlna_min",
+                                  "dlna_max", "cc_min")], np.float64)
+    if imeas not in (IMEAS_CC_TT, IMEAS_CC_DLNA):
+        raise ValueError("imeas must be 11 (CC-TT) or 12 (CC-DLNA)")
+    adj = np.zeros(s.shape, np.float32)
+    mf = np.zeros(ntrace)
+    meas = np.zeros((ntrace, win.shape[0], 8))
+    _lib.call("fwat_cc_adjoint_batch_", nstep, ntrace, float(dt), o, s, win.shape[0], win, float(taper_percentage),
+              int(itaper_type), int(imeas), cv, sos.shape[0], sos if sos.size else np.zeros(6, np.float32), mf, meas,
+              adj)
+    total = 0.0
+    for v in mf:
+        total += v
+    return total, adj, meas
 
 
 def last_time_ms():

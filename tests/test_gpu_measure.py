@@ -61,3 +61,36 @@ def test_batch_matches_the_host_statement():
     total, adj = misfit.event_adjoint_sources(obs, syn, dt, [[0.2, 9.0]])
     m, a, _, _ = misfit.waveform_adjoint_source(obs[:, 2, 1], syn[:, 2, 1], dt, [[0.2, 9.0]])
     np.testing.assert_allclose(adj[:, 2, 1], a, rtol=0, atol=1e-6 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("imeas", [11, 12])
+def test_cc_batch_matches_oracle(imeas):
+    rng = np.random.default_rng(imeas)
+    nstep, nrec, dt = 1400, 6, 0.01
+    t = np.arange(nstep) * dt
+    obs = np.zeros((nstep, nrec, 3), np.float32)
+    syn = np.zeros_like(obs)
+    for r in range(nrec):
+        for c in range(3):
+            f, t0 = rng.uniform(0.6, 1.5), rng.uniform(4, 8)
+            w = np.exp(-((t - t0) * f) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+            obs[:, r, c] = w + 0.02 * rng.standard_normal(nstep)
+            syn[:, r, c] = np.roll(w, rng.integers(-40, 40)) * rng.uniform(0.6, 1.4)
+    syn[:, 0, 0] = rng.standard_normal(nstep)  # one trace below CC_MIN
+    win = [[1.0, 7.0], [3.0, 12.5]]
+    total, adj, meas = misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=imeas)
+    tot = 0.0
+    for r in range(nrec):
+        for c in range(3):
+            m, a, o = orc.cc_adjoint(obs[:, r, c], syn[:, r, c], dt, win, imeas=imeas)
+            tot += m
+            q = meas[3 * r + c]
+            np.testing.assert_array_equal(q[:, 6].astype(bool), o["selected"])
+            np.testing.assert_allclose(q[:, 4], o["tshift"], rtol=0, atol=1e-12)
+            np.testing.assert_allclose(q[:, 5], o["dlna"], rtol=1e-12, atol=1e-14)
+            np.testing.assert_allclose(q[:, 2], o["error"], rtol=1e-12)
+            np.testing.assert_allclose(q[:, 3], o["cc_max"], rtol=1e-12)
+            assert np.abs(adj[:, r, c] - a).max() <= 1e-6 * max(np.abs(a).max(), 1e-30)
+    assert not meas[0, :, 6].any() and meas[1:, :, 6].any()
+    assert abs(total - tot) <= 1e-12 * max(tot, 1e-30)
+    print(f"imeas {imeas}: misfit {total:.6e}, selected {int(meas[:, :, 6].sum())} of {meas.shape[0] * 2}")

@@ -67,3 +67,58 @@ def test_filter_adj_window_and_taper():
     sn, sd = O.bandpass_design(dt, np.float32(1 / 20.0), np.float32(1 / 0.5))
     f = O.bandpass_apply(x.astype(np.float32), sn, sd).astype(np.float64)
     np.testing.assert_array_equal(out[nb - 1:ne], O.window_taper(f[nb - 1:ne], 0.05, 1))
+
+
+def _wavelet(n, dt, t0, f):
+    t = np.arange(n) * dt
+    return np.exp(-((t - t0) * f) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+
+
+@pytest.mark.parametrize("k,amp", [(7, 0.8), (-12, 1.3), (0, 1.0)])
+def test_cc_measures_a_pure_shift_and_gain(k, amp):
+    dt, n = 0.01, 1200
+    d = _wavelet(n, dt, 5.0, 1.5)
+    s = np.roll(d, k) * amp  # synthetic delayed by k samples
+    tot, adj, o = O.cc_adjoint(d, s, dt, [[0.5, 10.0]], imeas=11)
+    assert abs(o["tshift"][0] + k * dt) < 1e-12
+    assert abs(o["dlna"][0] + np.log(amp)) < 1e-12
+    assert abs(o["cc_max"][0] - 1.0) < 1e-12
+    assert o["error"][0] == 1.0  # a perfect fit floors sigma_dt at dt_sigma_min
+    assert abs(tot - 0.5 * (k * dt) ** 2) < 1e-15
+    _, _, q = O.cc_adjoint(d, s, dt, [[0.5, 10.0]], imeas=12)
+    assert abs(q["misfit"][0] - 0.5 * (np.log(amp) / 0.5) ** 2) < 1e-12
+
+
+def test_cc_adjoint_sources_are_normalised_by_the_synthetic_energy():
+    """Without taper: simpson(adj_p * ds/dt) = dT / sigma^2 and
+    simpson(adj_q * s) = -dlnA / sigma^2 (ds/dt by numpy's gradient)."""
+    from specfwat_amd.misfit import simpson
+    dt, n = 0.01, 1200
+    d = _wavelet(n, dt, 5.0, 1.2)
+    s = np.roll(d, 9) * 0.9 + 0.05 * _wavelet(n, dt, 5.5, 2.0)
+    win = [[1.0, 9.0]]
+    nb, ne = int(1.0 / dt) + 1, int(9.0 / dt)
+    _, adj, o = O.cc_adjoint(d, s, dt, win, imeas=11, taper_pct=0.0)
+    sw = s[nb - 1:ne]
+    ds = np.gradient(sw, dt, edge_order=1)
+    sig = o["error"][0]
+    assert abs(simpson(adj[nb - 1:ne] * ds, dt) - o["tshift"][0] / sig ** 2) < 1e-10
+    assert np.all(adj[:nb - 1] == 0) and np.all(adj[ne:] == 0)
+    _, adj, o = O.cc_adjoint(d, s, dt, win, imeas=12, taper_pct=0.0)
+    sig = o["error"][0]
+    assert abs(simpson(adj[nb - 1:ne] * sw, dt) + o["dlna"][0] / sig ** 2) < 1e-10
+
+
+def test_cc_selection_thresholds():
+    dt, n = 0.01, 1500
+    rng = np.random.default_rng(4)
+    d = _wavelet(n, dt, 6.0, 1.5)
+    noise = rng.standard_normal(n)
+    tot, adj, o = O.cc_adjoint(d, noise, dt, [[0.5, 12.0]])
+    assert o["cc_max"][0] < 0.7 and not o["selected"][0] and tot == 0.0 and np.all(adj == 0)
+    d2 = _wavelet(n, dt, 9.0, 1.5)
+    s2 = np.roll(d2, -550)  # synthetic 5.5 s early, both inside the window
+    tot, adj, o = O.cc_adjoint(d2, s2, dt, [[0.5, 14.5]])  # 5.5 s > TSHIFT_MAX
+    assert abs(o["tshift"][0] - 5.5) < 1e-12 and not o["selected"][0] and tot == 0.0
+    tot, adj, o = O.cc_adjoint(d2, s2, dt, [[0.5, 14.5]], tshift_max=7.0)
+    assert o["selected"][0] and tot > 0
