@@ -38,13 +38,14 @@ def parse():
     p.add_argument("--steps", type=int, default=None,
                    help="NSTEP of the timed event (default: the configuration's, 10000 for c2, 2000 for c1)")
     p.add_argument("--warmup", type=int, default=100)
-    p.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5", "post"], default="c2",
+    p.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5", "post", "measure"], default="c2",
                    help="c2: one production-size event (default); c1: the small event; "
                         "c3: c1 with bf16 / fp16 statics against the FP64 oracle; "
                         "c4: 32 uneven events batched vs one at a time on the 64x64x32 mesh; "
                         "c5: HTI anisotropic event with 21 c_ij kernels + smoothing of all 21; "
                         "post: kernel post-processing chain (event sums, preconditioner, smoothing, "
-                        "grid projection, taper) on the 64x64x32 mesh")
+                        "grid projection, taper) on the 64x64x32 mesh; measure: waveform and cross-"
+                        "correlation measurements of a C2-sized event (400 receivers x 3 x 10000 samples)")
     p.add_argument("--post-events", type=int, default=32, help="events summed by the post leg")
     p.add_argument("--c4-events", type=int, default=32)
     p.add_argument("--c4-nstep-scale", type=float, default=0.25,
@@ -95,6 +96,8 @@ def main():
         return precision(a)
     if a.config == "post":
         return post_chain(a)
+    if a.config == "measure":
+        return measure(a)
     if a.config == "c5":
         return aniso_smooth(a)
     from specfwat_amd.mesh import BoxMesh
@@ -672,6 +675,67 @@ def aniso_smooth(a):
                                       "GBps": v[0] / (v[1] / max(v[2], 1) * 1e-3) / 1e9 if v[1] > 0 else None}
                                   for q, v in kinds.items()}},
     }
+    print(json.dumps(result), flush=True)
+
+
+def measure(a):
+    """Adjoint-source measurements of one C2-sized event on the device: every
+    trace (400 receivers x 3 components, 10000 samples at dt 0.004 s) through the
+    waveform misfit and the CC traveltime / amplitude measurements over two
+    windows, adjoint sources band-passed 0.4-10 s.  Synthetic traces: a random
+    wavelet per trace, the synthetic shifted and scaled."""
+    from specfwat_amd import misfit
+    nrec, nstep, dt = 400, a.steps or 10000, 0.004
+    rng = np.random.Generator(np.random.Philox(3))
+    t = np.arange(nstep) * dt
+    obs = np.zeros((nstep, nrec, 3), np.float32)
+    syn = np.zeros_like(obs)
+    f = rng.uniform(0.6, 1.4, (nrec, 3))
+    t0 = rng.uniform(8, 30, (nrec, 3))
+    for r in range(nrec):
+        for c in range(3):
+            w = np.exp(-((t - t0[r, c]) * f[r, c]) ** 2) * np.sin(2 * np.pi * f[r, c] * (t - t0[r, c]))
+            obs[:, r, c] = w
+            syn[:, r, c] = np.roll(w, int(rng.integers(-60, 60))) * rng.uniform(0.7, 1.3)
+    win = [[2.0, 22.0], [15.0, 39.0]]
+    stages = {}
+    for label, fn in (("waveform", lambda: misfit.event_adjoint_sources(obs, syn, dt, win, min_period=0.4,
+                                                                          max_period=10.0)),
+                      ("cc_tt", lambda: misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=11,
+                                                                          min_period=0.4, max_period=10.0)),
+                      ("cc_dlna", lambda: misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=12,
+                                                                            min_period=0.4, max_period=10.0))):
+        fn()  # warm-up
+        ts = time.time()
+        out = fn()
+        stages[label] = {"wall_s": time.time() - ts, "device_ms": misfit.last_time_ms(), "misfit": out[0]}
+        if not np.isfinite(out[1]).all():
+            raise RuntimeError(f"non-finite {label} adjoint sources")
+    log("measure: " + ", ".join(f"{k} {v['device_ms']:.1f} ms" for k, v in stages.items()))
+    ntr = nrec * 3
+    dev = stages["cc_tt"]["device_ms"]
+    result = {
+        "metric": "traces/s (CC traveltime measurement with adjoint sources)",
+        "value": ntr / (dev * 1e-3), "unit": "traces/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": dev, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{nrec} receivers x 3 components x {nstep} samples (dt {dt}), windows {win}, "
+                                "adjoint sources band-passed 0.4-10 s")},
+        "roofline": None,
+        "stages": stages,
+    }
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        sample = 6
+        ts = time.time()
+        for q in range(sample):
+            r, c = divmod(q, 3)
+            _, adj_o, _ = orc.cc_adjoint(obs[:, r, c], syn[:, r, c], dt, win, imeas=11)
+            orc.filter_adj(adj_o, dt, np.float32(1 / 10.0), np.float32(1 / 0.4), win)
+        wall = time.time() - ts
+        result["cpu_baseline"] = {"value": sample / wall, "unit": "traces/s", "cores": 1, "kind": "port",
+                                  "sample": f"{sample} traces of the same event through the oracle's CC-TT "
+                                            f"measurement + filter_adj (C++, one thread), {wall:.1f}s"}
     print(json.dumps(result), flush=True)
 
 

@@ -72,7 +72,9 @@ __global__ void __launch_bounds__(MTPB) k_waveform_adj(int nstep, int ntrace, do
   }
 }
 
-// zero-phase cascade of second-order sections over each trace, in place
+// zero-phase cascade of second-order sections over each trace, in place;
+// samples are fetched 16 ahead into registers, the recursion is SAC's
+constexpr int SOS_AHEAD = 16;
 __global__ void __launch_bounds__(MTPB) k_sos_filter(int nstep, int ntrace, int nsect, const float* __restrict__ sos,
                                                      float* __restrict__ x) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,16 +85,29 @@ __global__ void __launch_bounds__(MTPB) k_sos_filter(int nstep, int ntrace, int 
       const float b0 = sos[6 * j], b1 = sos[6 * j + 1], b2 = sos[6 * j + 2];
       const float a1 = sos[6 * j + 4], a2 = sos[6 * j + 5];
       float x1 = 0.0f, x2 = 0.0f, y1 = 0.0f, y2 = 0.0f;
-      for (int k = 0; k < nstep; ++k) {
-        const size_t a = (size_t)(pass == 0 ? k : nstep - 1 - k) * T + t;
-        const float xi = x[a];
-        float out = b0 * xi + b1 * x1 + b2 * x2;
-        out = out - (a1 * y1 + a2 * y2);
-        y2 = y1;
-        y1 = out;
-        x2 = x1;
-        x1 = xi;
-        x[a] = out;
+      for (int k0 = 0; k0 < nstep; k0 += SOS_AHEAD) {
+        float buf[SOS_AHEAD];
+#pragma unroll
+        for (int u = 0; u < SOS_AHEAD; ++u) {
+          const int k = k0 + u;
+          buf[u] = k < nstep ? x[(size_t)(pass == 0 ? k : nstep - 1 - k) * T + t] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < SOS_AHEAD; ++u) {
+          const float xi = buf[u];
+          float out = b0 * xi + b1 * x1 + b2 * x2;
+          out = out - (a1 * y1 + a2 * y2);
+          y2 = y1;
+          y1 = out;
+          x2 = x1;
+          x1 = xi;
+          buf[u] = out;
+        }
+#pragma unroll
+        for (int u = 0; u < SOS_AHEAD; ++u) {
+          const int k = k0 + u;
+          if (k < nstep) x[(size_t)(pass == 0 ? k : nstep - 1 - k) * T + t] = buf[u];
+        }
       }
     }
 }
@@ -132,131 +147,200 @@ __global__ void __launch_bounds__(256) k_cc_rows(int ntrace, int nb, int nl, con
   }
 }
 
-// xcorr_shift: one workgroup per trace, lanes over the 2 nl - 1 lags of the
-// full correlation; each sum runs over i1 in increasing order as in
-// myconvolution_dp; the first maximum wins (maxloc)
-__global__ void __launch_bounds__(256) k_cc_shift(int ntrace, int nl, const double* __restrict__ D,
-                                                  const double* __restrict__ S, int* __restrict__ ishift) {
-  const int t = blockIdx.x;
-  if (t >= ntrace) return;
+// xcorr_shift, first stage: workgroup (trace, block of 256 lags) of the full
+// correlation; d and the matching stretch of s come through LDS in chunks of
+// 1024 samples taken in increasing i1, so each lag's sum runs over i1 in
+// increasing order as in myconvolution_dp.  Writes the block's first maximum.
+constexpr int CC_LAGS = 256, CC_CHUNK = 1024;
+__global__ void __launch_bounds__(CC_LAGS) k_cc_corr(int nl, const double* __restrict__ D,
+                                                     const double* __restrict__ S, double* __restrict__ bval,
+                                                     int* __restrict__ bidx) {
+  __shared__ double sd[CC_CHUNK], ss[CC_CHUNK + CC_LAGS];
+  const int t = blockIdx.y, nblk = gridDim.x;
   const double* d = D + (size_t)t * nl;
   const double* s = S + (size_t)t * nl;
-  double best = -INFINITY;
-  int bm = 0x7fffffff;
-  for (int m = threadIdx.x; m < 2 * nl - 1; m += blockDim.x) {
-    const int lo = m - (nl - 1) > 0 ? m - (nl - 1) : 0, hi = m < nl - 1 ? m : nl - 1;
-    double c = 0.0;
-    for (int i1 = lo; i1 <= hi; ++i1) c = c + d[i1] * s[nl - 1 - (m - i1)];
-    if (c > best) { best = c; bm = m; }
+  const int m0 = blockIdx.x * CC_LAGS, m = m0 + (int)threadIdx.x;
+  const int nlag = 2 * nl - 1;
+  // lag m reads s[i1 + nl - 1 - m] for i1 in [max(0, m - nl + 1), min(nl - 1, m)]
+  const int lo = m - (nl - 1) > 0 ? m - (nl - 1) : 0, hi = m < nl - 1 ? m : nl - 1;
+  const int blo = m0 - (nl - 1) > 0 ? m0 - (nl - 1) : 0;
+  const int mlast = min(m0 + CC_LAGS - 1, nlag - 1);
+  const int bhi = mlast < nl - 1 ? mlast : nl - 1;
+  double c = 0.0;
+  for (int i0 = blo; i0 <= bhi; i0 += CC_CHUNK) {
+    const int i1end = min(i0 + CC_CHUNK - 1, bhi);
+    // s indices of this chunk over the block's lags: [i0 + nl - 1 - (m0 + 255), i1end + nl - 1 - m0]
+    const int sbase = i0 + nl - 1 - (m0 + CC_LAGS - 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < CC_CHUNK; k += CC_LAGS) sd[k] = (i0 + k <= i1end) ? d[i0 + k] : 0.0;
+    for (int k = threadIdx.x; k < CC_CHUNK + CC_LAGS; k += CC_LAGS) {
+      const int j = sbase + k;
+      ss[k] = (j >= 0 && j < nl) ? s[j] : 0.0;
+    }
+    __syncthreads();
+    if (m < nlag) {
+      const int a = max(lo, i0), b = min(hi, i1end);
+      // s[i1 + nl - 1 - m] sits at ss[i1 - i0 + (m0 + 255 - m)]
+      const int off = m0 + CC_LAGS - 1 - m - i0;
+      for (int i1 = a; i1 <= b; ++i1) c = c + sd[i1 - i0] * ss[i1 + off];
+    }
   }
-  __shared__ double sv[256];
-  __shared__ int sm[256];
-  sv[threadIdx.x] = best;
-  sm[threadIdx.x] = bm;
+  __shared__ double rv[CC_LAGS];
+  __shared__ int ri[CC_LAGS];
+  rv[threadIdx.x] = m < nlag ? c : -INFINITY;
+  ri[threadIdx.x] = m < nlag ? m : 0x7fffffff;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = CC_LAGS / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
-      const double v2 = sv[threadIdx.x + o];
-      const int m2 = sm[threadIdx.x + o];
-      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && m2 < sm[threadIdx.x])) {
-        sv[threadIdx.x] = v2;
-        sm[threadIdx.x] = m2;
+      const double v2 = rv[threadIdx.x + o];
+      const int m2 = ri[threadIdx.x + o];
+      if (v2 > rv[threadIdx.x] || (v2 == rv[threadIdx.x] && m2 < ri[threadIdx.x])) {
+        rv[threadIdx.x] = v2;
+        ri[threadIdx.x] = m2;
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) ishift[t] = sm[0] + 1 - nl;
+  if (threadIdx.x == 0) {
+    bval[(size_t)t * nblk + blockIdx.x] = rv[0];
+    bidx[(size_t)t * nblk + blockIdx.x] = ri[0];
+  }
+}
+
+// xcorr_shift, second stage: first maximum over the blocks of a trace
+__global__ void k_cc_pick(int ntrace, int nblk, int nl, const double* __restrict__ bval,
+                          const int* __restrict__ bidx, int* __restrict__ ishift) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrace) return;
+  double best = bval[(size_t)t * nblk];
+  int bm = bidx[(size_t)t * nblk];
+  for (int b = 1; b < nblk; ++b) {
+    const double v = bval[(size_t)t * nblk + b];
+    if (v > best) { best = v; bm = bidx[(size_t)t * nblk + b]; }
+  }
+  ishift[t] = bm + 1 - nl;
 }
 
 struct CcCfg {
   double dt_sigma_min, dlna_sigma_min, tshift_min, tshift_max, dlna_min, dlna_max, cc_min;
 };
 
-// per trace: dlna, calc_cc_error (cc_correction + gradient), cc_max_coef,
-// cc_measure_select and calc_cc_adjsrc; window samples read on the fly from
-// the sample-major traces; out[8] per (window, trace): misfit, residual,
-// error, cc_max, tshift, dlna, selected, ishift
-__global__ void __launch_bounds__(MTPB) k_cc_finish(int ntrace, double dt, int nb, int nl, int imeas, CcCfg cfg,
-                                                    const double* __restrict__ tw, const float* __restrict__ obs,
-                                                    const float* __restrict__ syn, const int* __restrict__ ishift_in,
-                                                    float* __restrict__ adj, double* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntrace) return;
-  const size_t T = (size_t)ntrace;
-  auto sv = [&](int k) { return (double)syn[(size_t)(nb + k) * T + t] * tw[k]; };
-  auto dv = [&](int k) { return (double)obs[(size_t)(nb + k) * T + t] * tw[k]; };
+// fixed-order tree sum over a 256-lane workgroup, result on every lane
+__device__ double wg_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// per trace (one workgroup on its tapered rows d, s): dlna, calc_cc_error
+// (cc_correction + gradient), cc_max_coef, cc_measure_select and
+// calc_cc_adjsrc; the sums are fixed-order tree sums.  out[8] per (window,
+// trace): misfit, residual, error, cc_max, tshift, dlna, selected, ishift; the
+// selected traces' tapered adjoint sources go to rows A [ntrace][nl]
+__global__ void __launch_bounds__(256) k_cc_finish(int nl, double dt, int imeas, CcCfg cfg,
+                                                   const double* __restrict__ tw, const double* __restrict__ D,
+                                                   const double* __restrict__ S, const int* __restrict__ ishift_in,
+                                                   double* __restrict__ A, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int t = blockIdx.x;
+  const double* d = D + (size_t)t * nl;
+  const double* s = S + (size_t)t * nl;
   const int ish = ishift_in[t];
   const double tshift = ish * dt;
-  double sd2 = 0.0, ss2 = 0.0;
-  for (int i = 0; i < nl; ++i) { const double x = dv(i); sd2 = sd2 + x * x; }
-  for (int i = 0; i < nl; ++i) { const double x = sv(i); ss2 = ss2 + x * x; }
+  double p1 = 0.0, p2 = 0.0;
+  for (int i = threadIdx.x; i < nl; i += 256) { p1 += d[i] * d[i]; p2 += s[i] * s[i]; }
+  const double sd2 = wg_sum(p1, sh), ss2 = wg_sum(p2, sh);
   const double dlna = 0.5 * log(sd2 / ss2);
   const double ea = exp(dlna);
-  auto scc = [&](int i) { const int j = i - ish; return (j >= 0 && j < nl) ? sv(j) : 0.0; };
-  auto sccd = [&](int i) { const int j = i - ish; return (j >= 0 && j < nl) ? ea * sv(j) : 0.0; };
-  auto vel = [&](int i) {
+  auto scc = [&](int i) { const int j = i - ish; return (j >= 0 && j < nl) ? s[j] : 0.0; };
+  auto sccd = [&](int i) { const int j = i - ish; return (j >= 0 && j < nl) ? ea * s[j] : 0.0; };
+  auto grad = [&](auto f, int i) {
     if (nl < 2) return 0.0;
-    if (i == 0) return (sccd(1) - sccd(0)) / dt;
-    if (i == nl - 1) return (sccd(nl - 1) - sccd(nl - 2)) / dt;
-    return (sccd(i + 1) - sccd(i - 1)) / (2.0 * dt);
+    if (i == 0) return (f(1) - f(0)) / dt;
+    if (i == nl - 1) return (f(nl - 1) - f(nl - 2)) / dt;
+    return (f(i + 1) - f(i - 1)) / (2.0 * dt);
   };
-  double num = 0.0, den = 0.0, den2 = 0.0;
-  for (int i = 0; i < nl; ++i) { const double r = dv(i) - sccd(i); num = num + r * r; }
-  for (int i = 0; i < nl; ++i) { const double v = vel(i); den = den + v * v; }
-  for (int i = 0; i < nl; ++i) { const double c = scc(i); den2 = den2 + c * c; }
-  double sig_dt = sqrt(num / den), sig_dlna = sqrt(num / den2);
-  if (sig_dt < cfg.dt_sigma_min || isnan(sig_dt)) sig_dt = cfg.dt_sigma_min;
-  if (sig_dlna < cfg.dlna_sigma_min || isnan(sig_dlna)) sig_dlna = cfg.dlna_sigma_min;
-  // cc_max_coef: d with the edge the shifted synthetic leaves empty zeroed
   auto dsh = [&](int i) {
     if (ish > 0 && i < ish) return 0.0;
     if (ish < 0 && i >= nl + ish) return 0.0;
-    return dv(i);
+    return d[i];
   };
-  double a = 0.0, b = 0.0, c = 0.0;
-  for (int i = 0; i < nl; ++i) a = a + dsh(i) * scc(i);
-  for (int i = 0; i < nl; ++i) { const double x = dsh(i); b = b + x * x; }
-  for (int i = 0; i < nl; ++i) { const double x = scc(i); c = c + x * x; }
-  const double cmax = a / sqrt(b * c);
+  double q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
+  for (int i = threadIdx.x; i < nl; i += 256) {
+    const double r = d[i] - sccd(i), v = grad(sccd, i), c = scc(i), e = dsh(i);
+    q1 += r * r; q2 += v * v; q3 += c * c; q4 += e * c; q5 += e * e;
+  }
+  const double num = wg_sum(q1, sh), den = wg_sum(q2, sh), den2 = wg_sum(q3, sh);
+  const double a = wg_sum(q4, sh), b = wg_sum(q5, sh);
+  double sig_dt = sqrt(num / den), sig_dlna = sqrt(num / den2);
+  if (sig_dt < cfg.dt_sigma_min || isnan(sig_dt)) sig_dt = cfg.dt_sigma_min;
+  if (sig_dlna < cfg.dlna_sigma_min || isnan(sig_dlna)) sig_dlna = cfg.dlna_sigma_min;
+  const double cmax = a / sqrt(b * den2);
   const bool sel = !(cmax < cfg.cc_min || tshift < cfg.tshift_min || tshift > cfg.tshift_max ||
                      dlna < cfg.dlna_min || dlna > cfg.dlna_max);
-  double* o = out + 8 * (size_t)t;
-  o[1] = imeas == 11 ? tshift : dlna;
-  o[2] = imeas == 11 ? sig_dt : sig_dlna;
-  o[3] = cmax;
-  o[4] = tshift;
-  o[5] = dlna;
-  o[6] = sel ? 1.0 : 0.0;
-  o[7] = (double)ish;
-  o[0] = 0.0;
+  if (threadIdx.x == 0) {
+    double* o = out + 8 * (size_t)t;
+    o[1] = imeas == 11 ? tshift : dlna;
+    o[2] = imeas == 11 ? sig_dt : sig_dlna;
+    o[3] = cmax;
+    o[4] = tshift;
+    o[5] = dlna;
+    o[6] = sel ? 1.0 : 0.0;
+    o[7] = (double)ish;
+    o[0] = !sel ? 0.0
+                : (imeas == 11 ? 0.5 * (tshift / sig_dt) * (tshift / sig_dt)
+                               : 0.5 * (dlna / sig_dlna) * (dlna / sig_dlna));
+  }
   if (!sel) return;
-  // calc_cc_adjsrc on the tapered synthetic, then the window taper
-  auto dsdt = [&](int i) {
-    if (nl < 2) return 0.0;
-    if (i == 0) return (sv(1) - sv(0)) / dt;
-    if (i == nl - 1) return (sv(nl - 1) - sv(nl - 2)) / dt;
-    return (sv(i + 1) - sv(i - 1)) / (2.0 * dt);
-  };
-  auto yq = [&](int i) { const double x = imeas == 11 ? dsdt(i) : sv(i); return x * x; };
-  double nn = 0.0;  // simpson (utils.f90:1630-1663)
-  if (nl >= 2) {
+  // calc_cc_adjsrc: simpson (utils.f90:1630-1663) of ds/dt^2 (11) or s^2 (12) as a weighted sum
+  auto sf = [&](int i) { return s[i]; };
+  double pw = 0.0, tail = 0.0;
+  for (int i = threadIdx.x; i < nl; i += 256) {
+    const double x = imeas == 11 ? grad(sf, i) : s[i];
+    const double y = x * x;
+    double w;
     if (nl % 2 == 0) {
-      double sum = yq(0) + yq(nl - 2);
-      for (int k = 1; k <= nl - 3; k += 2) sum = sum + 4.0 * yq(k) + 2.0 * yq(k + 1);
-      nn = (dt / 3.0) * sum + 0.5 * dt * (yq(nl - 2) + yq(nl - 1));
+      // sum = y(1) + y(n-1) + sum over pairs 4 y(i) + 2 y(i+1), i = 2, 4, .., n-2 (1-based)
+      w = (i == 0 ? 1.0 : 0.0) + (i == nl - 2 ? 1.0 : 0.0) + ((i % 2 == 1 && i <= nl - 3) ? 4.0 : 0.0) +
+          ((i % 2 == 0 && i >= 2 && i <= nl - 2) ? 2.0 : 0.0);
+      if (i >= nl - 2) tail += y;
     } else {
-      double sum = yq(0) + yq(nl - 1);
-      for (int k = 1; k <= nl - 2; k += 2) sum = sum + 4.0 * yq(k);
-      for (int k = 2; k <= nl - 3; k += 2) sum = sum + 2.0 * yq(k);
-      nn = (dt / 3.0) * sum;
+      w = (i == 0 || i == nl - 1) ? 1.0 : (i % 2 == 1 ? 4.0 : 2.0);
     }
+    pw += w * y;
   }
-  for (int i = 0; i < nl; ++i) {
-    const double v = imeas == 11 ? dsdt(i) * tshift / nn / (sig_dt * sig_dt)
-                                 : -1.0 * sv(i) * dlna / nn / (sig_dlna * sig_dlna);
-    adj[(size_t)(nb + i) * T + t] = (float)(v * tw[i]);
+  const double sw = wg_sum(pw, sh), st = wg_sum(tail, sh);
+  double nn = 0.0;
+  if (nl >= 2) nn = (dt / 3.0) * sw + (nl % 2 == 0 ? 0.5 * dt * st : 0.0);
+  for (int i = threadIdx.x; i < nl; i += 256) {
+    const double v = imeas == 11 ? grad(sf, i) * tshift / nn / (sig_dt * sig_dt)
+                                 : -1.0 * s[i] * dlna / nn / (sig_dlna * sig_dlna);
+    A[(size_t)t * nl + i] = v * tw[i];
   }
-  o[0] = imeas == 11 ? 0.5 * (tshift / sig_dt) * (tshift / sig_dt) : 0.5 * (dlna / sig_dlna) * (dlna / sig_dlna);
+}
+
+// selected traces' adjoint-source rows back into the sample-major traces
+// (64 x 64 LDS tiles); unselected traces keep what earlier windows wrote
+__global__ void __launch_bounds__(256) k_cc_put(int ntrace, int nb, int nl, const double* __restrict__ A,
+                                                const double* __restrict__ out, float* __restrict__ adj) {
+  __shared__ double tile[64][65];
+  const int t0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int t = t0 + r, k = k0 + (threadIdx.x & 63);
+    if (t < ntrace && k < nl) tile[r][threadIdx.x & 63] = A[(size_t)t * nl + k];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int k = k0 + r, t = t0 + (threadIdx.x & 63);
+    if (t < ntrace && k < nl && out[8 * (size_t)t + 6] != 0.0)
+      adj[(size_t)(nb + k) * ntrace + t] = (float)tile[threadIdx.x & 63][r];
+  }
 }
 
 __global__ void k_fill_f32m(size_t n, float* __restrict__ x) {
@@ -387,6 +471,7 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   if (N <= 0 || T <= 0) SEM_FAIL("fwat_cc_adjoint_batch_: sizes must be positive");
   if (nw < 1 || nw > MAX_WIN) SEM_FAIL("fwat_cc_adjoint_batch_: 1..%d windows", MAX_WIN);
   if (*imeas != 11 && *imeas != 12) SEM_FAIL("fwat_cc_adjoint_batch_: imeas must be 11 (CC-TT) or 12 (CC-DLNA)");
+  if (T > 65535) SEM_FAIL("fwat_cc_adjoint_batch_: at most 65535 traces per call");
   if (*nsect < 0 || *nsect > 8) SEM_FAIL("fwat_cc_adjoint_batch_: 0..8 filter sections");
   if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
   const CcCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6]};
@@ -415,6 +500,10 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   double* d_D = dev_alloc<double>((size_t)T * maxnl);
   double* d_S = dev_alloc<double>((size_t)T * maxnl);
   double* d_tw = dev_alloc<double>(maxnl);
+  double* d_A = dev_alloc<double>((size_t)T * maxnl);
+  const int maxblk = (2 * maxnl - 1 + CC_LAGS - 1) / CC_LAGS;
+  double* d_bv = dev_alloc<double>((size_t)T * maxblk);
+  int* d_bi = dev_alloc<int>((size_t)T * maxblk);
   HIP_CHECK(hipEventRecord(e0, s));
   hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_adj);
   const unsigned nbk = (unsigned)((T + MTPB - 1) / MTPB);
@@ -422,9 +511,14 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     HIP_CHECK(hipMemcpyAsync(d_tw, tws[w].data(), nls[w] * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cc_rows, dim3((unsigned)((T + 63) / 64), (unsigned)((nls[w] + 63) / 64)), dim3(256), 0, s, T,
                        nbs[w], nls[w], d_tw, d_obs, d_syn, d_D, d_S);
-    hipLaunchKernelGGL(k_cc_shift, dim3((unsigned)T), dim3(256), 0, s, T, nls[w], d_D, d_S, d_ish);
-    hipLaunchKernelGGL(k_cc_finish, dim3(nbk), dim3(MTPB), 0, s, T, *dt, nbs[w], nls[w], *imeas, C, d_tw, d_obs, d_syn,
-                       d_ish, d_adj, d_out + 8 * (size_t)w * T);
+    const int nblk = (2 * nls[w] - 1 + CC_LAGS - 1) / CC_LAGS;
+    hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(CC_LAGS), 0, s, nls[w], d_D, d_S, d_bv,
+                       d_bi);
+    hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nls[w], d_bv, d_bi, d_ish);
+    hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nls[w], *dt, *imeas, C, d_tw, d_D, d_S,
+                       d_ish, d_A, d_out + 8 * (size_t)w * T);
+    hipLaunchKernelGGL(k_cc_put, dim3((unsigned)((T + 63) / 64), (unsigned)((nls[w] + 63) / 64)), dim3(256), 0, s, T,
+                       nbs[w], nls[w], d_A, d_out + 8 * (size_t)w * T, d_adj);
   }
   std::vector<double> fw;
   double* d_fw = nullptr;
@@ -458,7 +552,7 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     misfits[t] = m;
   }
   dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_ish); dev_free(d_out); dev_free(d_D); dev_free(d_S);
-  dev_free(d_tw); dev_free(d_fw); dev_free(d_sos);
+  dev_free(d_tw); dev_free(d_fw); dev_free(d_sos); dev_free(d_bv); dev_free(d_bi); dev_free(d_A);
   HIP_CHECK(hipEventDestroy(e0));
   HIP_CHECK(hipEventDestroy(e1));
   HIP_CHECK(hipStreamDestroy(s));

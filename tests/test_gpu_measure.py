@@ -77,7 +77,7 @@ def test_cc_batch_matches_oracle(imeas):
             obs[:, r, c] = w + 0.02 * rng.standard_normal(nstep)
             syn[:, r, c] = np.roll(w, rng.integers(-40, 40)) * rng.uniform(0.6, 1.4)
     syn[:, 0, 0] = rng.standard_normal(nstep)  # one trace below CC_MIN
-    win = [[1.0, 7.0], [3.0, 12.5]]
+    win = [[1.0, 7.01], [3.0, 12.5]]  # 601 (odd) and 950 (even) samples
     total, adj, meas = misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=imeas)
     tot = 0.0
     for r in range(nrec):
