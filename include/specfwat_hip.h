@@ -318,7 +318,8 @@ void fwat_opt_last_time_(double *ms);
 void fwat_waveform_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
                                   double *windows, double *taper_percentage, int *itaper_type, int *nsect,
                                   realw *sos, double *misfits, double *residuals, realw *adj);
-/* device time (ms) of the last fwat_waveform_adjoint_batch_ or fwat_cc_adjoint_batch_, copies excluded */
+/* device time (ms) of the last fwat_waveform_adjoint_batch_, fwat_cc_adjoint_batch_ or
+ * fwat_mt_adjoint_batch_, copies excluded */
 void fwat_waveform_adjoint_last_time_(double *ms);
 /* cross-correlation traveltime (imeas 11) / amplitude (12) measurement of every trace: CCTTMisfit%
  * calc_adjoint_source (src/preproc/adjoint_sources/cc_tt_misfit.f90:17-146 with utils/cc.f90:10-160)
@@ -326,6 +327,18 @@ void fwat_waveform_adjoint_last_time_(double *ms);
  * CC_MIN (adj_config.f90:18-23); meas(8,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
  * selected, ishift (or NULL) */
 void fwat_cc_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
+                            double *windows, double *taper_percentage, int *itaper_type, int *imeas,
+                            double *cfg, int *nsect, realw *sos, double *misfits, double *meas, realw *adj);
+/* multitaper traveltime (imeas 13) / amplitude (14) measurement of every trace with the reference's
+ * fall back to the cross-correlation measurement: MTTTMisfit%calc_adjoint_source
+ * (src/preproc/adjoint_sources/mt_tt_misfit.f90:29-791 with utils/dpss.f90:137-262 and
+ * process_cycle_skipping, signal.f90:510-598) and filter_adj; cfg(18) = the 7 values of
+ * fwat_cc_adjoint_batch_, transfunc_waterlevel, water_threshold, mt_nw, num_taper, dt_fac, err_fac,
+ * dt_max_scale, min_cycle_in_window, use_cc_error (0/1), min_period, max_period (adj_config.f90:18-26;
+ * use_mt_error = .false.); meas(10,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
+ * selected, ishift, measurement used (13/14, 11/12 after the fall back, 0 unselected), the other misfit
+ * (or NULL).  32 <= nstep <= 16383 */
+void fwat_mt_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
                             double *windows, double *taper_percentage, int *itaper_type, int *imeas,
                             double *cfg, int *nsect, realw *sos, double *misfits, double *meas, realw *adj);
 

@@ -27,6 +27,11 @@
 //                         signal.f90:146-220, cc_max_coef, calc_cc_adjsrc), gradient
 //                         (utils.f90:1613-1628) and cc_measure_select (cc_tt_misfit.f90:
 //                         126-146); double precision, loops in the reference's order
+//   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
+//   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
+//                         where the reference calls LAPACK dsterf
+//   orc_mt_adjoint        MTTTMisfit%calc_adjoint_source (mt_tt_misfit.f90:29-791) with
+//                         process_cycle_skipping (signal.f90:510-598); use_mt_error off
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -59,6 +64,223 @@ int xcorr_shift(const std::vector<double>& d, const std::vector<double>& s) {
   return best + 1 - n;
 }
 
+
+struct CcStats {
+  int ishift;
+  double tshift, dlna, sig_dt, sig_dlna, cmax;
+  bool sel;
+};
+
+// calc_cc_shift, calc_cc_error with cc_correction, cc_max_coef and
+// cc_measure_select on the tapered windows dv, sv
+CcStats cc_stats(const std::vector<double>& dv, const std::vector<double>& sv, double dt, const double* cfg) {
+  const int nl = (int)dv.size();
+  CcStats c;
+  c.ishift = xcorr_shift(dv, sv);
+  c.tshift = c.ishift * dt;
+  double sd2 = 0.0, ss2 = 0.0;
+  for (int i = 0; i < nl; ++i) sd2 = sd2 + dv[i] * dv[i];
+  for (int i = 0; i < nl; ++i) ss2 = ss2 + sv[i] * sv[i];
+  c.dlna = 0.5 * std::log(sd2 / ss2);
+  std::vector<double> scc(nl, 0.0), sccd(nl, 0.0);
+  for (int i = 0; i < nl; ++i) {
+    const int j = i - c.ishift;
+    if (j >= 0 && j < nl) { scc[i] = sv[j]; sccd[i] = std::exp(c.dlna) * sv[j]; }
+  }
+  const std::vector<double> vel = gradient(sccd, dt);
+  double num = 0.0, den = 0.0, den2 = 0.0;
+  for (int i = 0; i < nl; ++i) num = num + (dv[i] - sccd[i]) * (dv[i] - sccd[i]);
+  for (int i = 0; i < nl; ++i) den = den + vel[i] * vel[i];
+  for (int i = 0; i < nl; ++i) den2 = den2 + scc[i] * scc[i];
+  c.sig_dt = std::sqrt(num / den);
+  c.sig_dlna = std::sqrt(num / den2);
+  if (c.sig_dt < cfg[0] || std::isnan(c.sig_dt)) c.sig_dt = cfg[0];
+  if (c.sig_dlna < cfg[1] || std::isnan(c.sig_dlna)) c.sig_dlna = cfg[1];
+  // cc_max_coef (same shift; d's edges zeroed where the shifted s is empty)
+  std::vector<double> dsh(dv);
+  if (c.ishift > 0) for (int i = 0; i < c.ishift && i < nl; ++i) dsh[i] = 0.0;
+  else if (c.ishift < 0) for (int i = nl + c.ishift; i < nl; ++i) if (i >= 0) dsh[i] = 0.0;
+  double a = 0.0, b = 0.0, cc = 0.0;
+  for (int i = 0; i < nl; ++i) a = a + dsh[i] * scc[i];
+  for (int i = 0; i < nl; ++i) b = b + dsh[i] * dsh[i];
+  for (int i = 0; i < nl; ++i) cc = cc + scc[i] * scc[i];
+  c.cmax = a / std::sqrt(b * cc);
+  c.sel = !(c.cmax < cfg[6] || c.tshift < cfg[2] || c.tshift > cfg[3] || c.dlna < cfg[4] || c.dlna > cfg[5]);
+  return c;
+}
+
+// calc_cc_adjsrc (utils/cc.f90): both adjoint sources when imeas is 0, else
+// the one imeas (11 or 12) asks for; misfits 0.5 (tshift/sigma)^2, 0.5 (dlna/sigma)^2
+void cc_adjsrc(const std::vector<double>& sv, const CcStats& c, double dt, int imeas, std::vector<double>& p,
+               std::vector<double>& q, double& mp, double& mq) {
+  const int nl = (int)sv.size();
+  mp = 0.5 * (c.tshift / c.sig_dt) * (c.tshift / c.sig_dt);
+  mq = 0.5 * (c.dlna / c.sig_dlna) * (c.dlna / c.sig_dlna);
+  p.assign(nl, 0.0);
+  q.assign(nl, 0.0);
+  if (imeas != 12) {
+    const std::vector<double> dsdt = gradient(sv, dt);
+    std::vector<double> sq(nl);
+    for (int i = 0; i < nl; ++i) sq[i] = dsdt[i] * dsdt[i];
+    const double nn = orc_simpson(sq.data(), nl, dt);
+    for (int i = 0; i < nl; ++i) p[i] = dsdt[i] * c.tshift / nn / (c.sig_dt * c.sig_dt);
+  }
+  if (imeas != 11) {
+    std::vector<double> sq(nl);
+    for (int i = 0; i < nl; ++i) sq[i] = sv[i] * sv[i];
+    const double nn = orc_simpson(sq.data(), nl, dt);
+    for (int i = 0; i < nl; ++i) q[i] = -1.0 * sv[i] * c.dlna / nn / (c.sig_dlna * c.sig_dlna);
+  }
+}
+
+// complex over a real-valued complex, as Smith's division gives it (a/c, b/c);
+// the spectra sum(s conj(s)) have an exactly zero imaginary part
+std::complex<double> cdivr(std::complex<double> a, double c) { return {a.real() / c, a.imag() / c}; }
+
+// smallest power of two above n: 2 ** exponent(real(n)) of the reference
+int next_pow2_above(int n) {
+  int e = 0;
+  std::frexp((float)n, &e);
+  return 1 << e;
+}
+
+// fft_raw_dp (fftpack.f90:202-241): bit reversal as written, then radix-2
+// butterflies with twiddle exp(i pi ind (k - 1) / kmax)
+template <class T>
+void fft_raw(std::vector<std::complex<T>>& x, int ind, T pi) {
+  const int n = (int)x.size();
+  int j = 1;
+  for (int i = 1; i <= n; ++i) {
+    if (i < j) std::swap(x[j - 1], x[i - 1]);
+    int m = n / 2;
+    while (true) {
+      if (j > m) {
+        j = j - m;
+        m = m / 2;
+        if (m < 2) break;
+      } else {
+        break;
+      }
+    }
+    j = j + m;
+  }
+  for (int kmax = 1; kmax < n; kmax *= 2) {
+    const int istep = kmax * 2;
+    for (int k = 1; k <= kmax; ++k) {
+      const T th = pi * (T)ind * (T)(k - 1) / (T)kmax;
+      const std::complex<T> w(std::cos(th), std::sin(th));
+      for (int i = k; i <= n; i += istep) {
+        const int jj = i + kmax;
+        const std::complex<T> t = x[jj - 1] * w;
+        x[jj - 1] = x[i - 1] - t;
+        x[i - 1] = x[i - 1] + t;
+      }
+    }
+  }
+}
+
+// number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
+int sturm_below(const std::vector<double>& d, const std::vector<double>& e, double x) {
+  int cnt = 0;
+  double q = 1.0;
+  for (size_t i = 0; i < d.size(); ++i) {
+    const double e2 = i ? e[i - 1] * e[i - 1] : 0.0;
+    q = (d[i] - x) - (i ? e2 / q : 0.0);
+    if (q == 0.0) q = -1e-300;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+// tridisolve (dpss.f90:14-45), Golub and Van Loan's symmetric tridiagonal solve
+void tridisolve(const std::vector<double>& d, const std::vector<double>& e, std::vector<double>& x) {
+  const int n = (int)d.size();
+  std::vector<double> dw(d), ew(e);
+  for (int k = 1; k < n; ++k) {
+    const double t = ew[k - 1];
+    ew[k - 1] = t / dw[k - 1];
+    dw[k] = dw[k] - t * ew[k - 1];
+  }
+  for (int k = 1; k < n; ++k) x[k] = x[k] - ew[k - 1] * x[k - 1];
+  x[n - 1] = x[n - 1] / dw[n - 1];
+  for (int k = n - 2; k >= 0; --k) x[k] = x[k] / dw[k] - ew[k] * x[k + 1];
+}
+
+double l2norm(const std::vector<double>& x) {
+  double s = 0.0;
+  for (double v : x) s = s + v * v;
+  return std::sqrt(s);
+}
+
+// dpss_windows: the tridiagonal matrix of the discrete prolate spheroidal
+// sequences, its kmax largest eigenvalues, inverse iteration
+// (tridi_inverse_iteration, dpss.f90:49-107) from sin(k pi i / (n - 1)), signs
+void dpss(int n, double half_nbw, int kmax, std::vector<std::vector<double>>& v) {
+  const double pi = 3.141592653589793;
+  const double wbin = half_nbw / (double)n;
+  std::vector<double> dg(n), od(std::max(n - 1, 0));
+  for (int i = 0; i < n; ++i) {
+    const double q = ((double)(n - 1) - 2.0 * i) / 2.0;
+    dg[i] = q * q * std::cos(2.0 * pi * wbin);
+  }
+  for (int i = 0; i < n - 1; ++i) od[i] = (double)(i + 1) * ((double)n - (double)(i + 1)) / 2.0;
+  double lo = dg[0], hi = dg[0];
+  for (int i = 0; i < n; ++i) {
+    const double r = (i ? std::fabs(od[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(od[i]) : 0.0);
+    lo = std::min(lo, dg[i] - r);
+    hi = std::max(hi, dg[i] + r);
+  }
+  v.assign(kmax, std::vector<double>(n, 0.0));
+  for (int k = 0; k < kmax; ++k) {
+    // eigenvalue number n - 1 - k (0-based ascending) by bisection
+    const int target = n - 1 - k;
+    double a = lo, b = hi;
+    for (int it = 0; it < 200; ++it) {
+      const double m = 0.5 * (a + b);
+      if (m <= a || m >= b) break;
+      if (sturm_below(dg, od, m) <= target) a = m; else b = m;
+    }
+    const double lam = 0.5 * (a + b);
+    std::vector<double> ed(n), x(n), xp(n);
+    for (int i = 0; i < n; ++i) ed[i] = dg[i] - lam;
+    for (int i = 0; i < n; ++i) x[i] = std::sin((double)(k + 1) * (pi * (double)i / (double)(n - 1)));
+    double nx = l2norm(x);
+    for (double& t : x) t = t / nx;
+    for (int it = 1;; ++it) {
+      xp = x;
+      tridisolve(ed, od, x);
+      nx = l2norm(x);
+      for (double& t : x) t = t / nx;
+      double dn = 0.0;
+      for (int i = 0; i < n; ++i) dn = dn + (std::fabs(x[i]) - std::fabs(xp[i])) * (std::fabs(x[i]) - std::fabs(xp[i]));
+      if (std::sqrt(dn) <= 1e-8 || it > 1000) break;
+    }
+    v[k] = x;
+  }
+  for (int k = 0; k < kmax; k += 2) {
+    double s = 0.0;
+    for (double t : v[k]) s = s + t;
+    if (s < 0.0) for (double& t : v[k]) t = -t;
+  }
+  for (int k = 1; k < kmax; k += 2)
+    if (n > 1 && v[k][1] < 0.0) for (double& t : v[k]) t = -t;
+}
+
+// process_cycle_skipping (signal.f90:510-598): 2 pi jumps of the phase between
+// neighbouring frequencies, corrected from there on
+void cycle_skipping(std::vector<double>& phi, int nfmin, int nfmax, double step) {
+  const double pi = 3.141592653589793;
+  for (int iw = nfmin + 2; iw <= nfmax - 1; ++iw) {
+    const double p0 = phi[iw - 2], p1 = phi[iw - 1], p2 = phi[iw];
+    const double s0 = std::fabs(p2 + p0 - 2.0 * p1);
+    const double s1 = std::fabs((p2 + 2.0 * pi) + p0 - 2.0 * p1);
+    const double s2 = std::fabs((p2 - 2.0 * pi) + p0 - 2.0 * p1);
+    if (std::fabs(p1 - p2) > step) {
+      if (s1 < s0 && s1 < s2) for (int i = iw + 1; i <= nfmax; ++i) phi[i - 1] = phi[i - 1] + 2.0 * pi;
+      if (s2 < s0 && s2 < s1) for (int i = iw + 1; i <= nfmax; ++i) phi[i - 1] = phi[i - 1] - 2.0 * pi;
+    }
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -147,66 +369,256 @@ double orc_cc_adjoint(const double* dat, const double* syn, int n, double dt, co
     std::vector<double> sv(syn + nb - 1, syn + ne), dv(dat + nb - 1, dat + ne);
     orc_window_taper(sv.data(), nl, taper_pct, itaper);
     orc_window_taper(dv.data(), nl, taper_pct, itaper);
-    // calc_cc_shift
-    const int ishift = xcorr_shift(dv, sv);
-    const double tshift = ishift * dt;
-    double sd2 = 0.0, ss2 = 0.0;
-    for (int i = 0; i < nl; ++i) sd2 = sd2 + dv[i] * dv[i];
-    for (int i = 0; i < nl; ++i) ss2 = ss2 + sv[i] * sv[i];
-    const double dlna = 0.5 * std::log(sd2 / ss2);
-    // calc_cc_error with cc_correction
-    std::vector<double> scc(nl, 0.0), sccd(nl, 0.0);
-    for (int i = 0; i < nl; ++i) {
-      const int j = i - ishift;
-      if (j >= 0 && j < nl) { scc[i] = sv[j]; sccd[i] = std::exp(dlna) * sv[j]; }
-    }
-    const std::vector<double> vel = gradient(sccd, dt);
-    double num = 0.0, den = 0.0, den2 = 0.0;
-    for (int i = 0; i < nl; ++i) num = num + (dv[i] - sccd[i]) * (dv[i] - sccd[i]);
-    for (int i = 0; i < nl; ++i) den = den + vel[i] * vel[i];
-    for (int i = 0; i < nl; ++i) den2 = den2 + scc[i] * scc[i];
-    double sig_dt = std::sqrt(num / den), sig_dlna = std::sqrt(num / den2);
-    if (sig_dt < cfg[0] || std::isnan(sig_dt)) sig_dt = cfg[0];
-    if (sig_dlna < cfg[1] || std::isnan(sig_dlna)) sig_dlna = cfg[1];
-    // cc_max_coef (same shift; d's edges zeroed where the shifted s is empty)
-    std::vector<double> dsh(dv);
-    if (ishift > 0) for (int i = 0; i < ishift && i < nl; ++i) dsh[i] = 0.0;
-    else if (ishift < 0) for (int i = nl + ishift; i < nl; ++i) if (i >= 0) dsh[i] = 0.0;
-    double a = 0.0, b = 0.0, c = 0.0;
-    for (int i = 0; i < nl; ++i) a = a + dsh[i] * scc[i];
-    for (int i = 0; i < nl; ++i) b = b + dsh[i] * dsh[i];
-    for (int i = 0; i < nl; ++i) c = c + scc[i] * scc[i];
-    const double cmax = a / std::sqrt(b * c);
-    ccmax[w] = cmax;
-    tshift_o[w] = tshift;
-    dlna_o[w] = dlna;
-    const bool sel = !(cmax < cfg[6] || tshift < cfg[2] || tshift > cfg[3] || dlna < cfg[4] || dlna > cfg[5]);
-    selected[w] = sel;
-    residuals[w] = imeas == 11 ? tshift : dlna;
-    errors[w] = imeas == 11 ? sig_dt : sig_dlna;
+    const CcStats c = cc_stats(dv, sv, dt, cfg);
+    ccmax[w] = c.cmax;
+    tshift_o[w] = c.tshift;
+    dlna_o[w] = c.dlna;
+    selected[w] = c.sel;
+    residuals[w] = imeas == 11 ? c.tshift : c.dlna;
+    errors[w] = imeas == 11 ? c.sig_dt : c.sig_dlna;
     misfits[w] = 0.0;
-    if (!sel) continue;
-    // calc_cc_adjsrc
-    const double mp = 0.5 * (tshift / sig_dt) * (tshift / sig_dt);
-    const double mq = 0.5 * (dlna / sig_dlna) * (dlna / sig_dlna);
-    std::vector<double> out(nl);
-    if (imeas == 11) {
-      const std::vector<double> dsdt = gradient(sv, dt);
-      std::vector<double> sq(nl);
-      for (int i = 0; i < nl; ++i) sq[i] = dsdt[i] * dsdt[i];
-      const double nn = orc_simpson(sq.data(), nl, dt);
-      for (int i = 0; i < nl; ++i) out[i] = dsdt[i] * tshift / nn / (sig_dt * sig_dt);
-    } else {
-      std::vector<double> sq(nl);
-      for (int i = 0; i < nl; ++i) sq[i] = sv[i] * sv[i];
-      const double nn = orc_simpson(sq.data(), nl, dt);
-      for (int i = 0; i < nl; ++i) out[i] = -1.0 * sv[i] * dlna / nn / (sig_dlna * sig_dlna);
-    }
+    if (!c.sel) continue;
+    std::vector<double> p, q;
+    double mp, mq;
+    cc_adjsrc(sv, c, dt, imeas, p, q, mp, mq);
+    std::vector<double>& out = imeas == 11 ? p : q;
     orc_window_taper(out.data(), nl, taper_pct, itaper);
     const double mf = imeas == 11 ? mp : mq;
     misfits[w] = mf;
     total = total + mf;
     for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = out[i];
+  }
+  return total;
+}
+
+// fft_raw_dp / fft_raw (fftpack.f90:70-100, 202-241) in double or single
+// precision: forward ind = -1, inverse +1 (unnormalised)
+void orc_fft(double* re, double* im, int n, int ind, int single) {
+  if (single) {
+    std::vector<std::complex<float>> x(n);
+    for (int i = 0; i < n; ++i) x[i] = {(float)re[i], (float)im[i]};
+    fft_raw(x, ind, 3.14159265358979323846f);
+    for (int i = 0; i < n; ++i) { re[i] = x[i].real(); im[i] = x[i].imag(); }
+  } else {
+    std::vector<std::complex<double>> x(n);
+    for (int i = 0; i < n; ++i) x[i] = {re[i], im[i]};
+    fft_raw(x, ind, 3.14159265358979323846);
+    for (int i = 0; i < n; ++i) { re[i] = x[i].real(); im[i] = x[i].imag(); }
+  }
+}
+
+// dpss_windows (utils/dpss.f90:137-262): out[k][n], unit norm, sign convention
+// of the reference; the eigenvalues of the tridiagonal matrix come from
+// bisection (the reference calls LAPACK dsterf: any method yields them to
+// rounding, and inverse iteration converges to the same vector)
+void orc_dpss(int n, double half_nbw, int kmax, double* out) {
+  std::vector<std::vector<double>> v;
+  dpss(n, half_nbw, kmax, v);
+  for (int k = 0; k < kmax; ++k)
+    for (int i = 0; i < n; ++i) out[(size_t)k * n + i] = v[k][i];
+}
+
+// Multitaper traveltime (imeas 13) or amplitude (14) measurement of one trace,
+// MTTTMisfit%calc_adjoint_source (mt_tt_misfit.f90:29-209) with its helpers
+// (:211-300, 411-518, 520-791), process_cycle_skipping (signal.f90:510-598) and
+// dpss_windows; falls back to the cross-correlation measurement as the
+// reference does.  cfg: [0..6] as orc_cc_adjoint, 7 transfunc_waterlevel,
+// 8 water_threshold, 9 mt_nw, 10 num_taper, 11 dt_fac, 12 err_fac,
+// 13 dt_max_scale, 14 min_cycle_in_window, 15 use_cc_error, 16 min_period,
+// 17 max_period (use_mt_error = .false., the default).  Per window out[10]:
+// misfit, residual, error, cc_max, tshift, dlna, selected, ishift, measurement
+// used (13/14, or 11/12 after the fallback, 0 unselected), the other misfit
+// (misfit_q for 13, misfit_p for 14).  Returns the total misfit.
+double orc_mt_adjoint(const double* dat, const double* syn, int n, double dt, const double* windows, int nwin,
+                      double taper_pct, int itaper, int imeas, const double* cfg, double* adj, double* out) {
+  using cd = std::complex<double>;
+  const double pi = 3.141592653589793;
+  double total = 0.0;
+  for (int i = 0; i < n; ++i) adj[i] = 0.0;
+  const int nf = next_pow2_above(n);  // 2 ** exponent(real(nlen))
+  const int fnum = nf / 2 + 1;
+  const double tlen = n * dt;
+  // fftfreq (fftpack.f90:114-133), then df = freq(2) - freq(1), wvec = 2 pi freq
+  std::vector<double> wvec(nf);
+  for (int i = 1; i <= nf; ++i) {
+    const double f = i <= nf / 2 ? (double)(i - 1) / (nf * dt) : -(double)(nf - i + 1) / (nf * dt);
+    wvec[i - 1] = 2.0 * pi * f;
+  }
+  const double df = 1.0 / (nf * dt) - 0.0;
+  const int ntaper = (int)cfg[10];
+  for (int w = 0; w < nwin; ++w) {
+    double* o = out + 10 * (size_t)w;
+    const int nb = (int)(windows[2 * w] / dt) + 1, ne = (int)(windows[2 * w + 1] / dt);
+    const int nl = ne - nb + 1;
+    std::vector<double> sv(syn + nb - 1, syn + ne), dv(dat + nb - 1, dat + ne);
+    orc_window_taper(sv.data(), nl, taper_pct, itaper);
+    orc_window_taper(dv.data(), nl, taper_pct, itaper);
+    const CcStats c = cc_stats(dv, sv, dt, cfg);
+    o[0] = 0.0;
+    o[1] = imeas == 13 ? c.tshift : c.dlna;
+    o[2] = imeas == 13 ? c.sig_dt : c.sig_dlna;
+    o[3] = c.cmax;
+    o[4] = c.tshift;
+    o[5] = c.dlna;
+    o[6] = c.sel;
+    o[7] = c.ishift;
+    o[8] = 0.0;
+    o[9] = 0.0;
+    if (!c.sel) continue;
+    bool mtm = true;
+    double mp = 0.0, mq = 0.0;
+    std::vector<double> ap, aq;
+    do {
+      // check_time_series_acceptability (:571-584): compares samples with seconds as written
+      if (std::fabs(c.tshift) <= dt || cfg[14] * cfg[16] > nl) { mtm = false; break; }
+      // prepare_data_for_mtm (:586-613)
+      const int ish = (int)(c.tshift / dt);
+      const int nbd = std::max(1, nb + ish), ned = std::min(n, ne + ish);
+      if (ned - nbd + 1 != nl) { mtm = false; break; }
+      std::vector<double> dm(nl);
+      for (int i = 0; i < nl; ++i) dm[i] = dat[nbd - 1 + i] * std::exp(-c.dlna);
+      orc_window_taper(dm.data(), nl, taper_pct, itaper);
+      // calculate_freq_limits (:615-684) on the whole synthetic trace
+      std::vector<cd> sspec(nf, cd(0.0, 0.0));
+      for (int i = 0; i < n; ++i) sspec[i] = cd(syn[i], 0.0);
+      fft_raw(sspec, -1, pi);
+      double ampmax = -1.0;
+      int iamp = 1;
+      std::vector<double> amp(fnum);
+      for (int i = 1; i <= fnum; ++i) {
+        const cd v = sspec[i - 1] * dt;
+        amp[i - 1] = std::abs(v);
+        if (amp[i - 1] > ampmax) { ampmax = amp[i - 1]; iamp = i; }
+      }
+      const double wl = cfg[8] * ampmax;
+      const int ifmin = std::max(1, (int)(1.0 / (cfg[17] * df)));
+      const int ifmax = std::min(fnum, (int)(1.0 / (cfg[16] * df)));
+      int nfmax = fnum, nfmin = 1;
+      bool srch = true;
+      auto thermostat = [&](int iw, int& lim) {
+        if (amp[iw - 1] < wl && srch) { srch = false; lim = iw; }
+        if (amp[iw - 1] > 10 * wl && !srch) { srch = true; lim = iw; }
+      };
+      if (iamp < fnum) for (int iw = iamp + 1; iw <= fnum; ++iw) thermostat(iw, nfmax);
+      nfmax = std::min(std::min(nfmax, ifmax), (int)(1.0 / (2.0 * dt) / df));
+      srch = true;
+      if (iamp > 1) for (int iw = iamp - 1; iw >= 1; --iw) thermostat(iw, nfmin);
+      nfmin = std::max(std::max(nfmin, ifmin), std::max(1, (int)(cfg[14] / tlen / df)));
+      if ((nfmax - nfmin) * df < cfg[9] / (4.0 * tlen)) { mtm = false; break; }
+      // dpss tapers of the window length, scaled by sqrt(nlen_win)
+      std::vector<std::vector<double>> tap;
+      dpss(nl, cfg[9], ntaper, tap);
+      for (auto& t : tap) for (double& x : t) x = x * std::sqrt((double)nl);
+      // calculate_multitaper (:211-300)
+      std::vector<cd> top(nf, cd(0.0, 0.0)), bot(nf, cd(0.0, 0.0));
+      for (int k = 0; k < ntaper; ++k) {
+        std::vector<cd> dw(nf, cd(0.0, 0.0)), sw(nf, cd(0.0, 0.0));
+        for (int i = 0; i < nl; ++i) { dw[i] = cd(dm[i] * tap[k][i], 0.0); sw[i] = cd(sv[i] * tap[k][i], 0.0); }
+        fft_raw(dw, -1, pi);
+        fft_raw(sw, -1, pi);
+        for (int i = 0; i < nf; ++i) {
+          dw[i] = dw[i] * dt;
+          sw[i] = sw[i] * dt;
+          top[i] = top[i] + dw[i] * std::conj(sw[i]);
+          bot[i] = bot[i] + sw[i] * std::conj(sw[i]);
+        }
+      }
+      double bmax = 0.0;
+      for (int i = 0; i < fnum; ++i) bmax = std::max(bmax, std::abs(bot[i]));
+      const double wlevel = bmax * (cfg[7] * cfg[7]);
+      std::vector<double> phi(nf, 0.0), absw(nf, 0.0), dtau(nf, 0.0), dlnw(nf, 0.0);
+      std::vector<cd> tf(nf, cd(0.0, 0.0));
+      for (int i = nfmin; i <= nfmax; ++i)
+        tf[i - 1] = std::abs(bot[i - 1]) >= wlevel ? cdivr(top[i - 1], bot[i - 1].real())
+                                                   : cdivr(top[i - 1], bot[i - 1].real() + wlevel);
+      for (int i = nfmin; i <= nfmax; ++i) phi[i - 1] = std::atan2(tf[i - 1].imag(), tf[i - 1].real());
+      cycle_skipping(phi, nfmin, nfmax, 1.5);
+      for (int i = nfmin; i <= nfmax; ++i) absw[i - 1] = std::abs(tf[i - 1]);
+      dtau[0] = c.tshift;
+      for (int i = std::max(2, nfmin); i <= nfmax; ++i) dtau[i - 1] = -(1.0 / wvec[i - 1]) * phi[i - 1] + c.tshift;
+      for (int i = nfmin; i <= nfmax; ++i) dlnw[i - 1] = std::log(absw[i - 1]) + c.dlna;
+      // check_mtm_time_shift_acceptability (:520-569), MT errors zero
+      for (int j = nfmin; j <= nfmax && mtm; ++j) {
+        if (j * df > 0.0) {
+          const double per = 1.0 / (j * df);
+          if (std::fabs(dtau[j - 1]) > per / cfg[11]) mtm = false;
+          else if (0.0 > per / cfg[12]) mtm = false;
+          else if (std::fabs(dtau[j - 1]) > cfg[13] * std::fabs(c.tshift)) mtm = false;
+        }
+      }
+      if (!mtm) break;
+      // calculate_freq_domain_taper (:709-791)
+      const int L = nfmax - nfmin;
+      std::vector<double> wt(nf, 0.0), wtap(std::max(L, 0), 1.0);
+      orc_window_taper(wtap.data(), L, 1.0, 3);
+      for (int i = 0; i < L; ++i) wt[nfmin - 1 + i] = wtap[i];
+      double ssum = 0.0;
+      for (int i = nfmin; i <= nfmax - 1; ++i) ssum = ssum + wt[i - 1];
+      const double ffac = 2.0 * df * ssum;
+      std::vector<double> wp(nf), wq(nf);
+      for (int i = 0; i < nf; ++i) { wp[i] = wt[i] / ffac; wq[i] = wt[i] / ffac; }
+      if (cfg[15] != 0.0)
+        for (int i = 0; i < nf; ++i) { wp[i] = wp[i] / (c.sig_dt * c.sig_dt); wq[i] = wq[i] / (c.sig_dlna * c.sig_dlna); }
+      std::vector<double> yp(nf), yq(nf);
+      for (int i = 0; i < nf; ++i) { yp[i] = dtau[i] * dtau[i] * wp[i]; yq[i] = dlnw[i] * dlnw[i] * wq[i]; }
+      mp = orc_simpson(yp.data(), nf, df);
+      mq = orc_simpson(yq.data(), nf, df);
+      // calculate_mt_adjsrc (:411-518); the inverse transform in single precision
+      std::vector<std::vector<cd>> stw(ntaper), stvw(ntaper);
+      std::vector<cd> bp(nf, cd(0.0, 0.0)), bq(nf, cd(0.0, 0.0));
+      for (int k = 0; k < ntaper; ++k) {
+        std::vector<double> st(nl);
+        for (int i = 0; i < nl; ++i) st[i] = sv[i] * tap[k][i];
+        const std::vector<double> stv = gradient(st, dt);
+        stw[k].assign(nf, cd(0.0, 0.0));
+        stvw[k].assign(nf, cd(0.0, 0.0));
+        for (int i = 0; i < nl; ++i) { stw[k][i] = cd(st[i], 0.0); stvw[k][i] = cd(stv[i], 0.0); }
+        fft_raw(stw[k], -1, pi);
+        fft_raw(stvw[k], -1, pi);
+        for (int i = 0; i < nf; ++i) {
+          stw[k][i] = stw[k][i] * dt;
+          stvw[k][i] = stvw[k][i] * dt;
+          bp[i] = bp[i] + stvw[k][i] * std::conj(stvw[k][i]);
+          bq[i] = bq[i] + stw[k][i] * std::conj(stw[k][i]);
+        }
+      }
+      ap.assign(nl, 0.0);
+      aq.assign(nl, 0.0);
+      for (int k = 0; k < ntaper; ++k) {
+        std::vector<std::complex<float>> pw(nf, 0.0f), qw(nf, 0.0f);
+        for (int i = 1; i <= nf; ++i) {
+          cd p(0.0, 0.0), q(0.0, 0.0);
+          if (i >= nfmin && i <= nfmax) {
+            if (std::abs(bp[i - 1]) > 0.0) p = cdivr(stvw[k][i - 1], bp[i - 1].real());
+            if (std::abs(bq[i - 1]) > 0.0) q = -cdivr(stw[k][i - 1], bq[i - 1].real());
+          }
+          p = p * dtau[i - 1] * wp[i - 1];
+          q = q * dlnw[i - 1] * wq[i - 1];
+          pw[i - 1] = std::complex<float>((float)p.real(), (float)p.imag()) / (float)nf;
+          qw[i - 1] = std::complex<float>((float)q.real(), (float)q.imag()) / (float)nf;
+        }
+        fft_raw(pw, 1, 3.14159265358979323846f);
+        fft_raw(qw, 1, 3.14159265358979323846f);
+        for (int i = 0; i < nl; ++i) {
+          ap[i] = ap[i] + (double)pw[i].real() * 2.0 / dt * tap[k][i];
+          aq[i] = aq[i] + (double)qw[i].real() * 2.0 / dt * tap[k][i];
+        }
+      }
+    } while (false);
+    if (!mtm) {
+      cc_adjsrc(sv, c, dt, 0, ap, aq, mp, mq);
+      o[8] = imeas == 13 ? 11 : 12;
+    } else {
+      o[8] = imeas;
+    }
+    orc_window_taper(ap.data(), nl, taper_pct, itaper);
+    orc_window_taper(aq.data(), nl, taper_pct, itaper);
+    const std::vector<double>& a = imeas == 13 ? ap : aq;
+    const double mf = imeas == 13 ? mp : mq;
+    o[0] = mf;
+    o[9] = imeas == 13 ? mq : mp;
+    total = total + mf;
+    for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = adj[nb - 1 + i] + a[i];
   }
   return total;
 }

@@ -136,6 +136,11 @@ def lib():
         L.orc_cc_adjoint.restype = C.c_double
         L.orc_cc_adjoint.argtypes = ([_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
                                       _dp] + [_dp] * 7 + [_ip])
+        L.orc_fft.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
+        L.orc_dpss.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
+        L.orc_mt_adjoint.restype = C.c_double
+        L.orc_mt_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                     _dp, _dp, _dp]
         L.orc_sd_direction.argtypes = [C.c_long, _fp, _fp, _fp, C.c_int]
         L.orc_lbfgs_direction.argtypes = [C.c_long, C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
                                           C.c_float, _fp, _dp, C.c_int]
@@ -542,4 +547,54 @@ def cc_adjoint(dat, syn, dt, windows, imeas=11, taper_pct=0.05, itaper=1, **cfg)
                                *[_p(o[k], C.c_double) for k in ("misfit", "residual", "error", "cc_max", "tshift",
                                                                  "dlna")], _p(sel, C.c_int))
     o["selected"] = sel.astype(bool)
+    return tot, adj, o
+
+
+def fft(x, inverse=False, single=False):
+    """fft_raw_dp / fft_raw of the reference: unnormalised radix-2 transform,
+    forward exp(-i...), inverse exp(+i...); len(x) a power of two."""
+    x = np.asarray(x, np.complex128)
+    re = np.ascontiguousarray(x.real)
+    im = np.ascontiguousarray(x.imag)
+    lib().orc_fft(_p(re, C.c_double), _p(im, C.c_double), len(x), 1 if inverse else -1, int(single))
+    return re + 1j * im
+
+
+def dpss(n, half_nbw, kmax):
+    """dpss_windows: [kmax][n] unit-norm discrete prolate spheroidal sequences."""
+    out = np.zeros((kmax, n))
+    lib().orc_dpss(n, half_nbw, kmax, _p(out, C.c_double))
+    return out
+
+
+MT_DEFAULTS = dict(transfunc_waterlevel=1e-10, water_threshold=0.02, mt_nw=4.0, num_taper=5, dt_fac=2.0,
+                   err_fac=2.5, dt_max_scale=3.5, min_cycle_in_window=3, use_cc_error=True)  # adj_config.f90:18-26
+MT_FIELDS = ("misfit", "residual", "error", "cc_max", "tshift", "dlna", "selected", "ishift", "imeas", "misfit_other")
+
+
+def mt_config(min_period, max_period, **cfg):
+    c = dict(CC_DEFAULTS, **MT_DEFAULTS)
+    c.update(cfg)
+    return np.array([c["dt_sigma_min"], c["dlna_sigma_min"], c["tshift_min"], c["tshift_max"], c["dlna_min"],
+                     c["dlna_max"], c["cc_min"], c["transfunc_waterlevel"], c["water_threshold"], c["mt_nw"],
+                     c["num_taper"], c["dt_fac"], c["err_fac"], c["dt_max_scale"], c["min_cycle_in_window"],
+                     1.0 if c["use_cc_error"] else 0.0, min_period, max_period], np.float64)
+
+
+def mt_adjoint(dat, syn, dt, windows, min_period, max_period, imeas=13, taper_pct=0.05, itaper=1, **cfg):
+    """One trace's multitaper traveltime (13) or amplitude (14) measurement with
+    the reference's cross-correlation fallback: (total, adj, per-window dict of
+    MT_FIELDS; "imeas" is the measurement used, 11/12 after a fallback)."""
+    cv = mt_config(min_period, max_period, **cfg)
+    dat = np.ascontiguousarray(dat, np.float64)
+    syn = np.ascontiguousarray(syn, np.float64)
+    win = np.ascontiguousarray(np.reshape(windows, (-1, 2)), np.float64)
+    nw = win.shape[0]
+    adj = np.zeros(len(dat))
+    out = np.zeros((nw, 10))
+    tot = lib().orc_mt_adjoint(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt, _p(win, C.c_double), nw,
+                               taper_pct, itaper, imeas, _p(cv, C.c_double), _p(adj, C.c_double),
+                               _p(out, C.c_double))
+    o = {k: out[:, i].copy() for i, k in enumerate(MT_FIELDS)}
+    o["selected"] = o["selected"] != 0
     return tot, adj, o

@@ -99,6 +99,7 @@ SIGNATURES = {
     "fwat_waveform_adjoint_batch_": "iidFFiDdiiFDDF",
     "fwat_waveform_adjoint_last_time_": "d",
     "fwat_cc_adjoint_batch_": "iidFFiDdiiDiFDDF",
+    "fwat_mt_adjoint_batch_": "iidFFiDdiiDiFDDF",
 }
 
 _lib = None

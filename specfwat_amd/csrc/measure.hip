@@ -246,7 +246,7 @@ __device__ double wg_sum(double v, double* sh) {
 __global__ void __launch_bounds__(256) k_cc_finish(int nl, double dt, int imeas, CcCfg cfg,
                                                    const double* __restrict__ tw, const double* __restrict__ D,
                                                    const double* __restrict__ S, const int* __restrict__ ishift_in,
-                                                   double* __restrict__ A, double* __restrict__ out) {
+                                                   double* __restrict__ A, double* __restrict__ out, int ostride) {
   __shared__ double sh[256];
   const int t = blockIdx.x;
   const double* d = D + (size_t)t * nl;
@@ -285,7 +285,11 @@ __global__ void __launch_bounds__(256) k_cc_finish(int nl, double dt, int imeas,
   const bool sel = !(cmax < cfg.cc_min || tshift < cfg.tshift_min || tshift > cfg.tshift_max ||
                      dlna < cfg.dlna_min || dlna > cfg.dlna_max);
   if (threadIdx.x == 0) {
-    double* o = out + 8 * (size_t)t;
+    double* o = out + ostride * (size_t)t;
+    if (ostride >= 10) {  // both error estimates for the multitaper stage
+      o[8] = sig_dt;
+      o[9] = sig_dlna;
+    }
     o[1] = imeas == 11 ? tshift : dlna;
     o[2] = imeas == 11 ? sig_dt : sig_dlna;
     o[3] = cmax;
@@ -343,6 +347,446 @@ __global__ void __launch_bounds__(256) k_cc_put(int ntrace, int nb, int nl, cons
   }
 }
 
+
+// ---- multitaper family (mt_tt_misfit.f90, utils/dpss.f90)
+//
+// Real transforms of length nf = 2^p (64 <= nf <= 16384) inside one workgroup:
+// the nf real samples are packed as M = nf/2 complex values (even + i odd) in
+// LDS at bit-reversed positions, a radix-2 decimation-in-time transform of
+// length M runs in place, and the spectrum X_k (k = 0..M) is unpacked in pairs
+// (p, M - p), each pair owned by one lane.  Twiddles exp(-2 pi i j / nf),
+// j < nf/2, come from a table in device memory.
+constexpr int MT_THREADS = 1024;
+constexpr int MT_MAXM = 8192;
+constexpr int MT_PAIRS = MT_MAXM / 2 / MT_THREADS + 1;  // pair p = lane + 1024 r, p <= M/2
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int NT, class F>
+__device__ void mt_load_real(double2* buf, int M, int logM, int len, F f) {
+#pragma unroll 2
+  for (int m = threadIdx.x; m < M; m += NT) {
+    const int n0 = 2 * m, n1 = 2 * m + 1;
+    const double a = n0 < len ? f(n0) : 0.0, b = n1 < len ? f(n1) : 0.0;
+    buf[__brev((unsigned)m) >> (32 - logM)] = make_double2(a, b);
+  }
+  __syncthreads();
+}
+
+// in-place transform of bit-reversed buf; inverse uses the conjugate twiddles (unnormalised)
+template <int NT>
+__device__ void mt_fft(double2* buf, int M, int logM, int nf, const double2* __restrict__ tw, bool inverse) {
+  for (int st = 1; st <= logM; ++st) {
+    const int h = 1 << (st - 1), tstep = nf / (2 * h);
+#pragma unroll 2
+    for (int b = threadIdx.x; b < M / 2; b += NT) {
+      const int pos = b & (h - 1);
+      const int i1 = ((b >> (st - 1)) << st) + pos, i2 = i1 + h;
+      double2 w = tw[pos * tstep];
+      if (inverse) w.y = -w.y;
+      const double2 a = buf[i1], c = cmul(buf[i2], w);
+      buf[i1] = make_double2(a.x + c.x, a.y + c.y);
+      buf[i2] = make_double2(a.x - c.x, a.y - c.y);
+    }
+    __syncthreads();
+  }
+}
+
+// X_k = E - i W_k O with E = (Z_k + conj Z_{M-k}) / 2, O = (Z_k - conj Z_{M-k}) / 2
+__device__ inline double2 mt_half(double2 zk, double2 zm, double2 w) {
+  const double2 E = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+  const double2 O = make_double2(0.5 * (zk.x - zm.x), 0.5 * (zk.y + zm.y));
+  const double2 wo = cmul(w, O);
+  return make_double2(E.x + wo.y, E.y - wo.x);
+}
+
+// spectrum of the packed real sequence at k = p and k = M - p
+__device__ inline void mt_unpack(const double2* buf, int M, const double2* __restrict__ tw, int p, double2& xa,
+                                 double2& xb) {
+  if (p == 0) {
+    const double2 z = buf[0];
+    xa = make_double2(z.x + z.y, 0.0);
+    xb = make_double2(z.x - z.y, 0.0);
+    return;
+  }
+  const double2 zp = buf[p], zq = buf[M - p];
+  xa = mt_half(zp, zq, tw[p]);
+  xb = mt_half(zq, zp, tw[M - p]);
+}
+
+// inverse packing: Z'_k = ((H_k + conj H_{M-k}) + i V_k (H_k - conj H_{M-k})) / 2,
+// V_k = exp(+2 pi i k / nf), for a Hermitian spectrum H of a real sequence
+__device__ inline double2 mt_repack(double2 hk, double2 hm, double2 w) {
+  const double2 a = make_double2(hk.x + hm.x, hk.y - hm.y);
+  const double2 b = make_double2(hk.x - hm.x, hk.y + hm.y);
+  const double2 vb = cmul(make_double2(w.x, -w.y), b);
+  return make_double2(0.5 * (a.x - vb.y), 0.5 * (a.y + vb.x));
+}
+
+template <int NT>
+__device__ double mt_bsum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = NT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <int NT>
+__device__ double mt_bmax(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = NT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// calculate_freq_limits (mt_tt_misfit.f90:615-707) of every trace's whole
+// synthetic: amplitude spectrum, its first maximum, the water-level
+// "thermostat" search up and down from it, the period and cycle bounds and
+// the half-bandwidth test; flim[3 t] = nfreq_min, nfreq_max (1-based), accepted
+__global__ void __launch_bounds__(MT_THREADS) k_mt_flim(int nstep, int ntrace, double dt, int nf, int logM,
+                                                        const double2* __restrict__ tw, const float* __restrict__ syn,
+                                                        double water_threshold, double min_period, double max_period,
+                                                        double mt_nw, double min_cycle, int* __restrict__ flim) {
+  __shared__ double2 buf[MT_MAXM];
+  __shared__ double rv[MT_THREADS];
+  __shared__ int ri[MT_THREADS];
+  const int t = blockIdx.x, M = nf / 2;
+  mt_load_real<MT_THREADS>(buf, M, logM, nstep, [&](int n) { return (double)syn[(size_t)n * ntrace + t]; });
+  mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
+  // amplitudes |X_k dt| in place: k < M at buf[k].x, k = M at buf[0].y
+  double best = -1.0;
+  int bk = 0x7fffffff;
+  for (int r = 0; r < MT_PAIRS; ++r) {
+    const int p = threadIdx.x + MT_THREADS * r;
+    if (p > M / 2) break;
+    double2 xa, xb;
+    mt_unpack(buf, M, tw, p, xa, xb);
+    const double aa = hypot(xa.x * dt, xa.y * dt), ab = hypot(xb.x * dt, xb.y * dt);
+    const int kb = M - p;
+    if (aa > best || (aa == best && p < bk)) { best = aa; bk = p; }
+    if (kb != p && (ab > best || (ab == best && kb < bk))) { best = ab; bk = kb; }
+    buf[p].x = aa;
+    if (p == 0) buf[0].y = ab;
+    else if (kb != p) buf[kb].x = ab;
+  }
+  rv[threadIdx.x] = best;
+  ri[threadIdx.x] = bk;
+  __syncthreads();
+  for (int o = MT_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const double v2 = rv[threadIdx.x + o];
+      const int k2 = ri[threadIdx.x + o];
+      if (v2 > rv[threadIdx.x] || (v2 == rv[threadIdx.x] && k2 < ri[threadIdx.x])) {
+        rv[threadIdx.x] = v2;
+        ri[threadIdx.x] = k2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const int fnum = M + 1;
+  const double ampmax = rv[0];
+  const int iamp = ri[0] + 1;
+  auto amp = [&](int i) { return i - 1 < M ? buf[i - 1].x : buf[0].y; };
+  const double df = 1.0 / (nf * dt) - 0.0, tlen = nstep * dt;
+  const double wl = water_threshold * ampmax;
+  const int ifmin = max(1, (int)(1.0 / (max_period * df)));
+  const int ifmax = min(fnum, (int)(1.0 / (min_period * df)));
+  int nfmax = fnum, nfmin = 1;
+  bool srch = true;
+  if (iamp < fnum)
+    for (int iw = iamp + 1; iw <= fnum; ++iw) {
+      const double a = amp(iw);
+      if (a < wl && srch) { srch = false; nfmax = iw; }
+      if (a > 10 * wl && !srch) { srch = true; nfmax = iw; }
+    }
+  nfmax = min(min(nfmax, ifmax), (int)(1.0 / (2.0 * dt) / df));
+  srch = true;
+  if (iamp > 1)
+    for (int iw = iamp - 1; iw >= 1; --iw) {
+      const double a = amp(iw);
+      if (a < wl && srch) { srch = false; nfmin = iw; }
+      if (a > 10 * wl && !srch) { srch = true; nfmin = iw; }
+    }
+  nfmin = max(max(nfmin, ifmin), max(1, (int)(min_cycle / tlen / df)));
+  flim[3 * t] = nfmin;
+  flim[3 * t + 1] = nfmax;
+  flim[3 * t + 2] = (nfmax - nfmin) * df < mt_nw / (4.0 * tlen) ? 0 : 1;
+}
+
+struct MtCfg {
+  double wl_tf, dt_fac, err_fac, dt_max_scale, min_cycle, min_period;
+  int ntaper, use_cc_error;
+};
+
+// Multitaper measurement of one window of every selected trace (one
+// workgroup per trace), after the cross-correlation stage has written its
+// result and adjoint source: prepare_data_for_mtm, calculate_multitaper (sum
+// over the DPSS tapers of D conj(S) and |S|^2, water level, phase with
+// process_cycle_skipping, dtau and dlnA per frequency),
+// check_mtm_time_shift_acceptability, calculate_freq_domain_taper, the
+// Simpson misfits and calculate_mt_adjsrc.  A trace that fails a test keeps
+// the cross-correlation result, as the reference falls back to it.
+// Frequency k = lane + 1024 r of the M + 1 (k = M: Nyquist, lane 0); the
+// per-frequency state lives in three device rows of M + 1 per trace, each
+// lane touching only its own frequencies (coalesced, no cross-lane hazard):
+//   G0 (double2): the synthetic's spectrum of the current taper, then dlnA,
+//                 then the weighted adjoint spectrum H
+//   G1 (double2): sum D conj(S), then (dtau, w), then the adjoint factors
+//   G2 (double) : sum |S|^2, then the adjoint normalisation sum
+// out[10] per trace: misfit, residual, error, cc_max, tshift, dlna, selected,
+// ishift, then (on entry) sigma_dt, sigma_dlna and (on exit) the measurement
+// used and the other misfit.
+__global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace, double dt, int nf, int logM,
+                                                          const double2* __restrict__ tw, int nb, int nl, int imeas,
+                                                          MtCfg cfg, const double* __restrict__ wtap,
+                                                          const double* __restrict__ tap,
+                                                          const float* __restrict__ obs,
+                                                          const double* __restrict__ S, const int* __restrict__ flim,
+                                                          double2* __restrict__ G0, double2* __restrict__ G1,
+                                                          double* __restrict__ G2, double* __restrict__ A,
+                                                          double* __restrict__ out) {
+  __shared__ double2 buf[MT_MAXM];
+  __shared__ double red[MT_THREADS];
+  const double pi = 3.141592653589793;
+  const int t = blockIdx.x, M = nf / 2, lane = threadIdx.x;
+  const bool tt = imeas == 13;
+  double* o = out + 10 * (size_t)t;
+  const bool sel = o[6] != 0.0;
+  const double tshift = o[4], dlna = o[5], sig_dt = o[8], sig_dlna = o[9];
+  bool mt = sel;
+  if (mt && (fabs(tshift) <= dt || cfg.min_cycle * cfg.min_period > nl)) mt = false;
+  const int ish = (int)(tshift / dt);
+  const int nbd = max(1, nb + 1 + ish), ned = min(nstep, nb + nl + ish);
+  if (mt && ned - nbd + 1 != nl) mt = false;
+  const int nfmin = flim[3 * t], nfmax = flim[3 * t + 1];
+  if (mt && flim[3 * t + 2] == 0) mt = false;
+  const double mp_cc = 0.5 * (tshift / sig_dt) * (tshift / sig_dt);
+  const double mq_cc = 0.5 * (dlna / sig_dlna) * (dlna / sig_dlna);
+  __syncthreads();
+  if (!mt || nfmax > M + 1 || nfmin < 1) {
+    if (lane == 0) {
+      o[8] = sel ? (tt ? 11.0 : 12.0) : 0.0;
+      o[9] = sel ? (tt ? mq_cc : mp_cc) : 0.0;
+    }
+    return;
+  }
+  const double df = 1.0 / (nf * dt) - 0.0;
+  const double em = exp(-dlna);
+  const double* s = S + (size_t)t * nl;
+  double2* g0 = G0 + (size_t)t * (M + 1);
+  double2* g1 = G1 + (size_t)t * (M + 1);
+  double* g2 = G2 + (size_t)t * (M + 1);
+  // spectrum of the packed real sequence at 0 <= k < M, and at k = M
+  auto xk = [&](int k) {
+    if (k == M) { const double2 z = buf[0]; return make_double2(z.x - z.y, 0.0); }
+    if (k == 0) { const double2 z = buf[0]; return make_double2(z.x + z.y, 0.0); }
+    return mt_half(buf[k], buf[M - k], tw[k]);
+  };
+  // ---- calculate_multitaper
+  for (int j = 0; j < cfg.ntaper; ++j) {
+    const double* tj = tap + (size_t)j * nl;
+    mt_load_real<MT_THREADS>(buf, M, logM, nl, [&](int n) { return s[n] * tj[n]; });
+    mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
+    for (int k = lane; k <= M; k += MT_THREADS) g0[k] = xk(k);
+    __syncthreads();
+    mt_load_real<MT_THREADS>(buf, M, logM, nl, [&](int n) {
+      return (double)obs[(size_t)(nbd - 1 + n) * ntrace + t] * em * wtap[n] * tj[n];
+    });
+    mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
+    for (int k = lane; k <= M; k += MT_THREADS) {
+      const double2 x = xk(k), y = g0[k];
+      const double a = x.x * dt, b = x.y * dt, c = y.x * dt, d = -(y.y * dt);
+      double2 tp = j == 0 ? make_double2(0.0, 0.0) : g1[k];
+      double bt = j == 0 ? 0.0 : g2[k];
+      tp.x = tp.x + (a * c - b * d);
+      tp.y = tp.y + (a * d + b * c);
+      bt = bt + (c * c - d * (-d));
+      g1[k] = tp;
+      g2[k] = bt;
+    }
+    __syncthreads();
+  }
+  double lm = 0.0;
+  for (int k = lane; k <= M; k += MT_THREADS) lm = fmax(lm, fabs(g2[k]));
+  const double wlevel = mt_bmax<MT_THREADS>(lm, red) * (cfg.wl_tf * cfg.wl_tf);
+  // transfer function in the band: phase to LDS for the cycle-skipping pass, log amplitude + dlnA to G0
+  double* phiL = reinterpret_cast<double*>(buf);
+  for (int k = lane; k <= M; k += MT_THREADS) {
+    double lw = 0.0;
+    const int i = k + 1;
+    if (i >= nfmin && i <= nfmax) {
+      const double bt = g2[k];
+      const double2 tp = g1[k];
+      const double den = fabs(bt) >= wlevel ? bt : bt + wlevel;
+      const double tr = tp.x / den, ti = tp.y / den;
+      phiL[k] = atan2(ti, tr);
+      lw = log(hypot(tr, ti)) + dlna;
+    }
+    g0[k] = make_double2(lw, 0.0);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    // process_cycle_skipping (signal.f90:510-598), phase_step 1.5
+    for (int iw = nfmin + 2; iw <= nfmax - 1; ++iw) {
+      const double p0 = phiL[iw - 2], p1 = phiL[iw - 1], p2 = phiL[iw];
+      const double s0 = fabs(p2 + p0 - 2.0 * p1);
+      const double s1 = fabs((p2 + 2.0 * pi) + p0 - 2.0 * p1);
+      const double s2 = fabs((p2 - 2.0 * pi) + p0 - 2.0 * p1);
+      if (fabs(p1 - p2) > 1.5) {
+        if (s1 < s0 && s1 < s2)
+          for (int i = iw + 1; i <= nfmax; ++i) phiL[i - 1] = phiL[i - 1] + 2.0 * pi;
+        if (s2 < s0 && s2 < s1)
+          for (int i = iw + 1; i <= nfmax; ++i) phiL[i - 1] = phiL[i - 1] - 2.0 * pi;
+      }
+    }
+  }
+  __syncthreads();
+  // dtau, the acceptance test, the frequency-domain taper weights -> G1 = (dtau, w)
+  const int L = nfmax - nfmin, frac = (int)(L * 1.0 / 2.0), fden = 2 * frac - 1;
+  double bad = 0.0, wsum = 0.0;
+  for (int k = lane; k <= M; k += MT_THREADS) {
+    const int i = k + 1;
+    const double f = i <= nf / 2 ? (double)(i - 1) / (nf * dt) : -(double)(nf - i + 1) / (nf * dt);
+    const double wvec = 2.0 * pi * f;
+    double dtau = 0.0;
+    if (i == 1) dtau = tshift;
+    else if (i >= nfmin && i <= nfmax) dtau = -(1.0 / wvec) * phiL[k] + tshift;
+    if (i >= nfmin && i <= nfmax && i * df > 0.0) {
+      // check_mtm_time_shift_acceptability; the multitaper errors are zero
+      const double per = 1.0 / (i * df);
+      if (fabs(dtau) > per / cfg.dt_fac || fabs(dtau) > cfg.dt_max_scale * fabs(tshift)) bad = 1.0;
+    }
+    double w = 0.0;
+    if (i >= nfmin && i <= nfmax - 1) {
+      // calculate_freq_domain_taper: 100 % cosine taper over [nfreq_min, nfreq_max - 1]
+      const int q = i - nfmin;
+      w = 1.0;
+      if (frac > 0) {
+        if (q < frac) w = cos(pi * (double)q / fden - pi / 2.0);
+        else if (q >= L - frac) w = cos(pi * (double)(frac + q - (L - frac)) / fden - pi / 2.0);
+      }
+      wsum += w;
+    }
+    g1[k] = make_double2(dtau, w);
+  }
+  if (mt_bmax<MT_THREADS>(bad, red) != 0.0) {
+    if (lane == 0) {
+      o[8] = tt ? 11.0 : 12.0;
+      o[9] = tt ? mq_cc : mp_cc;
+    }
+    return;
+  }
+  const double ffac = 2.0 * df * mt_bsum<MT_THREADS>(wsum, red);
+  // Simpson misfits over the nf frequency samples (only k <= nf/2 < nf - 2 can be
+  // non-zero); G1 becomes the adjoint factors (dtau, wp) or (dlnA, wq)
+  double sp = 0.0, sq = 0.0;
+  for (int k = lane; k <= M; k += MT_THREADS) {
+    const double2 dw = g1[k];
+    const double lw = g0[k].x;
+    double wp = dw.y / ffac, wq = dw.y / ffac;
+    if (cfg.use_cc_error) { wp = wp / (sig_dt * sig_dt); wq = wq / (sig_dlna * sig_dlna); }
+    const double sw = k == 0 ? 1.0 : (k % 2 == 1 ? 4.0 : 2.0);
+    sp += sw * (dw.x * dw.x * wp);
+    sq += sw * (lw * lw * wq);
+    g1[k] = tt ? make_double2(dw.x, wp) : make_double2(lw, wq);
+  }
+  const double mp = (df / 3.0) * mt_bsum<MT_THREADS>(sp, red);
+  const double mq = (df / 3.0) * mt_bsum<MT_THREADS>(sq, red);
+  // ---- calculate_mt_adjsrc for the measurement asked (phase 13: d/dt of the
+  // tapered synthetic, amplitude 14: the tapered synthetic)
+  auto xin = [&](const double* tj, int n) {
+    if (!tt) return s[n] * tj[n];
+    if (nl < 2) return 0.0;
+    if (n == 0) return (s[1] * tj[1] - s[0] * tj[0]) / dt;
+    if (n == nl - 1) return (s[nl - 1] * tj[nl - 1] - s[nl - 2] * tj[nl - 2]) / dt;
+    return (s[n + 1] * tj[n + 1] - s[n - 1] * tj[n - 1]) / (2.0 * dt);
+  };
+  for (int j = 0; j < cfg.ntaper; ++j) {
+    const double* tj = tap + (size_t)j * nl;
+    mt_load_real<MT_THREADS>(buf, M, logM, nl, [&](int n) { return xin(tj, n); });
+    mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
+    for (int k = lane; k <= M; k += MT_THREADS) {
+      const double2 x = xk(k);
+      const double a = x.x * dt, b = x.y * dt;
+      g2[k] = (j == 0 ? 0.0 : g2[k]) + (a * a - b * (-b));
+    }
+    __syncthreads();
+  }
+  double* arow = A + (size_t)t * nl;
+  const unsigned shb = 32 - logM;
+  for (int j = 0; j < cfg.ntaper; ++j) {
+    const double* tj = tap + (size_t)j * nl;
+    mt_load_real<MT_THREADS>(buf, M, logM, nl, [&](int n) { return xin(tj, n); });
+    mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
+    // weighted one-sided spectrum P_k and its Hermitian part H_k (Re ifft(P) =
+    // ifft(H)) go to G0: the transform in LDS is still being read
+    for (int k = lane; k <= M; k += MT_THREADS) {
+      double2 P = make_double2(0.0, 0.0);
+      const int i = k + 1;
+      const double bs = g2[k];
+      if (i >= nfmin && i <= nfmax && fabs(bs) > 0.0) {
+        const double2 x = xk(k), f = g1[k];
+        const double a = x.x * dt, b = x.y * dt;
+        P = tt ? make_double2(a / bs, b / bs) : make_double2(-(a / bs), -(b / bs));
+        P = make_double2(P.x * f.x * f.y, P.y * f.x * f.y);
+      }
+      g0[k] = (k == 0 || k == M) ? make_double2(P.x, 0.0) : make_double2(0.5 * P.x, 0.5 * P.y);
+    }
+    __syncthreads();
+    for (int k = lane; k < M; k += MT_THREADS)
+      buf[__brev((unsigned)k) >> shb] = mt_repack(g0[k], g0[M - k], tw[k]);
+    __syncthreads();
+    mt_fft<MT_THREADS>(buf, M, logM, nf, tw, true);
+    const bool last = j == cfg.ntaper - 1;
+    for (int n = lane; n < nl && n < 2 * M; n += MT_THREADS) {
+      const double2 z = buf[n >> 1];
+      const double y = ((n & 1) ? z.y : z.x) / M;
+      double v = (j == 0 ? 0.0 : arow[n]) + y * 2.0 / dt * tj[n];
+      if (last) v = v * wtap[n];
+      arow[n] = v;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    o[0] = tt ? mp : mq;
+    o[8] = (double)imeas;
+    o[9] = tt ? mq : mp;
+  }
+}
+
+// selected traces' adjoint-source rows added into the sample-major traces
+// (the multitaper measurement accumulates its windows)
+__global__ void __launch_bounds__(256) k_mt_put(int ntrace, int nb, int nl, const double* __restrict__ A,
+                                                const double* __restrict__ out, float* __restrict__ adj) {
+  __shared__ double tile[64][65];
+  const int t0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int t = t0 + r, k = k0 + (threadIdx.x & 63);
+    if (t < ntrace && k < nl) tile[r][threadIdx.x & 63] = A[(size_t)t * nl + k];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int k = k0 + r, t = t0 + (threadIdx.x & 63);
+    if (t < ntrace && k < nl && out[10 * (size_t)t + 6] != 0.0) {
+      float* a = adj + (size_t)(nb + k) * ntrace + t;
+      *a = (float)((double)*a + tile[threadIdx.x & 63][r]);
+    }
+  }
+}
+
 __global__ void k_fill_f32m(size_t n, float* __restrict__ x) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = 0.0f;
@@ -379,6 +823,89 @@ void window_range(double w0, double w1, double dt, int nstep, int& nb, int& nl) 
   nl = e - b + 1;
 }
 
+
+// dpss_windows (utils/dpss.f90:137-262) for the multitaper measurement: the
+// kmax largest eigenvalues of the DPSS tridiagonal matrix by bisection on
+// Sturm counts (the reference calls LAPACK dsterf), eigenvectors by the
+// reference's inverse iteration from sin(k pi i / (n - 1)) with its
+// Golub-Van Loan solve, unit norm, the reference's sign convention; each
+// taper is returned scaled by sqrt(n) as calc_adjoint_source uses it
+void dpss_tapers(int n, double half_nbw, int kmax, std::vector<double>& out) {
+  const double pi = 3.141592653589793;
+  const double wbin = half_nbw / (double)n;
+  std::vector<double> dg(n), od(n > 1 ? n - 1 : 0);
+  for (int i = 0; i < n; ++i) {
+    const double q = ((double)(n - 1) - 2.0 * i) / 2.0;
+    dg[i] = q * q * std::cos(2.0 * pi * wbin);
+  }
+  for (int i = 0; i + 1 < n; ++i) od[i] = (double)(i + 1) * ((double)n - (double)(i + 1)) / 2.0;
+  auto below = [&](double x) {
+    int c = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {
+      q = (dg[i] - x) - (i ? od[i - 1] * od[i - 1] / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      if (q < 0.0) ++c;
+    }
+    return c;
+  };
+  double lo = dg[0], hi = dg[0];
+  for (int i = 0; i < n; ++i) {
+    const double r = (i ? std::fabs(od[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(od[i]) : 0.0);
+    lo = std::min(lo, dg[i] - r);
+    hi = std::max(hi, dg[i] + r);
+  }
+  out.assign((size_t)kmax * n, 0.0);
+  std::vector<double> ed(n), x(n), xp(n), dw(n), ew(od.size());
+  auto norm = [&](std::vector<double>& v) {
+    double ss = 0.0;
+    for (double a : v) ss = ss + a * a;
+    const double nv = std::sqrt(ss);
+    for (double& a : v) a = a / nv;
+  };
+  for (int k = 0; k < kmax; ++k) {
+    double a = lo, b = hi;
+    for (int it = 0; it < 200; ++it) {
+      const double m = 0.5 * (a + b);
+      if (m <= a || m >= b) break;
+      if (below(m) <= n - 1 - k) a = m; else b = m;
+    }
+    const double lam = 0.5 * (a + b);
+    for (int i = 0; i < n; ++i) ed[i] = dg[i] - lam;
+    for (int i = 0; i < n; ++i) x[i] = std::sin((double)(k + 1) * (pi * (double)i / (double)(n - 1)));
+    norm(x);
+    for (int it = 1;; ++it) {
+      xp = x;
+      dw = ed;
+      ew = od;
+      for (int i = 1; i < n; ++i) {
+        const double tq = ew[i - 1];
+        ew[i - 1] = tq / dw[i - 1];
+        dw[i] = dw[i] - tq * ew[i - 1];
+      }
+      for (int i = 1; i < n; ++i) x[i] = x[i] - ew[i - 1] * x[i - 1];
+      x[n - 1] = x[n - 1] / dw[n - 1];
+      for (int i = n - 2; i >= 0; --i) x[i] = x[i] / dw[i] - ew[i] * x[i + 1];
+      norm(x);
+      double dn = 0.0;
+      for (int i = 0; i < n; ++i) dn = dn + (std::fabs(x[i]) - std::fabs(xp[i])) * (std::fabs(x[i]) - std::fabs(xp[i]));
+      if (std::sqrt(dn) <= 1e-8 || it > 1000) break;
+    }
+    double sgn = 1.0;
+    if (k % 2 == 0) {
+      double sm = 0.0;
+      for (double v : x) sm = sm + v;
+      if (sm < 0.0) sgn = -1.0;
+    } else if (n > 1 && x[1] < 0.0) {
+      sgn = -1.0;
+    }
+    const double sc = std::sqrt((double)n);
+    for (int i = 0; i < n; ++i) {
+      if (!std::isfinite(x[i])) SEM_FAIL("dpss taper %d of length %d did not converge", k + 1, n);
+      out[(size_t)k * n + i] = sgn * x[i] * sc;
+    }
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -516,7 +1043,7 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
                        d_bi);
     hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nls[w], d_bv, d_bi, d_ish);
     hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nls[w], *dt, *imeas, C, d_tw, d_D, d_S,
-                       d_ish, d_A, d_out + 8 * (size_t)w * T);
+                       d_ish, d_A, d_out + 8 * (size_t)w * T, 8);
     hipLaunchKernelGGL(k_cc_put, dim3((unsigned)((T + 63) / 64), (unsigned)((nls[w] + 63) / 64)), dim3(256), 0, s, T,
                        nbs[w], nls[w], d_A, d_out + 8 * (size_t)w * T, d_adj);
   }
@@ -555,6 +1082,144 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   dev_free(d_tw); dev_free(d_fw); dev_free(d_sos); dev_free(d_bv); dev_free(d_bi); dev_free(d_A);
   HIP_CHECK(hipEventDestroy(e0));
   HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipStreamDestroy(s));
+}
+
+// mt_tt_misfit.f90 MTTTMisfit%calc_adjoint_source for every trace: imeas 13
+// (multitaper traveltime) or 14 (amplitude), with the cross-correlation
+// fallback of the reference.  cfg(18) = the 7 cross-correlation values of
+// fwat_cc_adjoint_batch_, then transfunc_waterlevel, water_threshold, mt_nw,
+// num_taper, dt_fac, err_fac, dt_max_scale, min_cycle_in_window,
+// use_cc_error (0/1), min_period, max_period (use_mt_error = .false.);
+// meas(10, nwin, ntrace) = misfit, residual, error, cc_max, tshift, dlna,
+// selected, ishift, measurement used (13/14, 11/12 after the fallback, 0 not
+// selected), the other misfit; window adjoint sources are summed as the
+// reference does, then band-passed (sos, nsect) and tapered as filter_adj.
+// Limits: 32 <= nstep <= 16383 (transforms of up to 16384 samples in LDS).
+void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw* syn, int* nwin, double* windows,
+                            double* taper_percentage, int* itaper_type, int* imeas, double* cfg, int* nsect,
+                            realw* sos, double* misfits, double* meas, realw* adj) {
+  const int N = *nstep, T = *ntrace, nw = *nwin;
+  if (N < 32 || N > 16383) SEM_FAIL("fwat_mt_adjoint_batch_: nstep must be in [32, 16383] (got %d)", N);
+  if (T <= 0) SEM_FAIL("fwat_mt_adjoint_batch_: ntrace must be positive");
+  if (nw < 1 || nw > MAX_WIN) SEM_FAIL("fwat_mt_adjoint_batch_: 1..%d windows", MAX_WIN);
+  if (*imeas != 13 && *imeas != 14) SEM_FAIL("fwat_mt_adjoint_batch_: imeas must be 13 (MT-TT) or 14 (MT-DLNA)");
+  if (T > 65535) SEM_FAIL("fwat_mt_adjoint_batch_: at most 65535 traces per call");
+  if (*nsect < 0 || *nsect > 8) SEM_FAIL("fwat_mt_adjoint_batch_: 0..8 filter sections");
+  if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
+  const int ntaper = (int)cfg[10];
+  if (ntaper < 2 || ntaper > 16) SEM_FAIL("fwat_mt_adjoint_batch_: num_taper must be in [2, 16]");
+  if (!(cfg[16] > 0.0) || !(cfg[17] > 0.0)) SEM_FAIL("fwat_mt_adjoint_batch_: min/max period must be positive");
+  const CcCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6]};
+  const MtCfg MC{cfg[7], cfg[11], cfg[12], cfg[13], cfg[14], cfg[16], ntaper, cfg[15] != 0.0 ? 1 : 0};
+  int e2 = 0;
+  std::frexp((float)N, &e2);
+  const int nf = 1 << e2, M = nf / 2;  // 2 ** exponent(real(nlen))
+  int logM = 0;
+  while ((1 << logM) < M) ++logM;
+  // twiddles exp(-2 pi i j / nf), j < nf / 2
+  std::vector<double> twh(2 * (size_t)M);
+  for (int j = 0; j < M; ++j) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)nf;
+    twh[2 * j] = (double)std::cos(a);
+    twh[2 * j + 1] = (double)std::sin(a);
+  }
+  const size_t n = (size_t)N * T;
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  float* d_obs = dev_upload(obs, n, s);
+  float* d_syn = dev_upload(syn, n, s);
+  float* d_adj = dev_alloc<float>(n);
+  double2* d_twd = reinterpret_cast<double2*>(dev_upload(twh.data(), twh.size(), s));
+  int* d_ish = dev_alloc<int>(T);
+  int* d_flim = dev_alloc<int>(3 * (size_t)T);
+  double* d_out = dev_alloc<double>(10 * (size_t)nw * T);
+  int maxnl = 0;
+  std::vector<int> nbs(nw), nls(nw);
+  std::vector<std::vector<double>> tws(nw), taps(nw);
+  double wmin = windows[0], wmax = windows[0];
+  for (int w = 0; w < nw; ++w) {
+    window_range(windows[2 * w], windows[2 * w + 1], *dt, N, nbs[w], nls[w]);
+    taper_weights(nls[w], *taper_percentage, *itaper_type, tws[w]);
+    bool same = false;
+    for (int v = 0; v < w && !same; ++v)
+      if (nls[v] == nls[w]) { taps[w] = taps[v]; same = true; }
+    if (!same) dpss_tapers(nls[w], cfg[9], ntaper, taps[w]);
+    maxnl = std::max(maxnl, nls[w]);
+    wmin = std::min(wmin, std::min(windows[2 * w], windows[2 * w + 1]));
+    wmax = std::max(wmax, std::max(windows[2 * w], windows[2 * w + 1]));
+  }
+  double* d_D = dev_alloc<double>((size_t)T * maxnl);
+  double* d_S = dev_alloc<double>((size_t)T * maxnl);
+  double* d_A = dev_alloc<double>((size_t)T * maxnl);
+  double2* d_g01 = dev_alloc<double2>(2 * (size_t)T * (M + 1));
+  double* d_g2 = dev_alloc<double>((size_t)T * (M + 1));
+  double* d_tw = dev_alloc<double>(maxnl);
+  double* d_tap = dev_alloc<double>((size_t)ntaper * maxnl);
+  const int maxblk = (2 * maxnl - 1 + CC_LAGS - 1) / CC_LAGS;
+  double* d_bv = dev_alloc<double>((size_t)T * maxblk);
+  int* d_bi = dev_alloc<int>((size_t)T * maxblk);
+  const int imcc = *imeas == 13 ? 11 : 12;
+  HIP_CHECK(hipEventRecord(ev0, s));
+  hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_adj);
+  hipLaunchKernelGGL(k_mt_flim, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, d_syn, cfg[8],
+                     cfg[16], cfg[17], cfg[9], cfg[14], d_flim);
+  const unsigned nbk = (unsigned)((T + MTPB - 1) / MTPB);
+  for (int w = 0; w < nw; ++w) {
+    const int nl = nls[w];
+    double* ow = d_out + 10 * (size_t)w * T;
+    HIP_CHECK(hipMemcpyAsync(d_tw, tws[w].data(), nl * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_tap, taps[w].data(), (size_t)ntaper * nl * sizeof(double), hipMemcpyHostToDevice, s));
+    const dim3 tiles((unsigned)((T + 63) / 64), (unsigned)((nl + 63) / 64));
+    hipLaunchKernelGGL(k_cc_rows, tiles, dim3(256), 0, s, T, nbs[w], nl, d_tw, d_obs, d_syn, d_D, d_S);
+    const int nblk = (2 * nl - 1 + CC_LAGS - 1) / CC_LAGS;
+    hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(CC_LAGS), 0, s, nl, d_D, d_S, d_bv, d_bi);
+    hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nl, d_bv, d_bi, d_ish);
+    hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nl, *dt, imcc, C, d_tw, d_D, d_S, d_ish, d_A,
+                       ow, 10);
+    hipLaunchKernelGGL(k_mt_window, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, nbs[w], nl,
+                       *imeas, MC, d_tw, d_tap, d_obs, d_S, d_flim, d_g01, d_g01 + (size_t)T * (M + 1), d_g2, d_A, ow);
+    hipLaunchKernelGGL(k_mt_put, tiles, dim3(256), 0, s, T, nbs[w], nl, d_A, ow, d_adj);
+  }
+  std::vector<double> fw;
+  double* d_fw = nullptr;
+  float* d_sos = nullptr;
+  if (*nsect > 0) {
+    int fnb = 0, fnl = 0;
+    window_range(wmin, wmax, *dt, N, fnb, fnl);
+    taper_weights(fnl, *taper_percentage, *itaper_type, fw);
+    d_fw = dev_upload(fw.data(), fw.size(), s);
+    d_sos = dev_upload(sos, 6 * (size_t)*nsect, s);
+    hipLaunchKernelGGL(k_sos_filter, dim3(nbk), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
+                       d_adj);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(ev1, s));
+  std::vector<double> out(10 * (size_t)nw * T);
+  HIP_CHECK(hipMemcpyAsync(adj, d_adj, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+  g_meas_ms = ms;
+  for (int t = 0; t < T; ++t) {
+    double m = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      const double* o = &out[10 * ((size_t)w * T + t)];
+      m += o[0];
+      if (meas) for (int q = 0; q < 10; ++q) meas[10 * ((size_t)t * nw + w) + q] = o[q];
+    }
+    misfits[t] = m;
+  }
+  dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_twd); dev_free(d_ish); dev_free(d_flim);
+  dev_free(d_out); dev_free(d_D); dev_free(d_S); dev_free(d_A); dev_free(d_tw); dev_free(d_tap); dev_free(d_bv);
+  dev_free(d_bi); dev_free(d_fw); dev_free(d_sos); dev_free(d_g01); dev_free(d_g2);
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));
 }
 

@@ -185,5 +185,52 @@ def event_cc_adjoint_sources(obs, syn, dt, windows=None, imeas=IMEAS_CC_TT, tape
     return total, adj, meas
 
 
+IMEAS_CC_TT_MT, IMEAS_CC_DLNA_MT = 13, 14  # adj_config
+MT_DEFAULTS = dict(transfunc_waterlevel=1e-10, water_threshold=0.02, mt_nw=4.0, num_taper=5, dt_fac=2.0,
+                   err_fac=2.5, dt_max_scale=3.5, min_cycle_in_window=3, use_cc_error=True)  # adj_config.f90:18-26
+MT_FIELDS = ("misfit", "residual", "error", "cc_max", "tshift", "dlna", "selected", "ishift", "imeas",
+             "misfit_other")
+
+
+def event_mt_adjoint_sources(obs, syn, dt, windows, min_period, max_period, imeas=IMEAS_CC_TT_MT,
+                             taper_percentage=0.05, itaper_type=1, filter_adj=True, **cfg):
+    """Multitaper traveltime (imeas 13) or amplitude (14) measurement of every
+    trace (MTTTMisfit%calc_adjoint_source, mt_tt_misfit.f90:29-209) in one
+    device pass, with the reference's fall back to the cross-correlation
+    measurement for windows the multitaper tests reject.  min_period and
+    max_period bound the measured band (cfg%min_period / max_period) and, with
+    filter_adj, the adjoint-source bandpass.  Returns (misfit, adjoint traces
+    float32 like syn, per-window measurements [nrec*3, nwin, 10] in MT_FIELDS
+    order; "imeas" is the measurement used: 13/14, 11/12 after a fall back, 0
+    when the window is not selected).  cfg overrides CC_DEFAULTS and
+    MT_DEFAULTS (use_mt_error = .false. as the reference's default)."""
+    nstep, ntrace, win, o, s, sos = _event_inputs(obs, syn, dt, windows, min_period, max_period)
+    unknown = set(cfg) - set(CC_DEFAULTS) - set(MT_DEFAULTS)
+    if unknown:
+        raise KeyError(f"unknown MT settings {sorted(unknown)}")
+    if imeas not in (IMEAS_CC_TT_MT, IMEAS_CC_DLNA_MT):
+        raise ValueError("imeas must be 13 (MT-TT) or 14 (MT-DLNA)")
+    if not 32 <= nstep <= 16383:
+        raise ValueError("the multitaper measurement takes traces of 32 to 16383 samples")
+    c = dict(CC_DEFAULTS, **MT_DEFAULTS)
+    c.update(cfg)
+    cv = np.array([c["dt_sigma_min"], c["dlna_sigma_min"], c["tshift_min"], c["tshift_max"], c["dlna_min"],
+                   c["dlna_max"], c["cc_min"], c["transfunc_waterlevel"], c["water_threshold"], c["mt_nw"],
+                   c["num_taper"], c["dt_fac"], c["err_fac"], c["dt_max_scale"], c["min_cycle_in_window"],
+                   1.0 if c["use_cc_error"] else 0.0, min_period, max_period], np.float64)
+    if not filter_adj:
+        sos = np.zeros((0, 6), np.float32)
+    adj = np.zeros(s.shape, np.float32)
+    mf = np.zeros(ntrace)
+    meas = np.zeros((ntrace, win.shape[0], 10))
+    _lib.call("fwat_mt_adjoint_batch_", nstep, ntrace, float(dt), o, s, win.shape[0], win, float(taper_percentage),
+              int(itaper_type), int(imeas), cv, sos.shape[0], sos if sos.size else np.zeros(6, np.float32), mf, meas,
+              adj)
+    total = 0.0
+    for v in mf:
+        total += v
+    return total, adj, meas
+
+
 def last_time_ms():
     return float(_lib.call("fwat_waveform_adjoint_last_time_", 0.0)[0])

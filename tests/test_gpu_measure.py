@@ -94,3 +94,53 @@ def test_cc_batch_matches_oracle(imeas):
     assert not meas[0, :, 6].any() and meas[1:, :, 6].any()
     assert abs(total - tot) <= 1e-12 * max(tot, 1e-30)
     print(f"imeas {imeas}: misfit {total:.6e}, selected {int(meas[:, :, 6].sum())} of {meas.shape[0] * 2}")
+
+
+def _mt_event(seed, nstep=3000, dt=1 / 64):
+    """Traces whose synthetics are shifted by sub-sample, whole-sample, sub-one-sample
+    and too-large amounts, plus a noise trace below CC_MIN: every branch of the
+    multitaper measurement (MT result, both fall backs, not selected)."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(nstep) * dt
+    shifts = [0.2345, 0.05, -0.0999, 0.3 / 64, -0.3111, 9 / 64, 0.171, -0.14]
+    nrec = len(shifts)
+    obs = np.zeros((nstep, nrec, 3), np.float32)
+    syn = np.zeros_like(obs)
+    for r, tau in enumerate(shifts):
+        for c in range(3):
+            f, t0 = rng.uniform(0.7, 1.3), rng.uniform(18, 24)
+            w = lambda x: np.exp(-((t - x) * f) ** 2) * np.sin(2 * np.pi * f * (t - x))
+            obs[:, r, c] = w(t0 + tau) * rng.uniform(0.8, 1.25) + 0.005 * rng.standard_normal(nstep)
+            syn[:, r, c] = w(t0)
+    syn[:, 0, 2] = rng.standard_normal(nstep)  # below CC_MIN
+    return obs, syn, dt
+
+
+@pytest.mark.parametrize("imeas,filt", [(13, False), (14, False), (13, True)])
+def test_mt_batch_matches_oracle(imeas, filt):
+    obs, syn, dt = _mt_event(imeas)
+    win = [[10.0, 32.0], [12.0, 40.03]]  # 1408 (even) and 1922 samples
+    total, adj, meas = misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.5, 10.0, imeas=imeas,
+                                                       filter_adj=filt)
+    tot = 0.0
+    used = []
+    for r in range(obs.shape[1]):
+        for c in range(3):
+            m, a, o = orc.mt_adjoint(obs[:, r, c], syn[:, r, c], dt, win, 0.5, 10.0, imeas=imeas)
+            if filt:
+                a = orc.filter_adj(a, dt, np.float32(1 / 10.0), np.float32(1 / 0.5), win, 0.05, 1)
+            tot += m
+            q = meas[3 * r + c]
+            np.testing.assert_array_equal(q[:, 8], o["imeas"])
+            np.testing.assert_array_equal(q[:, 6].astype(bool), o["selected"])
+            np.testing.assert_allclose(q[:, 4], o["tshift"], rtol=0, atol=1e-12)
+            np.testing.assert_allclose(q[:, 0], o["misfit"], rtol=1e-8, atol=1e-300)
+            np.testing.assert_allclose(q[:, 9], o["misfit_other"], rtol=1e-8, atol=1e-300)
+            np.testing.assert_allclose(q[:, 2], o["error"], rtol=1e-12)
+            # the reference inverts the adjoint spectrum in single precision, the device in double
+            assert np.abs(adj[:, r, c] - a).max() <= 2e-5 * max(np.abs(a).max(), 1e-30)
+            used += list(o["imeas"])
+    assert abs(total - tot) <= 1e-8 * max(tot, 1e-30)
+    assert imeas in used and (imeas - 2) in used and 0 in used  # MT, fall back and unselected windows
+    print(f"imeas {imeas} filter {filt}: misfit {total:.6e}, windows by measurement "
+          f"{ {int(k): used.count(k) for k in set(used)} }")

@@ -9,6 +9,14 @@
                     state; zero phase (a centred impulse stays symmetric)
   filter_adj        zero outside [min window start, max window end], tapered
                     filtered trace inside
+  fft               the discrete Fourier transform (numpy) in both directions
+  dpss              scipy.signal.windows.dpss (independent implementation): same
+                    tapers and sign convention to 1e-9
+  MT measurement    a sub-sample delay is recovered (misfit 0.5 (tau/sigma)^2
+                    where the cross-correlation one is quantised); with a pure
+                    gain and whole-sample shift dtau = tshift, dlnA = ln A and
+                    Parseval closes dt sum(adj_q s) = -ln A / sigma^2; the
+                    cross-correlation fall back is bit-identical to orc_cc_adjoint
 """
 import os
 import sys
@@ -122,3 +130,79 @@ def test_cc_selection_thresholds():
     assert abs(o["tshift"][0] - 5.5) < 1e-12 and not o["selected"][0] and tot == 0.0
     tot, adj, o = O.cc_adjoint(d2, s2, dt, [[0.5, 14.5]], tshift_max=7.0)
     assert o["selected"][0] and tot > 0
+
+
+# ---- multitaper family: FFT, DPSS and the MT measurement of the oracle
+
+def test_fft_is_the_discrete_fourier_transform():
+    x = np.random.default_rng(5).standard_normal(256) + 1j * np.random.default_rng(6).standard_normal(256)
+    assert np.abs(O.fft(x) - np.fft.fft(x)).max() < 1e-12
+    assert np.abs(O.fft(x, inverse=True) - np.fft.ifft(x) * 256).max() < 1e-12
+    # single precision (the reference's inverse of the adjoint source)
+    assert np.abs(O.fft(x, inverse=True, single=True) - np.fft.ifft(x) * 256).max() < 2e-5
+
+
+@pytest.mark.parametrize("n,nw", [(500, 4.0), (1537, 4.0), (2048, 2.5)])
+def test_dpss_equals_scipy(n, nw):
+    """Independent implementation (scipy.signal.windows.dpss, unit norm): same
+    tapers, same sign convention (symmetric tapers sum positive,
+    antisymmetric ones start positive)."""
+    ref = signal.windows.dpss(n, nw, 5)
+    v = O.dpss(n, nw, 5)
+    assert np.abs(v - ref).max() < 1e-9
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-12)
+
+
+def _mt_case(tau, amp=1.0, f=1.0, dt=1 / 64, n=3000):
+    t = np.arange(n) * dt
+    w = lambda t0: np.exp(-((t - t0) * f) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    return w(20.0 + tau) * amp, w(20.0), dt
+
+
+@pytest.mark.parametrize("tau", [0.2345, 0.05, -0.0999])
+def test_mt_traveltime_recovers_a_sub_sample_shift(tau):
+    """Data = synthetic delayed by tau (not a whole number of samples): the
+    cross-correlation measurement is quantised to the sample, the multitaper one
+    measures the residual shift per frequency, misfit 0.5 (tau / sigma)^2."""
+    d, s, dt = _mt_case(tau)
+    tot, adj, o = O.mt_adjoint(d, s, dt, [[10.0, 32.0]], 0.5, 10.0, imeas=13)
+    assert o["imeas"][0] == 13 and o["selected"][0] and o["error"][0] == 1.0
+    assert abs(o["misfit"][0] - 0.5 * tau ** 2) < 5e-3 * 0.5 * tau ** 2  # taper edges bias the estimate slightly
+    assert abs(o["tshift"][0] - round(tau / dt) * dt) < 1e-12  # the CC stage's shift
+    _, _, c = O.cc_adjoint(d, s, dt, [[10.0, 32.0]], imeas=11)
+    assert abs(c["misfit"][0] - 0.5 * tau ** 2) > 10 * abs(o["misfit"][0] - 0.5 * tau ** 2)
+
+
+def test_mt_adjoint_sources_close_the_parseval_identities():
+    """Pure gain A and whole-sample shift, no taper: dtau(f) = tshift and
+    dlnA(f) = ln A in the band, and Parseval gives dt * sum(adj_q * s) =
+    -ln A / sigma^2 (single-precision inverse transform, as the reference);
+    dt * sum(adj_p * ds/dt) = tshift / sigma^2 up to the tapers' own slope."""
+    dt, n, k, A = 1 / 64, 3000, 9, 1.3
+    _, s, _ = _mt_case(0.0)
+    d = np.roll(s, k) * A
+    nb, ne = int(10.0 / dt) + 1, int(32.0 / dt)
+    sw = s[nb - 1:ne]
+    _, adj, o = O.mt_adjoint(d, s, dt, [[10.0, 32.0]], 0.5, 10.0, imeas=14, taper_pct=0.0)
+    assert o["imeas"][0] == 14 and abs(o["dlna"][0] - np.log(A)) < 1e-12
+    assert abs(dt * np.sum(adj[nb - 1:ne] * sw) + np.log(A) / o["error"][0] ** 2) < 1e-6
+    assert np.all(adj[:nb - 1] == 0) and np.all(adj[ne:] == 0)
+    assert abs(o["misfit"][0] - 0.5 * (np.log(A) / 0.5) ** 2) < 1e-9
+    _, adj, o = O.mt_adjoint(d, s, dt, [[10.0, 32.0]], 0.5, 10.0, imeas=13, taper_pct=0.0)
+    assert o["imeas"][0] == 13 and abs(o["misfit"][0] - 0.5 * (k * dt) ** 2) < 1e-12
+    ds = np.gradient(sw, dt, edge_order=1)
+    assert abs(dt * np.sum(adj[nb - 1:ne] * ds) - k * dt) < 0.05 * k * dt
+
+
+def test_mt_falls_back_to_the_cross_correlation_measurement():
+    """|tshift| <= dt fails check_time_series_acceptability: the window takes the
+    cross-correlation misfit and adjoint source (imeas 11), bit for bit."""
+    d, s, dt = _mt_case(0.3 / 64)  # under one sample
+    tot, adj, o = O.mt_adjoint(d, s, dt, [[10.0, 32.0]], 0.5, 10.0, imeas=13)
+    ct, cadj, c = O.cc_adjoint(d, s, dt, [[10.0, 32.0]], imeas=11)
+    assert o["imeas"][0] == 11 and tot == ct
+    np.testing.assert_array_equal(adj, cadj)
+    # a shift beyond half the shortest period fails the per-frequency test: fall back as well
+    d, s, dt = _mt_case(-0.3111)
+    _, _, o = O.mt_adjoint(d, s, dt, [[10.0, 32.0]], 0.5, 10.0, imeas=13)
+    assert o["imeas"][0] == 11
