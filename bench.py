@@ -681,8 +681,8 @@ def aniso_smooth(a):
 def measure(a):
     """Adjoint-source measurements of one C2-sized event on the device: every
     trace (400 receivers x 3 components, 10000 samples at dt 0.004 s) through the
-    waveform misfit and the CC traveltime / amplitude measurements over two
-    windows, adjoint sources band-passed 0.4-10 s.  Synthetic traces: a random
+    waveform misfit, the CC and the multitaper traveltime / amplitude
+    measurements over two windows, adjoint sources band-passed 0.4-10 s.  Synthetic traces: a random
     wavelet per trace, the synthetic shifted and scaled."""
     from specfwat_amd import misfit
     nrec, nstep, dt = 400, a.steps or 10000, 0.004
@@ -704,11 +704,17 @@ def measure(a):
                       ("cc_tt", lambda: misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=11,
                                                                           min_period=0.4, max_period=10.0)),
                       ("cc_dlna", lambda: misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=12,
-                                                                            min_period=0.4, max_period=10.0))):
+                                                                            min_period=0.4, max_period=10.0)),
+                      ("mt_tt", lambda: misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.4, 10.0, imeas=13)),
+                      ("mt_dlna", lambda: misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.4, 10.0,
+                                                                          imeas=14))):
         fn()  # warm-up
         ts = time.time()
         out = fn()
         stages[label] = {"wall_s": time.time() - ts, "device_ms": misfit.last_time_ms(), "misfit": out[0]}
+        if label.startswith("mt"):
+            used = out[2][:, :, 8]
+            stages[label]["windows_by_measurement"] = {str(int(k)): int((used == k).sum()) for k in np.unique(used)}
         if not np.isfinite(out[1]).all():
             raise RuntimeError(f"non-finite {label} adjoint sources")
     log("measure: " + ", ".join(f"{k} {v['device_ms']:.1f} ms" for k, v in stages.items()))
@@ -726,7 +732,7 @@ def measure(a):
     }
     if not a.no_cpu:
         from oracle import oracle as orc
-        sample = 6
+        sample = 300
         ts = time.time()
         for q in range(sample):
             r, c = divmod(q, 3)
@@ -736,6 +742,16 @@ def measure(a):
         result["cpu_baseline"] = {"value": sample / wall, "unit": "traces/s", "cores": 1, "kind": "port",
                                   "sample": f"{sample} traces of the same event through the oracle's CC-TT "
                                             f"measurement + filter_adj (C++, one thread), {wall:.1f}s"}
+        sample = 36
+        ts = time.time()
+        for q in range(sample):
+            r, c = divmod(q, 3)
+            _, adj_o, _ = orc.mt_adjoint(obs[:, r, c], syn[:, r, c], dt, win, 0.4, 10.0, imeas=13)
+            orc.filter_adj(adj_o, dt, np.float32(1 / 10.0), np.float32(1 / 0.4), win)
+        wall = time.time() - ts
+        stages["mt_tt"]["cpu_traces_per_s"] = sample / wall
+        stages["mt_tt"]["cpu_sample"] = f"{sample} traces through the oracle's MT-TT + filter_adj, one thread, " \
+                                        f"{wall:.1f}s"
     print(json.dumps(result), flush=True)
 
 

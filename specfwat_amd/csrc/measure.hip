@@ -375,23 +375,50 @@ __device__ void mt_load_real(double2* buf, int M, int logM, int len, F f) {
   __syncthreads();
 }
 
-// in-place transform of bit-reversed buf; inverse uses the conjugate twiddles (unnormalised)
+// R consecutive radix-2 stages (half-spans h, 2h, .., 2^(R-1) h) fused: each
+// lane takes groups of 2^R elements i0 + q h through the R stages in
+// registers, with the same butterflies and twiddles as stage-by-stage, so the
+// result is bit-for-bit that of the radix-2 transform with 1/R of the LDS
+// passes and barriers
+template <int NT, int R>
+__device__ void mt_pass(double2* buf, int M, int lh, int nf, const double2* __restrict__ tw, bool inverse) {
+  constexpr int G = 1 << R;
+  const int h = 1 << lh;
+#pragma unroll 1
+  for (int g = threadIdx.x; g < M / G; g += NT) {
+    const int pos = g & (h - 1);
+    const int i0 = ((g >> lh) << (lh + R)) + pos;
+    double2 x[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) x[q] = buf[i0 + q * h];
+#pragma unroll
+    for (int st = 0; st < R; ++st) {
+      const int half = 1 << st;
+      const int tstep = nf >> (lh + st + 1);  // nf / (2 half h)
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        if (q & half) continue;
+        double2 w = tw[(pos + (q & (half - 1)) * h) * tstep];
+        if (inverse) w.y = -w.y;
+        const double2 a = x[q], c = cmul(x[q + half], w);
+        x[q] = make_double2(a.x + c.x, a.y + c.y);
+        x[q + half] = make_double2(a.x - c.x, a.y - c.y);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) buf[i0 + q * h] = x[q];
+  }
+  __syncthreads();
+}
+
+// in-place transform of bit-reversed buf (decimation in time, radix-2 stages
+// fused three at a time); inverse uses the conjugate twiddles (unnormalised)
 template <int NT>
 __device__ void mt_fft(double2* buf, int M, int logM, int nf, const double2* __restrict__ tw, bool inverse) {
-  for (int st = 1; st <= logM; ++st) {
-    const int h = 1 << (st - 1), tstep = nf / (2 * h);
-#pragma unroll 2
-    for (int b = threadIdx.x; b < M / 2; b += NT) {
-      const int pos = b & (h - 1);
-      const int i1 = ((b >> (st - 1)) << st) + pos, i2 = i1 + h;
-      double2 w = tw[pos * tstep];
-      if (inverse) w.y = -w.y;
-      const double2 a = buf[i1], c = cmul(buf[i2], w);
-      buf[i1] = make_double2(a.x + c.x, a.y + c.y);
-      buf[i2] = make_double2(a.x - c.x, a.y - c.y);
-    }
-    __syncthreads();
-  }
+  int lh = 0;
+  for (; lh + 3 <= logM; lh += 3) mt_pass<NT, 3>(buf, M, lh, nf, tw, inverse);
+  if (logM - lh == 2) mt_pass<NT, 2>(buf, M, lh, nf, tw, inverse);
+  else if (logM - lh == 1) mt_pass<NT, 1>(buf, M, lh, nf, tw, inverse);
 }
 
 // X_k = E - i W_k O with E = (Z_k + conj Z_{M-k}) / 2, O = (Z_k - conj Z_{M-k}) / 2
@@ -451,19 +478,37 @@ __device__ double mt_bmax(double v, double* red) {
   return r;
 }
 
+// sample-major traces [nstep][ntrace] to trace-major rows [ntrace][nstep]
+// through 64 x 64 LDS tiles (reads and writes coalesced)
+__global__ void __launch_bounds__(256) k_rows_f32(int nstep, int ntrace, const float* __restrict__ x,
+                                                  float* __restrict__ y) {
+  __shared__ float tile[64][65];
+  const int t0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int n = n0 + r, t = t0 + (threadIdx.x & 63);
+    if (n < nstep && t < ntrace) tile[r][threadIdx.x & 63] = x[(size_t)n * ntrace + t];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int t = t0 + r, n = n0 + (threadIdx.x & 63);
+    if (n < nstep && t < ntrace) y[(size_t)t * nstep + n] = tile[threadIdx.x & 63][r];
+  }
+}
+
 // calculate_freq_limits (mt_tt_misfit.f90:615-707) of every trace's whole
 // synthetic: amplitude spectrum, its first maximum, the water-level
 // "thermostat" search up and down from it, the period and cycle bounds and
 // the half-bandwidth test; flim[3 t] = nfreq_min, nfreq_max (1-based), accepted
 __global__ void __launch_bounds__(MT_THREADS) k_mt_flim(int nstep, int ntrace, double dt, int nf, int logM,
-                                                        const double2* __restrict__ tw, const float* __restrict__ syn,
+                                                        const double2* __restrict__ tw, const float* __restrict__ synT,
                                                         double water_threshold, double min_period, double max_period,
                                                         double mt_nw, double min_cycle, int* __restrict__ flim) {
   __shared__ double2 buf[MT_MAXM];
   __shared__ double rv[MT_THREADS];
   __shared__ int ri[MT_THREADS];
   const int t = blockIdx.x, M = nf / 2;
-  mt_load_real<MT_THREADS>(buf, M, logM, nstep, [&](int n) { return (double)syn[(size_t)n * ntrace + t]; });
+  const float* row = synT + (size_t)t * nstep;
+  mt_load_real<MT_THREADS>(buf, M, logM, nstep, [&](int n) { return (double)row[n]; });
   mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
   // amplitudes |X_k dt| in place: k < M at buf[k].x, k = M at buf[0].y
   double best = -1.0;
@@ -495,32 +540,67 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_flim(int nstep, int ntrace, d
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
   const int fnum = M + 1;
   const double ampmax = rv[0];
   const int iamp = ri[0] + 1;
+  __syncthreads();
   auto amp = [&](int i) { return i - 1 < M ? buf[i - 1].x : buf[0].y; };
   const double df = 1.0 / (nf * dt) - 0.0, tlen = nstep * dt;
   const double wl = water_threshold * ampmax;
   const int ifmin = max(1, (int)(1.0 / (max_period * df)));
   const int ifmax = min(fnum, (int)(1.0 / (min_period * df)));
+  // block-wide integer min / max through ri
+  auto ired = [&](int v, bool mx) {
+    ri[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = MT_THREADS / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) ri[threadIdx.x] = mx ? max(ri[threadIdx.x], ri[threadIdx.x + o])
+                                                     : min(ri[threadIdx.x], ri[threadIdx.x + o]);
+      __syncthreads();
+    }
+    const int r = ri[0];
+    __syncthreads();
+    return r;
+  };
+  // search_frequency_limit's "thermostat" over i = lo..hi in scan order (up:
+  // ascending, else descending), starting in the searching state: a sample
+  // below wl stops the search, one above 10 wl restarts it, and the limit is
+  // the sample of the last change of state.  That sample is the first
+  // decisive sample after the last decisive sample of the other kind (or the
+  // first stop, when the scan never restarts); found with block reductions.
+  auto thermostat = [&](int lo, int hi, bool up, int init) {
+    const int none = up ? -1 : 0x7fffffff;
+    auto later = [&](int x, int y) { return up ? max(x, y) : min(x, y); };
+    int lastL = none, lastH = none;
+    for (int i = lo + (int)threadIdx.x; i <= hi; i += MT_THREADS) {
+      const double a = amp(i);
+      if (a < wl) lastL = later(lastL, i);
+      else if (a > 10 * wl) lastH = later(lastH, i);
+    }
+    lastL = ired(lastL, up);
+    lastH = ired(lastH, up);
+    const bool hasL = lastL != none, hasH = lastH != none;
+    if (!hasL && !hasH) return init;
+    const bool endL = hasL && (!hasH || later(lastL, lastH) == lastL);
+    const int q = endL ? lastH : lastL;
+    const bool hasQ = endL ? hasH : hasL;
+    if (!hasQ && !endL) return init;  // never stopped
+    // first sample of the final kind after q (or from the start when there is no q)
+    int first = up ? 0x7fffffff : -1;
+    for (int i = lo + (int)threadIdx.x; i <= hi; i += MT_THREADS) {
+      if (hasQ && (up ? i <= q : i >= q)) continue;
+      const double a = amp(i);
+      const bool kind = endL ? (a < wl) : (a > 10 * wl);
+      if (kind) first = up ? min(first, i) : max(first, i);
+    }
+    return ired(first, !up);
+  };
   int nfmax = fnum, nfmin = 1;
-  bool srch = true;
-  if (iamp < fnum)
-    for (int iw = iamp + 1; iw <= fnum; ++iw) {
-      const double a = amp(iw);
-      if (a < wl && srch) { srch = false; nfmax = iw; }
-      if (a > 10 * wl && !srch) { srch = true; nfmax = iw; }
-    }
+  if (iamp < fnum) nfmax = thermostat(iamp + 1, fnum, true, fnum);
   nfmax = min(min(nfmax, ifmax), (int)(1.0 / (2.0 * dt) / df));
-  srch = true;
-  if (iamp > 1)
-    for (int iw = iamp - 1; iw >= 1; --iw) {
-      const double a = amp(iw);
-      if (a < wl && srch) { srch = false; nfmin = iw; }
-      if (a > 10 * wl && !srch) { srch = true; nfmin = iw; }
-    }
+  if (iamp > 1) nfmin = thermostat(1, iamp - 1, false, 1);
   nfmin = max(max(nfmin, ifmin), max(1, (int)(min_cycle / tlen / df)));
+  if (threadIdx.x != 0) return;
   flim[3 * t] = nfmin;
   flim[3 * t + 1] = nfmax;
   flim[3 * t + 2] = (nfmax - nfmin) * df < mt_nw / (4.0 * tlen) ? 0 : 1;
@@ -553,7 +633,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
                                                           const double2* __restrict__ tw, int nb, int nl, int imeas,
                                                           MtCfg cfg, const double* __restrict__ wtap,
                                                           const double* __restrict__ tap,
-                                                          const float* __restrict__ obs,
+                                                          const float* __restrict__ obsT,
                                                           const double* __restrict__ S, const int* __restrict__ flim,
                                                           double2* __restrict__ G0, double2* __restrict__ G1,
                                                           double* __restrict__ G2, double* __restrict__ A,
@@ -586,6 +666,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
   const double df = 1.0 / (nf * dt) - 0.0;
   const double em = exp(-dlna);
   const double* s = S + (size_t)t * nl;
+  const float* drow = obsT + (size_t)t * nstep + (nbd - 1);  // data window shifted by the CC lag
   double2* g0 = G0 + (size_t)t * (M + 1);
   double2* g1 = G1 + (size_t)t * (M + 1);
   double* g2 = G2 + (size_t)t * (M + 1);
@@ -603,7 +684,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
     for (int k = lane; k <= M; k += MT_THREADS) g0[k] = xk(k);
     __syncthreads();
     mt_load_real<MT_THREADS>(buf, M, logM, nl, [&](int n) {
-      return (double)obs[(size_t)(nbd - 1 + n) * ntrace + t] * em * wtap[n] * tj[n];
+      return (double)drow[n] * em * wtap[n] * tj[n];
     });
     mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
     for (int k = lane; k <= M; k += MT_THREADS) {
@@ -1163,10 +1244,15 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   double* d_bv = dev_alloc<double>((size_t)T * maxblk);
   int* d_bi = dev_alloc<int>((size_t)T * maxblk);
   const int imcc = *imeas == 13 ? 11 : 12;
+  float* d_obsT = dev_alloc<float>(n);
+  float* d_synT = dev_alloc<float>(n);
   HIP_CHECK(hipEventRecord(ev0, s));
   hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_adj);
-  hipLaunchKernelGGL(k_mt_flim, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, d_syn, cfg[8],
-                     cfg[16], cfg[17], cfg[9], cfg[14], d_flim);
+  const dim3 ttiles((unsigned)((T + 63) / 64), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_obs, d_obsT);
+  hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_syn, d_synT);
+  hipLaunchKernelGGL(k_mt_flim, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, d_synT,
+                     cfg[8], cfg[16], cfg[17], cfg[9], cfg[14], d_flim);
   const unsigned nbk = (unsigned)((T + MTPB - 1) / MTPB);
   for (int w = 0; w < nw; ++w) {
     const int nl = nls[w];
@@ -1181,7 +1267,8 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nl, *dt, imcc, C, d_tw, d_D, d_S, d_ish, d_A,
                        ow, 10);
     hipLaunchKernelGGL(k_mt_window, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, nbs[w], nl,
-                       *imeas, MC, d_tw, d_tap, d_obs, d_S, d_flim, d_g01, d_g01 + (size_t)T * (M + 1), d_g2, d_A, ow);
+                       *imeas, MC, d_tw, d_tap, d_obsT, d_S, d_flim, d_g01, d_g01 + (size_t)T * (M + 1), d_g2, d_A,
+                       ow);
     hipLaunchKernelGGL(k_mt_put, tiles, dim3(256), 0, s, T, nbs[w], nl, d_A, ow, d_adj);
   }
   std::vector<double> fw;
@@ -1217,7 +1304,7 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   }
   dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_twd); dev_free(d_ish); dev_free(d_flim);
   dev_free(d_out); dev_free(d_D); dev_free(d_S); dev_free(d_A); dev_free(d_tw); dev_free(d_tap); dev_free(d_bv);
-  dev_free(d_bi); dev_free(d_fw); dev_free(d_sos); dev_free(d_g01); dev_free(d_g2);
+  dev_free(d_bi); dev_free(d_fw); dev_free(d_sos); dev_free(d_g01); dev_free(d_g2); dev_free(d_obsT); dev_free(d_synT);
   HIP_CHECK(hipEventDestroy(ev0));
   HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));

@@ -113,6 +113,13 @@ def _mt_event(seed, nstep=3000, dt=1 / 64):
             obs[:, r, c] = w(t0 + tau) * rng.uniform(0.8, 1.25) + 0.005 * rng.standard_normal(nstep)
             syn[:, r, c] = w(t0)
     syn[:, 0, 2] = rng.standard_normal(nstep)  # below CC_MIN
+    # two narrow bands (0.6 and 1.6 Hz): the synthetic's spectrum falls below the
+    # water level between them and rises past ten times it again, so the
+    # frequency-limit search stops, restarts and stops
+    nb2 = lambda x: (np.exp(-((t - x) / 2.0) ** 2) * np.sin(2 * np.pi * 0.6 * (t - x))
+                     + 0.5 * np.exp(-((t - x - 1.0) / 2.0) ** 2) * np.sin(2 * np.pi * 1.6 * (t - x - 1.0)))
+    obs[:, nrec - 1, 2] = nb2(20.0 + shifts[-1]) * 1.1
+    syn[:, nrec - 1, 2] = nb2(20.0)
     return obs, syn, dt
 
 
