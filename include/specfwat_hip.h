@@ -322,8 +322,8 @@ void fwat_waveform_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *ob
  * fwat_mt_adjoint_batch_, copies excluded */
 void fwat_waveform_adjoint_last_time_(double *ms);
 /* cross-correlation traveltime (imeas 11) / amplitude (12) measurement of every trace: CCTTMisfit%
- * calc_adjoint_source (src/preproc/adjoint_sources/cc_tt_misfit.f90:17-146 with utils/cc.f90:10-160)
- * and filter_adj; cfg(7) = dt_sigma_min, dlna_sigma_min, TSHIFT_MIN, TSHIFT_MAX, DLNA_MIN, DLNA_MAX,
+ * calc_adjoint_source (src/preproc/adjoint_sources/cc_tt_misfit.f90:17-146 with utils/cc.f90:10-160),
+ * or the zero-lag correlation coefficient (imeas 5, CCCMisfit, ccc_misfit.f90:15-71), and filter_adj; cfg(7) = dt_sigma_min, dlna_sigma_min, TSHIFT_MIN, TSHIFT_MAX, DLNA_MIN, DLNA_MAX,
  * CC_MIN (adj_config.f90:18-23); meas(8,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
  * selected, ishift (or NULL) */
 void fwat_cc_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,

@@ -27,6 +27,7 @@
 //                         signal.f90:146-220, cc_max_coef, calc_cc_adjsrc), gradient
 //                         (utils.f90:1613-1628) and cc_measure_select (cc_tt_misfit.f90:
 //                         126-146); double precision, loops in the reference's order
+//                         imeas 5: CCCMisfit%calc_adjoint_source (ccc_misfit.f90:15-71)
 //   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
 //   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
 //                         where the reference calls LAPACK dsterf
@@ -369,6 +370,32 @@ double orc_cc_adjoint(const double* dat, const double* syn, int n, double dt, co
     std::vector<double> sv(syn + nb - 1, syn + ne), dv(dat + nb - 1, dat + ne);
     orc_window_taper(sv.data(), nl, taper_pct, itaper);
     orc_window_taper(dv.data(), nl, taper_pct, itaper);
+    if (imeas == 5) {
+      // CCCMisfit%calc_adjoint_source (ccc_misfit.f90:15-71): zero-lag correlation coefficient
+      double p = 0.0, de = 0.0, se = 0.0;
+      for (int i = 0; i < nl; ++i) p = p + dv[i] * sv[i];
+      for (int i = 0; i < nl; ++i) de = de + dv[i] * dv[i];
+      for (int i = 0; i < nl; ++i) se = se + sv[i] * sv[i];
+      p = p * dt;
+      de = de * dt;
+      se = se * dt;
+      const double coef = p / std::sqrt(de * se);
+      ccmax[w] = coef;
+      tshift_o[w] = 0.0;
+      dlna_o[w] = 0.0;
+      residuals[w] = coef;
+      errors[w] = 0.0;
+      misfits[w] = 0.0;
+      selected[w] = !(coef < cfg[6]);
+      if (!selected[w]) continue;
+      std::vector<double> a(nl);
+      for (int i = 0; i < nl; ++i) a[i] = -(dv[i] - (p / se) * sv[i]) / std::sqrt(de * se);
+      orc_window_taper(a.data(), nl, taper_pct, itaper);
+      misfits[w] = 1 - coef;
+      total = total + misfits[w];
+      for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = a[i];
+      continue;
+    }
     const CcStats c = cc_stats(dv, sv, dt, cfg);
     ccmax[w] = c.cmax;
     tshift_o[w] = c.tshift;

@@ -329,6 +329,39 @@ __global__ void __launch_bounds__(256) k_cc_finish(int nl, double dt, int imeas,
   }
 }
 
+// CCCMisfit%calc_adjoint_source (ccc_misfit.f90:15-71) per trace (one
+// workgroup on its tapered rows d, s): zero-lag correlation coefficient
+// p / sqrt(E_d E_s) (fixed-order tree sums), selection by CC_MIN, misfit
+// 1 - coef, adjoint source -(d - (p / E_s) s) / sqrt(E_d E_s) tapered into A.
+// out[8] as k_cc_finish: misfit, residual (coef), error 0, cc_max (coef),
+// tshift 0, dlna 0, selected, ishift 0
+__global__ void __launch_bounds__(256) k_ccc_finish(int nl, double dt, double cc_min, const double* __restrict__ tw,
+                                                    const double* __restrict__ D, const double* __restrict__ S,
+                                                    double* __restrict__ A, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int t = blockIdx.x;
+  const double* d = D + (size_t)t * nl;
+  const double* s = S + (size_t)t * nl;
+  double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+  for (int i = threadIdx.x; i < nl; i += 256) { q1 += d[i] * s[i]; q2 += d[i] * d[i]; q3 += s[i] * s[i]; }
+  const double p = wg_sum(q1, sh) * dt, de = wg_sum(q2, sh) * dt, se = wg_sum(q3, sh) * dt;
+  const double coef = p / sqrt(de * se);
+  const bool sel = !(coef < cc_min);
+  if (threadIdx.x == 0) {
+    double* o = out + 8 * (size_t)t;
+    o[0] = sel ? 1 - coef : 0.0;
+    o[1] = coef;
+    o[2] = 0.0;
+    o[3] = coef;
+    o[4] = 0.0;
+    o[5] = 0.0;
+    o[6] = sel ? 1.0 : 0.0;
+    o[7] = 0.0;
+  }
+  if (!sel) return;
+  for (int i = threadIdx.x; i < nl; i += 256) A[(size_t)t * nl + i] = -(d[i] - (p / se) * s[i]) / sqrt(de * se) * tw[i];
+}
+
 // selected traces' adjoint-source rows back into the sample-major traces
 // (64 x 64 LDS tiles); unselected traces keep what earlier windows wrote
 __global__ void __launch_bounds__(256) k_cc_put(int ntrace, int nb, int nl, const double* __restrict__ A,
@@ -1068,7 +1101,8 @@ void fwat_waveform_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* ob
 void fwat_waveform_adjoint_last_time_(double* ms) { *ms = g_meas_ms; }
 
 // cc_tt_misfit.f90 CCTTMisfit%calc_adjoint_source for every trace: imeas 11
-// (traveltime) or 12 (amplitude); cfg(7) = dt_sigma_min, dlna_sigma_min,
+// (traveltime) or 12 (amplitude), or ccc_misfit.f90 CCCMisfit (imeas 5,
+// zero-lag correlation coefficient; uses CC_MIN only); cfg(7) = dt_sigma_min, dlna_sigma_min,
 // TSHIFT_MIN, TSHIFT_MAX, DLNA_MIN, DLNA_MAX, CC_MIN; meas(8, nwin, ntrace) =
 // misfit, residual, error, cc_max, tshift, dlna, selected, ishift; misfits
 // (ntrace) summed over windows; sos / nsect as for the waveform measurement
@@ -1078,7 +1112,8 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   const int N = *nstep, T = *ntrace, nw = *nwin;
   if (N <= 0 || T <= 0) SEM_FAIL("fwat_cc_adjoint_batch_: sizes must be positive");
   if (nw < 1 || nw > MAX_WIN) SEM_FAIL("fwat_cc_adjoint_batch_: 1..%d windows", MAX_WIN);
-  if (*imeas != 11 && *imeas != 12) SEM_FAIL("fwat_cc_adjoint_batch_: imeas must be 11 (CC-TT) or 12 (CC-DLNA)");
+  if (*imeas != 11 && *imeas != 12 && *imeas != 5)
+    SEM_FAIL("fwat_cc_adjoint_batch_: imeas must be 11 (CC-TT), 12 (CC-DLNA) or 5 (CCC)");
   if (T > 65535) SEM_FAIL("fwat_cc_adjoint_batch_: at most 65535 traces per call");
   if (*nsect < 0 || *nsect > 8) SEM_FAIL("fwat_cc_adjoint_batch_: 0..8 filter sections");
   if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
@@ -1119,12 +1154,17 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     HIP_CHECK(hipMemcpyAsync(d_tw, tws[w].data(), nls[w] * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cc_rows, dim3((unsigned)((T + 63) / 64), (unsigned)((nls[w] + 63) / 64)), dim3(256), 0, s, T,
                        nbs[w], nls[w], d_tw, d_obs, d_syn, d_D, d_S);
-    const int nblk = (2 * nls[w] - 1 + CC_LAGS - 1) / CC_LAGS;
-    hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(CC_LAGS), 0, s, nls[w], d_D, d_S, d_bv,
-                       d_bi);
-    hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nls[w], d_bv, d_bi, d_ish);
-    hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nls[w], *dt, *imeas, C, d_tw, d_D, d_S,
-                       d_ish, d_A, d_out + 8 * (size_t)w * T, 8);
+    if (*imeas == 5) {
+      hipLaunchKernelGGL(k_ccc_finish, dim3((unsigned)T), dim3(256), 0, s, nls[w], *dt, C.cc_min, d_tw, d_D, d_S, d_A,
+                         d_out + 8 * (size_t)w * T);
+    } else {
+      const int nblk = (2 * nls[w] - 1 + CC_LAGS - 1) / CC_LAGS;
+      hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(CC_LAGS), 0, s, nls[w], d_D, d_S, d_bv,
+                         d_bi);
+      hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nls[w], d_bv, d_bi, d_ish);
+      hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nls[w], *dt, *imeas, C, d_tw, d_D, d_S,
+                         d_ish, d_A, d_out + 8 * (size_t)w * T, 8);
+    }
     hipLaunchKernelGGL(k_cc_put, dim3((unsigned)((T + 63) / 64), (unsigned)((nls[w] + 63) / 64)), dim3(256), 0, s, T,
                        nbs[w], nls[w], d_A, d_out + 8 * (size_t)w * T, d_adj);
   }

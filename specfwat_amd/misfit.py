@@ -132,7 +132,7 @@ def event_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, ita
     return total, adj
 
 
-IMEAS_CC_TT, IMEAS_CC_DLNA = 11, 12  # adj_config
+IMEAS_CCC, IMEAS_CC_TT, IMEAS_CC_DLNA = 5, 11, 12  # adj_config
 CC_DEFAULTS = dict(dt_sigma_min=1.0, dlna_sigma_min=0.5, tshift_min=-5.0, tshift_max=5.0, dlna_min=-1.5,
                    dlna_max=1.5, cc_min=0.7)  # ADJConfig defaults, adj_config.f90:18-23
 
@@ -160,7 +160,9 @@ def _event_inputs(obs, syn, dt, windows, min_period, max_period):
 def event_cc_adjoint_sources(obs, syn, dt, windows=None, imeas=IMEAS_CC_TT, taper_percentage=0.05,
                              itaper_type=1, min_period=None, max_period=None, **cfg):
     """Cross-correlation traveltime (imeas 11) or amplitude (12) measurement of
-    every trace (CCTTMisfit%calc_adjoint_source) in one device pass.  Returns
+    every trace (CCTTMisfit%calc_adjoint_source), or the zero-lag correlation
+    coefficient (imeas 5, CCCMisfit: misfit 1 - coef, residual and cc_max
+    = coef), in one device pass.  Returns
     (misfit, adjoint traces float32 like syn, per-window measurements
     [nrec*3, nwin, 8]: misfit, residual, error, cc_max, tshift, dlna, selected,
     ishift).  cfg overrides CC_DEFAULTS."""
@@ -171,8 +173,8 @@ def event_cc_adjoint_sources(obs, syn, dt, windows=None, imeas=IMEAS_CC_TT, tape
     c = dict(CC_DEFAULTS, **cfg)
     cv = np.array([c[k] for k in ("dt_sigma_min", "dlna_sigma_min", "tshift_min", "tshift_max", "dlna_min",
                                   "dlna_max", "cc_min")], np.float64)
-    if imeas not in (IMEAS_CC_TT, IMEAS_CC_DLNA):
-        raise ValueError("imeas must be 11 (CC-TT) or 12 (CC-DLNA)")
+    if imeas not in (IMEAS_CC_TT, IMEAS_CC_DLNA, IMEAS_CCC):
+        raise ValueError("imeas must be 11 (CC-TT), 12 (CC-DLNA) or 5 (CCC)")
     adj = np.zeros(s.shape, np.float32)
     mf = np.zeros(ntrace)
     meas = np.zeros((ntrace, win.shape[0], 8))

@@ -63,7 +63,7 @@ def test_batch_matches_the_host_statement():
     np.testing.assert_allclose(adj[:, 2, 1], a, rtol=0, atol=1e-6 * np.abs(a).max())
 
 
-@pytest.mark.parametrize("imeas", [11, 12])
+@pytest.mark.parametrize("imeas", [11, 12, 5])
 def test_cc_batch_matches_oracle(imeas):
     rng = np.random.default_rng(imeas)
     nstep, nrec, dt = 1400, 6, 0.01

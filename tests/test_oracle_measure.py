@@ -206,3 +206,23 @@ def test_mt_falls_back_to_the_cross_correlation_measurement():
     d, s, dt = _mt_case(-0.3111)
     _, _, o = O.mt_adjoint(d, s, dt, [[10.0, 32.0]], 0.5, 10.0, imeas=13)
     assert o["imeas"][0] == 11
+
+
+def test_ccc_closed_forms():
+    """CCCMisfit: s = A d gives coef 1, misfit 0 and a zero adjoint source; for
+    s = d + e the coefficient is the normalised inner product (numpy) and the
+    adjoint source -(d - (p/E_s) s)/sqrt(E_d E_s) is orthogonal to s."""
+    dt, n = 0.01, 1200
+    d = _wavelet(n, dt, 5.0, 1.5)
+    tot, adj, o = O.cc_adjoint(d, 1.7 * d, dt, [[0.5, 10.0]], imeas=5, taper_pct=0.0)
+    assert abs(o["cc_max"][0] - 1.0) < 1e-14 and abs(tot) < 1e-14 and np.abs(adj).max() < 1e-12
+    e = 0.3 * _wavelet(n, dt, 5.4, 2.5)
+    s = d + e
+    nb, ne = int(0.5 / dt) + 1, int(10.0 / dt)
+    dw, sw = d[nb - 1:ne], s[nb - 1:ne]
+    coef = np.dot(dw, sw) / np.sqrt(np.dot(dw, dw) * np.dot(sw, sw))
+    tot, adj, o = O.cc_adjoint(d, s, dt, [[0.5, 10.0]], imeas=5, taper_pct=0.0)
+    assert abs(o["residual"][0] - coef) < 1e-13 and abs(tot - (1 - coef)) < 1e-13
+    assert abs(np.dot(adj[nb - 1:ne], sw)) < 1e-10 * np.abs(adj).max() * np.abs(sw).sum()
+    tot, adj, o = O.cc_adjoint(d, -s, dt, [[0.5, 10.0]], imeas=5)
+    assert not o["selected"][0] and tot == 0.0 and np.all(adj == 0)
