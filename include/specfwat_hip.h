@@ -338,6 +338,14 @@ void fwat_cc_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, rea
  * use_mt_error = .false.); meas(10,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
  * selected, ishift, measurement used (13/14, 11/12 after the fall back, 0 unselected), the other misfit
  * (or NULL).  32 <= nstep <= 16383 */
+/* exponentiated phase measurement of every trace (imeas 4): ExponentiatedPhaseMisfit%calc_adjoint_source
+ * (src/preproc/adjoint_sources/exponentiated_phase_misfit.f90:21-104 with hilbert, fftpack.f90:135-201)
+ * and filter_adj; wtr_env = the envelope water level of adj_config (adj_config.f90:20);
+ * meas(8,nwin,ntrace) = misfit, residual, misfit_real, misfit_imag, residual_real, residual_imag,
+ * selected (1), 0 (or NULL).  Windows of 32 to 16383 samples */
+void fwat_ep_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
+                            double *windows, double *taper_percentage, int *itaper_type, double *wtr_env,
+                            int *nsect, realw *sos, double *misfits, double *meas, realw *adj);
 void fwat_mt_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
                             double *windows, double *taper_percentage, int *itaper_type, int *imeas,
                             double *cfg, int *nsect, realw *sos, double *misfits, double *meas, realw *adj);

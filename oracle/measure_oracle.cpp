@@ -28,6 +28,9 @@
 //                         (utils.f90:1613-1628) and cc_measure_select (cc_tt_misfit.f90:
 //                         126-146); double precision, loops in the reference's order
 //                         imeas 5: CCCMisfit%calc_adjoint_source (ccc_misfit.f90:15-71)
+//   orc_hilbert           hilbert (src/shared/fftpack.f90:135-201)
+//   orc_ep_adjoint        ExponentiatedPhaseMisfit%calc_adjoint_source
+//                         (exponentiated_phase_misfit.f90:21-104)
 //   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
 //   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
 //                         where the reference calls LAPACK dsterf
@@ -178,6 +181,30 @@ void fft_raw(std::vector<std::complex<T>>& x, int ind, T pi) {
       }
     }
   }
+}
+
+
+// hilbert (fftpack.f90:135-201): analytic signal of x on nft = 2 ** exponent(real(n))
+// points; the Nyquist weight follows the parity of the input length n
+std::vector<std::complex<double>> hilbert(const std::vector<double>& x) {
+  using cd = std::complex<double>;
+  const int n = (int)x.size(), nft = next_pow2_above(n);
+  std::vector<cd> xf(nft, cd(0.0, 0.0));
+  for (int i = 0; i < n; ++i) xf[i] = cd(x[i], 0.0);
+  fft_raw(xf, -1, 3.14159265358979323846);
+  std::vector<double> h(nft, 0.0);
+  h[0] = 1.0;
+  if (n % 2 == 0) {
+    h[nft / 2] = 1.0;
+    for (int i = 1; i < nft / 2; ++i) h[i] = 2.0;
+  } else {
+    for (int i = 1; i < (nft + 1) / 2; ++i) h[i] = 2.0;
+  }
+  for (int i = 0; i < nft; ++i) xf[i] = cd(xf[i].real() * h[i], xf[i].imag() * h[i]);
+  fft_raw(xf, 1, 3.14159265358979323846);
+  std::vector<cd> xa(n);
+  for (int i = 0; i < n; ++i) xa[i] = cd(xf[i].real() / (double)nft, xf[i].imag() / (double)nft);
+  return xa;
 }
 
 // number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
@@ -646,6 +673,81 @@ double orc_mt_adjoint(const double* dat, const double* syn, int n, double dt, co
     o[9] = imeas == 13 ? mq : mp;
     total = total + mf;
     for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = adj[nb - 1 + i] + a[i];
+  }
+  return total;
+}
+
+// analytic signal of x (hilbert) for tests: re/im of length n
+void orc_hilbert(const double* x, int n, double* re, double* im) {
+  const auto xa = hilbert(std::vector<double>(x, x + n));
+  for (int i = 0; i < n; ++i) { re[i] = xa[i].real(); im[i] = xa[i].imag(); }
+}
+
+// Exponentiated phase measurement of one trace, ExponentiatedPhaseMisfit%
+// calc_adjoint_source (exponentiated_phase_misfit.f90:21-104); per window out[8]:
+// misfit, residual, misfit_real, misfit_imag, residual_real, residual_imag,
+// selected (always 1), 0.  Returns the total misfit.
+double orc_ep_adjoint(const double* dat, const double* syn, int n, double dt, const double* windows, int nwin,
+                      double taper_pct, int itaper, double wtr_env, double* adj, double* out) {
+  double total = 0.0;
+  for (int i = 0; i < n; ++i) adj[i] = 0.0;
+  for (int w = 0; w < nwin; ++w) {
+    const int nb = (int)(windows[2 * w] / dt) + 1, ne = (int)(windows[2 * w + 1] / dt);
+    const int nl = ne - nb + 1;
+    std::vector<double> sv(syn + nb - 1, syn + ne), dv(dat + nb - 1, dat + ne);
+    orc_window_taper(sv.data(), nl, taper_pct, itaper);
+    orc_window_taper(dv.data(), nl, taper_pct, itaper);
+    const auto hs = hilbert(sv), hd = hilbert(dv);
+    std::vector<double> es(nl), ed(nl);
+    double ms = 0.0, md = 0.0;
+    for (int i = 0; i < nl; ++i) {
+      es[i] = std::abs(hs[i]);
+      ed[i] = std::abs(hd[i]);
+      ms = std::max(ms, es[i]);
+      md = std::max(md, ed[i]);
+    }
+    const double ts = wtr_env * ms, td = wtr_env * md;
+    std::vector<double> dr(nl), di(nl), dr2(nl), di2(nl), cub(nl);
+    double rr = 0.0, ri = 0.0;
+    for (int i = 0; i < nl; ++i) {
+      const double esw = es[i] + ts, edw = ed[i] + td;
+      dr[i] = dv[i] / edw - sv[i] / esw;
+      di[i] = hd[i].imag() / edw - hs[i].imag() / esw;
+      dr2[i] = dr[i] * dr[i];
+      di2[i] = di[i] * di[i];
+      cub[i] = esw * esw * esw;
+      rr = rr + dr[i];
+      ri = ri + di[i];
+    }
+    rr = rr / nl;
+    ri = ri / nl;
+    const double mr = 0.5 * orc_simpson(dr2.data(), nl, dt), mi = 0.5 * orc_simpson(di2.data(), nl, dt);
+    double* o = out + 8 * (size_t)w;
+    o[0] = mr + mi;
+    o[1] = rr + ri;
+    o[2] = mr;
+    o[3] = mi;
+    o[4] = rr;
+    o[5] = ri;
+    o[6] = 1.0;
+    o[7] = 0.0;
+    total = total + o[0];
+    std::vector<double> u1(nl), u2(nl);
+    for (int i = 0; i < nl; ++i) {
+      const double h = hs[i].imag();
+      u1[i] = dr[i] * sv[i] * h / cub[i];
+      u2[i] = di[i] * (sv[i] * sv[i]) / cub[i];
+    }
+    const auto h1 = hilbert(u1), h2 = hilbert(u2);
+    std::vector<double> ar(nl), ai(nl);
+    for (int i = 0; i < nl; ++i) {
+      const double h = hs[i].imag();
+      ar[i] = -1.0 * (dr[i] * (h * h) / cub[i]) - h1[i].imag();
+      ai[i] = di[i] * sv[i] * h / cub[i] + h2[i].imag();
+    }
+    orc_window_taper(ar.data(), nl, taper_pct, itaper);
+    orc_window_taper(ai.data(), nl, taper_pct, itaper);
+    for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = adj[nb - 1 + i] + (ar[i] + ai[i]);
   }
   return total;
 }

@@ -137,6 +137,10 @@ def lib():
         L.orc_cc_adjoint.argtypes = ([_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
                                       _dp] + [_dp] * 7 + [_ip])
         L.orc_fft.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
+        L.orc_hilbert.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.orc_ep_adjoint.restype = C.c_double
+        L.orc_ep_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
+                                     _dp, _dp]
         L.orc_dpss.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
         L.orc_mt_adjoint.restype = C.c_double
         L.orc_mt_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
@@ -598,3 +602,28 @@ def mt_adjoint(dat, syn, dt, windows, min_period, max_period, imeas=13, taper_pc
     o = {k: out[:, i].copy() for i, k in enumerate(MT_FIELDS)}
     o["selected"] = o["selected"] != 0
     return tot, adj, o
+
+
+def hilbert(x):
+    """The reference's analytic signal (fftpack hilbert): complex, len(x)."""
+    x = np.ascontiguousarray(x, np.float64)
+    re = np.zeros(len(x))
+    im = np.zeros(len(x))
+    lib().orc_hilbert(_p(x, C.c_double), len(x), _p(re, C.c_double), _p(im, C.c_double))
+    return re + 1j * im
+
+
+EP_FIELDS = ("misfit", "residual", "misfit_real", "misfit_imag", "residual_real", "residual_imag", "selected")
+
+
+def ep_adjoint(dat, syn, dt, windows, taper_pct=0.05, itaper=1, wtr_env=0.2):
+    """One trace's exponentiated-phase measurement: (total, adj, per-window dict of EP_FIELDS)."""
+    dat = np.ascontiguousarray(dat, np.float64)
+    syn = np.ascontiguousarray(syn, np.float64)
+    win = np.ascontiguousarray(np.reshape(windows, (-1, 2)), np.float64)
+    nw = win.shape[0]
+    adj = np.zeros(len(dat))
+    out = np.zeros((nw, 8))
+    tot = lib().orc_ep_adjoint(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt, _p(win, C.c_double), nw,
+                               taper_pct, itaper, wtr_env, _p(adj, C.c_double), _p(out, C.c_double))
+    return tot, adj, {k: out[:, i].copy() for i, k in enumerate(EP_FIELDS)}

@@ -100,6 +100,7 @@ SIGNATURES = {
     "fwat_waveform_adjoint_last_time_": "d",
     "fwat_cc_adjoint_batch_": "iidFFiDdiiDiFDDF",
     "fwat_mt_adjoint_batch_": "iidFFiDdiiDiFDDF",
+    "fwat_ep_adjoint_batch_": "iidFFiDdidiFDDF",
 }
 
 _lib = None

@@ -881,10 +881,132 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
   }
 }
 
+// Exponentiated phase (exponentiated_phase_misfit.f90:21-104) of one window
+// of every trace, one workgroup per trace on its tapered rows d, s.  hilbert
+// (fftpack.f90:135-201) on nfe = 2 ** exponent(real(nl)) points: the real
+// transform of the row (LDS, as the multitaper pass), then Re(xa) and Im(xa)
+// as two real-output inverse transforms of the Hermitian parts of h X and
+// -i h X (h: 1 at DC, 2 on positive frequencies, at the Nyquist 1 for an
+// even and 0 for an odd row length).  Envelopes with water level, the real
+// and imaginary differences (sums and Simpson misfits by fixed-order tree
+// sums), then the adjoint source with the Hilbert transforms of
+// dr s H / E^3 and di s^2 / E^3; both parts tapered and added into A.
+// Scratch rows per trace: G0, G1 (nfe/2 + 1 double2), R0..R3 (nl doubles).
+// out[8]: misfit, residual, misfit_real, misfit_imag, residual_real,
+// residual_imag, selected (1), 0.
+__global__ void __launch_bounds__(MT_THREADS) k_ep_window(int nl, double dt, int nfe, int logM, double wtr_env,
+                                                          const double2* __restrict__ tw,
+                                                          const double* __restrict__ wtap,
+                                                          const double* __restrict__ D, const double* __restrict__ S,
+                                                          double2* __restrict__ G0, double2* __restrict__ G1,
+                                                          double* __restrict__ R, double* __restrict__ A,
+                                                          double* __restrict__ out) {
+  __shared__ double2 buf[MT_MAXM];
+  __shared__ double red[MT_THREADS];
+  const int t = blockIdx.x, M = nfe / 2, lane = threadIdx.x, T = gridDim.x;
+  const unsigned shb = 32 - logM;
+  const double* s = S + (size_t)t * nl;
+  const double* d = D + (size_t)t * nl;
+  double2* g0 = G0 + (size_t)t * (M + 1);
+  double2* g1 = G1 + (size_t)t * (M + 1);
+  double* r0 = R + (size_t)t * nl;           // Re xa(s), then -(dr H^2 / E^3)
+  double* r1 = R + ((size_t)T + t) * nl;     // Im xa(s), then di s H / E^3
+  double* r2 = R + (2 * (size_t)T + t) * nl;  // Re xa(d), then dr s H / E^3
+  double* r3 = R + (3 * (size_t)T + t) * nl;  // Im xa(d), then di s^2 / E^3
+  const double hnyq = (nl % 2 == 0) ? 1.0 : 0.0;
+  auto xk = [&](int k) {
+    if (k == M) { const double2 z = buf[0]; return make_double2(z.x - z.y, 0.0); }
+    if (k == 0) { const double2 z = buf[0]; return make_double2(z.x + z.y, 0.0); }
+    return mt_half(buf[k], buf[M - k], tw[k]);
+  };
+  // real-output inverse of the Hermitian spectrum in g (k = 0..M) into y[0..nl)
+  auto inverse_to = [&](double2* g, double* y) {
+    for (int k = lane; k < M; k += MT_THREADS)
+      buf[__brev((unsigned)k) >> shb] = mt_repack(g[k], g[M - k], tw[k]);
+    __syncthreads();
+    mt_fft<MT_THREADS>(buf, M, logM, nfe, tw, true);
+    for (int n = lane; n < nl; n += MT_THREADS) {
+      const double2 z = buf[n >> 1];
+      y[n] = ((n & 1) ? z.y : z.x) / M;
+    }
+    __syncthreads();
+  };
+  // hilbert of row x: real part to yr (when wanted) and imaginary part to yi
+  auto hilbert = [&](const double* x, double* yr, double* yi) {
+    mt_load_real<MT_THREADS>(buf, M, logM, nl, [&](int n) { return x[n]; });
+    mt_fft<MT_THREADS>(buf, M, logM, nfe, tw, false);
+    for (int k = lane; k <= M; k += MT_THREADS) {
+      const double2 X = xk(k);
+      // Hermitian parts of h X and -i h X
+      if (yr) g0[k] = k == 0 ? make_double2(X.x, 0.0) : (k == M ? make_double2(hnyq * X.x, 0.0) : X);
+      g1[k] = (k == 0 || k == M) ? make_double2(0.0, 0.0) : make_double2(X.y, -X.x);
+    }
+    __syncthreads();
+    if (yr) inverse_to(g0, yr);
+    inverse_to(g1, yi);
+  };
+  hilbert(s, r0, r1);
+  hilbert(d, r2, r3);
+  double ms = 0.0, md = 0.0;
+  for (int n = lane; n < nl; n += MT_THREADS) {
+    ms = fmax(ms, hypot(r0[n], r1[n]));
+    md = fmax(md, hypot(r2[n], r3[n]));
+  }
+  const double ts = wtr_env * mt_bmax<MT_THREADS>(ms, red), td = wtr_env * mt_bmax<MT_THREADS>(md, red);
+  double rr = 0.0, ri = 0.0, sr = 0.0, si = 0.0;
+  for (int n = lane; n < nl; n += MT_THREADS) {
+    const double hs = r1[n], esw = hypot(r0[n], hs) + ts, edw = hypot(r2[n], r3[n]) + td;
+    const double dr = d[n] / edw - s[n] / esw;
+    const double di = r3[n] / edw - hs / esw;
+    rr += dr;
+    ri += di;
+    // simpson (utils.f90:1630-1663) as a weighted sum over the nl samples
+    double w, tail = 0.0;
+    if (nl % 2 == 0) {
+      w = (n == 0 ? 1.0 : 0.0) + (n == nl - 2 ? 1.0 : 0.0) + ((n % 2 == 1 && n <= nl - 3) ? 4.0 : 0.0) +
+          ((n % 2 == 0 && n >= 2 && n <= nl - 2) ? 2.0 : 0.0);
+      if (n >= nl - 2) tail = 1.0;
+    } else {
+      w = (n == 0 || n == nl - 1) ? 1.0 : (n % 2 == 1 ? 4.0 : 2.0);
+    }
+    sr += (w * (dt / 3.0) + tail * 0.5 * dt) * (dr * dr);
+    si += (w * (dt / 3.0) + tail * 0.5 * dt) * (di * di);
+    const double cub = esw * esw * esw;
+    r0[n] = -1.0 * (dr * (hs * hs) / cub);
+    r2[n] = dr * s[n] * hs / cub;
+    r1[n] = di * s[n] * hs / cub;
+    r3[n] = di * (s[n] * s[n]) / cub;
+  }
+  rr = mt_bsum<MT_THREADS>(rr, red) / nl;
+  ri = mt_bsum<MT_THREADS>(ri, red) / nl;
+  const double mr = 0.5 * (nl >= 2 ? mt_bsum<MT_THREADS>(sr, red) : 0.0);
+  const double mi = 0.5 * (nl >= 2 ? mt_bsum<MT_THREADS>(si, red) : 0.0);
+  if (lane == 0) {
+    double* o = out + 8 * (size_t)t;
+    o[0] = mr + mi;
+    o[1] = rr + ri;
+    o[2] = mr;
+    o[3] = mi;
+    o[4] = rr;
+    o[5] = ri;
+    o[6] = 1.0;
+    o[7] = 0.0;
+  }
+  // Im hilbert of dr s H / E^3 (r2 -> r2) and of di s^2 / E^3 (r3 -> r3)
+  hilbert(r2, nullptr, r2);
+  hilbert(r3, nullptr, r3);
+  double* a = A + (size_t)t * nl;
+  for (int n = lane; n < nl; n += MT_THREADS) {
+    const double ar = r0[n] - r2[n], ai = r1[n] + r3[n];
+    a[n] = ar * wtap[n] + ai * wtap[n];
+  }
+}
+
 // selected traces' adjoint-source rows added into the sample-major traces
 // (the multitaper measurement accumulates its windows)
 __global__ void __launch_bounds__(256) k_mt_put(int ntrace, int nb, int nl, const double* __restrict__ A,
-                                                const double* __restrict__ out, float* __restrict__ adj) {
+                                                const double* __restrict__ out, int ostride,
+                                                float* __restrict__ adj) {
   __shared__ double tile[64][65];
   const int t0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
   for (int r = threadIdx.x / 64; r < 64; r += 4) {
@@ -894,7 +1016,7 @@ __global__ void __launch_bounds__(256) k_mt_put(int ntrace, int nb, int nl, cons
   __syncthreads();
   for (int r = threadIdx.x / 64; r < 64; r += 4) {
     const int k = k0 + r, t = t0 + (threadIdx.x & 63);
-    if (t < ntrace && k < nl && out[10 * (size_t)t + 6] != 0.0) {
+    if (t < ntrace && k < nl && out[ostride * (size_t)t + 6] != 0.0) {
       float* a = adj + (size_t)(nb + k) * ntrace + t;
       *a = (float)((double)*a + tile[threadIdx.x & 63][r]);
     }
@@ -1309,7 +1431,7 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     hipLaunchKernelGGL(k_mt_window, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, nbs[w], nl,
                        *imeas, MC, d_tw, d_tap, d_obsT, d_S, d_flim, d_g01, d_g01 + (size_t)T * (M + 1), d_g2, d_A,
                        ow);
-    hipLaunchKernelGGL(k_mt_put, tiles, dim3(256), 0, s, T, nbs[w], nl, d_A, ow, d_adj);
+    hipLaunchKernelGGL(k_mt_put, tiles, dim3(256), 0, s, T, nbs[w], nl, d_A, ow, 10, d_adj);
   }
   std::vector<double> fw;
   double* d_fw = nullptr;
@@ -1345,6 +1467,120 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_twd); dev_free(d_ish); dev_free(d_flim);
   dev_free(d_out); dev_free(d_D); dev_free(d_S); dev_free(d_A); dev_free(d_tw); dev_free(d_tap); dev_free(d_bv);
   dev_free(d_bi); dev_free(d_fw); dev_free(d_sos); dev_free(d_g01); dev_free(d_g2); dev_free(d_obsT); dev_free(d_synT);
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  HIP_CHECK(hipStreamDestroy(s));
+}
+
+// exponentiated_phase_misfit.f90 ExponentiatedPhaseMisfit%calc_adjoint_source
+// (imeas 4) for every trace: wtr_env the envelope water level (adj_config);
+// meas(8, nwin, ntrace) = misfit, residual, misfit_real, misfit_imag,
+// residual_real, residual_imag, selected (1), 0; misfits(ntrace) summed over
+// windows; window adjoint sources summed, then band-passed (sos, nsect) and
+// tapered as filter_adj.  Windows of 32 to 16383 samples.
+void fwat_ep_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw* syn, int* nwin, double* windows,
+                            double* taper_percentage, int* itaper_type, double* wtr_env, int* nsect, realw* sos,
+                            double* misfits, double* meas, realw* adj) {
+  const int N = *nstep, T = *ntrace, nw = *nwin;
+  if (N <= 0 || T <= 0) SEM_FAIL("fwat_ep_adjoint_batch_: sizes must be positive");
+  if (nw < 1 || nw > MAX_WIN) SEM_FAIL("fwat_ep_adjoint_batch_: 1..%d windows", MAX_WIN);
+  if (T > 65535) SEM_FAIL("fwat_ep_adjoint_batch_: at most 65535 traces per call");
+  if (*nsect < 0 || *nsect > 8) SEM_FAIL("fwat_ep_adjoint_batch_: 0..8 filter sections");
+  if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
+  const size_t n = (size_t)N * T;
+  int maxnl = 0, maxM = 0;
+  std::vector<int> nbs(nw), nls(nw), nfes(nw);
+  std::vector<std::vector<double>> tws(nw);
+  double wmin = windows[0], wmax = windows[0];
+  for (int w = 0; w < nw; ++w) {
+    window_range(windows[2 * w], windows[2 * w + 1], *dt, N, nbs[w], nls[w]);
+    if (nls[w] < 32 || nls[w] > 16383) SEM_FAIL("fwat_ep_adjoint_batch_: windows of 32 to 16383 samples (got %d)", nls[w]);
+    taper_weights(nls[w], *taper_percentage, *itaper_type, tws[w]);
+    int e2 = 0;
+    std::frexp((float)nls[w], &e2);
+    nfes[w] = 1 << e2;  // 2 ** exponent(real(n)) of hilbert
+    maxnl = std::max(maxnl, nls[w]);
+    maxM = std::max(maxM, nfes[w] / 2);
+    wmin = std::min(wmin, std::min(windows[2 * w], windows[2 * w + 1]));
+    wmax = std::max(wmax, std::max(windows[2 * w], windows[2 * w + 1]));
+  }
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  float* d_obs = dev_upload(obs, n, s);
+  float* d_syn = dev_upload(syn, n, s);
+  float* d_adj = dev_alloc<float>(n);
+  double* d_out = dev_alloc<double>(8 * (size_t)nw * T);
+  double* d_D = dev_alloc<double>((size_t)T * maxnl);
+  double* d_S = dev_alloc<double>((size_t)T * maxnl);
+  double* d_A = dev_alloc<double>((size_t)T * maxnl);
+  double* d_R = dev_alloc<double>(4 * (size_t)T * maxnl);
+  double2* d_g01 = dev_alloc<double2>(2 * (size_t)T * (maxM + 1));
+  // twiddles exp(-2 pi i j / nfe), j < nfe / 2, and the window tapers of every window, uploaded once
+  std::vector<double> twh, twin;
+  std::vector<size_t> toff(nw), woff(nw);
+  for (int w = 0; w < nw; ++w) {
+    const int nfe = nfes[w], M = nfe / 2;
+    toff[w] = twh.size() / 2;
+    for (int j = 0; j < M; ++j) {
+      const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)nfe;
+      twh.push_back((double)std::cos(a));
+      twh.push_back((double)std::sin(a));
+    }
+    woff[w] = twin.size();
+    twin.insert(twin.end(), tws[w].begin(), tws[w].end());
+  }
+  double* d_tw = dev_upload(twin.data(), twin.size(), s);
+  double2* d_twd = reinterpret_cast<double2*>(dev_upload(twh.data(), twh.size(), s));
+  HIP_CHECK(hipEventRecord(ev0, s));
+  hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_adj);
+  for (int w = 0; w < nw; ++w) {
+    const int nl = nls[w], nfe = nfes[w], M = nfe / 2;
+    int logM = 0;
+    while ((1 << logM) < M) ++logM;
+    double* ow = d_out + 8 * (size_t)w * T;
+    const dim3 tiles((unsigned)((T + 63) / 64), (unsigned)((nl + 63) / 64));
+    hipLaunchKernelGGL(k_cc_rows, tiles, dim3(256), 0, s, T, nbs[w], nl, d_tw + woff[w], d_obs, d_syn, d_D, d_S);
+    hipLaunchKernelGGL(k_ep_window, dim3((unsigned)T), dim3(MT_THREADS), 0, s, nl, *dt, nfe, logM, *wtr_env,
+                       d_twd + toff[w], d_tw + woff[w], d_D, d_S, d_g01, d_g01 + (size_t)T * (M + 1), d_R, d_A, ow);
+    hipLaunchKernelGGL(k_mt_put, tiles, dim3(256), 0, s, T, nbs[w], nl, d_A, ow, 8, d_adj);
+  }
+  const unsigned nbk = (unsigned)((T + MTPB - 1) / MTPB);
+  std::vector<double> fw;
+  double* d_fw = nullptr;
+  float* d_sos = nullptr;
+  if (*nsect > 0) {
+    int fnb = 0, fnl = 0;
+    window_range(wmin, wmax, *dt, N, fnb, fnl);
+    taper_weights(fnl, *taper_percentage, *itaper_type, fw);
+    d_fw = dev_upload(fw.data(), fw.size(), s);
+    d_sos = dev_upload(sos, 6 * (size_t)*nsect, s);
+    hipLaunchKernelGGL(k_sos_filter, dim3(nbk), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
+                       d_adj);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(ev1, s));
+  std::vector<double> out(8 * (size_t)nw * T);
+  HIP_CHECK(hipMemcpyAsync(adj, d_adj, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+  g_meas_ms = ms;
+  for (int t = 0; t < T; ++t) {
+    double m = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      const double* o = &out[8 * ((size_t)w * T + t)];
+      m += o[0];
+      if (meas) for (int q = 0; q < 8; ++q) meas[8 * ((size_t)t * nw + w) + q] = o[q];
+    }
+    misfits[t] = m;
+  }
+  dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_out); dev_free(d_D); dev_free(d_S); dev_free(d_A);
+  dev_free(d_R); dev_free(d_g01); dev_free(d_tw); dev_free(d_twd); dev_free(d_fw); dev_free(d_sos);
   HIP_CHECK(hipEventDestroy(ev0));
   HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));

@@ -234,5 +234,31 @@ def event_mt_adjoint_sources(obs, syn, dt, windows, min_period, max_period, imea
     return total, adj, meas
 
 
+IMEAS_EXP_PHASE = 4  # adj_config
+EP_FIELDS = ("misfit", "residual", "misfit_real", "misfit_imag", "residual_real", "residual_imag", "selected")
+
+
+def event_ep_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, itaper_type=1, wtr_env=0.2,
+                             min_period=None, max_period=None):
+    """Exponentiated phase measurement (imeas 4, ExponentiatedPhaseMisfit%
+    calc_adjoint_source, exponentiated_phase_misfit.f90:21-104) of every trace in
+    one device pass: envelope-normalised real and Hilbert parts of data and
+    synthetic, their differences' Simpson misfits and mean residuals, the
+    adjoint source with its two Hilbert-transform terms.  wtr_env is the
+    envelope water level (adj_config).  Returns (misfit, adjoint traces float32
+    like syn, per-window measurements [nrec*3, nwin, 8] in EP_FIELDS order)."""
+    nstep, ntrace, win, o, s, sos = _event_inputs(obs, syn, dt, windows, min_period, max_period)
+    adj = np.zeros(s.shape, np.float32)
+    mf = np.zeros(ntrace)
+    meas = np.zeros((ntrace, win.shape[0], 8))
+    _lib.call("fwat_ep_adjoint_batch_", nstep, ntrace, float(dt), o, s, win.shape[0], win, float(taper_percentage),
+              int(itaper_type), float(wtr_env), sos.shape[0], sos if sos.size else np.zeros(6, np.float32), mf, meas,
+              adj)
+    total = 0.0
+    for v in mf:
+        total += v
+    return total, adj, meas
+
+
 def last_time_ms():
     return float(_lib.call("fwat_waveform_adjoint_last_time_", 0.0)[0])

@@ -226,3 +226,38 @@ def test_ccc_closed_forms():
     assert abs(np.dot(adj[nb - 1:ne], sw)) < 1e-10 * np.abs(adj).max() * np.abs(sw).sum()
     tot, adj, o = O.cc_adjoint(d, -s, dt, [[0.5, 10.0]], imeas=5)
     assert not o["selected"][0] and tot == 0.0 and np.all(adj == 0)
+
+
+# ---- exponentiated phase
+
+def test_hilbert_is_the_analytic_signal():
+    """Even length: scipy.signal.hilbert on the same padded length; odd length:
+    the same imaginary part, and a real part without the Nyquist component (the
+    reference's weights follow the parity of the input length)."""
+    x = np.random.default_rng(7).standard_normal(1000)
+    assert np.abs(O.hilbert(x) - signal.hilbert(x, N=1024)[:1000]).max() < 1e-12
+    x = np.random.default_rng(8).standard_normal(1001)
+    h, r = O.hilbert(x), signal.hilbert(x, N=1024)[:1001]
+    assert np.abs(h.imag - r.imag).max() < 1e-12
+    xny = np.fft.fft(x, 1024)[512].real / 1024 * (-1.0) ** np.arange(1001)
+    assert np.abs(h.real - (x - xny)).max() < 1e-12
+
+
+def test_exponentiated_phase_gain_invariance_and_gradient():
+    """The measurement ignores amplitude (d = A s: misfit and adjoint source
+    vanish); with the envelope well above the water level the adjoint source is
+    the gradient of the misfit: sum(adj * ds) dt equals the finite difference."""
+    dt, n = 0.01, 2000
+    t = np.arange(n) * dt
+    s = (1 + 0.3 * np.sin(0.5 * t)) * np.sin(2 * np.pi * t)
+    tot, adj, o = O.ep_adjoint(1.7 * s, s, dt, [[2.0, 14.0]])
+    assert tot < 1e-25 and np.abs(adj).max() < 1e-13
+    d = (1.2 + 0.2 * np.sin(0.4 * t)) * np.sin(2 * np.pi * (t - 0.07))
+    tot, adj, o = O.ep_adjoint(d, s, dt, [[2.0, 14.0]], taper_pct=0.0, wtr_env=1e-6)
+    assert abs(o["misfit"][0] - o["misfit_real"][0] - o["misfit_imag"][0]) < 1e-15
+    ds = np.sin(2 * np.pi * 0.8 * t + 0.3) * np.exp(-((t - 8) / 3) ** 2)
+    eps = 1e-6
+    mp, _, _ = O.ep_adjoint(d, s + eps * ds, dt, [[2.0, 14.0]], taper_pct=0.0, wtr_env=1e-6)
+    mm, _, _ = O.ep_adjoint(d, s - eps * ds, dt, [[2.0, 14.0]], taper_pct=0.0, wtr_env=1e-6)
+    fd = (mp - mm) / (2 * eps)
+    assert abs(np.sum(adj * ds) * dt - fd) < 5e-3 * abs(fd)
