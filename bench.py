@@ -680,9 +680,11 @@ def aniso_smooth(a):
 
 def measure(a):
     """Adjoint-source measurements of one C2-sized event on the device: every
-    trace (400 receivers x 3 components, 10000 samples at dt 0.004 s) through the
-    waveform misfit, the CC and the multitaper traveltime / amplitude
-    measurements over two windows, adjoint sources band-passed 0.4-10 s.  Synthetic traces: a random
+    trace (400 receivers x 3 components, 10000 samples at dt 0.004 s, a wavelet
+    plus 1 % noise) through the
+    waveform misfit, the CC, coda-correlation, exponentiated-phase and multitaper
+    measurements over two windows, adjoint sources band-passed 0.4-10 s, and the
+    window selection (sliding correlation at every sample).  Synthetic traces: a random
     wavelet per trace, the synthetic shifted and scaled."""
     from specfwat_amd import misfit
     nrec, nstep, dt = 400, a.steps or 10000, 0.004
@@ -697,6 +699,10 @@ def measure(a):
             w = np.exp(-((t - t0[r, c]) * f[r, c]) ** 2) * np.sin(2 * np.pi * f[r, c] * (t - t0[r, c]))
             obs[:, r, c] = w
             syn[:, r, c] = np.roll(w, int(rng.integers(-60, 60))) * rng.uniform(0.7, 1.3)
+    # background noise: no window is empty (an all-zero window makes the reference's
+    # zero-lag correlation coefficient 0/0)
+    obs += (0.01 * rng.standard_normal(obs.shape)).astype(np.float32)
+    syn += (0.01 * rng.standard_normal(syn.shape)).astype(np.float32)
     win = [[2.0, 22.0], [15.0, 39.0]]
     stages = {}
     for label, fn in (("waveform", lambda: misfit.event_adjoint_sources(obs, syn, dt, win, min_period=0.4,
@@ -705,6 +711,10 @@ def measure(a):
                                                                           min_period=0.4, max_period=10.0)),
                       ("cc_dlna", lambda: misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=12,
                                                                             min_period=0.4, max_period=10.0)),
+                      ("ccc", lambda: misfit.event_cc_adjoint_sources(obs, syn, dt, win, imeas=5, min_period=0.4,
+                                                                      max_period=10.0)),
+                      ("exp_phase", lambda: misfit.event_ep_adjoint_sources(obs, syn, dt, win, min_period=0.4,
+                                                                            max_period=10.0)),
                       ("mt_tt", lambda: misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.4, 10.0, imeas=13)),
                       ("mt_dlna", lambda: misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.4, 10.0,
                                                                           imeas=14))):
@@ -717,6 +727,14 @@ def measure(a):
             stages[label]["windows_by_measurement"] = {str(int(k)): int((used == k).sum()) for k in np.unique(used)}
         if not np.isfinite(out[1]).all():
             raise RuntimeError(f"non-finite {label} adjoint sources")
+    # window selection (WIN_SELECTOR_TYPE) of the same traces: first arrival 3 s before each wavelet
+    tp = t0[:, 0] - 3.0
+    dist = 2.4 * (t0[:, 0] + 8.0)
+    misfit.select_windows(obs, syn, dt, 0.0, tp, dist, 0.4)  # warm-up
+    ts = time.time()
+    wins = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, 0.4)
+    stages["win_sel"] = {"wall_s": time.time() - ts, "device_ms": misfit.last_time_ms(),
+                         "windows": int(sum(len(w) for w in wins))}
     log("measure: " + ", ".join(f"{k} {v['device_ms']:.1f} ms" for k, v in stages.items()))
     ntr = nrec * 3
     dev = stages["cc_tt"]["device_ms"]
@@ -752,6 +770,15 @@ def measure(a):
         stages["mt_tt"]["cpu_traces_per_s"] = sample / wall
         stages["mt_tt"]["cpu_sample"] = f"{sample} traces through the oracle's MT-TT + filter_adj, one thread, " \
                                         f"{wall:.1f}s"
+        sample = 3
+        ts = time.time()
+        for q in range(sample):
+            r, c = divmod(q, 3)
+            orc.win_sel(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], 0.4)
+        wall = time.time() - ts
+        stages["win_sel"]["cpu_traces_per_s"] = sample / wall
+        stages["win_sel"]["cpu_sample"] = f"{sample} traces through the oracle's window selection, one thread, " \
+                                          f"{wall:.1f}s"
     print(json.dumps(result), flush=True)
 
 

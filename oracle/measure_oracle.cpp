@@ -31,6 +31,8 @@
 //   orc_hilbert           hilbert (src/shared/fftpack.f90:135-201)
 //   orc_ep_adjoint        ExponentiatedPhaseMisfit%calc_adjoint_source
 //                         (exponentiated_phase_misfit.f90:21-104)
+//   orc_win_sel           win_sel_type%select_windows (window_selector/win_sel.f90:56-277,
+//                         439-469) without phase splitting
 //   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
 //   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
 //                         where the reference calls LAPACK dsterf
@@ -205,6 +207,38 @@ std::vector<std::complex<double>> hilbert(const std::vector<double>& x) {
   std::vector<cd> xa(n);
   for (int i = 0; i < n; ++i) xa[i] = cd(xf[i].real() / (double)nft, xf[i].imag() / (double)nft);
   return xa;
+}
+
+
+// find_maxima (signal.f90:600-631): number of interior samples with a strict
+// rise before and a strict fall after
+int count_maxima(const double* x, int n, double sign) {
+  if (n < 3) return 0;
+  int c = 0;
+  for (int i = 1; i < n - 1; ++i)
+    if (sign * x[i] - sign * x[i - 1] > 0.0 && sign * x[i + 1] - sign * x[i] < 0.0) ++c;
+  return c;
+}
+
+// xcorr_shift_coe (win_sel.f90:439-469): lag of the first correlation maximum and
+// the correlation coefficient of d with s shifted by it (d's emptied edge zeroed)
+void xcorr_shift_coe(const std::vector<double>& d, const std::vector<double>& s, double dt, double& tshift,
+                     double& coef) {
+  const int n = (int)s.size();
+  const int ish = xcorr_shift(d, s);
+  std::vector<double> ss(n, 0.0), ds(d);
+  for (int i = 0; i < n; ++i) {
+    const int j = i - ish;
+    if (j >= 0 && j < n) ss[i] = s[j];
+  }
+  tshift = (double)ish * dt;
+  if (ish > 0) for (int i = 0; i < ish && i < n; ++i) ds[i] = 0.0;
+  else if (ish < 0) for (int i = n + ish; i < n; ++i) if (i >= 0) ds[i] = 0.0;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = 0; i < n; ++i) a = a + ds[i] * ss[i];
+  for (int i = 0; i < n; ++i) b = b + ds[i] * ds[i];
+  for (int i = 0; i < n; ++i) c = c + ss[i] * ss[i];
+  coef = a / std::sqrt(b * c);
 }
 
 // number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
@@ -750,6 +784,85 @@ double orc_ep_adjoint(const double* dat, const double* syn, int n, double dt, co
     for (int i = 0; i < nl; ++i) adj[nb - 1 + i] = adj[nb - 1 + i] + (ar[i] + ai[i]);
   }
   return total;
+}
+
+// win_sel_type (src/preproc/window_selector/win_sel.f90:56-277) without phase
+// splitting: normalisation by max|syn|, sliding_cc (Hann-tapered windows of
+// int(sliding_win_len_fac min_period / dt) + 1 samples at every start),
+// good samples (cc, shift, time range), jump exclusion, contiguous groups and
+// the length / peak count / SNR / energy-ratio checks.  cfg: sliding_win_len_fac,
+// min_velocity, threshold_corr, threshold_shift_fac, jump_fac, min_win_len_fac,
+// min_snr_window, min_energy_ratio, min_peaks_troughs.  twin[maxwin][2] (s);
+// returns the window count.  cc_out / ts_out (npts - nlen) receive the sliding
+// correlation when not null.
+int orc_win_sel(const double* dat_in, const double* syn_in, int npts, double dt, double t0, double tp, double dis,
+                double min_period, const double* cfg, int maxwin, double* twin, double* cc_out, double* ts_out) {
+  double smax = 0.0;
+  for (int i = 0; i < npts; ++i) smax = std::max(smax, std::fabs(syn_in[i]));
+  std::vector<double> dat(npts), syn(npts);
+  for (int i = 0; i < npts; ++i) { dat[i] = dat_in[i] / smax; syn[i] = syn_in[i] / smax; }
+  const double tstart = tp - min_period * 1.5 + t0;
+  const double tend = dis / cfg[1] + min_period * 1.5 + t0;
+  const double jump = cfg[4] * min_period;
+  const int nstart = std::min(std::max((int)(tstart / dt) + 1, 1), npts);  // clamped to the trace
+  double e = 0.0;
+  for (int i = 0; i < nstart; ++i) e = e + dat_in[i] * dat_in[i];  // the un-normalised data, as written
+  const double noise = std::sqrt(e / (double)nstart);
+  // sliding_cc
+  const int nlen = (int)(cfg[0] * min_period / dt) + 1;
+  const int nw = npts - nlen;
+  if (nw <= 0) return 0;
+  std::vector<double> ts(nw), cc(nw), tc(nw);
+  std::vector<double> dw(nlen), sw(nlen);
+  for (int i = 0; i < nw; ++i) {
+    for (int j = 0; j < nlen; ++j) { dw[j] = dat[i + j]; sw[j] = syn[i + j]; }
+    orc_window_taper(dw.data(), nlen, 1.0, 1);
+    orc_window_taper(sw.data(), nlen, 1.0, 1);
+    xcorr_shift_coe(dw, sw, dt, ts[i], cc[i]);
+    tc[i] = ((double)i + (double)nlen / 2.0) * dt;
+    if (cc_out) cc_out[i] = cc[i];
+    if (ts_out) ts_out[i] = ts[i];
+  }
+  std::vector<char> good(nw);
+  for (int i = 0; i < nw; ++i)
+    good[i] = cc[i] > cfg[2] && std::fabs(ts[i]) < cfg[3] * min_period && tc[i] >= tstart && tc[i] <= tend;
+  for (int i = 0; i < nw; ++i) {
+    const double g = i == 0 ? 0.0 : ts[i] - ts[i - 1];
+    if (std::fabs(g) > jump) {
+      const int nj = (int)std::round(jump / dt);  // nint: half away from zero
+      for (int k = std::max(0, i - nj); k <= std::min(nw - 1, i + nj); ++k) good[k] = 0;
+    }
+  }
+  int count = 0;
+  auto check = [&](int a, int b) {  // 0-based group [a, b] of the sliding windows
+    if ((b - a) * dt < cfg[5] * min_period) return;
+    int ib = (int)(tc[a] / dt) + 1, ie = (int)(tc[b] / dt) + 1;
+    ib = std::max(1, std::min(ib, npts));
+    ie = std::max(1, std::min(ie, npts));
+    if (ie <= ib) return;
+    const int m = ie - ib + 1;
+    const double* dp = dat.data() + ib - 1;
+    const double* sp = syn.data() + ib - 1;
+    const int np = std::min(count_maxima(dp, m, 1.0) + count_maxima(dp, m, -1.0),
+                            count_maxima(sp, m, 1.0) + count_maxima(sp, m, -1.0));
+    if (np < (int)cfg[8]) return;
+    double amax = 0.0;
+    for (int i = 0; i < m; ++i) amax = std::max(amax, std::fabs(dp[i]));
+    if (amax / noise < cfg[6]) return;
+    double ed = 0.0, es = 0.0;
+    for (int i = 0; i < m; ++i) ed = ed + dp[i] * dp[i];
+    for (int i = 0; i < m; ++i) es = es + sp[i] * sp[i];
+    if (ed / es > cfg[7] || ed / es < 1 / cfg[7]) return;
+    if (count < maxwin) { twin[2 * count] = tc[a]; twin[2 * count + 1] = tc[b]; }
+    ++count;
+  };
+  int start = -1;
+  for (int i = 0; i < nw; ++i) {
+    if (good[i] && start < 0) start = i;
+    else if (!good[i] && start >= 0) { check(start, i - 1); start = -1; }
+  }
+  if (start >= 0) check(start, nw - 1);
+  return count;
 }
 
 }  // extern "C"

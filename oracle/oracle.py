@@ -138,6 +138,8 @@ def lib():
                                       _dp] + [_dp] * 7 + [_ip])
         L.orc_fft.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
         L.orc_hilbert.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.orc_win_sel.restype = C.c_int
+        L.orc_win_sel.argtypes = [_dp, _dp, C.c_int] + [C.c_double] * 5 + [_dp, C.c_int, _dp, _dp, _dp]
         L.orc_ep_adjoint.restype = C.c_double
         L.orc_ep_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
                                      _dp, _dp]
@@ -627,3 +629,25 @@ def ep_adjoint(dat, syn, dt, windows, taper_pct=0.05, itaper=1, wtr_env=0.2):
     tot = lib().orc_ep_adjoint(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt, _p(win, C.c_double), nw,
                                taper_pct, itaper, wtr_env, _p(adj, C.c_double), _p(out, C.c_double))
     return tot, adj, {k: out[:, i].copy() for i, k in enumerate(EP_FIELDS)}
+
+
+WIN_DEFAULTS = dict(sliding_win_len_fac=2.0, min_velocity=2.4, threshold_corr=0.8, threshold_shift_fac=0.3,
+                    jump_fac=0.1, min_win_len_fac=1.5, min_snr_window=5.0, min_energy_ratio=5.0,
+                    min_peaks_troughs=3)  # input_params.f90:389-397
+
+
+def win_sel(dat, syn, dt, t0, tp, dis, min_period, maxwin=1000, **cfg):
+    """win_sel_type%select_windows without phase splitting: (windows [n][2] in s,
+    sliding cc, sliding time shifts)."""
+    c = dict(WIN_DEFAULTS, **cfg)
+    cv = np.array([c[k] for k in WIN_DEFAULTS], np.float64)
+    dat = np.ascontiguousarray(dat, np.float64)
+    syn = np.ascontiguousarray(syn, np.float64)
+    n = len(dat)
+    nlen = int(c["sliding_win_len_fac"] * min_period / dt) + 1
+    cc = np.zeros(max(n - nlen, 1))
+    ts = np.zeros(max(n - nlen, 1))
+    tw = np.zeros((maxwin, 2))
+    k = lib().orc_win_sel(_p(dat, C.c_double), _p(syn, C.c_double), n, dt, t0, tp, dis, min_period,
+                          _p(cv, C.c_double), maxwin, _p(tw, C.c_double), _p(cc, C.c_double), _p(ts, C.c_double))
+    return tw[:min(k, maxwin)].copy(), cc[:max(n - nlen, 0)], ts[:max(n - nlen, 0)]

@@ -101,6 +101,7 @@ SIGNATURES = {
     "fwat_cc_adjoint_batch_": "iidFFiDdiiDiFDDF",
     "fwat_mt_adjoint_batch_": "iidFFiDdiiDiFDDF",
     "fwat_ep_adjoint_batch_": "iidFFiDdidiFDDF",
+    "fwat_win_sel_batch_": "iidFFdDDdDiID",
 }
 
 _lib = None

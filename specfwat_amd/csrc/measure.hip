@@ -1002,6 +1002,186 @@ __global__ void __launch_bounds__(MT_THREADS) k_ep_window(int nl, double dt, int
   }
 }
 
+// ---- window selection (window_selector/win_sel.f90, WIN_SELECTOR_TYPE)
+
+struct WsCfg {
+  double win_len_fac, min_velocity, thr_corr, thr_shift_fac, jump_fac, min_win_len_fac, min_snr, min_energy_ratio;
+  int min_peaks;
+};
+
+// per trace (one workgroup): max|syn| normalisation of the trace-major rows
+// (double), and the noise level sqrt(sum(dat(1:nstart)^2) / nstart) of the
+// un-normalised data as initialize computes it (win_sel.f90:56-74)
+__global__ void __launch_bounds__(256) k_ws_prep(int nstep, double dt, double t0, double min_period,
+                                                 const double* __restrict__ tp, const float* __restrict__ obsT,
+                                                 const float* __restrict__ synT, double* __restrict__ Dn,
+                                                 double* __restrict__ Sn, double* __restrict__ noise) {
+  __shared__ double sh[256];
+  const int t = blockIdx.x;
+  const float* d = obsT + (size_t)t * nstep;
+  const float* s = synT + (size_t)t * nstep;
+  const double tstart = tp[t] - min_period * 1.5 + t0;
+  const int nstart = min(max((int)(tstart / dt) + 1, 1), nstep);
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nstep; i += 256) m = fmax(m, fabs((double)s[i]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double smax = sh[0];
+  for (int i = threadIdx.x; i < nstep; i += 256) {
+    Dn[(size_t)t * nstep + i] = (double)d[i] / smax;
+    Sn[(size_t)t * nstep + i] = (double)s[i] / smax;
+  }
+  if (threadIdx.x == 0) {  // the sum in the reference's order
+    double e = 0.0;
+    for (int i = 0; i < nstart; ++i) e = e + (double)d[i] * (double)d[i];
+    noise[t] = sqrt(e / (double)nstart);
+  }
+}
+
+// sliding_cc (win_sel.f90:76-100): one wavefront per window start i; the Hann
+// tapered windows of d and s in LDS, the full correlation with every lag's sum
+// in the reference's order (lanes over lags), first maximum, then
+// xcorr_shift_coe's coefficient with its three sums in order on one lane
+__global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw, double dt,
+                                                             const double* __restrict__ wt,
+                                                             const double* __restrict__ Dn,
+                                                             const double* __restrict__ Sn, double* __restrict__ ts,
+                                                             double* __restrict__ cc) {
+  extern __shared__ double wsh[];
+  const int nwv = blockDim.x / 64;  // one window start per wavefront, 2 nlen doubles of LDS each
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, t = blockIdx.y;
+  const int i = blockIdx.x * nwv + wave;
+  double* dw = wsh + (size_t)wave * 2 * nlen;
+  double* sw = dw + nlen;
+  if (i < nw) {
+    const double* d = Dn + (size_t)t * nstep + i;
+    const double* s = Sn + (size_t)t * nstep + i;
+    for (int j = lane; j < nlen; j += 64) { dw[j] = d[j] * wt[j]; sw[j] = s[j] * wt[j]; }
+  }
+  __syncthreads();
+  if (i >= nw) return;
+  double best = -INFINITY;
+  int bm = 0x7fffffff;
+  for (int m = lane; m < 2 * nlen - 1; m += 64) {
+    const int lo = m - (nlen - 1) > 0 ? m - (nlen - 1) : 0, hi = m < nlen - 1 ? m : nlen - 1;
+    double c = 0.0;
+    for (int i1 = lo; i1 <= hi; ++i1) c = c + dw[i1] * sw[i1 + nlen - 1 - m];
+    if (c > best) { best = c; bm = m; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v2 = __shfl_xor(best, o);
+    const int m2 = __shfl_xor(bm, o);
+    if (v2 > best || (v2 == best && m2 < bm)) { best = v2; bm = m2; }
+  }
+  if (lane != 0) return;
+  const int ish = bm + 1 - nlen;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int j = 0; j < nlen; ++j) {
+    const int k = j - ish;
+    const double sv = (k >= 0 && k < nlen) ? sw[k] : 0.0;
+    const double dv = (ish > 0 && j < ish) || (ish < 0 && j >= nlen + ish) ? 0.0 : dw[j];
+    a = a + dv * sv;
+  }
+  for (int j = 0; j < nlen; ++j) {
+    const double dv = (ish > 0 && j < ish) || (ish < 0 && j >= nlen + ish) ? 0.0 : dw[j];
+    b = b + dv * dv;
+  }
+  for (int j = 0; j < nlen; ++j) {
+    const int k = j - ish;
+    const double sv = (k >= 0 && k < nlen) ? sw[k] : 0.0;
+    c = c + sv * sv;
+  }
+  ts[(size_t)t * nw + i] = (double)ish * dt;
+  cc[(size_t)t * nw + i] = a / sqrt(b * c);
+}
+
+// select_windows (win_sel.f90:102-277) without phase splitting, one lane per
+// trace: good samples (cc, shift, time range), samples within
+// nint(jump / dt) of a time-shift jump dropped, contiguous groups, and the
+// length, peak-count, SNR and energy-ratio checks (one pass over each
+// group's samples, sums in the reference's order).  Windows in seconds,
+// win[t][k][2], k < maxwin; nwin[t] the count found.
+__global__ void k_ws_select(int ntrace, int nstep, int nlen, int nw, double dt, double t0, double min_period,
+                            WsCfg cfg, const double* __restrict__ tp, const double* __restrict__ dist,
+                            const double* __restrict__ noise, const double* __restrict__ Dn,
+                            const double* __restrict__ Sn, const double* __restrict__ ts,
+                            const double* __restrict__ cc, int maxwin, int* __restrict__ nwin,
+                            double* __restrict__ win) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrace) return;
+  const double tstart = tp[t] - min_period * 1.5 + t0;
+  const double tend = dist[t] / cfg.min_velocity + min_period * 1.5 + t0;
+  const double jump = cfg.jump_fac * min_period;
+  const int nj = (int)round(jump / dt);
+  const double* tsr = ts + (size_t)t * nw;
+  const double* ccr = cc + (size_t)t * nw;
+  const double* dat = Dn + (size_t)t * nstep;
+  const double* syn = Sn + (size_t)t * nstep;
+  auto tcc = [&](int i) { return ((double)i + (double)nlen / 2.0) * dt; };
+  auto raw_good = [&](int i) {
+    const double tc = tcc(i);
+    return ccr[i] > cfg.thr_corr && fabs(tsr[i]) < cfg.thr_shift_fac * min_period && tc >= tstart && tc <= tend;
+  };
+  auto is_jump = [&](int i) { return i > 0 && fabs(tsr[i] - tsr[i - 1]) > jump; };
+  // a sample is dropped when a jump lies within nj of it; keep the last jump index seen
+  int last_jump = -(1 << 29);
+  int next_jump = -1;  // first jump index >= i, found lazily
+  auto good = [&](int i) {
+    if (is_jump(i)) last_jump = i;
+    if (next_jump < i) {
+      next_jump = i;
+      while (next_jump < nw && !is_jump(next_jump)) ++next_jump;
+    }
+    if (i - last_jump <= nj) return false;
+    if (next_jump < nw && next_jump - i <= nj) return false;
+    return raw_good(i);
+  };
+  int count = 0;
+  auto check = [&](int a, int b) {
+    if ((b - a) * dt < cfg.min_win_len_fac * min_period) return;
+    int ib = (int)(tcc(a) / dt) + 1, ie = (int)(tcc(b) / dt) + 1;
+    ib = max(1, min(ib, nstep));
+    ie = max(1, min(ie, nstep));
+    if (ie <= ib) return;
+    const double* dp = dat + ib - 1;
+    const double* sp = syn + ib - 1;
+    const int m = ie - ib + 1;
+    int pd = 0, td = 0, ps = 0, tq = 0;
+    double amax = 0.0, ed = 0.0, es = 0.0;
+    for (int k = 0; k < m; ++k) {
+      if (k >= 1 && k < m - 1) {
+        if (dp[k] - dp[k - 1] > 0.0 && dp[k + 1] - dp[k] < 0.0) ++pd;
+        if (-dp[k] - -dp[k - 1] > 0.0 && -dp[k + 1] - -dp[k] < 0.0) ++td;
+        if (sp[k] - sp[k - 1] > 0.0 && sp[k + 1] - sp[k] < 0.0) ++ps;
+        if (-sp[k] - -sp[k - 1] > 0.0 && -sp[k + 1] - -sp[k] < 0.0) ++tq;
+      }
+      amax = fmax(amax, fabs(dp[k]));
+      ed = ed + dp[k] * dp[k];
+      es = es + sp[k] * sp[k];
+    }
+    if (min(pd + td, ps + tq) < cfg.min_peaks) return;
+    if (amax / noise[t] < cfg.min_snr) return;
+    if (ed / es > cfg.min_energy_ratio || ed / es < 1 / cfg.min_energy_ratio) return;
+    if (count < maxwin) {
+      win[((size_t)t * maxwin + count) * 2] = tcc(a);
+      win[((size_t)t * maxwin + count) * 2 + 1] = tcc(b);
+    }
+    ++count;
+  };
+  int start = -1;
+  for (int i = 0; i < nw; ++i) {
+    const bool g = good(i);
+    if (g && start < 0) start = i;
+    else if (!g && start >= 0) { check(start, i - 1); start = -1; }
+  }
+  if (start >= 0) check(start, nw - 1);
+  nwin[t] = count;
+}
+
 // selected traces' adjoint-source rows added into the sample-major traces
 // (the multitaper measurement accumulates its windows)
 __global__ void __launch_bounds__(256) k_mt_put(int ntrace, int nb, int nl, const double* __restrict__ A,
@@ -1581,6 +1761,78 @@ void fwat_ep_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   }
   dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_out); dev_free(d_D); dev_free(d_S); dev_free(d_A);
   dev_free(d_R); dev_free(d_g01); dev_free(d_tw); dev_free(d_twd); dev_free(d_fw); dev_free(d_sos);
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  HIP_CHECK(hipStreamDestroy(s));
+}
+
+// win_sel_type%select_windows (window_selector/win_sel.f90:35-277) for every
+// trace of an event: obs, syn [nstep][ntrace] (band-passed as the caller does
+// before selection), tp(ntrace) first-arrival times, dist(ntrace) epicentral
+// distances, t0 the simulation start offset; cfg(10) = sliding_win_len_fac,
+// min_velocity, threshold_corr, threshold_shift_fac, jump_fac, min_win_len_fac,
+// min_snr_window, min_energy_ratio, min_peaks_troughs, is_split_phases (must
+// be 0: phase splitting is not built); nwin(ntrace) out (windows found, may
+// exceed maxwin), windows(2, maxwin, ntrace) out in seconds
+void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw* syn, double* t0, double* tp,
+                         double* dist, double* min_period, double* cfg, int* maxwin, int* nwin, double* windows) {
+  const int N = *nstep, T = *ntrace;
+  if (N <= 0 || T <= 0) SEM_FAIL("fwat_win_sel_batch_: sizes must be positive");
+  if (T > 65535) SEM_FAIL("fwat_win_sel_batch_: at most 65535 traces per call");
+  if (cfg[9] != 0.0) SEM_FAIL("fwat_win_sel_batch_: phase splitting (is_split_phases) is not supported");
+  if (*maxwin < 1) SEM_FAIL("fwat_win_sel_batch_: maxwin must be positive");
+  const int nlen = (int)(cfg[0] * *min_period / *dt) + 1;
+  if (nlen < 2 || nlen > 4096) SEM_FAIL("fwat_win_sel_batch_: sliding window of %d samples (2..4096)", nlen);
+  const int nw = N - nlen;
+  const WsCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6], cfg[7], (int)cfg[8]};
+  if (nw <= 0) {
+    for (int t = 0; t < T; ++t) nwin[t] = 0;
+    return;
+  }
+  std::vector<double> wt;
+  taper_weights(nlen, 1.0, 1, wt);
+  const size_t n = (size_t)N * T;
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  float* d_obs = dev_upload(obs, n, s);
+  float* d_syn = dev_upload(syn, n, s);
+  double* d_tp = dev_upload(tp, (size_t)T, s);
+  double* d_dist = dev_upload(dist, (size_t)T, s);
+  double* d_wt = dev_upload(wt.data(), wt.size(), s);
+  float* d_obsT = dev_alloc<float>(n);
+  float* d_synT = dev_alloc<float>(n);
+  double* d_Dn = dev_alloc<double>(n);
+  double* d_Sn = dev_alloc<double>(n);
+  double* d_noise = dev_alloc<double>(T);
+  double* d_ts = dev_alloc<double>((size_t)nw * T);
+  double* d_cc = dev_alloc<double>((size_t)nw * T);
+  int* d_nwin = dev_alloc<int>(T);
+  double* d_win = dev_alloc<double>(2 * (size_t)*maxwin * T);
+  HIP_CHECK(hipEventRecord(ev0, s));
+  const dim3 ttiles((unsigned)((T + 63) / 64), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_obs, d_obsT);
+  hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_syn, d_synT);
+  hipLaunchKernelGGL(k_ws_prep, dim3((unsigned)T), dim3(256), 0, s, N, *dt, *t0, *min_period, d_tp, d_obsT, d_synT,
+                     d_Dn, d_Sn, d_noise);
+  const int nwv = nlen <= 1024 ? 4 : (nlen <= 2048 ? 2 : 1);  // 64 KB of LDS per workgroup
+  hipLaunchKernelGGL(k_ws_sliding, dim3((unsigned)((nw + nwv - 1) / nwv), (unsigned)T), dim3(64 * nwv),
+                     (size_t)nwv * 2 * nlen * sizeof(double), s, N, nlen, nw, *dt, d_wt, d_Dn, d_Sn, d_ts, d_cc);
+  hipLaunchKernelGGL(k_ws_select, dim3((unsigned)((T + 63) / 64)), dim3(64), 0, s, T, N, nlen, nw, *dt, *t0,
+                     *min_period, C, d_tp, d_dist, d_noise, d_Dn, d_Sn, d_ts, d_cc, *maxwin, d_nwin, d_win);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(ev1, s));
+  HIP_CHECK(hipMemcpyAsync(nwin, d_nwin, T * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(windows, d_win, 2 * (size_t)*maxwin * T * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+  g_meas_ms = ms;
+  dev_free(d_obs); dev_free(d_syn); dev_free(d_tp); dev_free(d_dist); dev_free(d_wt); dev_free(d_obsT);
+  dev_free(d_synT); dev_free(d_Dn); dev_free(d_Sn); dev_free(d_noise); dev_free(d_ts); dev_free(d_cc);
+  dev_free(d_nwin); dev_free(d_win);
   HIP_CHECK(hipEventDestroy(ev0));
   HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));

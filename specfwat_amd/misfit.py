@@ -260,5 +260,80 @@ def event_ep_adjoint_sources(obs, syn, dt, windows=None, taper_percentage=0.05, 
     return total, adj, meas
 
 
+IMEAS_WAVEFORM, IMEAS_WAVEFORM_CONV, IMEAS_RF = 1, 2, 3  # adj_config
+
+
+def calculate_adjoint_sources(obs, syn, dt, windows, short_p, long_p, imeasure_type, taper_percentage=0.05,
+                              itaper_type=1, **cfg):
+    """calculate_adjoint_source (adjoint_source.f90:14-89) for every trace of an
+    event: dispatch on imeasure_type to the batched measurement, then filter_adj
+    (bandpass 1/long_p .. 1/short_p and the taper over the windows) as the
+    reference always does.  Types 2 (waveform convolution) and 3 (receiver
+    function) stop with an error there and raise here.  Returns (misfit,
+    adjoint traces, per-window measurements or None for the waveform type)."""
+    kw = dict(taper_percentage=taper_percentage, itaper_type=itaper_type)
+    if imeasure_type == IMEAS_WAVEFORM:
+        if cfg:
+            raise KeyError(f"settings {sorted(cfg)} do not apply to the waveform misfit")
+        total, adj = event_adjoint_sources(obs, syn, dt, windows, min_period=short_p, max_period=long_p, **kw)
+        return total, adj, None
+    if imeasure_type == IMEAS_WAVEFORM_CONV:
+        raise ValueError("WAVEFORM_CONV misfit requires radial/vertical components "
+                         "(calc_adjoint_source_conv in the reference)")
+    if imeasure_type == IMEAS_RF:
+        raise ValueError("RF misfit requires additional parameters (calc_adjoint_source_rf in the reference)")
+    if imeasure_type == IMEAS_EXP_PHASE:
+        return event_ep_adjoint_sources(obs, syn, dt, windows, min_period=short_p, max_period=long_p, **kw, **cfg)
+    if imeasure_type in (IMEAS_CCC, IMEAS_CC_TT, IMEAS_CC_DLNA):
+        return event_cc_adjoint_sources(obs, syn, dt, windows, imeas=imeasure_type, min_period=short_p,
+                                        max_period=long_p, **kw, **cfg)
+    if imeasure_type in (IMEAS_CC_TT_MT, IMEAS_CC_DLNA_MT):
+        return event_mt_adjoint_sources(obs, syn, dt, windows, short_p, long_p, imeas=imeasure_type, **kw, **cfg)
+    raise ValueError(f"Unknown measurement type: {imeasure_type}")
+
+
+WIN_DEFAULTS = dict(sliding_win_len_fac=2.0, min_velocity=2.4, threshold_corr=0.8, threshold_shift_fac=0.3,
+                    jump_fac=0.1, min_win_len_fac=1.5, min_snr_window=5.0, min_energy_ratio=5.0,
+                    min_peaks_troughs=3, is_split_phases=False)  # WIN_TYPE_1 defaults, input_params.f90:389-398
+
+
+def select_windows(obs, syn, dt, t0, tp, dist, min_period, maxwin=64, **cfg):
+    """win_sel_type%select_windows (window_selector/win_sel.f90:35-277) for every
+    trace of an event in one device pass: the sliding cross-correlation (one full
+    correlation of Hann-tapered windows per sample), good-sample mask with the
+    time-shift jump rule, contiguous groups and the length / peak / SNR / energy
+    checks.  obs, syn (NSTEP, nrec, 3), already filtered as the caller does;
+    tp (nrec,) first-arrival times and dist (nrec,) distances (the receiver's
+    three components share them), t0 the simulation start.  Returns a list per
+    trace (nrec*3, receiver-major) of [n, 2] window arrays in seconds.  Phase
+    splitting (is_split_phases) is not built."""
+    o = np.ascontiguousarray(obs, np.float32)
+    s = np.ascontiguousarray(syn, np.float32)
+    if o.shape != s.shape:
+        raise ValueError("observed and synthetic traces differ in shape")
+    unknown = set(cfg) - set(WIN_DEFAULTS)
+    if unknown:
+        raise KeyError(f"unknown window settings {sorted(unknown)}")
+    c = dict(WIN_DEFAULTS, **cfg)
+    if c["is_split_phases"]:
+        raise NotImplementedError("phase splitting of the window selector is not built")
+    nstep = s.shape[0]
+    ntrace = s.size // nstep
+    ncomp = ntrace // np.size(tp)
+    tpv = np.ascontiguousarray(np.repeat(np.asarray(tp, np.float64), ncomp))
+    dv = np.ascontiguousarray(np.repeat(np.asarray(dist, np.float64), ncomp))
+    if tpv.size != ntrace or dv.size != ntrace:
+        raise ValueError("tp and dist need one value per receiver")
+    cv = np.array([c[k] for k in WIN_DEFAULTS], np.float64)
+    cv[-1] = 1.0 if c["is_split_phases"] else 0.0
+    nwin = np.zeros(ntrace, np.int32)
+    win = np.zeros((ntrace, maxwin, 2))
+    _lib.call("fwat_win_sel_batch_", nstep, ntrace, float(dt), o, s, float(t0), tpv, dv, float(min_period), cv,
+              int(maxwin), nwin, win)
+    if (nwin > maxwin).any():
+        raise RuntimeError(f"a trace has more than maxwin={maxwin} windows; raise maxwin")
+    return [win[t, :nwin[t]].copy() for t in range(ntrace)]
+
+
 def last_time_ms():
     return float(_lib.call("fwat_waveform_adjoint_last_time_", 0.0)[0])

@@ -188,3 +188,48 @@ def test_ep_batch_matches_oracle(filt):
             assert np.abs(adj[:, r, c] - a).max() <= bar * max(np.abs(a).max(), 1e-30)
     assert abs(total - tot) <= 1e-9 * tot
     print(f"exponentiated phase, filter {filt}: misfit {total:.6e}")
+
+
+def test_calculate_adjoint_sources_dispatch():
+    """The calculate_adjoint_source mirror runs every type the reference
+    dispatches, always with filter_adj; each result is the direct batched call's."""
+    obs, syn, dt = _mt_event(3)
+    win = [[10.0, 32.0]]
+    for im in (1, 4, 5, 11, 12, 13, 14):
+        tot, adj, meas = misfit.calculate_adjoint_sources(obs, syn, dt, win, 0.5, 10.0, im)
+        assert np.isfinite(tot) and np.isfinite(adj).all() and np.abs(adj).max() > 0
+        if im == 13:
+            t2, a2, m2 = misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.5, 10.0, imeas=13)
+            assert t2 == tot and np.array_equal(a2, adj)
+        print(f"imeasure_type {im}: misfit {tot:.6e}")
+
+
+def test_window_selection_matches_oracle():
+    """win_sel_type%select_windows for every trace: the same windows as the
+    oracle (bit-identical correlation sums and decisions), including traces
+    split by a time-shift jump and a trace with nothing to select."""
+    rng = np.random.default_rng(17)
+    nstep, nrec, dt, T = 3000, 5, 0.02, 2.0
+    t = np.arange(nstep) * dt
+    w = lambda t0, f: np.exp(-((t - t0) * f * 0.6) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    obs = np.zeros((nstep, nrec, 3), np.float32)
+    syn = np.zeros_like(obs)
+    tp = rng.uniform(8, 14, nrec)
+    dist = rng.uniform(60, 120, nrec)
+    for r in range(nrec):
+        for c in range(3):
+            a1, a2 = tp[r] + 4 + rng.uniform(0, 3), tp[r] + 20 + rng.uniform(0, 5)
+            syn[:, r, c] = w(a1, 0.5) + 0.7 * w(a2, 0.45)
+            jump = rng.choice([0.0, 0.0, 0.4])
+            obs[:, r, c] = w(a1 + rng.uniform(-0.1, 0.1), 0.5) + 0.7 * w(a2 + jump, 0.45) \
+                + 0.003 * rng.standard_normal(nstep)
+    obs[:, 4, 1] = 0.003 * rng.standard_normal(nstep)  # nothing to select
+    got = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, T)
+    counts = []
+    for r in range(nrec):
+        for c in range(3):
+            ref, _, _ = orc.win_sel(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], T)
+            np.testing.assert_array_equal(got[3 * r + c], ref)
+            counts.append(len(ref))
+    assert max(counts) >= 2 and min(counts) == 0
+    print(f"windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")

@@ -216,3 +216,16 @@ def test_measurement_rejects_windows_outside_the_trace():
         event_adjoint_sources(obs, obs, 0.01, [[0.5, 2.0]])
     with pytest.raises(ValueError):
         event_adjoint_sources(obs, obs, 0.01, None, min_period=1.0)
+
+
+def test_adjoint_source_dispatch_errors():
+    """calculate_adjoint_source stops for the types it cannot dispatch, as the
+    reference (adjoint_source.f90:30-42, 74-86); checked before any device call."""
+    from specfwat_amd import misfit
+    obs = np.zeros((100, 2, 3), np.float32)
+    for im, msg in ((2, "WAVEFORM_CONV"), (3, "RF"), (7, "Unknown measurement type")):
+        with pytest.raises(ValueError, match=msg):
+            misfit.calculate_adjoint_sources(obs, obs, 0.01, [[0.1, 0.9]], 0.5, 10.0, im)
+    with pytest.raises(ValueError, match="16383"):
+        misfit.event_mt_adjoint_sources(np.zeros((20000, 1, 3), np.float32), np.zeros((20000, 1, 3), np.float32),
+                                        0.01, [[1.0, 9.0]], 0.5, 10.0)

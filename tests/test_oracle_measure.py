@@ -261,3 +261,70 @@ def test_exponentiated_phase_gain_invariance_and_gradient():
     mm, _, _ = O.ep_adjoint(d, s - eps * ds, dt, [[2.0, 14.0]], taper_pct=0.0, wtr_env=1e-6)
     fd = (mp - mm) / (2 * eps)
     assert abs(np.sum(adj * ds) * dt - fd) < 5e-3 * abs(fd)
+
+
+# ---- window selection (win_sel_type, no phase splitting)
+
+def _hann_full(n):
+    """window_taper(., 1.0, 1) weights written out: frac = n/2, Hann halves."""
+    frac = n // 2
+    w = np.ones(n)
+    r = np.arange(frac)
+    w[:frac] = 0.5 - 0.5 * np.cos(2 * np.pi * r / (2 * frac - 1))
+    w[n - frac:] = 0.5 - 0.5 * np.cos(2 * np.pi * (frac + r) / (2 * frac - 1))
+    return w
+
+
+def test_sliding_correlation_is_numpy_correlate():
+    """Every window start: lag of np.correlate's first maximum on the tapered
+    windows and the coefficient of d with s shifted by it."""
+    dt, n = 0.05, 320
+    rng = np.random.default_rng(9)
+    syn = np.convolve(rng.standard_normal(n), np.hanning(15), "same")
+    dat = np.roll(syn, 2) + 0.1 * rng.standard_normal(n)
+    tw, cc, ts = O.win_sel(dat, syn, dt, 0.0, 2.0, 40.0, 1.0)
+    nlen = int(2.0 * 1.0 / dt) + 1
+    w = _hann_full(nlen)
+    scale = np.abs(syn).max()
+    for i in range(0, n - nlen, 7):
+        d = dat[i:i + nlen] / scale * w
+        s = syn[i:i + nlen] / scale * w
+        ish = int(np.argmax(np.correlate(d, s, "full"))) - (nlen - 1)
+        assert ts[i] == ish * dt
+        ss = np.zeros(nlen)
+        src = np.arange(nlen) - ish
+        ok = (src >= 0) & (src < nlen)
+        ss[ok] = s[src[ok]]
+        dd = d.copy()
+        if ish > 0:
+            dd[:ish] = 0
+        elif ish < 0:
+            dd[nlen + ish:] = 0
+        assert abs(cc[i] - np.dot(dd, ss) / np.sqrt(np.dot(dd, dd) * np.dot(ss, ss))) < 1e-12
+
+
+def test_windows_follow_the_selection_rules():
+    """One clean arrival selected inside [tp - 1.5 T + t0, dist / v + 1.5 T + t0];
+    a time-shift jump between two arrivals splits them into two windows that
+    keep nint(jump_fac T / dt) samples away from the jump."""
+    dt, n, T = 0.02, 3000, 2.0
+    t = np.arange(n) * dt
+    rng = np.random.default_rng(10)
+    w = lambda t0, f: np.exp(-((t - t0) * f * 0.6) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    syn = w(20, 0.5)
+    dat = syn + 0.002 * rng.standard_normal(n)
+    tw, cc, ts = O.win_sel(dat, syn, dt, 0.0, 15.0, 72.0, T)
+    assert len(tw) == 1 and 15.0 - 1.5 * T <= tw[0, 0] < 20.0 < tw[0, 1] <= 72.0 / 2.4 + 1.5 * T
+    syn2 = w(20, 0.5) + w(38, 0.5)
+    dat2 = w(20, 0.5) + w(38.4, 0.5) + 0.002 * rng.standard_normal(n)
+    tw2, cc2, ts2 = O.win_sel(dat2, syn2, dt, 0.0, 15.0, 100.0, T)
+    assert len(tw2) == 2 and tw2[0, 1] < tw2[1, 0]
+    nlen = int(2.0 * T / dt) + 1
+    jumps = np.nonzero(np.abs(np.diff(ts2)) > 0.1 * T)[0] + 1
+    assert len(jumps) > 0
+    tc = (np.arange(len(ts2)) + nlen / 2.0) * dt
+    nj = int(round(0.1 * T / dt))
+    for a, b in tw2:
+        ia, ib = int(round(a / dt - nlen / 2.0)), int(round(b / dt - nlen / 2.0))
+        assert np.all(np.abs(jumps[:, None] - np.arange(ia, ib + 1)[None, :]) > nj)
+        assert abs(tc[ia] - a) < 1e-9 and abs(tc[ib] - b) < 1e-9
