@@ -147,50 +147,73 @@ __global__ void __launch_bounds__(256) k_cc_rows(int ntrace, int nb, int nl, con
   }
 }
 
-// xcorr_shift, first stage: workgroup (trace, block of 256 lags) of the full
-// correlation; d and the matching stretch of s come through LDS in chunks of
-// 1024 samples taken in increasing i1, so each lag's sum runs over i1 in
-// increasing order as in myconvolution_dp.  Writes the block's first maximum.
-constexpr int CC_LAGS = 256, CC_CHUNK = 1024;
-__global__ void __launch_bounds__(CC_LAGS) k_cc_corr(int nl, const double* __restrict__ D,
-                                                     const double* __restrict__ S, double* __restrict__ bval,
-                                                     int* __restrict__ bidx) {
+// xcorr_shift, first stage: workgroup (trace, block of 1024 lags, 4 consecutive
+// lags per lane) of the full correlation; d and the matching stretch of s come
+// through LDS in chunks of 1024 samples taken in increasing i1, so each lag's
+// sum runs over i1 in increasing order as in myconvolution_dp.  A lane keeps
+// its four lags' s samples in a sliding register window (one LDS read per
+// step for four products); it runs over the union of its lags' i1 ranges, and
+// the staged s is zero outside the trace, so each lag only gains exact zeros.
+// Writes the block's first maximum.
+constexpr int CC_LPL = 4, CC_LAGS = 256 * CC_LPL, CC_CHUNK = 1024;
+__global__ void __launch_bounds__(256) k_cc_corr(int nl, const double* __restrict__ D,
+                                                 const double* __restrict__ S, double* __restrict__ bval,
+                                                 int* __restrict__ bidx) {
   __shared__ double sd[CC_CHUNK], ss[CC_CHUNK + CC_LAGS];
   const int t = blockIdx.y, nblk = gridDim.x;
   const double* d = D + (size_t)t * nl;
   const double* s = S + (size_t)t * nl;
-  const int m0 = blockIdx.x * CC_LAGS, m = m0 + (int)threadIdx.x;
+  const int m0 = blockIdx.x * CC_LAGS, mb = m0 + (int)threadIdx.x * CC_LPL;
   const int nlag = 2 * nl - 1;
-  // lag m reads s[i1 + nl - 1 - m] for i1 in [max(0, m - nl + 1), min(nl - 1, m)]
-  const int lo = m - (nl - 1) > 0 ? m - (nl - 1) : 0, hi = m < nl - 1 ? m : nl - 1;
+  // lag m reads s[i1 + nl - 1 - m] for i1 in [max(0, m - nl + 1), min(nl - 1, m)]; union over mb..mb+3
+  const int lo = mb - (nl - 1) > 0 ? mb - (nl - 1) : 0, hi = mb + CC_LPL - 1 < nl - 1 ? mb + CC_LPL - 1 : nl - 1;
   const int blo = m0 - (nl - 1) > 0 ? m0 - (nl - 1) : 0;
   const int mlast = min(m0 + CC_LAGS - 1, nlag - 1);
   const int bhi = mlast < nl - 1 ? mlast : nl - 1;
-  double c = 0.0;
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
   for (int i0 = blo; i0 <= bhi; i0 += CC_CHUNK) {
     const int i1end = min(i0 + CC_CHUNK - 1, bhi);
-    // s indices of this chunk over the block's lags: [i0 + nl - 1 - (m0 + 255), i1end + nl - 1 - m0]
+    // s indices of this chunk over the block's lags: [i0 + nl - 1 - (m0 + CC_LAGS - 1), i1end + nl - 1 - m0]
     const int sbase = i0 + nl - 1 - (m0 + CC_LAGS - 1);
     __syncthreads();
-    for (int k = threadIdx.x; k < CC_CHUNK; k += CC_LAGS) sd[k] = (i0 + k <= i1end) ? d[i0 + k] : 0.0;
-    for (int k = threadIdx.x; k < CC_CHUNK + CC_LAGS; k += CC_LAGS) {
+    for (int k = threadIdx.x; k < CC_CHUNK; k += 256) sd[k] = (i0 + k <= i1end) ? d[i0 + k] : 0.0;
+    for (int k = threadIdx.x; k < CC_CHUNK + CC_LAGS; k += 256) {
       const int j = sbase + k;
       ss[k] = (j >= 0 && j < nl) ? s[j] : 0.0;
     }
     __syncthreads();
-    if (m < nlag) {
+    if (mb < nlag) {
       const int a = max(lo, i0), b = min(hi, i1end);
-      // s[i1 + nl - 1 - m] sits at ss[i1 - i0 + (m0 + 255 - m)]
-      const int off = m0 + CC_LAGS - 1 - m - i0;
-      for (int i1 = a; i1 <= b; ++i1) c = c + sd[i1 - i0] * ss[i1 + off];
+      // s[i1 + nl - 1 - (mb + q)] sits at ss[i1 + off - q], off in [3, CC_LAGS - 1] - i0
+      const int off = m0 + CC_LAGS - 1 - mb - i0;
+      if (a <= b) {
+        double r0 = ss[a + off], r1 = ss[a + off - 1], r2 = ss[a + off - 2], r3 = ss[a + off - 3];
+        for (int i1 = a; i1 <= b; ++i1) {
+          const double dv = sd[i1 - i0];
+          c0 = c0 + dv * r0;
+          c1 = c1 + dv * r1;
+          c2 = c2 + dv * r2;
+          c3 = c3 + dv * r3;
+          r3 = r2;
+          r2 = r1;
+          r1 = r0;
+          r0 = ss[i1 + 1 + off];
+        }
+      }
     }
   }
-  __shared__ double rv[CC_LAGS];
-  __shared__ int ri[CC_LAGS];
-  rv[threadIdx.x] = m < nlag ? c : -INFINITY;
-  ri[threadIdx.x] = m < nlag ? m : 0x7fffffff;
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  // the lane's first maximum over its lags (in increasing lag), then over lanes
+  double best = -INFINITY;
+  int bm = 0x7fffffff;
+  const double cv[CC_LPL] = {c0, c1, c2, c3};
+  for (int q = 0; q < CC_LPL; ++q)
+    if (mb + q < nlag && cv[q] > best) { best = cv[q]; bm = mb + q; }
+  rv[threadIdx.x] = best;
+  ri[threadIdx.x] = bm;
   __syncthreads();
-  for (int o = CC_LAGS / 2; o > 0; o >>= 1) {
+  for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       const double v2 = rv[threadIdx.x + o];
       const int m2 = ri[threadIdx.x + o];
@@ -1052,25 +1075,43 @@ __global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
                                                              const double* __restrict__ Sn, double* __restrict__ ts,
                                                              double* __restrict__ cc) {
   extern __shared__ double wsh[];
-  const int nwv = blockDim.x / 64;  // one window start per wavefront, 2 nlen doubles of LDS each
+  const int nwv = blockDim.x / 64;  // one window start per wavefront, 2 nlen + 7 doubles of LDS each
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, t = blockIdx.y;
   const int i = blockIdx.x * nwv + wave;
-  double* dw = wsh + (size_t)wave * 2 * nlen;
-  double* sw = dw + nlen;
+  double* dw = wsh + (size_t)wave * (2 * nlen + 7);
+  double* sw = dw + nlen + 3;  // s with three zeros before and four after (indices -3 .. nlen + 3)
   if (i < nw) {
     const double* d = Dn + (size_t)t * nstep + i;
     const double* s = Sn + (size_t)t * nstep + i;
     for (int j = lane; j < nlen; j += 64) { dw[j] = d[j] * wt[j]; sw[j] = s[j] * wt[j]; }
+    if (lane < 3) sw[-1 - lane] = 0.0;
+    if (lane < 4) sw[nlen + lane] = 0.0;
   }
   __syncthreads();
   if (i >= nw) return;
+  // four consecutive lags per lane with a sliding register window over s (as k_cc_corr)
   double best = -INFINITY;
   int bm = 0x7fffffff;
-  for (int m = lane; m < 2 * nlen - 1; m += 64) {
-    const int lo = m - (nlen - 1) > 0 ? m - (nlen - 1) : 0, hi = m < nlen - 1 ? m : nlen - 1;
-    double c = 0.0;
-    for (int i1 = lo; i1 <= hi; ++i1) c = c + dw[i1] * sw[i1 + nlen - 1 - m];
-    if (c > best) { best = c; bm = m; }
+  const int nlag = 2 * nlen - 1;
+  for (int mb = 4 * lane; mb < nlag; mb += 4 * 64) {
+    const int lo = mb - (nlen - 1) > 0 ? mb - (nlen - 1) : 0, hi = mb + 3 < nlen - 1 ? mb + 3 : nlen - 1;
+    const int off = nlen - 1 - mb;  // s index of lag mb + q at i1: i1 + off - q, in [-3, nlen + 3]
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    double r0 = sw[lo + off], r1 = sw[lo + off - 1], r2 = sw[lo + off - 2], r3 = sw[lo + off - 3];
+    for (int i1 = lo; i1 <= hi; ++i1) {
+      const double dv = dw[i1];
+      c0 = c0 + dv * r0;
+      c1 = c1 + dv * r1;
+      c2 = c2 + dv * r2;
+      c3 = c3 + dv * r3;
+      r3 = r2;
+      r2 = r1;
+      r1 = r0;
+      r0 = sw[i1 + 1 + off];
+    }
+    const double cv[4] = {c0, c1, c2, c3};
+    for (int q = 0; q < 4; ++q)
+      if (mb + q < nlag && cv[q] > best) { best = cv[q]; bm = mb + q; }
   }
   for (int o = 32; o > 0; o >>= 1) {
     const double v2 = __shfl_xor(best, o);
@@ -1079,21 +1120,25 @@ __global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
   }
   if (lane != 0) return;
   const int ish = bm + 1 - nlen;
+  // the three sums of xcorr_shift_coe, each in the reference's order, in one
+  // pass whose LDS reads are issued eight at a time
   double a = 0.0, b = 0.0, c = 0.0;
-  for (int j = 0; j < nlen; ++j) {
-    const int k = j - ish;
-    const double sv = (k >= 0 && k < nlen) ? sw[k] : 0.0;
-    const double dv = (ish > 0 && j < ish) || (ish < 0 && j >= nlen + ish) ? 0.0 : dw[j];
-    a = a + dv * sv;
-  }
-  for (int j = 0; j < nlen; ++j) {
-    const double dv = (ish > 0 && j < ish) || (ish < 0 && j >= nlen + ish) ? 0.0 : dw[j];
-    b = b + dv * dv;
-  }
-  for (int j = 0; j < nlen; ++j) {
-    const int k = j - ish;
-    const double sv = (k >= 0 && k < nlen) ? sw[k] : 0.0;
-    c = c + sv * sv;
+  constexpr int U = 8;
+  for (int j0 = 0; j0 < nlen; j0 += U) {
+    double dv[U], sv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u, k = j - ish;
+      sv[u] = (j < nlen && k >= 0 && k < nlen) ? sw[k] : 0.0;
+      dv[u] = (j >= nlen || (ish > 0 && j < ish) || (ish < 0 && j >= nlen + ish)) ? 0.0 : dw[j];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j0 + u >= nlen) break;
+      a = a + dv[u] * sv[u];
+      b = b + dv[u] * dv[u];
+      c = c + sv[u] * sv[u];
+    }
   }
   ts[(size_t)t * nw + i] = (double)ish * dt;
   cc[(size_t)t * nw + i] = a / sqrt(b * c);
@@ -1461,7 +1506,7 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
                          d_out + 8 * (size_t)w * T);
     } else {
       const int nblk = (2 * nls[w] - 1 + CC_LAGS - 1) / CC_LAGS;
-      hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(CC_LAGS), 0, s, nls[w], d_D, d_S, d_bv,
+      hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(256), 0, s, nls[w], d_D, d_S, d_bv,
                          d_bi);
       hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nls[w], d_bv, d_bi, d_ish);
       hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nls[w], *dt, *imeas, C, d_tw, d_D, d_S,
@@ -1604,7 +1649,7 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     const dim3 tiles((unsigned)((T + 63) / 64), (unsigned)((nl + 63) / 64));
     hipLaunchKernelGGL(k_cc_rows, tiles, dim3(256), 0, s, T, nbs[w], nl, d_tw, d_obs, d_syn, d_D, d_S);
     const int nblk = (2 * nl - 1 + CC_LAGS - 1) / CC_LAGS;
-    hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(CC_LAGS), 0, s, nl, d_D, d_S, d_bv, d_bi);
+    hipLaunchKernelGGL(k_cc_corr, dim3((unsigned)nblk, (unsigned)T), dim3(256), 0, s, nl, d_D, d_S, d_bv, d_bi);
     hipLaunchKernelGGL(k_cc_pick, dim3(nbk), dim3(MTPB), 0, s, T, nblk, nl, d_bv, d_bi, d_ish);
     hipLaunchKernelGGL(k_cc_finish, dim3((unsigned)T), dim3(256), 0, s, nl, *dt, imcc, C, d_tw, d_D, d_S, d_ish, d_A,
                        ow, 10);
@@ -1782,7 +1827,7 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   if (cfg[9] != 0.0) SEM_FAIL("fwat_win_sel_batch_: phase splitting (is_split_phases) is not supported");
   if (*maxwin < 1) SEM_FAIL("fwat_win_sel_batch_: maxwin must be positive");
   const int nlen = (int)(cfg[0] * *min_period / *dt) + 1;
-  if (nlen < 2 || nlen > 4096) SEM_FAIL("fwat_win_sel_batch_: sliding window of %d samples (2..4096)", nlen);
+  if (nlen < 2 || nlen > 4090) SEM_FAIL("fwat_win_sel_batch_: sliding window of %d samples (2..4090)", nlen);
   const int nw = N - nlen;
   const WsCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6], cfg[7], (int)cfg[8]};
   if (nw <= 0) {
@@ -1817,9 +1862,10 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_syn, d_synT);
   hipLaunchKernelGGL(k_ws_prep, dim3((unsigned)T), dim3(256), 0, s, N, *dt, *t0, *min_period, d_tp, d_obsT, d_synT,
                      d_Dn, d_Sn, d_noise);
-  const int nwv = nlen <= 1024 ? 4 : (nlen <= 2048 ? 2 : 1);  // 64 KB of LDS per workgroup
+  const int nwv = nlen <= 1020 ? 4 : (nlen <= 2044 ? 2 : 1);  // at most 64 KB of LDS per workgroup
   hipLaunchKernelGGL(k_ws_sliding, dim3((unsigned)((nw + nwv - 1) / nwv), (unsigned)T), dim3(64 * nwv),
-                     (size_t)nwv * 2 * nlen * sizeof(double), s, N, nlen, nw, *dt, d_wt, d_Dn, d_Sn, d_ts, d_cc);
+                     (size_t)nwv * (2 * nlen + 7) * sizeof(double), s, N, nlen, nw, *dt, d_wt, d_Dn, d_Sn, d_ts,
+                     d_cc);
   hipLaunchKernelGGL(k_ws_select, dim3((unsigned)((T + 63) / 64)), dim3(64), 0, s, T, N, nlen, nw, *dt, *t0,
                      *min_period, C, d_tp, d_dist, d_noise, d_Dn, d_Sn, d_ts, d_cc, *maxwin, d_nwin, d_win);
   HIP_CHECK(hipGetLastError());
