@@ -1144,87 +1144,103 @@ __global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
   cc[(size_t)t * nw + i] = a / sqrt(b * c);
 }
 
-// select_windows (win_sel.f90:102-277) without phase splitting, one lane per
-// trace: good samples (cc, shift, time range), samples within
-// nint(jump / dt) of a time-shift jump dropped, contiguous groups, and the
-// length, peak-count, SNR and energy-ratio checks (one pass over each
-// group's samples, sums in the reference's order).  Windows in seconds,
-// win[t][k][2], k < maxwin; nwin[t] the count found.
-__global__ void k_ws_select(int ntrace, int nstep, int nlen, int nw, double dt, double t0, double min_period,
-                            WsCfg cfg, const double* __restrict__ tp, const double* __restrict__ dist,
-                            const double* __restrict__ noise, const double* __restrict__ Dn,
-                            const double* __restrict__ Sn, const double* __restrict__ ts,
-                            const double* __restrict__ cc, int maxwin, int* __restrict__ nwin,
-                            double* __restrict__ win) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntrace) return;
+// select_windows (win_sel.f90:102-277) without phase splitting, one workgroup
+// per trace: the good-sample mask (cc, shift, time range, and no time-shift
+// jump within nint(jump / dt)) over the lanes into G (one byte per sliding
+// window); lane 0 collects the contiguous groups (at most WS_MAXG, the
+// reference's max_good_win); one lane per group runs the length, peak-count,
+// SNR and energy-ratio checks (sums in the reference's order); lane 0 writes
+// the accepted windows in group order (seconds, win[t][k][2], k < maxwin) and
+// their count to nwin[t] (-1 when the groups overflow WS_MAXG).
+constexpr int WS_MAXG = 1000;
+__global__ void __launch_bounds__(256) k_ws_select(int nstep, int nlen, int nw, double dt, double t0,
+                                                   double min_period, WsCfg cfg, const double* __restrict__ tp,
+                                                   const double* __restrict__ dist, const double* __restrict__ noise,
+                                                   const double* __restrict__ Dn, const double* __restrict__ Sn,
+                                                   const double* __restrict__ ts, const double* __restrict__ cc,
+                                                   unsigned char* __restrict__ G, int maxwin,
+                                                   int* __restrict__ nwin, double* __restrict__ win) {
+  __shared__ int ga[WS_MAXG], gb[WS_MAXG];
+  __shared__ unsigned char ok[WS_MAXG];
+  __shared__ int ng;
+  const int t = blockIdx.x;
   const double tstart = tp[t] - min_period * 1.5 + t0;
   const double tend = dist[t] / cfg.min_velocity + min_period * 1.5 + t0;
   const double jump = cfg.jump_fac * min_period;
   const int nj = (int)round(jump / dt);
   const double* tsr = ts + (size_t)t * nw;
   const double* ccr = cc + (size_t)t * nw;
+  unsigned char* g = G + (size_t)t * nw;
+  auto tcc = [&](int i) { return ((double)i + (double)nlen / 2.0) * dt; };
+  auto is_jump = [&](int i) { return i > 0 && i < nw && fabs(tsr[i] - tsr[i - 1]) > jump; };
+  for (int i = threadIdx.x; i < nw; i += 256) {
+    const double tc = tcc(i);
+    bool good = ccr[i] > cfg.thr_corr && fabs(tsr[i]) < cfg.thr_shift_fac * min_period && tc >= tstart && tc <= tend;
+    for (int k = max(1, i - nj); good && k <= min(nw - 1, i + nj); ++k)
+      if (is_jump(k)) good = false;
+    g[i] = good ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0, start = -1;
+    for (int i = 0; i <= nw; ++i) {
+      const bool gd = i < nw && g[i];
+      if (gd && start < 0) start = i;
+      else if (!gd && start >= 0) {
+        if (n < WS_MAXG) { ga[n] = start; gb[n] = i - 1; }
+        ++n;
+        start = -1;
+      }
+    }
+    ng = n;
+  }
+  __syncthreads();
+  const int n = min(ng, WS_MAXG);
   const double* dat = Dn + (size_t)t * nstep;
   const double* syn = Sn + (size_t)t * nstep;
-  auto tcc = [&](int i) { return ((double)i + (double)nlen / 2.0) * dt; };
-  auto raw_good = [&](int i) {
-    const double tc = tcc(i);
-    return ccr[i] > cfg.thr_corr && fabs(tsr[i]) < cfg.thr_shift_fac * min_period && tc >= tstart && tc <= tend;
-  };
-  auto is_jump = [&](int i) { return i > 0 && fabs(tsr[i] - tsr[i - 1]) > jump; };
-  // a sample is dropped when a jump lies within nj of it; keep the last jump index seen
-  int last_jump = -(1 << 29);
-  int next_jump = -1;  // first jump index >= i, found lazily
-  auto good = [&](int i) {
-    if (is_jump(i)) last_jump = i;
-    if (next_jump < i) {
-      next_jump = i;
-      while (next_jump < nw && !is_jump(next_jump)) ++next_jump;
-    }
-    if (i - last_jump <= nj) return false;
-    if (next_jump < nw && next_jump - i <= nj) return false;
-    return raw_good(i);
-  };
-  int count = 0;
-  auto check = [&](int a, int b) {
-    if ((b - a) * dt < cfg.min_win_len_fac * min_period) return;
+  for (int q = threadIdx.x; q < n; q += 256) {
+    const int a = ga[q], b = gb[q];
+    bool pass = true;
+    if ((b - a) * dt < cfg.min_win_len_fac * min_period) pass = false;
     int ib = (int)(tcc(a) / dt) + 1, ie = (int)(tcc(b) / dt) + 1;
     ib = max(1, min(ib, nstep));
     ie = max(1, min(ie, nstep));
-    if (ie <= ib) return;
-    const double* dp = dat + ib - 1;
-    const double* sp = syn + ib - 1;
-    const int m = ie - ib + 1;
-    int pd = 0, td = 0, ps = 0, tq = 0;
-    double amax = 0.0, ed = 0.0, es = 0.0;
-    for (int k = 0; k < m; ++k) {
-      if (k >= 1 && k < m - 1) {
-        if (dp[k] - dp[k - 1] > 0.0 && dp[k + 1] - dp[k] < 0.0) ++pd;
-        if (-dp[k] - -dp[k - 1] > 0.0 && -dp[k + 1] - -dp[k] < 0.0) ++td;
-        if (sp[k] - sp[k - 1] > 0.0 && sp[k + 1] - sp[k] < 0.0) ++ps;
-        if (-sp[k] - -sp[k - 1] > 0.0 && -sp[k + 1] - -sp[k] < 0.0) ++tq;
+    if (ie <= ib) pass = false;
+    if (pass) {
+      const double* dp = dat + ib - 1;
+      const double* sp = syn + ib - 1;
+      const int m = ie - ib + 1;
+      int pd = 0, td = 0, ps = 0, tq = 0;
+      double amax = 0.0, ed = 0.0, es = 0.0;
+      for (int k = 0; k < m; ++k) {
+        if (k >= 1 && k < m - 1) {
+          if (dp[k] - dp[k - 1] > 0.0 && dp[k + 1] - dp[k] < 0.0) ++pd;
+          if (-dp[k] - -dp[k - 1] > 0.0 && -dp[k + 1] - -dp[k] < 0.0) ++td;
+          if (sp[k] - sp[k - 1] > 0.0 && sp[k + 1] - sp[k] < 0.0) ++ps;
+          if (-sp[k] - -sp[k - 1] > 0.0 && -sp[k + 1] - -sp[k] < 0.0) ++tq;
+        }
+        amax = fmax(amax, fabs(dp[k]));
+        ed = ed + dp[k] * dp[k];
+        es = es + sp[k] * sp[k];
       }
-      amax = fmax(amax, fabs(dp[k]));
-      ed = ed + dp[k] * dp[k];
-      es = es + sp[k] * sp[k];
+      if (min(pd + td, ps + tq) < cfg.min_peaks) pass = false;
+      else if (amax / noise[t] < cfg.min_snr) pass = false;
+      else if (ed / es > cfg.min_energy_ratio || ed / es < 1 / cfg.min_energy_ratio) pass = false;
     }
-    if (min(pd + td, ps + tq) < cfg.min_peaks) return;
-    if (amax / noise[t] < cfg.min_snr) return;
-    if (ed / es > cfg.min_energy_ratio || ed / es < 1 / cfg.min_energy_ratio) return;
+    ok[q] = pass ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int count = 0;
+  for (int q = 0; q < n; ++q) {
+    if (!ok[q]) continue;
     if (count < maxwin) {
-      win[((size_t)t * maxwin + count) * 2] = tcc(a);
-      win[((size_t)t * maxwin + count) * 2 + 1] = tcc(b);
+      win[((size_t)t * maxwin + count) * 2] = tcc(ga[q]);
+      win[((size_t)t * maxwin + count) * 2 + 1] = tcc(gb[q]);
     }
     ++count;
-  };
-  int start = -1;
-  for (int i = 0; i < nw; ++i) {
-    const bool g = good(i);
-    if (g && start < 0) start = i;
-    else if (!g && start >= 0) { check(start, i - 1); start = -1; }
   }
-  if (start >= 0) check(start, nw - 1);
-  nwin[t] = count;
+  nwin[t] = ng > WS_MAXG ? -1 : count;
 }
 
 // selected traces' adjoint-source rows added into the sample-major traces
@@ -1856,6 +1872,7 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   double* d_cc = dev_alloc<double>((size_t)nw * T);
   int* d_nwin = dev_alloc<int>(T);
   double* d_win = dev_alloc<double>(2 * (size_t)*maxwin * T);
+  unsigned char* d_good = dev_alloc<unsigned char>((size_t)nw * T);
   HIP_CHECK(hipEventRecord(ev0, s));
   const dim3 ttiles((unsigned)((T + 63) / 64), (unsigned)((N + 63) / 64));
   hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_obs, d_obsT);
@@ -1866,8 +1883,8 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   hipLaunchKernelGGL(k_ws_sliding, dim3((unsigned)((nw + nwv - 1) / nwv), (unsigned)T), dim3(64 * nwv),
                      (size_t)nwv * (2 * nlen + 7) * sizeof(double), s, N, nlen, nw, *dt, d_wt, d_Dn, d_Sn, d_ts,
                      d_cc);
-  hipLaunchKernelGGL(k_ws_select, dim3((unsigned)((T + 63) / 64)), dim3(64), 0, s, T, N, nlen, nw, *dt, *t0,
-                     *min_period, C, d_tp, d_dist, d_noise, d_Dn, d_Sn, d_ts, d_cc, *maxwin, d_nwin, d_win);
+  hipLaunchKernelGGL(k_ws_select, dim3((unsigned)T), dim3(256), 0, s, N, nlen, nw, *dt, *t0, *min_period, C, d_tp,
+                     d_dist, d_noise, d_Dn, d_Sn, d_ts, d_cc, d_good, *maxwin, d_nwin, d_win);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(ev1, s));
   HIP_CHECK(hipMemcpyAsync(nwin, d_nwin, T * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1878,7 +1895,9 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   g_meas_ms = ms;
   dev_free(d_obs); dev_free(d_syn); dev_free(d_tp); dev_free(d_dist); dev_free(d_wt); dev_free(d_obsT);
   dev_free(d_synT); dev_free(d_Dn); dev_free(d_Sn); dev_free(d_noise); dev_free(d_ts); dev_free(d_cc);
-  dev_free(d_nwin); dev_free(d_win);
+  dev_free(d_nwin); dev_free(d_win); dev_free(d_good);
+  for (int t = 0; t < T; ++t)
+    if (nwin[t] < 0) SEM_FAIL("fwat_win_sel_batch_: trace %d has more than %d good groups", t, WS_MAXG);
   HIP_CHECK(hipEventDestroy(ev0));
   HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));
