@@ -347,11 +347,12 @@ void fwat_ep_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, rea
                             double *windows, double *taper_percentage, int *itaper_type, double *wtr_env,
                             int *nsect, realw *sos, double *misfits, double *meas, realw *adj);
 /* window selection of every trace (WIN_SELECTOR_TYPE): win_sel_type%select_windows
- * (src/preproc/window_selector/win_sel.f90:35-277, called at src/preproc/leq_data.f90:108-114) without
- * phase splitting; obs, syn (NDIM,nrec,NSTEP) filtered as the caller does; tp(ntrace), dist(ntrace) the
+ * (src/preproc/window_selector/win_sel.f90:35-469 with sta_lta.f90:6-104, called at
+ * src/preproc/leq_data.f90:108-114); obs, syn (NDIM,nrec,NSTEP) filtered as the caller does; tp(ntrace), dist(ntrace) the
  * first-arrival time and distance of each trace's receiver; cfg(10) = sliding_win_len_fac, min_velocity,
  * threshold_corr, threshold_shift_fac, jump_fac, min_win_len_fac, min_snr_window, min_energy_ratio,
- * min_peaks_troughs, is_split_phases (must be 0); nwin(ntrace) out; windows(2,maxwin,ntrace) out (s) */
+ * min_peaks_troughs, is_split_phases (0/1; splitting takes nstep <= 16383); nwin(ntrace) out;
+ * windows(2,maxwin,ntrace) out (s) */
 void fwat_win_sel_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, double *t0, double *tp,
                          double *dist, double *min_period, double *cfg, int *maxwin, int *nwin, double *windows);
 void fwat_mt_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,

@@ -31,8 +31,8 @@
 //   orc_hilbert           hilbert (src/shared/fftpack.f90:135-201)
 //   orc_ep_adjoint        ExponentiatedPhaseMisfit%calc_adjoint_source
 //                         (exponentiated_phase_misfit.f90:21-104)
-//   orc_win_sel           win_sel_type%select_windows (window_selector/win_sel.f90:56-277,
-//                         439-469) without phase splitting
+//   orc_win_sel           win_sel_type%select_windows (window_selector/win_sel.f90:56-469,
+//                         sta_lta.f90:6-104)
 //   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
 //   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
 //                         where the reference calls LAPACK dsterf
@@ -41,6 +41,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <limits>
+#include <utility>
 #include <vector>
 
 extern "C" void orc_window_taper(double* sig, int npts, double pct, int itype);
@@ -239,6 +241,38 @@ void xcorr_shift_coe(const std::vector<double>& d, const std::vector<double>& s,
   for (int i = 0; i < n; ++i) b = b + ds[i] * ds[i];
   for (int i = 0; i < n; ++i) c = c + ss[i] * ss[i];
   coef = a / std::sqrt(b * c);
+}
+
+
+// STA_LTA (window_selector/sta_lta.f90:6-104, TWEAK off): Bai & Kennett
+// recursive averages of the envelope over a 1000-sample pre-extension at the
+// noise level max(env) / 1e5
+std::vector<double> sta_lta(const std::vector<double>& env, double dt, double min_period) {
+  const int n = (int)env.size(), next = 1000;
+  const double Cs = std::pow(10.0, -dt / min_period), Cl = std::pow(10.0, -dt / (12.0 * min_period));
+  const double tol = 1.0e-9;
+  double mx = env[0];
+  for (double v : env) mx = std::max(mx, v);
+  const double noise = mx / 1e5;
+  double sta = 0.0, lta = 0.0;
+  for (int i = 0; i < next; ++i) { sta = Cs * sta + noise; lta = Cl * lta + noise; }
+  std::vector<double> r(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const double x = env[i] + noise;
+    sta = Cs * sta + x;
+    lta = Cl * lta + x;
+    r[i] = lta > tol ? sta / lta : noise;
+  }
+  return r;
+}
+
+// find_maxima as index lists (1-based, as the reference)
+std::vector<int> maxima(const std::vector<double>& x, double sign) {
+  std::vector<int> r;
+  const int n = (int)x.size();
+  for (int i = 1; i < n - 1; ++i)
+    if (sign * x[i] - sign * x[i - 1] > 0.0 && sign * x[i + 1] - sign * x[i] < 0.0) r.push_back(i + 1);
+  return r;
 }
 
 // number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
@@ -786,13 +820,13 @@ double orc_ep_adjoint(const double* dat, const double* syn, int n, double dt, co
   return total;
 }
 
-// win_sel_type (src/preproc/window_selector/win_sel.f90:56-277) without phase
-// splitting: normalisation by max|syn|, sliding_cc (Hann-tapered windows of
+// win_sel_type (src/preproc/window_selector/win_sel.f90:56-437): normalisation by max|syn|, sliding_cc (Hann-tapered windows of
 // int(sliding_win_len_fac min_period / dt) + 1 samples at every start),
 // good samples (cc, shift, time range), jump exclusion, contiguous groups and
-// the length / peak count / SNR / energy-ratio checks.  cfg: sliding_win_len_fac,
+// the length / peak count / SNR / energy-ratio checks, with cfg[9] the groups
+// split at the STA/LTA troughs between phases first.  cfg: sliding_win_len_fac,
 // min_velocity, threshold_corr, threshold_shift_fac, jump_fac, min_win_len_fac,
-// min_snr_window, min_energy_ratio, min_peaks_troughs.  twin[maxwin][2] (s);
+// min_snr_window, min_energy_ratio, min_peaks_troughs, is_split_phases.  twin[maxwin][2] (s);
 // returns the window count.  cc_out / ts_out (npts - nlen) receive the sliding
 // correlation when not null.
 int orc_win_sel(const double* dat_in, const double* syn_in, int npts, double dt, double t0, double tp, double dis,
@@ -856,12 +890,64 @@ int orc_win_sel(const double* dat_in, const double* syn_in, int npts, double dt,
     if (count < maxwin) { twin[2 * count] = tc[a]; twin[2 * count + 1] = tc[b]; }
     ++count;
   };
+  // phase splitting (cfg[9]): envelope of the normalised synthetic, STA/LTA,
+  // its peaks and troughs; split_window_by_phases (win_sel.f90:279-437)
+  std::vector<double> stl;
+  std::vector<int> peaks, troughs;
+  if (cfg[9] != 0.0) {
+    const auto xa = hilbert(syn);
+    std::vector<double> env(npts);
+    for (int i = 0; i < npts; ++i) env[i] = std::abs(xa[i]);
+    stl = sta_lta(env, dt, min_period);
+    peaks = maxima(stl, 1.0);
+    troughs = maxima(stl, -1.0);
+  }
+  const int minlen = (int)(cfg[5] * min_period / dt);
+  auto split = [&](int a, int b) {  // 0-based group -> 0-based sub-windows
+    std::vector<std::pair<int, int>> sub;
+    const int ns = (int)stl.size();
+    int s0 = (int)(tc[a] / dt) + 1, s1 = (int)(tc[b] / dt) + 1;
+    s0 = std::max(1, std::min(s0, ns));
+    s1 = std::max(1, std::min(s1, ns));
+    if (s0 >= s1) return std::vector<std::pair<int, int>>{{a, b}};
+    std::vector<int> wp;
+    for (int p : peaks) if (p >= s0 && p <= s1) wp.push_back(p);
+    if (wp.size() <= 1) return std::vector<std::pair<int, int>>{{a, b}};
+    std::vector<int> sp{s0};
+    for (size_t i = 0; i + 1 < wp.size(); ++i) {
+      const int p1 = wp[i], p2 = wp[i + 1];
+      double mv = std::numeric_limits<double>::max();
+      int mi = -1;
+      for (int tr : troughs)
+        if (tr > p1 && tr < p2) {
+          if (mi < 0) mi = tr;
+          if (stl[tr - 1] < mv) { mv = stl[tr - 1]; mi = tr; }
+        }
+      sp.push_back(mi >= 0 ? mi : (p1 + p2) / 2);
+    }
+    sp.push_back(s1);
+    for (size_t i = 0; i + 1 < sp.size(); ++i) {
+      if (sp[i + 1] - sp[i] < minlen) continue;
+      int j0 = a, j1 = b;
+      for (int q = a; q <= b; ++q) {
+        if (std::fabs(tc[q] - sp[i] * dt) < std::fabs(tc[j0] - sp[i] * dt)) j0 = q;
+        if (std::fabs(tc[q] - sp[i + 1] * dt) < std::fabs(tc[j1] - sp[i + 1] * dt)) j1 = q;
+      }
+      sub.push_back({std::max(a, std::min(j0, b)), std::max(a, std::min(j1, b))});
+    }
+    if (sub.empty()) sub.push_back({a, b});
+    return sub;
+  };
+  auto group = [&](int a, int b) {
+    if (cfg[9] == 0.0) { check(a, b); return; }
+    for (const auto& ab : split(a, b)) check(ab.first, ab.second);
+  };
   int start = -1;
   for (int i = 0; i < nw; ++i) {
     if (good[i] && start < 0) start = i;
-    else if (!good[i] && start >= 0) { check(start, i - 1); start = -1; }
+    else if (!good[i] && start >= 0) { group(start, i - 1); start = -1; }
   }
-  if (start >= 0) check(start, nw - 1);
+  if (start >= 0) group(start, nw - 1);
   return count;
 }
 

@@ -633,14 +633,14 @@ def ep_adjoint(dat, syn, dt, windows, taper_pct=0.05, itaper=1, wtr_env=0.2):
 
 WIN_DEFAULTS = dict(sliding_win_len_fac=2.0, min_velocity=2.4, threshold_corr=0.8, threshold_shift_fac=0.3,
                     jump_fac=0.1, min_win_len_fac=1.5, min_snr_window=5.0, min_energy_ratio=5.0,
-                    min_peaks_troughs=3)  # input_params.f90:389-397
+                    min_peaks_troughs=3, is_split_phases=0.0)  # input_params.f90:389-398
 
 
 def win_sel(dat, syn, dt, t0, tp, dis, min_period, maxwin=1000, **cfg):
-    """win_sel_type%select_windows without phase splitting: (windows [n][2] in s,
-    sliding cc, sliding time shifts)."""
+    """win_sel_type%select_windows (is_split_phases: split at STA/LTA troughs):
+    (windows [n][2] in s, sliding cc, sliding time shifts)."""
     c = dict(WIN_DEFAULTS, **cfg)
-    cv = np.array([c[k] for k in WIN_DEFAULTS], np.float64)
+    cv = np.array([float(c[k]) for k in WIN_DEFAULTS], np.float64)
     dat = np.ascontiguousarray(dat, np.float64)
     syn = np.ascontiguousarray(syn, np.float64)
     n = len(dat)

@@ -1029,7 +1029,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_ep_window(int nl, double dt, int
 
 struct WsCfg {
   double win_len_fac, min_velocity, thr_corr, thr_shift_fac, jump_fac, min_win_len_fac, min_snr, min_energy_ratio;
-  int min_peaks;
+  int min_peaks, split;
 };
 
 // per trace (one workgroup): max|syn| normalisation of the trace-major rows
@@ -1144,6 +1144,48 @@ __global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
   cc[(size_t)t * nw + i] = a / sqrt(b * c);
 }
 
+// envelope |hilbert(syn)| of every trace's normalised synthetic for the phase
+// splitting (win_sel.f90:121-133): one workgroup per trace, the analytic signal
+// on nf = 2 ** exponent(real(nstep)) points through the LDS real transform (as
+// k_ep_window), Re and Im as real-output inverses, env = |xa|.  G0, G1: M + 1
+// double2 per trace
+__global__ void __launch_bounds__(MT_THREADS) k_ws_env(int nstep, int nf, int logM, const double2* __restrict__ tw,
+                                                       const double* __restrict__ Sn, double2* __restrict__ G0,
+                                                       double2* __restrict__ G1, double* __restrict__ env) {
+  __shared__ double2 buf[MT_MAXM];
+  const int t = blockIdx.x, M = nf / 2, lane = threadIdx.x;
+  const unsigned shb = 32 - logM;
+  const double* x = Sn + (size_t)t * nstep;
+  double* e = env + (size_t)t * nstep;
+  double2* g0 = G0 + (size_t)t * (M + 1);
+  double2* g1 = G1 + (size_t)t * (M + 1);
+  const double hnyq = (nstep % 2 == 0) ? 1.0 : 0.0;
+  mt_load_real<MT_THREADS>(buf, M, logM, nstep, [&](int n) { return x[n]; });
+  mt_fft<MT_THREADS>(buf, M, logM, nf, tw, false);
+  for (int k = lane; k <= M; k += MT_THREADS) {
+    double2 X;
+    if (k == M) { const double2 z = buf[0]; X = make_double2(z.x - z.y, 0.0); }
+    else if (k == 0) { const double2 z = buf[0]; X = make_double2(z.x + z.y, 0.0); }
+    else X = mt_half(buf[k], buf[M - k], tw[k]);
+    g0[k] = k == 0 ? make_double2(X.x, 0.0) : (k == M ? make_double2(hnyq * X.x, 0.0) : X);
+    g1[k] = (k == 0 || k == M) ? make_double2(0.0, 0.0) : make_double2(X.y, -X.x);
+  }
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    const double2* g = pass == 0 ? g0 : g1;
+    for (int k = lane; k < M; k += MT_THREADS)
+      buf[__brev((unsigned)k) >> shb] = mt_repack(g[k], g[M - k], tw[k]);
+    __syncthreads();
+    mt_fft<MT_THREADS>(buf, M, logM, nf, tw, true);
+    for (int n = lane; n < nstep; n += MT_THREADS) {
+      const double2 z = buf[n >> 1];
+      const double y = ((n & 1) ? z.y : z.x) / M;
+      e[n] = pass == 0 ? y : hypot(e[n], y);
+    }
+    __syncthreads();
+  }
+}
+
 // select_windows (win_sel.f90:102-277) without phase splitting, one workgroup
 // per trace: the good-sample mask (cc, shift, time range, and no time-shift
 // jump within nint(jump / dt)) over the lanes into G (one byte per sliding
@@ -1158,8 +1200,9 @@ __global__ void __launch_bounds__(256) k_ws_select(int nstep, int nlen, int nw, 
                                                    const double* __restrict__ dist, const double* __restrict__ noise,
                                                    const double* __restrict__ Dn, const double* __restrict__ Sn,
                                                    const double* __restrict__ ts, const double* __restrict__ cc,
-                                                   unsigned char* __restrict__ G, int maxwin,
-                                                   int* __restrict__ nwin, double* __restrict__ win) {
+                                                   unsigned char* __restrict__ G, const double* __restrict__ env,
+                                                   double* __restrict__ stl, int maxwin, int* __restrict__ nwin,
+                                                   double* __restrict__ win) {
   __shared__ int ga[WS_MAXG], gb[WS_MAXG];
   __shared__ unsigned char ok[WS_MAXG];
   __shared__ int ng;
@@ -1198,8 +1241,7 @@ __global__ void __launch_bounds__(256) k_ws_select(int nstep, int nlen, int nw, 
   const int n = min(ng, WS_MAXG);
   const double* dat = Dn + (size_t)t * nstep;
   const double* syn = Sn + (size_t)t * nstep;
-  for (int q = threadIdx.x; q < n; q += 256) {
-    const int a = ga[q], b = gb[q];
+  auto check = [&](int a, int b) {
     bool pass = true;
     if ((b - a) * dt < cfg.min_win_len_fac * min_period) pass = false;
     int ib = (int)(tcc(a) / dt) + 1, ie = (int)(tcc(b) / dt) + 1;
@@ -1227,8 +1269,97 @@ __global__ void __launch_bounds__(256) k_ws_select(int nstep, int nlen, int nw, 
       else if (amax / noise[t] < cfg.min_snr) pass = false;
       else if (ed / es > cfg.min_energy_ratio || ed / es < 1 / cfg.min_energy_ratio) pass = false;
     }
-    ok[q] = pass ? 1 : 0;
+    return pass;
+  };
+  if (cfg.split) {
+    // phase splitting: lane 0 runs STA_LTA (sta_lta.f90:6-104) over the envelope,
+    // then split_window_by_phases (win_sel.f90:279-437) and the checks group by group
+    __shared__ double rm[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nstep; i += 256) m = fmax(m, env[(size_t)t * nstep + i]);
+    rm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double emax = rm[0];
+    const double* e = env + (size_t)t * nstep;
+    double* st = stl + (size_t)t * nstep;
+    const double Cs = pow(10.0, -dt / min_period), Cl = pow(10.0, -dt / (12.0 * min_period));
+    const double nz = emax / 1e5;
+    double sta = 0.0, lta = 0.0;
+    for (int i = 0; i < 1000; ++i) { sta = Cs * sta + nz; lta = Cl * lta + nz; }
+    for (int i = 0; i < nstep; ++i) {
+      const double x = e[i] + nz;
+      sta = Cs * sta + x;
+      lta = Cl * lta + x;
+      st[i] = lta > 1.0e-9 ? sta / lta : nz;
+    }
+    auto peak = [&](int i, double sg) {  // 1-based sample i is a maximum of sg * stalta
+      return i >= 2 && i <= nstep - 1 && sg * st[i - 1] - sg * st[i - 2] > 0.0 && sg * st[i] - sg * st[i - 1] < 0.0;
+    };
+    const int minlen = (int)(cfg.min_win_len_fac * min_period / dt);
+    int count = 0;
+    auto emit = [&](int a, int b) {
+      if (!check(a, b)) return;
+      if (count < maxwin) {
+        win[((size_t)t * maxwin + count) * 2] = tcc(a);
+        win[((size_t)t * maxwin + count) * 2 + 1] = tcc(b);
+      }
+      ++count;
+    };
+    for (int q = 0; q < n; ++q) {
+      const int a = ga[q], b = gb[q];
+      int s0 = (int)(tcc(a) / dt) + 1, s1 = (int)(tcc(b) / dt) + 1;
+      s0 = max(1, min(s0, nstep));
+      s1 = max(1, min(s1, nstep));
+      int npk = 0;
+      if (s0 < s1)
+        for (int i = s0; i <= s1; ++i) npk += peak(i, 1.0);
+      if (s0 >= s1 || npk <= 1) { emit(a, b); continue; }
+      // split points: s0, the lowest trough between consecutive peaks (midpoint if none), s1
+      int prev = s0, p1 = -1, nsub = 0;
+      auto sub = [&](int x0, int x1, bool go) {
+        if (x1 - x0 < minlen) return;
+        ++nsub;
+        if (!go) return;
+        int j0 = a, j1 = b;
+        for (int r = a; r <= b; ++r) {
+          if (fabs(tcc(r) - x0 * dt) < fabs(tcc(j0) - x0 * dt)) j0 = r;
+          if (fabs(tcc(r) - x1 * dt) < fabs(tcc(j1) - x1 * dt)) j1 = r;
+        }
+        emit(max(a, min(j0, b)), max(a, min(j1, b)));
+      };
+      // two passes: count the valid sub-windows, then emit them (or the group when none)
+      for (int go = 0; go < 2; ++go) {
+        prev = s0;
+        p1 = -1;
+        if (go == 1 && nsub == 0) { emit(a, b); break; }
+        for (int i = s0; i <= s1; ++i) {
+          if (!peak(i, 1.0)) continue;
+          if (p1 >= 0) {
+            int mi = -1;
+            double mv = 1.79769313486231570815e308;
+            for (int k = p1 + 1; k < i; ++k)
+              if (peak(k, -1.0)) {
+                if (mi < 0) mi = k;
+                if (st[k - 1] < mv) { mv = st[k - 1]; mi = k; }
+              }
+            const int spt = mi >= 0 ? mi : (p1 + i) / 2;
+            sub(prev, spt, go == 1);
+            prev = spt;
+          }
+          p1 = i;
+        }
+        sub(prev, s1, go == 1);
+      }
+    }
+    nwin[t] = ng > WS_MAXG ? -1 : count;
+    return;
   }
+  for (int q = threadIdx.x; q < n; q += 256) ok[q] = check(ga[q], gb[q]) ? 1 : 0;
   __syncthreads();
   if (threadIdx.x != 0) return;
   int count = 0;
@@ -1832,20 +1963,22 @@ void fwat_ep_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
 // before selection), tp(ntrace) first-arrival times, dist(ntrace) epicentral
 // distances, t0 the simulation start offset; cfg(10) = sliding_win_len_fac,
 // min_velocity, threshold_corr, threshold_shift_fac, jump_fac, min_win_len_fac,
-// min_snr_window, min_energy_ratio, min_peaks_troughs, is_split_phases (must
-// be 0: phase splitting is not built); nwin(ntrace) out (windows found, may
-// exceed maxwin), windows(2, maxwin, ntrace) out in seconds
+// min_snr_window, min_energy_ratio, min_peaks_troughs, is_split_phases (groups
+// split at the STA/LTA troughs between phases; traces of at most 16383 samples);
+// nwin(ntrace) out (windows found, may exceed maxwin), windows(2, maxwin,
+// ntrace) out in seconds
 void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw* syn, double* t0, double* tp,
                          double* dist, double* min_period, double* cfg, int* maxwin, int* nwin, double* windows) {
   const int N = *nstep, T = *ntrace;
   if (N <= 0 || T <= 0) SEM_FAIL("fwat_win_sel_batch_: sizes must be positive");
   if (T > 65535) SEM_FAIL("fwat_win_sel_batch_: at most 65535 traces per call");
-  if (cfg[9] != 0.0) SEM_FAIL("fwat_win_sel_batch_: phase splitting (is_split_phases) is not supported");
+  const bool split = cfg[9] != 0.0;
+  if (split && N > 16383) SEM_FAIL("fwat_win_sel_batch_: phase splitting takes traces of at most 16383 samples");
   if (*maxwin < 1) SEM_FAIL("fwat_win_sel_batch_: maxwin must be positive");
   const int nlen = (int)(cfg[0] * *min_period / *dt) + 1;
   if (nlen < 2 || nlen > 4090) SEM_FAIL("fwat_win_sel_batch_: sliding window of %d samples (2..4090)", nlen);
   const int nw = N - nlen;
-  const WsCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6], cfg[7], (int)cfg[8]};
+  const WsCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6], cfg[7], (int)cfg[8], split ? 1 : 0};
   if (nw <= 0) {
     for (int t = 0; t < T; ++t) nwin[t] = 0;
     return;
@@ -1873,6 +2006,27 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   int* d_nwin = dev_alloc<int>(T);
   double* d_win = dev_alloc<double>(2 * (size_t)*maxwin * T);
   unsigned char* d_good = dev_alloc<unsigned char>((size_t)nw * T);
+  // phase splitting: envelope transform on nf = 2 ** exponent(real(nstep)) points
+  int nfe = 0, Me = 0, logMe = 0;
+  double *d_env = nullptr, *d_stl = nullptr;
+  double2 *d_g01 = nullptr, *d_twe = nullptr;
+  if (split) {
+    int e2 = 0;
+    std::frexp((float)N, &e2);
+    nfe = 1 << e2;
+    Me = nfe / 2;
+    while ((1 << logMe) < Me) ++logMe;
+    std::vector<double> twh(2 * (size_t)Me);
+    for (int j = 0; j < Me; ++j) {
+      const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)nfe;
+      twh[2 * j] = (double)std::cos(a);
+      twh[2 * j + 1] = (double)std::sin(a);
+    }
+    d_twe = reinterpret_cast<double2*>(dev_upload(twh.data(), twh.size(), s));
+    d_env = dev_alloc<double>(n);
+    d_stl = dev_alloc<double>(n);
+    d_g01 = dev_alloc<double2>(2 * (size_t)T * (Me + 1));
+  }
   HIP_CHECK(hipEventRecord(ev0, s));
   const dim3 ttiles((unsigned)((T + 63) / 64), (unsigned)((N + 63) / 64));
   hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_obs, d_obsT);
@@ -1883,8 +2037,11 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   hipLaunchKernelGGL(k_ws_sliding, dim3((unsigned)((nw + nwv - 1) / nwv), (unsigned)T), dim3(64 * nwv),
                      (size_t)nwv * (2 * nlen + 7) * sizeof(double), s, N, nlen, nw, *dt, d_wt, d_Dn, d_Sn, d_ts,
                      d_cc);
+  if (split)
+    hipLaunchKernelGGL(k_ws_env, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, nfe, logMe, d_twe, d_Sn, d_g01,
+                       d_g01 + (size_t)T * (Me + 1), d_env);
   hipLaunchKernelGGL(k_ws_select, dim3((unsigned)T), dim3(256), 0, s, N, nlen, nw, *dt, *t0, *min_period, C, d_tp,
-                     d_dist, d_noise, d_Dn, d_Sn, d_ts, d_cc, d_good, *maxwin, d_nwin, d_win);
+                     d_dist, d_noise, d_Dn, d_Sn, d_ts, d_cc, d_good, d_env, d_stl, *maxwin, d_nwin, d_win);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(ev1, s));
   HIP_CHECK(hipMemcpyAsync(nwin, d_nwin, T * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1895,7 +2052,8 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   g_meas_ms = ms;
   dev_free(d_obs); dev_free(d_syn); dev_free(d_tp); dev_free(d_dist); dev_free(d_wt); dev_free(d_obsT);
   dev_free(d_synT); dev_free(d_Dn); dev_free(d_Sn); dev_free(d_noise); dev_free(d_ts); dev_free(d_cc);
-  dev_free(d_nwin); dev_free(d_win); dev_free(d_good);
+  dev_free(d_nwin); dev_free(d_win); dev_free(d_good); dev_free(d_env); dev_free(d_stl); dev_free(d_g01);
+  dev_free(d_twe);
   for (int t = 0; t < T; ++t)
     if (nwin[t] < 0) SEM_FAIL("fwat_win_sel_batch_: trace %d has more than %d good groups", t, WS_MAXG);
   HIP_CHECK(hipEventDestroy(ev0));

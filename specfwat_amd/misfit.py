@@ -304,9 +304,10 @@ def select_windows(obs, syn, dt, t0, tp, dist, min_period, maxwin=64, **cfg):
     time-shift jump rule, contiguous groups and the length / peak / SNR / energy
     checks.  obs, syn (NSTEP, nrec, 3), already filtered as the caller does;
     tp (nrec,) first-arrival times and dist (nrec,) distances (the receiver's
-    three components share them), t0 the simulation start.  Returns a list per
-    trace (nrec*3, receiver-major) of [n, 2] window arrays in seconds.  Phase
-    splitting (is_split_phases) is not built."""
+    three components share them), t0 the simulation start.  With
+    is_split_phases the groups are split at the STA/LTA troughs between phases
+    (split_window_by_phases).  Returns a list per trace (nrec*3, receiver-major)
+    of [n, 2] window arrays in seconds."""
     o = np.ascontiguousarray(obs, np.float32)
     s = np.ascontiguousarray(syn, np.float32)
     if o.shape != s.shape:
@@ -315,8 +316,6 @@ def select_windows(obs, syn, dt, t0, tp, dist, min_period, maxwin=64, **cfg):
     if unknown:
         raise KeyError(f"unknown window settings {sorted(unknown)}")
     c = dict(WIN_DEFAULTS, **cfg)
-    if c["is_split_phases"]:
-        raise NotImplementedError("phase splitting of the window selector is not built")
     nstep = s.shape[0]
     ntrace = s.size // nstep
     ncomp = ntrace // np.size(tp)

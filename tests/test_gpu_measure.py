@@ -204,10 +204,12 @@ def test_calculate_adjoint_sources_dispatch():
         print(f"imeasure_type {im}: misfit {tot:.6e}")
 
 
-def test_window_selection_matches_oracle():
+@pytest.mark.parametrize("split", [0, 1])
+def test_window_selection_matches_oracle(split):
     """win_sel_type%select_windows for every trace: the same windows as the
     oracle (bit-identical correlation sums and decisions), including traces
-    split by a time-shift jump and a trace with nothing to select."""
+    split by a time-shift jump and a trace with nothing to select; with phase
+    splitting the groups are also cut at the STA/LTA troughs."""
     rng = np.random.default_rng(17)
     nstep, nrec, dt, T = 3000, 5, 0.02, 2.0
     t = np.arange(nstep) * dt
@@ -224,12 +226,12 @@ def test_window_selection_matches_oracle():
             obs[:, r, c] = w(a1 + rng.uniform(-0.1, 0.1), 0.5) + 0.7 * w(a2 + jump, 0.45) \
                 + 0.003 * rng.standard_normal(nstep)
     obs[:, 4, 1] = 0.003 * rng.standard_normal(nstep)  # nothing to select
-    got = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, T)
+    got = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, T, is_split_phases=bool(split))
     counts = []
     for r in range(nrec):
         for c in range(3):
-            ref, _, _ = orc.win_sel(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], T)
+            ref, _, _ = orc.win_sel(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], T, is_split_phases=split)
             np.testing.assert_array_equal(got[3 * r + c], ref)
             counts.append(len(ref))
     assert max(counts) >= 2 and min(counts) == 0
-    print(f"windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")
+    print(f"split {split}: windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")

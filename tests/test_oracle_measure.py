@@ -328,3 +328,25 @@ def test_windows_follow_the_selection_rules():
         ia, ib = int(round(a / dt - nlen / 2.0)), int(round(b / dt - nlen / 2.0))
         assert np.all(np.abs(jumps[:, None] - np.arange(ia, ib + 1)[None, :]) > nj)
         assert abs(tc[ia] - a) < 1e-9 and abs(tc[ib] - b) < 1e-9
+
+
+def test_phase_splitting_cuts_at_the_sta_lta_trough():
+    """Two arrivals that merge into one window are split (is_split_phases) into
+    two touching windows whose common edge lies between the arrivals; a single
+    arrival is left whole."""
+    dt, n, T = 0.02, 3000, 2.0
+    t = np.arange(n) * dt
+    w = lambda t0, f: np.exp(-((t - t0) * f * 0.6) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    rng = np.random.default_rng(3)
+    syn = w(20, 0.5) + 0.8 * w(27, 0.5)
+    dat = syn + 0.002 * rng.standard_normal(n)
+    one, _, _ = O.win_sel(dat, syn, dt, 0.0, 15.0, 90.0, T)
+    two, _, _ = O.win_sel(dat, syn, dt, 0.0, 15.0, 90.0, T, is_split_phases=1)
+    assert len(one) == 1 and len(two) == 2
+    assert two[0, 0] == one[0, 0] and two[1, 1] == one[0, 1] and two[0, 1] == two[1, 0]
+    assert 20.0 < two[0, 1] < 27.0
+    syn1 = w(20, 0.5)
+    dat1 = syn1 + 0.002 * rng.standard_normal(n)
+    a, _, _ = O.win_sel(dat1, syn1, dt, 0.0, 15.0, 90.0, T)
+    b, _, _ = O.win_sel(dat1, syn1, dt, 0.0, 15.0, 90.0, T, is_split_phases=1)
+    np.testing.assert_array_equal(a, b)
