@@ -355,6 +355,14 @@ void fwat_ep_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, rea
  * windows(2,maxwin,ntrace) out (s) */
 void fwat_win_sel_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, double *t0, double *tp,
                          double *dist, double *min_period, double *cfg, int *maxwin, int *nwin, double *windows);
+/* cross-convolution misfit of every receiver (imeas 2): WaveformConvMisfit%calc_adjoint_source
+ * (src/preproc/adjoint_sources/waveform_conv_misfit.f90:17-64 with myconvolution_dp part 0,
+ * src/shared/signal.f90:146-189) as src/preproc/telecc_data.f90:175-178 calls it; dat_r, dat_z, syn_r,
+ * syn_z (nrec,NSTEP) = [NSTEP][nrec]; windows(2,nrec) in seconds; misfits(nrec), residuals(nrec) (or
+ * NULL) out; adj_r, adj_z out (zero outside each window) */
+void fwat_conv_adjoint_batch_(int *nstep, int *nrec, double *dt, realw *dat_r, realw *dat_z, realw *syn_r,
+                              realw *syn_z, double *windows, double *taper_percentage, int *itaper_type,
+                              double *misfits, double *residuals, realw *adj_r, realw *adj_z);
 void fwat_mt_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, int *nwin,
                             double *windows, double *taper_percentage, int *itaper_type, int *imeas,
                             double *cfg, int *nsect, realw *sos, double *misfits, double *meas, realw *adj);

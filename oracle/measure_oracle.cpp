@@ -33,6 +33,8 @@
 //                         (exponentiated_phase_misfit.f90:21-104)
 //   orc_win_sel           win_sel_type%select_windows (window_selector/win_sel.f90:56-469,
 //                         sta_lta.f90:6-104)
+//   orc_conv_adjoint      WaveformConvMisfit%calc_adjoint_source (waveform_conv_misfit.f90:17-64)
+//                         with myconvolution_dp part 0 (signal.f90:146-189)
 //   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
 //   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
 //                         where the reference calls LAPACK dsterf
@@ -273,6 +275,19 @@ std::vector<int> maxima(const std::vector<double>& x, double sign) {
   for (int i = 1; i < n - 1; ++i)
     if (sign * x[i] - sign * x[i - 1] > 0.0 && sign * x[i + 1] - sign * x[i] < 0.0) r.push_back(i + 1);
   return r;
+}
+
+
+// myconvolution_dp (signal.f90:146-189) with part 0: the n2 samples of the full
+// convolution from index n2/2 + 1 (even n2) or n2/2 (odd n2), 1-based, each
+// output summed over sig1's index in increasing order
+std::vector<double> conv_mid(const std::vector<double>& x, const std::vector<double>& y) {
+  const int n1 = (int)x.size(), n2 = (int)y.size();
+  std::vector<double> full(n1 + n2 - 1, 0.0);
+  for (int i1 = 0; i1 < n1; ++i1)
+    for (int i2 = 0; i2 < n2; ++i2) full[i1 + i2] = full[i1 + i2] + x[i1] * y[i2];
+  const int ind = (n2 % 2 == 0) ? n2 / 2 + 1 : n2 / 2;
+  return std::vector<double>(full.begin() + (ind - 1), full.begin() + (ind - 1) + n2);
 }
 
 // number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
@@ -949,6 +964,36 @@ int orc_win_sel(const double* dat_in, const double* syn_in, int npts, double dt,
   }
   if (start >= 0) group(start, nw - 1);
   return count;
+}
+
+// WaveformConvMisfit%calc_adjoint_source (waveform_conv_misfit.f90:17-64): the
+// cross-convolution of radial and vertical components, c1 = syn_r * dat_z,
+// c2 = dat_r * syn_z, diff = (c1 - c2) dt, adjoint sources
+// conv(rev(dat_z), diff) dt and conv(-rev(dat_r), diff) dt on the window;
+// out[0] misfit 0.5 simpson(tapered diff^2), out[1] residual sum(diff) / nlen_win
+void orc_conv_adjoint(const double* dat_r, const double* dat_z, const double* syn_r, const double* syn_z, int n,
+                      double dt, double w0, double w1, double taper_pct, int itaper, double* adj_r, double* adj_z,
+                      double* out) {
+  const int nb = (int)(w0 / dt) + 1, ne = (int)(w1 / dt), nl = ne - nb + 1;
+  const std::vector<double> dr(dat_r, dat_r + n), dz(dat_z, dat_z + n), sr(syn_r, syn_r + n), sz(syn_z, syn_z + n);
+  const std::vector<double> c1 = conv_mid(sr, dz), c2 = conv_mid(dr, sz);
+  std::vector<double> cd(n);
+  for (int i = 0; i < n; ++i) cd[i] = (c1[i] - c2[i]) * dt;
+  std::vector<double> rz(n), rr(n);
+  for (int i = 0; i < n; ++i) { rz[i] = dz[n - 1 - i]; rr[i] = -dr[n - 1 - i]; }
+  std::vector<double> ar = conv_mid(rz, cd), az = conv_mid(rr, cd);
+  std::vector<double> tr(nl), tz(nl), tcd(cd.begin() + nb - 1, cd.begin() + ne);
+  for (int i = 0; i < nl; ++i) { tr[i] = ar[nb - 1 + i] * dt; tz[i] = az[nb - 1 + i] * dt; }
+  orc_window_taper(tcd.data(), nl, taper_pct, itaper);
+  orc_window_taper(tr.data(), nl, taper_pct, itaper);
+  orc_window_taper(tz.data(), nl, taper_pct, itaper);
+  std::vector<double> sq(nl);
+  double sm = 0.0;
+  for (int i = 0; i < nl; ++i) { sq[i] = tcd[i] * tcd[i]; sm = sm + tcd[i]; }
+  out[0] = 0.5 * orc_simpson(sq.data(), nl, dt);
+  out[1] = sm / nl;
+  for (int i = 0; i < n; ++i) { adj_r[i] = 0.0; adj_z[i] = 0.0; }
+  for (int i = 0; i < nl; ++i) { adj_r[nb - 1 + i] = tr[i]; adj_z[nb - 1 + i] = tz[i]; }
 }
 
 }  // extern "C"

@@ -138,6 +138,8 @@ def lib():
                                       _dp] + [_dp] * 7 + [_ip])
         L.orc_fft.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
         L.orc_hilbert.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.orc_conv_adjoint.argtypes = [_dp] * 4 + [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_int, _dp, _dp, _dp]
         L.orc_win_sel.restype = C.c_int
         L.orc_win_sel.argtypes = [_dp, _dp, C.c_int] + [C.c_double] * 5 + [_dp, C.c_int, _dp, _dp, _dp]
         L.orc_ep_adjoint.restype = C.c_double
@@ -651,3 +653,15 @@ def win_sel(dat, syn, dt, t0, tp, dis, min_period, maxwin=1000, **cfg):
     k = lib().orc_win_sel(_p(dat, C.c_double), _p(syn, C.c_double), n, dt, t0, tp, dis, min_period,
                           _p(cv, C.c_double), maxwin, _p(tw, C.c_double), _p(cc, C.c_double), _p(ts, C.c_double))
     return tw[:min(k, maxwin)].copy(), cc[:max(n - nlen, 0)], ts[:max(n - nlen, 0)]
+
+
+def conv_adjoint(dat_r, dat_z, syn_r, syn_z, dt, window, taper_pct=0.05, itaper=1):
+    """Cross-convolution misfit of one receiver: (misfit, residual, adj_r, adj_z)."""
+    a = [np.ascontiguousarray(v, np.float64) for v in (dat_r, dat_z, syn_r, syn_z)]
+    n = len(a[0])
+    ar = np.zeros(n)
+    az = np.zeros(n)
+    out = np.zeros(2)
+    lib().orc_conv_adjoint(*[_p(v, C.c_double) for v in a], n, dt, float(window[0]), float(window[1]), taper_pct,
+                           itaper, _p(ar, C.c_double), _p(az, C.c_double), _p(out, C.c_double))
+    return out[0], out[1], ar, az

@@ -1025,6 +1025,140 @@ __global__ void __launch_bounds__(MT_THREADS) k_ep_window(int nl, double dt, int
   }
 }
 
+// ---- cross-convolution (waveform_conv_misfit.f90, IMEAS_WAVEFORM_CONV)
+
+// samples kb[r] + j (j < nout[r]) of the full convolution of row r of X (n1,
+// optionally reversed and signed) with row r of Y (n2): each output summed over
+// X's index in increasing order as myconvolution_dp; workgroup (row, 1024
+// outputs), four consecutive outputs per lane on a sliding register window of
+// Y, X and the matching stretch of Y through LDS in 1024-sample chunks (Y
+// zero outside the row, so an output only gains exact zeros).  mode 0: O = c
+// scale; mode 1: O = (O - c) scale (the difference of two convolutions)
+constexpr int CV_OUT = 1024, CV_CHUNK = 1024;
+__global__ void __launch_bounds__(256) k_conv_mid(int n1, int n2, const double* __restrict__ X, int xrev,
+                                                  double xsign, const double* __restrict__ Y,
+                                                  const int* __restrict__ kb, const int* __restrict__ nout,
+                                                  int ostride, int mode, double scale, double* __restrict__ O) {
+  __shared__ double sx[CV_CHUNK], sy[CV_CHUNK + CV_OUT];
+  const int r = blockIdx.y, no = nout[r], j0 = blockIdx.x * CV_OUT;
+  if (j0 >= no) return;
+  const double* x = X + (size_t)r * n1;
+  const double* y = Y + (size_t)r * n2;
+  const int kbase = kb[r] + j0, jl = j0 + 4 * (int)threadIdx.x, kl = kb[r] + jl;
+  const int kmaxb = kb[r] + min(j0 + CV_OUT, no) - 1;
+  const int ablo = max(0, kbase - (n2 - 1)), abhi = min(n1 - 1, kmaxb);
+  const int alo = max(0, kl - (n2 - 1)), ahi = min(n1 - 1, kl + 3);
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+  for (int a0 = ablo; a0 <= abhi; a0 += CV_CHUNK) {
+    const int aend = min(a0 + CV_CHUNK - 1, abhi);
+    const int ybase = kbase - (a0 + CV_CHUNK - 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < CV_CHUNK; i += 256) {
+      const int a = a0 + i;
+      sx[i] = a <= aend ? xsign * x[xrev ? n1 - 1 - a : a] : 0.0;
+    }
+    for (int i = threadIdx.x; i < CV_CHUNK + CV_OUT; i += 256) {
+      const int j = ybase + i;
+      sy[i] = (j >= 0 && j < n2) ? y[j] : 0.0;
+    }
+    __syncthreads();
+    if (jl < no) {
+      const int a = max(alo, a0), b = min(ahi, aend);
+      const int off = kl - ybase;  // y[kl + q - a] sits at sy[off + q - a], in [0, CV_CHUNK + CV_OUT - 2]
+      if (a <= b) {
+        double r0 = sy[off - a], r1 = sy[off + 1 - a], r2 = sy[off + 2 - a], r3 = sy[off + 3 - a];
+        for (int i = a; i <= b; ++i) {
+          const double xv = sx[i - a0];
+          c0 = c0 + xv * r0;
+          c1 = c1 + xv * r1;
+          c2 = c2 + xv * r2;
+          c3 = c3 + xv * r3;
+          r3 = r2;
+          r2 = r1;
+          r1 = r0;
+          r0 = i < b ? sy[off - i - 1] : 0.0;
+        }
+      }
+    }
+  }
+  const double cv[4] = {c0, c1, c2, c3};
+  for (int q = 0; q < 4; ++q) {
+    const int j = jl + q;
+    if (j >= no) break;
+    double* o = O + (size_t)r * ostride + j;
+    *o = mode == 0 ? cv[q] * scale : (*o - cv[q]) * scale;
+  }
+}
+
+// window_taper weight of sample i of an npts-sample window (signal.f90:419-491)
+__device__ double taper_w(int i, int npts, double pct, int type) {
+  const int frac = (pct == 0.0 || pct == 1.0) ? (int)(npts * pct / 2.0) : (int)(npts * pct / 2.0 + 0.5);
+  if (frac <= 0) return 1.0;
+  int rr;
+  if (i < frac) rr = i;
+  else if (i >= npts - frac) rr = frac + (i - (npts - frac));
+  else return 1.0;
+  const double pi = 3.14159265358979323846;
+  const int den = 2 * frac - 1;
+  switch (type) {
+    case 1: return 0.5 - 0.5 * cos(2.0 * pi * rr / den);
+    case 2: return 0.54 - 0.46 * cos(2.0 * pi * rr / den);
+    case 3: return cos(pi * rr / den - pi / 2.0);
+    default: return 1.0 - pow(cos(pi * rr / den), 10.0);
+  }
+}
+
+// per receiver (one workgroup): the tapered windowed difference, its Simpson
+// misfit and mean residual, and both tapered adjoint sources into the
+// sample-major outputs
+__global__ void __launch_bounds__(256) k_conv_finish(int nstep, int nrec, double dt, const int* __restrict__ nbw,
+                                                     const int* __restrict__ nlw, double pct, int type,
+                                                     const double* __restrict__ CD, const double* __restrict__ AR,
+                                                     const double* __restrict__ AZ, double* __restrict__ out,
+                                                     float* __restrict__ adj_r, float* __restrict__ adj_z) {
+  __shared__ double sh[256];
+  const int r = blockIdx.x, nb = nbw[r], nl = nlw[r];
+  const double* cd = CD + (size_t)r * nstep + nb;
+  double sq = 0.0, sm = 0.0;
+  for (int i = threadIdx.x; i < nl; i += 256) {
+    const double w = taper_w(i, nl, pct, type);
+    const double v = cd[i] * w;
+    double wt, tail = 0.0;
+    if (nl % 2 == 0) {
+      wt = (i == 0 ? 1.0 : 0.0) + (i == nl - 2 ? 1.0 : 0.0) + ((i % 2 == 1 && i <= nl - 3) ? 4.0 : 0.0) +
+           ((i % 2 == 0 && i >= 2 && i <= nl - 2) ? 2.0 : 0.0);
+      if (i >= nl - 2) tail = 1.0;
+    } else {
+      wt = (i == 0 || i == nl - 1) ? 1.0 : (i % 2 == 1 ? 4.0 : 2.0);
+    }
+    sq += (wt * (dt / 3.0) + tail * 0.5 * dt) * (v * v);
+    sm += v;
+    adj_r[(size_t)(nb + i) * nrec + r] = (float)(AR[(size_t)r * nstep + i] * w);
+    adj_z[(size_t)(nb + i) * nrec + r] = (float)(AZ[(size_t)r * nstep + i] * w);
+  }
+  const double mf = wg_sum(sq, sh), rs = wg_sum(sm, sh);
+  if (threadIdx.x == 0) {
+    out[2 * r] = nl >= 2 ? 0.5 * mf : 0.0;
+    out[2 * r + 1] = rs / nl;
+  }
+}
+
+// float sample-major traces [nstep][ntrace] to double trace-major rows
+__global__ void __launch_bounds__(256) k_rows_f64(int nstep, int ntrace, const float* __restrict__ x,
+                                                  double* __restrict__ y) {
+  __shared__ float tile[64][65];
+  const int t0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int n = n0 + r, t = t0 + (threadIdx.x & 63);
+    if (n < nstep && t < ntrace) tile[r][threadIdx.x & 63] = x[(size_t)n * ntrace + t];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x / 64; r < 64; r += 4) {
+    const int t = t0 + r, n = n0 + (threadIdx.x & 63);
+    if (n < nstep && t < ntrace) y[(size_t)t * nstep + n] = (double)tile[threadIdx.x & 63][r];
+  }
+}
+
 // ---- window selection (window_selector/win_sel.f90, WIN_SELECTOR_TYPE)
 
 struct WsCfg {
@@ -2056,6 +2190,88 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   dev_free(d_twe);
   for (int t = 0; t < T; ++t)
     if (nwin[t] < 0) SEM_FAIL("fwat_win_sel_batch_: trace %d has more than %d good groups", t, WS_MAXG);
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  HIP_CHECK(hipStreamDestroy(s));
+}
+
+// waveform_conv_misfit.f90 WaveformConvMisfit%calc_adjoint_source for every
+// receiver (telecc_data.f90:175-178): dat_r, dat_z, syn_r, syn_z [nstep][nrec]
+// single precision; windows(2, nrec) in seconds; misfits(nrec), residuals(nrec)
+// out; adj_r, adj_z [nstep][nrec] out (zero outside each window; the caller
+// rotates R to N/E and weights as telecc_data does)
+void fwat_conv_adjoint_batch_(int* nstep, int* nrec, double* dt, realw* dat_r, realw* dat_z, realw* syn_r,
+                              realw* syn_z, double* windows, double* taper_percentage, int* itaper_type,
+                              double* misfits, double* residuals, realw* adj_r, realw* adj_z) {
+  const int N = *nstep, R = *nrec;
+  if (N < 2 || R <= 0) SEM_FAIL("fwat_conv_adjoint_batch_: sizes must be positive");
+  if (R > 65535) SEM_FAIL("fwat_conv_adjoint_batch_: at most 65535 receivers per call");
+  if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
+  const int ind0 = (N % 2 == 0) ? N / 2 : N / 2 - 1;  // 0-based first sample of myconvolution_dp's part 0
+  std::vector<int> nbw(R), nlw(R), kb0(R, ind0), nall(R, N), kbw(R);
+  int maxnl = 0;
+  for (int r = 0; r < R; ++r) {
+    window_range(windows[2 * r], windows[2 * r + 1], *dt, N, nbw[r], nlw[r]);
+    kbw[r] = ind0 + nbw[r];
+    maxnl = std::max(maxnl, nlw[r]);
+  }
+  const size_t n = (size_t)N * R;
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  float* d_in[4];
+  realw* h_in[4] = {dat_r, dat_z, syn_r, syn_z};
+  for (int q = 0; q < 4; ++q) d_in[q] = dev_upload(h_in[q], n, s);
+  double* d_rows = dev_alloc<double>(4 * n);  // dat_r, dat_z, syn_r, syn_z rows
+  double* d_cd = dev_alloc<double>(n);
+  double* d_ar = dev_alloc<double>(n);
+  double* d_az = dev_alloc<double>(n);
+  int* d_nb = dev_upload(nbw.data(), (size_t)R, s);
+  int* d_nl = dev_upload(nlw.data(), (size_t)R, s);
+  int* d_kb0 = dev_upload(kb0.data(), (size_t)R, s);
+  int* d_nall = dev_upload(nall.data(), (size_t)R, s);
+  int* d_kbw = dev_upload(kbw.data(), (size_t)R, s);
+  double* d_out = dev_alloc<double>(2 * (size_t)R);
+  float* d_ar_out = dev_alloc<float>(n);
+  float* d_az_out = dev_alloc<float>(n);
+  HIP_CHECK(hipEventRecord(ev0, s));
+  hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_ar_out);
+  hipLaunchKernelGGL(k_fill_f32m, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d_az_out);
+  const dim3 tt((unsigned)((R + 63) / 64), (unsigned)((N + 63) / 64));
+  for (int q = 0; q < 4; ++q) hipLaunchKernelGGL(k_rows_f64, tt, dim3(256), 0, s, N, R, d_in[q], d_rows + q * n);
+  const double* dr = d_rows;
+  const double* dz = d_rows + n;
+  const double* sr = d_rows + 2 * n;
+  const double* sz = d_rows + 3 * n;
+  const dim3 gall((unsigned)((N + CV_OUT - 1) / CV_OUT), (unsigned)R);
+  const dim3 gwin((unsigned)((maxnl + CV_OUT - 1) / CV_OUT), (unsigned)R);
+  // diff = (syn_r * dat_z - dat_r * syn_z) dt
+  hipLaunchKernelGGL(k_conv_mid, gall, dim3(256), 0, s, N, N, sr, 0, 1.0, dz, d_kb0, d_nall, N, 0, 1.0, d_cd);
+  hipLaunchKernelGGL(k_conv_mid, gall, dim3(256), 0, s, N, N, dr, 0, 1.0, sz, d_kb0, d_nall, N, 1, *dt, d_cd);
+  // adjoint sources on the window: conv(rev(dat_z), diff) dt, conv(-rev(dat_r), diff) dt
+  hipLaunchKernelGGL(k_conv_mid, gwin, dim3(256), 0, s, N, N, dz, 1, 1.0, d_cd, d_kbw, d_nl, N, 0, *dt, d_ar);
+  hipLaunchKernelGGL(k_conv_mid, gwin, dim3(256), 0, s, N, N, dr, 1, -1.0, d_cd, d_kbw, d_nl, N, 0, *dt, d_az);
+  hipLaunchKernelGGL(k_conv_finish, dim3((unsigned)R), dim3(256), 0, s, N, R, *dt, d_nb, d_nl, *taper_percentage,
+                     *itaper_type, d_cd, d_ar, d_az, d_out, d_ar_out, d_az_out);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(ev1, s));
+  std::vector<double> out(2 * (size_t)R);
+  HIP_CHECK(hipMemcpyAsync(adj_r, d_ar_out, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(adj_z, d_az_out, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+  g_meas_ms = ms;
+  for (int r = 0; r < R; ++r) {
+    misfits[r] = out[2 * r];
+    if (residuals) residuals[r] = out[2 * r + 1];
+  }
+  for (int q = 0; q < 4; ++q) dev_free(d_in[q]);
+  dev_free(d_rows); dev_free(d_cd); dev_free(d_ar); dev_free(d_az); dev_free(d_nb); dev_free(d_nl); dev_free(d_kb0);
+  dev_free(d_nall); dev_free(d_kbw); dev_free(d_out); dev_free(d_ar_out); dev_free(d_az_out);
   HIP_CHECK(hipEventDestroy(ev0));
   HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));

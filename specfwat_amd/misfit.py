@@ -279,7 +279,7 @@ def calculate_adjoint_sources(obs, syn, dt, windows, short_p, long_p, imeasure_t
         return total, adj, None
     if imeasure_type == IMEAS_WAVEFORM_CONV:
         raise ValueError("WAVEFORM_CONV misfit requires radial/vertical components "
-                         "(calc_adjoint_source_conv in the reference)")
+                         "(event_conv_adjoint_sources; calc_adjoint_source_conv in the reference)")
     if imeasure_type == IMEAS_RF:
         raise ValueError("RF misfit requires additional parameters (calc_adjoint_source_rf in the reference)")
     if imeasure_type == IMEAS_EXP_PHASE:
@@ -332,6 +332,55 @@ def select_windows(obs, syn, dt, t0, tp, dist, min_period, maxwin=64, **cfg):
     if (nwin > maxwin).any():
         raise RuntimeError(f"a trace has more than maxwin={maxwin} windows; raise maxwin")
     return [win[t, :nwin[t]].copy() for t in range(ntrace)]
+
+
+def event_conv_adjoint_sources(dat_r, dat_z, syn_r, syn_z, dt, windows, taper_percentage=0.05, itaper_type=1):
+    """Cross-convolution misfit (imeas 2, WaveformConvMisfit%calc_adjoint_source,
+    waveform_conv_misfit.f90:17-64, as telecc_data.f90:175-178 calls it) of every
+    receiver in one device pass: dat_*, syn_* (NSTEP, nrec) radial and vertical
+    traces, windows (nrec, 2) in seconds.  Returns (total misfit, misfits (nrec,),
+    residuals (nrec,), adj_r, adj_z float32 (NSTEP, nrec)).  The caller rotates
+    adj_r to N/E and applies the source weight as telecc_data does."""
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (dat_r, dat_z, syn_r, syn_z)]
+    nstep, nrec = arrs[0].shape
+    if any(a.shape != (nstep, nrec) for a in arrs):
+        raise ValueError("the four traces must share their shape (NSTEP, nrec)")
+    win = np.ascontiguousarray(np.reshape(np.asarray(windows, np.float64), (nrec, 2)))
+    for w0, w1 in win:
+        nb, ne, _ = get_window_info((w0, w1), dt)
+        if nb < 1 or ne > nstep:
+            raise ValueError(f"measurement window [{w0}, {w1}] s lies outside the {nstep}-sample trace")
+    mf = np.zeros(nrec)
+    rs = np.zeros(nrec)
+    ar = np.zeros((nstep, nrec), np.float32)
+    az = np.zeros((nstep, nrec), np.float32)
+    _lib.call("fwat_conv_adjoint_batch_", nstep, nrec, float(dt), *arrs, win, float(taper_percentage),
+              int(itaper_type), mf, rs, ar, az)
+    total = 0.0
+    for v in mf:
+        total += v
+    return total, mf, rs, ar, az
+
+
+WIN_SELECTOR_TYPE, WIN_FLEX_TYPE, WIN_GROUPVEL_TYPE, WIN_ARRIVAL_TYPE = 1, 2, 3, 4  # config.f90:58-61
+
+
+def fixed_windows(win_type, nstep, dt, t0, tp=None, dist=None, time_win=None, groupvel=None, long_p=None):
+    """The one-window types of leq_data.f90:121-135 for every receiver: group
+    velocity (3): [dist / vmax - long_p / 2, dist / vmin + long_p / 2]; arrival
+    (4): [tp + time_win[0], tp + time_win[1]]; both clipped to
+    [-t0, (NSTEP - 2) dt - t0].  Returns [nrec, 2] in seconds."""
+    if win_type == WIN_GROUPVEL_TYPE:
+        d = np.asarray(dist, np.float64)
+        w0 = d / groupvel[1] - long_p / 2.0
+        w1 = d / groupvel[0] + long_p / 2.0
+    elif win_type == WIN_ARRIVAL_TYPE:
+        p = np.asarray(tp, np.float64)
+        w0 = p + time_win[0]
+        w1 = p + time_win[1]
+    else:
+        raise ValueError(f"window type {win_type} is not a one-window type (3 or 4); use select_windows for 1")
+    return np.stack([np.maximum(w0, -t0), np.minimum(w1, (nstep - 2) * dt - t0)], axis=1)
 
 
 def last_time_ms():

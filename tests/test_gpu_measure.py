@@ -235,3 +235,33 @@ def test_window_selection_matches_oracle(split):
             counts.append(len(ref))
     assert max(counts) >= 2 and min(counts) == 0
     print(f"split {split}: windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")
+
+
+@pytest.mark.parametrize("nstep", [3000, 2999])
+def test_conv_batch_matches_oracle(nstep):
+    """Cross-convolution of every receiver (even and odd trace lengths: the two
+    part-0 offsets of myconvolution_dp), per-receiver windows, against the oracle."""
+    rng = np.random.default_rng(nstep)
+    nrec, dt = 7, 0.02
+    t = np.arange(nstep) * dt
+    w = lambda t0, f: np.exp(-((t - t0) * f) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    tr = [np.zeros((nstep, nrec), np.float32) for _ in range(4)]
+    win = np.zeros((nrec, 2))
+    for r in range(nrec):
+        t0, f = rng.uniform(15, 30), rng.uniform(0.3, 0.8)
+        a, b = rng.uniform(0.5, 1.0), rng.uniform(0.3, 0.8)
+        tr[2][:, r] = a * w(t0, f)
+        tr[3][:, r] = b * w(t0 + 0.4, f)
+        tr[0][:, r] = a * w(t0 + rng.uniform(-0.3, 0.3), f) + 0.01 * rng.standard_normal(nstep)
+        tr[1][:, r] = b * w(t0 + 0.5, f) + 0.01 * rng.standard_normal(nstep)
+        win[r] = [t0 - 8 - rng.uniform(0, 3), t0 + 10 + rng.uniform(0, 5)]
+    total, mf, rs, ar, az = misfit.event_conv_adjoint_sources(*tr, dt, win)
+    tot = 0.0
+    for r in range(nrec):
+        m, res, a, b = orc.conv_adjoint(tr[0][:, r], tr[1][:, r], tr[2][:, r], tr[3][:, r], dt, win[r])
+        tot += m
+        assert abs(mf[r] - m) <= 1e-9 * m and abs(rs[r] - res) <= 1e-9 * max(abs(res), 1e-12)
+        assert np.abs(ar[:, r] - a).max() <= 1e-6 * np.abs(a).max()
+        assert np.abs(az[:, r] - b).max() <= 1e-6 * np.abs(b).max()
+    assert abs(total - tot) <= 1e-9 * tot
+    print(f"cross-convolution nstep {nstep}: misfit {total:.6e}, {misfit.last_time_ms():.2f} ms")

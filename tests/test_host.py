@@ -229,3 +229,14 @@ def test_adjoint_source_dispatch_errors():
     with pytest.raises(ValueError, match="16383"):
         misfit.event_mt_adjoint_sources(np.zeros((20000, 1, 3), np.float32), np.zeros((20000, 1, 3), np.float32),
                                         0.01, [[1.0, 9.0]], 0.5, 10.0)
+
+
+def test_fixed_window_types():
+    """leq_data.f90:121-135: group-velocity and arrival windows, clipped to the trace."""
+    from specfwat_amd import misfit
+    w = misfit.fixed_windows(3, 1000, 0.05, 2.0, dist=[30.0, 100.0], groupvel=(2.0, 4.0), long_p=10.0)
+    np.testing.assert_allclose(w, [[30 / 4 - 5, 30 / 2 + 5], [100 / 4 - 5, min(100 / 2 + 5, 998 * 0.05 - 2.0)]])
+    w = misfit.fixed_windows(4, 1000, 0.05, 2.0, tp=[1.0, 40.0], time_win=(-5.0, 20.0))
+    np.testing.assert_allclose(w, [[-2.0, 21.0], [35.0, 47.9]])
+    with pytest.raises(ValueError):
+        misfit.fixed_windows(1, 1000, 0.05, 2.0)

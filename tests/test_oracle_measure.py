@@ -350,3 +350,44 @@ def test_phase_splitting_cuts_at_the_sta_lta_trough():
     a, _, _ = O.win_sel(dat1, syn1, dt, 0.0, 15.0, 90.0, T)
     b, _, _ = O.win_sel(dat1, syn1, dt, 0.0, 15.0, 90.0, T, is_split_phases=1)
     np.testing.assert_array_equal(a, b)
+
+
+# ---- cross-convolution (waveform_conv_misfit)
+
+def _conv_case(n=600, dt=0.05):
+    t = np.arange(n) * dt
+    w = lambda t0, f: np.exp(-((t - t0) * f) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    return w(8, 0.5), 0.7 * w(8.3, 0.5), w(8.4, 0.5), 0.7 * w(8.6, 0.5), t, dt
+
+
+def test_cross_convolution_is_source_independent():
+    """Data = the synthetic convolved with any common source wavelet on both
+    components: c1 = c2, misfit ~ 0; a different model gives a finite misfit."""
+    sr, sz, dr, dz, t, dt = _conv_case()
+    n = len(t)
+    k = np.exp(-((t - 2) / 0.3) ** 2)
+    cr, cz = np.convolve(sr, k)[:n], np.convolve(sz, k)[:n]
+    m0, _, _, _ = O.conv_adjoint(cr, cz, sr, sz, dt, [2.0, 25.0])
+    m1, _, _, _ = O.conv_adjoint(dr, dz, sr, sz, dt, [2.0, 25.0])
+    assert m0 < 1e-15 * m1 and m1 > 1e-3
+    m2, r2, a2, b2 = O.conv_adjoint(sr, sz, sr, sz, dt, [2.0, 25.0])
+    assert m2 == 0.0 and r2 == 0.0 and not a2.any() and not b2.any()
+
+
+@pytest.mark.parametrize("n", [600, 601])
+def test_cross_convolution_matches_numpy_convolve(n):
+    """myconvolution_dp part 0 (samples from n/2 + 1, or n/2 for odd n, 1-based)
+    and the misfit / adjoint construction written with numpy's convolve."""
+    sr, sz, dr, dz, t, dt = _conv_case(n)
+    from specfwat_amd.misfit import simpson, window_taper
+    ind = n // 2 + 1 if n % 2 == 0 else n // 2
+    mid = lambda x, y: np.convolve(x, y)[ind - 1:ind - 1 + n]
+    cd = (mid(sr, dz) - mid(dr, sz)) * dt
+    nb, ne = int(2.0 / dt) + 1, int(25.0 / dt)
+    ar = (mid(dz[::-1], cd) * dt)[nb - 1:ne]
+    az = (mid(-dr[::-1], cd) * dt)[nb - 1:ne]
+    c = window_taper(cd[nb - 1:ne], 0.05, 1)
+    m, r, a, b = O.conv_adjoint(dr, dz, sr, sz, dt, [2.0, 25.0])
+    assert abs(m - 0.5 * simpson(c ** 2, dt)) < 1e-12 * m and abs(r - c.sum() / len(c)) < 1e-12
+    assert np.abs(a[nb - 1:ne] - window_taper(ar, 0.05, 1)).max() < 1e-12 * np.abs(ar).max()
+    assert np.abs(b[nb - 1:ne] - window_taper(az, 0.05, 1)).max() < 1e-12 * np.abs(az).max()
