@@ -257,25 +257,49 @@ struct AsmField {
 // fp32 block partial sums when rho_part is given) and the approximate Hessian
 // a0 . a1 (fp64) accumulated.  Every assembly pass calls this, so they share
 // one expression order.
+// The point's state (inverse mass, u and v of each field) as loaded before its
+// force sums, so that these loads overlap the gather's dependent chain.
+struct PointState {
+  realw r[3], u0[3], v0[3], u1[3], v1[3];
+};
+
 template <bool TWO>
-__device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict__ rm, realw f0x, realw f0y,
-                                              realw f0z, realw f1x, realw f1y, realw f1z, const AsmField& F0,
+__device__ __forceinline__ PointState load_point(size_t g, const realw* __restrict__ rm, const AsmField& F0,
+                                                 const AsmField& F1) {
+  PointState P;
+  const size_t b = 3 * g;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    P.r[c] = rm[b + c];
+    P.u0[c] = F0.u[b + c];
+    P.v0[c] = F0.v[b + c];
+    if (TWO) {
+      P.u1[c] = F1.u[b + c];
+      P.v1[c] = F1.v[b + c];
+    }
+  }
+  return P;
+}
+
+template <bool TWO>
+__device__ __forceinline__ void newmark_point(size_t g, const PointState& P, realw f0x, realw f0y, realw f0z,
+                                              realw f1x, realw f1y, realw f1z, const AsmField& F0,
                                               const AsmField& F1, bool last, double* __restrict__ rho_kl,
                                               float* __restrict__ rho_part, bool flush,
                                               double* __restrict__ hess_kl, double dtk) {
   const size_t b = 3 * g;
-  const realw r0 = rm[b], r1 = rm[b + 1], r2 = rm[b + 2];
+  const realw r0 = P.r[0], r1 = P.r[1], r2 = P.r[2];
   const realw A[3] = {f0x * r0, f0y * r1, f0z * r2};
   const realw d0 = F0.dt, h0 = 0.5f * F0.dt, s0 = 0.5f * F0.dt * F0.dt;
   realw V[3], U[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) { V[c] = F0.v[b + c] + h0 * A[c]; U[c] = F0.u[b + c]; }
+  for (int c = 0; c < 3; ++c) { V[c] = P.v0[c] + h0 * A[c]; U[c] = P.u0[c]; }
   if (TWO) {
     const realw BA[3] = {f1x * r0, f1y * r1, f1z * r2};
     const realw d1 = F1.dt, h1 = 0.5f * F1.dt, s1 = 0.5f * F1.dt * F1.dt;
     realw BV[3], BU[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { BV[c] = F1.v[b + c] + h1 * BA[c]; BU[c] = F1.u[b + c]; }
+    for (int c = 0; c < 3; ++c) { BV[c] = P.v1[c] + h1 * BA[c]; BU[c] = P.u1[c]; }
     kernel_acc(rho_kl, rho_part, g, A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2], dtk, flush);
     if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
 #pragma unroll
@@ -299,6 +323,16 @@ __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict_
       F0.v[b + c] = V[c] + h0 * A[c];
     }
   }
+}
+
+template <bool TWO>
+__device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict__ rm, realw f0x, realw f0y,
+                                              realw f0z, realw f1x, realw f1y, realw f1z, const AsmField& F0,
+                                              const AsmField& F1, bool last, double* __restrict__ rho_kl,
+                                              float* __restrict__ rho_part, bool flush,
+                                              double* __restrict__ hess_kl, double dtk) {
+  newmark_point<TWO>(g, load_point<TWO>(g, rm, F0, F1), f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last, rho_kl,
+                     rho_part, flush, hess_kl, dtk);
 }
 
 // internal launchers shared between the C-ABI files

@@ -94,6 +94,7 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
   if (h >= npts) return;
   const int g = glist ? glist[h] : h;
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
+  const PointState P = load_point<TWO>((size_t)g, rm, F0, F1);
   const int q1 = off[g + 1];
   int q = off[g];
   // pairs of element-local contributions (most of them): both rows are loaded
@@ -140,7 +141,7 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
       if (TWO && F1.adj) { f1x += h * F1.adj[r]; f1y += h * F1.adj[r + 1]; f1z += h * F1.adj[r + 2]; }
     }
   }
-  newmark_point<TWO>((size_t)g, rm, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, LAST, rho_kl, rho_part, flush != 0,
+  newmark_point<TWO>((size_t)g, P, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, LAST, rho_kl, rho_part, flush != 0,
                      hess_kl, dtk);
 }
 
