@@ -261,13 +261,15 @@ struct AsmField {
 // force sums, so that these loads overlap the gather's dependent chain.
 struct PointState {
   realw r[3], u0[3], v0[3], u1[3], v1[3];
+  float part;  // the density kernel's fp32 partial sum (two fields, partial sums on)
 };
 
 template <bool TWO>
 __device__ __forceinline__ PointState load_point(size_t g, const realw* __restrict__ rm, const AsmField& F0,
-                                                 const AsmField& F1) {
+                                                 const AsmField& F1, const float* __restrict__ rho_part = nullptr) {
   PointState P;
   const size_t b = 3 * g;
+  P.part = (TWO && rho_part) ? rho_part[g] : 0.0f;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     P.r[c] = rm[b + c];
@@ -300,7 +302,15 @@ __device__ __forceinline__ void newmark_point(size_t g, const PointState& P, rea
     realw BV[3], BU[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) { BV[c] = P.v1[c] + h1 * BA[c]; BU[c] = P.u1[c]; }
-    kernel_acc(rho_kl, rho_part, g, A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2], dtk, flush);
+    const float x = A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2];
+    if (!rho_part) {
+      kernel_acc(rho_kl, rho_part, g, x, dtk, flush);
+    } else if (flush) {  // kernel_acc with the partial sum read before the gather
+      rho_kl[g] += dtk * (double)(P.part + x);
+      rho_part[g] = 0.0f;
+    } else {
+      rho_part[g] = P.part + x;
+    }
     if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -331,8 +341,8 @@ __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict_
                                               const AsmField& F1, bool last, double* __restrict__ rho_kl,
                                               float* __restrict__ rho_part, bool flush,
                                               double* __restrict__ hess_kl, double dtk) {
-  newmark_point<TWO>(g, load_point<TWO>(g, rm, F0, F1), f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last, rho_kl,
-                     rho_part, flush, hess_kl, dtk);
+  newmark_point<TWO>(g, load_point<TWO>(g, rm, F0, F1, rho_part), f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last,
+                     rho_kl, rho_part, flush, hess_kl, dtk);
 }
 
 // internal launchers shared between the C-ABI files

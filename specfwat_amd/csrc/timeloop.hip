@@ -94,7 +94,7 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
   if (h >= npts) return;
   const int g = glist ? glist[h] : h;
   realw f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0;
-  const PointState P = load_point<TWO>((size_t)g, rm, F0, F1);
+  const PointState P = load_point<TWO>((size_t)g, rm, F0, F1, rho_part);
   const int q1 = off[g + 1];
   int q = off[g];
   // pairs of element-local contributions (most of them): both rows are loaded
