@@ -39,7 +39,57 @@ def _shape(xi, eta, gam):
     return N, np.stack([dNa, dNb, dNd], axis=-1)  # (P,8), (P,8,3)
 
 
-class BoxMesh:
+class SemMesh:
+    """Helpers over the SPECFEM database arrays shared by every mesh source
+    (ibool0, xyz, jacobian_regular64, jacobian64_irr, irregular_element_number, ...)."""
+
+    def distance_min_glob(self):
+        """Minimum distance between neighbouring GLL points of an element
+        (check_mesh_distances' distance_min_glob, used by smooth_sem_pde)."""
+        i = np.arange(125)
+        x = self.xyz[self.ibool0]  # (nspec,125,3)
+        best = np.inf
+        for stride, cond in ((1, i % 5 < 4), (5, (i // 5) % 5 < 4), (25, i // 25 < 4)):
+            src = i[cond]
+            d = np.linalg.norm(x[:, src + stride] - x[:, src], axis=-1)
+            best = min(best, d.min())
+        return float(best)
+
+    def distance_max_glob(self):
+        """Maximum distance between neighbouring GLL points of an element
+        (check_mesh_distances' distance_max_glob, used by taper_kernel_grid)."""
+        i = np.arange(125)
+        x = self.xyz[self.ibool0]
+        best = 0.0
+        for stride, cond in ((1, i % 5 < 4), (5, (i // 5) % 5 < 4), (25, i // 25 < 4)):
+            src = i[cond]
+            d = np.linalg.norm(x[:, src + stride] - x[:, src], axis=-1)
+            best = max(best, d.max())
+        return float(best)
+
+    def local_weights(self):
+        """jacobian * w_i w_j w_k per local point (float64, [nspec,125])."""
+        w = self.gll.w
+        ii = np.arange(125)
+        w3 = w[ii % 5] * w[(ii // 5) % 5] * w[ii // 25]
+        jac = np.full((self.nspec, 125), self.jacobian_regular64)
+        sel = self.irregular_element_number > 0
+        jac[sel] = self.jacobian64_irr
+        return jac * w3[None, :]
+
+    def smooth_arrays(self):
+        """The arrays smooth_sem_pde reads, flattened (1-based ibool)."""
+        d = dict(nspec=self.nspec, nglob=self.nglob, ibool=self.ibool.ravel(),
+                 xstore=self.xstore, ystore=self.ystore, zstore=self.zstore,
+                 irregular_element_number=self.irregular_element_number,
+                 xix_regular=float(self.xix_regular), jacobian_regular=float(self.jacobian_regular),
+                 nspec_irregular=self.nspec_irregular)
+        for nm in ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz", "jacobian"):
+            d[nm] = getattr(self, nm).ravel()
+        return d
+
+
+class BoxMesh(SemMesh):
     def __init__(self, nex, ney, nez, lx, ly, lz, perturb=0.0, force_irregular=False):
         self.nex, self.ney, self.nez = nex, ney, nez
         self.lx, self.ly, self.lz = float(lx), float(ly), float(lz)
@@ -196,52 +246,6 @@ class BoxMesh:
         self.abs_boundary_ijk = np.ascontiguousarray(np.concatenate(ijks), dtype=np.int32)
         self.abs_boundary_normal = np.ascontiguousarray(np.concatenate(normals), dtype=np.float32)
         self.abs_boundary_jacobian2Dw = np.ascontiguousarray(np.concatenate(jws), dtype=np.float32)
-
-    # ------------------------------------------------------------ helpers
-    def distance_min_glob(self):
-        """Minimum distance between neighbouring GLL points of an element
-        (check_mesh_distances' distance_min_glob, used by smooth_sem_pde)."""
-        i = np.arange(125)
-        x = self.xyz[self.ibool0]  # (nspec,125,3)
-        best = np.inf
-        for stride, cond in ((1, i % 5 < 4), (5, (i // 5) % 5 < 4), (25, i // 25 < 4)):
-            src = i[cond]
-            d = np.linalg.norm(x[:, src + stride] - x[:, src], axis=-1)
-            best = min(best, d.min())
-        return float(best)
-
-    def distance_max_glob(self):
-        """Maximum distance between neighbouring GLL points of an element
-        (check_mesh_distances' distance_max_glob, used by taper_kernel_grid)."""
-        i = np.arange(125)
-        x = self.xyz[self.ibool0]
-        best = 0.0
-        for stride, cond in ((1, i % 5 < 4), (5, (i // 5) % 5 < 4), (25, i // 25 < 4)):
-            src = i[cond]
-            d = np.linalg.norm(x[:, src + stride] - x[:, src], axis=-1)
-            best = max(best, d.max())
-        return float(best)
-
-    def local_weights(self):
-        """jacobian * w_i w_j w_k per local point (float64, [nspec,125])."""
-        w = self.gll.w
-        ii = np.arange(125)
-        w3 = w[ii % 5] * w[(ii // 5) % 5] * w[ii // 25]
-        jac = np.full((self.nspec, 125), self.jacobian_regular64)
-        sel = self.irregular_element_number > 0
-        jac[sel] = self.jacobian64_irr
-        return jac * w3[None, :]
-
-    def smooth_arrays(self):
-        """The arrays smooth_sem_pde reads, flattened (1-based ibool)."""
-        d = dict(nspec=self.nspec, nglob=self.nglob, ibool=self.ibool.ravel(),
-                 xstore=self.xstore, ystore=self.ystore, zstore=self.zstore,
-                 irregular_element_number=self.irregular_element_number,
-                 xix_regular=float(self.xix_regular), jacobian_regular=float(self.jacobian_regular),
-                 nspec_irregular=self.nspec_irregular)
-        for nm in ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz", "jacobian"):
-            d[nm] = getattr(self, nm).ravel()
-        return d
 
     def locate(self, x, y, z):
         """First element (ascending ispec) containing the point and its

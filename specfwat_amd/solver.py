@@ -21,6 +21,12 @@ F32 = np.float32
 
 def mass_matrices(mesh, rho, rho_vp, rho_vs, dt, stacey=True):
     """Inverted rmassx/y/z (float32): lumped mass plus the dt/2 Stacey term."""
+    _, rmx, rmy, rmz = mass_sums(mesh, rho, rho_vp, rho_vs, dt, stacey)
+    return (1.0 / rmx).astype(F32), (1.0 / rmy).astype(F32), (1.0 / rmz).astype(F32)
+
+
+def mass_sums(mesh, rho, rho_vp, rho_vs, dt, stacey=True):
+    """float64 rmass and rmassx/y/z (rmass plus the dt/2 Stacey term on absorbing faces)."""
     w = mesh.local_weights() * rho.astype(np.float64)
     rm = np.bincount(mesh.ibool0.ravel(), weights=w.ravel(), minlength=mesh.nglob)
     rmx, rmy, rmz = rm.copy(), rm.copy(), rm.copy()
@@ -38,7 +44,7 @@ def mass_matrices(mesh, rho, rho_vp, rho_vs, dt, stacey=True):
         for comp, arr in enumerate((rmx, rmy, rmz)):
             t = rvp * vn * n[..., comp] + rvs * (dto2 - vn * n[..., comp])
             np.add.at(arr, g.ravel(), (t * jw).ravel())
-    return (1.0 / rmx).astype(F32), (1.0 / rmy).astype(F32), (1.0 / rmz).astype(F32)
+    return rm, rmx, rmy, rmz
 
 
 class ElasticRun:
@@ -46,19 +52,28 @@ class ElasticRun:
 
     def __init__(self, mesh, rho, vp, vs, sources, receivers, dt, nstep, simulation_type=1,
                  save_forward=False, cij=None, stacey=True, compute_and_store_strain=False,
-                 approximate_hess=False, static_precision=0):
+                 approximate_hess=False, static_precision=0, materials=None):
+        """materials: the stored arrays of a solver database (rho, kappa, mu, rho_vp, rho_vs
+        per local point, rmassx/y/z per point, not inverted); when given they replace the
+        ones derived from rho, vp and vs, as restore_rmass (specfem_api.f90:393-411) and
+        read_mesh_databases (read_mesh_databases.f90:183-185, 455-487, 802-804) do."""
         self.mesh, self.dt, self.nstep = mesh, float(dt), int(nstep)
         self.stacey = bool(stacey)
         self.simulation_type = simulation_type
         self.aniso = cij is not None
         self.nrec = receivers.nrec if receivers is not None else 0
         self.nsrc = sources.nsources if sources is not None else 0
-        rho = np.ascontiguousarray(rho, F32).reshape(mesh.nspec, 125)
-        vp = np.ascontiguousarray(vp, F32).reshape(mesh.nspec, 125)
-        vs = np.ascontiguousarray(vs, F32).reshape(mesh.nspec, 125)
-        kappa, mu, rho_vp, rho_vs = moduli(rho, vp, vs)
+        loc = lambda a: np.ascontiguousarray(a, F32).reshape(mesh.nspec, 125)
+        if materials is not None:
+            rho, kappa, mu = loc(materials["rho"]), loc(materials["kappa"]), loc(materials["mu"])
+            rho_vp, rho_vs = loc(materials["rho_vp"]), loc(materials["rho_vs"])
+            rmx, rmy, rmz = (F32(1.0) / np.ascontiguousarray(materials[k], F32)
+                             for k in ("rmassx", "rmassy", "rmassz"))
+        else:
+            rho = loc(rho)
+            kappa, mu, rho_vp, rho_vs = moduli(rho, loc(vp), loc(vs))
+            rmx, rmy, rmz = mass_matrices(mesh, rho, rho_vp, rho_vs, self.dt, stacey)
         self.rho, self.kappa, self.mu = rho, kappa, mu
-        rmx, rmy, rmz = mass_matrices(mesh, rho, rho_vp, rho_vs, self.dt, stacey)
         self.h = _lib.HandleBox()
         g = mesh.gll
         i32 = lambda a: np.ascontiguousarray(a, np.int32)

@@ -1,0 +1,229 @@
+"""Solver databases (proc%06d_external_mesh.bin): record framing, the record
+order of read_mesh_databases.f90, and every stored array checked by what it
+means (geometry from the coordinates, masses from the FP64 oracle, moduli from
+the velocities, boundary faces by their areas) rather than by the writer that
+made the file."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from specfwat_amd import databases as D
+from specfwat_amd.gll import GLL
+from specfwat_amd.mesh import BoxMesh
+from specfwat_amd.model import hti_model
+from specfwat_amd.sources import Receivers, Sources
+
+NEX, NEY, NEZ, H = 4, 4, 3, 2500.0
+
+
+def _case(perturb=0.12, seed=0, aniso=False):
+    m = BoxMesh(NEX, NEY, NEZ, NEX * H, NEY * H, NEZ * H, perturb=perturb)
+    rng = np.random.default_rng(seed)
+    f = (1 + 0.03 * rng.standard_normal(m.nglob))[m.ibool0]
+    rho, vp, vs = (2700 * f).astype(np.float32), (6000 * f).astype(np.float32), (3500 * f).astype(np.float32)
+    cij = hti_model(m, rho, vp, vs, seed=seed + 5, block=2 * H) if aniso else None
+    return m, rho, vp, vs, cij
+
+
+@pytest.fixture(scope="module")
+def written(tmp_path_factory):
+    m, rho, vp, vs, _ = _case()
+    p = str(tmp_path_factory.mktemp("db") / "proc000000_external_mesh.bin")
+    D.write_external_mesh(p, m, rho, vp, vs, 0.02)
+    return p, m, rho, vp, vs
+
+
+def test_record_framing_subrecords_and_byte_order(tmp_path):
+    """Markers around every record; a record split into gfortran subrecords
+    (negative leading marker while it continues) reads as one; a big-endian
+    file (every 4-byte word swapped) reads the same."""
+    a = np.arange(10, dtype=np.int32)
+    p = tmp_path / "r.bin"
+    with open(p, "wb") as f:
+        f.write(struct.pack("<i", 4) + struct.pack("<i", 7) + struct.pack("<i", 4))
+        f.write(struct.pack("<i", -16) + a[:4].tobytes() + struct.pack("<i", 16))
+        f.write(struct.pack("<i", 24) + a[4:].tobytes() + struct.pack("<i", -24))
+    r = D._Records(str(p))
+    assert r.scalar(np.int32, "n") == 7
+    assert np.array_equal(r.ints("a", 10), a)
+    r.close()
+    be = tmp_path / "be.bin"
+    np.fromfile(p, "<i4").astype(">i4").tofile(be)
+    r = D._Records(str(be))
+    assert r.bo == ">" and r.scalar(np.int32, "n") == 7 and np.array_equal(r.ints("a", 10), a)
+    r.close()
+
+
+def test_round_trip_and_big_endian(written, tmp_path):
+    p, m, rho, vp, vs = written
+    d = D.read_external_mesh(p)
+    dm = D.DatabaseMesh(d)
+    assert (d["NSPEC_AB"], d["NGLOB_AB"], d["NSPEC_IRREGULAR"]) == (m.nspec, m.nglob, m.nspec_irregular)
+    for nm in ("ibool", "irregular_element_number", "xix", "xiy", "xiz", "etax", "etay", "etaz",
+               "gammax", "gammay", "gammaz", "jacobian", "abs_boundary_ispec", "abs_boundary_ijk",
+               "abs_boundary_normal", "abs_boundary_jacobian2Dw", "phase_ispec_inner", "xstore",
+               "ystore", "zstore", "jacobian_all"):
+        assert np.array_equal(getattr(dm, nm), getattr(m, nm)), nm
+    assert dm.xix_regular == m.xix_regular and dm.jacobian_regular == m.jacobian_regular
+    assert (dm.nspec_inner, dm.nspec_outer, dm.num_phase_ispec) == (m.nspec_inner, m.nspec_outer, m.num_phase_ispec)
+    assert dm.stacey and d["records"] == 74
+    assert np.array_equal(np.sort(np.unique(dm.ibool)), np.arange(1, m.nglob + 1))
+    # all 4-byte words: swapping the whole file gives the big-endian database
+    be = tmp_path / "be.bin"
+    np.fromfile(p, "<i4").astype(">i4").tofile(be)
+    d2 = D.read_external_mesh(str(be))
+    for k, v in d.items():
+        assert np.array_equal(np.asarray(v), np.asarray(d2[k])), k
+
+
+def test_geometry_records_are_the_inverse_jacobian_of_the_coordinates(written):
+    """xix..gammaz and jacobianstore against J = sum_l l_l'(xi) x_l computed from
+    the stored coordinates (any two of the nine records swapped fails), and
+    against the FP64 oracle's own mesh."""
+    p, m, *_ = written
+    dm = D.load_mesh(p)
+    g = GLL()
+    X = dm.xyz[dm.ibool0].reshape(dm.nspec, 5, 5, 5, 3)  # [e, k, j, i, c]
+    Dm = g.hprime64
+    dxi = np.einsum("il,ekjlc->ekjic", Dm, X)
+    deta = np.einsum("jl,eklic->ekjic", Dm, X)
+    dgam = np.einsum("kl,eljic->ekjic", Dm, X)
+    J = np.stack([dxi, deta, dgam], axis=-1).reshape(dm.nspec, 125, 3, 3)  # [c][d] = dx_c/dref_d
+    Ji = np.linalg.inv(J)  # [d][c] = dref_d/dx_c
+    irr = dm.irregular_element_number
+    assert (irr > 0).all()
+    sel = irr - 1
+    names = ["xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz"]
+    for q, nm in enumerate(names):
+        want = Ji[:, :, q // 3, q % 3]
+        got = getattr(dm, nm)[sel]
+        assert np.abs(got - want).max() < 2e-5 * np.abs(Ji).max(), nm
+    det = np.linalg.det(J)
+    assert np.abs(dm.jacobian[sel] / det - 1).max() < 2e-5
+    o = orc.Oracle(NEX, NEY, NEZ, NEX * H, NEY * H, NEZ * H, np.ones(m.nspec * 125, np.float32),
+                   np.ones(m.nspec * 125, np.float32), np.ones(m.nspec * 125, np.float32), dt=0.02,
+                   nstep=2, src_xyz=[[1.0, 1.0, -1.0]], src_mt=[[0] * 6], src_f0=[1.0],
+                   rec_xyz=[[1.0, 1.0, 0.0]], perturb=0.12)
+    om = o.mesh()
+    assert np.array_equal(om["ibool"] + 1, dm.ibool.ravel())
+    jinv = om["jinv"].reshape(dm.nspec, 125, 9)
+    for q, nm in enumerate(names):
+        assert np.abs(getattr(dm, nm)[sel] - jinv[..., q]).max() < 1e-6 * np.abs(jinv).max(), nm
+    assert np.abs(dm.xyz - om["xyz"]).max() < 1e-3
+
+
+def test_mass_and_material_records(written):
+    """rmass = sum rho J w; rmassx/y/z = the FP64 oracle's Stacey-augmented masses;
+    rho_vp^2 = rho (kappa + 4/3 mu), rho_vs^2 = rho mu; velocities recovered."""
+    p, m, rho, vp, vs = written
+    dm = D.load_mesh(p)
+    d = dm.db
+    g = GLL()
+    ii = np.arange(125)
+    w3 = g.w[ii % 5] * g.w[(ii // 5) % 5] * g.w[ii // 25]
+    want = np.bincount(dm.ibool0.ravel(), (dm.jacobian_all.astype(np.float64) * w3 *
+                                            d["rhostore"].astype(np.float64)).ravel(), dm.nglob)
+    assert np.abs(d["rmass"] / want - 1).max() < 2e-6
+    o = orc.Oracle(NEX, NEY, NEZ, NEX * H, NEY * H, NEZ * H, rho.ravel(), vp.ravel(), vs.ravel(),
+                   dt=0.02, nstep=2, src_xyz=[[1.0, 1.0, -1.0]], src_mt=[[0] * 6], src_f0=[1.0],
+                   rec_xyz=[[1.0, 1.0, 0.0]], perturb=0.12)
+    for c, inv in zip("xyz", o.rmass_inv()):
+        assert np.abs(d["rmass" + c] * inv.astype(np.float64) - 1).max() < 2e-6, c
+    assert not np.allclose(d["rmassx"], d["rmassz"])  # the Stacey terms differ by component
+    r64 = d["rhostore"].astype(np.float64)
+    assert np.abs(d["rho_vp"].astype(np.float64) ** 2 / (r64 * (d["kappastore"] + 4.0 / 3.0 * d["mustore"])) - 1).max() < 1e-5
+    assert np.abs(d["rho_vs"].astype(np.float64) ** 2 / (r64 * d["mustore"]) - 1).max() < 1e-5
+    mat = dm.materials()
+    assert np.abs(mat["rho_vp"].astype(np.float64) ** 2 / (r64 * (mat["kappa"] + 4.0 / 3.0 * mat["mu"])) - 1).max() < 1e-5
+    assert np.abs(mat["rho_vs"].astype(np.float64) ** 2 / (r64 * mat["mu"]) - 1).max() < 1e-5
+    assert np.array_equal(mat["rho"], rho)
+    for c, inv in zip("xyz", o.rmass_inv()):
+        assert np.abs(mat["rmass" + c] * inv.astype(np.float64) - 1).max() < 2e-6, c
+    r2, vp2, vs2 = dm.model()
+    assert np.array_equal(r2, rho) and np.abs(vp2 / vp - 1).max() < 1e-6 and np.abs(vs2 / vs - 1).max() < 1e-6
+
+
+def test_boundary_face_records(written):
+    """Absorbing and free-surface faces: unit outward normals; the weighted face
+    Jacobians of each flat side sum to that side's area; ibelm lists name the
+    elements of each side; surface flags cover the box's outer points."""
+    p, m, *_ = written
+    d = D.read_external_mesh(p)
+    lx, ly, lz = NEX * H, NEY * H, NEZ * H
+    for pre, areas in (("abs_boundary", [ly * lz, ly * lz, lx * lz, lx * lz, lx * ly]),
+                       ("free_surface", [lx * ly])):
+        n = d[pre + "_normal"].astype(np.float64)
+        assert np.abs(np.linalg.norm(n, axis=-1) - 1).max() < 1e-6
+        jw = d[pre + "_jacobian2Dw"].astype(np.float64)
+        # group faces by their dominant outward normal
+        axis = np.argmax(np.abs(n.mean(axis=1)), axis=1)
+        sign = np.sign(n.mean(axis=1)[np.arange(len(axis)), axis])
+        got = []
+        for ax, sg in ((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1)):
+            s = (axis == ax) & (sign == sg)
+            if s.any():
+                got.append(jw[s].sum())
+        assert np.allclose(got, areas, rtol=1e-5), (pre, got)
+    assert np.array_equal(d["ibelm_top"], d["free_surface_ispec"])
+    assert d["nspec2D_bottom"] == NEX * NEY and d["nspec2D_xmin"] == NEY * NEZ
+    onb = d["iglob_is_surface_external_mesh"]
+    x, y, z = d["xstore"], d["ystore"], d["zstore"]
+    tol = 1e-3
+    outer = ((np.abs(x) < tol) | (np.abs(x - lx) < tol) | (np.abs(y) < tol) | (np.abs(y - ly) < tol)
+             | (np.abs(z + lz) < tol) | (np.abs(z) < tol))
+    assert np.array_equal(onb, outer)
+    xadj, adj = d["neighbors_xadj"], d["neighbors_adjncy"]
+    assert xadj[0] == 0 and xadj[-1] == len(adj) and xadj[1] - xadj[0] == 7  # corner element
+
+
+def test_sources_and_receivers_located_on_the_database_mesh(written):
+    p, m, *_ = written
+    dm = D.load_mesh(p)
+    rng = np.random.default_rng(4)
+    sx = [[4300.0, 5700.0, -3375.0]]
+    mt = [list(1e15 * rng.standard_normal(6))]
+    rx = [[1000.0, 2000.0, 0.0], [7000.0, 3000.0, 0.0], [5100.0, 4900.0, -10.0]]
+    s1, s2 = Sources(m, sx, mt, [1.0]), Sources(dm, sx, mt, [1.0])
+    r1, r2 = Receivers(m, rx), Receivers(dm, rx)
+    assert np.array_equal(s1.ispec, s2.ispec) and np.array_equal(r1.ispec, r2.ispec)
+    assert np.abs(s1.sourcearrays - s2.sourcearrays).max() < 1e-5 * np.abs(s1.sourcearrays).max()
+    assert np.abs(r1.hlagrange - r2.hlagrange).max() < 1e-5
+    with pytest.raises(ValueError):
+        dm.locate(-100.0, 10.0, -10.0)
+
+
+def test_anisotropic_records_and_flags(tmp_path):
+    m, rho, vp, vs, cij = _case(aniso=True, seed=2)
+    p = str(tmp_path / "a.bin")
+    D.write_external_mesh(p, m, rho, vp, vs, 0.02, cij=cij)
+    with pytest.raises(D.DatabaseError, match="ispec_is_inner"):
+        D.read_external_mesh(p)  # ANISOTROPY off: the first c_ij record is taken for ispec_is_inner
+    dm = D.load_mesh(p, anisotropy=True)
+    assert np.array_equal(dm.cij(), np.asarray(cij, np.float32).reshape(m.nspec, 125, 21))
+
+
+def test_corrupt_files_are_refused(written, tmp_path):
+    p = written[0]
+    raw = bytearray(open(p, "rb").read())
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(bytes(raw[: len(raw) // 2]))
+    with pytest.raises(D.DatabaseError, match="truncated|ends before"):
+        D.read_external_mesh(str(cut))
+    # the 9999 sentinel: find it after the three domain-flag records
+    words = np.frombuffer(bytes(raw), "<i4").copy()
+    pos = int(np.nonzero(words == 9999)[0][0])
+    words[pos] = 1234
+    bad = tmp_path / "bad.bin"
+    words.tofile(bad)
+    with pytest.raises(D.DatabaseError, match="position 1"):
+        D.read_external_mesh(str(bad))
+    words[pos] = 9999
+    words[0] = 8  # first marker
+    words.tofile(bad)
+    with pytest.raises(D.DatabaseError):
+        D.read_external_mesh(str(bad))
+    with pytest.raises(D.DatabaseError):
+        D.read_external_mesh(p, stacey_absorbing_conditions=False)  # rmassx taken for the sentinel
