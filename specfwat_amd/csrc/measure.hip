@@ -1199,15 +1199,28 @@ __global__ void __launch_bounds__(256) k_ws_prep(int nstep, double dt, double t0
   }
 }
 
-// sliding_cc (win_sel.f90:76-100): one wavefront per window start i; the Hann
-// tapered windows of d and s in LDS, the full correlation with every lag's sum
-// in the reference's order (lanes over lags), first maximum, then
-// xcorr_shift_coe's coefficient with its three sums in order on one lane
-__global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw, double dt,
+// sliding_cc (win_sel.f90:76-100), correlation part: one wavefront per window
+// start i; the Hann tapered windows of d and s in LDS, the full correlation
+// with every lag's sum in the reference's order, and its first maximum; the
+// lag goes to ish, the coefficient is k_ws_coef's.  Lags come in blocks of
+// four consecutive lags (a sliding register window over s, as k_cc_corr,
+// unrolled by four so the window needs no register moves).  Block b costs
+// about min(4 b, 2 nlen - 4 b) multiply-adds and a wavefront runs as long as
+// its slowest lane, so the blocks are dealt to the lanes in descending cost
+// (the block holding lag nlen - 1 first, then alternately its neighbours):
+// the 64 dearest blocks run together, then the next 64
+__device__ __forceinline__ int ws_block_of_rank(int r, int c, int nb) {
+  const int L = c, R = nb - 1 - c, m = L < R ? L : R;
+  if (r == 0) return c;
+  if (r <= 2 * m) { const int k = (r + 1) / 2; return (r & 1) ? c - k : c + k; }
+  const int k = r - m;
+  return L > R ? c - k : c + k;
+}
+
+__global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
                                                              const double* __restrict__ wt,
                                                              const double* __restrict__ Dn,
-                                                             const double* __restrict__ Sn, double* __restrict__ ts,
-                                                             double* __restrict__ cc) {
+                                                             const double* __restrict__ Sn, int* __restrict__ ish_out) {
   extern __shared__ double wsh[];
   const int nwv = blockDim.x / 64;  // one window start per wavefront, 2 nlen + 7 doubles of LDS each
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, t = blockIdx.y;
@@ -1223,16 +1236,26 @@ __global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
   }
   __syncthreads();
   if (i >= nw) return;
-  // four consecutive lags per lane with a sliding register window over s (as k_cc_corr)
   double best = -INFINITY;
   int bm = 0x7fffffff;
-  const int nlag = 2 * nlen - 1;
-  for (int mb = 4 * lane; mb < nlag; mb += 4 * 64) {
+  const int nlag = 2 * nlen - 1, nb = (nlag + 3) / 4, cb = (nlen - 1) / 4;
+  for (int r = lane; r < nb; r += 64) {
+    const int mb = 4 * ws_block_of_rank(r, cb, nb);
     const int lo = mb - (nlen - 1) > 0 ? mb - (nlen - 1) : 0, hi = mb + 3 < nlen - 1 ? mb + 3 : nlen - 1;
     const int off = nlen - 1 - mb;  // s index of lag mb + q at i1: i1 + off - q, in [-3, nlen + 3]
     double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
     double r0 = sw[lo + off], r1 = sw[lo + off - 1], r2 = sw[lo + off - 2], r3 = sw[lo + off - 3];
-    for (int i1 = lo; i1 <= hi; ++i1) {
+    int i1 = lo;
+    for (; i1 + 3 <= hi; i1 += 4) {
+      const double d0 = dw[i1], d1 = dw[i1 + 1], d2 = dw[i1 + 2], d3 = dw[i1 + 3];
+      const double s0 = sw[i1 + 1 + off], s1 = sw[i1 + 2 + off], s2 = sw[i1 + 3 + off], s3 = sw[i1 + 4 + off];
+      c0 = c0 + d0 * r0; c1 = c1 + d0 * r1; c2 = c2 + d0 * r2; c3 = c3 + d0 * r3;
+      c0 = c0 + d1 * s0; c1 = c1 + d1 * r0; c2 = c2 + d1 * r1; c3 = c3 + d1 * r2;
+      c0 = c0 + d2 * s1; c1 = c1 + d2 * s0; c2 = c2 + d2 * r0; c3 = c3 + d2 * r1;
+      c0 = c0 + d3 * s2; c1 = c1 + d3 * s1; c2 = c2 + d3 * s0; c3 = c3 + d3 * r0;
+      r0 = s3; r1 = s2; r2 = s1; r3 = s0;
+    }
+    for (; i1 <= hi; ++i1) {
       const double dv = dw[i1];
       c0 = c0 + dv * r0;
       c1 = c1 + dv * r1;
@@ -1244,35 +1267,40 @@ __global__ void __launch_bounds__(256) k_ws_sliding(int nstep, int nlen, int nw,
       r0 = sw[i1 + 1 + off];
     }
     const double cv[4] = {c0, c1, c2, c3};
-    for (int q = 0; q < 4; ++q)
-      if (mb + q < nlag && cv[q] > best) { best = cv[q]; bm = mb + q; }
+    for (int q = 0; q < 4; ++q)  // blocks arrive out of lag order: ties go to the smaller lag
+      if (mb + q < nlag && (cv[q] > best || (cv[q] == best && mb + q < bm))) { best = cv[q]; bm = mb + q; }
   }
   for (int o = 32; o > 0; o >>= 1) {
     const double v2 = __shfl_xor(best, o);
     const int m2 = __shfl_xor(bm, o);
     if (v2 > best || (v2 == best && m2 < bm)) { best = v2; bm = m2; }
   }
-  if (lane != 0) return;
-  const int ish = bm + 1 - nlen;
-  // the three sums of xcorr_shift_coe, each in the reference's order, in one
-  // pass whose LDS reads are issued eight at a time
+  if (lane == 0) ish_out[(size_t)t * nw + i] = bm + 1 - nlen;
+}
+
+// xcorr_shift_coe's coefficient (win_sel.f90:439-469) of every window start,
+// one window start per lane: a = sum(d s_shifted), b = sum(d^2), c =
+// sum(s_shifted^2) over the tapered window with d's emptied edge zeroed, each
+// sum in the reference's order; neighbouring lanes read neighbouring samples
+__global__ void __launch_bounds__(256) k_ws_coef(int nstep, int nlen, int nw, double dt,
+                                                 const double* __restrict__ wt, const double* __restrict__ Dn,
+                                                 const double* __restrict__ Sn, const int* __restrict__ ish_in,
+                                                 double* __restrict__ ts, double* __restrict__ cc) {
+  const int t = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int ish = ish_in[(size_t)t * nw + i];
+  const double* d = Dn + (size_t)t * nstep + i;
+  const double* s = Sn + (size_t)t * nstep + i;
+  // d's kept range [jd0, jd1) and s's shifted range [js0, js1) within 0 .. nlen
+  const int jd0 = ish > 0 ? ish : 0, jd1 = ish < 0 ? nlen + ish : nlen;
+  const int js0 = ish > 0 ? ish : 0, js1 = ish < 0 ? nlen + ish : nlen;
   double a = 0.0, b = 0.0, c = 0.0;
-  constexpr int U = 8;
-  for (int j0 = 0; j0 < nlen; j0 += U) {
-    double dv[U], sv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int j = j0 + u, k = j - ish;
-      sv[u] = (j < nlen && k >= 0 && k < nlen) ? sw[k] : 0.0;
-      dv[u] = (j >= nlen || (ish > 0 && j < ish) || (ish < 0 && j >= nlen + ish)) ? 0.0 : dw[j];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (j0 + u >= nlen) break;
-      a = a + dv[u] * sv[u];
-      b = b + dv[u] * dv[u];
-      c = c + sv[u] * sv[u];
-    }
+  for (int j = 0; j < nlen; ++j) {
+    const double dv = (j >= jd0 && j < jd1) ? d[j] * wt[j] : 0.0;
+    const double sv = (j >= js0 && j < js1) ? s[j - ish] * wt[j - ish] : 0.0;
+    a = a + dv * sv;
+    b = b + dv * dv;
+    c = c + sv * sv;
   }
   ts[(size_t)t * nw + i] = (double)ish * dt;
   cc[(size_t)t * nw + i] = a / sqrt(b * c);
@@ -2137,6 +2165,7 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   double* d_noise = dev_alloc<double>(T);
   double* d_ts = dev_alloc<double>((size_t)nw * T);
   double* d_cc = dev_alloc<double>((size_t)nw * T);
+  int* d_ish = dev_alloc<int>((size_t)nw * T);
   int* d_nwin = dev_alloc<int>(T);
   double* d_win = dev_alloc<double>(2 * (size_t)*maxwin * T);
   unsigned char* d_good = dev_alloc<unsigned char>((size_t)nw * T);
@@ -2169,8 +2198,9 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
                      d_Dn, d_Sn, d_noise);
   const int nwv = nlen <= 1020 ? 4 : (nlen <= 2044 ? 2 : 1);  // at most 64 KB of LDS per workgroup
   hipLaunchKernelGGL(k_ws_sliding, dim3((unsigned)((nw + nwv - 1) / nwv), (unsigned)T), dim3(64 * nwv),
-                     (size_t)nwv * (2 * nlen + 7) * sizeof(double), s, N, nlen, nw, *dt, d_wt, d_Dn, d_Sn, d_ts,
-                     d_cc);
+                     (size_t)nwv * (2 * nlen + 7) * sizeof(double), s, N, nlen, nw, d_wt, d_Dn, d_Sn, d_ish);
+  hipLaunchKernelGGL(k_ws_coef, dim3((unsigned)((nw + 255) / 256), (unsigned)T), dim3(256), 0, s, N, nlen, nw, *dt,
+                     d_wt, d_Dn, d_Sn, d_ish, d_ts, d_cc);
   if (split)
     hipLaunchKernelGGL(k_ws_env, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, nfe, logMe, d_twe, d_Sn, d_g01,
                        d_g01 + (size_t)T * (Me + 1), d_env);
@@ -2185,7 +2215,7 @@ void fwat_win_sel_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw*
   HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
   g_meas_ms = ms;
   dev_free(d_obs); dev_free(d_syn); dev_free(d_tp); dev_free(d_dist); dev_free(d_wt); dev_free(d_obsT);
-  dev_free(d_synT); dev_free(d_Dn); dev_free(d_Sn); dev_free(d_noise); dev_free(d_ts); dev_free(d_cc);
+  dev_free(d_synT); dev_free(d_Dn); dev_free(d_Sn); dev_free(d_noise); dev_free(d_ts); dev_free(d_cc); dev_free(d_ish);
   dev_free(d_nwin); dev_free(d_win); dev_free(d_good); dev_free(d_env); dev_free(d_stl); dev_free(d_g01);
   dev_free(d_twe);
   for (int t = 0; t < T; ++t)

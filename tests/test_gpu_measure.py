@@ -204,8 +204,8 @@ def test_calculate_adjoint_sources_dispatch():
         print(f"imeasure_type {im}: misfit {tot:.6e}")
 
 
-@pytest.mark.parametrize("split", [0, 1])
-def test_window_selection_matches_oracle(split):
+@pytest.mark.parametrize("split,len_fac", [(0, 2.0), (1, 2.0), (0, 4.5)])
+def test_window_selection_matches_oracle(split, len_fac):
     """win_sel_type%select_windows for every trace: the same windows as the
     oracle (bit-identical correlation sums and decisions), including traces
     split by a time-shift jump and a trace with nothing to select; with phase
@@ -226,15 +226,18 @@ def test_window_selection_matches_oracle(split):
             obs[:, r, c] = w(a1 + rng.uniform(-0.1, 0.1), 0.5) + 0.7 * w(a2 + jump, 0.45) \
                 + 0.003 * rng.standard_normal(nstep)
     obs[:, 4, 1] = 0.003 * rng.standard_normal(nstep)  # nothing to select
-    got = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, T, is_split_phases=bool(split))
+    # sliding_win_len_fac 4.5: windows of 451 samples, 901 lags, two rounds of lag-block pairs per lane
+    got = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, T, is_split_phases=bool(split),
+                                sliding_win_len_fac=len_fac)
     counts = []
     for r in range(nrec):
         for c in range(3):
-            ref, _, _ = orc.win_sel(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], T, is_split_phases=split)
+            ref, _, _ = orc.win_sel(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], T, is_split_phases=split,
+                                    sliding_win_len_fac=len_fac)
             np.testing.assert_array_equal(got[3 * r + c], ref)
             counts.append(len(ref))
-    assert max(counts) >= 2 and min(counts) == 0
-    print(f"split {split}: windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")
+    assert max(counts) >= (2 if len_fac < 3 else 1) and min(counts) == 0
+    print(f"split {split} len_fac {len_fac}: windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")
 
 
 @pytest.mark.parametrize("nstep", [3000, 2999])
