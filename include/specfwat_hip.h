@@ -130,6 +130,13 @@ void prepare_cleanup_device_(long *Mesh_pointer, int *ACOUSTIC_SIMULATION, int *
 /* whole-run source time functions (NSOURCES,NSTEP) and adjoint traces (NDIM,nrec,NSTEP) */
 void fwat_set_stf_(long *Mesh_pointer, double *h_stf, int *NSTEP);
 void fwat_set_adjoint_sources_(long *Mesh_pointer, realw *h_adj, int *NSTEP);
+/* FK injection of a teleseismic incident field into the forward run's Stacey term; replaces the
+ * Veloc_FK / Tract_FK arrays that couple_with_injection_prepare_boundary_fwat builds
+ * (src/shared/fk_coupling.f90:71-297) and SPECFEM's Stacey routine injects: veloc_fk, tract_fk
+ * [nsamp][npt][NDIM], npt = num_abs_boundary_faces*NGLLSQUARE in face order, sample k at step
+ * 1 + k*resamp, linear in between; nsamp = 0 switches it off */
+void fwat_set_fk_injection_(long *Mesh_pointer, int *npt, int *nsamp, int *resamp, const realw *veloc_fk,
+                            const realw *tract_fk);
 /* a whole SIMULATION_TYPE 1 or 3 run on the device; fused 1 = fused Newmark and
  * two-field force pass with in-pass fp64 kernel accumulation, 0 = SPECFEM stage order */
 void fwat_iterate_time_(long *Mesh_pointer, int *SIMULATION_TYPE, realw *DT, int *fused, int *timed);

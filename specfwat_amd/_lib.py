@@ -46,6 +46,7 @@ SIGNATURES = {
     "prepare_cleanup_device_": "Liii",
     "fwat_set_stf_": "LDi",
     "fwat_set_adjoint_sources_": "LFi",
+    "fwat_set_fk_injection_": "LiiiFF",
     "fwat_iterate_time_": "Lifii",
     "fwat_last_force_time_": "LDI",
     "fwat_last_assembly_time_": "LDI",

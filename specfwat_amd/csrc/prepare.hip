@@ -521,6 +521,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   dev_free(m->d_displ); dev_free(m->d_veloc); dev_free(m->d_accel);
   dev_free(m->d_b_displ); dev_free(m->d_b_veloc); dev_free(m->d_b_accel);
   dev_free(m->d_face_iglob); dev_free(m->d_face_coef); dev_free(m->d_stacey_store);
+  dev_free(m->d_fk_veloc); dev_free(m->d_fk_tract);
   dev_free(m->d_src_iglob); dev_free(m->d_sourcearrays); dev_free(m->d_stf); dev_free(m->d_stf_step);
   dev_free(m->d_rec_iglob); dev_free(m->d_rec_h); dev_free(m->d_seismograms); dev_free(m->d_adj);
   dev_free(m->d_adj_step);

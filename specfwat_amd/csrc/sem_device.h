@@ -110,6 +110,12 @@ struct SemMesh {
   realw* d_face_coef = nullptr;
   realw* d_stacey_store = nullptr;
   size_t stacey_store_steps = 0;
+  // FK injection of an incident wavefield (forward runs): velocity and
+  // traction per face point, [fk_nsamp][nfaces*25][3] each; sample k belongs
+  // to step 1 + k * fk_resamp, steps between samples interpolate linearly
+  realw* d_fk_veloc = nullptr;
+  realw* d_fk_tract = nullptr;
+  int fk_nsamp = 0, fk_resamp = 1;
 
   // point sources
   int nsources = 0;

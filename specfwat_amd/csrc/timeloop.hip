@@ -145,20 +145,49 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
                      hess_kl, dtk);
 }
 
+// Incident wavefield of one step for FK injection: rows k and k + 1 of the
+// stored velocity and traction and the weight w of row k + 1 (no injection
+// when v0 is null)
+struct FkStep {
+  const realw *v0 = nullptr, *v1 = nullptr, *t0 = nullptr, *t1 = nullptr;
+  realw w = 0;
+};
+
+// Stacey term of face point q, SPECFEM's compute_stacey_viscoelastic with
+// injection: the absorbing traction of the velocity relative to the
+// incident one, minus the incident traction, times jacobian2Dw
+__device__ __forceinline__ void stacey_point(size_t q, const realw* __restrict__ C, realw vx, realw vy, realw vz,
+                                             const FkStep& fk, realw& tx, realw& ty, realw& tz) {
+  const realw nx = C[0], ny = C[1], nz = C[2], jw = C[3], rvp = C[4], rvs = C[5];
+  realw ix = 0, iy = 0, iz = 0;
+  if (fk.v0) {
+    const realw w0 = (realw)1 - fk.w, w1 = fk.w;
+    vx -= w0 * fk.v0[3 * q] + w1 * fk.v1[3 * q];
+    vy -= w0 * fk.v0[3 * q + 1] + w1 * fk.v1[3 * q + 1];
+    vz -= w0 * fk.v0[3 * q + 2] + w1 * fk.v1[3 * q + 2];
+    ix = w0 * fk.t0[3 * q] + w1 * fk.t1[3 * q];
+    iy = w0 * fk.t0[3 * q + 1] + w1 * fk.t1[3 * q + 1];
+    iz = w0 * fk.t0[3 * q + 2] + w1 * fk.t1[3 * q + 2];
+  }
+  const realw vn = vx * nx + vy * ny + vz * nz;
+  tx = (rvp * vn * nx + rvs * (vx - vn * nx) - ix) * jw;
+  ty = (rvp * vn * ny + rvs * (vy - vn * ny) - iy) * jw;
+  tz = (rvp * vn * nz + rvs * (vz - vn * nz) - iz) * jw;
+}
+
 // Stacey traction of one field written per face point (no scatter):
-// out[q] = (rho_vp (v.n) n + rho_vs (v - (v.n) n)) * jacobian2Dw
+// out[q] = (rho_vp (v.n) n + rho_vs (v - (v.n) n)) * jacobian2Dw, with v
+// relative to the incident field and its traction subtracted under FK injection
 __global__ void k_stacey_face(int nfp, const int* __restrict__ figlob, const realw* __restrict__ fcoef,
-                              const realw* __restrict__ v, realw* __restrict__ out) {
+                              const realw* __restrict__ v, realw* __restrict__ out, FkStep fk) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nfp) return;
   const size_t g = (size_t)figlob[q];
-  const realw* C = fcoef + 8 * (size_t)q;
-  const realw nx = C[0], ny = C[1], nz = C[2], jw = C[3], rvp = C[4], rvs = C[5];
-  const realw vx = v[3 * g], vy = v[3 * g + 1], vz = v[3 * g + 2];
-  const realw vn = vx * nx + vy * ny + vz * nz;
-  out[3 * (size_t)q] = (rvp * vn * nx + rvs * (vx - vn * nx)) * jw;
-  out[3 * (size_t)q + 1] = (rvp * vn * ny + rvs * (vy - vn * ny)) * jw;
-  out[3 * (size_t)q + 2] = (rvp * vn * nz + rvs * (vz - vn * nz)) * jw;
+  realw tx, ty, tz;
+  stacey_point((size_t)q, fcoef + 8 * (size_t)q, v[3 * g], v[3 * g + 1], v[3 * g + 2], fk, tx, ty, tz);
+  out[3 * (size_t)q] = tx;
+  out[3 * (size_t)q + 1] = ty;
+  out[3 * (size_t)q + 2] = tz;
 }
 
 // Start of a reconstruction: the saved last state (u_N, v_N, a_N) becomes
@@ -226,7 +255,7 @@ __global__ void k_kernels_from_strain(int nspec, const realw* __restrict__ ea_al
 // Stacey: mode 0 absorb, 1 absorb + store, 2 replay a stored record
 __global__ void k_stacey(int nfp, const int* __restrict__ figlob, const realw* __restrict__ fcoef,
                          const realw* __restrict__ v, realw* __restrict__ a, realw* __restrict__ rec,
-                         int mode) {
+                         int mode, FkStep fk) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nfp) return;
   const size_t g = (size_t)figlob[q];
@@ -234,13 +263,7 @@ __global__ void k_stacey(int nfp, const int* __restrict__ figlob, const realw* _
   if (mode == 2) {
     tx = rec[3 * (size_t)q]; ty = rec[3 * (size_t)q + 1]; tz = rec[3 * (size_t)q + 2];
   } else {
-    const realw* C = fcoef + 8 * (size_t)q;
-    const realw nx = C[0], ny = C[1], nz = C[2], jw = C[3], rvp = C[4], rvs = C[5];
-    const realw vx = v[3 * g], vy = v[3 * g + 1], vz = v[3 * g + 2];
-    const realw vn = vx * nx + vy * ny + vz * nz;
-    tx = (rvp * vn * nx + rvs * (vx - vn * nx)) * jw;
-    ty = (rvp * vn * ny + rvs * (vy - vn * ny)) * jw;
-    tz = (rvp * vn * nz + rvs * (vz - vn * nz)) * jw;
+    stacey_point((size_t)q, fcoef + 8 * (size_t)q, v[3 * g], v[3 * g + 1], v[3 * g + 2], fk, tx, ty, tz);
     if (mode == 1) { rec[3 * (size_t)q] = tx; rec[3 * (size_t)q + 1] = ty; rec[3 * (size_t)q + 2] = tz; }
   }
   unsafeAtomicAdd(a + 3 * g, -tx);
@@ -343,6 +366,21 @@ void stage_corrector(SemMesh* m, bool backward, realw dt) {
                      m->d_rmass_inv, 0.5f * sdt);
 }
 
+// the incident field of forward step it (1-based) under FK injection
+FkStep fk_step(const SemMesh* m, int it) {
+  FkStep f;
+  if (m->simulation_type != 1 || m->fk_nsamp == 0 || it < 1) return f;
+  const int nfp3 = m->nfaces * 75;
+  const int k = std::min((it - 1) / m->fk_resamp, m->fk_nsamp - 1);
+  const int k1 = std::min(k + 1, m->fk_nsamp - 1);
+  f.v0 = m->d_fk_veloc + (size_t)k * nfp3;
+  f.v1 = m->d_fk_veloc + (size_t)k1 * nfp3;
+  f.t0 = m->d_fk_tract + (size_t)k * nfp3;
+  f.t1 = m->d_fk_tract + (size_t)k1 * nfp3;
+  f.w = k1 == k ? (realw)0 : (realw)((it - 1) - k * m->fk_resamp) / (realw)m->fk_resamp;
+  return f;
+}
+
 void stage_stacey(SemMesh* m, bool backward, int mode, size_t record) {
   if (!m->absorbing || m->nfaces == 0) return;
   const int nfp = m->nfaces * 25;
@@ -354,8 +392,9 @@ void stage_stacey(SemMesh* m, bool backward, int mode, size_t record) {
     if (record >= m->stacey_store_steps) SEM_FAIL("Stacey record %zu outside store of %zu steps", record, m->stacey_store_steps);
     rec = m->d_stacey_store + record * (size_t)nfp * 3;
   }
+  const FkStep fk = backward ? FkStep{} : fk_step(m, (int)record + 1);
   hipLaunchKernelGGL(k_stacey, dim3(nblk(nfp)), dim3(TPB), 0, m->stream, nfp, m->d_face_iglob,
-                     m->d_face_coef, v, a, rec, mode);
+                     m->d_face_coef, v, a, rec, mode, fk);
 }
 
 void stage_sources(SemMesh* m, bool backward, const double* d_stf_row) {
@@ -371,11 +410,12 @@ void stage_adjoint_sources(SemMesh* m, const realw* d_adj_row) {
                      m->nrec, m->d_rec_iglob, m->d_rec_h, d_adj_row, m->d_accel);
 }
 
-void stage_stacey_face(SemMesh* m, const realw* v, realw* out) {
+// it: the forward step (1-based) whose incident field is injected, 0 for none
+void stage_stacey_face(SemMesh* m, const realw* v, realw* out, int it = 0) {
   if (!m->absorbing || m->nfaces == 0) return;
   const int nfp = m->nfaces * 25;
   hipLaunchKernelGGL(k_stacey_face, dim3(nblk(nfp)), dim3(TPB), 0, m->stream, nfp, m->d_face_iglob,
-                     m->d_face_coef, v, out);
+                     m->d_face_coef, v, out, fk_step(m, it));
 }
 
 // Assembly + Newmark of points 0..nglob-1, or of slab `slab`'s point list.
@@ -655,6 +695,33 @@ void fwat_set_stf_(long* Mesh_pointer, double* h_stf, int* NSTEP) {
   HIP_CHECK(hipStreamSynchronize(m->stream));
 }
 
+// FK injection for teleseismic events: the incident velocity and traction
+// (Veloc_FK, Tract_FK of couple_with_injection_prepare_boundary_fwat,
+// fk_coupling.f90:71-297, layout [nsamp][npt][NDIM], npt = nfaces*25 face
+// points in absorbing-face order) that the forward run's Stacey term injects;
+// sample k belongs to step 1 + k*resamp, steps between samples interpolate
+// linearly.  nsamp = 0 switches injection off.  The series stays in HBM.
+void fwat_set_fk_injection_(long* Mesh_pointer, int* npt, int* nsamp, int* resamp, const realw* veloc_fk,
+                            const realw* tract_fk) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  dev_free(m->d_fk_veloc);
+  dev_free(m->d_fk_tract);
+  m->d_fk_veloc = m->d_fk_tract = nullptr;
+  m->fk_nsamp = 0;
+  if (*nsamp == 0) return;
+  if (!m->absorbing || m->nfaces == 0) SEM_FAIL("FK injection needs the absorbing faces");
+  if (*npt != m->nfaces * 25) SEM_FAIL("FK injection: npt %d, the mesh has %d face points", *npt, m->nfaces * 25);
+  if (*nsamp < 0 || *resamp < 1) SEM_FAIL("FK injection: nsamp %d, resamp %d", *nsamp, *resamp);
+  if ((size_t)(m->nstep - 1) / (size_t)*resamp >= (size_t)*nsamp)
+    SEM_FAIL("FK injection: %d samples every %d steps do not cover NSTEP %d", *nsamp, *resamp, m->nstep);
+  const size_t n = (size_t)*nsamp * *npt * 3;
+  m->d_fk_veloc = dev_upload(veloc_fk, n, m->stream);
+  m->d_fk_tract = dev_upload(tract_fk, n, m->stream);
+  m->fk_nsamp = *nsamp;
+  m->fk_resamp = *resamp;
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
 void fwat_set_adjoint_sources_(long* Mesh_pointer, realw* h_adj, int* NSTEP) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (*NSTEP != m->nstep) SEM_FAIL("NSTEP mismatch");
@@ -724,7 +791,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         if (tm) tm->mark(s);
         sem_launch_brick(m, 1, F0, F0, it == N, 0.0);
         if (tm) { tm->mark(s); m->last_force_launches += 1; }
-        if (face) stage_stacey_face(m, m->d_veloc, face);
+        if (face) stage_stacey_face(m, m->d_veloc, face, it);
         stage_seismo(m, it - 1);
         if (ta) ta->mark(s);
         sem_launch_halo(m, 1, F0, F0, it == N, 0.0);
@@ -740,7 +807,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         realw* face = nullptr;
         if (m->absorbing && m->nfaces) {
           face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
-          stage_stacey_face(m, m->d_veloc, face);
+          stage_stacey_face(m, m->d_veloc, face, it);
         }
         stage_seismo(m, it - 1);
         AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
@@ -764,7 +831,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         realw* face = nullptr;
         if (m->absorbing && m->nfaces) {
           face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
-          stage_stacey_face(m, m->d_veloc, face);
+          stage_stacey_face(m, m->d_veloc, face, it);
         }
         stage_seismo(m, it - 1);
         AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,

@@ -139,6 +139,22 @@ class ElasticRun:
         return int(nh), int(ns), int(nc)
 
     # ------------------------------------------------------------ runs
+    def set_fk_injection(self, veloc, tract, resamp=1):
+        """Incident wavefield injected through the Stacey faces of a forward run
+        (teleseismic FK coupling, fk_coupling.f90:71-297): veloc, tract
+        (nsamp, nfaces*25, 3) velocity and traction at the absorbing-face points
+        in face order, sample k at step 1 + k*resamp (time k*resamp*dt, the
+        source-time convention), linear in between.  veloc=None switches it off."""
+        if veloc is None:
+            _lib.call("fwat_set_fk_injection_", self.h, 0, 0, 1, None, None)
+            return
+        v = np.ascontiguousarray(veloc, F32)
+        t = np.ascontiguousarray(tract, F32)
+        npt = self.mesh.num_abs_boundary_faces * 25
+        if v.shape != t.shape or v.ndim != 3 or v.shape[1:] != (npt, 3):
+            raise ValueError(f"FK fields must be (nsamp, {npt}, 3), got {v.shape} and {t.shape}")
+        _lib.call("fwat_set_fk_injection_", self.h, npt, v.shape[0], int(resamp), v, t)
+
     def set_adjoint_sources(self, adj):
         adj = np.ascontiguousarray(adj, F32).reshape(self.nstep, self.nrec, 3)
         _lib.call("fwat_set_adjoint_sources_", self.h, adj, self.nstep)
