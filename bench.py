@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--no-tune", action="store_true", help="skip the element-pass launch-shape autotuner")
+    p.add_argument("--fk", type=int, default=0, metavar="R",
+                   help="c2: inject a synthetic incident field (FK coupling) through the Stacey faces of the "
+                        "forward run, one sample every R steps, the series resident in HBM")
     p.add_argument("--tuning", default="", metavar="R,F,A",
                    help="fixed launch shape instead of the autotuner: regular_path, forward and adjoint "
                         "workgroups per CU (e.g. 1,16,8), as the autotuner picks them")
@@ -149,6 +152,17 @@ def main():
                         compute_and_store_strain=not fused, static_precision=a.static_precision)
         f.set_tuning(**tuning)
         ad.set_tuning(**tuning)
+        if a.fk:
+            # synthetic incident field: a separable pulse times a fixed pattern over the face points
+            nsamp = (nstep - 1) // a.fk + 1
+            nfp3 = mesh.num_abs_boundary_faces * 75
+            tt = np.arange(nsamp) * a.fk * dt
+            pulse = (np.exp(-((tt - 2.0) / 0.5) ** 2) * 1e-3).astype(np.float32)
+            pat = np.cos(np.arange(nfp3) * 0.001).astype(np.float32)
+            vfk = np.multiply.outer(pulse, pat).reshape(nsamp, nfp3 // 3, 3)
+            f.set_fk_injection(vfk, vfk * np.float32(1e7), resamp=a.fk)
+            log(f"FK injection: {nsamp} samples x {nfp3 // 3} face points "
+                f"({2 * vfk.nbytes / 1e9:.2f} GB in HBM)")
         return f, ad
 
     def event(f, ad, obs, timed):
@@ -276,7 +290,8 @@ def main():
                    "path": (f"brick-fused {'x'.join(map(str, brick))}" if brick else
                             (f"fused, {nsl} element slabs" if nsl > 1 else "fused"))
                            if fused else "SPECFEM stage order (strain stores)",
-                   "element_pass_tuning": tuning if fused else None},
+                   "element_pass_tuning": tuning if fused else None,
+                   **({"fk_injection": {"every_steps": a.fk}} if a.fk else {})},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
