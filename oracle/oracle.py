@@ -96,6 +96,7 @@ def lib():
         L.orc_get_final.argtypes = [C.c_void_p, _fp, _fp, _fp]
         L.orc_set_final.argtypes = [C.c_void_p, _fp, _fp, _fp]
         L.orc_get_stacey.argtypes = [C.c_void_p, _fp]
+        L.orc_set_fk.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _fp, _fp]
         L.orc_adjoint.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _dp, _fp]
         L.orc_energy_series.argtypes = [C.c_void_p, _fp, _dp]
         L.orc_apply_k.argtypes = [C.c_void_p, _dp, _dp]
@@ -271,6 +272,14 @@ class Oracle:
         out = [np.zeros(self.nglob, np.float32) for _ in range(3)]
         lib().orc_get_rmass(self.h, *[_p(o, C.c_float) for o in out])
         return out
+
+    def set_fk(self, veloc, tract, resamp=1):
+        """Incident velocity and traction (nsamp, nfaces*25, 3) injected through the
+        Stacey faces of later forward runs, sample k at step 1 + k*resamp."""
+        v = np.ascontiguousarray(veloc, np.float32)
+        t = np.ascontiguousarray(tract, np.float32)
+        self._keep += [v, t]
+        lib().orc_set_fk(self.h, v.shape[1], v.shape[0], int(resamp), _p(v, C.c_float), _p(t, C.c_float))
 
     def forward(self, save=True):
         seis = np.zeros((self.nstep, self.nrec, 3), np.float32)

@@ -22,7 +22,10 @@
 //    per include/config.h:98), absent from the tree, so this is a restatement of
 //    the published SPECFEM3D Cartesian algorithm: GLL weak form with
 //    hprime/hprimewgll contractions, Newmark central difference, Stacey with the
-//    dt/2 mass-matrix term, moment-tensor point source with a Ricker STF,
+//    dt/2 mass-matrix term, FK injection of an incident velocity and traction
+//    through the Stacey faces (the Veloc_FK / Tract_FK of
+//    src/shared/fk_coupling.f90:71-297; the closed-form plane wave of
+//    tests/fk_plane_wave.py pins it), moment-tensor point source with a Ricker STF,
 //    Lagrange-interpolated receivers, backward reconstruction from the last
 //    state plus replayed Stacey terms, and rho/kappa/mu (Tromp-Tape-Liu 2005)
 //    or 21 c_ij kernels.  Parity against SPECFEM itself is unpinned; this file
@@ -492,6 +495,10 @@ struct SimBase {
   virtual void set_final(const float* u, const float* v, const float* a) = 0;
   virtual void get_stacey(float* buf) = 0;
   virtual void energy_series(double* e) = 0;
+  // FK injection of an incident field through the Stacey faces (forward runs):
+  // velocity and traction [fk_nsamp][nfaces*25][3], sample k at step 1 + k*fk_resamp
+  std::vector<float> fk_v, fk_t;
+  int fk_nsamp = 0, fk_resamp = 1;
   virtual void apply_k(const double* u, double* f) = 0;
 };
 
@@ -731,8 +738,21 @@ struct Sim : SimBase {
 
   // Stacey traction from the predicted velocity; mode 0: absorb, 1: absorb and
   // store, 2: replay the stored record (backward reconstruction).
-  void stacey(const std::vector<T>& V, std::vector<T>& A, int mode, T* rec) {
+  // it > 0: forward step it, whose incident field (linear between the stored
+  // samples) is injected: the absorbing traction of v - v_inc minus t_inc
+  // (SPECFEM's compute_stacey_viscoelastic under COUPLE_WITH_INJECTION_TECHNIQUE)
+  void stacey(const std::vector<T>& V, std::vector<T>& A, int mode, T* rec, int it = 0) {
     if (!p.stacey) return;
+    const bool inj = it > 0 && fk_nsamp > 0 && mode != 2;
+    size_t k0 = 0, k1 = 0;
+    T w1 = 0;
+    if (inj) {
+      k0 = (size_t)std::min((it - 1) / fk_resamp, fk_nsamp - 1);
+      k1 = std::min(k0 + 1, (size_t)fk_nsamp - 1);
+      w1 = k1 == k0 ? (T)0 : (T)((it - 1) - (int)k0 * fk_resamp) / (T)fk_resamp;
+    }
+    const T w0 = (T)1 - w1;
+    const size_t nfp3 = (size_t)m.nfaces * NG2 * 3;
     for (int f = 0; f < m.nfaces; ++f) {
       int e = m.f_ispec[f];
       for (int q = 0; q < NG2; ++q) {
@@ -747,10 +767,21 @@ struct Sim : SimBase {
           T nx = (T)m.f_n[fq * 3], ny = (T)m.f_n[fq * 3 + 1], nz = (T)m.f_n[fq * 3 + 2];
           T jw = (T)m.f_jw[fq];
           T vx = V[3 * gI], vy = V[3 * gI + 1], vz = V[3 * gI + 2];
+          T ix = 0, iy = 0, iz = 0;
+          if (inj) {
+            const float *v0 = &fk_v[k0 * nfp3 + fq * 3], *v1 = &fk_v[k1 * nfp3 + fq * 3];
+            const float *t0 = &fk_t[k0 * nfp3 + fq * 3], *t1 = &fk_t[k1 * nfp3 + fq * 3];
+            vx -= w0 * (T)v0[0] + w1 * (T)v1[0];
+            vy -= w0 * (T)v0[1] + w1 * (T)v1[1];
+            vz -= w0 * (T)v0[2] + w1 * (T)v1[2];
+            ix = w0 * (T)t0[0] + w1 * (T)t1[0];
+            iy = w0 * (T)t0[1] + w1 * (T)t1[1];
+            iz = w0 * (T)t0[2] + w1 * (T)t1[2];
+          }
           T vn = vx * nx + vy * ny + vz * nz;
-          tx = (rvp[L] * vn * nx + rvs[L] * (vx - vn * nx)) * jw;
-          ty = (rvp[L] * vn * ny + rvs[L] * (vy - vn * ny)) * jw;
-          tz = (rvp[L] * vn * nz + rvs[L] * (vz - vn * nz)) * jw;
+          tx = (rvp[L] * vn * nx + rvs[L] * (vx - vn * nx) - ix) * jw;
+          ty = (rvp[L] * vn * ny + rvs[L] * (vy - vn * ny) - iy) * jw;
+          tz = (rvp[L] * vn * nz + rvs[L] * (vz - vn * nz) - iz) * jw;
           if (mode == 1) { rec[fq * 3] = tx; rec[fq * 3 + 1] = ty; rec[fq * 3 + 2] = tz; }
         }
         A[3 * gI] -= tx;
@@ -813,7 +844,7 @@ struct Sim : SimBase {
     for (int it = 1; it <= p.nstep; ++it) {
       predictor(u, v, a, dt);
       forces(u.data(), a.data());
-      stacey(v, a, save ? 1 : 0, save && p.stacey ? &store[rec_sz * (it - 1)] : nullptr);
+      stacey(v, a, save ? 1 : 0, save && p.stacey ? &store[rec_sz * (it - 1)] : nullptr, it);
       add_sources(a, it);
       corrector(v, a, dt);
       if (seis) record(u, &seis[(size_t)(it - 1) * p.nrec * 3]);
@@ -1146,6 +1177,15 @@ void orc_forward(void* h, float* seis, int save) { static_cast<SimBase*>(h)->for
 void orc_get_final(void* h, float* u, float* v, float* a) { static_cast<SimBase*>(h)->get_final(u, v, a); }
 void orc_set_final(void* h, const float* u, const float* v, const float* a) { static_cast<SimBase*>(h)->set_final(u, v, a); }
 void orc_get_stacey(void* h, float* buf) { static_cast<SimBase*>(h)->get_stacey(buf); }
+// incident field for FK injection in later forward runs (nsamp 0: off)
+void orc_set_fk(void* h, int npt, int nsamp, int resamp, const float* v, const float* t) {
+  SimBase* b = static_cast<SimBase*>(h);
+  const size_t n = (size_t)nsamp * npt * 3;
+  b->fk_v.assign(v, v + n);
+  b->fk_t.assign(t, t + n);
+  b->fk_nsamp = nsamp;
+  b->fk_resamp = resamp;
+}
 
 void orc_adjoint(void* h, const float* adj, double* rho_kl, double* mu_kl, double* kappa_kl,
                  double* cijkl_kl, float* b_final) {

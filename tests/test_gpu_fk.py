@@ -1,62 +1,26 @@
 """FK injection of an incident wavefield through the Stacey faces (teleseismic
-coupling, fk_coupling.f90:71-297), checked against the closed-form field of a
-vertically incident P plane wave in a homogeneous half-space, whose free-surface
-reflection is the same pulse travelling down:
-
-    u_z(z, t) = g(t - tc - (z + L)/alpha) + g(t - tc - (L - z)/alpha),  u_x = u_y = 0
-    sigma_zz = rho alpha^2 du_z/dz,  sigma_xx = sigma_yy = lambda du_z/dz
-
-Injected on the bottom and the four sides (velocity and traction sigma.n), the
-box must reproduce u_z in its interior.
-"""
+coupling, fk_coupling.f90:71-297): against the closed-form vertically incident P
+plane wave (fk_plane_wave), and against the oracle's injection on a perturbed,
+heterogeneous mesh with a generic incident field (seismograms and kernels)."""
 import numpy as np
 import pytest
 
+from fk_plane_wave import ALPHA, BETA, RHO, incident_fields as _fields, plane_wave
 from specfwat_amd.mesh import BoxMesh
 from specfwat_amd.sources import Receivers, Sources
 from specfwat_amd.solver import ElasticRun
+from test_gpu_parity import KERN_TOL, SEIS_TOL, make_case, oracle_of, rel
 
 pytestmark = pytest.mark.gpu
 
-RHO, ALPHA, BETA = 2700.0, 6000.0, 3464.0
-SIG, TC = 0.4, 1.4  # Gaussian pulse g(t) = exp(-(t/SIG)^2), centred at the bottom at TC
-
-
-def g(t):
-    return np.exp(-(t / SIG) ** 2)
-
-
-def dg(t):
-    return -2.0 * t / SIG**2 * g(t)
-
-
-def plane_wave(z, t, L):
-    t1 = t - TC - (z + L) / ALPHA
-    t2 = t - TC - (L - z) / ALPHA
-    u = g(t1) + g(t2)
-    v = dg(t1) + dg(t2)
-    duz = (-dg(t1) + dg(t2)) / ALPHA
-    return u, v, duz
-
 
 def incident_fields(mesh, nsamp, dt_s, L):
-    """Velocity and traction of the plane wave at every absorbing-face point,
-    sample k at time k*dt_s: (nsamp, nfaces*25, 3)."""
+    """The plane wave's velocity and traction at every absorbing-face point of mesh."""
     e = mesh.abs_boundary_ispec - 1
     ijk = mesh.abs_boundary_ijk - 1
     loc = ijk[..., 0] + 5 * ijk[..., 1] + 25 * ijk[..., 2]
     z = mesh.xyz[mesh.ibool0[e[:, None], loc], 2].ravel()
-    n = mesh.abs_boundary_normal.reshape(-1, 3).astype(np.float64)
-    lam = RHO * (ALPHA**2 - 2 * BETA**2)
-    t = (np.arange(nsamp) * dt_s)[:, None]
-    _, v, duz = plane_wave(z[None, :], t, L)
-    V = np.zeros((nsamp, len(z), 3))
-    V[..., 2] = v
-    T = np.zeros_like(V)
-    T[..., 0] = lam * duz * n[None, :, 0]
-    T[..., 1] = lam * duz * n[None, :, 1]
-    T[..., 2] = RHO * ALPHA**2 * duz * n[None, :, 2]
-    return V.astype(np.float32), T.astype(np.float32)
+    return _fields(z, mesh.abs_boundary_normal.reshape(-1, 3), nsamp, dt_s, L)
 
 
 def setup(nstep=1125, dt=0.004):
@@ -142,3 +106,48 @@ def test_plane_wave_injection_matches_closed_form():
     assert err4 < 3e-2
     # without injection the box stays at rest
     assert np.abs(run(mesh, rho, vp, vs, rec, 50, dt)).max() == 0.0
+
+
+def test_injection_matches_oracle():
+    """A perturbed heterogeneous mesh, a point source and a generic incident field
+    (random per-point amplitudes and delays, stored every 3rd step): seismograms of
+    the injected forward run and the kernels of its adjoint run vs the oracle."""
+    c = make_case(seed=11, perturb=0.1, nstep=150)
+    m = c["mesh"]
+    o64 = oracle_of(c, 64)
+    fo = o64.faces()
+    assert np.array_equal(fo["ispec"] + 1, m.abs_boundary_ispec)
+    assert np.array_equal(fo["ijk"] + 1, m.abs_boundary_ijk)
+    rng = np.random.default_rng(5)
+    npt, resamp = m.num_abs_boundary_faces * 25, 3
+    nsamp = (c["nstep"] - 1) // resamp + 1
+    tk = np.arange(nsamp) * resamp * c["dt"]
+    amp = rng.standard_normal((npt, 3))
+    dly = rng.uniform(0.5, 1.5, (npt, 1))
+    pulse = np.exp(-(((tk[:, None] - dly.T) / 0.3) ** 2))  # (nsamp, npt)
+    V = (1e-2 * pulse[..., None] * amp[None]).astype(np.float32)
+    T = (3e5 * pulse[..., None] * amp[None, :, ::-1]).astype(np.float32)
+    o64.set_fk(V, T, resamp=resamp)
+    obs = o64.forward(save=True)
+    src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+    rec = Receivers(m, c["rxyz"])
+    f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
+    f.set_fk_injection(V, T, resamp=resamp)
+    f.iterate_time()
+    syn = f.seismograms()
+    e = rel(syn, obs)
+    print(f"injected seismograms vs FP64 oracle: rel L2 {e:.2e}", flush=True)
+    assert np.abs(obs).max() > 0 and e < SEIS_TOL
+    adj = obs.astype(np.float32)
+    a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
+    a.take_forward(f)
+    f.close()
+    a.set_adjoint_sources(adj)
+    a.iterate_time()
+    k = a.kernels()
+    a.close()
+    ko = o64.adjoint(adj)
+    for n in ("rho_kl", "mu_kl", "kappa_kl"):
+        ek = rel(k[n], ko[n])
+        print(f"{n} with injection: rel L2 vs FP64 oracle {ek:.2e}", flush=True)
+        assert ek < KERN_TOL, n
