@@ -112,6 +112,62 @@ __global__ void __launch_bounds__(MTPB) k_sos_filter(int nstep, int ntrace, int 
     }
 }
 
+// The order-2 bandpass (two sections) on a lane pair per trace: the even lane
+// runs section 0 on sample n while the odd lane runs section 1 on sample
+// n - 1, fed section 0's previous output across the pair; each lane's
+// instruction stream is one section's, and each section sees the same inputs
+// in the same order as section by section.
+__global__ void __launch_bounds__(MTPB) k_sos_filter2(int nstep, int ntrace, const float* __restrict__ sos,
+                                                      float* __restrict__ x) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x, t = gid >> 1, sec = gid & 1;
+  if (t >= ntrace) return;  // both lanes of a pair leave together
+  const int src = (threadIdx.x & ~1);  // the pair's section-0 lane
+  const size_t T = (size_t)ntrace;
+  const float* c = sos + 6 * sec;
+  const float b0 = c[0], b1 = c[1], b2 = c[2], a1 = c[4], a2 = c[5];
+  for (int pass = 0; pass < 2; ++pass) {
+    auto at = [&](int k) -> size_t { return (size_t)(pass == 0 ? k : nstep - 1 - k) * T + t; };
+    float x1 = 0.0f, x2 = 0.0f, y1 = 0.0f, y2 = 0.0f, prev = 0.0f;
+    // step n: section 0 filters sample n (n < nstep), section 1 sample n - 1 (n >= 1)
+    auto step = [&](int n, float xin) {
+      const float from0 = __shfl(prev, src, 64);
+      const float in = sec == 0 ? xin : from0;
+      const bool valid = sec == 0 ? n < nstep : n >= 1;
+      if (valid) {
+        float out = b0 * in + b1 * x1 + b2 * x2;
+        out = out - (a1 * y1 + a2 * y2);
+        y2 = y1; y1 = out; x2 = x1; x1 = in;
+        if (sec == 0) prev = out;
+        else x[at(n - 1)] = out;
+      }
+    };
+    for (int k0 = 0; k0 < nstep; k0 += SOS_AHEAD) {
+      float buf[SOS_AHEAD];
+#pragma unroll
+      for (int u = 0; u < SOS_AHEAD; ++u) {
+        const int k = k0 + u;
+        buf[u] = (sec == 0 && k < nstep) ? x[at(k)] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < SOS_AHEAD; ++u) {
+        if (k0 + u > nstep) break;
+        step(k0 + u, buf[u]);
+      }
+    }
+    if (nstep % SOS_AHEAD == 0) step(nstep, 0.0f);  // section 1 on the last sample
+    __threadfence();  // the backward pass reads what the pair's other lane wrote
+  }
+}
+
+void launch_sos(hipStream_t s, int N, int T, int nsect, const float* d_sos, float* d_x) {
+  if (nsect == 2)
+    hipLaunchKernelGGL(k_sos_filter2, dim3((unsigned)((2 * (size_t)T + MTPB - 1) / MTPB)), dim3(MTPB), 0, s, N, T,
+                       d_sos, d_x);
+  else
+    hipLaunchKernelGGL(k_sos_filter, dim3((unsigned)((T + MTPB - 1) / MTPB)), dim3(MTPB), 0, s, N, T, nsect, d_sos,
+                       d_x);
+}
+
 // taper over [nb, nb + nl) in double, zero outside
 __global__ void k_window_final(int nstep, int ntrace, int nb, int nl, const double* __restrict__ tw,
                                float* __restrict__ x) {
@@ -1726,7 +1782,7 @@ void fwat_waveform_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* ob
   hipLaunchKernelGGL(k_waveform_adj, dim3(nb), dim3(MTPB), 0, s, N, T, *dt, nw, W, d_wts, d_obs, d_syn, d_adj, d_mr,
                      d_mr + (size_t)nw * T);
   if (*nsect > 0) {
-    hipLaunchKernelGGL(k_sos_filter, dim3(nb), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    launch_sos(s, N, T, *nsect, d_sos, d_adj);
     hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
                        d_adj);
   }
@@ -1833,7 +1889,7 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     taper_weights(fnl, *taper_percentage, *itaper_type, fw);
     d_fw = dev_upload(fw.data(), fw.size(), s);
     d_sos = dev_upload(sos, 6 * (size_t)*nsect, s);
-    hipLaunchKernelGGL(k_sos_filter, dim3(nbk), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    launch_sos(s, N, T, *nsect, d_sos, d_adj);
     hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
                        d_adj);
   }
@@ -1976,7 +2032,7 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     taper_weights(fnl, *taper_percentage, *itaper_type, fw);
     d_fw = dev_upload(fw.data(), fw.size(), s);
     d_sos = dev_upload(sos, 6 * (size_t)*nsect, s);
-    hipLaunchKernelGGL(k_sos_filter, dim3(nbk), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    launch_sos(s, N, T, *nsect, d_sos, d_adj);
     hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
                        d_adj);
   }
@@ -2091,7 +2147,7 @@ void fwat_ep_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
     taper_weights(fnl, *taper_percentage, *itaper_type, fw);
     d_fw = dev_upload(fw.data(), fw.size(), s);
     d_sos = dev_upload(sos, 6 * (size_t)*nsect, s);
-    hipLaunchKernelGGL(k_sos_filter, dim3(nbk), dim3(MTPB), 0, s, N, T, *nsect, d_sos, d_adj);
+    launch_sos(s, N, T, *nsect, d_sos, d_adj);
     hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
                        d_adj);
   }
