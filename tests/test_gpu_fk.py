@@ -151,3 +151,27 @@ def test_injection_matches_oracle():
         ek = rel(k[n], ko[n])
         print(f"{n} with injection: rel L2 vs FP64 oracle {ek:.2e}", flush=True)
         assert ek < KERN_TOL, n
+
+
+@pytest.mark.parametrize("path", ["brick", "slabs"])
+def test_injection_on_brick_and_slab_paths(path):
+    """The brick-fused pass and the element-slab order inject the same incident
+    field as the plain fused pass (plane wave on the regular lattice box)."""
+    mesh, rho, vp, vs, rec, depths, L, _, dt = setup()
+    nstep = 400
+    V, T = incident_fields(mesh, nstep, dt, L)
+
+    def fwd(tuning):
+        r = ElasticRun(mesh, rho, vp, vs, None, rec, dt, nstep, 1, False)
+        r.set_tuning(1, 0, 8, **tuning)
+        r.set_fk_injection(V, T)
+        r.iterate_time()
+        s = r.seismograms()
+        r.close()
+        return s
+
+    base = fwd({})
+    other = fwd({"brick": (2, 2, 2)} if path == "brick" else {"slab_elems": 64})
+    e = rel(other, base)
+    print(f"{path} path with injection vs plain fused: rel L2 {e:.2e}", flush=True)
+    assert np.abs(base).max() > 0 and e < 1e-6
