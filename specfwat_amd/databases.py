@@ -481,14 +481,18 @@ def _box_adjacency(mesh):
     return np.array(xadj, I32), np.array(adj, I32)
 
 
-def write_external_mesh(path, mesh, rho, vp, vs, dt, cij=None, stacey=True):
+def write_external_mesh(path, mesh, rho, vp, vs, dt, cij=None, stacey=True, rmass_ocean_load=None,
+                        elastic_colors=None):
     """Write a BoxMesh and its model as an elastic, single-slice
     proc%06d_external_mesh.bin in the record order read_external_mesh reads.
 
     Materials as get_model_fwat stores them (kappa, mu, rho_vp, rho_vs in
     float32 from rho, vp, vs); rmass and, with Stacey faces, rmassx/y/z =
     rmass + dt/2 C summed in float64 and stored in float32; free surface on
-    top; every element inner; no PML, coupling, MPI interfaces or colouring.
+    top; every element inner; no PML, coupling or MPI interfaces.  Optional
+    records: rmass_ocean_load (nglob, APPROXIMATE_OCEAN_LOAD) and
+    elastic_colors = (num_colors_outer, num_colors_inner, num_elem_colors)
+    (USE_MESH_COLORING_GPU).
     """
     from .solver import mass_sums  # the solver imports this module's users, not the reverse
 
@@ -516,6 +520,8 @@ def write_external_mesh(path, mesh, rho, vp, vs, dt, cij=None, stacey=True):
         w.rec(np.zeros(nspec), I32)      # ispec_is_poroelastic
         w.ints(9999)
         w.rec(rm, F32)
+        if rmass_ocean_load is not None:
+            w.rec(rmass_ocean_load, F32)
         w.rec(rho_vp, F32)
         w.rec(rho_vs, F32)
         nf = mesh.num_abs_boundary_faces
@@ -555,6 +561,10 @@ def write_external_mesh(path, mesh, rho, vp, vs, dt, cij=None, stacey=True):
         ph = np.zeros((2, nspec), I32)
         ph[1] = np.arange(1, nspec + 1)
         w.rec(ph, I32)
+        if elastic_colors is not None:
+            no, ni, ne = elastic_colors
+            w.ints(no, ni)
+            w.rec(ne, I32)
         surf = np.zeros(nspec, bool)
         for s in sides:
             surf |= s

@@ -227,3 +227,28 @@ def test_corrupt_files_are_refused(written, tmp_path):
         D.read_external_mesh(str(bad))
     with pytest.raises(D.DatabaseError):
         D.read_external_mesh(p, stacey_absorbing_conditions=False)  # rmassx taken for the sentinel
+
+
+def test_ocean_load_and_colouring_records(tmp_path):
+    """The optional APPROXIMATE_OCEAN_LOAD and USE_MESH_COLORING_GPU records sit
+    where read_mesh_databases.f90:461-475 and :1186-1198 read them; reading with
+    the switches off takes the next record for the wrong array and fails."""
+    m, rho, vp, vs, _ = _case()
+    ocean = np.linspace(1.0, 2.0, m.nglob).astype(np.float32)
+    colors = (1, 3, np.array([5, 20, 15, 8], np.int32))
+    p = str(tmp_path / "o.bin")
+    D.write_external_mesh(p, m, rho, vp, vs, 0.02, rmass_ocean_load=ocean, elastic_colors=colors)
+    d = D.read_external_mesh(p, approximate_ocean_load=True, use_mesh_coloring_gpu=True)
+    assert np.array_equal(d["rmass_ocean_load"], ocean)
+    assert (d["num_colors_outer_elastic"], d["num_colors_inner_elastic"]) == (1, 3)
+    assert np.array_equal(d["num_elem_colors_elastic"], colors[2])
+    assert d["records"] == 74 + 3
+    assert np.array_equal(d["rho_vp"], D.read_external_mesh(str(tmp_path / "o.bin"), approximate_ocean_load=True,
+                                                           use_mesh_coloring_gpu=True)["rho_vp"])
+    plain = str(tmp_path / "p.bin")
+    D.write_external_mesh(plain, m, rho, vp, vs, 0.02)
+    assert np.array_equal(D.read_external_mesh(plain)["rho_vp"], d["rho_vp"])
+    with pytest.raises(D.DatabaseError):
+        D.read_external_mesh(p, use_mesh_coloring_gpu=True)  # ocean record taken for rho_vp
+    with pytest.raises(D.DatabaseError):
+        D.read_external_mesh(p, approximate_ocean_load=True)  # colour counts taken for the surface count
