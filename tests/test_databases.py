@@ -252,3 +252,18 @@ def test_ocean_load_and_colouring_records(tmp_path):
         D.read_external_mesh(p, use_mesh_coloring_gpu=True)  # ocean record taken for rho_vp
     with pytest.raises(D.DatabaseError):
         D.read_external_mesh(p, approximate_ocean_load=True)  # colour counts taken for the surface count
+
+
+def test_engine_refuses_what_it_cannot_run(written):
+    """Acoustic or poroelastic elements, C-PML layers and one slice of a
+    partitioned mesh are read but refused by the engine's mesh adaptor."""
+    d = D.read_external_mesh(written[0])
+    D.DatabaseMesh(dict(d))  # the elastic single-slice file itself is accepted
+    for key, val, msg in (("ACOUSTIC_SIMULATION", True, "purely elastic"),
+                          ("POROELASTIC_SIMULATION", True, "purely elastic"),
+                          ("NSPEC_CPML", 12, "C-PML"),
+                          ("num_interfaces_ext_mesh", 2, "partitioned")):
+        bad = dict(d)
+        bad[key] = val
+        with pytest.raises(D.DatabaseError, match=msg):
+            D.DatabaseMesh(bad)
