@@ -315,7 +315,8 @@ class DatabaseMesh(SemMesh):
             raise DatabaseError("C-PML databases are not supported (Stacey absorbing boundaries only)")
         if db["num_interfaces_ext_mesh"]:
             raise DatabaseError("the database is one slice of a partitioned mesh "
-                                f"({db['num_interfaces_ext_mesh']} MPI interfaces); one GPU runs one whole mesh")
+                                f"({db['num_interfaces_ext_mesh']} MPI interfaces); one GPU runs one whole mesh: "
+                                "load every slice with load_partitioned_mesh")
         self.gll = GLL()
         self.nspec, self.nglob = db["NSPEC_AB"], db["NGLOB_AB"]
         self.nspec_irregular = db["NSPEC_IRREGULAR"]
@@ -584,3 +585,117 @@ def write_external_mesh(path, mesh, rho, vp, vs, dt, cij=None, stacey=True, rmas
         w.ints(9995)
     finally:
         w.close()
+
+
+# ---------------------------------------------------------------------------- partitioned meshes
+def merge_slices(dbs):
+    """One mesh from the proc%06d_external_mesh.bin records of every slice of a
+    partitioned mesh (read_external_mesh of each rank, in rank order), for a
+    single GPU running the whole mesh.
+
+    Shared points are matched through the MPI interface lists: the k-th entry of
+    rank r's list towards rank q and the k-th entry of rank q's list towards r
+    are the same point (the pairing the reference's assembly of
+    ibool_interfaces_ext_mesh relies on, read_mesh_databases.f90:1003-1041).
+    Merged points are numbered by first appearance in rank order; elements,
+    faces and per-element arrays are concatenated in rank order; the masses
+    rmass and rmassx/y/z, which each slice stores unassembled, are summed over
+    the slices that share a point (in rank order, float64, stored as float32).
+    """
+    nrank = len(dbs)
+    for r, d in enumerate(dbs):
+        if d["ACOUSTIC_SIMULATION"] or d["POROELASTIC_SIMULATION"] or not d["ELASTIC_SIMULATION"]:
+            raise DatabaseError(f"slice {r}: only purely elastic slices are merged")
+        if d["NSPEC_CPML"]:
+            raise DatabaseError(f"slice {r}: C-PML slices are not merged")
+    reg = [d for d in dbs if d["NSPEC_IRREGULAR"] < d["NSPEC_AB"]]  # slices holding regular elements
+    if any(float(d["xix_regular"]) != float(reg[0]["xix_regular"]) for d in reg):
+        raise DatabaseError("slices hold regular elements of different sizes")
+    # union-find over (rank, 0-based point)
+    base = np.cumsum([0] + [d["NGLOB_AB"] for d in dbs])
+    parent = np.arange(base[-1])
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for r, d in enumerate(dbs):
+        for i in range(d["num_interfaces_ext_mesh"]):
+            q = int(d["my_neighbors_ext_mesh"][i])
+            if not 0 <= q < nrank:
+                raise DatabaseError(f"slice {r}: neighbour {q} outside 0..{nrank - 1}")
+            dq = dbs[q]
+            back = [j for j in range(dq["num_interfaces_ext_mesh"]) if int(dq["my_neighbors_ext_mesh"][j]) == r]
+            if len(back) != 1:
+                raise DatabaseError(f"slices {r} and {q}: interface lists do not pair up")
+            j = back[0]
+            n, nq = int(d["nibool_interfaces_ext_mesh"][i]), int(dq["nibool_interfaces_ext_mesh"][j])
+            if n != nq:
+                raise DatabaseError(f"slices {r} and {q}: interface of {n} and {nq} points")
+            a = d["ibool_interfaces_ext_mesh"][i, :n].astype(np.int64) - 1 + base[r]
+            b = dq["ibool_interfaces_ext_mesh"][j, :n].astype(np.int64) - 1 + base[q]
+            for u, v in zip(a, b):
+                ru, rv = find(u), find(v)
+                if ru != rv:
+                    parent[max(ru, rv)] = min(ru, rv)
+    roots = np.array([find(a) for a in range(base[-1])])
+    uniq, first = np.unique(roots, return_index=True)
+    order = np.argsort(first)  # first appearance in rank order
+    newid = np.empty(base[-1], np.int64)
+    rank_of_root = np.empty(len(uniq), np.int64)
+    rank_of_root[order] = np.arange(len(uniq))
+    newid[:] = rank_of_root[np.searchsorted(uniq, roots)]
+    nglob = len(uniq)
+    out = {"NGLOB_AB": nglob, "ACOUSTIC_SIMULATION": False, "ELASTIC_SIMULATION": True,
+           "POROELASTIC_SIMULATION": False, "NSPEC_CPML": 0, "num_interfaces_ext_mesh": 0,
+           "xix_regular": (reg or dbs)[0]["xix_regular"], "jacobian_regular": (reg or dbs)[0]["jacobian_regular"]}
+    out["NSPEC_AB"] = nspec = int(sum(d["NSPEC_AB"] for d in dbs))
+    out["NSPEC_IRREGULAR"] = int(sum(d["NSPEC_IRREGULAR"] for d in dbs))
+    out["ibool"] = np.concatenate([newid[d["ibool"].astype(np.int64) - 1 + base[r]] + 1
+                                   for r, d in enumerate(dbs)]).astype(I32)
+    for c in "xyz":
+        a = np.zeros(nglob, F32)
+        for r, d in enumerate(dbs):
+            a[newid[base[r]:base[r + 1]]] = d[c + "store"]
+        out[c + "store"] = a
+    irr, off = [], 0
+    for d in dbs:
+        e = d["irregular_element_number"].astype(np.int64)
+        irr.append(np.where(e > 0, e + off, 0))
+        off += d["NSPEC_IRREGULAR"]
+    out["irregular_element_number"] = np.concatenate(irr).astype(I32)
+    for nm in _JNAMES + ["jacobian"]:
+        parts = [d[nm + "store"].reshape(-1, NG3) for d in dbs if d["NSPEC_IRREGULAR"]]
+        out[nm + "store"] = np.concatenate(parts) if parts else np.zeros(1, F32)
+    for nm in ("kappastore", "mustore", "rhostore", "rho_vp", "rho_vs"):
+        out[nm] = np.concatenate([d[nm] for d in dbs])
+    if all(CIJ_NAMES[0] + "store" in d for d in dbs):
+        for nm in CIJ_NAMES:
+            out[nm + "store"] = np.concatenate([d[nm + "store"] for d in dbs])
+    for nm in ("rmass", "rmassx", "rmassy", "rmassz"):
+        if all(nm in d for d in dbs):
+            acc = np.zeros(nglob)
+            for r, d in enumerate(dbs):
+                np.add.at(acc, newid[base[r]:base[r + 1]], d[nm].astype(np.float64))
+            out[nm] = acc.astype(F32)
+    eoff = np.cumsum([0] + [d["NSPEC_AB"] for d in dbs])
+    for pre, key in (("abs_boundary", "num_abs_boundary_faces"), ("free_surface", "num_free_surface_faces")):
+        have = [r for r, d in enumerate(dbs) if d[key] > 0]
+        out[key] = int(sum(dbs[r][key] for r in have))
+        if have:
+            out[pre + "_ispec"] = np.concatenate([dbs[r][pre + "_ispec"] + eoff[r] for r in have]).astype(I32)
+            for suf in ("_ijk", "_jacobian2Dw", "_normal"):
+                out[pre + suf] = np.concatenate([dbs[r][pre + suf] for r in have])
+    out["nspec_inner_elastic"], out["nspec_outer_elastic"] = nspec, 0
+    out["num_phase_ispec_elastic"] = nspec
+    ph = np.zeros((2, nspec), I32)
+    ph[1] = np.arange(1, nspec + 1)
+    out["phase_ispec_inner_elastic"] = ph
+    return out
+
+
+def load_partitioned_mesh(paths, **flags):
+    """DatabaseMesh of the whole mesh from every slice's file (rank order)."""
+    return DatabaseMesh(merge_slices([read_external_mesh(p, **flags) for p in paths]))

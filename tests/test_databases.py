@@ -267,3 +267,176 @@ def test_engine_refuses_what_it_cannot_run(written):
         bad[key] = val
         with pytest.raises(D.DatabaseError, match=msg):
             D.DatabaseMesh(bad)
+
+
+def _write_slice(path, d):
+    """Write a record dict (read_external_mesh's keys) of an elastic slice with
+    Stacey and free-surface faces and MPI interfaces, in the reader's order."""
+    w = D._Writer(path)
+    n, g = d["NSPEC_AB"], d["NGLOB_AB"]
+    w.ints(n); w.ints(g); w.ints(d["NSPEC_IRREGULAR"])
+    w.rec(d["ibool"], np.int32)
+    for c in "xyz":
+        w.rec(d[c + "store"], np.float32)
+    w.rec(d["irregular_element_number"], np.int32)
+    w.rec([d["xix_regular"]], np.float32); w.rec([d["jacobian_regular"]], np.float32)
+    for nm in D._JNAMES + ["jacobian"]:
+        w.rec(d[nm + "store"], np.float32)
+    for nm in ("kappastore", "mustore", "rhostore"):
+        w.rec(d[nm], np.float32)
+    w.rec(np.zeros(n), np.int32); w.rec(np.ones(n), np.int32); w.rec(np.zeros(n), np.int32)
+    w.ints(9999)
+    for nm in ("rmass", "rho_vp", "rho_vs"):
+        w.rec(d[nm], np.float32)
+    w.ints(d["num_abs_boundary_faces"])
+    if d["num_abs_boundary_faces"]:
+        for suf, dt in (("_ispec", np.int32), ("_ijk", np.int32), ("_jacobian2Dw", np.float32), ("_normal", np.float32)):
+            w.rec(d["abs_boundary" + suf], dt)
+        for c in "xyz":
+            w.rec(d["rmass" + c], np.float32)
+    w.ints(9998)
+    for sname in ("xmin", "xmax", "ymin", "ymax", "bottom", "top"):
+        w.ints(0)
+    w.ints(d["num_free_surface_faces"])
+    if d["num_free_surface_faces"]:
+        for suf, dt in (("_ispec", np.int32), ("_ijk", np.int32), ("_jacobian2Dw", np.float32), ("_normal", np.float32)):
+            w.rec(d["free_surface" + suf], dt)
+    w.ints(0); w.ints(0); w.ints(0)
+    w.ints(9997)
+    ni = d["num_interfaces_ext_mesh"]
+    w.ints(ni); w.ints(d["max_nibool_interfaces_ext_mesh"])
+    if ni:
+        w.rec(d["my_neighbors_ext_mesh"], np.int32)
+        w.rec(d["nibool_interfaces_ext_mesh"], np.int32)
+        w.rec(d["ibool_interfaces_ext_mesh"], np.int32)
+    w.ints(9996)
+    w.rec(np.ones(n), np.int32)
+    w.ints(n, 0); w.ints(n)
+    ph = np.zeros((2, n), np.int32); ph[1] = np.arange(1, n + 1)
+    w.rec(ph, np.int32)
+    w.ints(0); w.rec(np.zeros(n), np.int32); w.rec(np.zeros(g), np.int32)
+    w.ints(0); w.rec(np.zeros(n + 1), np.int32); w.rec(np.zeros(0), np.int32)
+    w.ints(9995)
+    w.close()
+
+
+def split_database(p, m, rho, cuts, tmp_path):
+    """Write the slices of the database at p cut at element indices cuts: each
+    slice numbers its own points, stores its own unassembled masses (rmass and
+    the Stacey terms of its own faces) and lists the points it shares with each
+    neighbour in one common order (ascending whole-mesh number).  Returns the
+    slice paths in rank order."""
+    full = D.read_external_mesh(p)
+    g = GLL()
+    ii = np.arange(125)
+    w3 = g.w[ii % 5] * g.w[(ii // 5) % 5] * g.w[ii // 25]
+    dm = D.DatabaseMesh(full)
+    elem_mass = dm.jacobian_all.astype(np.float64) * w3 * full["rhostore"].astype(np.float64)
+    ib0 = full["ibool"] - 1
+    owner_pts = [np.unique(ib0[cuts[r]:cuts[r + 1]]) for r in range(len(cuts) - 1)]
+    # the Stacey part of rmassx/y/z per face point: rmassx - rmass of the whole mesh, split by face element
+    fe = full["abs_boundary_ispec"] - 1
+    fl = (full["abs_boundary_ijk"][..., 0] - 1) + 5 * (full["abs_boundary_ijk"][..., 1] - 1) + 25 * (full["abs_boundary_ijk"][..., 2] - 1)
+    from specfwat_amd.solver import mass_sums
+    _, rmx, rmy, rmz = mass_sums(m, rho.reshape(m.nspec, 125), full["rho_vp"], full["rho_vs"], 0.02, True)
+    paths = []
+    for r in range(len(cuts) - 1):
+        e0, e1 = cuts[r], cuts[r + 1]
+        pts = owner_pts[r]  # global 0-based points of this slice, local number = position
+        loc = np.full(full["NGLOB_AB"], -1)
+        loc[pts] = np.arange(len(pts))
+        d = {"NSPEC_AB": e1 - e0, "NGLOB_AB": len(pts), "xix_regular": full["xix_regular"],
+             "jacobian_regular": full["jacobian_regular"]}
+        d["ibool"] = (loc[ib0[e0:e1]] + 1).astype(np.int32)
+        for c in "xyz":
+            d[c + "store"] = full[c + "store"][pts]
+        irr = full["irregular_element_number"][e0:e1]
+        sel = irr > 0
+        d["NSPEC_IRREGULAR"] = int(sel.sum())
+        d["irregular_element_number"] = np.where(sel, np.cumsum(sel), 0).astype(np.int32)
+        for nm in D._JNAMES + ["jacobian"]:
+            d[nm + "store"] = full[nm + "store"][irr[sel] - 1]
+        for nm in ("kappastore", "mustore", "rhostore", "rho_vp", "rho_vs"):
+            d[nm] = full[nm][e0:e1]
+        d["rmass"] = np.bincount(loc[ib0[e0:e1]].ravel(), elem_mass[e0:e1].ravel(), len(pts)).astype(np.float32)
+        fs = (fe >= e0) & (fe < e1)
+        d["num_abs_boundary_faces"] = int(fs.sum())
+        d["abs_boundary_ispec"] = (fe[fs] - e0 + 1).astype(np.int32)
+        for suf in ("_ijk", "_jacobian2Dw", "_normal"):
+            d["abs_boundary" + suf] = full["abs_boundary" + suf][fs]
+        gface = ib0[fe[fs][:, None], fl[fs]]
+        for c, whole in zip("xyz", (rmx, rmy, rmz)):
+            # this slice's rmass plus the Stacey terms of its own faces (from the whole mesh's split)
+            stacey = np.zeros(full["NGLOB_AB"])
+            d["rmass" + c] = d["rmass"].astype(np.float64)
+            term = (whole - np.bincount(ib0.ravel(), elem_mass.ravel(), full["NGLOB_AB"]))
+            # attribute each point's Stacey term to the slice of the first face that holds it
+            firstface = np.full(full["NGLOB_AB"], 10**9)
+            np.minimum.at(firstface, ib0[fe[:, None], fl].ravel(), np.repeat(fe, 25))
+            mine = np.zeros(full["NGLOB_AB"], bool)
+            mine[gface.ravel()] = True
+            mine &= (firstface >= e0) & (firstface < e1)
+            stacey[mine] = term[mine]
+            d["rmass" + c] = (d["rmass" + c] + stacey[pts]).astype(np.float32)
+        ft = (full["free_surface_ispec"] - 1 >= e0) & (full["free_surface_ispec"] - 1 < e1)
+        d["num_free_surface_faces"] = int(ft.sum())
+        d["free_surface_ispec"] = (full["free_surface_ispec"][ft] - e0).astype(np.int32)
+        for suf in ("_ijk", "_jacobian2Dw", "_normal"):
+            d["free_surface" + suf] = full["free_surface" + suf][ft]
+        nbrs, lists = [], []
+        for q in range(len(cuts) - 1):
+            if q == r:
+                continue
+            shared = np.intersect1d(pts, owner_pts[q])  # ascending global number: the same order on both sides
+            if len(shared):
+                nbrs.append(q)
+                lists.append(loc[shared] + 1)
+        mx = max(len(a) for a in lists)
+        d["num_interfaces_ext_mesh"] = len(nbrs)
+        d["max_nibool_interfaces_ext_mesh"] = mx
+        d["my_neighbors_ext_mesh"] = np.array(nbrs, np.int32)
+        d["nibool_interfaces_ext_mesh"] = np.array([len(a) for a in lists], np.int32)
+        ibi = np.zeros((len(nbrs), mx), np.int32)
+        for k, a in enumerate(lists):
+            ibi[k, :len(a)] = a
+        d["ibool_interfaces_ext_mesh"] = ibi
+        path = str(tmp_path / f"proc{r:06d}_external_mesh.bin")
+        _write_slice(path, d)
+        paths.append(path)
+    return paths
+
+
+def test_partitioned_mesh_merges_back_to_the_whole(written, tmp_path):
+    """Three slices cut by element ranges, written, read and merged: the same
+    elements, points, geometry, faces and assembled masses as the whole mesh."""
+    p, m, rho, vp, vs = written
+    full = D.read_external_mesh(p)
+    dm = D.DatabaseMesh(full)
+    ib0 = full["ibool"] - 1
+    paths = split_database(p, m, rho, [0, 17, 31, m.nspec], tmp_path)
+    slices = [D.read_external_mesh(q) for q in paths]
+    assert all(s["num_interfaces_ext_mesh"] == 2 for s in slices)
+    with pytest.raises(D.DatabaseError, match="partitioned"):
+        D.DatabaseMesh(slices[1])
+    mm = D.load_partitioned_mesh(paths)
+    assert (mm.nspec, mm.nglob, mm.nspec_irregular) == (m.nspec, m.nglob, m.nspec_irregular)
+    # a bijection between merged and whole-mesh points, consistent on every local point
+    pi = np.full(m.nglob, -1)
+    pi[mm.ibool0.ravel()] = ib0.ravel()
+    assert np.array_equal(pi[mm.ibool0], ib0) and len(np.unique(pi)) == m.nglob
+    assert np.array_equal(mm.xyz, dm.xyz[pi])
+    for nm in ("xix", "etay", "gammaz", "jacobian", "jacobian_all"):
+        assert np.array_equal(getattr(mm, nm), getattr(dm, nm)), nm
+    # absorbing faces come slice by slice instead of side by side: the same set
+    def faces(x):
+        key = np.lexsort((x.abs_boundary_ijk[:, 12, 2], x.abs_boundary_ijk[:, 12, 1], x.abs_boundary_ijk[:, 12, 0],
+                          x.abs_boundary_ispec))
+        return [x.abs_boundary_ispec[key], x.abs_boundary_ijk[key], x.abs_boundary_normal[key],
+                x.abs_boundary_jacobian2Dw[key]]
+    for a, b in zip(faces(mm), faces(dm)):
+        assert np.array_equal(a, b)
+    mat, whole = mm.materials(), dm.materials()
+    for nm in ("rho", "kappa", "mu", "rho_vp", "rho_vs"):
+        assert np.array_equal(mat[nm], whole[nm]), nm
+    for c in "xyz":
+        assert np.abs(mat["rmass" + c] / whole["rmass" + c][pi] - 1).max() < 2e-6, c
