@@ -83,6 +83,7 @@ def lib():
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int]
         L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_plant.argtypes = [C.c_int]
         for n in ("orc_nglob", "orc_nfaces", "orc_nspec_irregular", "orc_ncolors"):
             getattr(L, n).restype = C.c_int
             getattr(L, n).argtypes = [C.c_void_p]
@@ -326,6 +327,24 @@ class Oracle:
         f = np.zeros_like(u)
         lib().orc_apply_k(self.h, _p(u, C.c_double), _p(f, C.c_double))
         return f
+
+
+class planted:
+    """Context manager: Oracles created inside run with a planted fault, so a
+    test can show that its check fails on a plausible mistake.
+    1 = Stacey impedances exchanged (rho*vs normal, rho*vp tangential),
+    2 = Mxz and Myz exchanged when the moment tensor is unpacked."""
+
+    def __init__(self, code):
+        self.code = int(code)
+
+    def __enter__(self):
+        lib().orc_set_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0)
+        return False
 
 
 def interp3(vx, vy, vz, valin_fortran, x, y, z):

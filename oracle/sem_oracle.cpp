@@ -475,6 +475,12 @@ struct OrcParams {
 }  // extern "C"
 
 namespace {
+// Planted faults, set only by the tests that show a check fails on a
+// plausible mistake: 1 = Stacey impedances exchanged (rho*vs on the normal
+// term, rho*vp on the tangential term, in the traction and in the mass
+// matrix), 2 = Mxz and Myz exchanged when a moment tensor is unpacked.
+int g_plant = 0;
+
 
 struct SimBase {
   virtual ~SimBase() {}
@@ -547,6 +553,7 @@ struct Sim : SimBase {
       mu[L] = r * b_ * b_;
       rvp[L] = r * a_;
       rvs[L] = r * b_;
+      if (g_plant == 1) std::swap(rvp[L], rvs[L]);
     }
     if (p.aniso) {
       cij.resize(nloc * 21);
@@ -1059,6 +1066,11 @@ void setup_sources(SimBase* s) {
     lagrange(s->g, r[2], hz, hzd);
     const double* M6 = &p.src_mt[6 * q];
     double M[3][3] = {{M6[0], M6[3], M6[4]}, {M6[3], M6[1], M6[5]}, {M6[4], M6[5], M6[2]}};
+    if (g_plant == 2) {
+      std::swap(M[0][2], M[1][2]);
+      M[2][0] = M[0][2];
+      M[2][1] = M[1][2];
+    }
     // G[c][d] = sum_j M[c][j] dref_d/dx_j
     double G[3][3];
     for (int c = 0; c < 3; ++c)
@@ -1110,6 +1122,9 @@ void* orc_create(const OrcParams* p, int precision) {
 }
 
 void orc_destroy(void* h) { delete static_cast<SimBase*>(h); }
+
+// Planted fault for later orc_create calls (0 = none); see g_plant.
+void orc_set_plant(int code) { g_plant = code; }
 
 int orc_nglob(void* h) { return static_cast<SimBase*>(h)->m.nglob; }
 int orc_nfaces(void* h) { return static_cast<SimBase*>(h)->m.nfaces; }

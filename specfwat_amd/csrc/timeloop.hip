@@ -456,24 +456,47 @@ void stage_seismo(SemMesh* m, int it0) {
                      m->d_rec_h, m->d_displ, m->d_seismograms + (size_t)it0 * m->nrec * 3);
 }
 
-// With element slabs a step has 2 x nslabs launches; timing samples every
-// timer_stride(N)-th step (about 256 steps) instead of all of them.
+// Timing samples every timer_stride(N)-th step (at most about 256 steps of a
+// run), so a timed run records a bounded number of events whatever NSTEP is.
 inline int timer_stride(int nstep) { return std::max(1, nstep / 256); }
 
-// Force launches bracketed by HIP events on the simulation stream; the
-// elapsed times are summed after the run, so timing adds no host sync.
+// Steps between host synchronisations of a whole-run loop: bounds the work
+// queued ahead of the device (tens of thousands of launches at NSTEP 10,000).
+constexpr int kSyncEvery = 1024;
+
+// Force or assembly launches bracketed by HIP events on the simulation stream
+// on the sampled steps only; exactly two events per sampled launch are
+// created, and the elapsed times are summed after the run, so timing adds no
+// host sync.
 struct ForceTimer {
   std::vector<hipEvent_t> ev;
   size_t used = 0;
-  void begin(size_t n) {
+  int stride = 1;
+  bool active = false;
+  int launches = 0;
+  // room for per_step bracketed launches on every sampled step of an nstep run
+  void begin(int nstep, size_t per_step) {
     for (auto e : ev) HIP_CHECK(hipEventDestroy(e));
-    ev.assign(2 * n, nullptr);
+    stride = timer_stride(nstep);
+    const size_t samples = (size_t)(nstep + stride - 1) / stride;
+    ev.assign(2 * samples * per_step, nullptr);
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     used = 0;
+    launches = 0;
+    active = false;
   }
-  void mark(hipStream_t s) {
+  // it: 1-based step about to be enqueued
+  void at_step(int it) { active = (it - 1) % stride == 0; }
+  void open(hipStream_t s) {
+    if (!active) return;
     if (used >= ev.size()) SEM_FAIL("force timer overflow");
     HIP_CHECK(hipEventRecord(ev[used++], s));
+  }
+  void close(hipStream_t s, int nlaunch = 1) {
+    if (!active) return;
+    if (used >= ev.size()) SEM_FAIL("force timer overflow");
+    HIP_CHECK(hipEventRecord(ev[used++], s));
+    launches += nlaunch;
   }
   double total_ms() {
     double t = 0;
@@ -490,12 +513,9 @@ struct ForceTimer {
 };
 
 void stage_forces_timed(SemMesh* m, int nfields, int kmode, realw dt, ForceTimer* tm, bool gather) {
-  if (tm) tm->mark(m->stream);
+  if (tm) tm->open(m->stream);
   sem_launch_forces(m, nfields, kmode, 0, m->nspec, dt, nullptr, gather);
-  if (tm) {
-    tm->mark(m->stream);
-    m->last_force_launches += 1;
-  }
+  if (tm) tm->close(m->stream);
 }
 
 }  // namespace
@@ -751,8 +771,10 @@ void fwat_copy_forward_to_backward_(long* Mesh_pointer_from, long* Mesh_pointer_
 // adjoint + reconstruction + kernels.  fused = 1 uses the fused Newmark and
 // the two-field force pass with in-pass kernel accumulation; fused = 0 runs
 // the SPECFEM-style stage sequence (separate predictor/corrector, strain
-// stores, compute_kernels).  timed = 1 brackets every force launch with HIP
-// events on the simulation stream (fwat_last_force_time_ reports the sum).
+// stores, compute_kernels).  timed = 1 brackets the force and assembly
+// launches of every timer_stride(NSTEP)-th step with HIP events on the
+// simulation stream (fwat_last_force_time_ / fwat_last_assembly_time_ report
+// the sums over the sampled launches and their count).
 void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int* fused, int* timed) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   const realw dt = *DT;
@@ -766,13 +788,18 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   ForceTimer* tm = nullptr;
   ForceTimer* ta = nullptr;
   if (*timed) {
-    const size_t per_step = m->nslabs > 0 ? (size_t)m->nslabs : 2;  // timed launches per step
-    const size_t steps = m->nslabs > 0 ? (size_t)(N + timer_stride(N) - 1) / timer_stride(N) : (size_t)N;
-    timer.begin(steps * per_step);
-    atimer.begin(steps * std::max<size_t>(per_step, 1));
+    const size_t per_step = m->nslabs > 0 ? (size_t)m->nslabs : 1;  // bracketed launches per step
+    timer.begin(N, per_step);
+    atimer.begin(N, per_step);
     tm = &timer;
     ta = &atimer;
   }
+  // called at the top of every step of every path below
+  const auto step_begin = [&](int it) {
+    if (tm) tm->at_step(it);
+    if (ta) ta->at_step(it);
+    if (it > 1 && (it - 1) % kSyncEvery == 0) HIP_CHECK(hipStreamSynchronize(m->stream));
+  };
   if (m->nsources > 0 && !m->d_stf) SEM_FAIL("fwat_set_stf_ not called");
   hipStream_t s = m->stream;
   const realw dt2o2 = 0.5f * dt * dt, dto2 = 0.5f * dt;
@@ -783,27 +810,28 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       const int nfp = m->nfaces * 25;
       stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         realw* face = nullptr;
         if (m->absorbing && m->nfaces)
           face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
         AsmField F0{nullptr, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
                     nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
-        if (tm) tm->mark(s);
+        if (tm) tm->open(s);
         sem_launch_brick(m, 1, F0, F0, it == N, 0.0);
-        if (tm) { tm->mark(s); m->last_force_launches += 1; }
+        if (tm) tm->close(s);
         if (face) stage_stacey_face(m, m->d_veloc, face, it);
         stage_seismo(m, it - 1);
-        if (ta) ta->mark(s);
+        if (ta) ta->open(s);
         sem_launch_halo(m, 1, F0, F0, it == N, 0.0);
-        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+        if (ta) ta->close(s);
       }
     } else if (fz && m->nslabs > 0) {
       // slab order: Stacey terms and seismograms read the predicted state only,
       // so they come first; then element pass + assembly slab after slab
       const int nfp = m->nfaces * 25;
-      const int ts = timer_stride(N);
       stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         realw* face = nullptr;
         if (m->absorbing && m->nfaces) {
           face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
@@ -812,21 +840,21 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         stage_seismo(m, it - 1);
         AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
                     nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
-        const bool sample = (it - 1) % ts == 0;
         for (int sl = 0; sl < m->nslabs; ++sl) {
           const int e0 = sl * m->slab_elems, ne = std::min(m->slab_elems, m->nspec - e0);
-          if (tm && sample) tm->mark(s);
+          if (tm) tm->open(s);
           sem_launch_forces(m, 1, 0, e0, ne, dt, nullptr, true);
-          if (tm && sample) { tm->mark(s); m->last_force_launches += 1; }
-          if (ta && sample) ta->mark(s);
+          if (tm) tm->close(s);
+          if (ta) ta->open(s);
           stage_assemble(m, it == N, false, F0, F0, 0.0, sl);
-          if (ta && sample) { ta->mark(s); m->last_asm_launches += 1; }
+          if (ta) ta->close(s);
         }
       }
     } else if (fz) {
       const int nfp = m->nfaces * 25;
       stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         stage_forces_timed(m, 1, 0, dt, tm, true);
         realw* face = nullptr;
         if (m->absorbing && m->nfaces) {
@@ -836,12 +864,13 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         stage_seismo(m, it - 1);
         AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
                     nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
-        if (ta) ta->mark(s);
+        if (ta) ta->open(s);
         stage_assemble(m, it == N, false, F0, F0, 0.0);
-        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+        if (ta) ta->close(s);
       }
     } else {
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         stage_predictor(m, false, dt);
         stage_forces_timed(m, 1, 0, dt, tm, false);
         stage_stacey(m, false, smode, (size_t)(it - 1));
@@ -883,6 +912,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       const int nfp = m->nfaces * 25;
       stage_predictor(m, false, dt);  // adjoint field (from rest)
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
         realw* face_a = (m->absorbing && m->nfaces) ? m->d_face_scratch : nullptr;
         const realw* face_b = (m->absorbing && m->nfaces) ? m->d_stacey_store + (size_t)rec0 * nfp * 3 : nullptr;
@@ -891,19 +921,19 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         AsmField F1{nullptr, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
                     nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
         m->kernel_flush = closes_block(it);
-        if (tm) tm->mark(s);
+        if (tm) tm->open(s);
         sem_launch_brick(m, 2, F0, F1, it == N, dtk);
-        if (tm) { tm->mark(s); m->last_force_launches += 1; }
+        if (tm) tm->close(s);
         if (face_a) stage_stacey_face(m, m->d_veloc, face_a);
-        if (ta) ta->mark(s);
+        if (ta) ta->open(s);
         sem_launch_halo(m, 2, F0, F1, it == N, dtk);
-        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+        if (ta) ta->close(s);
       }
     } else if (fz && m->nslabs > 0) {
       const int nfp = m->nfaces * 25;
-      const int ts = timer_stride(N);
       stage_predictor(m, false, dt);  // adjoint field (from rest)
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
         m->kernel_flush = closes_block(it);
         realw* face_a = nullptr;
@@ -916,21 +946,21 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
                     m->d_displ, m->d_veloc, m->d_accel, dt};
         AsmField F1{m->d_floc_b, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
                     nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
-        const bool sample = (it - 1) % ts == 0;
         for (int sl = 0; sl < m->nslabs; ++sl) {
           const int e0 = sl * m->slab_elems, ne = std::min(m->slab_elems, m->nspec - e0);
-          if (tm && sample) tm->mark(s);
+          if (tm) tm->open(s);
           sem_launch_forces(m, 2, 2, e0, ne, dt, nullptr, true);
-          if (tm && sample) { tm->mark(s); m->last_force_launches += 1; }
-          if (ta && sample) ta->mark(s);
+          if (tm) tm->close(s);
+          if (ta) ta->open(s);
           stage_assemble(m, it == N, true, F0, F1, dtk, sl);
-          if (ta && sample) { ta->mark(s); m->last_asm_launches += 1; }
+          if (ta) ta->close(s);
         }
       }
     } else if (fz) {
       const int nfp = m->nfaces * 25;
       stage_predictor(m, false, dt);  // adjoint field (from rest)
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
         m->kernel_flush = closes_block(it);
         stage_forces_timed(m, 2, 2, dt, tm, true);
@@ -944,23 +974,21 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
                     m->d_displ, m->d_veloc, m->d_accel, dt};
         AsmField F1{m->d_floc_b, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
                     nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
-        if (ta) ta->mark(s);
+        if (ta) ta->open(s);
         stage_assemble(m, it == N, true, F0, F1, dtk);
-        if (ta) { ta->mark(s); m->last_asm_launches += 1; }
+        if (ta) ta->close(s);
       }
     } else {
       for (int it = 1; it <= N; ++it) {
+        step_begin(it);
         const int rec0 = N - it;
         if (it >= 2) stage_predictor(m, true, dt);
         else hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n3)), dim3(TPB), 0, s, m->d_b_accel, n3);
         stage_predictor(m, false, dt);
-        if (tm) tm->mark(s);
+        if (tm) tm->open(s);
         sem_launch_forces(m, 3, 1, 0, m->nspec, dt, nullptr, false);
         sem_launch_forces(m, 1, 1, 0, m->nspec, dt, nullptr, false);
-        if (tm) {
-          tm->mark(s);
-          m->last_force_launches += 2;
-        }
+        if (tm) tm->close(s, 2);
         stage_stacey(m, true, 2, (size_t)rec0);
         if (m->nsources) stage_sources(m, true, m->d_stf + (size_t)rec0 * m->nsources);
         stage_stacey(m, false, 0, 0);
@@ -976,8 +1004,14 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   m->kernel_flush = 1;
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
-  if (tm) m->last_force_ms = tm->total_ms();
-  if (ta) m->last_asm_ms = ta->total_ms();
+  if (tm) {
+    m->last_force_ms = tm->total_ms();
+    m->last_force_launches = tm->launches;
+  }
+  if (ta) {
+    m->last_asm_ms = ta->total_ms();
+    m->last_asm_launches = ta->launches;
+  }
 }
 
 // Sum of the force-kernel durations of the last fwat_iterate_time_ call (ms)

@@ -1,0 +1,301 @@
+"""The engine at the sizes of the benchmark configurations, through the C ABI.
+
+C1  16x16x8 box (40 x 40 x 20 km), seeded 2 % random model, 1 Hz Ricker MT
+    source at 10 km depth, 16 surface receivers, dt 0.005 s, 2,000 steps:
+    observed data in the true model, synthetics in the starting model,
+    band-passed waveform adjoint sources, adjoint + reconstruction + kernels,
+    all against the FP32 and the FP64 oracle (seismograms 1e-5, kernels 1e-4
+    relative L2).
+C3  the C1 event with bf16 and fp16-scaled statics against the FP64 oracle
+    (1e-3), in the perturbed model where the encoding is not exact.
+C2  96x96x48 mesh (28.6 M points): a short window with an early source
+    onset and receivers close to it against the FP32 oracle, and the full
+    10,000-step event (83 GB Stacey store) checked by what does not depend on
+    size: the reconstruction returns to rest, kernels are finite and non-zero.
+C4  64x64x32 mesh: a batch of four events of uneven NSTEP against their
+    single-event runs, and one event against the FP32 oracle.
+C5  64x64x32 HTI model: an event's 21 c_ij kernels against the FP32 oracle;
+    smoothing of 21 kernels in one run against single-field runs and the
+    smoother's loop restatement.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from specfwat_amd.mesh import BoxMesh
+from specfwat_amd.misfit import event_adjoint_sources
+from specfwat_amd.model import hti_model, perturbed_model
+from specfwat_amd.solver import ElasticRun
+from specfwat_amd.sources import Receivers, Sources, surface_grid
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL, KERN_TOL, MIXED_TOL = 1e-5, 1e-4, 1e-3
+LX, LY, LZ = 40000.0, 40000.0, 20000.0
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def oracle(mesh, model, src_xyz, mt, f0, rec_xyz, dt, nstep, precision, cij=None, tshift=None):
+    rho, vp, vs = model
+    return orc.Oracle(mesh.nex, mesh.ney, mesh.nez, mesh.lx, mesh.ly, mesh.lz, rho.ravel(), vp.ravel(),
+                      vs.ravel(), cij=None if cij is None else cij.reshape(-1, 21), dt=dt, nstep=nstep,
+                      src_xyz=src_xyz, src_mt=mt, src_f0=f0, src_tshift=tshift, rec_xyz=rec_xyz,
+                      precision=precision)
+
+
+def gpu_event(mesh, model, src, rec, dt, nstep, adj_from=None, static_precision=0, cij=None):
+    """Forward (with the Stacey store) and, given adjoint sources or a function
+    of the synthetics that makes them, adjoint + kernels.  Returns (syn, adj,
+    kernels, reconstruction end state, forward final state)."""
+    f = ElasticRun(mesh, *model, src, rec, dt, nstep, 1, True, cij=cij, static_precision=static_precision)
+    f.iterate_time()
+    syn = f.seismograms()
+    if adj_from is None:
+        f.close()
+        return syn, None, None, None, None
+    adj = adj_from(syn) if callable(adj_from) else adj_from
+    u_fwd = f.fields()[0]
+    a = ElasticRun(mesh, *model, src, rec, dt, nstep, 3, False, cij=cij, static_precision=static_precision)
+    a.take_forward(f)
+    f.close()
+    a.set_adjoint_sources(adj)
+    a.iterate_time()
+    k = a.kernels()
+    b_end = a.b_fields()[0]
+    a.close()
+    return syn, adj, k, b_end, u_fwd
+
+
+# ------------------------------------------------------------------ C1 / C3
+
+@pytest.fixture(scope="module")
+def c1():
+    say("C1: 16x16x8, 2000 steps, dt 0.005, true + starting model, 16 receivers")
+    mesh = BoxMesh(16, 16, 8, LX, LY, LZ)
+    true_model = perturbed_model(mesh, seed=2024)
+    smooth_model = perturbed_model(mesh, smooth=True)
+    rng = np.random.Generator(np.random.Philox(7))
+    sxyz = [[rng.uniform(0.3, 0.7) * LX, rng.uniform(0.3, 0.7) * LY, -10000.0]]
+    mt = [list(1e15 * rng.standard_normal(6))]
+    rxyz = surface_grid(mesh, 4, 4)
+    src, rec = Sources(mesh, sxyz, mt, [1.0]), Receivers(mesh, rxyz)
+    dt, nstep = 0.005, 2000
+    obs, _, _, _, _ = gpu_event(mesh, true_model, src, rec, dt, nstep)
+
+    def adjoint_sources(syn):
+        return event_adjoint_sources(obs, syn, dt, min_period=0.4, max_period=10.0)[1]
+
+    syn, adj, k, b_end, u_fwd = gpu_event(mesh, smooth_model, src, rec, dt, nstep, adjoint_sources)
+    say("C1: GPU runs done, oracle runs (FP64 and FP32, true and starting model)")
+    ref = {}
+    for prec in (64, 32):
+        ot = oracle(mesh, true_model, sxyz, mt, [1.0], rxyz, dt, nstep, prec)
+        os_ = oracle(mesh, smooth_model, sxyz, mt, [1.0], rxyz, dt, nstep, prec)
+        ref[prec] = dict(obs=ot.forward(save=True), syn=os_.forward(save=True), k=os_.adjoint(adj))
+        if prec == 64:
+            # kernels in the perturbed model too (the mixed-precision statics are exact in
+            # the homogeneous starting model)
+            ref[prec]["k_true"] = ot.adjoint(adj)
+    return dict(mesh=mesh, true_model=true_model, smooth_model=smooth_model, src=src, rec=rec, dt=dt,
+                nstep=nstep, obs=obs, syn=syn, adj=adj, k=k, b_end=b_end, u_fwd=u_fwd, ref=ref)
+
+
+def test_c1_full_event_matches_fp32_and_fp64_oracle(c1):
+    assert np.abs(c1["adj"]).max() > 0
+    for prec in (32, 64):
+        r = c1["ref"][prec]
+        eo, es = rel(c1["obs"], r["obs"]), rel(c1["syn"], r["syn"])
+        ek = {n: rel(c1["k"][n], r["k"][n]) for n in ("rho_kl", "mu_kl", "kappa_kl")}
+        say(f"C1 vs FP{prec} oracle: observed {eo:.2e}, synthetics {es:.2e}, "
+            + ", ".join(f"{n} {v:.2e}" for n, v in ek.items()))
+        assert eo < SEIS_TOL and es < SEIS_TOL
+        assert max(ek.values()) < KERN_TOL
+    rest = np.abs(c1["b_end"]).max() / np.abs(c1["u_fwd"]).max()
+    say(f"C1 reconstruction end state / forward final state: {rest:.2e}")
+    assert rest < 1e-3
+
+
+@pytest.mark.parametrize("sp", [1, 2])
+def test_c3_mixed_precision_statics_vs_fp64_oracle(c1, sp):
+    """bf16 (1) and fp16-scaled (2) Jacobian/moduli statics, fp32 fields, fp64
+    kernels: the C1 event in the perturbed model against the FP64 oracle."""
+    r = c1["ref"][64]
+    say(f"C3: static precision {sp}")
+    obs, _, k, _, _ = gpu_event(c1["mesh"], c1["true_model"], c1["src"], c1["rec"], c1["dt"], c1["nstep"],
+                                c1["adj"], static_precision=sp)
+    es = rel(obs, r["obs"])
+    ek = {n: rel(k[n], r["k_true"][n]) for n in ("rho_kl", "mu_kl", "kappa_kl")}
+    say(f"C3 precision {sp}: seismograms {es:.2e}, " + ", ".join(f"{n} {v:.2e}" for n, v in ek.items()))
+    assert es < MIXED_TOL and max(ek.values()) < MIXED_TOL
+
+
+# ------------------------------------------------------------------ C2
+
+def near_source_case(mesh, seed, f0=4.0, onset=0.05, nrec=8):
+    """A source with its Ricker peak at `onset` s (t0 = 1.2/f0 shifted) and
+    receivers 150-450 m from it, so that a few dozen steps carry a non-zero
+    forward and adjoint field."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    xs = np.array([rng.uniform(0.3, 0.7) * mesh.lx, rng.uniform(0.3, 0.7) * mesh.ly, -rng.uniform(0.3, 0.7) * mesh.lz])
+    mt = [list(1e15 * rng.standard_normal(6))]
+    d = rng.standard_normal((nrec, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rxyz = xs + d * rng.uniform(150.0, 450.0, nrec)[:, None]
+    return [list(xs)], mt, [f0], [onset - 1.2 / f0], rxyz
+
+
+def test_c2_full_mesh_short_window_matches_fp32_oracle():
+    nstep, dt = 20, 0.004
+    say(f"C2 short: 96x96x48 mesh, {nstep} steps, near-source receivers, FP32 oracle")
+    mesh = BoxMesh(96, 96, 48, LX, LY, LZ)
+    model = perturbed_model(mesh, seed=2024)
+    sxyz, mt, f0, tsh, rxyz = near_source_case(mesh, 11)
+    src = Sources(mesh, sxyz, mt, f0, tshift=tsh)
+    rec = Receivers(mesh, rxyz)
+    o = oracle(mesh, model, sxyz, mt, f0, rxyz, dt, nstep, 32, tshift=tsh)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)
+    syn, _, k, b_end, u_fwd = gpu_event(mesh, model, src, rec, dt, nstep, adj)
+    ko = o.adjoint(adj)
+    es = rel(syn, obs)
+    ek = {n: rel(k[n], ko[n]) for n in ("rho_kl", "mu_kl", "kappa_kl")}
+    rest = np.abs(b_end).max() / np.abs(u_fwd).max()
+    say(f"C2 short: seismograms {es:.2e}, " + ", ".join(f"{n} {v:.2e}" for n, v in ek.items())
+        + f", reconstruction end/forward {rest:.2e}")
+    assert np.abs(obs).max() > 0 and all(np.abs(k[n]).max() > 0 for n in ek)
+    assert es < SEIS_TOL and max(ek.values()) < KERN_TOL and rest < 1e-3
+
+
+def test_c2_full_event_reconstruction_returns_to_rest():
+    """The benchmark's C2 event at its full NSTEP: 10,000 Stacey records
+    (8.3 MB each, offsets beyond 2^31 floats) replayed backwards."""
+    nstep, dt = 10000, 0.004
+    say(f"C2 full: 96x96x48 mesh, {nstep} steps")
+    mesh = BoxMesh(96, 96, 48, LX, LY, LZ)
+    model = perturbed_model(mesh, smooth=True)
+    rng = np.random.Generator(np.random.Philox(7))
+    sxyz = [[rng.uniform(0.3, 0.7) * LX, rng.uniform(0.3, 0.7) * LY, -10000.0]]
+    src = Sources(mesh, sxyz, [list(1e15 * rng.standard_normal(6))], [1.0])
+    rec = Receivers(mesh, surface_grid(mesh, 20, 20))
+    syn, adj, k, b_end, u_fwd = gpu_event(mesh, model, src, rec, dt, nstep,
+                                          lambda s: (0.5 * s).astype(np.float32))
+    rest = np.abs(b_end).max() / np.abs(u_fwd).max()
+    say(f"C2 full: max |syn| {np.abs(syn).max():.3e}, reconstruction end/forward final {rest:.2e}, "
+        + ", ".join(f"{n} max {np.abs(v).max():.3e}" for n, v in k.items()))
+    assert np.abs(syn).max() > 0
+    for n, v in k.items():
+        assert np.isfinite(v).all() and np.abs(v).max() > 0, n
+    assert rest < 1e-2
+
+
+# ------------------------------------------------------------------ C4
+
+def test_c4_batch_of_uneven_events_matches_single_event_runs():
+    from specfwat_amd.batch import EventBatch
+    dt = 0.005
+    say("C4: 64x64x32 mesh, batch of 4 events, NSTEP 700/520/900/610")
+    mesh = BoxMesh(64, 64, 32, LX, LY, LZ)
+    model = perturbed_model(mesh, seed=2024)
+    rec = Receivers(mesh, surface_grid(mesh, 20, 20))
+    rng = np.random.Generator(np.random.Philox(31))
+    nsteps = [700, 520, 900, 610]
+    srcs = []
+    for _ in nsteps:
+        xs = [[rng.uniform(0.0, 1.0) * LX, rng.uniform(0.0, 1.0) * LY, -rng.uniform(2000.0, 6000.0)]]
+        f0 = rng.uniform(0.5, 1.5)
+        srcs.append(Sources(mesh, xs, [list(1e15 * rng.standard_normal(6))], [f0], tshift=[0.3 - 1.2 / f0]))
+    b = EventBatch(mesh, *model, srcs, rec, dt, nsteps)
+    b.forward()
+    bsyn = b.seismograms()
+    adj = (0.5 * bsyn).astype(np.float32)
+    b.set_adjoint_sources(adj)
+    b.adjoint()
+    bk = b.kernels()
+    b.close()
+    for q, (src, n) in enumerate(zip(srcs, nsteps)):
+        syn, _, k, _, _ = gpu_event(mesh, model, src, rec, dt, n, np.ascontiguousarray(adj[q, :n]))
+        es = rel(bsyn[q, :n], syn)
+        ek = {name: rel(bk[name][q], k[name]) for name in ("rho_kl", "mu_kl", "kappa_kl")}
+        say(f"C4 event {q} (NSTEP {n}): batch vs single seismograms {es:.2e}, "
+            + ", ".join(f"{nm} {v:.2e}" for nm, v in ek.items()))
+        assert np.abs(syn).max() > 0
+        assert not np.any(bsyn[q, n:])
+        assert es < SEIS_TOL and max(ek.values()) < KERN_TOL
+
+
+def test_c4_mesh_event_matches_fp32_oracle():
+    nstep, dt = 24, 0.005
+    say(f"C4 oracle: 64x64x32 mesh, {nstep} steps, near-source receivers")
+    mesh = BoxMesh(64, 64, 32, LX, LY, LZ)
+    model = perturbed_model(mesh, seed=2024)
+    sxyz, mt, f0, tsh, rxyz = near_source_case(mesh, 13)
+    o = oracle(mesh, model, sxyz, mt, f0, rxyz, dt, nstep, 32, tshift=tsh)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)
+    syn, _, k, _, _ = gpu_event(mesh, model, Sources(mesh, sxyz, mt, f0, tshift=tsh), Receivers(mesh, rxyz),
+                                dt, nstep, adj)
+    ko = o.adjoint(adj)
+    es = rel(syn, obs)
+    ek = {n: rel(k[n], ko[n]) for n in ("rho_kl", "mu_kl", "kappa_kl")}
+    say(f"C4 vs FP32 oracle: seismograms {es:.2e}, " + ", ".join(f"{n} {v:.2e}" for n, v in ek.items()))
+    assert np.abs(obs).max() > 0
+    assert es < SEIS_TOL and max(ek.values()) < KERN_TOL
+
+
+# ------------------------------------------------------------------ C5
+
+def test_c5_hti_event_cijkl_kernels_match_fp32_oracle():
+    nstep, dt = 24, 0.005
+    say(f"C5: 64x64x32 HTI mesh, {nstep} steps, 21 c_ij kernels vs FP32 oracle")
+    mesh = BoxMesh(64, 64, 32, LX, LY, LZ)
+    model = perturbed_model(mesh, seed=2024)
+    cij = hti_model(mesh, *model, seed=77)
+    sxyz, mt, f0, tsh, rxyz = near_source_case(mesh, 17)
+    o = oracle(mesh, model, sxyz, mt, f0, rxyz, dt, nstep, 32, cij=cij, tshift=tsh)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)
+    syn, _, k, _, _ = gpu_event(mesh, model, Sources(mesh, sxyz, mt, f0, tshift=tsh), Receivers(mesh, rxyz),
+                                dt, nstep, adj, cij=cij)
+    ko = o.adjoint(adj)
+    es = rel(syn, obs)
+    ec = rel(k["cijkl_kl"].reshape(-1, 21), ko["cijkl_kl"])
+    er = rel(k["rho_kl"], ko["rho_kl"])
+    say(f"C5 vs FP32 oracle: seismograms {es:.2e}, cijkl_kl {ec:.2e}, rho_kl {er:.2e}")
+    assert np.abs(obs).max() > 0
+    assert es < SEIS_TOL and ec < KERN_TOL and er < KERN_TOL
+
+
+def test_c5_smoothing_of_21_kernels():
+    """21 fields smoothed in one run against single-field runs (three of them)
+    over the whole smoothing, and one field against the loop restatement of
+    smooth.f90 over its first 12 steps (sigma_h 1.2 km, sigma_v 0.6 km on the
+    64x64x32 mesh: 557 steps)."""
+    from specfwat_amd.smooth import smooth_batch, smooth_sem_pde
+    say("C5 smoothing: 64x64x32 mesh, 21 fields")
+    mesh = BoxMesh(64, 64, 32, LX, LY, LZ)
+    rng = np.random.default_rng(21)
+    fields = rng.standard_normal((21, mesh.nspec, 125)).astype(np.float32)
+    sh, sv = 1200.0, 600.0
+    out, nt, _, _ = smooth_batch(mesh, fields, sh, sv, faithful=False)
+    assert np.isfinite(out).all()
+    for q in (0, 10, 20):
+        _, d1, nt1 = smooth_sem_pde(mesh, fields[q], sh, sv, faithful=False)
+        e = rel(out[q], d1)
+        say(f"C5 smoothing field {q}: {nt} steps, batch vs single {e:.2e}")
+        assert nt1 == nt and e < 1e-6
+    steps = 12
+    _, d12, _ = smooth_sem_pde(mesh, fields[3], sh, sv, faithful=False, nstep=steps)
+    _, dgo, _ = orc.smooth_pde(mesh.smooth_arrays(), fields[3].ravel(), sh, sv, mesh.distance_min_glob(),
+                               faithful=False, max_steps=steps)
+    e = rel(d12, dgo)
+    say(f"C5 smoothing vs loop restatement over {steps} steps: {e:.2e}")
+    assert e < 1e-5
