@@ -30,6 +30,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# steps per fp32 partial-sum block of the time-integrated kernels (fp64 added at each block's end)
+KERNEL_BLOCK = 16
+
+
+def kernel_acc_bytes(kb):
+    """HBM bytes per point-step of one time-integrated kernel value: fp32 partial sum read +
+    written every step, fp64 accumulator read + written once per kb-step block."""
+    return 8.0 + 16.0 / kb if kb > 1 else 16.0
 
 
 def parse():
@@ -63,6 +71,9 @@ def parse():
     p.add_argument("--tuning", default="", metavar="R,F,A",
                    help="fixed launch shape instead of the autotuner: regular_path, forward and adjoint "
                         "workgroups per CU (e.g. 1,16,8), as the autotuner picks them")
+    p.add_argument("--window-start", type=int, default=-1, metavar="S0",
+                   help="the timed event covers steps S0+1..S0+K, the state at S0 from an untimed forward run "
+                        "(default: 0 for K >= 2000, else the first P arrival at the nearest receiver + 50 steps)")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     return p.parse_args()
@@ -112,7 +123,7 @@ def main():
 
     w = workload(a.config)
     K, W = a.steps or w["nstep"], a.warmup
-    cpu_steps = a.cpu_steps or (K if a.config == "c1" else 4)
+    cpu_steps = a.cpu_steps or (K if a.config == "c1" else 6)
     t0 = time.time()
     mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
     log(f"mesh {mesh.nspec} elements, {mesh.nglob} points ({time.time() - t0:.1f}s)")
@@ -122,9 +133,20 @@ def main():
     sxyz = [[rng.uniform(0.3, 0.7) * w["lx"], rng.uniform(0.3, 0.7) * w["ly"], -10000.0]]
     mt = [list(1e15 * rng.standard_normal(6))]
     src = Sources(mesh, sxyz, mt, [1.0])
-    rec = Receivers(mesh, surface_grid(mesh, w["nrx"], w["nrx"]))
+    rxyz = surface_grid(mesh, w["nrx"], w["nrx"])
+    rec = Receivers(mesh, rxyz)
     dt = w["dt"]
-    log(f"model/source/receivers ready ({time.time() - t0:.1f}s)")
+    # A short timed event (the driver's --steps 20) would end before any wave reaches a
+    # receiver, leaving a zero misfit and adjoint field.  It then covers the event's steps
+    # S0+1..S0+K, S0 being the first P arrival at the nearest receiver plus 50 steps: the
+    # forward state at step S0 comes from an untimed run of the same event (starting and true
+    # model alike) and the reconstruction returns to it.
+    S0 = a.window_start
+    if S0 < 0:
+        dmin = float(np.min(np.linalg.norm(rxyz - np.asarray(sxyz[0]), axis=1)))
+        S0 = 0 if K >= 2000 else int(np.ceil((1.2 / 1.0 + dmin / 6000.0) / dt)) + 50
+    src_w = Sources(mesh, sxyz, mt, [1.0], tshift=[-S0 * dt]) if S0 else src
+    log(f"model/source/receivers ready ({time.time() - t0:.1f}s); timed window: steps {S0 + 1}..{S0 + K}")
     fused = not a.unfused
     tuning = {"regular_path": 1, "blocks_per_cu_fwd": 0, "blocks_per_cu_adj": 8}
     if a.tuning:
@@ -137,21 +159,41 @@ def main():
         log("autotuner (regular_path, blocks/CU, fwd ms, adj ms): "
             + ", ".join(f"({r},{b},{x:.3f},{y:.3f})" for r, b, x, y in table) + f" -> {tuning}")
 
-    def observed(nstep):
-        r = ElasticRun(mesh, *true_model, src, rec, dt, nstep, 1, False)
+    def primed_state(model):
+        """Forward state of the event at step S0 (None when the window starts at rest)."""
+        if not S0:
+            return None
+        r = ElasticRun(mesh, *model, src, rec, dt, S0, 1, False)
         r.set_tuning(**tuning)
+        r.iterate_time()
+        st = r.fields()
+        r.close()
+        return st
+
+    state_true = primed_state(true_model)
+    state_smooth = primed_state(smooth_model)
+
+    def observed(nstep, window=False):
+        r = ElasticRun(mesh, *true_model, src_w if window else src, rec, dt, nstep, 1, False)
+        r.set_tuning(**tuning)
+        if window and state_true is not None:
+            r.set_fields(*state_true)
         r.iterate_time()
         obs = r.seismograms()
         r.close()
         return obs
 
-    def prepare(nstep):
-        f = ElasticRun(mesh, *smooth_model, src, rec, dt, nstep, 1, True,
+    def prepare(nstep, window=False):
+        s_ = src_w if window else src
+        f = ElasticRun(mesh, *smooth_model, s_, rec, dt, nstep, 1, True,
                        static_precision=a.static_precision)
-        ad = ElasticRun(mesh, *smooth_model, src, rec, dt, nstep, 3, False,
+        ad = ElasticRun(mesh, *smooth_model, s_, rec, dt, nstep, 3, False,
                         compute_and_store_strain=not fused, static_precision=a.static_precision)
         f.set_tuning(**tuning)
         ad.set_tuning(**tuning)
+        ad.set_kernel_block(KERNEL_BLOCK)
+        if window and state_smooth is not None:
+            f.set_fields(*state_smooth)
         if a.fk:
             # synthetic incident field: a separable pulse times a fixed pattern over the face points
             nsamp = (nstep - 1) // a.fk + 1
@@ -182,9 +224,9 @@ def main():
         event(f, ad, obs_w, False)
         f.close(); ad.close()
         log(f"warmup event of {W} steps done ({time.time() - t0:.1f}s)")
-    obs = observed(K)
+    obs = observed(K, window=True)
     log(f"observed data ({K} steps) generated ({time.time() - t0:.1f}s)")
-    f, ad = prepare(K)
+    f, ad = prepare(K, window=True)
     _lib.call("fwat_synchronize_", f.h)
     _lib.call("fwat_synchronize_", ad.h)
     ts = time.time()
@@ -212,37 +254,39 @@ def main():
     adj_asm_ms, adj_asm_n = a_asm
     # Algorithmic bytes per launch (each array touched once, in bytes):
     kmu = 8.0 if a.static_precision == 0 else 4.0  # kappa,mu bytes per local point
+    kacc = kernel_acc_bytes(KERNEL_BLOCK)  # one time-integrated kernel value per point-step
     if brick:
         nh, nslot, nch = binfo
         ninner = nglob - nh
         bname = "x".join(str(v) for v in brick)
         kinds = {
-            # brick pass, two fields: ibool 4 + kappa,mu per local point, fp64 mu/kappa read+write
-            # 32 per local point, displacement gathers 12 + 12 per unique point; brick-inner
-            # points: rmass_inv 12, u,v of both fields read + written 96, fp64 rho kernel 16;
+            # brick pass, two fields: ibool 4 + kappa,mu per local point, mu/kappa accumulators
+            # 2 x kacc per local point, displacement gathers 12 + 12 per unique point; brick-inner
+            # points: rmass_inv 12, u,v of both fields read + written 96, rho kernel kacc;
             # partial sums of both fields written 24 per slot
-            f"k_brick<{bname},two-field>": ((4.0 + kmu) * nloc + 32.0 * nloc + 24.0 * nglob
-                                            + 124.0 * ninner + 24.0 * nslot, adj_ms, adj_n),
+            f"k_brick<{bname},two-field>": ((4.0 + kmu + 2 * kacc) * nloc + 24.0 * nglob
+                                            + (108.0 + kacc) * ninner + 24.0 * nslot, adj_ms, adj_n),
             # halo pass, two fields: point list + CSR offsets 8 per halo point, 4 per contribution,
             # partial sums 24 per slot, Stacey terms 24 per face point, rmass_inv 12, u,v of both
-            # fields 96 and the rho kernel 16 per halo point
-            "k_halo<two-field>": (8.0 * nh + 4.0 * nch + 24.0 * nslot + 24.0 * nfp + 124.0 * nh,
+            # fields 96 and the rho kernel kacc per halo point
+            "k_halo<two-field>": (8.0 * nh + 4.0 * nch + 24.0 * nslot + 24.0 * nfp + (108.0 + kacc) * nh,
                                   adj_asm_ms, adj_asm_n),
         }
         fwd_bytes = (4.0 + kmu) * nloc + 12.0 * nglob + 60.0 * ninner + 12.0 * nslot
         fwd_asm_bytes = 8.0 * nh + 4.0 * nch + 12.0 * nslot + 12.0 * nfp + 60.0 * nh
     elif fused:
         # two-field element pass: ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local
-        # point, fp64 mu/kappa accumulators read+write 32, element-local forces written
-        # for both fields 24; displacement gather of both fields 12 + 12 per unique point
+        # point, mu/kappa accumulators 2 x kacc, element-local forces written for both fields
+        # 24; displacement gather of both fields 12 + 12 per unique point
         kinds = {
-            "k_forces<2,iso,fused-kernels>": ((60.0 + kmu) * nloc + 40.0 * nirr + 24.0 * nglob, adj_ms, adj_n),
+            "k_forces<2,iso,fused-kernels>": ((28.0 + kmu + 2 * kacc) * nloc + 40.0 * nirr + 24.0 * nglob,
+                                              adj_ms, adj_n),
             # two-field assembly: CSR offsets 4 per point + indices 4 per contribution,
             # element-local forces of both fields 24 per local point, Stacey terms of both
             # fields 24 per face point, rmass_inv 12, u,v of both fields read 48 + write 48,
-            # fp64 rho kernel read+write 16 per unique point
+            # rho kernel kacc per unique point
             "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp
-                                      + 12.0 * nglob + 96.0 * nglob + 16.0 * nglob, adj_asm_ms, adj_asm_n),
+                                      + 12.0 * nglob + 96.0 * nglob + kacc * nglob, adj_asm_ms, adj_asm_n),
         }
         fwd_bytes = (16.0 + kmu) * nloc + 40.0 * nirr + 12.0 * nglob
         fwd_asm_bytes = 4.0 * nglob + 4.0 * ncontrib + 12.0 * nloc + 12.0 * nfp + 12.0 * nglob + 48.0 * nglob
@@ -286,7 +330,12 @@ def main():
                                 f"{w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, NGLL=5, "
                                 f"{rec.nrec} receivers, dt={dt}, NSTEP={K}, fwd + waveform misfit (adjoint sources "
                                 "band-passed 0.4-10 s) + adjoint/reconstruction + rho/mu/kappa kernels "
-                                "(fp64 accum)"),
+                                f"(fp64 accum, fp32 partial sums over {KERNEL_BLOCK} steps)"),
+                   "window": {"steps": [S0 + 1, S0 + K], "misfit": misfit,
+                              "note": ("whole event from rest" if S0 == 0 else
+                                       f"event steps {S0 + 1}..{S0 + K}: the forward state at step {S0} (after "
+                                       "the first P arrival at the nearest receiver) from an untimed run; "
+                                       "the reconstruction returns to it")},
                    "path": (f"brick-fused {'x'.join(map(str, brick))}" if brick else
                             (f"fused, {nsl} element slabs" if nsl > 1 else "fused"))
                            if fused else "SPECFEM stage order (strain stores)",
@@ -308,7 +357,9 @@ def main():
     }
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, sxyz, mt, rec, cpu_steps,
-                                              label=("C1 event" if a.config == "c1" else "C2 event"))
+                                              label=("C1 event" if a.config == "c1" else "C2 event"),
+                                              nstep_event=w["nstep"],
+                                              one_thread_steps=min(cpu_steps, 20 if a.config == "c1" else 1))
     print(json.dumps(result), flush=True)
 
 
@@ -484,7 +535,8 @@ def batched(a):
     if not a.no_cpu:
         s0 = srcs[0]
         result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, s0.xyz.tolist(), s0.mt.tolist(), rec,
-                                              a.cpu_steps or 4, f0=float(s0.f0[0]), label="C4 event")
+                                              a.cpu_steps or 4, f0=float(s0.f0[0]), label="C4 event",
+                                              nstep_event=int(np.mean(nsteps)))
     print(json.dumps(result), flush=True)
 
 
@@ -981,30 +1033,74 @@ def post_cpu_baseline(mesh, grid, files, hfiles, opt_hist):
                        "and numpy, one thread; smoothing not sampled")}
 
 
-def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event"):
-    """The oracle (C++/OpenMP FP32 restatement) on the host cores, bounded
-    sample: the same mesh/model/source/receivers, nsteps forward steps with
-    the Stacey store then nsteps adjoint+reconstruction+kernel steps."""
+def host_info():
+    """Host threads available to this process and the CPU model name."""
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, model
+
+
+def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event", nstep_event=None, cij=None,
+                 one_thread_steps=1, repeats=3):
+    """The oracle (C++/OpenMP FP32 restatement of the SPECFEM CPU time loop) on the
+    host cores, on a bounded sample of the same event: nsteps forward steps with the
+    Stacey store, then nsteps adjoint + reconstruction + kernel steps, each timed
+    apart; the median of `repeats` runs on all host threads, plus one run of
+    one_thread_steps on one thread.  The per-step times extrapolate linearly to
+    the event's NSTEP (set-up excluded)."""
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    threads, cpu_model = host_info()
     rho, vp, vs = model
-    rxyz = None
     from specfwat_amd.sources import surface_grid
     rxyz = surface_grid(mesh, w["nrx"], w["nrx"])
+
+    def make(n, nthreads):
+        return orc.Oracle(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], rho.ravel(), vp.ravel(),
+                          vs.ravel(), cij=None if cij is None else cij.reshape(-1, 21), dt=w["dt"], nstep=n,
+                          src_xyz=sxyz, src_mt=mt, src_f0=[f0], rec_xyz=rxyz, precision=32, nthreads=nthreads)
+
+    def timed_run(o):
+        ts = time.time()
+        s = o.forward(save=True)
+        tf = time.time()
+        o.adjoint(s.astype(np.float32))
+        return tf - ts, time.time() - tf
+
     t0 = time.time()
-    o = orc.Oracle(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], rho.ravel(), vp.ravel(),
-                   vs.ravel(), dt=w["dt"], nstep=nsteps, src_xyz=sxyz, src_mt=mt, src_f0=[f0],
-                   rec_xyz=rxyz, precision=32, nthreads=threads)
-    log(f"cpu baseline set-up {time.time() - t0:.1f}s ({threads} threads)")
-    ts = time.time()
-    s = o.forward(save=True)
-    o.adjoint(s.astype(np.float32))
-    wall = time.time() - ts
-    log(f"cpu baseline {nsteps} fwd+adj steps: {wall:.2f}s")
-    return {"value": mesh.nglob * nsteps / wall, "unit": "GLL-pt-steps/s", "cores": threads,
+    o = make(nsteps, threads)
+    log(f"cpu baseline set-up {time.time() - t0:.1f}s ({threads} threads, {cpu_model})")
+    runs = [timed_run(o) for _ in range(repeats)]
+    del o
+    fwd = float(np.median([r[0] for r in runs]))
+    adj = float(np.median([r[1] for r in runs]))
+    log(f"cpu baseline {nsteps} fwd + {nsteps} adj steps, median of {repeats}: forward {fwd:.2f}s, "
+        f"adjoint+kernels {adj:.2f}s")
+    o1 = make(one_thread_steps, 1)
+    f1, a1 = timed_run(o1)
+    del o1
+    log(f"cpu baseline one thread, {one_thread_steps} fwd + adj steps: forward {f1:.2f}s, adjoint {a1:.2f}s")
+    ne = nstep_event or nsteps
+    return {"value": mesh.nglob * nsteps / (fwd + adj), "unit": "GLL-pt-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"same {label}, {nsteps} forward + {nsteps} adjoint/reconstruction/kernel steps, "
-                      f"FP32 oracle (C++/OpenMP, colour-ordered assembly), {wall:.1f}s"}
+            "sample": (f"same {label} from rest, {nsteps} forward + {nsteps} adjoint/reconstruction/kernel "
+                       f"steps, FP32 oracle (C++/OpenMP, colour-ordered assembly), median of {repeats}"),
+            "host": {"threads": threads, "cpu_model": cpu_model},
+            "forward_s_per_step": fwd / nsteps, "adjoint_kernels_s_per_step": adj / nsteps,
+            "one_thread": {"forward_s_per_step": f1 / one_thread_steps,
+                           "adjoint_kernels_s_per_step": a1 / one_thread_steps,
+                           "value": mesh.nglob * one_thread_steps / (f1 + a1)},
+            "event_extrapolated_s": {"nstep": ne, "forward": fwd / nsteps * ne, "adjoint_kernels": adj / nsteps * ne,
+                                     "note": "linear in NSTEP from the sample, set-up excluded"}}
 
 
 if __name__ == "__main__":

@@ -173,6 +173,18 @@ void fwat_set_slabs_(long *Mesh_pointer, int *elems_per_slab);
  * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
  * fp64).  The rho kernel (summed in the assembly pass) uses the same blocks. */
 void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
+/* Host-only (no device work): the gather-assembly CSR the preparation entry points build for
+ * these arrays -- for every point its element terms in the order of SPECFEM's CPU element loop
+ * over phase_ispec_inner_elastic (outer elements, then inner; the loop shape of
+ * smooth.f90:243-247; phase arrays as read at read_mesh_databases.f90:1129-1143), then its
+ * Stacey face, source and receiver terms.  off (NGLOB_AB+1), idx (0-based contributions:
+ * 125 per element, 25 per face, 125 per source, 125 per receiver).  No counterpart in the
+ * reference: it exposes the summation order so it can be checked without a GPU. */
+void fwat_assembly_csr_(int *NSPEC_AB, int *NGLOB_AB, int *h_ibool, int *num_abs_boundary_faces,
+                        int *h_abs_boundary_ispec, int *h_abs_boundary_ijk, int *NSOURCES,
+                        int *h_ispec_selected_source, int *nrec, int *h_ispec_selected_rec,
+                        int *num_phase_ispec_elastic, int *phase_ispec_inner_elastic,
+                        int *nspec_outer_elastic, int *nspec_inner_elastic, int *off, int *idx);
 /* 0 fp32, 1 bf16, 2 fp16 with per-array scaling, for kappa/mu and the Jacobian */
 void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 

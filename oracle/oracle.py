@@ -58,6 +58,7 @@ class OrcParams(C.Structure):
         ("nrec", C.c_int),
         ("rec_xyz", _dp),
         ("nthreads", C.c_int),
+        ("elem_order", _ip),
     ]
 
 
@@ -84,6 +85,7 @@ def lib():
         L.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_plant.argtypes = [C.c_int]
+        L.orc_assembly_order.argtypes = [C.c_void_p, _ip, _ip]
         for n in ("orc_nglob", "orc_nfaces", "orc_nspec_irregular", "orc_ncolors"):
             getattr(L, n).restype = C.c_int
             getattr(L, n).argtypes = [C.c_void_p]
@@ -192,7 +194,11 @@ class Oracle:
 
     def __init__(self, nex, ney, nez, lx, ly, lz, rho, vp, vs, *, cij=None, dt=0.005, nstep=10,
                  stacey=True, src_xyz=(), src_mt=(), src_f0=(), src_tshift=None, rec_xyz=(),
-                 perturb=0.0, force_irregular=False, numbering=1, precision=32, nthreads=0):
+                 perturb=0.0, force_irregular=False, numbering=1, precision=32, nthreads=0,
+                 elem_order=None):
+        """elem_order: 0-based visiting order of the element loop that assembles the
+        internal forces (SPECFEM's CPU loop over phase_ispec_inner_elastic); None
+        assembles colour by colour."""
         self._keep = []
 
         def f32(a):
@@ -220,11 +226,17 @@ class Oracle:
             _p(rho, C.c_float), _p(vp, C.c_float), _p(vs, C.c_float), _p(cijp, C.c_float),
             dt, nstep, int(stacey), nsrc, _p(src_xyz, C.c_double), _p(src_mt, C.c_double),
             _p(src_f0, C.c_double), _p(src_tshift, C.c_double), self.nrec, _p(rec_xyz, C.c_double),
-            nthreads)
+            nthreads, None)
+        if elem_order is not None:
+            eo = np.ascontiguousarray(elem_order, np.int32)
+            self._keep.append(eo)
+            self.params.elem_order = _p(eo, C.c_int)
         self.nspec = nex * ney * nez
         self.nstep = nstep
         self.aniso = cij is not None
         self.h = lib().orc_create(C.byref(self.params), precision)
+        if not self.h:
+            raise ValueError("elem_order is not a permutation of the elements")
         self.nglob = lib().orc_nglob(self.h)
         self.nfaces = lib().orc_nfaces(self.h)
 
@@ -247,6 +259,14 @@ class Oracle:
                            _p(color, C.c_int))
         return dict(ibool=ibool, xyz=xyz, irregular=irreg, xix_regular=reg[0],
                     jacobian_regular=reg[1], jinv=jinv, jacobian=jac, color=color)
+
+    def assembly_order(self):
+        """(off [nglob+1], idx [nspec*125]): for every point, the element terms
+        (e*125 + L) in the order the oracle adds them."""
+        off = np.zeros(self.nglob + 1, np.int32)
+        idx = np.zeros(self.nspec * 125, np.int32)
+        lib().orc_assembly_order(self.h, _p(off, C.c_int), _p(idx, C.c_int))
+        return off, idx
 
     def faces(self):
         nf = self.nfaces

@@ -470,6 +470,9 @@ struct OrcParams {
   int nrec;
   const double* rec_xyz;    // [nrec][3]
   int nthreads;
+  // visiting order of the element loop that assembles the internal forces
+  // (0-based, a permutation of the elements), or null: colour by colour
+  const int* elem_order;
 };
 
 }  // extern "C"
@@ -505,8 +508,27 @@ struct SimBase {
   // velocity and traction [fk_nsamp][nfaces*25][3], sample k at step 1 + k*fk_resamp
   std::vector<float> fk_v, fk_t;
   int fk_nsamp = 0, fk_resamp = 1;
+  // per-point assembly order of the element terms (contribution e*125 + L) when
+  // the forces are assembled in an element visiting order (OrcParams::elem_order)
+  std::vector<int> asm_off, asm_idx;
   virtual void apply_k(const double* u, double* f) = 0;
 };
+
+// Per-point lists of the element terms in the order a sequential loop over the
+// elements in `visit` order adds them (each element's points in k, j, i order).
+void assembly_order(const Mesh& m, const std::vector<int>& visit, std::vector<int>& off,
+                    std::vector<int>& idx) {
+  off.assign((size_t)m.nglob + 1, 0);
+  for (size_t L = 0; L < m.ibool.size(); ++L) off[(size_t)m.ibool[L] + 1]++;
+  for (int g = 0; g < m.nglob; ++g) off[(size_t)g + 1] += off[g];
+  idx.assign(m.ibool.size(), -1);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int e : visit)
+    for (int L = 0; L < NG3; ++L) {
+      const size_t c = (size_t)e * NG3 + L;
+      idx[fill[m.ibool[c]]++] = (int)c;
+    }
+}
 
 template <class T>
 struct Sim : SimBase {
@@ -521,6 +543,7 @@ struct Sim : SimBase {
   std::vector<T> rh;                                  // [nrec][125]
   std::vector<double> energy;
   bool saved = false;
+  std::vector<T> floc;                                // element terms [nspec*125][3] (ordered assembly)
 
   void setup() {
     size_t nloc = (size_t)m.nspec * NG3;
@@ -599,6 +622,7 @@ struct Sim : SimBase {
       rmy[gI] = (T)1 / rmy[gI];
       rmz[gI] = (T)1 / rmz[gI];
     }
+    if (!asm_off.empty()) floc.assign(nloc * 3, (T)0);
     sarr.resize(srcarr.size());
     for (size_t q = 0; q < srcarr.size(); ++q) sarr[q] = (T)srcarr[q];
     rh.resize(rech.size());
@@ -607,7 +631,9 @@ struct Sim : SimBase {
 
   // Element internal forces: accel -= K_e u  (weak form); optionally the 6
   // strain components per local point (exx,eyy,ezz,exy,exz,eyz).
-  void elem(int e, const T* U, T* A, T* strain) const {
+  // F: when given, the element's nodal force terms are stored at F[3 (125 e + L)]
+  // (already negated) instead of being subtracted from A
+  void elem(int e, const T* U, T* A, T* strain, T* F = nullptr) const {
     T ux[NG3], uy[NG3], uz[NG3];
     T t1x[NG3], t1y[NG3], t1z[NG3], t2x[NG3], t2y[NG3], t2z[NG3], t3x[NG3], t3y[NG3], t3z[NG3];
     const int* ib = &m.ibool[(size_t)e * NG3];
@@ -705,6 +731,13 @@ struct Sim : SimBase {
             nx3 += t3x[Lc] * hpw[l][k]; ny3 += t3y[Lc] * hpw[l][k]; nz3 += t3z[Lc] * hpw[l][k];
           }
           T f1 = wxy[j][k], f2 = wxy[i][k], f3 = wxy[i][j];
+          if (F) {
+            T* o = &F[3 * ((size_t)e * NG3 + L)];
+            o[0] = -(f1 * nx1 + f2 * nx2 + f3 * nx3);
+            o[1] = -(f1 * ny1 + f2 * ny2 + f3 * ny3);
+            o[2] = -(f1 * nz1 + f2 * nz2 + f3 * nz3);
+            continue;
+          }
           size_t gI = (size_t)ib[L];
           A[3 * gI] -= f1 * nx1 + f2 * nx2 + f3 * nx3;
           A[3 * gI + 1] -= f1 * ny1 + f2 * ny2 + f3 * ny3;
@@ -712,11 +745,28 @@ struct Sim : SimBase {
         }
   }
 
+  // Internal forces added into A.  Colour by colour (each colour race-free),
+  // or, with an element visiting order, element terms first and then every
+  // point's terms added in the order a sequential loop over that visiting
+  // order would add them (the same per-point sums).
   void forces(const T* U, T* A) {
-    for (auto& set : m.color_sets) {
+    if (asm_off.empty()) {
+      for (auto& set : m.color_sets) {
 #pragma omp parallel for schedule(static) num_threads(p.nthreads)
-      for (size_t q = 0; q < set.size(); ++q) elem(set[q], U, A, nullptr);
+        for (size_t q = 0; q < set.size(); ++q) elem(set[q], U, A, nullptr);
+      }
+      return;
     }
+#pragma omp parallel for schedule(static) num_threads(p.nthreads)
+    for (int e = 0; e < m.nspec; ++e) elem(e, U, nullptr, nullptr, floc.data());
+#pragma omp parallel for schedule(static) num_threads(p.nthreads)
+    for (int gI = 0; gI < m.nglob; ++gI)
+      for (int q = asm_off[gI]; q < asm_off[gI + 1]; ++q) {
+        const T* f = &floc[3 * (size_t)asm_idx[q]];
+        A[3 * (size_t)gI] += f[0];
+        A[3 * (size_t)gI + 1] += f[1];
+        A[3 * (size_t)gI + 2] += f[2];
+      }
   }
 
   void predictor(std::vector<T>& U, std::vector<T>& V, std::vector<T>& A, T dt) {
@@ -1116,12 +1166,35 @@ void* orc_create(const OrcParams* p, int precision) {
   build_mesh(s->m, s->g, p->nex, p->ney, p->nez, p->lx, p->ly, p->lz, p->perturb,
              p->force_irregular, p->numbering);
   setup_sources(s);
+  if (p->elem_order) {
+    std::vector<int> visit(p->elem_order, p->elem_order + s->m.nspec), seen(s->m.nspec, 0);
+    for (int e : visit) {
+      if (e < 0 || e >= s->m.nspec || seen[e]++) { delete s; return nullptr; }
+    }
+    assembly_order(s->m, visit, s->asm_off, s->asm_idx);
+  }
   if (precision == 64) static_cast<Sim<double>*>(s)->setup();
   else static_cast<Sim<float>*>(s)->setup();
   return s;
 }
 
 void orc_destroy(void* h) { delete static_cast<SimBase*>(h); }
+
+// Per-point order in which the oracle adds the element terms of the internal
+// forces: off[nglob+1], idx[nspec*125] (contribution e*125 + L).
+void orc_assembly_order(void* h, int* off, int* idx) {
+  SimBase* s = static_cast<SimBase*>(h);
+  std::vector<int> o, i;
+  if (s->asm_off.empty()) {
+    std::vector<int> visit;
+    for (auto& set : s->m.color_sets) visit.insert(visit.end(), set.begin(), set.end());
+    assembly_order(s->m, visit, o, i);
+  }
+  const std::vector<int>& O = s->asm_off.empty() ? o : s->asm_off;
+  const std::vector<int>& I = s->asm_off.empty() ? i : s->asm_idx;
+  std::memcpy(off, O.data(), O.size() * sizeof(int));
+  std::memcpy(idx, I.data(), I.size() * sizeof(int));
+}
 
 // Planted fault for later orc_create calls (0 = none); see g_plant.
 void orc_set_plant(int code) { g_plant = code; }

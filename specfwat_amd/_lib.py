@@ -60,6 +60,7 @@ SIGNATURES = {
     "fwat_set_brick_": "Liiii",
     "fwat_brick_info_": "Lill",
     "fwat_set_kernel_block_": "Li",
+    "fwat_assembly_csr_": "iiIiIIiIiIiIiiII",
     "fwat_set_slabs_": "Li",
     "fwat_batch_create_": "LLiIfFID",
     "fwat_batch_set_tuning_": "Li",

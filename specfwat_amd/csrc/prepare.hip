@@ -33,6 +33,78 @@ uint16_t f2h16(float f) {
 
 uint16_t f2lp(float f, int sp) { return sp == SEM_STATIC_BF16 ? f2bf16(f) : f2h16(f); }
 
+// Element visiting order of SPECFEM's CPU element loop: phase_ispec_inner_elastic
+// (num_phase_ispec, 2), 1-based, outer elements (iphase 1) then inner ones
+// (iphase 2) -- the loop shape of smooth.f90:243-247.  False (visit empty)
+// unless the two lists together hold every element exactly once.
+bool phase_visit_order(int nspec, int num_phase_ispec, const int* phase_ispec_inner, int nspec_outer,
+                       int nspec_inner, std::vector<int>& visit) {
+  visit.clear();
+  if (nspec_outer < 0 || nspec_inner < 0 || nspec_outer + nspec_inner != nspec ||
+      std::max(nspec_outer, nspec_inner) > num_phase_ispec)
+    return false;
+  std::vector<int> seen(nspec, 0);
+  visit.reserve(nspec);
+  for (int iphase = 0; iphase < 2; ++iphase) {
+    const int n = iphase == 0 ? nspec_outer : nspec_inner;
+    for (int q = 0; q < n; ++q) {
+      const int e = phase_ispec_inner[(size_t)iphase * num_phase_ispec + q] - 1;
+      if (e < 0 || e >= nspec || seen[e]++) {
+        visit.clear();
+        return false;
+      }
+      visit.push_back(e);
+    }
+  }
+  return true;
+}
+
+// Gather-assembly CSR.  owner[q] is the global point of contribution q: the
+// element-local points (125 per element, element-major) first, then the Stacey
+// face points, the source points and the receiver points.  Every point's row
+// lists its element terms in the order a sequential loop over the elements in
+// `visit` order adds them, then its face, source and receiver terms.  Summing
+// a row in this order makes the assembled force independent of scheduling.
+void build_assembly_csr(int nglob, int nspec, const std::vector<int>& owner, const std::vector<int>& visit,
+                        std::vector<int>& off, std::vector<int>& idx) {
+  const size_t nloc = (size_t)nspec * 125;
+  if (owner.size() >= (size_t)INT32_MAX) SEM_FAIL("too many contributions for 32-bit CSR indices");
+  off.assign((size_t)nglob + 1, 0);
+  for (size_t q = 0; q < owner.size(); ++q) off[(size_t)owner[q] + 1]++;
+  for (int g = 0; g < nglob; ++g) off[(size_t)g + 1] += off[g];
+  idx.assign(owner.size(), 0);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int e : visit)
+    for (int L = 0; L < 125; ++L) {
+      const size_t q = (size_t)e * 125 + L;
+      idx[fill[owner[q]]++] = (int)q;
+    }
+  for (size_t q = nloc; q < owner.size(); ++q) idx[fill[owner[q]]++] = (int)q;
+}
+
+// Host copy of the contribution owners of a prepared mesh (see build_assembly_csr).
+std::vector<int> contribution_owners(SemMesh* m) {
+  const size_t nloc = (size_t)m->nspec * 125, nfp = (size_t)m->nfaces * 25;
+  const size_t nsp = (size_t)m->nsources * 125, nrp = (size_t)m->nrec * 125;
+  std::vector<int> owner(nloc + nfp + nsp + nrp);
+  hipStream_t s = m->stream;
+  HIP_CHECK(hipMemcpyAsync(owner.data(), m->d_ibool, nloc * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (nfp) HIP_CHECK(hipMemcpyAsync(owner.data() + nloc, m->d_face_iglob, nfp * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (nsp) HIP_CHECK(hipMemcpyAsync(owner.data() + nloc + nfp, m->d_src_iglob, nsp * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (nrp)
+    HIP_CHECK(hipMemcpyAsync(owner.data() + nloc + nfp + nsp, m->d_rec_iglob, nrp * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return owner;
+}
+
+void upload_assembly_csr(SemMesh* m, const std::vector<int>& off, const std::vector<int>& idx) {
+  dev_free(m->d_asm_off);
+  dev_free(m->d_asm_idx);
+  m->n_contrib = idx.size();
+  m->d_asm_off = dev_upload(off.data(), off.size(), m->stream);
+  m->d_asm_idx = dev_upload(idx.data(), idx.size(), m->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -176,33 +248,15 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
     m->d_adj_step = dev_alloc<realw>((size_t)m->nrec * 3);
   }
 
-  // Gather-assembly CSR: for every global point, its contributions in a fixed
-  // order -- element-local points (element order), Stacey face points, source
-  // points, receiver points.  Summing in this order makes the assembled force
-  // independent of scheduling.
+  // Gather-assembly CSR, element terms in element order until the phase lists
+  // (prepare_fields_elastic_device_) give the element loop's visiting order.
   {
-    const size_t nfp = (size_t)m->nfaces * 25, nsp = (size_t)m->nsources * 125, nrp = (size_t)m->nrec * 125;
-    std::vector<int> owner(nloc + nfp + nsp + nrp);
-    for (size_t L = 0; L < nloc; ++L) owner[L] = ib[L];
-    if (nfp) {
-      std::vector<int> fi(nfp);
-      HIP_CHECK(hipMemcpyAsync(fi.data(), m->d_face_iglob, nfp * sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      for (size_t q = 0; q < nfp; ++q) owner[nloc + q] = fi[q];
-    }
-    for (int q = 0; q < m->nsources; ++q)
-      for (int L = 0; L < 125; ++L) owner[nloc + nfp + (size_t)q * 125 + L] = ib[(size_t)(h_ispec_selected_source[q] - 1) * 125 + L];
-    for (int q = 0; q < m->nrec; ++q)
-      for (int L = 0; L < 125; ++L) owner[nloc + nfp + nsp + (size_t)q * 125 + L] = ib[(size_t)(h_ispec_selected_rec[q] - 1) * 125 + L];
-    m->n_contrib = owner.size();
-    if (m->n_contrib >= (size_t)INT32_MAX) SEM_FAIL("too many contributions for 32-bit CSR indices");
-    std::vector<int> off((size_t)m->nglob + 1, 0), idx(m->n_contrib);
-    for (size_t q = 0; q < owner.size(); ++q) off[(size_t)owner[q] + 1]++;
-    for (int g = 0; g < m->nglob; ++g) off[(size_t)g + 1] += off[g];
-    std::vector<int> fill(off.begin(), off.end() - 1);
-    for (size_t q = 0; q < owner.size(); ++q) idx[fill[owner[q]]++] = (int)q;
-    m->d_asm_off = dev_upload(off.data(), off.size(), s);
-    m->d_asm_idx = dev_upload(idx.data(), idx.size(), s);
+    const size_t nfp = (size_t)m->nfaces * 25;
+    std::vector<int> owner = contribution_owners(m);
+    std::vector<int> visit(m->nspec), off, idx;
+    for (int e = 0; e < m->nspec; ++e) visit[e] = e;
+    build_assembly_csr(m->nglob, m->nspec, owner, visit, off, idx);
+    upload_assembly_csr(m, off, idx);
     m->d_floc = dev_alloc<realw>(nloc * 3);
     if (nfp) m->d_face_scratch = dev_alloc<realw>(nfp * 3);
     HIP_CHECK(hipStreamSynchronize(s));
@@ -270,6 +324,19 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
     std::vector<int> ph((size_t)m->num_phase_ispec * 2);
     for (size_t q = 0; q < ph.size(); ++q) ph[q] = phase_ispec_inner_elastic[q] - 1;
     m->d_phase_ispec = dev_upload(ph.data(), ph.size(), s);
+    // element terms of every point in the order of the phase loop (outer, then
+    // inner elements); lists that do not cover every element once keep element order
+    std::vector<int> visit;
+    if (phase_visit_order(m->nspec, m->num_phase_ispec, phase_ispec_inner_elastic, m->nspec_outer, m->nspec_inner,
+                          visit)) {
+      bool identity = true;
+      for (int e = 0; e < m->nspec && identity; ++e) identity = visit[e] == e;
+      if (!identity) {
+        std::vector<int> off, idx;
+        build_assembly_csr(m->nglob, m->nspec, contribution_owners(m), visit, off, idx);
+        upload_assembly_csr(m, off, idx);
+      }
+    }
   }
   m->compute_and_store_strain = *COMPUTE_AND_STORE_STRAIN != 0;
   m->d_displ = dev_alloc<realw>(n3);
@@ -425,6 +492,59 @@ void fwat_set_static_precision_(long* Mesh_pointer, int* precision) {
     m->d_jinv10_base = dev_upload(base.data(), base.size(), s);
   }
   HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Host-only: the gather-assembly CSR that prepare_constants_device_ and
+// prepare_fields_elastic_device_ build for these arrays (no device work), so
+// its per-point order can be checked without a GPU.  Arguments as there;
+// outputs off (NGLOB_AB+1) and idx (contributions: 125 NSPEC_AB element-local
+// points, then 25 per Stacey face, 125 per source, 125 per receiver; 0-based).
+// Phase lists that do not cover every element once give element order.
+void fwat_assembly_csr_(int* NSPEC_AB, int* NGLOB_AB, int* h_ibool, int* num_abs_boundary_faces,
+                        int* h_abs_boundary_ispec, int* h_abs_boundary_ijk, int* NSOURCES, int* h_ispec_selected_source,
+                        int* nrec, int* h_ispec_selected_rec, int* num_phase_ispec_elastic,
+                        int* phase_ispec_inner_elastic, int* nspec_outer_elastic, int* nspec_inner_elastic, int* off,
+                        int* idx) {
+  const int nspec = *NSPEC_AB, nglob = *NGLOB_AB;
+  const size_t nloc = (size_t)nspec * 125;
+  std::vector<int> owner;
+  owner.reserve(nloc + (size_t)*num_abs_boundary_faces * 25 + (size_t)(*NSOURCES + *nrec) * 125);
+  for (size_t L = 0; L < nloc; ++L) {
+    const int g = h_ibool[L] - 1;
+    if (g < 0 || g >= nglob) SEM_FAIL("ibool(%zu) = %d outside 1..NGLOB_AB=%d", L, g + 1, nglob);
+    owner.push_back(g);
+  }
+  const auto elem = [nspec](int e1, const char* what, int q) {
+    if (e1 < 1 || e1 > nspec) SEM_FAIL("%s(%d) = %d out of range", what, q + 1, e1);
+    return e1 - 1;
+  };
+  for (int f = 0; f < *num_abs_boundary_faces; ++f) {
+    const int e = elem(h_abs_boundary_ispec[f], "abs_boundary_ispec", f);
+    for (int q = 0; q < 25; ++q) {
+      const int* ijk = &h_abs_boundary_ijk[((size_t)f * 25 + q) * 3];
+      if (ijk[0] < 1 || ijk[0] > 5 || ijk[1] < 1 || ijk[1] > 5 || ijk[2] < 1 || ijk[2] > 5)
+        SEM_FAIL("abs_boundary_ijk out of range");
+      owner.push_back(owner[(size_t)e * 125 + (ijk[0] - 1) + 5 * (ijk[1] - 1) + 25 * (ijk[2] - 1)]);
+    }
+  }
+  for (int q = 0; q < *NSOURCES; ++q) {
+    const int e = elem(h_ispec_selected_source[q], "ispec_selected_source", q);
+    for (int L = 0; L < 125; ++L) owner.push_back(owner[(size_t)e * 125 + L]);
+  }
+  for (int q = 0; q < *nrec; ++q) {
+    const int e = elem(h_ispec_selected_rec[q], "ispec_selected_rec", q);
+    for (int L = 0; L < 125; ++L) owner.push_back(owner[(size_t)e * 125 + L]);
+  }
+  std::vector<int> visit;
+  if (!phase_visit_order(nspec, *num_phase_ispec_elastic, phase_ispec_inner_elastic, *nspec_outer_elastic,
+                         *nspec_inner_elastic, visit)) {
+    visit.resize(nspec);
+    for (int e = 0; e < nspec; ++e) visit[e] = e;
+  }
+  std::vector<int> o, i;
+  build_assembly_csr(nglob, nspec, owner, visit, o, i);
+  std::memcpy(off, o.data(), o.size() * sizeof(int));
+  std::memcpy(idx, i.data(), i.size() * sizeof(int));
 }
 
 // transfers of the forward / adjoint field (Fortran displ(3,NGLOB_AB) layout)

@@ -27,6 +27,25 @@ from .gll import GLL
 _CORNER = np.array([[(n >> 0) & 1, (n >> 1) & 1, (n >> 2) & 1] for n in range(8)])
 
 
+def element_colours(ibool0, nglob):
+    """Mesh colouring (the role of setup_color_perm, create_regions_mesh_fwat.f90:283,
+    whose algorithm is SPECFEM code absent from the tree): greedy first fit in element
+    order -- element e takes the lowest colour not used by an earlier element that
+    shares a point with it, so no two elements of a colour share a point.
+    ibool0: [nspec, 125] 0-based.  Returns int32 colours [nspec]."""
+    ib = np.asarray(ibool0).reshape(-1, 125)
+    used = np.zeros(nglob, np.int64)  # bit c set: a colour-c element holds the point
+    col = np.zeros(len(ib), np.int32)
+    for e, pts in enumerate(ib):
+        taken = int(np.bitwise_or.reduce(used[pts]))
+        c = (~taken & (taken + 1)).bit_length() - 1  # lowest clear bit
+        if c > 62:
+            raise ValueError("more than 63 colours")
+        col[e] = c
+        used[pts] |= np.int64(1) << np.int64(c)
+    return col
+
+
 def _shape(xi, eta, gam):
     s = 2 * _CORNER - 1  # (8,3) signs
     fa = 1 + s[:, 0][None, :] * xi[:, None]
@@ -194,6 +213,17 @@ class BoxMesh(SemMesh):
         self.num_phase_ispec = self.nspec
         self.phase_ispec_inner = np.zeros((2, self.nspec), np.int32)
         self.phase_ispec_inner[1] = np.arange(1, self.nspec + 1, dtype=np.int32)
+
+    def colour_sort_phases(self):
+        """Order each phase list colour by colour (element_colours; elements of a
+        colour in their list order), as SPECFEM's mesh colouring reorders the
+        elements: the element loop, and so every point's assembly order, then
+        runs colour by colour.  Returns the colours."""
+        col = element_colours(self.ibool0, self.nglob)
+        for iphase, n in ((0, self.nspec_outer), (1, self.nspec_inner)):
+            lst = self.phase_ispec_inner[iphase, :n].copy()
+            self.phase_ispec_inner[iphase, :n] = lst[np.argsort(col[lst - 1], kind="stable")]
+        return col
 
     def _jac(self, es):
         """J[e, p, c, d] = dx_c/dref_d at the 125 GLL points of elements es."""

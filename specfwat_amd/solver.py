@@ -186,6 +186,13 @@ class ElasticRun:
         _lib.call("transfer_fields_el_from_device_", n3, u, v, a, self.h)
         return u, v, a
 
+    def set_fields(self, u, v, a):
+        """Initial state of a forward run (displ, veloc, accel [nglob, 3]), e.g. the state
+        of the same event at an earlier step: the run then continues from it."""
+        n3 = 3 * self.mesh.nglob
+        f = lambda x: np.ascontiguousarray(x, F32).reshape(self.mesh.nglob, 3)
+        _lib.call("transfer_fields_el_to_device_", n3, f(u), f(v), f(a), self.h)
+
     def b_fields(self):
         n3 = 3 * self.mesh.nglob
         u, v, a = (np.zeros((self.mesh.nglob, 3), F32) for _ in range(3))

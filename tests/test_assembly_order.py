@@ -1,0 +1,125 @@
+"""Colouring and assembly order, bit-exact against the oracle (integer work, no GPU).
+
+The product assembles the internal forces by a per-point gather whose rows list
+each point's element terms in the order of SPECFEM's CPU element loop over
+phase_ispec_inner_elastic (outer elements, then inner: the loop shape of
+smooth.f90:243-247, phase arrays read at read_mesh_databases.f90:1129-1143).
+The library builds that CSR on the host (fwat_assembly_csr_, the builder the
+preparation entry points use); the oracle's sequential element loop, run in the
+same visiting order, fixes the order each point receives its terms.  With the
+phase lists sorted colour by colour (SPECFEM's mesh colouring,
+setup_color_perm, create_regions_mesh_fwat.f90:283) the order is the oracle's
+colour-by-colour assembly.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from specfwat_amd import _lib
+from specfwat_amd.mesh import BoxMesh, element_colours
+
+
+def oracle_for(mesh, perturb, elem_order=None):
+    n = mesh.nspec * 125
+    return orc.Oracle(mesh.nex, mesh.ney, mesh.nez, mesh.lx, mesh.ly, mesh.lz, np.full(n, 2700, np.float32),
+                      np.full(n, 6000, np.float32), np.full(n, 3500, np.float32), dt=0.01, nstep=2,
+                      perturb=perturb, elem_order=elem_order, precision=64)
+
+
+def product_csr(mesh, src_ispec=(), rec_ispec=()):
+    nloc = mesh.nspec * 125
+    nfp = mesh.num_abs_boundary_faces * 25
+    ncontrib = nloc + nfp + 125 * (len(src_ispec) + len(rec_ispec))
+    off = np.zeros(mesh.nglob + 1, np.int32)
+    idx = np.zeros(ncontrib, np.int32)
+    si = np.asarray(src_ispec, np.int32) if len(src_ispec) else np.ones(1, np.int32)
+    ri = np.asarray(rec_ispec, np.int32) if len(rec_ispec) else np.ones(1, np.int32)
+    _lib.call("fwat_assembly_csr_", mesh.nspec, mesh.nglob, np.ascontiguousarray(mesh.ibool, np.int32),
+              mesh.num_abs_boundary_faces, mesh.abs_boundary_ispec, np.ascontiguousarray(mesh.abs_boundary_ijk),
+              len(src_ispec), si, len(rec_ispec), ri, mesh.num_phase_ispec,
+              np.ascontiguousarray(mesh.phase_ispec_inner), mesh.nspec_outer, mesh.nspec_inner, off, idx)
+    return off, idx
+
+
+def element_rows(off, idx, nloc):
+    """Per point, its element terms only (the CSR rows with the face, source and
+    receiver terms, which follow them, removed)."""
+    keep = idx < nloc
+    cnt = np.add.reduceat(keep.astype(np.int64), off[:-1]) if len(idx) else np.zeros(len(off) - 1)
+    cnt[off[:-1] == off[1:]] = 0
+    eoff = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    return eoff, idx[keep]
+
+
+@pytest.mark.parametrize("shape,perturb", [((4, 3, 3), 0.0), ((5, 4, 3), 0.15), ((3, 3, 2), 0.0)])
+def test_product_colours_equal_oracle_colours(shape, perturb):
+    mesh = BoxMesh(*shape, shape[0] * 1000.0, shape[1] * 1000.0, shape[2] * 1000.0, perturb=perturb)
+    o = oracle_for(mesh, perturb)
+    om = o.mesh()
+    assert np.array_equal(om["ibool"], mesh.ibool0.ravel())
+    col = element_colours(mesh.ibool0, mesh.nglob)
+    assert np.array_equal(col, om["color"])
+    # a valid colouring: no point held by two elements of one colour
+    for c in range(col.max() + 1):
+        pts = mesh.ibool0[col == c].ravel()
+        assert len(np.unique(pts)) == len(pts)
+
+
+@pytest.mark.parametrize("perturb", [0.0, 0.15])
+def test_phase_order_csr_equals_oracle_element_loop(perturb):
+    mesh = BoxMesh(5, 4, 3, 5000.0, 4000.0, 3000.0, perturb=perturb)
+    nloc = mesh.nspec * 125
+    # the box's own phase lists (every element inner, ascending) ...
+    cases = [None]
+    # ... and a split into outer and inner elements in a scrambled order
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(mesh.nspec) + 1
+    cases.append((7, perm))
+    for case in cases:
+        if case is not None:
+            nout, perm = case
+            mesh.nspec_outer, mesh.nspec_inner = nout, mesh.nspec - nout
+            mesh.phase_ispec_inner = np.zeros((2, mesh.nspec), np.int32)
+            mesh.phase_ispec_inner[0, :nout] = perm[:nout]
+            mesh.phase_ispec_inner[1, :mesh.nspec - nout] = perm[nout:]
+        visit = np.concatenate([mesh.phase_ispec_inner[0, :mesh.nspec_outer],
+                                mesh.phase_ispec_inner[1, :mesh.nspec_inner]]) - 1
+        off, idx = product_csr(mesh, src_ispec=[3], rec_ispec=[1, 8])
+        eoff, eidx = element_rows(off, idx, nloc)
+        ooff, oidx = oracle_for(mesh, perturb, elem_order=visit).assembly_order()
+        assert np.array_equal(eoff, ooff) and np.array_equal(eidx, oidx)
+        # face, source and receiver terms follow each point's element terms, in that order
+        for g in np.unique(mesh.ibool0[2]):
+            row = idx[off[g]:off[g + 1]]
+            assert np.all(np.diff((row >= nloc).astype(int)) >= 0)
+            assert np.all(np.diff(row[row >= nloc]) > 0)
+
+
+@pytest.mark.parametrize("perturb", [0.0, 0.15])
+def test_colour_sorted_phases_give_oracle_colour_order(perturb):
+    mesh = BoxMesh(5, 4, 3, 5000.0, 4000.0, 3000.0, perturb=perturb)
+    col = mesh.colour_sort_phases()
+    assert np.all(np.diff(col[mesh.phase_ispec_inner[1] - 1]) >= 0)
+    off, idx = product_csr(mesh)
+    eoff, eidx = element_rows(off, idx, mesh.nspec * 125)
+    ooff, oidx = oracle_for(mesh, perturb).assembly_order()  # colour-by-colour assembly
+    assert np.array_equal(eoff, ooff) and np.array_equal(eidx, oidx)
+
+
+def test_oracle_ordered_assembly_matches_colour_assembly():
+    """The oracle's two assembly schedules differ only in each point's summation
+    order: FP64 seismograms agree to rounding."""
+    mesh = BoxMesh(4, 4, 3, 4000.0, 4000.0, 3000.0, perturb=0.1)
+    n = mesh.nspec * 125
+    rng = np.random.default_rng(9)
+    rho = (2700 * (1 + 0.03 * rng.standard_normal(n))).astype(np.float32)
+    kw = dict(dt=0.01, nstep=120, src_xyz=[[1700.0, 2100.0, -1400.0]], src_mt=[[1e15, -2e14, 3e14, 5e14, -1e14, 2e14]],
+              src_f0=[2.0], rec_xyz=[[1000.0, 3000.0, 0.0], [3000.0, 1000.0, -500.0]], perturb=0.1, precision=64)
+    args = (mesh.nex, mesh.ney, mesh.nez, mesh.lx, mesh.ly, mesh.lz, rho, np.full(n, 6000, np.float32),
+            np.full(n, 3500, np.float32))
+    a = orc.Oracle(*args, **kw).forward(save=False)
+    b = orc.Oracle(*args, elem_order=rng.permutation(mesh.nspec), **kw).forward(save=False)
+    assert np.abs(a).max() > 0
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-12
+    with pytest.raises(ValueError):
+        orc.Oracle(*args, elem_order=np.zeros(mesh.nspec, np.int32), **kw)
