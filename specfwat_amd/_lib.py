@@ -6,6 +6,7 @@ pointers according to a per-function signature string:
 
     L  long* Mesh_pointer (a HandleBox)      i / f / d / l  int / float / double / long scalar
     I  int32 array   F  float32 array   D  float64 array   (None -> NULL)
+    P  device address (a Python int from the caller's device allocation; None -> NULL)
 
 There is no fallback: if the HIP library is missing the import-time load
 raises, so a GPU run can never silently take another path.
@@ -151,6 +152,8 @@ def call(name, *args):
             if not isinstance(a, HandleBox):
                 raise TypeError(f"{name}: Mesh_pointer must be a HandleBox")
             cargs.append(C.cast(C.pointer(a.value), C.c_void_p))
+        elif code == "P":
+            cargs.append(None if a is None else C.c_void_p(int(a)))
         elif code in _SCALAR:
             v = _SCALAR[code](a)
             keep.append(v)

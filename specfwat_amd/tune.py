@@ -2,7 +2,8 @@
 
 Times short forward and adjoint runs on the real mesh for a set of
 candidate shapes (general kernel, folded-table kernel with one workgroup
-per element, persistent workgroups per CU) and returns the fastest; on
+per element, persistent workgroups per CU, its software-pipelined form) and
+returns the fastest; on
 lattice meshes it then compares the brick-fused element + assembly pass
 (brick shapes 2x2x2, 4x2x2, 4x4x2) against the best gather shapes.  The
 caller applies the result with ElasticRun.set_tuning.  The measurement is
@@ -12,7 +13,9 @@ import numpy as np
 
 from .solver import ElasticRun
 
-CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32)]
+# (regular_path, workgroups per CU): 0 general kernel, 1 folded-table kernel, 2 its
+# software-pipelined form (isotropic); 0 workgroups per CU = one workgroup per element
+CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32), (2, 6), (2, 8), (2, 12), (2, 16)]
 BRICKS = [(2, 2, 2), (4, 2, 2), (4, 4, 2)]
 SLABS = [6, 12, 24]  # slabs per mesh
 
@@ -38,10 +41,9 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
         a.iterate_time(timed=True)
         ams, an = a.force_time()
         table.append((reg, bpc, fms / max(fn, 1), ams / max(an, 1)))
-    best_f = min(table, key=lambda r: r[2])
-    best_a = min(table, key=lambda r: r[3])
-    # one regular_path switch for both passes: keep the better total
-    reg = 1 if (best_f[0] + best_a[0]) >= 1 else 0
+    # one regular_path switch for both passes: the value with the best forward + adjoint total
+    regs = sorted({r[0] for r in table})
+    reg = min(regs, key=lambda g: min(r[2] for r in table if r[0] == g) + min(r[3] for r in table if r[0] == g))
     bf = min((r for r in table if r[0] == reg), key=lambda r: r[2])[1]
     ba = min((r for r in table if r[0] == reg), key=lambda r: r[3])[1]
     best = {"regular_path": reg, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba}

@@ -244,8 +244,10 @@ def test_smoother_batch_matches_single(nfield):
 
 
 def test_element_pass_launch_shapes_agree():
-    """General kernel, folded-table kernel with one workgroup per element and
-    persistent workgroups give the same seismograms and kernels."""
+    """General kernel, folded-table kernel with one workgroup per element,
+    persistent workgroups and the software-pipelined form (one workgroup per
+    element, persistent workgroups with several elements each) give the same
+    seismograms and kernels."""
     c = make_case(seed=9, nstep=80)
     o = oracle_of(c, 64)
     obs = o.forward(save=True)
@@ -253,7 +255,8 @@ def test_element_pass_launch_shapes_agree():
     ko = o.adjoint(adj)
     m = c["mesh"]
     results = []
-    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8)):
+    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8), (2, 0), (2, 1), (2, 8)):
+        print(f"launch shape: regular_path {reg}, {bpc} workgroups per CU", flush=True)
         src = Sources(m, c["sxyz"], c["mt"], c["f0"])
         rec = Receivers(m, c["rxyz"])
         f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
@@ -274,6 +277,44 @@ def test_element_pass_launch_shapes_agree():
         results.append(syn)
     for s in results[1:]:
         assert rel(s, results[0]) < 1e-6
+
+
+def test_pipelined_pass_with_several_elements_per_workgroup():
+    """The software-pipelined element pass (regular_path 2) walks several
+    elements per workgroup when workgroups per CU x CUs < elements: 2,048
+    elements on 256 or 512 workgroups (8 or 4 each, the last ones ragged),
+    against the plain folded-table pass and the FP64 oracle."""
+    c = make_case(nex=16, ney=16, nez=8, seed=31, nstep=60, nrec=6)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)
+    ko = o.adjoint(adj)
+    m = c["mesh"]
+    out = []
+    for reg, bpc in ((1, 1), (2, 1), (2, 2), (2, 3)):
+        print(f"launch shape: regular_path {reg}, {bpc} workgroups per CU", flush=True)
+        src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+        rec = Receivers(m, c["rxyz"])
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
+        f.set_tuning(reg, bpc, bpc)
+        f.iterate_time()
+        syn = f.seismograms()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
+        a.set_tuning(reg, bpc, bpc)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj)
+        a.iterate_time()
+        k = a.kernels()
+        a.close()
+        assert rel(syn, obs) < SEIS_TOL
+        for n in ("rho_kl", "mu_kl", "kappa_kl"):
+            assert rel(k[n], ko[n]) < KERN_TOL, (reg, bpc, n)
+        out.append((syn, k))
+    for syn, k in out[1:]:
+        assert rel(syn, out[0][0]) < 1e-6
+        for n in k:
+            assert rel(k[n], out[0][1][n]) < 1e-6, n
 
 
 @pytest.mark.parametrize("aniso", [False, True])
