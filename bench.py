@@ -63,6 +63,8 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=None,
                    help="steps of the CPU baseline sample (default: 4 for c2/c4, the whole run for c1)")
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
+    p.add_argument("--graph", action="store_true",
+                   help="c1/c2: capture each run's step loop into one HIP graph (no per-kernel timing)")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--no-tune", action="store_true", help="skip the element-pass launch-shape autotuner")
     p.add_argument("--fk", type=int, default=0, metavar="R",
@@ -192,6 +194,9 @@ def main():
         f.set_tuning(**tuning)
         ad.set_tuning(**tuning)
         ad.set_kernel_block(KERNEL_BLOCK)
+        if a.graph:
+            f.set_graph(True)
+            ad.set_graph(True)
         if window and state_smooth is not None:
             f.set_fields(*state_smooth)
         if a.fk:
@@ -230,7 +235,7 @@ def main():
     _lib.call("fwat_synchronize_", f.h)
     _lib.call("fwat_synchronize_", ad.h)
     ts = time.time()
-    misfit, tf = event(f, ad, obs, True)
+    misfit, tf = event(f, ad, obs, not a.graph)
     _lib.call("fwat_synchronize_", ad.h)
     te = time.time()
     wall = te - ts
@@ -305,7 +310,8 @@ def main():
     kname = max(kinds, key=lambda k: kinds[k][1])
     kbytes, kms, kn = kinds[kname]
     avg_ms = kms / max(kn, 1)
-    achieved = kbytes / (avg_ms * 1e-3) / 1e9
+    # a graph-captured run carries no per-kernel times
+    achieved = kbytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     traffic = None
     tf_path = os.path.join(HERE, "profiles", "c2_fwd_adj_hbm_traffic.md")
     if os.path.exists(tf_path) and fused and a.config == "c2" and a.static_precision == 0 and nsl == 1:
@@ -336,13 +342,14 @@ def main():
                                        f"event steps {S0 + 1}..{S0 + K}: the forward state at step {S0} (after "
                                        "the first P arrival at the nearest receiver) from an untimed run; "
                                        "the reconstruction returns to it")},
-                   "path": (f"brick-fused {'x'.join(map(str, brick))}" if brick else
-                            (f"fused, {nsl} element slabs" if nsl > 1 else "fused"))
-                           if fused else "SPECFEM stage order (strain stores)",
+                   "path": ((f"brick-fused {'x'.join(map(str, brick))}" if brick else
+                             (f"fused, {nsl} element slabs" if nsl > 1 else "fused"))
+                            if fused else "SPECFEM stage order (strain stores)")
+                           + (", step loops captured as HIP graphs" if a.graph else ""),
                    "element_pass_tuning": tuning if fused else None,
                    **({"fk_injection": {"every_steps": a.fk}} if a.fk else {})},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
         "breakdown": {"wall_s": wall, "forward_s": tf - ts, "adjoint_s": te - tf,
                       "kernels": {k: {"avg_ms": v[1] / max(v[2], 1),

@@ -175,6 +175,12 @@ void fwat_set_slabs_(long *Mesh_pointer, int *elems_per_slab);
  * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
  * fp64).  The rho kernel (summed in the assembly pass) uses the same blocks. */
 void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
+/* HIP graph capture of the whole-run loop (fused gather path): on = 1 makes every
+ * fwat_iterate_time_ call capture its step loop from the simulation stream into one graph and
+ * launch it as a unit (the GPU then runs the steps' kernels back to back, independent of the
+ * host's launch rate); captured runs are not timed.  Replaces the per-step launches of
+ * iterate_time (prepare_fwd.f90:364); no counterpart in the reference. */
+void fwat_set_graph_(long *Mesh_pointer, int *on);
 /* Host-only (no device work): the gather-assembly CSR the preparation entry points build for
  * these arrays -- for every point its element terms in the order of SPECFEM's CPU element loop
  * over phase_ispec_inner_elastic (outer elements, then inner; the loop shape of
@@ -202,6 +208,11 @@ void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 void fwat_batch_create_(long *Mesh_pointer, long *Batch_pointer, int *nevents, int *nstep, realw *DT,
                         realw *h_sourcearrays, int *h_ispec_selected_source, double *h_stf);
 void fwat_batch_set_tuning_(long *Batch_pointer, int *blocks_per_cu);
+/* Stacey store of the batch's next forward run: 0 fp32 (12 bytes per face point, event and step),
+ * 1 fp16 with one scale per (step, event) record (6 bytes): the record's largest value maps to
+ * 16384, so each stored value keeps 11 significant bits; halves the reconstruction store that
+ * replaces absorb_field.bin (specfem_api.f90:359-361). */
+void fwat_batch_set_store_precision_(long *Batch_pointer, int *precision);
 /* forward runs with SAVE_FORWARD (per-event Stacey stores kept in HBM) */
 void fwat_batch_forward_(long *Batch_pointer, int *timed);
 /* h_adj(NDIM,nrec,NSTEP_max,nevents), forward time order; event e uses its first NSTEP_e rows */

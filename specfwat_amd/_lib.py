@@ -61,10 +61,12 @@ SIGNATURES = {
     "fwat_set_brick_": "Liiii",
     "fwat_brick_info_": "Lill",
     "fwat_set_kernel_block_": "Li",
+    "fwat_set_graph_": "Li",
     "fwat_assembly_csr_": "iiIiIIiIiIiIiiII",
     "fwat_set_slabs_": "Li",
     "fwat_batch_create_": "LLiIfFID",
     "fwat_batch_set_tuning_": "Li",
+    "fwat_batch_set_store_precision_": "Li",
     "fwat_batch_forward_": "Li",
     "fwat_batch_set_adjoint_sources_": "LF",
     "fwat_batch_adjoint_": "Li",

@@ -45,10 +45,17 @@ class EventBatch:
             stf[q, :self.nsteps[q]] = s.stf(int(self.nsteps[q]), self.dt)[:, 0]
         self.stf = stf
         self.h = _lib.HandleBox()
+        self.store_precision = 0
         _lib.call("fwat_batch_create_", self.run.h, self.h, e, self.nsteps, self.dt, sarr, isp, stf)
 
     def set_tuning(self, blocks_per_cu=8):
         _lib.call("fwat_batch_set_tuning_", self.h, int(blocks_per_cu))
+
+    def set_store_precision(self, precision):
+        """Stacey store of the next forward run: 0 fp32, 1 fp16 with one scale per
+        step and event (half the HBM; the reconstruction replays the rounded terms)."""
+        _lib.call("fwat_batch_set_store_precision_", self.h, int(precision))
+        self.store_precision = int(precision)
 
     def forward(self, timed=False):
         _lib.call("fwat_batch_forward_", self.h, int(timed))
@@ -134,14 +141,15 @@ def plan_batches(nsteps, events_per_batch=8):
     return groups
 
 
-def batch_device_bytes(mesh, nevents, nmax):
-    """HBM held by one EventBatch (fields, element-local forces, kernels,
-    Stacey rows, CSR), besides the mesh handle."""
+def batch_device_bytes(mesh, nevents, nmax, store_precision=0):
+    """HBM held by one EventBatch (fields, element-local forces, kernels and their
+    fp32 partial sums, Stacey rows, CSR), besides the mesh handle."""
     nloc, ng = mesh.nspec * 125, mesh.nglob
     nfp = mesh.num_abs_boundary_faces * 25
     e = nevents
-    return (6 * ng * e * 12 + 2 * nloc * e * 12 + (2 * nloc + ng) * e * 8
-            + (nmax + 1) * nfp * e * 12 + 4 * (ng + nloc + nfp) + 8 * e * nmax)
+    store = nmax * nfp * e * (6 if store_precision else 12) + (nmax * e * 4 + nfp * e * 12 if store_precision else 0)
+    return (6 * ng * e * 12 + 2 * nloc * e * 12 + (2 * nloc + ng) * e * 12
+            + store + nfp * e * 12 + 4 * (ng + nloc + nfp) + 8 * e * nmax)
 
 
 def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_cu=(4, 8, 16), nstep=12,

@@ -8,7 +8,8 @@
 // data layout and what is read once for all events:
 //   fields u, v, a              [nglob][E][3]      (events innermost)
 //   element-local forces        [nloc][E][3]
-//   Stacey store                [NSTEP_max][nfp][E][3]
+//   Stacey store                [NSTEP_max][nfp][E][3]  fp32, or fp16 with one scale per
+//                                                        (record, event) (fwat_batch_set_store_precision_)
 //   rho kernel                  [nglob][E] fp64;  mu, kappa kernels [E][nloc] fp64
 //   statics (ibool, kappa/mu), CSR, rmass_inv, face geometry: once per element
 //   or point for all E events.
@@ -282,7 +283,7 @@ __global__ void k_batch_asm(BatchAsmArgs A) {
   // added (two loads in flight per lane), same summation order
   for (; k + 1 < q1; k += 2) {
     const int c0 = A.idx[k], c1 = A.idx[k + 1];
-    if (c1 >= A.nloc) break;  // c0 < c1
+    if (c1 >= A.nloc) break;  // element terms lead each row: c0 is one too
     const realw* P0 = A.floc0 + ((size_t)c0 * E + ev) * 3;
     const realw* P1 = A.floc0 + ((size_t)c1 * E + ev) * 3;
     const realw a0 = P0[0], a1 = P0[1], a2 = P0[2], b0 = P1[0], b1 = P1[1], b2 = P1[2];
@@ -373,6 +374,69 @@ __global__ void k_batch_stacey(int nfp, int E, int it, int nmax, int adjoint, co
   o[2] = (rvp * vn * nz + rvs * (vz - vn * nz)) * jw;
 }
 
+// Compact Stacey store: a record (one step of one event, nfp*3 values) keeps
+// fp16 values v/s with s = max|v| / 16384 (the record's largest value maps to
+// 16384, well inside fp16's range, so the rounding error of every value is at
+// most 2^-11 of its own size above s/16384's subnormal floor).  The scale is
+// the record's max|v| held as float bits (non-negative floats order like their
+// bits, so an integer atomicMax finds it).
+constexpr float kStoreMaxMapped = 16384.0f;
+
+// max|v| of each event's traction in this step's face scratch [nfp][E][3],
+// reduced over the lanes of a wavefront that hold the same event (lanes q and
+// q' hold one event when q - q' is a multiple of E) and folded into
+// maxbits[ev] with one integer atomicMax per wavefront and event
+__global__ void k_batch_face_absmax(int nfp, int E, const realw* __restrict__ face, unsigned int* __restrict__ maxbits) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.0f;
+  if (q < (size_t)nfp * E) {
+    const realw* o = face + 3 * q;
+    m = fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
+  }
+  for (int d = E; d < 64; d <<= 1) m = fmaxf(m, __shfl_xor(m, d));
+  // lane ev < E of every wavefront that holds any face point reports event ev
+  const int lane = threadIdx.x & 63;
+  if (lane < E && q - lane < (size_t)nfp * E) atomicMax(maxbits + lane, __float_as_uint(m));
+}
+
+__device__ __forceinline__ float store_scale_of(unsigned int maxbits) { return __uint_as_float(maxbits) / kStoreMaxMapped; }
+
+// face scratch [nfp][E][3] -> fp16 record with the scales of this step
+__global__ void k_batch_store_pack(int nfp, int E, const realw* __restrict__ face, const unsigned int* __restrict__ maxbits,
+                                   uint16_t* __restrict__ rec) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)nfp * E) return;
+  const float mx = __uint_as_float(maxbits[q % E]);
+  const float inv = mx > 0.0f ? kStoreMaxMapped / mx : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const _Float16 h = (_Float16)(face[3 * q + c] * inv);
+    uint16_t b;
+    __builtin_memcpy(&b, &h, 2);
+    rec[3 * q + c] = b;
+  }
+}
+
+// fp16 record -> face scratch [nfp][E][3] for the replay
+__global__ void k_batch_store_unpack(int nfp, int E, const uint16_t* __restrict__ rec,
+                                     const unsigned int* __restrict__ maxbits, realw* __restrict__ face) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)nfp * E) return;
+  const float sc = store_scale_of(maxbits[q % E]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    _Float16 h;
+    const uint16_t b = rec[3 * q + c];
+    __builtin_memcpy(&h, &b, 2);
+    face[3 * q + c] = (float)h * sc;
+  }
+}
+
+__global__ void k_fill_u32(unsigned int* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 // seismograms of the forward field, one block per (receiver, event), into
 // seis [E][NSTEP_max][nrec][3] row it-1
 __global__ __launch_bounds__(128) void k_batch_seismo(int nrec, int E, int it, int nmax,
@@ -431,7 +495,11 @@ struct SemBatch {
   realw *bu = nullptr, *bv = nullptr, *ba = nullptr;  // reconstructed field
   realw *floc0 = nullptr, *floc1 = nullptr;           // [nloc][E][3]
   realw* face_scratch = nullptr;                      // [nfp][E][3]
-  realw* store = nullptr;                             // [nmax][nfp][E][3]
+  realw* store = nullptr;                             // [nmax][nfp][E][3] fp32 store
+  int store_precision = 0;                            // 0 fp32, 1 fp16 with a scale per record and event
+  uint16_t* store16 = nullptr;                        // [nmax][nfp][E][3] fp16 store
+  unsigned int* store_max = nullptr;                  // [nmax][E] max|v| per record and event (float bits)
+  realw* face_replay = nullptr;                       // [nfp][E][3] unpacked replay row (fp16 store)
   realw* seis = nullptr;                              // [E][nmax][nrec][3]
   realw* adj = nullptr;                               // [E][nmax][nrec][3]
   double *mu_kl = nullptr, *kappa_kl = nullptr;       // [E][nloc]
@@ -509,9 +577,9 @@ void asm_pass(SemBatch* B, bool two, int it) {
   const size_t row = B->nfp * B->E * 3;
   if (two) {
     A.face0 = B->face_scratch;
-    A.face1 = B->store + (size_t)(B->nmax - it) * row;
+    A.face1 = B->store_precision ? B->face_replay : B->store + (size_t)(B->nmax - it) * row;
   } else {
-    A.face0 = B->store + (size_t)(it - 1) * row;
+    A.face0 = B->store_precision ? B->face_scratch : B->store + (size_t)(it - 1) * row;
   }
   const dim3 gr(nblk((size_t)m->nglob * B->E)), bl(TPB);
   hipStream_t s = m->stream;
@@ -529,30 +597,80 @@ void asm_pass(SemBatch* B, bool two, int it) {
   HIP_CHECK(hipGetLastError());
 }
 
+// Stacey terms of this step.  Forward: the traction of field 0 goes to the
+// store row it-1 (fp32 store) or to the scratch, then into the fp16 record
+// it-1 with its scales; adjoint: the adjoint field's traction to the scratch
+// and, with the fp16 store, record NSTEP_max - it unpacked for the replay.
 void stacey_pass(SemBatch* B, int it, bool adjoint) {
   if (!B->nfp) return;
   SemMesh* m = B->m;
-  realw* out = adjoint ? B->face_scratch : B->store + (size_t)(it - 1) * B->nfp * B->E * 3;
-  hipLaunchKernelGGL(k_batch_stacey, dim3(nblk(B->nfp * B->E)), dim3(TPB), 0, m->stream, (int)B->nfp, B->E, it,
-                     B->nmax, (int)adjoint, B->d_nstep, m->d_face_iglob, m->d_face_coef, B->v, out);
+  hipStream_t s = m->stream;
+  const int nfp = (int)B->nfp, E = B->E;
+  const size_t row = B->nfp * E * 3;
+  const bool compact = B->store_precision != 0;
+  realw* out = (adjoint || compact) ? B->face_scratch : B->store + (size_t)(it - 1) * row;
+  hipLaunchKernelGGL(k_batch_stacey, dim3(nblk(B->nfp * E)), dim3(TPB), 0, s, nfp, E, it, B->nmax, (int)adjoint,
+                     B->d_nstep, m->d_face_iglob, m->d_face_coef, B->v, out);
+  if (compact && !adjoint) {
+    unsigned int* mx = B->store_max + (size_t)(it - 1) * E;
+    hipLaunchKernelGGL(k_batch_face_absmax, dim3(nblk(B->nfp * E)), dim3(TPB), 0, s, nfp, E, B->face_scratch, mx);
+    hipLaunchKernelGGL(k_batch_store_pack, dim3(nblk(B->nfp * E)), dim3(TPB), 0, s, nfp, E, B->face_scratch, mx,
+                       B->store16 + (size_t)(it - 1) * row);
+  } else if (compact && adjoint) {
+    const size_t r = (size_t)(B->nmax - it);
+    hipLaunchKernelGGL(k_batch_store_unpack, dim3(nblk(B->nfp * E)), dim3(TPB), 0, s, nfp, E, B->store16 + r * row,
+                       B->store_max + r * E, B->face_replay);
+  }
   HIP_CHECK(hipGetLastError());
 }
 
-// element / assembly pass times summed over a timed run
+// the Stacey store in the precision asked for, allocated before a forward run
+void ensure_store(SemBatch* B) {
+  if (!B->nfp) return;
+  const size_t n = (size_t)B->nmax * B->nfp * B->E * 3;
+  hipStream_t s = B->m->stream;
+  if (B->store_precision == 0) {
+    dev_free(B->store16);
+    dev_free(B->store_max);
+    dev_free(B->face_replay);
+    if (!B->store) B->store = dev_alloc<realw>(n);
+  } else {
+    dev_free(B->store);
+    if (!B->store16) B->store16 = dev_alloc<uint16_t>(n);
+    if (!B->store_max) B->store_max = dev_alloc<unsigned int>((size_t)B->nmax * B->E);
+    if (!B->face_replay) B->face_replay = dev_alloc<realw>(B->nfp * B->E * 3);
+    const size_t ns = (size_t)B->nmax * B->E;
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ns)), dim3(TPB), 0, s, B->store_max, ns);
+  }
+}
+
+// element / assembly pass times summed over the sampled steps of a timed run
+// (every stride-th step, at most about 256 of them): four events per sampled
+// step, whatever NSTEP is
 struct PassTimer {
   std::vector<hipEvent_t> ev;
-  explicit PassTimer(int steps) : ev(4 * (size_t)steps) {
+  int stride = 1, samples = 0;
+  explicit PassTimer(int steps) {
+    stride = std::max(1, steps / 256);
+    ev.resize(4 * (size_t)((steps + stride - 1) / stride));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
   }
-  void mark(int it, int k, hipStream_t s) { HIP_CHECK(hipEventRecord(ev[4 * (size_t)(it - 1) + k], s)); }
-  void sum(int steps, double& elem, double& asmb) {
+  bool sampled(int it) const { return (it - 1) % stride == 0; }
+  void mark(int it, int k, hipStream_t s) {
+    if (!sampled(it)) return;
+    HIP_CHECK(hipEventRecord(ev[4 * (size_t)((it - 1) / stride) + k], s));
+  }
+  // sums over the sampled steps and their count
+  int sum(int steps, double& elem, double& asmb) {
     elem = asmb = 0;
-    for (int it = 0; it < steps; ++it) {
+    const int n = (steps + stride - 1) / stride;
+    for (int k = 0; k < n; ++k) {
       float a = 0, b = 0;
-      HIP_CHECK(hipEventElapsedTime(&a, ev[4 * it], ev[4 * it + 1]));
-      HIP_CHECK(hipEventElapsedTime(&b, ev[4 * it + 2], ev[4 * it + 3]));
+      HIP_CHECK(hipEventElapsedTime(&a, ev[4 * (size_t)k], ev[4 * (size_t)k + 1]));
+      HIP_CHECK(hipEventElapsedTime(&b, ev[4 * (size_t)k + 2], ev[4 * (size_t)k + 3]));
       elem += a; asmb += b;
     }
+    return n;
   }
   ~PassTimer() {
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -586,7 +704,7 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
   B->nfp = m->absorbing ? (size_t)m->nfaces * 25 : 0;
   hipStream_t s = m->stream;
   const size_t nloc = (size_t)m->nspec * 125, ng = (size_t)m->nglob, nfp = B->nfp;
-  // CSR with the batch's E sources in the source segment
+  // CSR with the batch's E sources in the source segment (E*125 source points)
   std::vector<int> ib(nloc), fi(nfp), ri((size_t)m->nrec * 125);
   HIP_CHECK(hipMemcpyAsync(ib.data(), m->d_ibool, nloc * sizeof(int), hipMemcpyDeviceToHost, s));
   if (nfp) HIP_CHECK(hipMemcpyAsync(fi.data(), m->d_face_iglob, nfp * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -602,12 +720,13 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
     for (int L = 0; L < 125; ++L) owner[nloc + nfp + (size_t)e * 125 + L] = ib[(size_t)el * 125 + L];
   }
   for (size_t q = 0; q < ri.size(); ++q) owner[nloc + nfp + nsp + q] = ri[q];
-  if (owner.size() >= (size_t)INT32_MAX) SEM_FAIL("too many contributions for 32-bit CSR indices");
-  std::vector<int> off(ng + 1, 0), idx(owner.size());
-  for (size_t q = 0; q < owner.size(); ++q) off[(size_t)owner[q] + 1]++;
-  for (size_t g = 0; g < ng; ++g) off[g + 1] += off[g];
-  std::vector<int> fill(off.begin(), off.end() - 1);
-  for (size_t q = 0; q < owner.size(); ++q) idx[fill[owner[q]]++] = (int)q;
+  // every point's element terms in the mesh's phase-loop order, as the single-event path
+  std::vector<int> visit = m->h_visit, off, idx;
+  if (visit.empty()) {
+    visit.resize(m->nspec);
+    for (int e = 0; e < m->nspec; ++e) visit[e] = e;
+  }
+  sem_build_assembly_csr(m->nglob, m->nspec, owner, visit, off, idx);
   B->d_off = dev_upload(off.data(), off.size(), s);
   B->d_idx = dev_upload(idx.data(), idx.size(), s);
   B->d_sarr = dev_upload(h_sourcearrays, (size_t)E * 375, s);
@@ -616,10 +735,7 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
   const size_t nf3 = ng * E * 3;
   B->u = dev_alloc<realw>(nf3); B->v = dev_alloc<realw>(nf3); B->a = dev_alloc<realw>(nf3);
   B->floc0 = dev_alloc<realw>(nloc * E * 3);
-  if (nfp) {
-    B->face_scratch = dev_alloc<realw>(nfp * E * 3);
-    B->store = dev_alloc<realw>((size_t)B->nmax * nfp * E * 3);
-  }
+  if (nfp) B->face_scratch = dev_alloc<realw>(nfp * E * 3);  // the store: ensure_store, at the forward run
   if (m->nrec) {
     const size_t ns = (size_t)E * B->nmax * m->nrec * 3;
     B->seis = dev_alloc<realw>(ns);
@@ -636,12 +752,23 @@ void fwat_batch_set_tuning_(long* Batch_pointer, int* blocks_per_cu) {
   B->blocks_per_cu = *blocks_per_cu;
 }
 
+// Precision of the batch's Stacey store, for the next forward run: 0 fp32
+// (12 bytes per face point, event and step), 1 fp16 with one scale per record
+// and event (6 bytes).  The replayed traction of the reconstruction carries the
+// fp16 rounding (at most 2^-11 of a value above the record's max / 2^28).
+void fwat_batch_set_store_precision_(long* Batch_pointer, int* precision) {
+  SemBatch* B = batch_of(Batch_pointer);
+  if (*precision != 0 && *precision != 1) SEM_FAIL("store precision must be 0 (fp32) or 1 (fp16 + scale)");
+  B->store_precision = *precision;
+}
+
 // Forward runs of all events from rest (SAVE_FORWARD: Stacey rows kept in HBM).
 void fwat_batch_forward_(long* Batch_pointer, int* timed) {
   SemBatch* B = batch_of(Batch_pointer);
   SemMesh* m = B->m;
   hipStream_t s = m->stream;
   const int E = B->E, N = B->nmax;
+  ensure_store(B);
   // every call is a run from rest
   const size_t nf3 = (size_t)m->nglob * E * 3;
   for (realw* p : {B->u, B->v, B->a}) hipLaunchKernelGGL(k_fill, dim3(nblk(nf3)), dim3(TPB), 0, s, p, nf3);
@@ -670,8 +797,7 @@ void fwat_batch_forward_(long* Batch_pointer, int* timed) {
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
   if (tm) {
-    tm->sum(N, B->elem_ms, B->asm_ms);
-    B->elem_n = B->asm_n = N;
+    B->elem_n = B->asm_n = tm->sum(N, B->elem_ms, B->asm_ms);
     delete tm;
   }
 }
@@ -694,6 +820,7 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   const int E = B->E, N = B->nmax;
   const size_t nloc = (size_t)m->nspec * 125, ng = (size_t)m->nglob, nf3 = ng * E * 3;
   if (m->nrec && !B->adj) SEM_FAIL("fwat_batch_set_adjoint_sources_ not called");
+  if (B->nfp && !(B->store_precision ? (void*)B->store16 : (void*)B->store)) SEM_FAIL("fwat_batch_forward_ has not run");
   if (!B->bu) {
     B->bu = dev_alloc<realw>(nf3); B->bv = dev_alloc<realw>(nf3); B->ba = dev_alloc<realw>(nf3);
     B->floc1 = dev_alloc<realw>(nloc * E * 3);
@@ -742,8 +869,7 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
   if (tm) {
-    tm->sum(N, B->elem_ms, B->asm_ms);
-    B->elem_n = B->asm_n = N;
+    B->elem_n = B->asm_n = tm->sum(N, B->elem_ms, B->asm_ms);
     delete tm;
   }
 }
@@ -801,6 +927,7 @@ void fwat_batch_destroy_(long* Batch_pointer) {
   dev_free(B->d_nstep); dev_free(B->d_off); dev_free(B->d_idx); dev_free(B->d_sarr); dev_free(B->d_stf);
   dev_free(B->u); dev_free(B->v); dev_free(B->a); dev_free(B->bu); dev_free(B->bv); dev_free(B->ba);
   dev_free(B->floc0); dev_free(B->floc1); dev_free(B->face_scratch); dev_free(B->store);
+  dev_free(B->store16); dev_free(B->store_max); dev_free(B->face_replay);
   dev_free(B->seis); dev_free(B->adj);
   dev_free(B->mu_kl); dev_free(B->kappa_kl); dev_free(B->rho_kl);
   dev_free(B->mu_part); dev_free(B->kappa_part); dev_free(B->rho_part);

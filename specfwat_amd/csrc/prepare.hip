@@ -59,29 +59,6 @@ bool phase_visit_order(int nspec, int num_phase_ispec, const int* phase_ispec_in
   return true;
 }
 
-// Gather-assembly CSR.  owner[q] is the global point of contribution q: the
-// element-local points (125 per element, element-major) first, then the Stacey
-// face points, the source points and the receiver points.  Every point's row
-// lists its element terms in the order a sequential loop over the elements in
-// `visit` order adds them, then its face, source and receiver terms.  Summing
-// a row in this order makes the assembled force independent of scheduling.
-void build_assembly_csr(int nglob, int nspec, const std::vector<int>& owner, const std::vector<int>& visit,
-                        std::vector<int>& off, std::vector<int>& idx) {
-  const size_t nloc = (size_t)nspec * 125;
-  if (owner.size() >= (size_t)INT32_MAX) SEM_FAIL("too many contributions for 32-bit CSR indices");
-  off.assign((size_t)nglob + 1, 0);
-  for (size_t q = 0; q < owner.size(); ++q) off[(size_t)owner[q] + 1]++;
-  for (int g = 0; g < nglob; ++g) off[(size_t)g + 1] += off[g];
-  idx.assign(owner.size(), 0);
-  std::vector<int> fill(off.begin(), off.end() - 1);
-  for (int e : visit)
-    for (int L = 0; L < 125; ++L) {
-      const size_t q = (size_t)e * 125 + L;
-      idx[fill[owner[q]]++] = (int)q;
-    }
-  for (size_t q = nloc; q < owner.size(); ++q) idx[fill[owner[q]]++] = (int)q;
-}
-
 // Host copy of the contribution owners of a prepared mesh (see build_assembly_csr).
 std::vector<int> contribution_owners(SemMesh* m) {
   const size_t nloc = (size_t)m->nspec * 125, nfp = (size_t)m->nfaces * 25;
@@ -106,6 +83,29 @@ void upload_assembly_csr(SemMesh* m, const std::vector<int>& off, const std::vec
 }
 
 }  // namespace
+
+// Gather-assembly CSR.  owner[q] is the global point of contribution q: the
+// element-local points (125 per element, element-major) first, then the Stacey
+// face points, the source points and the receiver points.  Every point's row
+// lists its element terms in the order a sequential loop over the elements in
+// `visit` order adds them, then its face, source and receiver terms.  Summing
+// a row in this order makes the assembled force independent of scheduling.
+void sem_build_assembly_csr(int nglob, int nspec, const std::vector<int>& owner, const std::vector<int>& visit,
+                        std::vector<int>& off, std::vector<int>& idx) {
+  const size_t nloc = (size_t)nspec * 125;
+  if (owner.size() >= (size_t)INT32_MAX) SEM_FAIL("too many contributions for 32-bit CSR indices");
+  off.assign((size_t)nglob + 1, 0);
+  for (size_t q = 0; q < owner.size(); ++q) off[(size_t)owner[q] + 1]++;
+  for (int g = 0; g < nglob; ++g) off[(size_t)g + 1] += off[g];
+  idx.assign(owner.size(), 0);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int e : visit)
+    for (int L = 0; L < 125; ++L) {
+      const size_t q = (size_t)e * 125 + L;
+      idx[fill[owner[q]]++] = (int)q;
+    }
+  for (size_t q = nloc; q < owner.size(); ++q) idx[fill[owner[q]]++] = (int)q;
+}
 
 extern "C" {
 
@@ -255,7 +255,7 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
     std::vector<int> owner = contribution_owners(m);
     std::vector<int> visit(m->nspec), off, idx;
     for (int e = 0; e < m->nspec; ++e) visit[e] = e;
-    build_assembly_csr(m->nglob, m->nspec, owner, visit, off, idx);
+    sem_build_assembly_csr(m->nglob, m->nspec, owner, visit, off, idx);
     upload_assembly_csr(m, off, idx);
     m->d_floc = dev_alloc<realw>(nloc * 3);
     if (nfp) m->d_face_scratch = dev_alloc<realw>(nfp * 3);
@@ -331,9 +331,10 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
                           visit)) {
       bool identity = true;
       for (int e = 0; e < m->nspec && identity; ++e) identity = visit[e] == e;
+      if (!identity) m->h_visit = visit;
       if (!identity) {
         std::vector<int> off, idx;
-        build_assembly_csr(m->nglob, m->nspec, contribution_owners(m), visit, off, idx);
+        sem_build_assembly_csr(m->nglob, m->nspec, contribution_owners(m), visit, off, idx);
         upload_assembly_csr(m, off, idx);
       }
     }
@@ -542,7 +543,7 @@ void fwat_assembly_csr_(int* NSPEC_AB, int* NGLOB_AB, int* h_ibool, int* num_abs
     for (int e = 0; e < nspec; ++e) visit[e] = e;
   }
   std::vector<int> o, i;
-  build_assembly_csr(nglob, nspec, owner, visit, o, i);
+  sem_build_assembly_csr(nglob, nspec, owner, visit, o, i);
   std::memcpy(off, o.data(), o.size() * sizeof(int));
   std::memcpy(idx, i.data(), i.size() * sizeof(int));
 }

@@ -70,6 +70,7 @@ struct SemMesh {
   int regular_fast_path = 1;  // folded-table element pass on all-regular meshes
   int blocks_per_cu_fwd = 0;  // element-pass workgroups per CU (0: one per element)
   int blocks_per_cu_adj = 8;
+  int graph = 0;              // whole-run loops captured into a HIP graph (fwat_set_graph_)
   // reduced-precision statics hold value/scale (FP16 needs it for range)
   float static_scale_kappa = 1.0f, static_scale_mu = 1.0f;
   float static_scale_jinv = 1.0f, static_scale_jac = 1.0f;
@@ -160,6 +161,9 @@ struct SemMesh {
   int* d_asm_off = nullptr;     // [nglob+1]
   int* d_asm_idx = nullptr;     // [ncontrib]
   size_t n_contrib = 0;
+  // element visiting order of the phase lists when it is not element order
+  // (empty: element order); every point's element terms are summed in it
+  std::vector<int> h_visit;
 
   // element slabs: the fused loops run element pass + assembly slab after slab
   // (contiguous element ranges of slab_elems elements); a point is assembled
@@ -350,6 +354,13 @@ __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict_
   newmark_point<TWO>(g, load_point<TWO>(g, rm, F0, F1, rho_part), f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last,
                      rho_kl, rho_part, flush, hess_kl, dtk);
 }
+
+// Gather-assembly CSR (prepare.hip): owner[q] = global point of contribution q
+// (element-local points first, then face, source, receiver points); every
+// row lists its element terms in the `visit` order of the elements, then the
+// rest in contribution order.
+void sem_build_assembly_csr(int nglob, int nspec, const std::vector<int>& owner, const std::vector<int>& visit,
+                            std::vector<int>& off, std::vector<int>& idx);
 
 // internal launchers shared between the C-ABI files
 void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin, int elem_count,

@@ -101,7 +101,7 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
   // before either is added, two loads in flight per lane; same summation order
   for (; q + 1 < q1; q += 2) {
     const int c0 = idx[q], c1 = idx[q + 1];
-    if (c1 >= nloc) break;  // c0 < c1: both element-local below this line
+    if (c1 >= nloc) break;  // element terms lead each row: c0 is one too
     const realw* P0 = F0.floc + 3 * (size_t)c0;
     const realw* P1 = F0.floc + 3 * (size_t)c1;
     const realw a0 = P0[0], a1 = P0[1], a2 = P0[2], b0 = P1[0], b1 = P1[1], b2 = P1[2];
@@ -512,6 +512,31 @@ struct ForceTimer {
   }
 };
 
+// HIP graph of a whole-run step loop (fwat_set_graph_): the loop's launches
+// are captured from the simulation stream into one graph, which is
+// instantiated and launched as one unit, so the GPU runs the steps' kernels
+// back to back without waiting on the host's launch rate.  The captured loops
+// issue kernels only (no memset, copy or host synchronisation).
+struct LoopGraph {
+  SemMesh* m;
+  bool on;
+  LoopGraph(SemMesh* m_, bool on_) : m(m_), on(on_) {
+    if (on) HIP_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  }
+  void launch() {
+    if (!on) return;
+    on = false;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    HIP_CHECK(hipStreamEndCapture(m->stream, &g));
+    HIP_CHECK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphLaunch(x, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    HIP_CHECK(hipGraphExecDestroy(x));
+    HIP_CHECK(hipGraphDestroy(g));
+  }
+};
+
 void stage_forces_timed(SemMesh* m, int nfields, int kmode, realw dt, ForceTimer* tm, bool gather) {
   if (tm) tm->open(m->stream);
   sem_launch_forces(m, nfields, kmode, 0, m->nspec, dt, nullptr, gather);
@@ -787,7 +812,10 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   ForceTimer timer, atimer;
   ForceTimer* tm = nullptr;
   ForceTimer* ta = nullptr;
-  if (*timed) {
+  // graph capture covers the plain fused loops (not the brick or slab passes);
+  // a captured run is not timed
+  const bool graph = m->graph && fz && m->nslabs == 0 && !sem_brick_active(m, *SIMULATION_TYPE == 3 ? 2 : 1);
+  if (*timed && !graph) {
     const size_t per_step = m->nslabs > 0 ? (size_t)m->nslabs : 1;  // bracketed launches per step
     timer.begin(N, per_step);
     atimer.begin(N, per_step);
@@ -798,7 +826,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
   const auto step_begin = [&](int it) {
     if (tm) tm->at_step(it);
     if (ta) ta->at_step(it);
-    if (it > 1 && (it - 1) % kSyncEvery == 0) HIP_CHECK(hipStreamSynchronize(m->stream));
+    if (!graph && it > 1 && (it - 1) % kSyncEvery == 0) HIP_CHECK(hipStreamSynchronize(m->stream));
   };
   if (m->nsources > 0 && !m->d_stf) SEM_FAIL("fwat_set_stf_ not called");
   hipStream_t s = m->stream;
@@ -852,6 +880,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       }
     } else if (fz) {
       const int nfp = m->nfaces * 25;
+      LoopGraph lg(m, graph);
       stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
       for (int it = 1; it <= N; ++it) {
         step_begin(it);
@@ -868,6 +897,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         stage_assemble(m, it == N, false, F0, F0, 0.0);
         if (ta) ta->close(s);
       }
+      lg.launch();
     } else {
       for (int it = 1; it <= N; ++it) {
         step_begin(it);
@@ -958,6 +988,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       }
     } else if (fz) {
       const int nfp = m->nfaces * 25;
+      LoopGraph lg(m, graph);
       stage_predictor(m, false, dt);  // adjoint field (from rest)
       for (int it = 1; it <= N; ++it) {
         step_begin(it);
@@ -978,6 +1009,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         stage_assemble(m, it == N, true, F0, F1, dtk);
         if (ta) ta->close(s);
       }
+      lg.launch();
     } else {
       for (int it = 1; it <= N; ++it) {
         step_begin(it);
@@ -1048,6 +1080,14 @@ void fwat_copy_probe_(long* nbytes, int* reps, double* avg_ms) {
   HIP_CHECK(hipEventDestroy(e1));
   HIP_CHECK(hipFree(a));
   HIP_CHECK(hipFree(b));
+}
+
+// HIP graph capture of the whole-run loop (fused gather path): 1 = every
+// fwat_iterate_time_ run captures its step loop into one graph and launches
+// it (no per-kernel timing), 0 = launches on the stream (default).
+void fwat_set_graph_(long* Mesh_pointer, int* on) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  m->graph = *on != 0;
 }
 
 // Launch-shape knobs of the element passes (set by the autotuner in

@@ -129,6 +129,11 @@ class ElasticRun:
             _lib.call("fwat_set_brick_", self.h, int(brick[0]), int(brick[1]), int(brick[2]),
                       int(brick_blocks_per_cu))
 
+    def set_graph(self, on=True):
+        """Capture every whole-run loop of this handle into one HIP graph and launch
+        it as a unit (fused gather path; captured runs are not timed)."""
+        _lib.call("fwat_set_graph_", self.h, int(bool(on)))
+
     def set_kernel_block(self, nsteps):
         """Steps per fp32 partial-sum block of the mu/kappa (c_ij) kernels; 1 = fp64 every step."""
         _lib.call("fwat_set_kernel_block_", self.h, int(nsteps))

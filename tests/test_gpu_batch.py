@@ -126,3 +126,41 @@ def test_batch_event_matches_oracle():
         assert es < 1e-5
         for q, v in ek.items():
             assert v < 1e-4, q
+
+
+def test_fp16_stacey_store_matches_fp32_store_and_oracle():
+    """The batch's Stacey store in fp16 with one scale per (step, event)
+    record: half the HBM; the reconstruction replays the rounded terms.
+    Seismograms are untouched (the forward run uses the fp32 terms); kernels
+    stay within the mixed-precision bar (1e-3) of the FP64 oracle."""
+    nsteps = [160, 120, 140, 100]
+    c = make_batch_case(4, nsteps, seed=21)
+    m = c["mesh"]
+    rec = Receivers(m, c["rxyz"])
+    srcs = [Sources(m, ev["sxyz"], ev["mt"], ev["f0"]) for ev in c["events"]]
+    out = {}
+    for sp in (0, 1):
+        b = EventBatch(m, c["rho"], c["vp"], c["vs"], srcs, rec, c["dt"], nsteps)
+        b.set_store_precision(sp)
+        b.forward()
+        syn = b.seismograms()
+        adj = np.zeros_like(syn)
+        for e, n in enumerate(nsteps):
+            adj[e, :n] = adjoint_forcing(syn[e, :n])
+        b.set_adjoint_sources(adj)
+        b.adjoint()
+        out[sp] = (syn, b.kernels(), adj)
+        b.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for e, n in enumerate(nsteps):
+        ev = c["events"][e]
+        o = orc.Oracle(m.nex, m.ney, m.nez, m.lx, m.ly, m.lz, c["rho"].ravel(), c["vp"].ravel(),
+                       c["vs"].ravel(), dt=c["dt"], nstep=n, src_xyz=ev["sxyz"], src_mt=ev["mt"],
+                       src_f0=ev["f0"], rec_xyz=c["rxyz"], precision=64)
+        o.forward(save=True)
+        ko = o.adjoint(out[1][2][e, :n])
+        for q in ("rho_kl", "mu_kl", "kappa_kl"):
+            e32 = rel(out[1][1][q][e], out[0][1][q][e])
+            e64 = rel(out[1][1][q][e], ko[q])
+            print(f"event {e} {q}: fp16 store vs fp32 store {e32:.2e}, vs FP64 oracle {e64:.2e}", flush=True)
+            assert e32 < 1e-3 and e64 < 1e-3, q
