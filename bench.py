@@ -56,9 +56,15 @@ def parse():
                         "correlation measurements of a C2-sized event (400 receivers x 3 x 10000 samples)")
     p.add_argument("--post-events", type=int, default=32, help="events summed by the post leg")
     p.add_argument("--c4-events", type=int, default=32)
-    p.add_argument("--c4-nstep-scale", type=float, default=0.25,
+    p.add_argument("--c4-nstep-scale", type=float, default=1.0,
                    help="NSTEP of the c4 events is U[4000,12000] times this factor")
+    p.add_argument("--c4-store", type=int, default=1, choices=[0, 1],
+                   help="Stacey store of the batches: 0 fp32, 1 fp16 with a scale per (step, event) record")
+    p.add_argument("--c4-plain-events", type=int, default=4,
+                   help="events also run one at a time on the plain single-event path (its rate per "
+                        "event-step is taken from these)")
     p.add_argument("--events-per-batch", type=int, default=0, help="0: autotuned")
+    p.add_argument("--hbm-query", type=int, default=1, help="c4: size batches from the free HBM (torch)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-steps", type=int, default=None,
                    help="steps of the CPU baseline sample (default: 4 for c2/c4, the whole run for c1)")
@@ -400,18 +406,29 @@ def batched(a):
     log(f"c4: mesh {mesh.nspec} el, {mesh.nglob} pts; {len(srcs)} events, NSTEP {min(nsteps)}..{max(nsteps)} "
         f"({time.time() - t0:.1f}s)")
     sp = a.static_precision
+    # events per batch: what the tuner finds fastest among the sizes whose batch (fields,
+    # element-local forces, kernels, Stacey store of the longest event) fits the HBM left
+    # beside the mesh handle
+    from specfwat_amd.batch import batch_device_bytes
+    free_b, _ = __import__("torch").cuda.mem_get_info() if a.hbm_query else (288e9, None)
+    budget = 0.85 * free_b
+    fits = [e for e in (2, 4, 8) if batch_device_bytes(mesh, e, max(nsteps), a.c4_store) <= budget]
+    if not fits:
+        raise RuntimeError("no batch size fits the HBM budget")
     if a.events_per_batch:
         E, bpc, ttable = a.events_per_batch, 8, []
     else:
-        (E, bpc), ttable = tune_batch(mesh, smooth_model, srcs, rec, dt, static_precision=sp)
+        (E, bpc), ttable = tune_batch(mesh, smooth_model, srcs, rec, dt, static_precision=sp, sizes=tuple(fits))
         log("batch tuner (events, blocks/CU, fwd ms/event-step, adj ms/event-step): "
             + ", ".join(f"({e},{b},{x:.3f},{y:.3f})" for e, b, x, y in ttable) + f" -> E={E} bpc={bpc}")
+    log(f"batch sizes fitting {budget / 1e9:.0f} GB with the {'fp16' if a.c4_store else 'fp32'} store: {fits}")
     groups = plan_batches(nsteps, E)
 
     # observed data: batched forward runs in the true model (untimed)
     obs = [None] * len(srcs)
     for g in groups:
         b = EventBatch(mesh, *true_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g])
+        b.set_store_precision(a.c4_store)
         b.forward()
         s = b.seismograms()
         for k, i in enumerate(g):
@@ -423,6 +440,7 @@ def batched(a):
         b = EventBatch(mesh, *smooth_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g],
                        static_precision=sp)
         b.set_tuning(bpc)
+        b.set_store_precision(a.c4_store)
         _lib.call("fwat_synchronize_", b.run.h)
         ts = time.time()
         b.forward(timed=True)
@@ -457,7 +475,11 @@ def batched(a):
     # plain path: the same events one at a time (fused single-event path, tuned shapes)
     tuning, _ = tune_element_passes(mesh, smooth_model, srcs[0], rec, dt, static_precision=sp)
     pwall, perr, kerr = 0.0, 0.0, {}
-    for i, src in enumerate(srcs):
+    # the plain path on the first c4_plain_events events of the first batch (those whose kernels
+    # are compared), its rate per event-step applied to the whole set
+    plain = [i for g in groups for i in g][:max(1, a.c4_plain_events)]
+    for i in plain:
+        src = srcs[i]
         f = ElasticRun(mesh, *smooth_model, src, rec, dt, nsteps[i], 1, True, static_precision=sp)
         ad = ElasticRun(mesh, *smooth_model, src, rec, dt, nsteps[i], 3, False, static_precision=sp)
         f.set_tuning(**tuning)
@@ -481,8 +503,8 @@ def batched(a):
                 kerr[q] = max(kerr.get(q, 0.0),
                               float(np.linalg.norm(bkern[i][q] - k[q]) / np.linalg.norm(k[q])))
         ad.close()
-    pvalue = total / pwall
-    log(f"plain: {pwall:.2f}s -> {pvalue:.4e} GLL-pt-steps/s/event")
+    pvalue = mesh.nglob * float(sum(nsteps[i] for i in plain)) / pwall
+    log(f"plain: {pwall:.2f}s for {len(plain)} events -> {pvalue:.4e} GLL-pt-steps/s/event")
 
     # roofline of the largest batch's dominant pass (algorithmic bytes per launch)
     ftimes, atimes = times[max(times)]
@@ -491,16 +513,19 @@ def batched(a):
     nfp = mesh.num_abs_boundary_faces * 25
     ncontrib = nloc + nfp + 125 * (e + rec.nrec)
     kmu = 8.0 if sp == 0 else 4.0
+    kacc = kernel_acc_bytes(KERNEL_BLOCK)  # the batch handle's partial-sum blocks (library default)
     kinds = {
         # statics once (ibool 4 + kappa,mu) per local point; per event: both fields gathered
-        # 24 per unique point, forces of both fields written 24 and fp64 mu/kappa read+write 32
-        # per local point
-        f"k_batch_elem<{e},2>": ((4.0 + kmu) * nloc + e * (24.0 * nglob + 56.0 * nloc), atimes[0], atimes[1]),
+        # 24 per unique point, forces of both fields written 24 and the mu/kappa accumulators
+        # 2 x kacc per local point
+        f"k_batch_elem<{e},2>": ((4.0 + kmu) * nloc + e * (24.0 * nglob + (24.0 + 2 * kacc) * nloc),
+                                 atimes[0], atimes[1]),
         # CSR 4 per point + 4 per contribution, rmass_inv 12 per point once; per event: forces
         # of both fields 24 per local point, Stacey terms 24 per face point, u,v of both fields
-        # read + written 96 and the fp64 rho kernel read+written 16 per unique point
+        # read + written 96 and the rho kernel kacc per unique point
         f"k_batch_asm<{e},two-field>": (16.0 * nglob + 4.0 * ncontrib
-                                        + e * (24.0 * nloc + 24.0 * nfp + 112.0 * nglob), atimes[2], atimes[3]),
+                                        + e * (24.0 * nloc + 24.0 * nfp + (96.0 + kacc) * nglob),
+                                        atimes[2], atimes[3]),
     }
     kname = max(kinds, key=lambda k: kinds[k][1])
     kbytes, kms, kn = kinds[kname]
@@ -527,8 +552,10 @@ def batched(a):
                    "events_per_batch": E, "blocks_per_cu": bpc, "batches": [len(g) for g in groups],
                    "tuner": [list(r) for r in ttable]},
         "batched_vs_plain": {"plain_value": pvalue, "speedup": bvalue / pvalue,
+                             "plain_events_timed": len(plain),
                              "max_seismogram_rel_l2": perr, "kernel_rel_l2": kerr,
-                             "kernel_events_checked": sorted(check)},
+                             "kernel_events_checked": sorted(check),
+                             "stacey_store": "fp16 + scale per (step, event)" if a.c4_store else "fp32"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
@@ -706,11 +733,12 @@ def aniso_smooth(a):
     nglob, nloc = mesh.nglob, mesh.nspec * 125
     nfp = mesh.num_abs_boundary_faces * 25
     ncontrib = nloc + nfp + 125 * (src.nsources + rec.nrec)
-    # adjoint element pass, anisotropic: ibool 4 + 21 c_ij 84 per local point, fp64 c_ij kernels
-    # read+write 336, element-local forces of both fields 24; gathers 24 per unique point
+    # adjoint element pass, anisotropic: ibool 4 + 21 c_ij 84 per local point, 21 c_ij kernel
+    # accumulators 21 x kacc, element-local forces of both fields 24; gathers 24 per unique point
+    kacc = kernel_acc_bytes(KERNEL_BLOCK)
     kinds = {
-        "k_forces<2,aniso,fused-kernels>": (448.0 * nloc + 24.0 * nglob, adj_ms, adj_n),
-        "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp + 124.0 * nglob,
+        "k_forces<2,aniso,fused-kernels>": ((112.0 + 21 * kacc) * nloc + 24.0 * nglob, adj_ms, adj_n),
+        "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp + (108.0 + kacc) * nglob,
                                   a_asm[0], a_asm[1]),
     }
     kname = max(kinds, key=lambda q: kinds[q][1])

@@ -197,7 +197,7 @@ void fwat_assembly_csr_(int *NSPEC_AB, int *NGLOB_AB, int *h_ibool, int *num_abs
 void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 
 /* ---- multi-event batches: E = 1, 2, 4 or 8 events advanced together on one regular
- *      isotropic mesh.  Replaces SpecFWAT's serial event loop
+ *      (all-cube) mesh, isotropic or with 21 c_ij.  Replaces SpecFWAT's serial event loop
  *      (src/preproc/fwat_fwd_measure_adj.f90:43-51, one PrepareFWD%simulation per event,
  *      src/preproc/prepare_fwd.f90:297-326); per-event arithmetic is fwat_iterate_time_'s.
  *      The mesh handle holds the model, receivers and Stacey faces (its own sources are
@@ -225,6 +225,11 @@ void fwat_batch_seismograms_(long *Batch_pointer, realw *h_seis);
  * h_rhostore(NGLLX,NGLLY,NGLLZ,NSPEC_AB) scales rho_kl */
 void fwat_batch_kernels_(long *Batch_pointer, realw *h_rhostore, double *h_rho_kl, double *h_mu_kl,
                          double *h_kappa_kl);
+/* kernels of an anisotropic batch (a mesh prepared with ANISOTROPY, 21 c_ij): rho_kl
+ * (NGLLX,NGLLY,NGLLZ,NSPEC_AB,nevents) scaled by rhostore and cijkl_kl
+ * (21,NGLLX,NGLLY,NGLLZ,NSPEC_AB,nevents) in c11..c66 order (kernel_io.f90:100-156), as
+ * transfer_kernels_ani_to_host_dp_ returns them for one event */
+void fwat_batch_kernels_ani_(long *Batch_pointer, realw *h_rhostore, double *h_rho_kl, double *h_cijkl_kl);
 void fwat_batch_last_time_(long *Batch_pointer, double *elem_ms, int *elem_n, double *asm_ms, int *asm_n);
 void fwat_batch_destroy_(long *Batch_pointer);
 

@@ -72,6 +72,7 @@ SIGNATURES = {
     "fwat_batch_adjoint_": "Li",
     "fwat_batch_seismograms_": "LF",
     "fwat_batch_kernels_": "LFDDD",
+    "fwat_batch_kernels_ani_": "LFDD",
     "fwat_batch_last_time_": "LDIDI",
     "fwat_batch_destroy_": "L",
     "smooth_pde_cuda_": "iiiiiiIii" + "F" * 10 + "F" + "FFFFF" + "I" + "I" + "ff" + "iiIII" + "FFFFF" + "Ifi",

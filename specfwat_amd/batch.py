@@ -8,7 +8,7 @@ run with reconstruction -> rho/mu/kappa kernels) for E = 1, 2, 4 or 8 events
 at once through the fwat_batch_* entry points, so that one element pass
 reads the mesh and model once for all events.  Events may have different
 NSTEP; every event keeps its own source and source time function and shares
-the receivers.  Regular (all-cube) isotropic meshes only.
+the receivers.  Regular (all-cube) meshes, isotropic or with 21 c_ij.
 """
 import numpy as np
 
@@ -20,9 +20,10 @@ BATCH_SIZES = (1, 2, 4, 8)
 
 
 class EventBatch:
-    def __init__(self, mesh, rho, vp, vs, sources, receivers, dt, nsteps, static_precision=0):
+    def __init__(self, mesh, rho, vp, vs, sources, receivers, dt, nsteps, static_precision=0, cij=None):
         """sources: one Sources object per event (its first source is used);
-        nsteps: NSTEP per event."""
+        nsteps: NSTEP per event; cij: [nspec, 125, 21] anisotropic model (rho/vp/vs
+        then give the density and the Stacey impedances), kernels rho and 21 c_ij."""
         e = len(sources)
         if e not in BATCH_SIZES:
             raise ValueError(f"batch size {e} not supported {BATCH_SIZES}")
@@ -36,8 +37,9 @@ class EventBatch:
         self.nrec = receivers.nrec if receivers is not None else 0
         self.rho = np.ascontiguousarray(rho, F32).reshape(mesh.nspec, 125)
         # the mesh handle holds model, receivers and Stacey faces; no sources of its own
+        self.aniso = cij is not None
         self.run = ElasticRun(mesh, rho, vp, vs, None, receivers, dt, self.nmax, 1, False,
-                              static_precision=static_precision)
+                              static_precision=static_precision, cij=cij)
         sarr = np.ascontiguousarray(np.stack([s.sourcearrays[0] for s in sources]), F32)
         isp = np.array([s.ispec[0] for s in sources], np.int32)
         stf = np.zeros((e, self.nmax))
@@ -76,8 +78,13 @@ class EventBatch:
         _lib.call("fwat_batch_adjoint_", self.h, int(timed))
 
     def kernels(self):
-        """save_adjoint_kernels-scaled kernels per event, float64 [E][nspec][125]."""
+        """save_adjoint_kernels-scaled kernels per event, float64 [E][nspec][125]
+        (cijkl_kl [E][nspec][125][21] for an anisotropic batch)."""
         shape = (self.nevents, self.mesh.nspec, 125)
+        if self.aniso:
+            rho_kl, cijkl = np.zeros(shape), np.zeros(shape + (21,))
+            _lib.call("fwat_batch_kernels_ani_", self.h, self.rho, rho_kl, cijkl)
+            return {"rho_kl": rho_kl, "cijkl_kl": cijkl}
         rho_kl, mu_kl, kappa_kl = np.zeros(shape), np.zeros(shape), np.zeros(shape)
         _lib.call("fwat_batch_kernels_", self.h, self.rho, rho_kl, mu_kl, kappa_kl)
         return {"rho_kl": rho_kl, "mu_kl": mu_kl, "kappa_kl": kappa_kl}

@@ -22,7 +22,9 @@
 // at the end (event e idles for the first NSTEP_max - NSTEP_e steps), so at
 // global adjoint step it every active event replays Stacey row NSTEP_max - it,
 // which is its own forward step NSTEP_e + 1 - it_e.
-// Regular (all-cube) isotropic meshes only; other meshes are refused.
+// Regular (all-cube) meshes, isotropic (kappa, mu; rho/mu/kappa kernels) or
+// anisotropic (21 c_ij; rho and 21 c_ij kernels [E][nspec][21][125]); meshes
+// with irregular elements are refused.
 #include "sem_device.h"
 
 #include <algorithm>
@@ -57,6 +59,9 @@ struct BatchElemArgs {
   int kflush;        // this step closes a partial-sum block
   double dt;
   int active_mask;  // bit e set: event e advances this step
+  const realw* cij;  // [nspec][21][125] (anisotropic)
+  double* cijkl_kl;  // [E][nspec][21][125]
+  float* cij_part;   // [E][nspec][21][125] fp32 partial sums
 };
 
 __device__ __forceinline__ float bf16f(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
@@ -207,6 +212,146 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
         const realw ltr = lam * (duxdx + duydy + duzdz);
         s_s[f][0][t] = ltr + m2 * duxdx; s_s[f][1][t] = ltr + m2 * duydy; s_s[f][2][t] = ltr + m2 * duzdz;
         s_s[f][3][t] = ml * exy2; s_s[f][4][t] = ml * exz2; s_s[f][5][t] = ml * eyz2;
+      }
+      __syncthreads();  // stresses complete; every lane is done reading s_u[*][ev]
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        realw fx = 0, fy = 0, fz = 0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l) {
+          const int La = La0 + l, Lb = Lb0 + 5 * l, Lc = Lc0 + 25 * l;
+          fx += s_s[f][0][La] * hwI[l]; fy += s_s[f][3][La] * hwI[l]; fz += s_s[f][4][La] * hwI[l];
+          fx += s_s[f][3][Lb] * hwJ[l]; fy += s_s[f][1][Lb] * hwJ[l]; fz += s_s[f][5][Lb] * hwJ[l];
+          fx += s_s[f][4][Lc] * hwK[l]; fy += s_s[f][5][Lc] * hwK[l]; fz += s_s[f][2][Lc] * hwK[l];
+        }
+        s_u[f][ev][0][t] = fx; s_u[f][ev][1][t] = fy; s_u[f][ev][2][t] = fz;  // own slot only
+      }
+      __syncthreads();  // s_s free for the next event
+    }
+    if (act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        realw out[3 * E];
+#pragma unroll
+        for (int ev = 0; ev < E; ++ev) {
+          out[3 * ev] = s_u[f][ev][0][t];
+          out[3 * ev + 1] = s_u[f][ev][1][t];
+          out[3 * ev + 2] = s_u[f][ev][2][t];
+        }
+        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + Lg * (3 * E), out);
+      }
+    }
+  }
+}
+
+// Anisotropic element pass of E events: as k_batch_elem, with the stress of
+// each event from the 21 c_ij of the point (read per event: the element's
+// 21 x 125 values come from HBM once and from the caches for the other
+// events) and, in the adjoint pass, the 21 c_ij kernel integrands
+// (e_I' e_J + e_J' e_I) / (1 + delta_IJ) of engineering Voigt strains
+// accumulated per event.
+template <int E, int NF>
+__global__ __launch_bounds__(128) void k_batch_elem_aniso(BatchElemArgs A) {
+  __shared__ realw s_tab[128];
+  __shared__ realw s_u[NF][E][3][128];
+  __shared__ realw s_s[NF][6][128];
+  const int t = threadIdx.x;
+  if (t < 125) s_tab[t] = A.tables[t];
+  const bool act = t < 125;
+  const int tt = act ? t : 124;
+  const int I = tt % 5, J = (tt / 5) % 5, K = tt / 25;
+  __syncthreads();
+  const realw xr = A.xix_regular;
+  const realw jx = A.jacobian_regular * A.xix_regular;
+  realw hpI[5], hpJ[5], hpK[5], hwI[5], hwJ[5], hwK[5];
+  {
+    const realw f1 = -jx * s_tab[50 + J + 5 * K];
+    const realw f2 = -jx * s_tab[75 + I + 5 * K];
+    const realw f3 = -jx * s_tab[100 + I + 5 * J];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      hpI[l] = s_tab[I + 5 * l] * xr;
+      hpJ[l] = s_tab[J + 5 * l] * xr;
+      hpK[l] = s_tab[K + 5 * l] * xr;
+      hwI[l] = s_tab[25 + l + 5 * I] * f1;
+      hwJ[l] = s_tab[25 + l + 5 * J] * f2;
+      hwK[l] = s_tab[25 + l + 5 * K] * f3;
+    }
+  }
+  const int La0 = 5 * J + 25 * K, Lb0 = I + 25 * K, Lc0 = I + 5 * J;
+  const int nb = gridDim.x;
+  const int x = blockIdx.x & 7, per_xcd = nb >> 3, slot = blockIdx.x >> 3;
+  const int chunk = (A.nspec + 7) >> 3;
+  const int e_end = min(A.nspec, (x + 1) * chunk);
+  const size_t nloc21 = (size_t)A.nspec * 21 * 125;
+  const int mask = A.active_mask;
+  for (int e = x * chunk + slot; e < e_end; e += per_xcd) {
+    const size_t Lg = (size_t)e * 125 + tt;
+    const int iglob = A.ibool[Lg];
+    const realw* C = A.cij + (size_t)e * 21 * 125 + tt;
+    __syncthreads();  // previous element done with s_u
+    if (act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        realw x3[3 * E];
+        load_pt<E>((f == 0 ? A.u0 : A.u1) + (size_t)iglob * (3 * E), x3);
+#pragma unroll
+        for (int ev = 0; ev < E; ++ev) {
+          s_u[f][ev][0][t] = x3[3 * ev];
+          s_u[f][ev][1][t] = x3[3 * ev + 1];
+          s_u[f][ev][2][t] = x3[3 * ev + 2];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int ev = 0; ev < E; ++ev) {
+      if (!((mask >> ev) & 1)) continue;  // uniform over the grid
+      realw va[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        realw duxdx = 0, duydx = 0, duzdx = 0, duxdy = 0, duydy = 0, duzdy = 0, duxdz = 0, duydz = 0, duzdz = 0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l) {
+          const int La = La0 + l, Lb = Lb0 + 5 * l, Lc = Lc0 + 25 * l;
+          duxdx += s_u[f][ev][0][La] * hpI[l]; duydx += s_u[f][ev][1][La] * hpI[l]; duzdx += s_u[f][ev][2][La] * hpI[l];
+          duxdy += s_u[f][ev][0][Lb] * hpJ[l]; duydy += s_u[f][ev][1][Lb] * hpJ[l]; duzdy += s_u[f][ev][2][Lb] * hpJ[l];
+          duxdz += s_u[f][ev][0][Lc] * hpK[l]; duydz += s_u[f][ev][1][Lc] * hpK[l]; duzdz += s_u[f][ev][2][Lc] * hpK[l];
+        }
+        const realw exy2 = duxdy + duydx, exz2 = duzdx + duxdz, eyz2 = duzdy + duydz;
+        // engineering Voigt strains 1..6 = xx yy zz yz xz xy
+        const realw e1 = duxdx, e2 = duydy, e3 = duzdz, e4 = eyz2, e5 = exz2, e6 = exy2;
+        if (NF == 2) {
+          if (f == 0) {
+            va[0] = e1; va[1] = e2; va[2] = e3; va[3] = 2.0f * (0.5f * e4); va[4] = 2.0f * (0.5f * e5);
+            va[5] = 2.0f * (0.5f * e6);
+          } else if (act) {
+            const realw vb[6] = {e1, e2, e3, 2.0f * (0.5f * e4), 2.0f * (0.5f * e5), 2.0f * (0.5f * e6)};
+            const size_t k0 = (size_t)ev * nloc21 + (size_t)e * 21 * 125 + t;
+            int q = 0;
+#pragma unroll
+            for (int P = 0; P < 6; ++P)
+#pragma unroll
+              for (int Q = P; Q < 6; ++Q) {
+                realw pr = va[P] * vb[Q];
+                if (Q > P) pr += va[Q] * vb[P];
+                kernel_acc(A.cijkl_kl, A.cij_part, k0 + (size_t)q * 125, pr, A.dt, A.kflush != 0);
+                ++q;
+              }
+          }
+        }
+        asm volatile("" ::: "memory");
+        const realw c11 = C[0], c12 = C[125], c13 = C[250], c14 = C[375], c15 = C[500], c16 = C[625];
+        const realw c22 = C[750], c23 = C[875], c24 = C[1000], c25 = C[1125], c26 = C[1250];
+        const realw c33 = C[1375], c34 = C[1500], c35 = C[1625], c36 = C[1750];
+        const realw c44 = C[1875], c45 = C[2000], c46 = C[2125];
+        const realw c55 = C[2250], c56 = C[2375], c66 = C[2500];
+        s_s[f][0][t] = c11 * e1 + c12 * e2 + c13 * e3 + c14 * e4 + c15 * e5 + c16 * e6;
+        s_s[f][1][t] = c12 * e1 + c22 * e2 + c23 * e3 + c24 * e4 + c25 * e5 + c26 * e6;
+        s_s[f][2][t] = c13 * e1 + c23 * e2 + c33 * e3 + c34 * e4 + c35 * e5 + c36 * e6;
+        s_s[f][5][t] = c14 * e1 + c24 * e2 + c34 * e3 + c44 * e4 + c45 * e5 + c46 * e6;  // syz
+        s_s[f][4][t] = c15 * e1 + c25 * e2 + c35 * e3 + c45 * e4 + c55 * e5 + c56 * e6;  // sxz
+        s_s[f][3][t] = c16 * e1 + c26 * e2 + c36 * e3 + c46 * e4 + c56 * e5 + c66 * e6;  // sxy
       }
       __syncthreads();  // stresses complete; every lane is done reading s_u[*][ev]
 #pragma unroll
@@ -503,6 +648,9 @@ struct SemBatch {
   realw* seis = nullptr;                              // [E][nmax][nrec][3]
   realw* adj = nullptr;                               // [E][nmax][nrec][3]
   double *mu_kl = nullptr, *kappa_kl = nullptr;       // [E][nloc]
+  bool aniso = false;                                 // 21 c_ij model and kernels
+  double* cijkl_kl = nullptr;                         // [E][nspec][21][125]
+  float* cij_part = nullptr;                          // [E][nspec][21][125] fp32 partial sums
   float *mu_part = nullptr, *kappa_part = nullptr;    // [E][nloc] fp32 partial sums
   float* rho_part = nullptr;                          // [nglob][E]
   int kflush = 1;
@@ -531,6 +679,7 @@ void launch_elem(SemBatch* B, int nf, const int* active) {
   A.u0 = B->u; A.u1 = B->bu; A.floc0 = B->floc0; A.floc1 = B->floc1;
   A.mu_kl = B->mu_kl; A.kappa_kl = B->kappa_kl; A.dt = (double)B->dt;
   A.mu_part = B->mu_part; A.kappa_part = B->kappa_part; A.kflush = B->kflush;
+  A.cij = m->d_cij; A.cijkl_kl = B->cijkl_kl; A.cij_part = B->cij_part;
   A.active_mask = 0;
   for (int e = 0; e < E; ++e) A.active_mask |= active[e] ? 1 << e : 0;
   int dev = 0, ncu = 0;
@@ -540,7 +689,10 @@ void launch_elem(SemBatch* B, int nf, const int* active) {
   const int sp = m->static_precision;
   hipStream_t s = m->stream;
 #define LB(NF, SP) hipLaunchKernelGGL((k_batch_elem<E, NF, SP>), dim3(nb), dim3(128), 0, s, A)
-  if (nf == 2) {
+  if (B->aniso) {
+    if (nf == 2) hipLaunchKernelGGL((k_batch_elem_aniso<E, 2>), dim3(nb), dim3(128), 0, s, A);
+    else hipLaunchKernelGGL((k_batch_elem_aniso<E, 1>), dim3(nb), dim3(128), 0, s, A);
+  } else if (nf == 2) {
     if (sp == SEM_STATIC_BF16) LB(2, SEM_STATIC_BF16);
     else if (sp == SEM_STATIC_FP16) LB(2, SEM_STATIC_FP16);
     else LB(2, SEM_STATIC_FP32);
@@ -691,13 +843,16 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
   SemMesh* m = sem_mesh(Mesh_pointer);
   const int E = *nevents;
   if (E != 1 && E != 2 && E != 4 && E != 8) SEM_FAIL("batch size %d not supported (1, 2, 4, 8)", E);
-  if (m->nspec_irregular != 0 || m->anisotropy) SEM_FAIL("batches need a regular isotropic mesh");
+  if (m->nspec_irregular != 0) SEM_FAIL("batches need a regular (all-cube) mesh");
+  if (m->anisotropy && m->static_precision != SEM_STATIC_FP32)
+    SEM_FAIL("anisotropic batches keep their c_ij in fp32");
   for (int e = 0; e < E; ++e)
     if (nstep[e] < 1) SEM_FAIL("event %d has NSTEP < 1", e + 1);
   SemBatch* B = new SemBatch();
   *Batch_pointer = reinterpret_cast<long>(B);
   B->m = m;
   B->E = E;
+  B->aniso = m->anisotropy != 0;
   B->dt = *DT;
   B->nstep.assign(nstep, nstep + E);
   B->nmax = *std::max_element(B->nstep.begin(), B->nstep.end());
@@ -824,8 +979,12 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   if (!B->bu) {
     B->bu = dev_alloc<realw>(nf3); B->bv = dev_alloc<realw>(nf3); B->ba = dev_alloc<realw>(nf3);
     B->floc1 = dev_alloc<realw>(nloc * E * 3);
-    B->mu_kl = dev_alloc<double>((size_t)E * nloc);
-    B->kappa_kl = dev_alloc<double>((size_t)E * nloc);
+    if (B->aniso) {
+      B->cijkl_kl = dev_alloc<double>((size_t)E * nloc * 21);
+    } else {
+      B->mu_kl = dev_alloc<double>((size_t)E * nloc);
+      B->kappa_kl = dev_alloc<double>((size_t)E * nloc);
+    }
     B->rho_kl = dev_alloc<double>((size_t)E * ng);
   }
   // reconstructed field <- final forward states (v shifted by dt/2 a); adjoint field from rest
@@ -834,20 +993,30 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   HIP_CHECK(hipMemcpyAsync(B->ba, B->a, nf3 * 4, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_batch_vshift, dim3(nblk(nf3)), dim3(TPB), 0, s, nf3, B->bv, B->ba, 0.5f * B->dt);
   for (realw* p : {B->u, B->v, B->a}) hipLaunchKernelGGL(k_fill, dim3(nblk(nf3)), dim3(TPB), 0, s, p, nf3);
-  hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_kl, (size_t)E * nloc);
-  hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_kl, (size_t)E * nloc);
+  if (B->aniso) {
+    hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc * 21)), dim3(TPB), 0, s, B->cijkl_kl, (size_t)E * nloc * 21);
+  } else {
+    hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_kl, (size_t)E * nloc);
+    hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_kl, (size_t)E * nloc);
+  }
   hipLaunchKernelGGL(k_fill_d, dim3(nblk((size_t)E * ng)), dim3(TPB), 0, s, B->rho_kl, (size_t)E * ng);
   // fp32 partial sums of mu/kappa over blocks of the mesh handle's kernel_block steps
   const int kb = m->kernel_block;
   dev_free(B->mu_part);
   dev_free(B->kappa_part);
+  dev_free(B->cij_part);
   dev_free(B->rho_part);
   if (kb > 1) {
-    B->mu_part = dev_alloc<float>((size_t)E * nloc);
-    B->kappa_part = dev_alloc<float>((size_t)E * nloc);
+    if (B->aniso) {
+      B->cij_part = dev_alloc<float>((size_t)E * nloc * 21);
+      hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc * 21)), dim3(TPB), 0, s, B->cij_part, (size_t)E * nloc * 21);
+    } else {
+      B->mu_part = dev_alloc<float>((size_t)E * nloc);
+      B->kappa_part = dev_alloc<float>((size_t)E * nloc);
+      hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_part, (size_t)E * nloc);
+      hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_part, (size_t)E * nloc);
+    }
     B->rho_part = dev_alloc<float>((size_t)E * ng);
-    hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->mu_part, (size_t)E * nloc);
-    hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * nloc)), dim3(TPB), 0, s, B->kappa_part, (size_t)E * nloc);
     hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)E * ng)), dim3(TPB), 0, s, B->rho_part, (size_t)E * ng);
   }
   HIP_CHECK(hipGetLastError());
@@ -890,6 +1059,7 @@ void fwat_batch_kernels_(long* Batch_pointer, realw* h_rhostore, double* h_rho_k
                          double* h_kappa_kl) {
   SemBatch* B = batch_of(Batch_pointer);
   SemMesh* m = B->m;
+  if (B->aniso) SEM_FAIL("anisotropic batch: its kernels come from fwat_batch_kernels_ani_");
   if (!B->mu_kl) SEM_FAIL("fwat_batch_adjoint_ has not run");
   const int E = B->E;
   const size_t nloc = (size_t)m->nspec * 125, ng = (size_t)m->nglob;
@@ -913,6 +1083,34 @@ void fwat_batch_kernels_(long* Batch_pointer, realw* h_rhostore, double* h_rho_k
     }
 }
 
+// Kernels of an anisotropic batch per event: rho_kl (NGLLX,NGLLY,NGLLZ,NSPEC_AB,nevents) scaled by
+// rhostore, and cijkl_kl (21,NGLLX,NGLLY,NGLLZ,NSPEC_AB,nevents) in the order kernel_io.f90:100-156
+// reads (c11..c66), scaled by -1 as the single-event transfer_kernels_ani_to_host_dp_.
+void fwat_batch_kernels_ani_(long* Batch_pointer, realw* h_rhostore, double* h_rho_kl, double* h_cijkl_kl) {
+  SemBatch* B = batch_of(Batch_pointer);
+  SemMesh* m = B->m;
+  if (!B->aniso) SEM_FAIL("isotropic batch: its kernels come from fwat_batch_kernels_");
+  if (!B->cijkl_kl) SEM_FAIL("fwat_batch_adjoint_ has not run");
+  const int E = B->E;
+  const size_t nloc = (size_t)m->nspec * 125, ng = (size_t)m->nglob;
+  std::vector<double> rg((size_t)E * ng), ck(nloc * 21);
+  std::vector<int> ib(nloc);
+  hipStream_t s = m->stream;
+  HIP_CHECK(hipMemcpyAsync(rg.data(), B->rho_kl, rg.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(ib.data(), m->d_ibool, nloc * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  for (int e = 0; e < E; ++e) {
+    HIP_CHECK(hipMemcpyAsync(ck.data(), B->cijkl_kl + (size_t)e * nloc * 21, ck.size() * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    double* out = h_cijkl_kl + (size_t)e * nloc * 21;
+    for (size_t el = 0; el < (size_t)m->nspec; ++el)
+      for (int q = 0; q < 21; ++q)
+        for (int L = 0; L < 125; ++L) out[(el * 125 + L) * 21 + q] = -ck[(el * 21 + q) * 125 + L];
+    for (size_t L = 0; L < nloc; ++L)
+      h_rho_kl[(size_t)e * nloc + L] = -(double)h_rhostore[L] * rg[(size_t)ib[L] * E + e];
+  }
+}
+
 // element / assembly pass time of the last timed batch run (ms) and step counts
 void fwat_batch_last_time_(long* Batch_pointer, double* elem_ms, int* elem_n, double* asm_ms, int* asm_n) {
   SemBatch* B = batch_of(Batch_pointer);
@@ -929,8 +1127,8 @@ void fwat_batch_destroy_(long* Batch_pointer) {
   dev_free(B->floc0); dev_free(B->floc1); dev_free(B->face_scratch); dev_free(B->store);
   dev_free(B->store16); dev_free(B->store_max); dev_free(B->face_replay);
   dev_free(B->seis); dev_free(B->adj);
-  dev_free(B->mu_kl); dev_free(B->kappa_kl); dev_free(B->rho_kl);
-  dev_free(B->mu_part); dev_free(B->kappa_part); dev_free(B->rho_part);
+  dev_free(B->mu_kl); dev_free(B->kappa_kl); dev_free(B->rho_kl); dev_free(B->cijkl_kl);
+  dev_free(B->mu_part); dev_free(B->kappa_part); dev_free(B->rho_part); dev_free(B->cij_part);
   delete B;
   *Batch_pointer = 0;
 }

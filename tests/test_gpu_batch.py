@@ -164,3 +164,54 @@ def test_fp16_stacey_store_matches_fp32_store_and_oracle():
             e64 = rel(out[1][1][q][e], ko[q])
             print(f"event {e} {q}: fp16 store vs fp32 store {e32:.2e}, vs FP64 oracle {e64:.2e}", flush=True)
             assert e32 < 1e-3 and e64 < 1e-3, q
+
+
+def test_anisotropic_batch_matches_single_event_runs():
+    """HTI model with 21 c_ij: a batch of 4 events of uneven NSTEP against each
+    event's single-event anisotropic run (rho and the 21 c_ij kernels) and one
+    event against the FP64 oracle."""
+    from specfwat_amd.model import hti_model
+    nsteps = [100, 70, 120, 90]
+    c = make_batch_case(4, nsteps, seed=33)
+    m = c["mesh"]
+    cij = hti_model(m, c["rho"], c["vp"], c["vs"], seed=9, block=2 * H)
+    rec = Receivers(m, c["rxyz"])
+    srcs = [Sources(m, ev["sxyz"], ev["mt"], ev["f0"]) for ev in c["events"]]
+    b = EventBatch(m, c["rho"], c["vp"], c["vs"], srcs, rec, c["dt"], nsteps, cij=cij)
+    b.forward()
+    bsyn = b.seismograms()
+    adj = np.zeros_like(bsyn)
+    for e, n in enumerate(nsteps):
+        adj[e, :n] = adjoint_forcing(bsyn[e, :n])
+    b.set_adjoint_sources(adj)
+    b.adjoint()
+    bk = b.kernels()
+    b.close()
+    for e, n in enumerate(nsteps):
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], srcs[e], rec, c["dt"], n, 1, True, cij=cij)
+        f.iterate_time()
+        syn = f.seismograms()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], srcs[e], rec, c["dt"], n, 3, False, cij=cij)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj[e, :n])
+        a.iterate_time()
+        k = a.kernels()
+        a.close()
+        es = rel(bsyn[e, :n], syn)
+        er = rel(bk["rho_kl"][e], k["rho_kl"])
+        ec = rel(bk["cijkl_kl"][e], k["cijkl_kl"])
+        print(f"aniso event {e} nstep {n}: seis {es:.2e}, rho_kl {er:.2e}, cijkl_kl {ec:.2e}", flush=True)
+        assert not np.any(bsyn[e, n:])
+        assert es < 1e-5 and er < 1e-4 and ec < 1e-4
+    ev = c["events"][0]
+    o = orc.Oracle(m.nex, m.ney, m.nez, m.lx, m.ly, m.lz, c["rho"].ravel(), c["vp"].ravel(), c["vs"].ravel(),
+                   cij=cij.reshape(-1, 21), dt=c["dt"], nstep=nsteps[0], src_xyz=ev["sxyz"], src_mt=ev["mt"],
+                   src_f0=ev["f0"], rec_xyz=c["rxyz"], precision=64)
+    obs = o.forward(save=True)
+    ko = o.adjoint(adj[0, :nsteps[0]])
+    es = rel(bsyn[0, :nsteps[0]], obs)
+    ec = rel(bk["cijkl_kl"][0].reshape(-1, 21), ko["cijkl_kl"])
+    er = rel(bk["rho_kl"][0], ko["rho_kl"])
+    print(f"aniso batch event 0 vs FP64 oracle: seis {es:.2e}, rho_kl {er:.2e}, cijkl_kl {ec:.2e}", flush=True)
+    assert es < 1e-5 and er < 1e-4 and ec < 1e-4

@@ -177,7 +177,10 @@ def test_c2_full_mesh_short_window_matches_fp32_oracle():
 
 def test_c2_full_event_reconstruction_returns_to_rest():
     """The benchmark's C2 event at its full NSTEP: 10,000 Stacey records
-    (8.3 MB each, offsets beyond 2^31 floats) replayed backwards."""
+    (8.3 MB each, 8.3e10 bytes in all) replayed backwards.  By the end of the
+    40 s event most of the field has left through the absorbing faces, so the
+    reconstruction's end state is measured against the field while the waves
+    are inside the box (step 1,000, 4 s), not against the small final state."""
     nstep, dt = 10000, 0.004
     say(f"C2 full: 96x96x48 mesh, {nstep} steps")
     mesh = BoxMesh(96, 96, 48, LX, LY, LZ)
@@ -186,15 +189,20 @@ def test_c2_full_event_reconstruction_returns_to_rest():
     sxyz = [[rng.uniform(0.3, 0.7) * LX, rng.uniform(0.3, 0.7) * LY, -10000.0]]
     src = Sources(mesh, sxyz, [list(1e15 * rng.standard_normal(6))], [1.0])
     rec = Receivers(mesh, surface_grid(mesh, 20, 20))
+    early = ElasticRun(mesh, *model, src, rec, dt, 1000, 1, False)
+    early.iterate_time()
+    u_early = np.abs(early.fields()[0]).max()
+    early.close()
     syn, adj, k, b_end, u_fwd = gpu_event(mesh, model, src, rec, dt, nstep,
                                           lambda s: (0.5 * s).astype(np.float32))
-    rest = np.abs(b_end).max() / np.abs(u_fwd).max()
-    say(f"C2 full: max |syn| {np.abs(syn).max():.3e}, reconstruction end/forward final {rest:.2e}, "
+    rest = np.abs(b_end).max() / u_early
+    say(f"C2 full: max |syn| {np.abs(syn).max():.3e}, max |u| at step 1000 {u_early:.3e}, final {np.abs(u_fwd).max():.3e}, "
+        f"reconstruction end {np.abs(b_end).max():.3e} ({rest:.2e} of the step-1000 field), "
         + ", ".join(f"{n} max {np.abs(v).max():.3e}" for n, v in k.items()))
     assert np.abs(syn).max() > 0
     for n, v in k.items():
         assert np.isfinite(v).all() and np.abs(v).max() > 0, n
-    assert rest < 1e-2
+    assert rest < 1e-3
 
 
 # ------------------------------------------------------------------ C4
