@@ -56,6 +56,8 @@ def parse():
                         "correlation measurements of a C2-sized event (400 receivers x 3 x 10000 samples)")
     p.add_argument("--post-events", type=int, default=32, help="events summed by the post leg")
     p.add_argument("--c4-events", type=int, default=32)
+    p.add_argument("--c5-events", type=int, default=16)
+    p.add_argument("--c5-plain-events", type=int, default=2)
     p.add_argument("--c4-nstep-scale", type=float, default=1.0,
                    help="NSTEP of the c4 events is U[4000,12000] times this factor")
     p.add_argument("--c4-store", type=int, default=1, choices=[0, 1],
@@ -665,10 +667,13 @@ def precision(a):
 
 
 def aniso_smooth(a):
-    """HTI anisotropic event on the 64x64x32 mesh with full 21-component c_ij
-    kernels in fp64, then Gaussian PDE smoothing (sigma_h 5 km, sigma_v 2 km)
-    of all 21 kernels in one device run sharing the geometry reads."""
+    """C5: HTI anisotropic events on the 64x64x32 mesh, batched (E events per
+    launch, the c_ij read once per element for all of them, fp16 Stacey store),
+    each with its rho and 21 c_ij kernels in fp64, against the plain single-event
+    path on a sample of the events; then Gaussian PDE smoothing (sigma_h 5 km,
+    sigma_v 2 km) of all 21 kernels of one event in one device run."""
     from specfwat_amd import _lib
+    from specfwat_amd.batch import EventBatch, batch_device_bytes, plan_batches
     from specfwat_amd.mesh import BoxMesh
     from specfwat_amd.misfit import event_adjoint_sources
     from specfwat_amd.model import hti_model, perturbed_model
@@ -687,41 +692,93 @@ def aniso_smooth(a):
     cij_true = hti_model(mesh, *true_model, seed=77)
     cij_smooth = hti_model(mesh, *smooth_model, seed=78, max_strength=0.0)
     rng = np.random.Generator(np.random.Philox(5))
-    sxyz = [[rng.uniform(0.2, 0.8) * w["lx"], rng.uniform(0.2, 0.8) * w["ly"], -rng.uniform(2000.0, 18000.0)]]
-    src = Sources(mesh, sxyz, [list(1e15 * rng.standard_normal(6))], [rng.uniform(0.5, 1.5)])
+    srcs = []
+    for _ in range(a.c5_events):
+        sxyz = [[rng.uniform(0.2, 0.8) * w["lx"], rng.uniform(0.2, 0.8) * w["ly"], -rng.uniform(2000.0, 18000.0)]]
+        srcs.append(Sources(mesh, sxyz, [list(1e15 * rng.standard_normal(6))], [rng.uniform(0.5, 1.5)]))
     rec = Receivers(mesh, surface_grid(mesh, w["nrx"], w["nrx"]))
-    log(f"c5: mesh {mesh.nspec} el, {mesh.nglob} pts, HTI c_ij ready ({time.time() - t0:.1f}s)")
-    tuning, _ = tune_element_passes(mesh, smooth_model, src, rec, dt)
-    r = ElasticRun(mesh, *true_model, src, rec, dt, K, 1, False, cij=cij_true)
-    r.set_tuning(**tuning)
-    r.iterate_time()
-    obs = r.seismograms()
-    r.close()
-    log(f"observed data ({K} steps, HTI true model) ({time.time() - t0:.1f}s)")
-    f = ElasticRun(mesh, *smooth_model, src, rec, dt, K, 1, True, cij=cij_smooth)
-    ad = ElasticRun(mesh, *smooth_model, src, rec, dt, K, 3, False, cij=cij_smooth)
-    f.set_tuning(**tuning)
-    ad.set_tuning(**tuning)
-    _lib.call("fwat_synchronize_", f.h)
-    ts = time.time()
-    f.iterate_time(timed=True)
-    syn = f.seismograms()
-    misfit, adj = event_adjoint_sources(obs, syn, dt)
-    ad.set_adjoint_sources(adj)
-    ad.take_forward(f)
-    ad.iterate_time(timed=True)
-    _lib.call("fwat_synchronize_", ad.h)
-    wall = time.time() - ts
-    fwd_ms, fwd_n = f.force_time()
-    adj_ms, adj_n = ad.force_time()
-    a_asm = ad.assembly_time()
-    k = ad.kernels()
-    f.close(); ad.close()
-    value = mesh.nglob * K / wall
-    log(f"aniso event: {wall:.2f}s -> {value:.4e} GLL-pt-steps/s, misfit {misfit:.4e}")
+    nsteps = [K] * len(srcs)
+    log(f"c5: mesh {mesh.nspec} el, {mesh.nglob} pts, HTI c_ij ready, {len(srcs)} events of {K} steps "
+        f"({time.time() - t0:.1f}s)")
+    free_b = __import__("torch").cuda.mem_get_info()[0] if a.hbm_query else 288e9
+    # the anisotropic batch adds the 21 fp64 kernels and their fp32 partial sums per event
+    per_aniso = lambda e: batch_device_bytes(mesh, e, K, a.c4_store) + e * mesh.nspec * 125 * 21 * 12
+    fits = [e for e in (1, 2, 4, 8) if per_aniso(e) <= 0.85 * free_b and e <= len(srcs)]
+    E = a.events_per_batch or max(fits)
+    groups = plan_batches(nsteps, E)
+    log(f"c5: {E} events per batch ({len(groups)} batches; sizes fitting the HBM: {fits})")
 
-    # smoothing of the 21 c_ij kernels (diagonal map on the all-regular mesh)
-    fields = np.ascontiguousarray(np.moveaxis(k["cijkl_kl"], -1, 0), np.float32)  # [21][nspec][125]
+    obs = [None] * len(srcs)
+    for g in groups:
+        b = EventBatch(mesh, *true_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g], cij=cij_true)
+        b.set_store_precision(a.c4_store)
+        b.forward()
+        sy = b.seismograms()
+        for k, i in enumerate(g):
+            obs[i] = sy[k].copy()
+        b.close()
+    log(f"observed data ({len(srcs)} events, HTI true model) ({time.time() - t0:.1f}s)")
+
+    bwall, times, bkern, misfits = 0.0, None, {}, []
+    check = set(groups[0][:a.c5_plain_events])
+    for g in groups:
+        b = EventBatch(mesh, *smooth_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g], cij=cij_smooth)
+        b.set_store_precision(a.c4_store)
+        _lib.call("fwat_synchronize_", b.run.h)
+        ts = time.time()
+        b.forward(timed=True)
+        ft = b.pass_times()
+        syn = b.seismograms()
+        adj = np.zeros_like(syn)
+        for k, i in enumerate(g):
+            m_, adj[k] = event_adjoint_sources(obs[i], syn[k], dt)
+            misfits.append(m_)
+        b.set_adjoint_sources(adj)
+        b.adjoint(timed=True)
+        _lib.call("fwat_synchronize_", b.run.h)
+        bwall += time.time() - ts
+        at = b.pass_times()
+        times = times or (ft, at)
+        if any(i in check for i in g):
+            k_ = b.kernels()
+            for k, i in enumerate(g):
+                if i in check:
+                    bkern[i] = {q: v[k] for q, v in k_.items()}
+        b.close()
+    total = mesh.nglob * float(sum(nsteps))
+    bvalue = total / bwall
+    log(f"batched: {bwall:.2f}s -> {bvalue:.4e} GLL-pt-steps/s/event")
+
+    # plain single-event path on the first events (tuned launch shapes)
+    tuning, _ = tune_element_passes(mesh, smooth_model, srcs[0], rec, dt)
+    pwall, kerr, k_plain = 0.0, {}, None
+    for i in sorted(check):
+        f = ElasticRun(mesh, *smooth_model, srcs[i], rec, dt, K, 1, True, cij=cij_smooth)
+        ad = ElasticRun(mesh, *smooth_model, srcs[i], rec, dt, K, 3, False, cij=cij_smooth)
+        f.set_tuning(**tuning)
+        ad.set_tuning(**tuning)
+        _lib.call("fwat_synchronize_", f.h)
+        ts = time.time()
+        f.iterate_time(timed=True)
+        syn = f.seismograms()
+        _, adj = event_adjoint_sources(obs[i], syn, dt)
+        ad.set_adjoint_sources(adj)
+        ad.take_forward(f)
+        ad.iterate_time(timed=True)
+        _lib.call("fwat_synchronize_", ad.h)
+        pwall += time.time() - ts
+        fwd_ms, fwd_n = f.force_time()
+        adj_ms, adj_n = ad.force_time()
+        a_asm = ad.assembly_time()
+        k_plain = ad.kernels()
+        f.close(); ad.close()
+        for q in ("rho_kl", "cijkl_kl"):
+            kerr[q] = max(kerr.get(q, 0.0), rel_l2(bkern[i][q], k_plain[q]))
+    pvalue = mesh.nglob * float(K * len(check)) / pwall
+    log(f"plain: {pwall:.2f}s for {len(check)} events -> {pvalue:.4e}; batched/plain {bvalue / pvalue:.2f}")
+
+    # smoothing of the 21 c_ij kernels of the last plain event (diagonal map on the all-regular mesh)
+    fields = np.ascontiguousarray(np.moveaxis(k_plain["cijkl_kl"], -1, 0), np.float32)  # [21][nspec][125]
     ts = time.time()
     sm, ntstep, sm_elem_ms, sm_upd_ms = smooth_batch(mesh, fields, 5000.0, 2000.0, faithful=False, timed=True)
     sm_wall = time.time() - ts
@@ -732,38 +789,43 @@ def aniso_smooth(a):
 
     nglob, nloc = mesh.nglob, mesh.nspec * 125
     nfp = mesh.num_abs_boundary_faces * 25
-    ncontrib = nloc + nfp + 125 * (src.nsources + rec.nrec)
-    # adjoint element pass, anisotropic: ibool 4 + 21 c_ij 84 per local point, 21 c_ij kernel
-    # accumulators 21 x kacc, element-local forces of both fields 24; gathers 24 per unique point
+    ncontrib = nloc + nfp + 125 * (1 + rec.nrec)
     kacc = kernel_acc_bytes(KERNEL_BLOCK)
+    e = len(groups[0])
+    ftimes, atimes = times
+    # batched adjoint element pass: ibool 4 + 21 c_ij 84 per local point once; per event: both
+    # fields gathered 24 per unique point, forces written 24 and 21 c_ij accumulators 21 x kacc
+    # per local point
     kinds = {
-        "k_forces<2,aniso,fused-kernels>": ((112.0 + 21 * kacc) * nloc + 24.0 * nglob, adj_ms, adj_n),
-        "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp + (108.0 + kacc) * nglob,
-                                  a_asm[0], a_asm[1]),
+        f"k_batch_elem_aniso<{e},2>": (88.0 * nloc + e * (24.0 * nglob + (24.0 + 21 * kacc) * nloc),
+                                       atimes[0], atimes[1]),
+        f"k_batch_asm<{e},two-field>": (16.0 * nglob + 4.0 * (ncontrib + 125 * (e - 1))
+                                        + e * (24.0 * nloc + 24.0 * nfp + (96.0 + kacc) * nglob),
+                                        atimes[2], atimes[3]),
     }
     kname = max(kinds, key=lambda q: kinds[q][1])
     kbytes, kms, kn = kinds[kname]
     avg_ms = kms / max(kn, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9
-    # smoothing bytes per diffusion step (21 fields padded to 24 in HBM): element pass ibool 4 +
-    # jacobian 4 per local point, gather of 24 values per unique point 96, element-local results
-    # 96 per local point; gather pass CSR 4 per point + 4 per local point, results 96 per local
-    # point, rvol 4 and dat read+write 2 x 84 per unique point
     sm_elem_b = 8.0 * nloc + 96.0 * nglob + 96.0 * nloc
     sm_upd_b = 4.0 * nglob + 4.0 * nloc + 96.0 * nloc + 4.0 * nglob + 168.0 * nglob
     result = {
         "metric": "GLL-pt-steps/s/event fwd+adj",
-        "value": value,
+        "value": bvalue,
         "unit": "GLL-pt-steps/s",
-        "n_gpus": 1, "steps": K, "warmup": 0,
-        "ms_per_step": wall / K * 1e3,
+        "n_gpus": 1, "steps": int(sum(nsteps)), "warmup": 0,
+        "ms_per_step": bwall / sum(nsteps) * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": (f"C5: HTI anisotropic box {w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, "
-                                f"21 c_ij (strength U[0,4%], azimuth per 10 km block), one event, NSTEP={K}, "
-                                f"dt={dt}, {rec.nrec} receivers, 21 c_ij kernels in fp64; then PDE smoothing of "
-                                "all 21 kernels, sigma_h 5 km, sigma_v 2 km (diagonal regular-element map)"),
-                   "element_pass_tuning": tuning},
+                                f"21 c_ij (strength U[0,4%], azimuth per 10 km block), {len(srcs)} events batched "
+                                f"{E} per launch, NSTEP={K}, dt={dt}, {rec.nrec} receivers, rho + 21 c_ij kernels in "
+                                "fp64; then PDE smoothing of all 21 kernels of one event, sigma_h 5 km, sigma_v 2 km "
+                                "(diagonal regular-element map)"),
+                   "events_per_batch": E,
+                   "stacey_store": "fp16 + scale per (step, event)" if a.c4_store else "fp32"},
+        "batched_vs_plain": {"plain_value": pvalue, "speedup": bvalue / pvalue, "plain_events_timed": len(check),
+                             "kernel_rel_l2": kerr, "plain_element_pass_tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
@@ -772,12 +834,34 @@ def aniso_smooth(a):
                       "elem_avg_ms": sm_elem_ms / ntstep, "update_avg_ms": sm_upd_ms / ntstep,
                       "elem_GBps": sm_elem_b / (sm_elem_ms / ntstep * 1e-3) / 1e9,
                       "update_GBps": sm_upd_b / (sm_upd_ms / ntstep * 1e-3) / 1e9},
-        "breakdown": {"wall_s": wall, "fwd_force_avg_ms": fwd_ms / max(fwd_n, 1),
+        "breakdown": {"batched_s": bwall, "plain_s": pwall,
+                      "plain_fwd_force_avg_ms": fwd_ms / max(fwd_n, 1),
+                      "plain_adj_force_avg_ms": adj_ms / max(adj_n, 1),
+                      "plain_adj_assembly_avg_ms": a_asm[0] / max(a_asm[1], 1),
                       "kernels": {q: {"avg_ms": v[1] / max(v[2], 1),
                                       "GBps": v[0] / (v[1] / max(v[2], 1) * 1e-3) / 1e9 if v[1] > 0 else None}
                                   for q, v in kinds.items()}},
     }
+    if not a.no_cpu:
+        s0 = srcs[0]
+        result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, s0.xyz.tolist(), s0.mt.tolist(), rec,
+                                              a.cpu_steps or 3, f0=float(s0.f0[0]), label="C5 HTI event",
+                                              nstep_event=K, cij=cij_smooth)
+        result["cpu_baseline"]["smoothing"] = smoothing_cpu_baseline(mesh, fields[0], 5000.0, 2000.0, ntstep)
     print(json.dumps(result), flush=True)
+
+
+def smoothing_cpu_baseline(mesh, field, sigma_h, sigma_v, ntstep, steps=3):
+    """The smoother's loop restatement (one thread) on one field for a few
+    diffusion steps, extrapolated to ntstep steps and 21 fields."""
+    from oracle import oracle as orc
+    arrays, dmin = mesh.smooth_arrays(), mesh.distance_min_glob()
+    ts = time.time()
+    orc.smooth_pde(arrays, np.ascontiguousarray(field).ravel(), sigma_h, sigma_v, dmin, faithful=False,
+                   max_steps=steps)
+    wall = time.time() - ts
+    return {"threads": 1, "sample": f"one field, {steps} diffusion steps (set-up included)",
+            "s_per_step_per_field": wall / steps, "extrapolated_21_fields_s": wall / steps * ntstep * 21}
 
 
 def measure(a):
