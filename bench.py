@@ -74,6 +74,9 @@ def parse():
     p.add_argument("--graph", action="store_true",
                    help="c1/c2: capture each run's step loop into one HIP graph (no per-kernel timing)")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
+    p.add_argument("--force-irregular", action="store_true",
+                   help="c1/c2: every element stores its inverse Jacobian per point (the general-hexahedron "
+                        "layout of meshes with topography), 40 B per local point and pass")
     p.add_argument("--no-tune", action="store_true", help="skip the element-pass launch-shape autotuner")
     p.add_argument("--fk", type=int, default=0, metavar="R",
                    help="c2: inject a synthetic incident field (FK coupling) through the Stacey faces of the "
@@ -135,8 +138,9 @@ def main():
     K, W = a.steps or w["nstep"], a.warmup
     cpu_steps = a.cpu_steps or (K if a.config == "c1" else 6)
     t0 = time.time()
-    mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
-    log(f"mesh {mesh.nspec} elements, {mesh.nglob} points ({time.time() - t0:.1f}s)")
+    mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], force_irregular=a.force_irregular)
+    log(f"mesh {mesh.nspec} elements, {mesh.nglob} points, {mesh.nspec_irregular} with stored Jacobians "
+        f"({time.time() - t0:.1f}s)")
     true_model = perturbed_model(mesh, seed=2024)
     smooth_model = perturbed_model(mesh, smooth=True)
     rng = np.random.Generator(np.random.Philox(7))
@@ -341,6 +345,8 @@ def main():
         "dtype": "f32" if a.static_precision == 0 else ("f32+bf16-statics" if a.static_precision == 1 else "f32+fp16-statics"),
         "data": "synthetic",
         "config": {"workload": (f"{'C2' if a.config == 'c2' else 'C1'} isotropic box "
+                                + ("(every element with stored per-point Jacobians) " if a.force_irregular else "")
+                                + 
                                 f"{w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, NGLL=5, "
                                 f"{rec.nrec} receivers, dt={dt}, NSTEP={K}, fwd + waveform misfit (adjoint sources "
                                 "band-passed 0.4-10 s) + adjoint/reconstruction + rho/mu/kappa kernels "
@@ -373,6 +379,7 @@ def main():
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, mesh, smooth_model, sxyz, mt, rec, cpu_steps,
                                               label=("C1 event" if a.config == "c1" else "C2 event"),
+                                              force_irregular=a.force_irregular,
                                               nstep_event=w["nstep"],
                                               one_thread_steps=min(cpu_steps, 20 if a.config == "c1" else 1))
     print(json.dumps(result), flush=True)
@@ -1170,7 +1177,7 @@ def host_info():
 
 
 def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event", nstep_event=None, cij=None,
-                 one_thread_steps=1, repeats=3):
+                 one_thread_steps=1, repeats=3, force_irregular=False):
     """The oracle (C++/OpenMP FP32 restatement of the SPECFEM CPU time loop) on the
     host cores, on a bounded sample of the same event: nsteps forward steps with the
     Stacey store, then nsteps adjoint + reconstruction + kernel steps, each timed
@@ -1186,7 +1193,8 @@ def cpu_baseline(w, mesh, model, sxyz, mt, rec, nsteps, f0=1.0, label="C2 event"
     def make(n, nthreads):
         return orc.Oracle(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"], rho.ravel(), vp.ravel(),
                           vs.ravel(), cij=None if cij is None else cij.reshape(-1, 21), dt=w["dt"], nstep=n,
-                          src_xyz=sxyz, src_mt=mt, src_f0=[f0], rec_xyz=rxyz, precision=32, nthreads=nthreads)
+                          src_xyz=sxyz, src_mt=mt, src_f0=[f0], rec_xyz=rxyz, precision=32, nthreads=nthreads,
+                          force_irregular=force_irregular)
 
     def timed_run(o):
         ts = time.time()
