@@ -87,6 +87,8 @@ def parse():
     p.add_argument("--window-start", type=int, default=-1, metavar="S0",
                    help="the timed event covers steps S0+1..S0+K, the state at S0 from an untimed forward run "
                         "(default: 0 for K >= 2000, else the first P arrival at the nearest receiver + 50 steps)")
+    p.add_argument("--copy-sweep", action="store_true",
+                   help="with --copy-probe: also the copy with 1-8 loads in flight per lane and several grid sizes")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     return p.parse_args()
@@ -114,8 +116,18 @@ def main():
     if a.copy_probe:
         ms = np.zeros(1)
         _lib.call("fwat_copy_probe_", a.copy_probe, 5, ms)
-        print(json.dumps({"copy_probe_bytes": a.copy_probe, "avg_ms": float(ms[0]),
-                          "GBps_read_plus_write": 2 * a.copy_probe / (ms[0] * 1e-3) / 1e9}), flush=True)
+        out = {"copy_probe_bytes": a.copy_probe, "avg_ms": float(ms[0]),
+               "GBps_read_plus_write": 2 * a.copy_probe / (ms[0] * 1e-3) / 1e9}
+        if a.copy_sweep:
+            sweep = []
+            for u in (1, 2, 4, 8):
+                for bpc in (0, 4, 8, 16, 32):
+                    _lib.call("fwat_copy_probe_unrolled_", a.copy_probe, 5, u, bpc, ms)
+                    sweep.append({"unroll": u, "blocks_per_cu": bpc, "avg_ms": float(ms[0]),
+                                  "GBps": 2 * a.copy_probe / (ms[0] * 1e-3) / 1e9})
+            out["sweep"] = sweep
+            out["best_GBps"] = max(r["GBps"] for r in sweep)
+        print(json.dumps(out), flush=True)
         return
     if a.config == "c4":
         return batched(a)

@@ -145,6 +145,9 @@ void fwat_last_assembly_time_(long *Mesh_pointer, double *ms, int *launches);
 void fwat_synchronize_(long *Mesh_pointer);
 /* plain device copy of nbytes repeated reps times (counter calibration probe) */
 void fwat_copy_probe_(long *nbytes, int *reps, double *avg_ms);
+/* the same copy with unroll (1, 2, 4, 8) float4 loads in flight per lane on ncu * blocks_per_cu
+ * workgroups (0: one float4 per lane over the array): what a streaming kernel reaches here */
+void fwat_copy_probe_unrolled_(long *nbytes, int *reps, int *unroll, int *blocks_per_cu, double *avg_ms);
 /* in-HBM replacements of save_forward_arrays.bin and absorb_field.bin */
 void fwat_copy_forward_to_backward_(long *Mesh_pointer_from, long *Mesh_pointer_to);
 void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);

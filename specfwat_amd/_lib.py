@@ -53,6 +53,7 @@ SIGNATURES = {
     "fwat_last_assembly_time_": "LDI",
     "fwat_synchronize_": "L",
     "fwat_copy_probe_": "liD",
+    "fwat_copy_probe_unrolled_": "liiiD",
     "fwat_copy_forward_to_backward_": "LL",
     "fwat_move_stacey_store_": "LL",
     "fwat_set_static_precision_": "Li",

@@ -323,6 +323,22 @@ __global__ void k_copy4(const float4* __restrict__ a, float4* __restrict__ b, si
     b[i] = a[i];
 }
 
+// The same copy with U float4 loads issued before the stores (U in flight per
+// lane), lanes of a workgroup on contiguous 16-byte slots of each of U chunks.
+template <int U>
+__global__ void k_copy4u(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n4; i += U * stride) {
+    float4 v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = a[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < U; ++k) b[i + k * stride] = v[k];
+  }
+  for (; i < n4; i += stride) b[i] = a[i];
+}
+
 __global__ void k_expand_glob(int nspec, const int* __restrict__ ibool, const double* __restrict__ g,
                               double scale_sign, const realw* __restrict__ rho, double* __restrict__ out) {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1073,6 +1089,51 @@ void fwat_copy_probe_(long* nbytes, int* reps, double* avg_ms) {
   for (int r = 0; r < *reps; ++r) hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, 0, a, b, n4);
   HIP_CHECK(hipEventRecord(e1, 0));
   HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / *reps;
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipFree(a));
+  HIP_CHECK(hipFree(b));
+}
+
+// Copy-rate probe: the plain copy of nbytes with unroll float4 loads in flight
+// per lane (1, 2, 4 or 8) on a grid of ncu * blocks_per_cu workgroups of 256
+// (0: one float4 per lane over the whole array), repeated reps times; avg_ms
+// is the mean duration.  Measures what a simple streaming kernel reaches on
+// this device, beside the nominal HBM peak.
+void fwat_copy_probe_unrolled_(long* nbytes, int* reps, int* unroll, int* blocks_per_cu, double* avg_ms) {
+  const size_t n4 = (size_t)*nbytes / 16;
+  if (n4 == 0 || *reps <= 0) SEM_FAIL("fwat_copy_probe_unrolled_: empty probe");
+  if (*unroll != 1 && *unroll != 2 && *unroll != 4 && *unroll != 8) SEM_FAIL("unroll must be 1, 2, 4 or 8");
+  int dev = 0, ncu = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  float4 *a = nullptr, *b = nullptr;
+  HIP_CHECK(hipMalloc((void**)&a, n4 * 16));
+  HIP_CHECK(hipMalloc((void**)&b, n4 * 16));
+  hipLaunchKernelGGL(k_copy4, dim3((unsigned)std::min<size_t>((n4 + 255) / 256, 1u << 20)), dim3(256), 0, 0,
+                     b, a, n4);  // touch both arrays
+  const int blocks = *blocks_per_cu > 0 ? ncu * *blocks_per_cu
+                                        : (int)std::min<size_t>((n4 / *unroll + 255) / 256, 1u << 20);
+  const auto launch = [&]() {
+    switch (*unroll) {
+      case 1: hipLaunchKernelGGL(k_copy4u<1>, dim3(blocks), dim3(256), 0, 0, a, b, n4); break;
+      case 2: hipLaunchKernelGGL(k_copy4u<2>, dim3(blocks), dim3(256), 0, 0, a, b, n4); break;
+      case 4: hipLaunchKernelGGL(k_copy4u<4>, dim3(blocks), dim3(256), 0, 0, a, b, n4); break;
+      default: hipLaunchKernelGGL(k_copy4u<8>, dim3(blocks), dim3(256), 0, 0, a, b, n4); break;
+    }
+  };
+  launch();
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, 0));
+  for (int r = 0; r < *reps; ++r) launch();
+  HIP_CHECK(hipEventRecord(e1, 0));
+  HIP_CHECK(hipEventSynchronize(e1));
+  HIP_CHECK(hipGetLastError());
   float ms = 0;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *avg_ms = ms / *reps;
