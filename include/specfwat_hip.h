@@ -153,7 +153,8 @@ void fwat_copy_forward_to_backward_(long *Mesh_pointer_from, long *Mesh_pointer_
 void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);
 /* element-pass launch shape (autotuner knobs): regular_path 1 = folded-table kernel on
  * all-regular meshes, 2 = its software-pipelined form (next element's gathers in flight during
- * the current one's contractions; isotropic); blocks_per_cu_* = persistent workgroups per CU,
+ * the current one's contractions; isotropic), 3 = the pipelined form with two barriers per
+ * element (both fields through ping-pong LDS buffers); blocks_per_cu_* = persistent workgroups per CU,
  * 0 = one per element */
 void fwat_set_tuning_(long *Mesh_pointer, int *regular_path, int *blocks_per_cu_fwd, int *blocks_per_cu_adj);
 /* Element lattice of the mesh: element (ix,iy,iz) = ix + nex*(iy + ney*iz).  Checked against

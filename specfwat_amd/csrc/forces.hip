@@ -632,6 +632,177 @@ __device__ __forceinline__ void forces_reg_pipe_body(const ForceArgs& A) {
   }
 }
 
+// The pipelined pass with two barriers per element instead of three per
+// field: both fields' displacements go to LDS at once, both fields' stresses
+// after them, and consecutive elements alternate between two LDS buffers, so
+// an element's writes never wait for the previous element's readers (those
+// finished before the previous element's second barrier).  Arithmetic and
+// results are forces_reg_pipe_body's.
+template <int NF, int KMODE, int SP>
+__device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
+  __shared__ realw s_tab[128];
+  __shared__ realw s_u[2][NF][3][128];
+  __shared__ realw s_s[2][NF][6][128];
+  const int t = threadIdx.x;
+  if (t < 125) s_tab[t] = A.tables[t];
+  const bool act = t < 125;
+  const int tt = act ? t : 124;
+  const int I = tt % 5, J = (tt / 5) % 5, K = tt / 25;
+  __syncthreads();
+  const realw xr = A.xix_regular;
+  const realw jx = A.jacobian_regular * A.xix_regular;
+  realw hpI[5], hpJ[5], hpK[5], hwI[5], hwJ[5], hwK[5];
+  {
+    const realw f1 = -jx * s_tab[50 + J + 5 * K];
+    const realw f2 = -jx * s_tab[75 + I + 5 * K];
+    const realw f3 = -jx * s_tab[100 + I + 5 * J];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      hpI[l] = s_tab[I + 5 * l] * xr;
+      hpJ[l] = s_tab[J + 5 * l] * xr;
+      hpK[l] = s_tab[K + 5 * l] * xr;
+      hwI[l] = s_tab[25 + l + 5 * I] * f1;
+      hwJ[l] = s_tab[25 + l + 5 * J] * f2;
+      hwK[l] = s_tab[25 + l + 5 * K] * f3;
+    }
+  }
+  const int La0 = 5 * J + 25 * K, Lb0 = I + 25 * K, Lc0 = I + 5 * J;
+  const int nblk = gridDim.x;
+  const int x = blockIdx.x & 7, per_xcd = nblk >> 3, slot = blockIdx.x >> 3;
+  const int chunk = (A.elem_count + 7) >> 3;
+  const int e_end = min(A.elem_count, (x + 1) * chunk);
+  const bool flush = A.kflush != 0;
+  const auto load_moduli = [&](size_t Lg, int e, realw& kl, realw& ml) {
+    if (SP == SEM_STATIC_FP32) {
+      kl = A.kappa[Lg];
+      ml = A.mu[Lg];
+    } else {
+      const uint32_t pk = reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg];
+      const float2 kb = reinterpret_cast<const float2*>(A.kmu_base)[e];
+      kl = kb.x + lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
+      ml = kb.y + lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
+    }
+  };
+  int le = x * chunk + slot;
+  realw un[NF][3];
+  realw kn = 0, mn = 0;
+  int ig_next = 0;
+  if (le < e_end) {
+    const int e = A.elem_begin + le;
+    const size_t Lg = (size_t)e * 125 + tt;
+    const int ig = A.ibool[Lg];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const realw* U = f == 0 ? A.u0 : A.u1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) un[f][c] = U[3 * (size_t)ig + c];
+    }
+    load_moduli(Lg, e, kn, mn);
+    if (le + per_xcd < e_end) ig_next = A.ibool[(size_t)(e + per_xcd) * 125 + tt];
+  }
+  int b = 0;
+  for (; le < e_end; le += per_xcd, b ^= 1) {
+    const int e = A.elem_begin + le;
+    const size_t Lg = (size_t)e * 125 + tt;
+    if (act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        s_u[b][f][0][t] = un[f][0];
+        s_u[b][f][1][t] = un[f][1];
+        s_u[b][f][2][t] = un[f][2];
+      }
+    }
+    const realw kl = kn, ml = mn;
+    if (le + per_xcd < e_end) {
+      const int e1 = e + per_xcd;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const realw* U = f == 0 ? A.u0 : A.u1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) un[f][c] = U[3 * (size_t)ig_next + c];
+      }
+      load_moduli((size_t)e1 * 125 + tt, e1, kn, mn);
+      if (le + 2 * per_xcd < e_end) ig_next = A.ibool[(size_t)(e1 + per_xcd) * 125 + tt];
+    }
+    float pm = 0.0f, pk = 0.0f;
+    if (KMODE == 2 && A.mu_part && act) {
+      pm = A.mu_part[Lg];
+      pk = A.kappa_part[Lg];
+    }
+    __syncthreads();  // displacements of both fields in buffer b
+    realw ea[6];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      realw duxdx = 0, duydx = 0, duzdx = 0, duxdy = 0, duydy = 0, duzdy = 0, duxdz = 0, duydz = 0, duzdz = 0;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        const int La = La0 + l, Lb = Lb0 + 5 * l, Lc = Lc0 + 25 * l;
+        duxdx += s_u[b][f][0][La] * hpI[l]; duydx += s_u[b][f][1][La] * hpI[l]; duzdx += s_u[b][f][2][La] * hpI[l];
+        duxdy += s_u[b][f][0][Lb] * hpJ[l]; duydy += s_u[b][f][1][Lb] * hpJ[l]; duzdy += s_u[b][f][2][Lb] * hpJ[l];
+        duxdz += s_u[b][f][0][Lc] * hpK[l]; duydz += s_u[b][f][1][Lc] * hpK[l]; duzdz += s_u[b][f][2][Lc] * hpK[l];
+      }
+      const realw exy2 = duxdy + duydx, exz2 = duzdx + duxdz, eyz2 = duzdy + duydz;
+      if (KMODE == 2) {
+        if (f == 0) {
+          ea[0] = duxdx; ea[1] = duydy; ea[2] = duzdz;
+          ea[3] = 0.5f * exy2; ea[4] = 0.5f * exz2; ea[5] = 0.5f * eyz2;
+        } else if (act) {
+          const realw eb0 = duxdx, eb1 = duydy, eb2 = duzdz;
+          const realw eb3 = 0.5f * exy2, eb4 = 0.5f * exz2, eb5 = 0.5f * eyz2;
+          const realw tra = ea[0] + ea[1] + ea[2], trb = eb0 + eb1 + eb2;
+          const realw t3a = tra / 3.0f, t3b = trb / 3.0f;
+          const realw prod = (ea[0] - t3a) * (eb0 - t3b) + (ea[1] - t3a) * (eb1 - t3b) +
+                             (ea[2] - t3a) * (eb2 - t3b) + 2.0f * (ea[3] * eb3 + ea[4] * eb4 + ea[5] * eb5);
+          const realw kx = tra * trb;
+          if (!A.mu_part) {
+            kernel_acc(A.mu_kl, nullptr, Lg, prod, A.dt, true);
+            kernel_acc(A.kappa_kl, nullptr, Lg, kx, A.dt, true);
+          } else if (flush) {
+            A.mu_kl[Lg] += A.dt * (double)(pm + prod);
+            A.kappa_kl[Lg] += A.dt * (double)(pk + kx);
+            A.mu_part[Lg] = 0.0f;
+            A.kappa_part[Lg] = 0.0f;
+          } else {
+            A.mu_part[Lg] = pm + prod;
+            A.kappa_part[Lg] = pk + kx;
+          }
+        }
+      }
+      const realw lam = kl - (2.0f / 3.0f) * ml;
+      const realw m2 = 2.0f * ml;
+      const realw ltr = lam * (duxdx + duydy + duzdz);
+      if (act) {
+        s_s[b][f][0][t] = ltr + m2 * duxdx; s_s[b][f][1][t] = ltr + m2 * duydy; s_s[b][f][2][t] = ltr + m2 * duzdz;
+        s_s[b][f][3][t] = ml * exy2; s_s[b][f][4][t] = ml * exz2; s_s[b][f][5][t] = ml * eyz2;
+      }
+    }
+    __syncthreads();  // stresses of both fields in buffer b
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      realw fx = 0, fy = 0, fz = 0;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        const int La = La0 + l, Lb = Lb0 + 5 * l, Lc = Lc0 + 25 * l;
+        fx += s_s[b][f][0][La] * hwI[l]; fy += s_s[b][f][3][La] * hwI[l]; fz += s_s[b][f][4][La] * hwI[l];
+        fx += s_s[b][f][3][Lb] * hwJ[l]; fy += s_s[b][f][1][Lb] * hwJ[l]; fz += s_s[b][f][5][Lb] * hwJ[l];
+        fx += s_s[b][f][4][Lc] * hwK[l]; fy += s_s[b][f][5][Lc] * hwK[l]; fz += s_s[b][f][2][Lc] * hwK[l];
+      }
+      if (act) {
+        realw* F = (f == 0 ? A.floc0 : A.floc1) + 3 * Lg;
+        F[0] = fx;
+        F[1] = fy;
+        F[2] = fz;
+      }
+    }
+  }
+}
+
+template <int NF, int KMODE, int SP>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : 5))) void k_forces_reg_pp(
+    ForceArgs A) {
+  forces_reg_pp_body<NF, KMODE, SP>(A);
+}
+
 // Two fields hold 6 more registers of look-ahead: 4 waves per SIMD (128 VGPRs)
 // keeps them out of scratch.
 template <int NF, int KMODE, int SP>
@@ -654,7 +825,7 @@ __global__ __launch_bounds__(128) void k_forces_reg_aniso(ForceArgs A) {
 }
 
 template <int NF, bool ANISO, int KMODE, int SP>
-void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, bool pipelined = false) {
+void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, int variant = 1) {
   int dev = 0, ncu = 0;
   HIP_CHECK(hipGetDevice(&dev));
   HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -663,7 +834,8 @@ void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, bool pipel
   int nb = blocks_per_cu > 0 ? std::min(all, (ncu * blocks_per_cu + 7) & ~7) : all;
   nb = std::max(8, nb);
   if constexpr (ANISO) hipLaunchKernelGGL((k_forces_reg_aniso<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
-  else if (pipelined) hipLaunchKernelGGL((k_forces_reg_pipe<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else if (variant == 2) hipLaunchKernelGGL((k_forces_reg_pipe<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else if (variant == 3) hipLaunchKernelGGL((k_forces_reg_pp<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
   else hipLaunchKernelGGL((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
 }
 
@@ -740,8 +912,9 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   const bool an = m->anisotropy != 0;
   hipStream_t s = m->stream;
   if (gather && m->nspec_irregular == 0 && m->regular_fast_path && kernel_mode != 1) {
-    // regular_fast_path 2: the software-pipelined isotropic pass
-    const bool pipe = m->regular_fast_path == 2;
+    // regular_fast_path 2: the software-pipelined isotropic pass, 3: the same with two
+    // barriers per element (ping-pong LDS buffers)
+    const int pipe = m->regular_fast_path;
     const int bpc = nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd;
     if (nfields == 2) {
       if (an ? !m->d_cijkl_kl : (!m->d_mu_kl || !m->d_kappa_kl))

@@ -1153,14 +1153,15 @@ void fwat_set_graph_(long* Mesh_pointer, int* on) {
 
 // Launch-shape knobs of the element passes (set by the autotuner in
 // specfwat_amd/tune.py): regular_path 1 = folded-table kernel on all-regular
-// meshes, 2 = its software-pipelined form (isotropic; anisotropic passes keep
-// 1), 0 = general kernel; blocks_per_cu_* = persistent workgroups per CU
+// meshes, 2 = its software-pipelined form, 3 = the pipelined form with two
+// barriers per element (isotropic; anisotropic passes keep 1), 0 = general
+// kernel; blocks_per_cu_* = persistent workgroups per CU
 // for the one-field and two-field passes, 0 = one workgroup per element.
 void fwat_set_tuning_(long* Mesh_pointer, int* regular_path, int* blocks_per_cu_fwd, int* blocks_per_cu_adj) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (*blocks_per_cu_fwd < 0 || *blocks_per_cu_adj < 0 || *blocks_per_cu_fwd > 64 || *blocks_per_cu_adj > 64)
     SEM_FAIL("fwat_set_tuning_: blocks per CU must be in 0..64");
-  if (*regular_path < 0 || *regular_path > 2) SEM_FAIL("fwat_set_tuning_: regular_path must be 0, 1 or 2");
+  if (*regular_path < 0 || *regular_path > 3) SEM_FAIL("fwat_set_tuning_: regular_path must be 0, 1, 2 or 3");
   m->regular_fast_path = *regular_path;
   m->blocks_per_cu_fwd = *blocks_per_cu_fwd;
   m->blocks_per_cu_adj = *blocks_per_cu_adj;

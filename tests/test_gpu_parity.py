@@ -255,7 +255,7 @@ def test_element_pass_launch_shapes_agree():
     ko = o.adjoint(adj)
     m = c["mesh"]
     results = []
-    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8), (2, 0), (2, 1), (2, 8)):
+    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8), (2, 0), (2, 1), (2, 8), (3, 0), (3, 1), (3, 8)):
         print(f"launch shape: regular_path {reg}, {bpc} workgroups per CU", flush=True)
         src = Sources(m, c["sxyz"], c["mt"], c["f0"])
         rec = Receivers(m, c["rxyz"])
@@ -280,7 +280,8 @@ def test_element_pass_launch_shapes_agree():
 
 
 def test_pipelined_pass_with_several_elements_per_workgroup():
-    """The software-pipelined element pass (regular_path 2) walks several
+    """The software-pipelined element passes (regular_path 2, and 3 with two
+    barriers per element and ping-pong LDS buffers) walk several
     elements per workgroup when workgroups per CU x CUs < elements: 2,048
     elements on 256 or 512 workgroups (8 or 4 each, the last ones ragged),
     against the plain folded-table pass and the FP64 oracle.  300 steps: the
@@ -293,7 +294,7 @@ def test_pipelined_pass_with_several_elements_per_workgroup():
     assert np.abs(obs).max() > 1e-5 and np.abs(ko["mu_kl"]).max() > 1e-20  # far above FP32 underflow
     m = c["mesh"]
     out = []
-    for reg, bpc in ((1, 1), (2, 1), (2, 2), (2, 3)):
+    for reg, bpc in ((1, 1), (2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3)):
         print(f"launch shape: regular_path {reg}, {bpc} workgroups per CU", flush=True)
         src = Sources(m, c["sxyz"], c["mt"], c["f0"])
         rec = Receivers(m, c["rxyz"])
