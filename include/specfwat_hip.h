@@ -362,6 +362,18 @@ void fwat_waveform_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *ob
 /* device time (ms) of the last fwat_waveform_adjoint_batch_, fwat_cc_adjoint_batch_ or
  * fwat_mt_adjoint_batch_, copies excluded */
 void fwat_waveform_adjoint_last_time_(double *ms);
+/* No counterpart in the reference: fwat_waveform_adjoint_batch_ for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  obs, syn (in) and adj, misfits, residuals (out; residuals may be NULL) are the
+ * caller's device buffers, windows and sos host arrays as above, *stream a hipStream_t (0: null
+ * stream).  Intermediates live in a static per-GPU scratch of 3.1 MB: ntrace <= 16384,
+ * nwin <= 8, nsect <= 8, window lengths summed <= 65536 samples, filter window <= 65536 samples;
+ * larger inputs are refused with a message.  Returns when the outputs are written; calls that
+ * overlap on one GPU share the scratch and race. */
+void fwat_waveform_adjoint_batch_noalloc_(int *nstep, int *ntrace, double *dt, realw *d_obs, realw *d_syn,
+                                          int *nwin, double *windows, double *taper_percentage,
+                                          int *itaper_type, int *nsect, realw *sos, double *d_misfits,
+                                          double *d_residuals, realw *d_adj, long *stream);
 /* cross-correlation traveltime (imeas 11) / amplitude (12) measurement of every trace: CCTTMisfit%
  * calc_adjoint_source (src/preproc/adjoint_sources/cc_tt_misfit.f90:17-146 with utils/cc.f90:10-160),
  * or the zero-lag correlation coefficient (imeas 5, CCCMisfit, ccc_misfit.f90:15-71), and filter_adj; cfg(7) = dt_sigma_min, dlna_sigma_min, TSHIFT_MIN, TSHIFT_MAX, DLNA_MIN, DLNA_MAX,

@@ -105,6 +105,7 @@ SIGNATURES = {
     "fwat_opt_last_time_": "d",
     "fwat_waveform_adjoint_batch_": "iidFFiDdiiFDDF",
     "fwat_waveform_adjoint_last_time_": "d",
+    "fwat_waveform_adjoint_batch_noalloc_": "iidPPiDdiiFPPPl",
     "fwat_cc_adjoint_batch_": "iidFFiDdiiDiFDDF",
     "fwat_mt_adjoint_batch_": "iidFFiDdiiDiFDDF",
     "fwat_ep_adjoint_batch_": "iidFFiDdidiFDDF",

@@ -14,9 +14,12 @@
 // weights are the same for every trace and come precomputed in double.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "sem_device.h"
+
+#include <mutex>
 
 namespace {
 
@@ -1622,8 +1625,16 @@ __global__ void k_fill_f32m(size_t n, float* __restrict__ x) {
 double g_meas_ms = 0.0;
 
 // window_taper's weights (signal.f90:419-491) for a window of npts samples
+void taper_weights_into(int npts, double pct, int type, double* w);
+
 void taper_weights(int npts, double pct, int type, std::vector<double>& w) {
   w.assign(npts, 1.0);
+  taper_weights_into(npts, pct, type, w.data());
+}
+
+// window_taper's weights (signal.f90:419-491) written into w[0..npts)
+void taper_weights_into(int npts, double pct, int type, double* w) {
+  for (int i = 0; i < npts; ++i) w[i] = 1.0;
   const int frac = (pct == 0.0 || pct == 1.0) ? (int)(npts * pct / 2.0) : (int)(npts * pct / 2.0 + 0.5);
   if (frac <= 0) return;
   const double pi = 3.14159265358979323846;
@@ -1732,6 +1743,61 @@ void dpss_tapers(int n, double half_nbw, int kmax, std::vector<double>& out) {
     }
   }
 }
+// per trace: its window misfits summed in window order (the host loop of
+// fwat_waveform_adjoint_batch_), residuals (ntrace, nwin) when asked for
+__global__ void k_trace_misfits(int ntrace, int nwin, const double* __restrict__ mr, double* __restrict__ misfits,
+                                double* __restrict__ residuals) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrace) return;
+  const size_t T = (size_t)ntrace;
+  double m = 0.0;
+  for (int w = 0; w < nwin; ++w) {
+    m += mr[(size_t)w * T + t];
+    if (residuals) residuals[(size_t)t * nwin + w] = mr[(size_t)nwin * T + (size_t)w * T + t];
+  }
+  misfits[t] = m;
+}
+
+// ---- scratch of the allocation-free waveform entry point
+// Limits of fwat_waveform_adjoint_batch_noalloc_ and the layout of its
+// device scratch (doubles): taper weights of the windows [NA_MAX_WEIGHTS],
+// taper weights of the filter window [NA_MAX_WEIGHTS], window misfits and
+// residuals [2][NA_MAX_WIN][NA_MAX_TRACES], filter sections (6 floats each).
+constexpr int NA_MAX_TRACES = 16384;
+constexpr int NA_MAX_WIN = 8;
+constexpr int NA_MAX_WEIGHTS = 65536;
+constexpr int NA_MAX_SECT = 8;
+constexpr size_t NA_OFF_FW = NA_MAX_WEIGHTS;
+constexpr size_t NA_OFF_MR = 2 * (size_t)NA_MAX_WEIGHTS;
+constexpr size_t NA_OFF_SOS = NA_OFF_MR + 2 * (size_t)NA_MAX_WIN * NA_MAX_TRACES;
+constexpr size_t NA_SCRATCH_DOUBLES = NA_OFF_SOS + 3 * (size_t)NA_MAX_SECT;  // 3.1 MB
+
+// One copy per GPU the library runs on, held until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device array: the
+// device address comes from hipGetSymbolAddress (na_scratch_device).
+static __device__ __align__(256) double scratch[NA_SCRATCH_DOUBLES];
+
+constexpr int NA_MAX_DEVICES = 64;
+double* g_na_scratch[NA_MAX_DEVICES] = {};
+std::mutex g_na_mutex;
+// host staging of the weights and filter sections copied into the scratch
+double g_na_stage[NA_OFF_MR + 3 * NA_MAX_SECT];
+
+// Device address of this GPU's scratch, looked up once per device with that
+// device current (the first call on the device).
+double* na_scratch_device() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEVICES) SEM_FAIL("device %d outside the %d the scratch table holds", dev, NA_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  if (!g_na_scratch[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_scratch[dev] = static_cast<double*>(p);
+  }
+  return g_na_scratch[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1811,6 +1877,74 @@ void fwat_waveform_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* ob
 }
 
 void fwat_waveform_adjoint_last_time_(double* ms) { *ms = g_meas_ms; }
+
+// fwat_waveform_adjoint_batch_ for callers that cannot allocate device memory
+// (code under HIP graph capture, serving loops): the same measurement with
+// obs, syn (inputs), adj, misfits and residuals (outputs) in the caller's
+// device memory, on the caller's stream (*stream: a hipStream_t, 0 = the null
+// stream), and the intermediate arrays in this library's per-GPU static
+// scratch -- no device allocation or release.  windows and sos are host arrays
+// as in fwat_waveform_adjoint_batch_; misfits (ntrace) and residuals
+// (nwin, ntrace, or NULL) are device doubles.  Limits: ntrace <= 16384, nwin
+// <= 8, nsect <= 8, window lengths summed <= 65536 samples and the filter
+// window <= 65536 samples.  Returns when the outputs are written.  Calls that
+// overlap on one GPU share its scratch and race.
+void fwat_waveform_adjoint_batch_noalloc_(int* nstep, int* ntrace, double* dt, realw* d_obs, realw* d_syn, int* nwin,
+                                          double* windows, double* taper_percentage, int* itaper_type, int* nsect,
+                                          realw* sos, double* d_misfits, double* d_residuals, realw* d_adj,
+                                          long* stream) {
+  const int N = *nstep, T = *ntrace, nw = *nwin;
+  if (N <= 0 || T <= 0) SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: sizes must be positive");
+  if (T > NA_MAX_TRACES) SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: %d traces, at most %d", T, NA_MAX_TRACES);
+  if (nw < 1 || nw > NA_MAX_WIN) SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: 1..%d windows", NA_MAX_WIN);
+  if (*nsect < 0 || *nsect > NA_MAX_SECT) SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: 0..%d filter sections", NA_MAX_SECT);
+  if (*taper_percentage < 0.0 || *taper_percentage > 1.0) SEM_FAIL("taper_percentage must be 0 < %% < 1");
+  if (!d_obs || !d_syn || !d_adj || !d_misfits) SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: null device buffer");
+  double* sc = na_scratch_device();
+  hipStream_t s = reinterpret_cast<hipStream_t>(*stream);
+  WinInfo W{};
+  int nwts = 0;
+  double wmin = windows[0], wmax = windows[0];
+  for (int w = 0; w < nw; ++w) {
+    window_range(windows[2 * w], windows[2 * w + 1], *dt, N, W.nb[w], W.nl[w]);
+    if (nwts + W.nl[w] > NA_MAX_WEIGHTS)
+      SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: windows hold more than %d samples", NA_MAX_WEIGHTS);
+    W.woff[w] = nwts;
+    taper_weights_into(W.nl[w], *taper_percentage, *itaper_type, g_na_stage + nwts);
+    nwts += W.nl[w];
+    wmin = std::min(wmin, std::min(windows[2 * w], windows[2 * w + 1]));
+    wmax = std::max(wmax, std::max(windows[2 * w], windows[2 * w + 1]));
+  }
+  int fnb = 0, fnl = 0;
+  if (*nsect > 0) {
+    window_range(wmin, wmax, *dt, N, fnb, fnl);
+    if (fnl > NA_MAX_WEIGHTS)
+      SEM_FAIL("fwat_waveform_adjoint_batch_noalloc_: filter window of %d samples, at most %d", fnl, NA_MAX_WEIGHTS);
+    taper_weights_into(fnl, *taper_percentage, *itaper_type, g_na_stage + NA_OFF_FW);
+    std::memcpy(g_na_stage + NA_OFF_MR, sos, 6 * sizeof(float) * (size_t)*nsect);
+  }
+  double* d_wts = sc;
+  double* d_fw = sc + NA_OFF_FW;
+  double* d_mr = sc + NA_OFF_MR;
+  float* d_sos = reinterpret_cast<float*>(sc + NA_OFF_SOS);
+  HIP_CHECK(hipMemcpyAsync(d_wts, g_na_stage, (size_t)nwts * sizeof(double), hipMemcpyHostToDevice, s));
+  if (*nsect > 0) {
+    HIP_CHECK(hipMemcpyAsync(d_fw, g_na_stage + NA_OFF_FW, (size_t)fnl * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_sos, g_na_stage + NA_OFF_MR, 6 * sizeof(float) * (size_t)*nsect, hipMemcpyHostToDevice, s));
+  }
+  const unsigned nb = (unsigned)((T + MTPB - 1) / MTPB);
+  hipLaunchKernelGGL(k_waveform_adj, dim3(nb), dim3(MTPB), 0, s, N, T, *dt, nw, W, d_wts, d_obs, d_syn, d_adj, d_mr,
+                     d_mr + (size_t)nw * T);
+  if (*nsect > 0) {
+    const size_t n = (size_t)N * T;
+    launch_sos(s, N, T, *nsect, d_sos, d_adj);
+    hipLaunchKernelGGL(k_window_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, T, fnb, fnl, d_fw,
+                       d_adj);
+  }
+  hipLaunchKernelGGL(k_trace_misfits, dim3(nb), dim3(MTPB), 0, s, T, nw, d_mr, d_misfits, d_residuals);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
 
 // cc_tt_misfit.f90 CCTTMisfit%calc_adjoint_source for every trace: imeas 11
 // (traveltime) or 12 (amplitude), or ccc_misfit.f90 CCCMisfit (imeas 5,
