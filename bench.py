@@ -34,6 +34,16 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_BLOCK = 16
 
 
+def element_kernel_name(tuning, nirr):
+    """Name of the element-pass kernel the chosen launch shape runs: the general
+    pass on meshes with irregular elements or regular_path 0, else the all-regular
+    pass (1), its pipelined form (2) or the pipelined form with ping-pong LDS
+    buffers (3)."""
+    if nirr > 0 or tuning.get("regular_path", 1) == 0:
+        return "k_forces"
+    return {1: "k_forces_reg", 2: "k_forces_reg_pipe", 3: "k_forces_reg_pp"}[tuning["regular_path"]]
+
+
 def kernel_acc_bytes(kb):
     """HBM bytes per point-step of one time-integrated kernel value: fp32 partial sum read +
     written every step, fp64 accumulator read + written once per kb-step block."""
@@ -307,8 +317,9 @@ def main():
         # two-field element pass: ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local
         # point, mu/kappa accumulators 2 x kacc, element-local forces written for both fields
         # 24; displacement gather of both fields 12 + 12 per unique point
+        ename = element_kernel_name(tuning, nirr)
         kinds = {
-            "k_forces<2,iso,fused-kernels>": ((28.0 + kmu + 2 * kacc) * nloc + 40.0 * nirr + 24.0 * nglob,
+            f"{ename}<2,iso,fused-kernels>": ((28.0 + kmu + 2 * kacc) * nloc + 40.0 * nirr + 24.0 * nglob,
                                               adj_ms, adj_n),
             # two-field assembly: CSR offsets 4 per point + indices 4 per contribution,
             # element-local forces of both fields 24 per local point, Stacey terms of both

@@ -805,7 +805,8 @@ struct PassTimer {
   explicit PassTimer(int steps) {
     stride = std::max(1, steps / 256);
     ev.resize(4 * (size_t)((steps + stride - 1) / stride));
-    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    // timing-only events: no system-scope fence (cache writeback) at each record
+    for (auto& e : ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
   bool sampled(int it) const { return (it - 1) % stride == 0; }
   void mark(int it, int k, hipStream_t s) {

@@ -270,7 +270,8 @@ void launch_steps(SmoothPlan& p, const SmoothArgs& a, int nstep, bool timed) {
   std::vector<hipEvent_t> ev;
   if (timed) {
     ev.resize(3 * (size_t)nstep);
-    for (auto& x : ev) HIP_CHECK(hipEventCreate(&x));
+    // timing-only events: no system-scope fence (cache writeback) at each record
+    for (auto& x : ev) HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableSystemFence));
   }
   for (int it = 0; it < nstep; ++it) {
     if (timed) HIP_CHECK(hipEventRecord(ev[3 * it], p.stream));

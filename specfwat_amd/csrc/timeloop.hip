@@ -496,7 +496,8 @@ struct ForceTimer {
     stride = timer_stride(nstep);
     const size_t samples = (size_t)(nstep + stride - 1) / stride;
     ev.assign(2 * samples * per_step, nullptr);
-    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    // timing-only events: no system-scope fence (cache writeback) at each record
+    for (auto& e : ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     used = 0;
     launches = 0;
     active = false;
