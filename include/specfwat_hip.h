@@ -385,10 +385,11 @@ void fwat_cc_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, rea
 /* multitaper traveltime (imeas 13) / amplitude (14) measurement of every trace with the reference's
  * fall back to the cross-correlation measurement: MTTTMisfit%calc_adjoint_source
  * (src/preproc/adjoint_sources/mt_tt_misfit.f90:29-791 with utils/dpss.f90:137-262 and
- * process_cycle_skipping, signal.f90:510-598) and filter_adj; cfg(18) = the 7 values of
+ * process_cycle_skipping, signal.f90:510-598) and filter_adj; cfg(19) = the 7 values of
  * fwat_cc_adjoint_batch_, transfunc_waterlevel, water_threshold, mt_nw, num_taper, dt_fac, err_fac,
- * dt_max_scale, min_cycle_in_window, use_cc_error (0/1), min_period, max_period (adj_config.f90:18-26;
- * use_mt_error = .false.); meas(10,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
+ * dt_max_scale, min_cycle_in_window, use_cc_error (0/1), min_period, max_period, use_mt_error (0/1:
+ * the jackknife errors of calculate_mt_error, mt_tt_misfit.f90:302-409; adj_config.f90:18-26 defaults
+ * to .false.); meas(10,nwin,ntrace) = misfit, residual, error, cc_max, tshift, dlna,
  * selected, ishift, measurement used (13/14, 11/12 after the fall back, 0 unselected), the other misfit
  * (or NULL).  32 <= nstep <= 16383 */
 /* exponentiated phase measurement of every trace (imeas 4): ExponentiatedPhaseMisfit%calc_adjoint_source

@@ -39,7 +39,8 @@
 //   orc_dpss              dpss_windows (utils/dpss.f90:137-262); eigenvalues by bisection
 //                         where the reference calls LAPACK dsterf
 //   orc_mt_adjoint        MTTTMisfit%calc_adjoint_source (mt_tt_misfit.f90:29-791) with
-//                         process_cycle_skipping (signal.f90:510-598); use_mt_error off
+//                         process_cycle_skipping (signal.f90:510-598) and, with use_mt_error,
+//                         the jackknife errors of calculate_mt_error (:302-409)
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -562,12 +563,16 @@ void orc_dpss(int n, double half_nbw, int kmax, double* out) {
 // reference does.  cfg: [0..6] as orc_cc_adjoint, 7 transfunc_waterlevel,
 // 8 water_threshold, 9 mt_nw, 10 num_taper, 11 dt_fac, 12 err_fac,
 // 13 dt_max_scale, 14 min_cycle_in_window, 15 use_cc_error, 16 min_period,
-// 17 max_period (use_mt_error = .false., the default).  Per window out[10]:
-// misfit, residual, error, cc_max, tshift, dlna, selected, ishift, measurement
-// used (13/14, or 11/12 after the fallback, 0 unselected), the other misfit
-// (misfit_q for 13, misfit_p for 14).  Returns the total misfit.
+// 17 max_period, 18 use_mt_error (0/1; the reference's default is .false.).
+// Per window out[10]: misfit, residual, error, cc_max, tshift, dlna, selected,
+// ishift, measurement used (13/14, or 11/12 after the fallback, 0 unselected),
+// the other misfit (misfit_q for 13, misfit_p for 14).  With use_mt_error and
+// err non-null, err[w][2][nf] receives the window's jackknife err_dtau and
+// err_dlna per frequency (zero when the window never reaches them).  Returns
+// the total misfit.
 double orc_mt_adjoint(const double* dat, const double* syn, int n, double dt, const double* windows, int nwin,
-                      double taper_pct, int itaper, int imeas, const double* cfg, double* adj, double* out) {
+                      double taper_pct, int itaper, int imeas, const double* cfg, double* adj, double* out,
+                      double* err) {
   using cd = std::complex<double>;
   const double pi = 3.141592653589793;
   double total = 0.0;
@@ -583,6 +588,8 @@ double orc_mt_adjoint(const double* dat, const double* syn, int n, double dt, co
   }
   const double df = 1.0 / (nf * dt) - 0.0;
   const int ntaper = (int)cfg[10];
+  const bool use_mt_error = cfg[18] != 0.0;
+  if (err) for (size_t i = 0; i < 2 * (size_t)nf * nwin; ++i) err[i] = 0.0;
   for (int w = 0; w < nwin; ++w) {
     double* o = out + 10 * (size_t)w;
     const int nb = (int)(windows[2 * w] / dt) + 1, ne = (int)(windows[2 * w + 1] / dt);
@@ -647,39 +654,93 @@ double orc_mt_adjoint(const double* dat, const double* syn, int n, double dt, co
       dpss(nl, cfg[9], ntaper, tap);
       for (auto& t : tap) for (double& x : t) x = x * std::sqrt((double)nl);
       // calculate_multitaper (:211-300)
-      std::vector<cd> top(nf, cd(0.0, 0.0)), bot(nf, cd(0.0, 0.0));
+      // the tapered spectra of every taper are kept: the jackknife below forms
+      // its leave-one-out sums from the same products
+      std::vector<std::vector<cd>> pdw(ntaper), psw(ntaper);
       for (int k = 0; k < ntaper; ++k) {
-        std::vector<cd> dw(nf, cd(0.0, 0.0)), sw(nf, cd(0.0, 0.0));
+        std::vector<cd>& dw = pdw[k];
+        std::vector<cd>& sw = psw[k];
+        dw.assign(nf, cd(0.0, 0.0));
+        sw.assign(nf, cd(0.0, 0.0));
         for (int i = 0; i < nl; ++i) { dw[i] = cd(dm[i] * tap[k][i], 0.0); sw[i] = cd(sv[i] * tap[k][i], 0.0); }
         fft_raw(dw, -1, pi);
         fft_raw(sw, -1, pi);
         for (int i = 0; i < nf; ++i) {
           dw[i] = dw[i] * dt;
           sw[i] = sw[i] * dt;
-          top[i] = top[i] + dw[i] * std::conj(sw[i]);
-          bot[i] = bot[i] + sw[i] * std::conj(sw[i]);
         }
       }
-      double bmax = 0.0;
-      for (int i = 0; i < fnum; ++i) bmax = std::max(bmax, std::abs(bot[i]));
-      const double wlevel = bmax * (cfg[7] * cfg[7]);
-      std::vector<double> phi(nf, 0.0), absw(nf, 0.0), dtau(nf, 0.0), dlnw(nf, 0.0);
-      std::vector<cd> tf(nf, cd(0.0, 0.0));
-      for (int i = nfmin; i <= nfmax; ++i)
-        tf[i - 1] = std::abs(bot[i - 1]) >= wlevel ? cdivr(top[i - 1], bot[i - 1].real())
-                                                   : cdivr(top[i - 1], bot[i - 1].real() + wlevel);
-      for (int i = nfmin; i <= nfmax; ++i) phi[i - 1] = std::atan2(tf[i - 1].imag(), tf[i - 1].real());
-      cycle_skipping(phi, nfmin, nfmax, 1.5);
-      for (int i = nfmin; i <= nfmax; ++i) absw[i - 1] = std::abs(tf[i - 1]);
-      dtau[0] = c.tshift;
-      for (int i = std::max(2, nfmin); i <= nfmax; ++i) dtau[i - 1] = -(1.0 / wvec[i - 1]) * phi[i - 1] + c.tshift;
-      for (int i = nfmin; i <= nfmax; ++i) dlnw[i - 1] = std::log(absw[i - 1]) + c.dlna;
-      // check_mtm_time_shift_acceptability (:520-569), MT errors zero
+      // calculate_multitaper over the tapers with use[k] set: dtau and dlnA
+      // per frequency (phase and amplitude only feed these)
+      auto multitaper = [&](const std::vector<char>& use, std::vector<double>& dtau, std::vector<double>& dlnw) {
+        std::vector<cd> top(nf, cd(0.0, 0.0)), bot(nf, cd(0.0, 0.0));
+        for (int k = 0; k < ntaper; ++k) {
+          if (!use[k]) continue;
+          for (int i = 0; i < nf; ++i) {
+            top[i] = top[i] + pdw[k][i] * std::conj(psw[k][i]);
+            bot[i] = bot[i] + psw[k][i] * std::conj(psw[k][i]);
+          }
+        }
+        double bmax = 0.0;
+        for (int i = 0; i < fnum; ++i) bmax = std::max(bmax, std::abs(bot[i]));
+        const double wlevel = bmax * (cfg[7] * cfg[7]);
+        std::vector<double> phi(nf, 0.0), absw(nf, 0.0);
+        dtau.assign(nf, 0.0);
+        dlnw.assign(nf, 0.0);
+        std::vector<cd> tf(nf, cd(0.0, 0.0));
+        for (int i = nfmin; i <= nfmax; ++i)
+          tf[i - 1] = std::abs(bot[i - 1]) >= wlevel ? cdivr(top[i - 1], bot[i - 1].real())
+                                                     : cdivr(top[i - 1], bot[i - 1].real() + wlevel);
+        for (int i = nfmin; i <= nfmax; ++i) phi[i - 1] = std::atan2(tf[i - 1].imag(), tf[i - 1].real());
+        cycle_skipping(phi, nfmin, nfmax, 1.5);
+        for (int i = nfmin; i <= nfmax; ++i) absw[i - 1] = std::abs(tf[i - 1]);
+        dtau[0] = c.tshift;
+        for (int i = std::max(2, nfmin); i <= nfmax; ++i) dtau[i - 1] = -(1.0 / wvec[i - 1]) * phi[i - 1] + c.tshift;
+        for (int i = nfmin; i <= nfmax; ++i) dlnw[i - 1] = std::log(absw[i - 1]) + c.dlna;
+      };
+      std::vector<double> dtau, dlnw;
+      multitaper(std::vector<char>(ntaper, 1), dtau, dlnw);
+      // calculate_mt_error (:302-409): jackknife over the tapers left out one at a
+      // time.  The reference copies rows 1:nlen_f of each kept taper into an
+      // nlen_t-row array; the rows that land in the array are those of the
+      // taper itself, so each estimate uses the other tapers unchanged.
+      std::vector<double> err_dtau(nf, 0.0), err_dlna(nf, 0.0);
+      if (use_mt_error) {
+        std::vector<std::vector<double>> xt(ntaper), xl(ntaper);
+        std::vector<double> et(nf, 0.0), el(nf, 0.0);
+        const double fn = (double)ntaper, fn1 = (double)(ntaper - 1);
+        for (int k = 0; k < ntaper; ++k) {
+          std::vector<char> use(ntaper, 1);
+          use[k] = 0;
+          multitaper(use, xt[k], xl[k]);
+          for (int i = nfmin; i <= nfmax; ++i) {
+            et[i - 1] = et[i - 1] + fn * dtau[i - 1] - fn1 * xt[k][i - 1];
+            el[i - 1] = el[i - 1] + fn * dlnw[i - 1] - fn1 * xl[k][i - 1];
+          }
+        }
+        for (int i = 0; i < nf; ++i) { et[i] = et[i] / fn; el[i] = el[i] / fn; }
+        for (int k = 0; k < ntaper; ++k)
+          for (int i = nfmin; i <= nfmax; ++i) {
+            err_dtau[i - 1] = err_dtau[i - 1] + (xt[k][i - 1] - et[i - 1]) * (xt[k][i - 1] - et[i - 1]);
+            err_dlna[i - 1] = err_dlna[i - 1] + (xl[k][i - 1] - el[i - 1]) * (xl[k][i - 1] - el[i - 1]);
+          }
+        const double nn = (double)(ntaper * (ntaper - 1));
+        for (int i = nfmin; i <= nfmax; ++i) {
+          err_dtau[i - 1] = std::sqrt(err_dtau[i - 1] / nn);
+          err_dlna[i - 1] = std::sqrt(err_dlna[i - 1] / nn);
+        }
+        if (err)
+          for (int i = 0; i < nf; ++i) {
+            err[(2 * (size_t)w) * nf + i] = err_dtau[i];
+            err[(2 * (size_t)w + 1) * nf + i] = err_dlna[i];
+          }
+      }
+      // check_mtm_time_shift_acceptability (:520-569)
       for (int j = nfmin; j <= nfmax && mtm; ++j) {
         if (j * df > 0.0) {
           const double per = 1.0 / (j * df);
           if (std::fabs(dtau[j - 1]) > per / cfg[11]) mtm = false;
-          else if (0.0 > per / cfg[12]) mtm = false;
+          else if (err_dtau[j - 1] > per / cfg[12]) mtm = false;
           else if (std::fabs(dtau[j - 1]) > cfg[13] * std::fabs(c.tshift)) mtm = false;
         }
       }
@@ -694,8 +755,23 @@ double orc_mt_adjoint(const double* dat, const double* syn, int n, double dt, co
       const double ffac = 2.0 * df * ssum;
       std::vector<double> wp(nf), wq(nf);
       for (int i = 0; i < nf; ++i) { wp[i] = wt[i] / ffac; wq[i] = wt[i] / ffac; }
-      if (cfg[15] != 0.0)
+      if (cfg[15] != 0.0) {
         for (int i = 0; i < nf; ++i) { wp[i] = wp[i] / (c.sig_dt * c.sig_dt); wq[i] = wq[i] / (c.sig_dlna * c.sig_dlna); }
+      } else if (use_mt_error) {
+        // water level of the multitaper errors: a fraction of the mean |dtau|,
+        // |dlnA| over [nfreq_min, nfreq_max - 1], added where an error is below it
+        double st = 0.0, sl = 0.0;
+        for (int i = nfmin; i <= nfmax - 1; ++i) { st = st + std::fabs(dtau[i - 1]); sl = sl + std::fabs(dlnw[i - 1]); }
+        const double dtau_wtr = cfg[8] * st / (double)(nfmax - nfmin);
+        const double dlna_wtr = cfg[8] * sl / (double)(nfmax - nfmin);
+        for (int i = nfmin; i <= nfmax; ++i) {
+          double et = err_dtau[i - 1], el = err_dlna[i - 1];
+          if (et < dtau_wtr) et = et + dtau_wtr;
+          if (el < dlna_wtr) el = el + dlna_wtr;
+          wp[i - 1] = wp[i - 1] / (et * et);
+          wq[i - 1] = wq[i - 1] / (el * el);
+        }
+      }
       std::vector<double> yp(nf), yq(nf);
       for (int i = 0; i < nf; ++i) { yp[i] = dtau[i] * dtau[i] * wp[i]; yq[i] = dlnw[i] * dlnw[i] * wq[i]; }
       mp = orc_simpson(yp.data(), nf, df);

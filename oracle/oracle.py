@@ -152,7 +152,7 @@ def lib():
         L.orc_dpss.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
         L.orc_mt_adjoint.restype = C.c_double
         L.orc_mt_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
-                                     _dp, _dp, _dp]
+                                     _dp, _dp, _dp, C.c_void_p]
         L.orc_sd_direction.argtypes = [C.c_long, _fp, _fp, _fp, C.c_int]
         L.orc_lbfgs_direction.argtypes = [C.c_long, C.c_long, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float,
                                           C.c_float, _fp, _dp, C.c_int]
@@ -624,7 +624,8 @@ def dpss(n, half_nbw, kmax):
 
 
 MT_DEFAULTS = dict(transfunc_waterlevel=1e-10, water_threshold=0.02, mt_nw=4.0, num_taper=5, dt_fac=2.0,
-                   err_fac=2.5, dt_max_scale=3.5, min_cycle_in_window=3, use_cc_error=True)  # adj_config.f90:18-26
+                   err_fac=2.5, dt_max_scale=3.5, min_cycle_in_window=3, use_cc_error=True,
+                   use_mt_error=False)  # adj_config.f90:18-26
 MT_FIELDS = ("misfit", "residual", "error", "cc_max", "tshift", "dlna", "selected", "ishift", "imeas", "misfit_other")
 
 
@@ -634,13 +635,17 @@ def mt_config(min_period, max_period, **cfg):
     return np.array([c["dt_sigma_min"], c["dlna_sigma_min"], c["tshift_min"], c["tshift_max"], c["dlna_min"],
                      c["dlna_max"], c["cc_min"], c["transfunc_waterlevel"], c["water_threshold"], c["mt_nw"],
                      c["num_taper"], c["dt_fac"], c["err_fac"], c["dt_max_scale"], c["min_cycle_in_window"],
-                     1.0 if c["use_cc_error"] else 0.0, min_period, max_period], np.float64)
+                     1.0 if c["use_cc_error"] else 0.0, min_period, max_period,
+                     1.0 if c["use_mt_error"] else 0.0], np.float64)
 
 
-def mt_adjoint(dat, syn, dt, windows, min_period, max_period, imeas=13, taper_pct=0.05, itaper=1, **cfg):
+def mt_adjoint(dat, syn, dt, windows, min_period, max_period, imeas=13, taper_pct=0.05, itaper=1, errors=False,
+               **cfg):
     """One trace's multitaper traveltime (13) or amplitude (14) measurement with
     the reference's cross-correlation fallback: (total, adj, per-window dict of
-    MT_FIELDS; "imeas" is the measurement used, 11/12 after a fallback)."""
+    MT_FIELDS; "imeas" is the measurement used, 11/12 after a fallback).  With
+    use_mt_error=True the jackknife errors weight the misfit; errors=True also
+    returns them, [nwin, 2, nf] (err_dtau, err_dlna per frequency)."""
     cv = mt_config(min_period, max_period, **cfg)
     dat = np.ascontiguousarray(dat, np.float64)
     syn = np.ascontiguousarray(syn, np.float64)
@@ -648,11 +653,15 @@ def mt_adjoint(dat, syn, dt, windows, min_period, max_period, imeas=13, taper_pc
     nw = win.shape[0]
     adj = np.zeros(len(dat))
     out = np.zeros((nw, 10))
+    nf = 2 ** (int(np.frexp(np.float32(len(dat)))[1]))  # 2 ** exponent(real(nlen))
+    err = np.zeros((nw, 2, nf))
     tot = lib().orc_mt_adjoint(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt, _p(win, C.c_double), nw,
                                taper_pct, itaper, imeas, _p(cv, C.c_double), _p(adj, C.c_double),
-                               _p(out, C.c_double))
+                               _p(out, C.c_double), err.ctypes.data_as(C.c_void_p))
     o = {k: out[:, i].copy() for i, k in enumerate(MT_FIELDS)}
     o["selected"] = o["selected"] != 0
+    if errors:
+        return tot, adj, o, err
     return tot, adj, o
 
 

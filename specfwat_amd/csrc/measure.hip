@@ -722,8 +722,19 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_flim(int nstep, int ntrace, d
 }
 
 struct MtCfg {
-  double wl_tf, dt_fac, err_fac, dt_max_scale, min_cycle, min_period;
-  int ntaper, use_cc_error;
+  double wl_tf, dt_fac, err_fac, dt_max_scale, min_cycle, min_period, water_threshold;
+  int ntaper, use_cc_error, use_mt_error;
+};
+
+// Per-trace rows of the jackknife errors (calculate_mt_error), used only with
+// use_mt_error: the products D_j conj(S_j) and |S_j|^2 of every taper j
+// ([ntaper][M + 1] per trace), the leave-one-out (dtau, dlnA) estimates
+// ([ntaper][M + 1]) and the resulting (err_dtau, err_dlna) per frequency.
+struct MtJack {
+  double2* pd;
+  double* pb;
+  double2* xj;
+  double2* er;
 };
 
 // Multitaper measurement of one window of every selected trace (one
@@ -741,6 +752,13 @@ struct MtCfg {
 //                 then the weighted adjoint spectrum H
 //   G1 (double2): sum D conj(S), then (dtau, w), then the adjoint factors
 //   G2 (double) : sum |S|^2, then the adjoint normalisation sum
+// With use_mt_error the jackknife of calculate_mt_error forms, for every
+// taper left out in turn, the multitaper estimate of the other tapers from the
+// stored per-taper products (summed in taper order from zero, as the
+// reference recomputes them), then the pseudo-value mean and the spread per
+// frequency; the errors enter the acceptance test and, without use_cc_error,
+// weight the misfit and adjoint source (with the water level of
+// calculate_freq_domain_taper).
 // out[10] per trace: misfit, residual, error, cc_max, tshift, dlna, selected,
 // ishift, then (on entry) sigma_dt, sigma_dlna and (on exit) the measurement
 // used and the other misfit.
@@ -752,9 +770,10 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
                                                           const double* __restrict__ S, const int* __restrict__ flim,
                                                           double2* __restrict__ G0, double2* __restrict__ G1,
                                                           double* __restrict__ G2, double* __restrict__ A,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out, MtJack J) {
   __shared__ double2 buf[MT_MAXM];
   __shared__ double red[MT_THREADS];
+  __shared__ double wtr[2];
   const double pi = 3.141592653589793;
   const int t = blockIdx.x, M = nf / 2, lane = threadIdx.x;
   const bool tt = imeas == 13;
@@ -785,6 +804,12 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
   double2* g0 = G0 + (size_t)t * (M + 1);
   double2* g1 = G1 + (size_t)t * (M + 1);
   double* g2 = G2 + (size_t)t * (M + 1);
+  const bool jk = cfg.use_mt_error != 0;
+  const size_t rows = (size_t)cfg.ntaper * (M + 1);
+  double2* pd = jk ? J.pd + (size_t)t * rows : nullptr;
+  double* pb = jk ? J.pb + (size_t)t * rows : nullptr;
+  double2* xj = jk ? J.xj + (size_t)t * rows : nullptr;
+  double2* er = jk ? J.er + (size_t)t * (M + 1) : nullptr;
   // spectrum of the packed real sequence at 0 <= k < M, and at k = M
   auto xk = [&](int k) {
     if (k == M) { const double2 z = buf[0]; return make_double2(z.x - z.y, 0.0); }
@@ -807,11 +832,17 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
       const double a = x.x * dt, b = x.y * dt, c = y.x * dt, d = -(y.y * dt);
       double2 tp = j == 0 ? make_double2(0.0, 0.0) : g1[k];
       double bt = j == 0 ? 0.0 : g2[k];
-      tp.x = tp.x + (a * c - b * d);
-      tp.y = tp.y + (a * d + b * c);
-      bt = bt + (c * c - d * (-d));
+      const double2 pj = make_double2(a * c - b * d, a * d + b * c);
+      const double bj = c * c - d * (-d);
+      tp.x = tp.x + pj.x;
+      tp.y = tp.y + pj.y;
+      bt = bt + bj;
       g1[k] = tp;
       g2[k] = bt;
+      if (jk) {
+        pd[(size_t)j * (M + 1) + k] = pj;
+        pb[(size_t)j * (M + 1) + k] = bj;
+      }
     }
     __syncthreads();
   }
@@ -834,8 +865,9 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
     g0[k] = make_double2(lw, 0.0);
   }
   __syncthreads();
-  if (lane == 0) {
-    // process_cycle_skipping (signal.f90:510-598), phase_step 1.5
+  // process_cycle_skipping (signal.f90:510-598), phase_step 1.5, on one lane
+  const auto cycle_skip = [&]() {
+    if (lane != 0) return;
     for (int iw = nfmin + 2; iw <= nfmax - 1; ++iw) {
       const double p0 = phiL[iw - 2], p1 = phiL[iw - 1], p2 = phiL[iw];
       const double s0 = fabs(p2 + p0 - 2.0 * p1);
@@ -848,7 +880,8 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
           for (int i = iw + 1; i <= nfmax; ++i) phiL[i - 1] = phiL[i - 1] - 2.0 * pi;
       }
     }
-  }
+  };
+  cycle_skip();
   __syncthreads();
   // dtau, the acceptance test, the frequency-domain taper weights -> G1 = (dtau, w)
   const int L = nfmax - nfmin, frac = (int)(L * 1.0 / 2.0), fden = 2 * frac - 1;
@@ -878,6 +911,79 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
     }
     g1[k] = make_double2(dtau, w);
   }
+  if (jk) {
+    __syncthreads();  // phiL is reused below
+    // calculate_mt_error: the estimate without taper q, for q in order
+    for (int q = 0; q < cfg.ntaper; ++q) {
+      double lq = 0.0;
+      for (int k = lane; k <= M; k += MT_THREADS) {
+        double bt = 0.0;
+        for (int j = 0; j < cfg.ntaper; ++j)
+          if (j != q) bt = bt + pb[(size_t)j * (M + 1) + k];
+        lq = fmax(lq, fabs(bt));
+      }
+      const double wlq = mt_bmax<MT_THREADS>(lq, red) * (cfg.wl_tf * cfg.wl_tf);
+      for (int k = lane; k <= M; k += MT_THREADS) {
+        const int i = k + 1;
+        double lw = 0.0;
+        if (i >= nfmin && i <= nfmax) {
+          double2 tp = make_double2(0.0, 0.0);
+          double bt = 0.0;
+          for (int j = 0; j < cfg.ntaper; ++j) {
+            if (j == q) continue;
+            const double2 p = pd[(size_t)j * (M + 1) + k];
+            tp.x = tp.x + p.x;
+            tp.y = tp.y + p.y;
+            bt = bt + pb[(size_t)j * (M + 1) + k];
+          }
+          const double den = fabs(bt) >= wlq ? bt : bt + wlq;
+          const double tr = tp.x / den, ti = tp.y / den;
+          phiL[k] = atan2(ti, tr);
+          lw = log(hypot(tr, ti)) + dlna;
+        }
+        xj[(size_t)q * (M + 1) + k] = make_double2(0.0, lw);
+      }
+      __syncthreads();
+      cycle_skip();
+      __syncthreads();
+      for (int k = lane; k <= M; k += MT_THREADS) {
+        const int i = k + 1;
+        const double f = i <= nf / 2 ? (double)(i - 1) / (nf * dt) : -(double)(nf - i + 1) / (nf * dt);
+        double dq = 0.0;
+        if (i == 1) dq = tshift;
+        else if (i >= nfmin && i <= nfmax) dq = -(1.0 / (2.0 * pi * f)) * phiL[k] + tshift;
+        xj[(size_t)q * (M + 1) + k].x = dq;
+      }
+      __syncthreads();
+    }
+    // pseudo-value means and the spread of the estimates about them
+    const double fn = (double)cfg.ntaper, fn1 = (double)(cfg.ntaper - 1);
+    const double nn = (double)(cfg.ntaper * (cfg.ntaper - 1));
+    for (int k = lane; k <= M; k += MT_THREADS) {
+      const int i = k + 1;
+      double2 e = make_double2(0.0, 0.0);
+      if (i >= nfmin && i <= nfmax) {
+        const double xt = g1[k].x, xl = g0[k].x;
+        double et = 0.0, el = 0.0;
+        for (int q = 0; q < cfg.ntaper; ++q) {
+          const double2 x = xj[(size_t)q * (M + 1) + k];
+          et = et + fn * xt - fn1 * x.x;
+          el = el + fn * xl - fn1 * x.y;
+        }
+        et = et / fn;
+        el = el / fn;
+        double st = 0.0, sl = 0.0;
+        for (int q = 0; q < cfg.ntaper; ++q) {
+          const double2 x = xj[(size_t)q * (M + 1) + k];
+          st = st + (x.x - et) * (x.x - et);
+          sl = sl + (x.y - el) * (x.y - el);
+        }
+        e = make_double2(sqrt(st / nn), sqrt(sl / nn));
+        if (i * df > 0.0 && e.x > (1.0 / (i * df)) / cfg.err_fac) bad = 1.0;
+      }
+      er[k] = e;
+    }
+  }
   if (mt_bmax<MT_THREADS>(bad, red) != 0.0) {
     if (lane == 0) {
       o[8] = tt ? 11.0 : 12.0;
@@ -886,6 +992,20 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
     return;
   }
   const double ffac = 2.0 * df * mt_bsum<MT_THREADS>(wsum, red);
+  if (jk && !cfg.use_cc_error) {
+    // water level of the multitaper errors: water_threshold times the mean
+    // |dtau|, |dlnA| over [nfreq_min, nfreq_max - 1], summed in order
+    if (lane == 0) {
+      double st = 0.0, sl = 0.0;
+      for (int i = nfmin; i <= nfmax - 1; ++i) {
+        st = st + fabs(g1[i - 1].x);
+        sl = sl + fabs(g0[i - 1].x);
+      }
+      wtr[0] = cfg.water_threshold * st / (double)(nfmax - nfmin);
+      wtr[1] = cfg.water_threshold * sl / (double)(nfmax - nfmin);
+    }
+    __syncthreads();
+  }
   // Simpson misfits over the nf frequency samples (only k <= nf/2 < nf - 2 can be
   // non-zero); G1 becomes the adjoint factors (dtau, wp) or (dlnA, wq)
   double sp = 0.0, sq = 0.0;
@@ -893,7 +1013,16 @@ __global__ void __launch_bounds__(MT_THREADS) k_mt_window(int nstep, int ntrace,
     const double2 dw = g1[k];
     const double lw = g0[k].x;
     double wp = dw.y / ffac, wq = dw.y / ffac;
-    if (cfg.use_cc_error) { wp = wp / (sig_dt * sig_dt); wq = wq / (sig_dlna * sig_dlna); }
+    if (cfg.use_cc_error) {
+      wp = wp / (sig_dt * sig_dt);
+      wq = wq / (sig_dlna * sig_dlna);
+    } else if (jk && k + 1 >= nfmin && k + 1 <= nfmax) {
+      const double2 e = er[k];
+      const double et = e.x < wtr[0] ? e.x + wtr[0] : e.x;
+      const double el = e.y < wtr[1] ? e.y + wtr[1] : e.y;
+      wp = wp / (et * et);
+      wq = wq / (el * el);
+    }
     const double sw = k == 0 ? 1.0 : (k % 2 == 1 ? 4.0 : 2.0);
     sp += sw * (dw.x * dw.x * wp);
     sq += sw * (lw * lw * wq);
@@ -2054,10 +2183,10 @@ void fwat_cc_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
 
 // mt_tt_misfit.f90 MTTTMisfit%calc_adjoint_source for every trace: imeas 13
 // (multitaper traveltime) or 14 (amplitude), with the cross-correlation
-// fallback of the reference.  cfg(18) = the 7 cross-correlation values of
+// fallback of the reference.  cfg(19) = the 7 cross-correlation values of
 // fwat_cc_adjoint_batch_, then transfunc_waterlevel, water_threshold, mt_nw,
 // num_taper, dt_fac, err_fac, dt_max_scale, min_cycle_in_window,
-// use_cc_error (0/1), min_period, max_period (use_mt_error = .false.);
+// use_cc_error (0/1), min_period, max_period, use_mt_error (0/1);
 // meas(10, nwin, ntrace) = misfit, residual, error, cc_max, tshift, dlna,
 // selected, ishift, measurement used (13/14, 11/12 after the fallback, 0 not
 // selected), the other misfit; window adjoint sources are summed as the
@@ -2078,7 +2207,8 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   if (ntaper < 2 || ntaper > 16) SEM_FAIL("fwat_mt_adjoint_batch_: num_taper must be in [2, 16]");
   if (!(cfg[16] > 0.0) || !(cfg[17] > 0.0)) SEM_FAIL("fwat_mt_adjoint_batch_: min/max period must be positive");
   const CcCfg C{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6]};
-  const MtCfg MC{cfg[7], cfg[11], cfg[12], cfg[13], cfg[14], cfg[16], ntaper, cfg[15] != 0.0 ? 1 : 0};
+  const MtCfg MC{cfg[7],   cfg[11], cfg[12], cfg[13], cfg[14], cfg[16], cfg[8], ntaper, cfg[15] != 0.0 ? 1 : 0,
+                 cfg[18] != 0.0 ? 1 : 0};
   int e2 = 0;
   std::frexp((float)N, &e2);
   const int nf = 1 << e2, M = nf / 2;  // 2 ** exponent(real(nlen))
@@ -2124,6 +2254,14 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   double* d_A = dev_alloc<double>((size_t)T * maxnl);
   double2* d_g01 = dev_alloc<double2>(2 * (size_t)T * (M + 1));
   double* d_g2 = dev_alloc<double>((size_t)T * (M + 1));
+  MtJack jack{nullptr, nullptr, nullptr, nullptr};
+  if (MC.use_mt_error) {
+    const size_t rows = (size_t)T * ntaper * (M + 1);
+    jack.pd = dev_alloc<double2>(rows);
+    jack.pb = dev_alloc<double>(rows);
+    jack.xj = dev_alloc<double2>(rows);
+    jack.er = dev_alloc<double2>((size_t)T * (M + 1));
+  }
   double* d_tw = dev_alloc<double>(maxnl);
   double* d_tap = dev_alloc<double>((size_t)ntaper * maxnl);
   const int maxblk = (2 * maxnl - 1 + CC_LAGS - 1) / CC_LAGS;
@@ -2154,7 +2292,7 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
                        ow, 10);
     hipLaunchKernelGGL(k_mt_window, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, T, *dt, nf, logM, d_twd, nbs[w], nl,
                        *imeas, MC, d_tw, d_tap, d_obsT, d_S, d_flim, d_g01, d_g01 + (size_t)T * (M + 1), d_g2, d_A,
-                       ow);
+                       ow, jack);
     hipLaunchKernelGGL(k_mt_put, tiles, dim3(256), 0, s, T, nbs[w], nl, d_A, ow, 10, d_adj);
   }
   std::vector<double> fw;
@@ -2191,6 +2329,7 @@ void fwat_mt_adjoint_batch_(int* nstep, int* ntrace, double* dt, realw* obs, rea
   dev_free(d_obs); dev_free(d_syn); dev_free(d_adj); dev_free(d_twd); dev_free(d_ish); dev_free(d_flim);
   dev_free(d_out); dev_free(d_D); dev_free(d_S); dev_free(d_A); dev_free(d_tw); dev_free(d_tap); dev_free(d_bv);
   dev_free(d_bi); dev_free(d_fw); dev_free(d_sos); dev_free(d_g01); dev_free(d_g2); dev_free(d_obsT); dev_free(d_synT);
+  dev_free(jack.pd); dev_free(jack.pb); dev_free(jack.xj); dev_free(jack.er);
   HIP_CHECK(hipEventDestroy(ev0));
   HIP_CHECK(hipEventDestroy(ev1));
   HIP_CHECK(hipStreamDestroy(s));

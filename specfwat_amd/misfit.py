@@ -189,7 +189,8 @@ def event_cc_adjoint_sources(obs, syn, dt, windows=None, imeas=IMEAS_CC_TT, tape
 
 IMEAS_CC_TT_MT, IMEAS_CC_DLNA_MT = 13, 14  # adj_config
 MT_DEFAULTS = dict(transfunc_waterlevel=1e-10, water_threshold=0.02, mt_nw=4.0, num_taper=5, dt_fac=2.0,
-                   err_fac=2.5, dt_max_scale=3.5, min_cycle_in_window=3, use_cc_error=True)  # adj_config.f90:18-26
+                   err_fac=2.5, dt_max_scale=3.5, min_cycle_in_window=3, use_cc_error=True,
+                   use_mt_error=False)  # adj_config.f90:18-26
 MT_FIELDS = ("misfit", "residual", "error", "cc_max", "tshift", "dlna", "selected", "ishift", "imeas",
              "misfit_other")
 
@@ -205,7 +206,10 @@ def event_mt_adjoint_sources(obs, syn, dt, windows, min_period, max_period, imea
     float32 like syn, per-window measurements [nrec*3, nwin, 10] in MT_FIELDS
     order; "imeas" is the measurement used: 13/14, 11/12 after a fall back, 0
     when the window is not selected).  cfg overrides CC_DEFAULTS and
-    MT_DEFAULTS (use_mt_error = .false. as the reference's default)."""
+    MT_DEFAULTS (use_mt_error = .false. as the reference's default; with
+    use_mt_error=True and use_cc_error=False the jackknife errors of
+    calculate_mt_error weight the misfit and adjoint source, and with either
+    setting they enter the acceptance test)."""
     nstep, ntrace, win, o, s, sos = _event_inputs(obs, syn, dt, windows, min_period, max_period)
     unknown = set(cfg) - set(CC_DEFAULTS) - set(MT_DEFAULTS)
     if unknown:
@@ -219,7 +223,8 @@ def event_mt_adjoint_sources(obs, syn, dt, windows, min_period, max_period, imea
     cv = np.array([c["dt_sigma_min"], c["dlna_sigma_min"], c["tshift_min"], c["tshift_max"], c["dlna_min"],
                    c["dlna_max"], c["cc_min"], c["transfunc_waterlevel"], c["water_threshold"], c["mt_nw"],
                    c["num_taper"], c["dt_fac"], c["err_fac"], c["dt_max_scale"], c["min_cycle_in_window"],
-                   1.0 if c["use_cc_error"] else 0.0, min_period, max_period], np.float64)
+                   1.0 if c["use_cc_error"] else 0.0, min_period, max_period,
+                   1.0 if c["use_mt_error"] else 0.0], np.float64)
     if not filter_adj:
         sos = np.zeros((0, 6), np.float32)
     adj = np.zeros(s.shape, np.float32)

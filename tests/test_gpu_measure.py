@@ -153,6 +153,34 @@ def test_mt_batch_matches_oracle(imeas, filt):
           f"{ {int(k): used.count(k) for k in set(used)} }")
 
 
+@pytest.mark.parametrize("imeas,use_cc_error", [(13, True), (13, False), (14, False)])
+def test_mt_batch_with_jackknife_errors_matches_oracle(imeas, use_cc_error):
+    """use_mt_error: the leave-one-taper-out errors of calculate_mt_error gate
+    acceptance and, without use_cc_error, weight the misfit and adjoint source;
+    every trace against the oracle."""
+    obs, syn, dt = _mt_event(imeas)
+    win = [[10.0, 32.0], [12.0, 40.03]]
+    kw = dict(use_mt_error=True, use_cc_error=use_cc_error)
+    total, adj, meas = misfit.event_mt_adjoint_sources(obs, syn, dt, win, 0.5, 10.0, imeas=imeas,
+                                                       filter_adj=False, **kw)
+    tot = 0.0
+    used = []
+    for r in range(obs.shape[1]):
+        for c in range(3):
+            m, a, o = orc.mt_adjoint(obs[:, r, c], syn[:, r, c], dt, win, 0.5, 10.0, imeas=imeas, **kw)
+            tot += m
+            q = meas[3 * r + c]
+            np.testing.assert_array_equal(q[:, 8], o["imeas"])
+            np.testing.assert_allclose(q[:, 0], o["misfit"], rtol=1e-7, atol=1e-300)
+            np.testing.assert_allclose(q[:, 9], o["misfit_other"], rtol=1e-7, atol=1e-300)
+            assert np.abs(adj[:, r, c] - a).max() <= 2e-5 * max(np.abs(a).max(), 1e-30)
+            used += list(o["imeas"])
+    assert abs(total - tot) <= 1e-7 * max(tot, 1e-30)
+    assert imeas in used
+    print(f"jackknife errors, imeas {imeas}, use_cc_error {use_cc_error}: misfit {total:.6e}, windows by "
+          f"measurement { {int(k): used.count(k) for k in set(used)} }")
+
+
 @pytest.mark.parametrize("filt", [False, True])
 def test_ep_batch_matches_oracle(filt):
     """Exponentiated phase over an even and an odd window (the two Nyquist
