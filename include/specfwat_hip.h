@@ -409,6 +409,19 @@ void fwat_ep_adjoint_batch_(int *nstep, int *ntrace, double *dt, realw *obs, rea
  * windows(2,maxwin,ntrace) out (s) */
 void fwat_win_sel_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, double *t0, double *tp,
                          double *dist, double *min_period, double *cfg, int *maxwin, int *nwin, double *windows);
+/* FLEXWIN window selection (WIN_TYPE 2, flexwin_type%select_windows, src/preproc/window_selector/
+ * flexwin.f90:75-734 with schedule_weighted_intervals, interval_scheduling.f90:13-165, as
+ * leq_data.f90:116-121 calls it) of every trace: obs, syn (NDIM, nrec, NSTEP) single precision
+ * (filtered), tp(ntrace), dist(ntrace) first-arrival time and distance, t0, min/max period of the
+ * band; cfg(16) = min_velocity, min_win_len_fac, threshold_corr, threshold_shift_fac,
+ * min_peaks_troughs, threshold_dlna, min_snr_window, is_split_phases (0/1),
+ * max_time_before_first_arrival, stalta_water_level, c_0, c_2, c_3a, c_3b, c_4a, c_4b
+ * (input_params.f90:400-418).  nwin(ntrace); windows(2, maxwin, ntrace) in s relative to t0 (twin);
+ * info(4, maxwin, ntrace) = weight, cc, time shift, dlnA.  3 <= nstep <= 16383; at most 16384
+ * candidates per trace survive the per-candidate passes (refused beyond) */
+void fwat_flexwin_batch_(int *nstep, int *ntrace, double *dt, realw *obs, realw *syn, double *t0, double *tp,
+                         double *dist, double *min_period, double *max_period, double *cfg, int *maxwin, int *nwin,
+                         double *windows, double *info);
 /* cross-convolution misfit of every receiver (imeas 2): WaveformConvMisfit%calc_adjoint_source
  * (src/preproc/adjoint_sources/waveform_conv_misfit.f90:17-64 with myconvolution_dp part 0,
  * src/shared/signal.f90:146-189) as src/preproc/telecc_data.f90:175-178 calls it; dat_r, dat_z, syn_r,

@@ -33,6 +33,8 @@
 //                         (exponentiated_phase_misfit.f90:21-104)
 //   orc_win_sel           win_sel_type%select_windows (window_selector/win_sel.f90:56-469,
 //                         sta_lta.f90:6-104)
+//   orc_flexwin           flexwin_type%select_windows (window_selector/flexwin.f90:75-734)
+//                         with schedule_weighted_intervals (interval_scheduling.f90:13-165)
 //   orc_conv_adjoint      WaveformConvMisfit%calc_adjoint_source (waveform_conv_misfit.f90:17-64)
 //                         with myconvolution_dp part 0 (signal.f90:146-189)
 //   orc_fft               fft_raw_dp / fft_raw (src/shared/fftpack.f90:70-100, 202-241)
@@ -44,6 +46,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <limits>
 #include <utility>
 #include <vector>
@@ -1070,6 +1073,301 @@ void orc_conv_adjoint(const double* dat_r, const double* dat_z, const double* sy
   out[1] = sm / nl;
   for (int i = 0; i < n; ++i) { adj_r[i] = 0.0; adj_z[i] = 0.0; }
   for (int i = 0; i < nl; ++i) { adj_r[nb - 1 + i] = tr[i]; adj_z[nb - 1 + i] = tz[i]; }
+}
+
+
+// schedule_weighted_intervals (interval_scheduling.f90:13-165): intervals
+// (left, right, weight) sorted by right end with the reference's quicksort
+// (middle pivot swapped to the end, ties within 1e-6 ordered by left end),
+// p(j) the last earlier interval ending at or before j's start (bisection),
+// opt(j) = max(w_j + opt(p(j)), opt(j - 1)), backtracking from the last one.
+// sel receives the chosen intervals' input positions in time order; returns
+// their count.
+int orc_schedule_intervals(int n, const double* left, const double* right, const double* weight, int* sel) {
+  struct Iv { int idx; double left, right, weight; };
+  std::vector<Iv> iv(n);
+  for (int i = 0; i < n; ++i) iv[i] = {i, left[i], right[i], weight[i]};
+  if (n == 0) return 0;
+  std::function<void(int, int)> qs = [&](int first, int last) {
+    if (first >= last) return;
+    const int pi = (first + last) / 2;
+    const Iv pv = iv[pi];
+    std::swap(iv[last], iv[pi]);
+    int i = first - 1;
+    for (int j = first; j <= last - 1; ++j) {
+      if (iv[j].right < pv.right - 1.0e-6 || (std::fabs(iv[j].right - pv.right) <= 1.0e-6 && iv[j].left <= pv.left)) {
+        ++i;
+        std::swap(iv[i], iv[j]);
+      }
+    }
+    std::swap(iv[i + 1], iv[last]);
+    qs(first, i);
+    qs(i + 2, last);
+  };
+  qs(0, n - 1);
+  std::vector<int> pj(n + 1, 0);
+  for (int j = 1; j <= n; ++j) {
+    int lo = 1, hi = j - 1, idx = 0;
+    while (lo <= hi) {
+      const int mid = (lo + hi) / 2;
+      if (iv[mid - 1].right <= iv[j - 1].left) { idx = mid; lo = mid + 1; }
+      else hi = mid - 1;
+    }
+    pj[j] = idx;
+  }
+  std::vector<double> opt(n + 1, 0.0);
+  for (int j = 1; j <= n; ++j) {
+    const double a = iv[j - 1].weight + opt[pj[j]], b = opt[j - 1];
+    opt[j] = a > b ? a : b;
+  }
+  std::vector<int> back;
+  for (int j = n; j > 0;) {
+    if (iv[j - 1].weight + opt[pj[j]] > opt[j - 1]) { back.push_back(iv[j - 1].idx); j = pj[j]; }
+    else j = j - 1;
+  }
+  const int ns = (int)back.size();
+  for (int k = 0; k < ns; ++k) sel[k] = back[ns - 1 - k];
+  return ns;
+}
+
+// STA_LTA of an envelope (for tests): out[n]
+void orc_sta_lta(const double* env, int n, double dt, double min_period, double* out) {
+  const std::vector<double> r = sta_lta(std::vector<double>(env, env + n), dt, min_period);
+  for (int i = 0; i < n; ++i) out[i] = r[i];
+}
+
+// FLEXWIN window selection of one trace (flexwin_type%select_windows,
+// src/preproc/window_selector/flexwin.f90:75-734, with schedule_weighted_intervals of
+// interval_scheduling.f90:13-165): STA/LTA of the synthetic's envelope, candidate
+// windows from every trough pair around each STA/LTA peak above the water level,
+// the rejection passes in the reference's order, the data-fit criteria of the
+// survivors, then merging (or, with is_split_phases, weighted interval scheduling).
+// cfg[16]: min_velocity, min_win_len_fac, threshold_corr, threshold_shift_fac,
+// min_peaks_troughs, threshold_dlna, min_snr_window, is_split_phases (0/1),
+// max_time_before_first_arrival, stalta_water_level, c_0, c_2, c_3a, c_3b, c_4a, c_4b.
+// twin[maxwin][2] (s, relative to t0 as the reference's twin), info[maxwin][4]:
+// weight, cc, time shift, dlnA.  Returns the window count (windows beyond maxwin
+// are counted, not written).  The noise level reads at most the trace's samples.
+struct FwCand {
+  int left, right, peak;
+  double cc, dlna, tshift, weight, snr_amp;
+  bool rejected;
+};
+
+int orc_flexwin(const double* dat_in, const double* syn_in, int npts, double dt, double t0, double tp, double dis,
+                double min_period, double max_period, const double* cfg, int maxwin, double* twin, double* info) {
+  // initialize (:75-108)
+  double smax = 0.0;
+  for (int i = 0; i < npts; ++i) smax = std::max(smax, std::fabs(syn_in[i]));
+  std::vector<double> dat(npts), syn(npts);
+  for (int i = 0; i < npts; ++i) { dat[i] = dat_in[i] / smax; syn[i] = syn_in[i] / smax; }
+  const double tstart = tp - cfg[8] + t0;
+  const double tend = dis / cfg[0] + max_period + t0;
+  const int nstart = (int)(tstart / dt) + 1;
+  const int nend = std::min(npts, (int)(tend / dt) + 1);
+  double noise;
+  if (nstart > 0) {
+    const int m = std::min(nstart, npts);
+    double e = 0.0;
+    for (int i = 0; i < m; ++i) e = e + dat_in[i] * dat_in[i];
+    noise = std::sqrt(e / (double)m);
+  } else {
+    int ntp = (int)((tp + t0) / dt) + 1;
+    if (ntp > 1) {
+      ntp = std::min(ntp, npts);
+      double e = 0.0;
+      for (int i = 0; i < ntp; ++i) e = e + dat_in[i] * dat_in[i];
+      noise = std::sqrt(e / (double)ntp);
+    } else {
+      noise = 0.001;
+    }
+  }
+  // calculate_criteria (:110-181) on samples [left, right] (1-based)
+  auto criteria = [&](FwCand& c) {
+    const int nl = c.right - c.left + 1;
+    std::vector<double> dw(dat.begin() + (c.left - 1), dat.begin() + c.right);
+    std::vector<double> sw(syn.begin() + (c.left - 1), syn.begin() + c.right);
+    const int ish = xcorr_shift(dw, sw);
+    c.tshift = ish * dt;
+    double sd2 = 0.0, ss2 = 0.0;
+    for (int i = 0; i < nl; ++i) sd2 = sd2 + dw[i] * dw[i];
+    for (int i = 0; i < nl; ++i) ss2 = ss2 + sw[i] * sw[i];
+    c.dlna = 0.5 * std::log(sd2 / ss2);
+    // cc_max_coef (utils/cc.f90:132-160)
+    std::vector<double> ssh(nl, 0.0), dsh(dw);
+    for (int i = 0; i < nl; ++i) {
+      const int j = i - ish;
+      if (j >= 0 && j < nl) ssh[i] = sw[j];
+    }
+    if (ish > 0) for (int i = 0; i < ish && i < nl; ++i) dsh[i] = 0.0;
+    else if (ish < 0) for (int i = nl + ish; i < nl; ++i) if (i >= 0) dsh[i] = 0.0;
+    double a = 0.0, b = 0.0, q = 0.0;
+    for (int i = 0; i < nl; ++i) a = a + dsh[i] * ssh[i];
+    for (int i = 0; i < nl; ++i) b = b + dsh[i] * dsh[i];
+    for (int i = 0; i < nl; ++i) q = q + ssh[i] * ssh[i];
+    c.cc = a / std::sqrt(b * q);
+    double amax = 0.0;
+    for (int i = 0; i < nl; ++i) amax = std::max(amax, std::fabs(dw[i]));
+    c.snr_amp = noise > 0.0 ? amax / noise : 1.0e5;
+    if (cfg[4] > 0.0 && count_maxima(sw.data(), nl, 1.0) + count_maxima(sw.data(), nl, -1.0) < (int)cfg[4])
+      c.rejected = true;
+    c.weight = (double)(c.right - c.left) * dt / min_period * c.cc;
+  };
+  // select_windows (:183-391)
+  const std::vector<std::complex<double>> xa = hilbert(syn);
+  std::vector<double> env(npts);
+  for (int i = 0; i < npts; ++i) env[i] = std::abs(xa[i]);
+  const std::vector<double> sl = sta_lta(env, dt, min_period);
+  const std::vector<int> peaks = maxima(sl, 1.0), troughs = maxima(sl, -1.0);
+  const int np = (int)peaks.size(), nt = (int)troughs.size();
+  if (np == 0 || nt < 2) return 0;
+  const int max_good_win = 500000;
+  std::vector<FwCand> cand;
+  for (int i = 0; i < np && (int)cand.size() < max_good_win; ++i) {
+    if (sl[peaks[i] - 1] < cfg[9]) continue;
+    for (int j = 0; j < nt && (int)cand.size() < max_good_win; ++j) {
+      if (troughs[j] >= peaks[i]) break;
+      for (int k = 0; k < nt; ++k) {
+        if (troughs[k] <= peaks[i]) continue;
+        if ((int)cand.size() >= max_good_win) break;
+        FwCand c{troughs[j], troughs[k], peaks[i], 0.0, 0.0, 0.0, 0.0, 0.0, false};
+        if ((double)(c.left - 1) * dt < tstart) c.left = nstart;
+        if ((double)(c.right - 1) * dt > tend) c.right = nend;
+        cand.push_back(c);
+      }
+    }
+  }
+  const int nc = (int)cand.size();
+  // 4.0 reject_on_traveltimes (:393-416)
+  const double min_time = tstart + cfg[8] - min_period, max_time = tend - max_period;
+  for (auto& c : cand) {
+    if (c.rejected) continue;
+    const double ws = (double)(c.left - 1) * dt, we = (double)(c.right - 1) * dt;
+    if (!(we >= min_time && ws <= max_time)) c.rejected = true;
+  }
+  // 4.1 minimum length
+  const auto min_length = [&]() {
+    for (auto& c : cand)
+      if (!c.rejected && (double)(c.right - c.left) * dt < cfg[1] * min_period) c.rejected = true;
+  };
+  min_length();
+  // 4.2 reject_on_minima_water_level (:418-446)
+  const double wmid = cfg[10] * cfg[9];
+  for (auto& c : cand) {
+    if (c.rejected) continue;
+    for (int j = 0; j < nt; ++j)
+      if (troughs[j] > c.left && troughs[j] < c.right && sl[troughs[j] - 1] <= wmid) { c.rejected = true; break; }
+  }
+  // 4.3 reject_on_prominence_of_central_peak (:448-499)
+  if (cfg[11] != 0.0) {
+    for (auto& c : cand) {
+      if (c.rejected) continue;
+      int lt = -1, rt = -1;
+      for (int j = nt - 1; j >= 0; --j)
+        if (troughs[j] < c.peak) { lt = troughs[j]; break; }
+      for (int j = 0; j < nt; ++j)
+        if (troughs[j] > c.peak) { rt = troughs[j]; break; }
+      if (lt == -1 || rt == -1) { c.rejected = true; continue; }
+      const double cv = sl[c.peak - 1], dl = cv - sl[lt - 1], dr = cv - sl[rt - 1];
+      if (dl < cfg[11] * cv || dr < cfg[11] * cv) c.rejected = true;
+    }
+  }
+  // 4.4 reject_on_phase_separation (:501-547)
+  for (auto& c : cand) {
+    if (c.rejected) continue;
+    double smin = std::numeric_limits<double>::max();
+    for (int j = 0; j < nt; ++j)
+      if (troughs[j] >= c.left && troughs[j] <= c.right && sl[troughs[j] - 1] < smin) smin = sl[troughs[j] - 1];
+    const double dcen = sl[c.peak - 1] - smin;
+    for (int j = 0; j < np; ++j) {
+      if (peaks[j] >= c.left && peaks[j] <= c.right && peaks[j] != c.peak) {
+        const double ds = sl[peaks[j] - 1] - smin;
+        const double dtm = (double)std::abs(c.peak - peaks[j]) * dt / min_period;
+        const double ft = dtm >= cfg[13] ? std::exp(-((dtm - cfg[13]) / cfg[13]) * ((dtm - cfg[13]) / cfg[13])) : 1.0;
+        if (ds > cfg[12] * dcen * ft) { c.rejected = true; break; }
+      }
+    }
+  }
+  // 4.5 curtail_length_of_windows (:549-601)
+  const double tdl = min_period * cfg[14] / dt, tdr = min_period * cfg[15] / dt;
+  for (auto& c : cand) {
+    if (c.rejected) continue;
+    int nin = 0, il = 0, ir = 0;
+    for (int j = 0; j < np; ++j)
+      if (peaks[j] >= c.left && peaks[j] <= c.right && peaks[j] != c.peak) {
+        if (nin == 0) il = peaks[j];
+        ir = peaks[j];
+        ++nin;
+      }
+    if (nin < 2) continue;
+    const double dl = (double)(il - c.left), dr = (double)(c.right - ir);
+    if (dl > tdl) c.left = (int)((double)il - tdl);
+    if (dr > tdr) c.right = (int)((double)ir + tdr);
+  }
+  // 4.6 remove_duplicates (:603-625)
+  for (int i = 0; i < nc; ++i) {
+    if (cand[i].rejected) continue;
+    cand[i].peak = (cand[i].left + cand[i].right) / 2;
+    for (int j = i + 1; j < nc; ++j)
+      if (!cand[j].rejected && cand[j].left == cand[i].left && cand[j].right == cand[i].right) cand[j].rejected = true;
+  }
+  // 4.7 minimum length again; 4.8 criteria and the data-fit rejections
+  min_length();
+  for (auto& c : cand) {
+    if (c.rejected) continue;
+    criteria(c);
+    if (c.cc < cfg[2]) c.rejected = true;
+    if (std::fabs(c.tshift) > cfg[3] * min_period) c.rejected = true;
+    if (std::fabs(c.dlna) > cfg[5]) c.rejected = true;
+    if (c.snr_amp < cfg[6]) c.rejected = true;
+  }
+  std::vector<FwCand> out;
+  if (cfg[7] == 0.0) {
+    // merge_windows (:627-700): valid candidates sorted by left (sort_candidates_by_left,
+    // :702-718), overlapping or touching windows merged, criteria recomputed
+    std::vector<FwCand> v;
+    for (auto& c : cand)
+      if (!c.rejected) v.push_back(c);
+    const int n = (int)v.size();
+    for (int i = 0; i < n - 1; ++i)
+      for (int j = i + 1; j < n; ++j)
+        if (v[j].left < v[i].left) std::swap(v[i], v[j]);
+    for (int i = 0; i < n; ++i) {
+      if (out.empty() || out.back().right + 1 < v[i].left) out.push_back(v[i]);
+      else out.back().right = std::max(out.back().right, v[i].right);
+    }
+    for (auto& c : out) {
+      c.peak = (c.left + c.right) / 2;
+      criteria(c);
+      c.rejected = false;
+    }
+  } else {
+    // schedule_weighted_intervals (interval_scheduling.f90:13-165)
+    std::vector<int> idx;
+    std::vector<double> lf, rt, wt;
+    for (int i = 0; i < nc; ++i)
+      if (!cand[i].rejected) {
+        idx.push_back(i);
+        lf.push_back((double)cand[i].left * dt);
+        rt.push_back((double)cand[i].right * dt);
+        wt.push_back(cand[i].weight);
+      }
+    const int n = (int)idx.size();
+    if (n == 0) return 0;
+    std::vector<int> sel(n);
+    const int ns = orc_schedule_intervals(n, lf.data(), rt.data(), wt.data(), sel.data());
+    for (int k = 0; k < ns; ++k) out.push_back(cand[idx[sel[k]]]);
+  }
+  const int nw = (int)out.size();
+  for (int i = 0; i < nw && i < maxwin; ++i) {
+    twin[2 * i] = (double)(out[i].left - 1) * dt - t0;
+    twin[2 * i + 1] = (double)(out[i].right - 1) * dt - t0;
+    info[4 * i] = out[i].weight;
+    info[4 * i + 1] = out[i].cc;
+    info[4 * i + 2] = out[i].tshift;
+    info[4 * i + 3] = out[i].dlna;
+  }
+  return nw;
 }
 
 }  // extern "C"

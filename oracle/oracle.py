@@ -150,6 +150,12 @@ def lib():
         L.orc_ep_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
                                      _dp, _dp]
         L.orc_dpss.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
+        L.orc_schedule_intervals.restype = C.c_int
+        L.orc_schedule_intervals.argtypes = [C.c_int, _dp, _dp, _dp, _ip]
+        L.orc_sta_lta.argtypes = [_dp, C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_flexwin.restype = C.c_int
+        L.orc_flexwin.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, _dp, C.c_int, _dp, _dp]
         L.orc_mt_adjoint.restype = C.c_double
         L.orc_mt_adjoint.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
                                      _dp, _dp, _dp, C.c_void_p]
@@ -710,6 +716,43 @@ def win_sel(dat, syn, dt, t0, tp, dis, min_period, maxwin=1000, **cfg):
     k = lib().orc_win_sel(_p(dat, C.c_double), _p(syn, C.c_double), n, dt, t0, tp, dis, min_period,
                           _p(cv, C.c_double), maxwin, _p(tw, C.c_double), _p(cc, C.c_double), _p(ts, C.c_double))
     return tw[:min(k, maxwin)].copy(), cc[:max(n - nlen, 0)], ts[:max(n - nlen, 0)]
+
+
+FLEXWIN_DEFAULTS = dict(min_velocity=2.4, min_win_len_fac=1.5, threshold_corr=0.8, threshold_shift_fac=2.0,
+                        min_peaks_troughs=3, threshold_dlna=1.0, min_snr_window=5.0, is_split_phases=0.0,
+                        max_time_before_first_arrival=10.0, stalta_water_level=0.08, c_0=1.0, c_2=0.0,
+                        c_3a=4.0, c_3b=2.5, c_4a=2.0, c_4b=6.0)  # input_params.f90:400-418 (WIN_TYPE_2)
+
+
+def flexwin(dat, syn, dt, t0, tp, dis, min_period, max_period, maxwin=1000, **cfg):
+    """flexwin_type%select_windows: (windows [n][2] in s, info [n][4] = weight,
+    cc, time shift, dlnA)."""
+    c = dict(FLEXWIN_DEFAULTS, **cfg)
+    cv = np.array([float(c[k]) for k in FLEXWIN_DEFAULTS], np.float64)
+    dat = np.ascontiguousarray(dat, np.float64)
+    syn = np.ascontiguousarray(syn, np.float64)
+    tw = np.zeros((maxwin, 2))
+    info = np.zeros((maxwin, 4))
+    k = lib().orc_flexwin(_p(dat, C.c_double), _p(syn, C.c_double), len(dat), dt, t0, tp, dis, min_period,
+                          max_period, _p(cv, C.c_double), maxwin, _p(tw, C.c_double), _p(info, C.c_double))
+    return tw[:min(k, maxwin)].copy(), info[:min(k, maxwin)].copy()
+
+
+def schedule_intervals(left, right, weight):
+    """schedule_weighted_intervals: input positions of the chosen intervals, in time order."""
+    lf, rt, wt = (np.ascontiguousarray(v, np.float64) for v in (left, right, weight))
+    sel = np.zeros(len(lf), np.int32)
+    k = lib().orc_schedule_intervals(len(lf), _p(lf, C.c_double), _p(rt, C.c_double), _p(wt, C.c_double),
+                                     _p(sel, C.c_int))
+    return sel[:k].copy()
+
+
+def sta_lta(env, dt, min_period):
+    """STA_LTA (sta_lta.f90:6-104) of an envelope."""
+    env = np.ascontiguousarray(env, np.float64)
+    out = np.zeros(len(env))
+    lib().orc_sta_lta(_p(env, C.c_double), len(env), dt, min_period, _p(out, C.c_double))
+    return out
 
 
 def conv_adjoint(dat_r, dat_z, syn_r, syn_z, dt, window, taper_pct=0.05, itaper=1):

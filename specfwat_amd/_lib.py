@@ -110,6 +110,7 @@ SIGNATURES = {
     "fwat_mt_adjoint_batch_": "iidFFiDdiiDiFDDF",
     "fwat_ep_adjoint_batch_": "iidFFiDdidiFDDF",
     "fwat_win_sel_batch_": "iidFFdDDdDiID",
+    "fwat_flexwin_batch_": "iidFFdDDddDiIDD",
     "fwat_conv_adjoint_batch_": "iidFFFFDdiDDFF",
 }
 

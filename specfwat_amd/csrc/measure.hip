@@ -2637,3 +2637,701 @@ void fwat_conv_adjoint_batch_(int* nstep, int* nrec, double* dt, realw* dat_r, r
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FLEXWIN window selection of every trace (flexwin_type%select_windows,
+// src/preproc/window_selector/flexwin.f90:75-734, dispatched for WIN_TYPE 2 at
+// leq_data.f90:116-121).  Stages, each a launch over all traces:
+//   k_fw_prep    normalisation by max|syn| and the noise level (one workgroup
+//                per trace; the noise sum on one lane in the reference's order)
+//   k_ws_env     envelope |hilbert(syn)| through the LDS real transform
+//   k_fw_stalta  STA/LTA on one lane (a recursion), then the STA/LTA peaks and
+//                troughs as ordered index lists (workgroup scans)
+//   k_fw_cand    every candidate (trough, peak, trough) in the reference's
+//                enumeration order, one lane per candidate: clipping to the
+//                time range and the per-candidate passes 4.0-4.5 (travel
+//                times, minimum length, minima water level, prominence, phase
+//                separation, curtailing); survivors compacted in order, then
+//                duplicates (the first of each (left, right) pair stays, peak
+//                recentred) and the second length pass
+//   k_fw_crit    one workgroup per surviving window: the full cross-correlation
+//                (one lag per lane, each lag's sum in the reference's order,
+//                first maximum), dlnA, cc_max_coef, SNR, peak/trough count
+//   k_fw_finish  one lane per trace: the data-fit rejections, then merging
+//                (windows sorted by left end, touching ones joined; criteria
+//                recomputed by k_fw_crit, written by k_fw_out) or weighted
+//                interval scheduling (quicksort by right end with the
+//                reference's pivot and 1e-6 ties, p(j) by bisection, optimum,
+//                backtrack)
+#pragma clang fp contract(off)
+namespace {
+
+constexpr int FW_THREADS = 256;
+constexpr int FW_CAP = 16384;       // candidates kept through the per-candidate passes, per trace
+constexpr int FW_MAXCAND = 500000;  // max_good_win (flexwin.f90:11)
+
+struct FwCfg {
+  double min_len;    // min_win_len_fac * min_period
+  double thr_corr;   // threshold_corr
+  double thr_shift;  // threshold_shift_fac * min_period
+  double thr_dlna, min_snr, wl, wmid, c2, c3a, c3b, tdl, tdr, min_period, dt;
+  int min_pt, split;
+};
+
+// per-trace time range: tstart, tend, min_time, max_time (s) and nstart, nend,
+// the number of samples of the noise level (0: fixed 0.001)
+struct FwTrace {
+  double tstart, tend, min_time, max_time;
+  int nstart, nend, nnoise, pad;
+};
+
+// exclusive prefix of v over the workgroup; *total gets the sum
+__device__ int fw_scan(int v, int* sh, int* total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < FW_THREADS; o <<= 1) {
+    const int x = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  *total = sh[FW_THREADS - 1];
+  const int r = sh[t] - v;
+  __syncthreads();
+  return r;
+}
+
+// first index in a[0..n) with a[i] >= v (a ascending)
+__device__ int fw_lower(const int* a, int n, int v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (a[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(FW_THREADS) k_fw_prep(int nstep, const FwTrace* __restrict__ tr,
+                                                       const float* __restrict__ obsT,
+                                                       const float* __restrict__ synT, double* __restrict__ Dn,
+                                                       double* __restrict__ Sn, double* __restrict__ noise) {
+  __shared__ double sh[FW_THREADS];
+  const int t = blockIdx.x;
+  const float* d = obsT + (size_t)t * nstep;
+  const float* s = synT + (size_t)t * nstep;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nstep; i += FW_THREADS) m = fmax(m, fabs((double)s[i]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = FW_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double smax = sh[0];
+  for (int i = threadIdx.x; i < nstep; i += FW_THREADS) {
+    Dn[(size_t)t * nstep + i] = (double)d[i] / smax;
+    Sn[(size_t)t * nstep + i] = (double)s[i] / smax;
+  }
+  if (threadIdx.x == 0) {  // initialize (flexwin.f90:92-106): the un-normalised data, summed in order
+    const int m2 = tr[t].nnoise;
+    double e = 0.0;
+    for (int i = 0; i < m2; ++i) e = e + (double)d[i] * (double)d[i];
+    noise[t] = m2 > 0 ? sqrt(e / (double)m2) : 0.001;
+  }
+}
+
+// STA_LTA (sta_lta.f90:6-104, TWEAK off) of the envelope on lane 0, then
+// find_maxima of STA/LTA and of its negative as 1-based index lists
+__global__ void __launch_bounds__(FW_THREADS) k_fw_stalta(int nstep, double Cs, double Cl,
+                                                         const double* __restrict__ env, double* __restrict__ SL,
+                                                         int* __restrict__ PK, int* __restrict__ TR,
+                                                         int* __restrict__ NPT) {
+  __shared__ double rm[FW_THREADS];
+  __shared__ int sh[FW_THREADS];
+  const int t = blockIdx.x, lane = threadIdx.x, cap = nstep / 2 + 1;
+  const double* e = env + (size_t)t * nstep;
+  double* sl = SL + (size_t)t * nstep;
+  double m = 0.0;
+  for (int i = lane; i < nstep; i += FW_THREADS) m = fmax(m, e[i]);
+  rm[lane] = m;
+  __syncthreads();
+  for (int o = FW_THREADS / 2; o > 0; o >>= 1) {
+    if (lane < o) rm[lane] = fmax(rm[lane], rm[lane + o]);
+    __syncthreads();
+  }
+  if (lane == 0) {
+    const double noise = rm[0] / 1e5;
+    double sta = 0.0, lta = 0.0;
+    for (int i = 0; i < 1000; ++i) {
+      sta = Cs * sta + noise;
+      lta = Cl * lta + noise;
+    }
+    for (int i = 0; i < nstep; ++i) {
+      const double x = e[i] + noise;
+      sta = Cs * sta + x;
+      lta = Cl * lta + x;
+      sl[i] = lta > 1.0e-9 ? sta / lta : noise;
+    }
+  }
+  __syncthreads();
+  int np = 0, nt = 0;
+  for (int base = 0; base < nstep; base += FW_THREADS) {
+    const int i = base + lane;
+    bool pk = false, tr = false;
+    if (i >= 1 && i < nstep - 1) {
+      const double a = sl[i - 1], b = sl[i], c = sl[i + 1];
+      pk = b - a > 0.0 && c - b < 0.0;
+      tr = (-b) - (-a) > 0.0 && (-c) - (-b) < 0.0;
+    }
+    int tot = 0;
+    const int op = fw_scan(pk ? 1 : 0, sh, &tot);
+    if (pk) PK[(size_t)t * cap + np + op] = i + 1;
+    np += tot;
+    const int ot = fw_scan(tr ? 1 : 0, sh, &tot);
+    if (tr) TR[(size_t)t * cap + nt + ot] = i + 1;
+    nt += tot;
+  }
+  if (lane == 0) {
+    NPT[2 * t] = np;
+    NPT[2 * t + 1] = nt;
+  }
+}
+
+// candidates and the passes before the data-fit criteria (flexwin.f90:205-294,
+// 393-625).  SV[t][k] = (left, right, peak) 1-based; NSV[t] = survivors, -1 when
+// the trace has no peak or fewer than two troughs, -2 when more than FW_CAP
+// candidates survive the per-candidate passes.
+__global__ void __launch_bounds__(FW_THREADS) k_fw_cand(int nstep, FwCfg c, const FwTrace* __restrict__ trc,
+                                                       const double* __restrict__ SL, const int* __restrict__ PK,
+                                                       const int* __restrict__ TR, const int* __restrict__ NPT,
+                                                       long long* __restrict__ PRE, int4* __restrict__ SV,
+                                                       int* __restrict__ NSV) {
+  __shared__ int sh[FW_THREADS];
+  __shared__ unsigned char keep[FW_CAP];
+  __shared__ long long total_s;
+  const int t = blockIdx.x, lane = threadIdx.x, cap = nstep / 2 + 1;
+  const int np = NPT[2 * t], nt = NPT[2 * t + 1];
+  if (np == 0 || nt < 2) {
+    if (lane == 0) NSV[t] = -1;
+    return;
+  }
+  const double* sl = SL + (size_t)t * nstep;
+  const int* pk = PK + (size_t)t * cap;
+  const int* tr = TR + (size_t)t * cap;
+  long long* pre = PRE + (size_t)t * (cap + 1);
+  int4* sv = SV + (size_t)t * FW_CAP;
+  const FwTrace T = trc[t];
+  if (lane == 0) {  // candidates per peak in the enumeration order, capped at max_good_win
+    long long acc = 0;
+    pre[0] = 0;
+    for (int i = 0; i < np; ++i) {
+      if (acc < FW_MAXCAND && !(sl[pk[i] - 1] < c.wl)) {
+        const int L = fw_lower(tr, nt, pk[i]);
+        acc += (long long)L * (nt - L);
+      }
+      pre[i + 1] = acc;
+    }
+    total_s = acc < FW_MAXCAND ? acc : FW_MAXCAND;
+  }
+  __syncthreads();
+  const long long total = total_s;
+  int ns = 0;
+  bool over = false;
+  for (long long base = 0; base < total; base += FW_THREADS) {
+    const long long cidx = base + lane;
+    bool ok = false;
+    int left = 0, right = 0, peak = 0;
+    if (cidx < total) {
+      int lo = 0, hi = np - 1;  // last peak with pre[i] <= cidx
+      while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (pre[m] <= cidx) lo = m;
+        else hi = m - 1;
+      }
+      const int i = lo;
+      peak = pk[i];
+      const int L = fw_lower(tr, nt, peak), R = nt - L;
+      const long long o = cidx - pre[i];
+      left = tr[(int)(o / R)];
+      right = tr[L + (int)(o % R)];
+      if ((double)(left - 1) * c.dt < T.tstart) left = T.nstart;
+      if ((double)(right - 1) * c.dt > T.tend) right = T.nend;
+      ok = true;
+      // 4.0 travel times
+      const double ws = (double)(left - 1) * c.dt, we = (double)(right - 1) * c.dt;
+      if (!(we >= T.min_time && ws <= T.max_time)) ok = false;
+      // 4.1 minimum length
+      if (ok && (double)(right - left) * c.dt < c.min_len) ok = false;
+      // 4.2 minima water level: troughs strictly inside
+      if (ok)
+        for (int j = fw_lower(tr, nt, left + 1); j < nt && tr[j] < right; ++j)
+          if (sl[tr[j] - 1] <= c.wmid) { ok = false; break; }
+      // 4.3 prominence of the central peak (skipped when c_2 = 0)
+      if (ok && c.c2 != 0.0) {
+        if (L == 0 || L == nt) ok = false;
+        else {
+          const double cv = sl[peak - 1], dl = cv - sl[tr[L - 1] - 1], dr = cv - sl[tr[L] - 1];
+          if (dl < c.c2 * cv || dr < c.c2 * cv) ok = false;
+        }
+      }
+      // 4.4 phase separation
+      if (ok) {
+        double smin = 1.7976931348623157e308;
+        for (int j = fw_lower(tr, nt, left); j < nt && tr[j] <= right; ++j)
+          if (sl[tr[j] - 1] < smin) smin = sl[tr[j] - 1];
+        const double dcen = sl[peak - 1] - smin;
+        for (int j = fw_lower(pk, np, left); j < np && pk[j] <= right; ++j) {
+          if (pk[j] == peak) continue;
+          const double ds = sl[pk[j] - 1] - smin;
+          const double dtm = (double)abs(peak - pk[j]) * c.dt / c.min_period;
+          double ft = 1.0;
+          if (dtm >= c.c3b) {
+            const double a = (dtm - c.c3b) / c.c3b;
+            ft = exp(-a * a);
+          }
+          if (ds > c.c3a * dcen * ft) { ok = false; break; }
+        }
+      }
+      // 4.5 curtail to the internal maxima
+      if (ok) {
+        int nin = 0, il = 0, ir = 0;
+        for (int j = fw_lower(pk, np, left); j < np && pk[j] <= right; ++j) {
+          if (pk[j] == peak) continue;
+          if (nin == 0) il = pk[j];
+          ir = pk[j];
+          ++nin;
+        }
+        if (nin >= 2) {
+          const double dl = (double)(il - left), dr = (double)(right - ir);
+          if (dl > c.tdl) left = (int)((double)il - c.tdl);
+          if (dr > c.tdr) right = (int)((double)ir + c.tdr);
+        }
+      }
+    }
+    int tot = 0;
+    const int off = fw_scan(ok ? 1 : 0, sh, &tot);
+    if (ok) {
+      if (ns + off < FW_CAP) sv[ns + off] = make_int4(left, right, peak, 0);
+    }
+    if (ns + tot > FW_CAP) over = true;
+    ns = min(ns + tot, FW_CAP);
+  }
+  if (over) {
+    if (lane == 0) NSV[t] = -2;
+    return;
+  }
+  // 4.6 duplicates: a pair stays with its first candidate; 4.7 minimum length
+  for (int k = lane; k < ns; k += FW_THREADS) {
+    const int4 a = sv[k];
+    bool dup = false;
+    for (int q = 0; q < k && !dup; ++q) {
+      const int4 b = sv[q];
+      dup = b.x == a.x && b.y == a.y;
+    }
+    keep[k] = (!dup && !((double)(a.y - a.x) * c.dt < c.min_len)) ? 1 : 0;
+  }
+  __syncthreads();
+  int nk = 0;
+  for (int base = 0; base < ns; base += FW_THREADS) {
+    const int k = base + lane;
+    int4 a = make_int4(0, 0, 0, 0);
+    const bool kp = k < ns && keep[k];
+    if (kp) a = sv[k];
+    int tot = 0;
+    const int off = fw_scan(kp ? 1 : 0, sh, &tot);
+    if (kp) sv[nk + off] = make_int4(a.x, a.y, (a.x + a.y) / 2, 0);
+    nk += tot;
+    __syncthreads();
+  }
+  if (lane == 0) NSV[t] = nk;
+}
+
+// calculate_criteria (flexwin.f90:110-181) of window e of trace blockIdx.y:
+// CR[t][e] = cc, dlnA, time shift, weight, SNR (amplitude), rejected by the
+// peak/trough count (0/1)
+__global__ void __launch_bounds__(FW_THREADS) k_fw_crit(int nstep, int stride, FwCfg c,
+                                                       const double* __restrict__ Dn,
+                                                       const double* __restrict__ Sn,
+                                                       const double* __restrict__ noise,
+                                                       const int4* __restrict__ W, const int* __restrict__ NW,
+                                                       double* __restrict__ CR) {
+  __shared__ double bv[FW_THREADS];
+  __shared__ int bi[FW_THREADS];
+  const int t = blockIdx.y, e = blockIdx.x, lane = threadIdx.x;
+  if (e >= NW[t]) return;
+  const int4 w = W[(size_t)t * stride + e];
+  const int nl = w.y - w.x + 1;
+  const double* d = Dn + (size_t)t * nstep + (w.x - 1);
+  const double* s = Sn + (size_t)t * nstep + (w.x - 1);
+  // mycorrelation_dp(d, s, cc, 1): lag m sums d(i1) s(nl - (m - i1)) over increasing i1
+  double best = -INFINITY;
+  int bm = 0x7fffffff;
+  for (int m = lane; m < 2 * nl - 1; m += FW_THREADS) {
+    const int lo = m - (nl - 1) > 0 ? m - (nl - 1) : 0, hi = m < nl - 1 ? m : nl - 1;
+    double acc = 0.0;
+    for (int i1 = lo; i1 <= hi; ++i1) acc = acc + d[i1] * s[nl - 1 - (m - i1)];
+    if (acc > best) { best = acc; bm = m; }
+  }
+  bv[lane] = best;
+  bi[lane] = bm;
+  __syncthreads();
+  for (int o = FW_THREADS / 2; o > 0; o >>= 1) {
+    if (lane < o) {
+      const double v2 = bv[lane + o];
+      const int i2 = bi[lane + o];
+      if (v2 > bv[lane] || (v2 == bv[lane] && i2 < bi[lane])) { bv[lane] = v2; bi[lane] = i2; }
+    }
+    __syncthreads();
+  }
+  if (lane != 0) return;
+  const int ish = bi[0] + 1 - nl;
+  double sd2 = 0.0, ss2 = 0.0;
+  for (int i = 0; i < nl; ++i) sd2 = sd2 + d[i] * d[i];
+  for (int i = 0; i < nl; ++i) ss2 = ss2 + s[i] * s[i];
+  const double dlna = 0.5 * log(sd2 / ss2);
+  // cc_max_coef (utils/cc.f90:132-160): s shifted by ish, d's emptied edge zeroed
+  double a = 0.0, b = 0.0, q = 0.0, amax = 0.0;
+  for (int i = 0; i < nl; ++i) {
+    const int j = i - ish;
+    const double sv = (j >= 0 && j < nl) ? s[j] : 0.0;
+    const bool z = (ish > 0 && i < ish) || (ish < 0 && i >= nl + ish);
+    const double dv = z ? 0.0 : d[i];
+    a = a + dv * sv;
+    b = b + dv * dv;
+    q = q + sv * sv;
+    amax = fmax(amax, fabs(d[i]));
+  }
+  const double cc = a / sqrt(b * q);
+  int npk = 0;
+  for (int i = 1; i < nl - 1; ++i) {
+    if (s[i] - s[i - 1] > 0.0 && s[i + 1] - s[i] < 0.0) ++npk;
+    if ((-s[i]) - (-s[i - 1]) > 0.0 && (-s[i + 1]) - (-s[i]) < 0.0) ++npk;
+  }
+  double* o = CR + 6 * ((size_t)t * stride + e);
+  o[0] = cc;
+  o[1] = dlna;
+  o[2] = ish * c.dt;
+  o[3] = (double)(w.y - w.x) * c.dt / c.min_period * cc;
+  o[4] = noise[t] > 0.0 ? amax / noise[t] : 1.0e5;
+  o[5] = (c.min_pt > 0 && npk < c.min_pt) ? 1.0 : 0.0;
+}
+
+// data-fit rejections (flexwin.f90:270-293), then merge_windows (:627-700) into
+// MW/NM (merging mode) or schedule_weighted_intervals (interval_scheduling.f90)
+// straight into the outputs; one lane per trace.  Scratch rows per trace of
+// `stride` entries: IX (int), IV (left, right, weight as double), PJ (int),
+// OPT (stride + 1 doubles); STK holds 2 stride + 2 ranges (int2) per trace.
+__global__ void k_fw_finish(int ntrace, int stride, FwCfg c, double t0, const int4* __restrict__ SV,
+                            const int* __restrict__ NSV, const double* __restrict__ CR, int* __restrict__ IX,
+                            double* __restrict__ IV, int* __restrict__ PJ, double* __restrict__ OPT,
+                            int2* __restrict__ STK, int4* __restrict__ MW, int* __restrict__ NM, int maxwin,
+                            int* __restrict__ nwin, double* __restrict__ win, double* __restrict__ info) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrace) return;
+  const int n0 = NSV[t];
+  nwin[t] = 0;
+  NM[t] = 0;
+  if (n0 <= 0) return;
+  const int4* sv = SV + (size_t)t * stride;
+  const double* cr = CR + 6 * (size_t)t * stride;
+  int* ix = IX + (size_t)t * stride;
+  int nv = 0;
+  for (int e = 0; e < n0; ++e) {
+    const double* o = cr + 6 * e;
+    bool rej = o[5] != 0.0;
+    if (o[0] < c.thr_corr) rej = true;
+    if (fabs(o[2]) > c.thr_shift) rej = true;
+    if (fabs(o[1]) > c.thr_dlna) rej = true;
+    if (o[4] < c.min_snr) rej = true;
+    if (!rej) ix[nv++] = e;
+  }
+  if (!c.split) {
+    // sort_candidates_by_left, then merge
+    for (int i = 0; i < nv - 1; ++i)
+      for (int j = i + 1; j < nv; ++j)
+        if (sv[ix[j]].x < sv[ix[i]].x) { const int tmp = ix[i]; ix[i] = ix[j]; ix[j] = tmp; }
+    int4* mw = MW + (size_t)t * stride;
+    int nm = 0;
+    for (int i = 0; i < nv; ++i) {
+      const int4 w = sv[ix[i]];
+      if (nm == 0 || mw[nm - 1].y + 1 < w.x) mw[nm++] = w;
+      else mw[nm - 1].y = max(mw[nm - 1].y, w.y);
+    }
+    for (int i = 0; i < nm; ++i) mw[i].z = (mw[i].x + mw[i].y) / 2;
+    NM[t] = nm;
+    return;
+  }
+  NM[t] = 0;
+  if (nv == 0) {
+    nwin[t] = 0;
+    return;
+  }
+  // intervals (left, right in s as left_idx dt, right_idx dt; weight)
+  double* iv = IV + 3 * (size_t)t * stride;
+  int* id = PJ + (size_t)t * stride;  // original candidate of each interval, then p(j)
+  for (int k = 0; k < nv; ++k) {
+    const int4 w = sv[ix[k]];
+    iv[3 * k] = (double)w.x * c.dt;
+    iv[3 * k + 1] = (double)w.y * c.dt;
+    iv[3 * k + 2] = cr[6 * ix[k] + 3];
+  }
+  for (int k = 0; k < nv; ++k) id[k] = ix[k];
+  auto swp = [&](int a, int b) {
+    for (int q = 0; q < 3; ++q) { const double x = iv[3 * a + q]; iv[3 * a + q] = iv[3 * b + q]; iv[3 * b + q] = x; }
+    const int y = id[a]; id[a] = id[b]; id[b] = y;
+  };
+  // quicksort (interval_scheduling.f90:130-163) with an explicit stack of ranges
+  const int sstride = 2 * stride + 2;
+  int2* stk = STK + (size_t)t * sstride;
+  int sp = 0;
+  stk[sp++] = make_int2(0, nv - 1);
+  while (sp > 0) {
+    const int2 r = stk[--sp];
+    const int first = r.x, last = r.y;
+    if (first >= last) continue;
+    const int pi = (first + last) / 2;
+    const double pl = iv[3 * pi], pr = iv[3 * pi + 1];
+    swp(last, pi);
+    int i = first - 1;
+    for (int j = first; j <= last - 1; ++j) {
+      const double l = iv[3 * j], rr = iv[3 * j + 1];
+      if (rr < pr - 1.0e-6 || (fabs(rr - pr) <= 1.0e-6 && l <= pl)) {
+        ++i;
+        swp(i, j);
+      }
+    }
+    swp(i + 1, last);
+    if (sp + 2 > sstride) break;  // cannot happen: pending ranges are disjoint, plus one empty one
+    stk[sp++] = make_int2(i + 2, last);
+    stk[sp++] = make_int2(first, i);
+  }
+  // p(j) (1-based) by bisection; orig index kept in ix
+  for (int k = 0; k < nv; ++k) ix[k] = id[k];
+  double* opt = OPT + (size_t)t * (stride + 1);
+  opt[0] = 0.0;
+  for (int j = 1; j <= nv; ++j) {
+    int lo = 1, hi = j - 1, p = 0;
+    const double v = iv[3 * (j - 1)];
+    while (lo <= hi) {
+      const int mid = (lo + hi) / 2;
+      if (iv[3 * (mid - 1) + 1] <= v) { p = mid; lo = mid + 1; }
+      else hi = mid - 1;
+    }
+    id[j - 1] = p;
+    const double a = iv[3 * (j - 1) + 2] + opt[p], b = opt[j - 1];
+    opt[j] = a > b ? a : b;
+  }
+  // backtrack from the last interval, then reverse into time order
+  int ns = 0;
+  int* selr = reinterpret_cast<int*>(stk);  // the stack is free again
+  for (int j = nv; j > 0;) {
+    if (iv[3 * (j - 1) + 2] + opt[id[j - 1]] > opt[j - 1]) { selr[ns++] = ix[j - 1]; j = id[j - 1]; }
+    else j = j - 1;
+  }
+  nwin[t] = ns;
+  for (int k = 0; k < ns && k < maxwin; ++k) {
+    const int e = selr[ns - 1 - k];
+    const int4 w = sv[e];
+    double* wo = win + 2 * ((size_t)t * maxwin + k);
+    wo[0] = (double)(w.x - 1) * c.dt - t0;
+    wo[1] = (double)(w.y - 1) * c.dt - t0;
+    double* io = info + 4 * ((size_t)t * maxwin + k);
+    io[0] = cr[6 * e + 3];
+    io[1] = cr[6 * e];
+    io[2] = cr[6 * e + 2];
+    io[3] = cr[6 * e + 1];
+  }
+}
+
+// outputs of the merging mode: merged windows with their recomputed criteria
+__global__ void k_fw_out(int ntrace, int stride, FwCfg c, double t0, const int4* __restrict__ MW,
+                         const int* __restrict__ NM, const double* __restrict__ CRM, int maxwin,
+                         int* __restrict__ nwin, double* __restrict__ win, double* __restrict__ info) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrace || c.split) return;
+  const int nm = NM[t];
+  nwin[t] = nm;
+  for (int k = 0; k < nm && k < maxwin; ++k) {
+    const int4 w = MW[(size_t)t * stride + k];
+    const double* o = CRM + 6 * ((size_t)t * stride + k);
+    double* wo = win + 2 * ((size_t)t * maxwin + k);
+    wo[0] = (double)(w.x - 1) * c.dt - t0;
+    wo[1] = (double)(w.y - 1) * c.dt - t0;
+    double* io = info + 4 * ((size_t)t * maxwin + k);
+    io[0] = o[3];
+    io[1] = o[0];
+    io[2] = o[2];
+    io[3] = o[1];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// FLEXWIN window selection (flexwin_type, src/preproc/window_selector/flexwin.f90)
+// of every trace: obs, syn [nstep][ntrace] (filtered as leq_data.f90:93-105 does),
+// tp, dist (ntrace) first-arrival time and distance, t0, min/max period of the
+// band; cfg(16) = min_velocity, min_win_len_fac, threshold_corr,
+// threshold_shift_fac, min_peaks_troughs, threshold_dlna, min_snr_window,
+// is_split_phases (0/1), max_time_before_first_arrival, stalta_water_level, c_0,
+// c_2, c_3a, c_3b, c_4a, c_4b (input_params.f90:400-418).  nwin(ntrace),
+// windows(2, maxwin, ntrace) in s relative to t0 (the reference's twin),
+// info(4, maxwin, ntrace) = weight, cc, time shift, dlnA.  A trace keeps at most
+// 16384 candidates through the per-candidate passes (refused beyond).
+void fwat_flexwin_batch_(int* nstep, int* ntrace, double* dt, realw* obs, realw* syn, double* t0, double* tp,
+                         double* dist, double* min_period, double* max_period, double* cfg, int* maxwin, int* nwin,
+                         double* windows, double* info) {
+  const int N = *nstep, T = *ntrace;
+  if (N < 3 || T <= 0) SEM_FAIL("fwat_flexwin_batch_: nstep >= 3 and ntrace > 0 needed");
+  if (N > 16383) SEM_FAIL("fwat_flexwin_batch_: traces of at most 16383 samples (envelope transform in LDS)");
+  if (T > 65535) SEM_FAIL("fwat_flexwin_batch_: at most 65535 traces per call");
+  if (*maxwin < 1) SEM_FAIL("fwat_flexwin_batch_: maxwin must be positive");
+  if (!(*min_period > 0.0) || !(*dt > 0.0)) SEM_FAIL("fwat_flexwin_batch_: dt and min_period must be positive");
+  if (!(cfg[1] >= 0.0) || !(cfg[0] > 0.0)) SEM_FAIL("fwat_flexwin_batch_: min_win_len_fac >= 0, min_velocity > 0");
+  const double mp = *min_period, Mp = *max_period, h = *dt;
+  FwCfg C;
+  C.min_len = cfg[1] * mp;
+  C.thr_corr = cfg[2];
+  C.thr_shift = cfg[3] * mp;
+  C.min_pt = (int)cfg[4];
+  C.thr_dlna = cfg[5];
+  C.min_snr = cfg[6];
+  C.split = cfg[7] != 0.0 ? 1 : 0;
+  C.wl = cfg[9];
+  C.wmid = cfg[10] * cfg[9];
+  C.c2 = cfg[11];
+  C.c3a = cfg[12];
+  C.c3b = cfg[13];
+  C.tdl = mp * cfg[14] / h;
+  C.tdr = mp * cfg[15] / h;
+  C.min_period = mp;
+  C.dt = h;
+  // per-trace time range (initialize, flexwin.f90:86-106; reject_on_traveltimes :402-403)
+  std::vector<FwTrace> trc(T);
+  for (int t = 0; t < T; ++t) {
+    FwTrace& r = trc[t];
+    r.tstart = tp[t] - cfg[8] + *t0;
+    r.tend = dist[t] / cfg[0] + Mp + *t0;
+    r.nstart = (int)(r.tstart / h) + 1;
+    r.nend = std::min(N, (int)(r.tend / h) + 1);
+    r.min_time = r.tstart + cfg[8] - mp;
+    r.max_time = r.tend - Mp;
+    if (r.nstart > 0) {
+      r.nnoise = std::min(r.nstart, N);
+    } else {
+      const int ntp = (int)((tp[t] + *t0) / h) + 1;
+      r.nnoise = ntp > 1 ? std::min(ntp, N) : 0;
+    }
+    r.pad = 0;
+  }
+  const double Cs = std::pow(10.0, -h / mp), Cl = std::pow(10.0, -h / (12.0 * mp));
+  int e2 = 0;
+  std::frexp((float)N, &e2);
+  const int nfe = 1 << e2, Me = nfe / 2;
+  int logMe = 0;
+  while ((1 << logMe) < Me) ++logMe;
+  std::vector<double> twh(2 * (size_t)Me);
+  for (int j = 0; j < Me; ++j) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)nfe;
+    twh[2 * j] = (double)std::cos(a);
+    twh[2 * j + 1] = (double)std::sin(a);
+  }
+  const size_t n = (size_t)N * T;
+  const int cap = N / 2 + 1;
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  float* d_obs = dev_upload(obs, n, s);
+  float* d_syn = dev_upload(syn, n, s);
+  FwTrace* d_trc = reinterpret_cast<FwTrace*>(dev_upload(reinterpret_cast<double*>(trc.data()),
+                                                         trc.size() * sizeof(FwTrace) / sizeof(double), s));
+  double2* d_twe = reinterpret_cast<double2*>(dev_upload(twh.data(), twh.size(), s));
+  float* d_obsT = dev_alloc<float>(n);
+  float* d_synT = dev_alloc<float>(n);
+  double* d_Dn = dev_alloc<double>(n);
+  double* d_Sn = dev_alloc<double>(n);
+  double* d_env = dev_alloc<double>(n);
+  double* d_sl = dev_alloc<double>(n);
+  double* d_noise = dev_alloc<double>(T);
+  double2* d_g01 = dev_alloc<double2>(2 * (size_t)T * (Me + 1));
+  int* d_pk = dev_alloc<int>((size_t)T * cap);
+  int* d_tr = dev_alloc<int>((size_t)T * cap);
+  int* d_npt = dev_alloc<int>(2 * (size_t)T);
+  long long* d_pre = dev_alloc<long long>((size_t)T * (cap + 1));
+  int4* d_sv = dev_alloc<int4>((size_t)T * FW_CAP);
+  int* d_nsv = dev_alloc<int>(T);
+  int* d_nwin = dev_alloc<int>(T);
+  int* d_nm = dev_alloc<int>(T);
+  double* d_win = dev_alloc<double>(2 * (size_t)*maxwin * T);
+  double* d_info = dev_alloc<double>(4 * (size_t)*maxwin * T);
+  HIP_CHECK(hipEventRecord(ev0, s));
+  const dim3 ttiles((unsigned)((T + 63) / 64), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_obs, d_obsT);
+  hipLaunchKernelGGL(k_rows_f32, ttiles, dim3(256), 0, s, N, T, d_syn, d_synT);
+  hipLaunchKernelGGL(k_fw_prep, dim3((unsigned)T), dim3(FW_THREADS), 0, s, N, d_trc, d_obsT, d_synT, d_Dn, d_Sn,
+                     d_noise);
+  hipLaunchKernelGGL(k_ws_env, dim3((unsigned)T), dim3(MT_THREADS), 0, s, N, nfe, logMe, d_twe, d_Sn, d_g01,
+                     d_g01 + (size_t)T * (Me + 1), d_env);
+  hipLaunchKernelGGL(k_fw_stalta, dim3((unsigned)T), dim3(FW_THREADS), 0, s, N, Cs, Cl, d_env, d_sl, d_pk, d_tr,
+                     d_npt);
+  hipLaunchKernelGGL(k_fw_cand, dim3((unsigned)T), dim3(FW_THREADS), 0, s, N, C, d_trc, d_sl, d_pk, d_tr, d_npt,
+                     d_pre, d_sv, d_nsv);
+  HIP_CHECK(hipGetLastError());
+  std::vector<int> nsv(T);
+  HIP_CHECK(hipMemcpyAsync(nsv.data(), d_nsv, T * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  int maxs = 1;
+  for (int t = 0; t < T; ++t) {
+    if (nsv[t] == -2) SEM_FAIL("fwat_flexwin_batch_: trace %d keeps more than %d candidates", t, FW_CAP);
+    maxs = std::max(maxs, nsv[t]);
+  }
+  // criteria of the survivors; then rejections and merging or scheduling
+  double* d_cr = dev_alloc<double>(6 * (size_t)T * maxs);
+  int* d_ix = dev_alloc<int>((size_t)T * maxs);
+  double* d_iv = dev_alloc<double>(3 * (size_t)T * maxs);
+  int* d_pj = dev_alloc<int>((size_t)T * maxs);
+  double* d_opt = dev_alloc<double>((size_t)T * (maxs + 1));
+  int2* d_stk = dev_alloc<int2>((size_t)T * (2 * (size_t)maxs + 2));
+  int4* d_mw = dev_alloc<int4>((size_t)T * maxs);
+  double* d_crm = dev_alloc<double>(6 * (size_t)T * maxs);
+  // the survivors' rows were written with stride FW_CAP; the criteria rows use maxs
+  int4* d_svc = dev_alloc<int4>((size_t)T * maxs);
+  HIP_CHECK(hipMemcpy2DAsync(d_svc, (size_t)maxs * sizeof(int4), d_sv, (size_t)FW_CAP * sizeof(int4),
+                             (size_t)maxs * sizeof(int4), (size_t)T, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_fw_crit, dim3((unsigned)maxs, (unsigned)T), dim3(FW_THREADS), 0, s, N, maxs, C, d_Dn, d_Sn,
+                     d_noise, d_svc, d_nsv, d_cr);
+  const unsigned nb = (unsigned)((T + 63) / 64);
+  hipLaunchKernelGGL(k_fw_finish, dim3(nb), dim3(64), 0, s, T, maxs, C, *t0, d_svc, d_nsv, d_cr, d_ix, d_iv, d_pj,
+                     d_opt, d_stk, d_mw, d_nm, *maxwin, d_nwin, d_win, d_info);
+  if (!C.split) {
+    hipLaunchKernelGGL(k_fw_crit, dim3((unsigned)maxs, (unsigned)T), dim3(FW_THREADS), 0, s, N, maxs, C, d_Dn, d_Sn,
+                       d_noise, d_mw, d_nm, d_crm);
+    hipLaunchKernelGGL(k_fw_out, dim3(nb), dim3(64), 0, s, T, maxs, C, *t0, d_mw, d_nm, d_crm, *maxwin, d_nwin, d_win,
+                       d_info);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(ev1, s));
+  HIP_CHECK(hipMemcpyAsync(nwin, d_nwin, T * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(windows, d_win, 2 * (size_t)*maxwin * T * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(info, d_info, 4 * (size_t)*maxwin * T * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+  g_meas_ms = ms;
+  dev_free(d_obs); dev_free(d_syn); dev_free(d_trc); dev_free(d_twe); dev_free(d_obsT); dev_free(d_synT);
+  dev_free(d_Dn); dev_free(d_Sn); dev_free(d_env); dev_free(d_sl); dev_free(d_noise); dev_free(d_g01);
+  dev_free(d_pk); dev_free(d_tr); dev_free(d_npt); dev_free(d_pre); dev_free(d_sv); dev_free(d_nsv);
+  dev_free(d_nwin); dev_free(d_nm); dev_free(d_win); dev_free(d_info); dev_free(d_cr); dev_free(d_ix);
+  dev_free(d_iv); dev_free(d_pj); dev_free(d_opt); dev_free(d_stk); dev_free(d_mw); dev_free(d_crm);
+  dev_free(d_svc);
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  HIP_CHECK(hipStreamDestroy(s));
+}
+
+}  // extern "C"

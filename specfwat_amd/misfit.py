@@ -339,6 +339,51 @@ def select_windows(obs, syn, dt, t0, tp, dist, min_period, maxwin=64, **cfg):
     return [win[t, :nwin[t]].copy() for t in range(ntrace)]
 
 
+FLEXWIN_DEFAULTS = dict(min_velocity=2.4, min_win_len_fac=1.5, threshold_corr=0.8, threshold_shift_fac=2.0,
+                        min_peaks_troughs=3, threshold_dlna=1.0, min_snr_window=5.0, is_split_phases=False,
+                        max_time_before_first_arrival=10.0, stalta_water_level=0.08, c_0=1.0, c_2=0.0,
+                        c_3a=4.0, c_3b=2.5, c_4a=2.0, c_4b=6.0)  # input_params.f90:400-418 (WIN_TYPE_2)
+
+
+def flexwin_windows(obs, syn, dt, t0, tp, dist, min_period, max_period, maxwin=64, **cfg):
+    """FLEXWIN window selection (WIN_TYPE 2: flexwin_type%select_windows,
+    window_selector/flexwin.f90:75-734, as leq_data.f90:116-121 calls it) for
+    every trace of an event in one device pass: STA/LTA of the synthetic's
+    envelope, candidate windows between STA/LTA troughs around each peak above the
+    water level, the rejection passes (travel times, length, minima water level,
+    prominence, phase separation, curtailing, duplicates), the data-fit criteria
+    (cross-correlation shift, cc_max_coef, dlnA, SNR, peak count), then merging
+    or, with is_split_phases, weighted interval scheduling.  obs, syn (NSTEP,
+    nrec, 3) filtered as the caller does; tp, dist (nrec,) first-arrival times
+    and distances; min/max_period the band.  Returns (windows, info): per trace
+    (nrec*3, receiver-major) an [n, 2] array of window times (s, relative to t0)
+    and an [n, 4] array of weight, cc, time shift, dlnA."""
+    o = np.ascontiguousarray(obs, np.float32)
+    s = np.ascontiguousarray(syn, np.float32)
+    if o.shape != s.shape:
+        raise ValueError("observed and synthetic traces differ in shape")
+    unknown = set(cfg) - set(FLEXWIN_DEFAULTS)
+    if unknown:
+        raise KeyError(f"unknown FLEXWIN settings {sorted(unknown)}")
+    c = dict(FLEXWIN_DEFAULTS, **cfg)
+    nstep = s.shape[0]
+    ntrace = s.size // nstep
+    ncomp = ntrace // np.size(tp)
+    tpv = np.ascontiguousarray(np.repeat(np.asarray(tp, np.float64), ncomp))
+    dv = np.ascontiguousarray(np.repeat(np.asarray(dist, np.float64), ncomp))
+    if tpv.size != ntrace or dv.size != ntrace:
+        raise ValueError("tp and dist need one value per receiver")
+    cv = np.array([float(c[k]) for k in FLEXWIN_DEFAULTS], np.float64)
+    nwin = np.zeros(ntrace, np.int32)
+    win = np.zeros((ntrace, maxwin, 2))
+    info = np.zeros((ntrace, maxwin, 4))
+    _lib.call("fwat_flexwin_batch_", nstep, ntrace, float(dt), o, s, float(t0), tpv, dv, float(min_period),
+              float(max_period), cv, int(maxwin), nwin, win, info)
+    if (nwin > maxwin).any():
+        raise RuntimeError(f"a trace has more than maxwin={maxwin} windows; raise maxwin")
+    return ([win[t, :nwin[t]].copy() for t in range(ntrace)], [info[t, :nwin[t]].copy() for t in range(ntrace)])
+
+
 def event_conv_adjoint_sources(dat_r, dat_z, syn_r, syn_z, dt, windows, taper_percentage=0.05, itaper_type=1):
     """Cross-convolution misfit (imeas 2, WaveformConvMisfit%calc_adjoint_source,
     waveform_conv_misfit.f90:17-64, as telecc_data.f90:175-178 calls it) of every

@@ -268,6 +268,41 @@ def test_window_selection_matches_oracle(split, len_fac):
     print(f"split {split} len_fac {len_fac}: windows per trace: {counts}, selection {misfit.last_time_ms():.2f} ms")
 
 
+@pytest.mark.parametrize("split,nstep", [(0.0, 4000), (1.0, 4000), (1.0, 3999)])
+def test_flexwin_matches_oracle(split, nstep):
+    """FLEXWIN (WIN_TYPE 2) for every trace: the same windows, weights, cc,
+    shifts and dlnA as the oracle, with merging and with interval scheduling,
+    on even and odd trace lengths; a noise-only trace selects nothing."""
+    rng = np.random.default_rng(23)
+    nrec, dt = 4, 0.05
+    t = np.arange(nstep) * dt
+    w = lambda t0, f, a: a * np.exp(-((t - t0) * f / 2) ** 2) * np.sin(2 * np.pi * f * (t - t0))
+    obs = np.zeros((nstep, nrec, 3), np.float32)
+    syn = np.zeros_like(obs)
+    tp = rng.uniform(20, 30, nrec)
+    dist = rng.uniform(200, 260, nrec)
+    for r in range(nrec):
+        for c in range(3):
+            arr = tp[r] + np.array([5.0, 28.0, 62.0]) + rng.uniform(-2, 2, 3)
+            f = rng.uniform(0.18, 0.32, 3)
+            a = rng.uniform(0.6, 1.6, 3)
+            syn[:, r, c] = sum(w(arr[k], f[k], a[k]) for k in range(3))
+            obs[:, r, c] = sum(w(arr[k] + rng.uniform(-0.6, 0.6), f[k], a[k] * rng.uniform(0.8, 1.25))
+                               for k in range(3)) + 0.01 * rng.standard_normal(nstep)
+    obs[:, 3, 2] = 0.01 * rng.standard_normal(nstep)  # nothing fits
+    got, ginfo = misfit.flexwin_windows(obs, syn, dt, 0.0, tp, dist, 2.0, 20.0, is_split_phases=bool(split))
+    counts = []
+    for r in range(nrec):
+        for c in range(3):
+            ref, rinfo = orc.flexwin(obs[:, r, c], syn[:, r, c], dt, 0.0, tp[r], dist[r], 2.0, 20.0,
+                                     is_split_phases=split)
+            np.testing.assert_allclose(got[3 * r + c], ref, rtol=0, atol=1e-9)
+            np.testing.assert_allclose(ginfo[3 * r + c], rinfo, rtol=1e-9, atol=1e-12)
+            counts.append(len(ref))
+    assert max(counts) >= (2 if split else 1) and counts[-1] == 0
+    print(f"flexwin split {split} nstep {nstep}: windows per trace {counts}, {misfit.last_time_ms():.2f} ms")
+
+
 @pytest.mark.parametrize("nstep", [3000, 2999])
 def test_conv_batch_matches_oracle(nstep):
     """Cross-convolution of every receiver (even and odd trace lengths: the two
