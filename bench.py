@@ -120,7 +120,8 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the event-parallel multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "(events would shard across GPUs, one process each; only the one-GPU share is measured)",
+              file=sys.stderr)
         sys.exit(2)
     from specfwat_amd import _lib
     if a.copy_probe:

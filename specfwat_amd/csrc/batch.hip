@@ -688,10 +688,10 @@ void launch_elem(SemBatch* B, int nf, const int* active) {
   const int nb = std::max(8, std::min((m->nspec + 7) & ~7, (ncu * B->blocks_per_cu + 7) & ~7));
   const int sp = m->static_precision;
   hipStream_t s = m->stream;
-#define LB(NF, SP) hipLaunchKernelGGL((k_batch_elem<E, NF, SP>), dim3(nb), dim3(128), 0, s, A)
+#define LB(NF, SP) SEM_LAUNCH((k_batch_elem<E, NF, SP>), dim3(nb), dim3(128), 0, s, A)
   if (B->aniso) {
-    if (nf == 2) hipLaunchKernelGGL((k_batch_elem_aniso<E, 2>), dim3(nb), dim3(128), 0, s, A);
-    else hipLaunchKernelGGL((k_batch_elem_aniso<E, 1>), dim3(nb), dim3(128), 0, s, A);
+    if (nf == 2) SEM_LAUNCH((k_batch_elem_aniso<E, 2>), dim3(nb), dim3(128), 0, s, A);
+    else SEM_LAUNCH((k_batch_elem_aniso<E, 1>), dim3(nb), dim3(128), 0, s, A);
   } else if (nf == 2) {
     if (sp == SEM_STATIC_BF16) LB(2, SEM_STATIC_BF16);
     else if (sp == SEM_STATIC_FP16) LB(2, SEM_STATIC_FP16);
@@ -736,8 +736,8 @@ void asm_pass(SemBatch* B, bool two, int it) {
   const dim3 gr(nblk((size_t)m->nglob * B->E)), bl(TPB);
   hipStream_t s = m->stream;
 #define LA(EE)                                                           \
-  if (two) hipLaunchKernelGGL((k_batch_asm<EE, true>), gr, bl, 0, s, A); \
-  else hipLaunchKernelGGL((k_batch_asm<EE, false>), gr, bl, 0, s, A);
+  if (two) SEM_LAUNCH((k_batch_asm<EE, true>), gr, bl, 0, s, A);       \
+  else SEM_LAUNCH((k_batch_asm<EE, false>), gr, bl, 0, s, A);
   switch (B->E) {
     case 1: LA(1); break;
     case 2: LA(2); break;
@@ -805,13 +805,16 @@ struct PassTimer {
   explicit PassTimer(int steps) {
     stride = std::max(1, steps / 256);
     ev.resize(4 * (size_t)((steps + stride - 1) / stride));
-    // timing-only events: no system-scope fence (cache writeback) at each record
-    for (auto& e : ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
   }
   bool sampled(int it) const { return (it - 1) % stride == 0; }
-  void mark(int it, int k, hipStream_t s) {
+  // k = 0 / 2 arm the element / assembly launch's start and end events, k = 1 / 3
+  // check that the launch took them
+  void mark(int it, int k, hipStream_t) {
     if (!sampled(it)) return;
-    HIP_CHECK(hipEventRecord(ev[4 * (size_t)((it - 1) / stride) + k], s));
+    const size_t q = 4 * (size_t)((it - 1) / stride) + k;
+    if (k % 2 == 0) sem_arm_launch_events(ev[q], ev[q + 1]);
+    else if (!sem_launch_events_consumed()) SEM_FAIL("timed batch pass without a kernel launch");
   }
   // sums over the sampled steps and their count
   int sum(int steps, double& elem, double& asmb) {

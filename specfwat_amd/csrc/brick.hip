@@ -277,7 +277,7 @@ void launch_brick_shape(SemMesh* m, int nf, const BrickArgs& a) {
   nb = std::max(8, (nb + 7) & ~7);
   const int sp = m->static_precision;
   hipStream_t s = m->stream;
-#define LK(NF, SP) hipLaunchKernelGGL((k_brick<BX, BY, BZ, NF, SP>), dim3(nb), dim3(128), 0, s, a)
+#define LK(NF, SP) SEM_LAUNCH((k_brick<BX, BY, BZ, NF, SP>), dim3(nb), dim3(128), 0, s, a)
   if (nf == 2) {
     if (sp == SEM_STATIC_BF16) LK(2, SEM_STATIC_BF16);
     else if (sp == SEM_STATIC_FP16) LK(2, SEM_STATIC_FP16);
@@ -344,12 +344,12 @@ void sem_launch_halo(SemMesh* m, int nfields, const AsmField& F0, const AsmField
   const int nsp = m->nsources * 125;
   const dim3 gr((m->nhalo + TPB - 1) / TPB), bl(TPB);
   if (nfields == 2)
-    hipLaunchKernelGGL(k_halo<true>, gr, bl, 0, m->stream, m->nhalo, m->d_halo_g, m->d_halo_off, m->d_halo_idx, nslot,
+    SEM_LAUNCH(k_halo<true>, gr, bl, 0, m->stream, m->nhalo, m->d_halo_g, m->d_halo_off, m->d_halo_idx, nslot,
                        nfp, nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, m->d_brick_part0, m->d_brick_part1,
                        F0, F1, last ? 1 : 0, m->d_rho_kl_glob, m->kernel_block > 1 ? m->d_rho_part : nullptr,
                        m->kernel_flush, m->d_hess_kl_glob, dtk);
   else
-    hipLaunchKernelGGL(k_halo<false>, gr, bl, 0, m->stream, m->nhalo, m->d_halo_g, m->d_halo_off, m->d_halo_idx, nslot,
+    SEM_LAUNCH(k_halo<false>, gr, bl, 0, m->stream, m->nhalo, m->d_halo_g, m->d_halo_off, m->d_halo_idx, nslot,
                        nfp, nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, m->d_brick_part0, m->d_brick_part1,
                        F0, F1, last ? 1 : 0, (double*)nullptr, (float*)nullptr, 1, (double*)nullptr, dtk);
   HIP_CHECK(hipGetLastError());

@@ -833,19 +833,19 @@ void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, int varian
   const int all = (a.elem_count + 7) & ~7;
   int nb = blocks_per_cu > 0 ? std::min(all, (ncu * blocks_per_cu + 7) & ~7) : all;
   nb = std::max(8, nb);
-  if constexpr (ANISO) hipLaunchKernelGGL((k_forces_reg_aniso<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
-  else if (variant == 2) hipLaunchKernelGGL((k_forces_reg_pipe<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
-  else if (variant == 3) hipLaunchKernelGGL((k_forces_reg_pp<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
-  else hipLaunchKernelGGL((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  if constexpr (ANISO) SEM_LAUNCH((k_forces_reg_aniso<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else if (variant == 2) SEM_LAUNCH((k_forces_reg_pipe<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else if (variant == 3) SEM_LAUNCH((k_forces_reg_pp<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else SEM_LAUNCH((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
 }
 
 template <int NF, bool ANISO, int KMODE, int SP>
 void launch_t(const ForceArgs& a, hipStream_t s) {
   const int nb = a.nblocks_padded;
   if (a.floc0)
-    hipLaunchKernelGGL((k_forces<NF, ANISO, KMODE, SP, 1>), dim3(nb), dim3(128), 0, s, a);
+    SEM_LAUNCH((k_forces<NF, ANISO, KMODE, SP, 1>), dim3(nb), dim3(128), 0, s, a);
   else
-    hipLaunchKernelGGL((k_forces<NF, ANISO, KMODE, SP, 0>), dim3(nb), dim3(128), 0, s, a);
+    SEM_LAUNCH((k_forces<NF, ANISO, KMODE, SP, 0>), dim3(nb), dim3(128), 0, s, a);
 }
 
 }  // namespace

@@ -21,6 +21,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <cstdint>
@@ -52,6 +53,43 @@ typedef float realw;
     std::fprintf(stderr, "\n");                                                          \
     std::fflush(stderr);                                                                 \
     std::exit(1);                                                                        \
+  } while (0)
+
+// Timing of one kernel launch by the dispatch's own start and end timestamps
+// (hipExtLaunchKernelGGL), the interval a profiler reports: a timer arms an
+// event pair before a sampled launch, and the next SEM_LAUNCH on this thread
+// consumes it.  Event markers recorded around a launch instead add the marker
+// packets' latency to the interval.
+struct SemLaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+inline SemLaunchEvents& sem_launch_events() {
+  static thread_local SemLaunchEvents e;
+  return e;
+}
+inline void sem_arm_launch_events(hipEvent_t start, hipEvent_t stop) {
+  SemLaunchEvents& e = sem_launch_events();
+  if (e.start) SEM_FAIL("timed launch armed twice");
+  e.start = start;
+  e.stop = stop;
+}
+// true when the armed pair was taken by a launch (and clears a stale pair)
+inline bool sem_launch_events_consumed() {
+  SemLaunchEvents& e = sem_launch_events();
+  const bool ok = e.start == nullptr;
+  e.start = e.stop = nullptr;
+  return ok;
+}
+#define SEM_LAUNCH(kernel, grid, block, shmem, stream, ...)                                          \
+  do {                                                                                               \
+    SemLaunchEvents& sle_ = sem_launch_events();                                                     \
+    if (sle_.start) {                                                                                \
+      hipEvent_t s0_ = sle_.start, s1_ = sle_.stop;                                                  \
+      sle_.start = sle_.stop = nullptr;                                                              \
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, s0_, s1_, 0u, __VA_ARGS__);          \
+    } else {                                                                                         \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                           \
+    }                                                                                                \
   } while (0)
 
 // storage precision of the static material / geometry arrays

@@ -265,21 +265,25 @@ void plan_free(SmoothPlan& p) {
   p = SmoothPlan();
 }
 
+// elem_ms / update_ms of a timed run: the element and gather launches of every
+// stride-th step (at most about 256 steps) timed by their dispatch timestamps,
+// scaled to all nstep steps
 template <int NFP>
 void launch_steps(SmoothPlan& p, const SmoothArgs& a, int nstep, bool timed) {
+  const int stride = std::max(1, nstep / 256), nsamp = (nstep + stride - 1) / stride;
   std::vector<hipEvent_t> ev;
   if (timed) {
-    ev.resize(3 * (size_t)nstep);
-    // timing-only events: no system-scope fence (cache writeback) at each record
-    for (auto& x : ev) HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableSystemFence));
+    ev.resize(4 * (size_t)nsamp);
+    for (auto& x : ev) HIP_CHECK(hipEventCreate(&x));
   }
   for (int it = 0; it < nstep; ++it) {
-    if (timed) HIP_CHECK(hipEventRecord(ev[3 * it], p.stream));
-    hipLaunchKernelGGL(k_smooth_elem<NFP>, dim3(a.nblocks_padded), dim3(128), 0, p.stream, a);
-    if (timed) HIP_CHECK(hipEventRecord(ev[3 * it + 1], p.stream));
-    hipLaunchKernelGGL(k_smooth_gather<NFP>, dim3(sm_blocks((size_t)p.nglob)), dim3(SM_TPB), 0, p.stream, p.nglob,
-                       p.d_off, p.d_idx, p.d_dloc, p.d_rvol, p.d_dat);
-    if (timed) HIP_CHECK(hipEventRecord(ev[3 * it + 2], p.stream));
+    const bool tm = timed && it % stride == 0;
+    const size_t q = 4 * (size_t)(it / stride);
+    if (tm) sem_arm_launch_events(ev[q], ev[q + 1]);
+    SEM_LAUNCH(k_smooth_elem<NFP>, dim3(a.nblocks_padded), dim3(128), 0, p.stream, a);
+    if (tm) sem_arm_launch_events(ev[q + 2], ev[q + 3]);
+    SEM_LAUNCH(k_smooth_gather<NFP>, dim3(sm_blocks((size_t)p.nglob)), dim3(SM_TPB), 0, p.stream, p.nglob,
+               p.d_off, p.d_idx, p.d_dloc, p.d_rvol, p.d_dat);
     if (p.npml > 0)
       hipLaunchKernelGGL(k_keep_pml, dim3((p.npml + SM_TPB - 1) / SM_TPB), dim3(SM_TPB), 0, p.stream, p.npml,
                          p.nfield, p.nfp, p.d_pml_iglob, p.d_pml_val, p.d_dat);
@@ -287,13 +291,16 @@ void launch_steps(SmoothPlan& p, const SmoothArgs& a, int nstep, bool timed) {
   HIP_CHECK(hipGetLastError());
   if (timed) {
     HIP_CHECK(hipStreamSynchronize(p.stream));
-    for (int it = 0; it < nstep; ++it) {
+    double e = 0.0, u = 0.0;
+    for (int k = 0; k < nsamp; ++k) {
       float a1 = 0, a2 = 0;
-      HIP_CHECK(hipEventElapsedTime(&a1, ev[3 * it], ev[3 * it + 1]));
-      HIP_CHECK(hipEventElapsedTime(&a2, ev[3 * it + 1], ev[3 * it + 2]));
-      p.elem_ms += a1;
-      p.update_ms += a2;
+      HIP_CHECK(hipEventElapsedTime(&a1, ev[4 * (size_t)k], ev[4 * (size_t)k + 1]));
+      HIP_CHECK(hipEventElapsedTime(&a2, ev[4 * (size_t)k + 2], ev[4 * (size_t)k + 3]));
+      e += a1;
+      u += a2;
     }
+    p.elem_ms += e * nstep / nsamp;
+    p.update_ms += u * nstep / nsamp;
     for (auto x : ev) HIP_CHECK(hipEventDestroy(x));
   }
 }

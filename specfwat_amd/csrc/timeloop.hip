@@ -446,20 +446,20 @@ void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const A
   const dim3 gr(nblk(npts)), bl(TPB);
   if (two) {
     if (last)
-      hipLaunchKernelGGL((k_assemble<true, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      SEM_LAUNCH((k_assemble<true, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, rho_part, m->kernel_flush,
                          m->d_hess_kl_glob, dtk);
     else
-      hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      SEM_LAUNCH((k_assemble<false, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, m->d_rho_kl_glob, rho_part, m->kernel_flush,
                          m->d_hess_kl_glob, dtk);
   } else {
     if (last)
-      hipLaunchKernelGGL((k_assemble<true, false>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      SEM_LAUNCH((k_assemble<true, false>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (float*)nullptr, 1,
                          (double*)nullptr, dtk);
     else
-      hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
+      SEM_LAUNCH((k_assemble<false, false>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
                          nsp, m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, (double*)nullptr, (float*)nullptr, 1,
                          (double*)nullptr, dtk);
   }
@@ -480,10 +480,10 @@ inline int timer_stride(int nstep) { return std::max(1, nstep / 256); }
 // queued ahead of the device (tens of thousands of launches at NSTEP 10,000).
 constexpr int kSyncEvery = 1024;
 
-// Force or assembly launches bracketed by HIP events on the simulation stream
-// on the sampled steps only; exactly two events per sampled launch are
-// created, and the elapsed times are summed after the run, so timing adds no
-// host sync.
+// Force or assembly launches of the sampled steps timed by their dispatch
+// timestamps (an event pair armed for the launch, SEM_LAUNCH); exactly two
+// events per sampled launch are created, and the elapsed times are summed
+// after the run, so timing adds no host sync and no packet between kernels.
 struct ForceTimer {
   std::vector<hipEvent_t> ev;
   size_t used = 0;
@@ -496,23 +496,23 @@ struct ForceTimer {
     stride = timer_stride(nstep);
     const size_t samples = (size_t)(nstep + stride - 1) / stride;
     ev.assign(2 * samples * per_step, nullptr);
-    // timing-only events: no system-scope fence (cache writeback) at each record
-    for (auto& e : ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     used = 0;
     launches = 0;
     active = false;
   }
   // it: 1-based step about to be enqueued
   void at_step(int it) { active = (it - 1) % stride == 0; }
-  void open(hipStream_t s) {
+  // arm the next launch's start/end events (one kernel launch per bracket)
+  void open(hipStream_t) {
     if (!active) return;
-    if (used >= ev.size()) SEM_FAIL("force timer overflow");
-    HIP_CHECK(hipEventRecord(ev[used++], s));
+    if (used + 2 > ev.size()) SEM_FAIL("force timer overflow");
+    sem_arm_launch_events(ev[used], ev[used + 1]);
   }
-  void close(hipStream_t s, int nlaunch = 1) {
+  void close(hipStream_t, int nlaunch = 1) {
     if (!active) return;
-    if (used >= ev.size()) SEM_FAIL("force timer overflow");
-    HIP_CHECK(hipEventRecord(ev[used++], s));
+    if (!sem_launch_events_consumed()) return;  // the bracket launched nothing (an empty slab)
+    used += 2;
     launches += nlaunch;
   }
   double total_ms() {
