@@ -952,6 +952,31 @@ def measure(a):
     wins = misfit.select_windows(obs, syn, dt, 0.0, tp, dist, 0.4)
     stages["win_sel"] = {"wall_s": time.time() - ts, "device_ms": misfit.last_time_ms(),
                          "windows": int(sum(len(w) for w in wins))}
+    # FLEXWIN (WIN_TYPE 2) on traces of its usual band: 1200 traces of 4000 samples at dt 0.05 s
+    # (three arrivals each, 1 % noise), periods 2-20 s; merging and interval scheduling
+    fw_dt, fw_n = 0.05, 4000
+    tf = np.arange(fw_n) * fw_dt
+    fobs = np.zeros((fw_n, nrec, 3), np.float32)
+    fsyn = np.zeros_like(fobs)
+    ftp = rng.uniform(20, 30, nrec)
+    fdist = rng.uniform(200, 260, nrec)
+    for r in range(nrec):
+        for c in range(3):
+            arr = ftp[r] + np.array([5.0, 28.0, 62.0]) + rng.uniform(-2, 2, 3)
+            fq, am = rng.uniform(0.18, 0.32, 3), rng.uniform(0.6, 1.6, 3)
+            for k in range(3):
+                fsyn[:, r, c] += am[k] * np.exp(-((tf - arr[k]) * fq[k] / 2) ** 2) * np.sin(2 * np.pi * fq[k] * (tf - arr[k]))
+                sh = arr[k] + rng.uniform(-0.6, 0.6)
+                fobs[:, r, c] += am[k] * rng.uniform(0.8, 1.25) * np.exp(-((tf - sh) * fq[k] / 2) ** 2) * \
+                    np.sin(2 * np.pi * fq[k] * (tf - sh))
+    fobs += (0.01 * rng.standard_normal(fobs.shape)).astype(np.float32)
+    for label, split in (("flexwin_merge", False), ("flexwin_split", True)):
+        misfit.flexwin_windows(fobs, fsyn, fw_dt, 0.0, ftp, fdist, 2.0, 20.0, is_split_phases=split)  # warm-up
+        ts = time.time()
+        fw, _ = misfit.flexwin_windows(fobs, fsyn, fw_dt, 0.0, ftp, fdist, 2.0, 20.0, is_split_phases=split)
+        stages[label] = {"wall_s": time.time() - ts, "device_ms": misfit.last_time_ms(),
+                         "windows": int(sum(len(w) for w in fw)),
+                         "traces": f"{nrec * 3} x {fw_n} samples, dt {fw_dt}, periods 2-20 s"}
     log("measure: " + ", ".join(f"{k} {v['device_ms']:.1f} ms" for k, v in stages.items()))
     ntr = nrec * 3
     dev = stages["cc_tt"]["device_ms"]
@@ -996,6 +1021,16 @@ def measure(a):
         stages["win_sel"]["cpu_traces_per_s"] = sample / wall
         stages["win_sel"]["cpu_sample"] = f"{sample} traces through the oracle's window selection, one thread, " \
                                           f"{wall:.1f}s"
+        for label, split in (("flexwin_merge", 0.0), ("flexwin_split", 1.0)):
+            sample = 120
+            ts = time.time()
+            for q in range(sample):
+                r, c = divmod(q, 3)
+                orc.flexwin(fobs[:, r, c], fsyn[:, r, c], fw_dt, 0.0, ftp[r], fdist[r], 2.0, 20.0,
+                            is_split_phases=split)
+            wall = time.time() - ts
+            stages[label]["cpu_traces_per_s"] = sample / wall
+            stages[label]["cpu_sample"] = f"{sample} traces through the oracle's FLEXWIN, one thread, {wall:.1f}s"
     print(json.dumps(result), flush=True)
 
 
