@@ -83,6 +83,8 @@ def parse():
     p.add_argument("--unfused", action="store_true", help="time the SPECFEM-order stage sequence")
     p.add_argument("--graph", action="store_true",
                    help="c1/c2: capture each run's step loop into one HIP graph (no per-kernel timing)")
+    p.add_argument("--asm-lanes", type=int, default=0,
+                   help="points per lane of the gather assembly (1 or 2; 0: the autotuner's choice)")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--force-irregular", action="store_true",
                    help="c1/c2: every element stores its inverse Jacobian per point (the general-hexahedron "
@@ -232,6 +234,9 @@ def main():
         if a.graph:
             f.set_graph(True)
             ad.set_graph(True)
+        if a.asm_lanes:
+            f.set_assembly_lanes(a.asm_lanes)
+            ad.set_assembly_lanes(a.asm_lanes)
         if window and state_smooth is not None:
             f.set_fields(*state_smooth)
         if a.fk:

@@ -179,6 +179,9 @@ void fwat_set_slabs_(long *Mesh_pointer, int *elems_per_slab);
  * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
  * fp64).  The rho kernel (summed in the assembly pass) uses the same blocks. */
 void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
+/* points per lane of the gather assembly, 1 (default) or 2: twice the loads in flight per lane, the
+ * same per-point sums (launch-shape knob; no counterpart in the reference) */
+void fwat_set_assembly_lanes_(long *Mesh_pointer, int *points_per_lane);
 /* HIP graph capture of the whole-run loop (fused gather path): on = 1 makes every
  * fwat_iterate_time_ call capture its step loop from the simulation stream into one graph and
  * launch it as a unit (the GPU then runs the steps' kernels back to back, independent of the

@@ -62,6 +62,7 @@ SIGNATURES = {
     "fwat_set_brick_": "Liiii",
     "fwat_brick_info_": "Lill",
     "fwat_set_kernel_block_": "Li",
+    "fwat_set_assembly_lanes_": "Li",
     "fwat_set_graph_": "Li",
     "fwat_assembly_csr_": "iiIiIIiIiIiIiiII",
     "fwat_set_slabs_": "Li",

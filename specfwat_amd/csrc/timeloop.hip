@@ -145,6 +145,131 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
                      hess_kl, dtk);
 }
 
+// Boundary and single element terms of point g's row from q on (the tail of
+// k_assemble's loop), added in row order.
+template <bool TWO>
+__device__ __forceinline__ void assemble_tail(int q, int q1, const int* __restrict__ idx, int nloc, int nfp, int nsp,
+                                              const realw* __restrict__ sarr, const realw* __restrict__ rech,
+                                              const AsmField& F0, const AsmField& F1, realw* f) {
+  for (; q < q1; ++q) {
+    const int c = idx[q];
+    if (c < nloc) {
+      const realw* P = F0.floc + 3 * (size_t)c;
+      f[0] += P[0]; f[1] += P[1]; f[2] += P[2];
+      if (TWO) {
+        const realw* B = F1.floc + 3 * (size_t)c;
+        f[3] += B[0]; f[4] += B[1]; f[5] += B[2];
+      }
+    } else if (c < nloc + nfp) {
+      const size_t p = 3 * (size_t)(c - nloc);
+      if (F0.face) { f[0] -= F0.face[p]; f[1] -= F0.face[p + 1]; f[2] -= F0.face[p + 2]; }
+      if (TWO && F1.face) { f[3] -= F1.face[p]; f[4] -= F1.face[p + 1]; f[5] -= F1.face[p + 2]; }
+    } else if (c < nloc + nfp + nsp) {
+      const int p = c - nloc - nfp;
+      const realw* S = sarr + 3 * (size_t)p;
+      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f[0] += S[0] * w; f[1] += S[1] * w; f[2] += S[2] * w; }
+      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f[3] += S[0] * w; f[4] += S[1] * w; f[5] += S[2] * w; }
+    } else {
+      const int p = c - nloc - nfp - nsp;
+      const realw h = rech[p];
+      const size_t r = 3 * (size_t)(p / 125);
+      if (F0.adj) { f[0] += h * F0.adj[r]; f[1] += h * F0.adj[r + 1]; f[2] += h * F0.adj[r + 2]; }
+      if (TWO && F1.adj) { f[3] += h * F1.adj[r]; f[4] += h * F1.adj[r + 1]; f[5] += h * F1.adj[r + 2]; }
+    }
+  }
+}
+
+// k_assemble with two points per lane (points h and h + blockDim.x of the
+// block's 2 * blockDim.x): both points' state, offsets, indices and
+// element-local rows are requested before either is summed, so a lane keeps
+// twice the loads in flight.  Each point's row is summed in the same order as
+// k_assemble (pairs of element terms, then the tail), so results are identical.
+template <bool LAST, bool TWO>
+__global__ void k_assemble2(int npts, const int* __restrict__ glist, const int* __restrict__ off,
+                            const int* __restrict__ idx, int nloc, int nfp, int nsp, const realw* __restrict__ sarr,
+                            const realw* __restrict__ rech, const realw* __restrict__ rm,
+                            AsmField F0, AsmField F1, double* __restrict__ rho_kl,
+                            float* __restrict__ rho_part, int flush, double* __restrict__ hess_kl, double dtk) {
+  const int ha = blockIdx.x * 2 * blockDim.x + threadIdx.x, hb = ha + blockDim.x;
+  if (ha >= npts) return;
+  const bool vb = hb < npts;
+  const int ga = glist ? glist[ha] : ha;
+  const int gb = vb ? (glist ? glist[hb] : hb) : ga;
+  const PointState Pa = load_point<TWO>((size_t)ga, rm, F0, F1, rho_part);
+  const PointState Pb = load_point<TWO>((size_t)gb, rm, F0, F1, rho_part);
+  int qa = off[ga], qb = off[gb];
+  const int qa1 = off[ga + 1], qb1 = vb ? off[gb + 1] : qb;
+  realw fa[6] = {0, 0, 0, 0, 0, 0}, fb[6] = {0, 0, 0, 0, 0, 0};
+  bool ra = true, rb = vb;  // still in the paired part of the row
+  while (ra || rb) {
+    ra = ra && qa + 1 < qa1;
+    rb = rb && qb + 1 < qb1;
+    int a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    if (ra) { a0 = idx[qa]; a1 = idx[qa + 1]; }
+    if (rb) { b0 = idx[qb]; b1 = idx[qb + 1]; }
+    ra = ra && a1 < nloc;  // element terms lead each row: a0 is one too
+    rb = rb && b1 < nloc;
+    realw x[12], y[12];
+    if (ra) {
+      const realw* P0 = F0.floc + 3 * (size_t)a0;
+      const realw* P1 = F0.floc + 3 * (size_t)a1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { x[c] = P0[c]; x[3 + c] = P1[c]; }
+      if (TWO) {
+        const realw* B0 = F1.floc + 3 * (size_t)a0;
+        const realw* B1 = F1.floc + 3 * (size_t)a1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { x[6 + c] = B0[c]; x[9 + c] = B1[c]; }
+      }
+    }
+    if (rb) {
+      const realw* P0 = F0.floc + 3 * (size_t)b0;
+      const realw* P1 = F0.floc + 3 * (size_t)b1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { y[c] = P0[c]; y[3 + c] = P1[c]; }
+      if (TWO) {
+        const realw* B0 = F1.floc + 3 * (size_t)b0;
+        const realw* B1 = F1.floc + 3 * (size_t)b1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { y[6 + c] = B0[c]; y[9 + c] = B1[c]; }
+      }
+    }
+    if (ra) {
+      if (TWO) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fa[3 + c] += x[6 + c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fa[3 + c] += x[9 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fa[c] += x[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fa[c] += x[3 + c];
+      qa += 2;
+    }
+    if (rb) {
+      if (TWO) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fb[3 + c] += y[6 + c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fb[3 + c] += y[9 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fb[c] += y[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fb[c] += y[3 + c];
+      qb += 2;
+    }
+  }
+  assemble_tail<TWO>(qa, qa1, idx, nloc, nfp, nsp, sarr, rech, F0, F1, fa);
+  if (vb) assemble_tail<TWO>(qb, qb1, idx, nloc, nfp, nsp, sarr, rech, F0, F1, fb);
+  newmark_point<TWO>((size_t)ga, Pa, fa[0], fa[1], fa[2], fa[3], fa[4], fa[5], F0, F1, LAST, rho_kl, rho_part,
+                     flush != 0, hess_kl, dtk);
+  if (vb)
+    newmark_point<TWO>((size_t)gb, Pb, fb[0], fb[1], fb[2], fb[3], fb[4], fb[5], F0, F1, LAST, rho_kl, rho_part,
+                       flush != 0, hess_kl, dtk);
+}
+
 // Incident wavefield of one step for FK injection: rows k and k + 1 of the
 // stored velocity and traction and the weight w of row k + 1 (no injection
 // when v0 is null)
@@ -443,7 +568,23 @@ void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const A
   const int* glist = slab >= 0 ? m->d_slab_g + m->h_slab_goff[slab] : nullptr;
   const int npts = slab >= 0 ? m->h_slab_goff[slab + 1] - m->h_slab_goff[slab] : m->nglob;
   if (npts == 0) return;
-  const dim3 gr(nblk(npts)), bl(TPB);
+  const dim3 bl(TPB);
+  if (m->asm_points_per_lane == 2) {
+    const dim3 g2((unsigned)((npts + 2 * TPB - 1) / (2 * TPB)));
+#define ASM2(L, T, RK, RP, FL, HK)                                                                              \
+  SEM_LAUNCH((k_assemble2<L, T>), g2, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp, nsp, \
+             m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, RK, RP, FL, HK, dtk)
+    if (two) {
+      if (last) ASM2(true, true, m->d_rho_kl_glob, rho_part, m->kernel_flush, m->d_hess_kl_glob);
+      else ASM2(false, true, m->d_rho_kl_glob, rho_part, m->kernel_flush, m->d_hess_kl_glob);
+    } else {
+      if (last) ASM2(true, false, (double*)nullptr, (float*)nullptr, 1, (double*)nullptr);
+      else ASM2(false, false, (double*)nullptr, (float*)nullptr, 1, (double*)nullptr);
+    }
+#undef ASM2
+    return;
+  }
+  const dim3 gr(nblk(npts));
   if (two) {
     if (last)
       SEM_LAUNCH((k_assemble<true, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
@@ -1142,6 +1283,14 @@ void fwat_copy_probe_unrolled_(long* nbytes, int* reps, int* unroll, int* blocks
   HIP_CHECK(hipEventDestroy(e1));
   HIP_CHECK(hipFree(a));
   HIP_CHECK(hipFree(b));
+}
+
+// Points per lane of the gather assembly: 1 (k_assemble) or 2 (k_assemble2,
+// twice the loads in flight per lane; the same sums)
+void fwat_set_assembly_lanes_(long* Mesh_pointer, int* points_per_lane) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*points_per_lane != 1 && *points_per_lane != 2) SEM_FAIL("fwat_set_assembly_lanes_: 1 or 2 points per lane");
+  m->asm_points_per_lane = *points_per_lane;
 }
 
 // HIP graph capture of the whole-run loop (fused gather path): 1 = every

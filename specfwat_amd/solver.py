@@ -129,6 +129,10 @@ class ElasticRun:
             _lib.call("fwat_set_brick_", self.h, int(brick[0]), int(brick[1]), int(brick[2]),
                       int(brick_blocks_per_cu))
 
+    def set_assembly_lanes(self, points_per_lane):
+        """Points per lane of the gather assembly (1 or 2; the same sums)."""
+        _lib.call("fwat_set_assembly_lanes_", self.h, int(points_per_lane))
+
     def set_graph(self, on=True):
         """Capture every whole-run loop of this handle into one HIP graph and launch
         it as a unit (fused gather path; captured runs are not timed)."""

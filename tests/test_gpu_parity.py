@@ -381,3 +381,35 @@ def test_element_slabs_are_bitwise_the_single_pass(kind):
         assert np.array_equal(syn, out[0][0])
         for n in k:
             assert np.array_equal(k[n], out[0][1][n]), n
+
+
+@pytest.mark.parametrize("kind", ["regular", "irregular"])
+def test_two_points_per_lane_assembly_is_bitwise_the_one_point_pass(kind):
+    """The assembly with two points per lane (k_assemble2) sums every point's
+    row in the same order as k_assemble: seismograms and kernels identical,
+    also for a mesh whose point count is not a multiple of the block."""
+    c = make_case(seed=31, nstep=70, perturb=0.12 if kind == "irregular" else 0.0)
+    m = c["mesh"]
+    o = oracle_of(c, 64)
+    adj = (0.5 * o.forward(save=True)).astype(np.float32)
+    out = []
+    for lanes in (1, 2):
+        src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+        rec = Receivers(m, c["rxyz"])
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
+        f.set_assembly_lanes(lanes)
+        f.iterate_time()
+        syn = f.seismograms()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
+        a.set_assembly_lanes(lanes)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj)
+        a.iterate_time()
+        out.append((syn, a.kernels()))
+        a.close()
+    assert m.nglob % 512 != 0
+    syn, k = out[1]
+    assert np.array_equal(syn, out[0][0])
+    for n in k:
+        assert np.array_equal(k[n], out[0][1][n]), n
