@@ -498,8 +498,9 @@ def batched(a):
 
     bwall, bsyn, bkern, times = 0.0, {}, {}, {}
     check = set(groups[0][:2])  # the two longest events: kernels compared with the plain path
-    for g in groups:
+    for gi, g in enumerate(groups):
         wall, syn, kern, ft, at = run_group(g, any(i in check for i in g))
+        log(f"batch {gi + 1}/{len(groups)} ({len(g)} events, NSTEP up to {max(nsteps[i] for i in g)}): {wall:.1f}s")
         bwall += wall
         for k, i in enumerate(g):
             bsyn[i] = syn[k, :nsteps[i]]
@@ -532,6 +533,7 @@ def batched(a):
         ad.iterate_time()
         _lib.call("fwat_synchronize_", ad.h)
         pwall += time.time() - ts
+        log(f"plain event {i} (NSTEP {nsteps[i]}): {time.time() - ts:.1f}s")
         f.close()
         if np.linalg.norm(syn) > 0:
             perr = max(perr, float(np.linalg.norm(bsyn[i] - syn) / np.linalg.norm(syn)))
@@ -745,7 +747,7 @@ def aniso_smooth(a):
     log(f"c5: {E} events per batch ({len(groups)} batches; sizes fitting the HBM: {fits})")
 
     obs = [None] * len(srcs)
-    for g in groups:
+    for gi, g in enumerate(groups):
         b = EventBatch(mesh, *true_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g], cij=cij_true)
         b.set_store_precision(a.c4_store)
         b.forward()
@@ -753,17 +755,19 @@ def aniso_smooth(a):
         for k, i in enumerate(g):
             obs[i] = sy[k].copy()
         b.close()
+        log(f"observed data batch {gi + 1}/{len(groups)} ({time.time() - t0:.1f}s)")
     log(f"observed data ({len(srcs)} events, HTI true model) ({time.time() - t0:.1f}s)")
 
     bwall, times, bkern, misfits = 0.0, None, {}, []
     check = set(groups[0][:a.c5_plain_events])
-    for g in groups:
+    for gi, g in enumerate(groups):
         b = EventBatch(mesh, *smooth_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g], cij=cij_smooth)
         b.set_store_precision(a.c4_store)
         _lib.call("fwat_synchronize_", b.run.h)
         ts = time.time()
         b.forward(timed=True)
         ft = b.pass_times()
+        log(f"batch {gi + 1}/{len(groups)} forward: {time.time() - ts:.1f}s")
         syn = b.seismograms()
         adj = np.zeros_like(syn)
         for k, i in enumerate(g):
@@ -773,6 +777,7 @@ def aniso_smooth(a):
         b.adjoint(timed=True)
         _lib.call("fwat_synchronize_", b.run.h)
         bwall += time.time() - ts
+        log(f"batch {gi + 1}/{len(groups)} forward + adjoint: {time.time() - ts:.1f}s")
         at = b.pass_times()
         times = times or (ft, at)
         if any(i in check for i in g):
@@ -803,6 +808,7 @@ def aniso_smooth(a):
         ad.iterate_time(timed=True)
         _lib.call("fwat_synchronize_", ad.h)
         pwall += time.time() - ts
+        log(f"plain event {i}: {time.time() - ts:.1f}s")
         fwd_ms, fwd_n = f.force_time()
         adj_ms, adj_n = ad.force_time()
         a_asm = ad.assembly_time()
