@@ -236,14 +236,14 @@ __global__ void k_halo(int nhalo, const int* __restrict__ halo_g, const int* __r
     } else if (c < nslot + nfp + nsp) {
       const int p = c - nslot - nfp;
       const realw* S = sarr + 3 * (size_t)p;
-      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f0x += S[0] * w; f0y += S[1] * w; f0z += S[2] * w; }
-      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f1x += S[0] * w; f1y += S[1] * w; f1z += S[2] * w; }
+      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f0x = fmaf(S[0], w, f0x); f0y = fmaf(S[1], w, f0y); f0z = fmaf(S[2], w, f0z); }
+      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f1x = fmaf(S[0], w, f1x); f1y = fmaf(S[1], w, f1y); f1z = fmaf(S[2], w, f1z); }
     } else {
       const int p = c - nslot - nfp - nsp;
       const realw hh = rech[p];
       const size_t r = 3 * (size_t)(p / 125);
-      if (F0.adj) { f0x += hh * F0.adj[r]; f0y += hh * F0.adj[r + 1]; f0z += hh * F0.adj[r + 2]; }
-      if (TWO && F1.adj) { f1x += hh * F1.adj[r]; f1y += hh * F1.adj[r + 1]; f1z += hh * F1.adj[r + 2]; }
+      if (F0.adj) { f0x = fmaf(hh, F0.adj[r], f0x); f0y = fmaf(hh, F0.adj[r + 1], f0y); f0z = fmaf(hh, F0.adj[r + 2], f0z); }
+      if (TWO && F1.adj) { f1x = fmaf(hh, F1.adj[r], f1x); f1y = fmaf(hh, F1.adj[r + 1], f1y); f1z = fmaf(hh, F1.adj[r + 2], f1z); }
     }
   }
   newmark_point<TWO>((size_t)halo_g[h], rm, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, last != 0, rho_kl, rho_part,

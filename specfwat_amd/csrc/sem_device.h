@@ -278,9 +278,9 @@ inline int xcd_pad(int n) { return (n + 7) & ~7; }
 __device__ __forceinline__ void kernel_acc(double* __restrict__ K, float* __restrict__ part, size_t i, float x,
                                            double dt, bool flush) {
   if (!part) {
-    K[i] += dt * (double)x;
+    K[i] = fma(dt, (double)x, K[i]);
   } else if (flush) {
-    K[i] += dt * (double)(part[i] + x);
+    K[i] = fma(dt, (double)(part[i] + x), K[i]);
     part[i] = 0.0f;
   } else {
     part[i] += x;
@@ -338,37 +338,39 @@ __device__ __forceinline__ void newmark_point(size_t g, const PointState& P, rea
                                               const AsmField& F1, bool last, double* __restrict__ rho_kl,
                                               float* __restrict__ rho_part, bool flush,
                                               double* __restrict__ hess_kl, double dtk) {
+  // every multiply-add is an explicit fma, so the result does not depend on how
+  // the compiler contracts the surrounding code (the assembly variants agree bitwise)
   const size_t b = 3 * g;
   const realw r0 = P.r[0], r1 = P.r[1], r2 = P.r[2];
   const realw A[3] = {f0x * r0, f0y * r1, f0z * r2};
   const realw d0 = F0.dt, h0 = 0.5f * F0.dt, s0 = 0.5f * F0.dt * F0.dt;
   realw V[3], U[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) { V[c] = P.v0[c] + h0 * A[c]; U[c] = P.u0[c]; }
+  for (int c = 0; c < 3; ++c) { V[c] = fmaf(h0, A[c], P.v0[c]); U[c] = P.u0[c]; }
   if (TWO) {
     const realw BA[3] = {f1x * r0, f1y * r1, f1z * r2};
     const realw d1 = F1.dt, h1 = 0.5f * F1.dt, s1 = 0.5f * F1.dt * F1.dt;
     realw BV[3], BU[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { BV[c] = P.v1[c] + h1 * BA[c]; BU[c] = P.u1[c]; }
-    const float x = A[0] * BU[0] + A[1] * BU[1] + A[2] * BU[2];
+    for (int c = 0; c < 3; ++c) { BV[c] = fmaf(h1, BA[c], P.v1[c]); BU[c] = P.u1[c]; }
+    const float x = fmaf(A[2], BU[2], fmaf(A[1], BU[1], A[0] * BU[0]));
     if (!rho_part) {
       kernel_acc(rho_kl, rho_part, g, x, dtk, flush);
     } else if (flush) {  // kernel_acc with the partial sum read before the gather
-      rho_kl[g] += dtk * (double)(P.part + x);
+      rho_kl[g] = fma(dtk, (double)(P.part + x), rho_kl[g]);
       rho_part[g] = 0.0f;
     } else {
       rho_part[g] = P.part + x;
     }
-    if (hess_kl) hess_kl[g] += dtk * (double)(A[0] * BA[0] + A[1] * BA[1] + A[2] * BA[2]);
+    if (hess_kl) hess_kl[g] = fma(dtk, (double)fmaf(A[2], BA[2], fmaf(A[1], BA[1], A[0] * BA[0])), hess_kl[g]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if (last) {
         F1.a[b + c] = BA[c];
         F1.v[b + c] = BV[c];
       } else {
-        F1.u[b + c] = BU[c] + d1 * BV[c] + s1 * BA[c];
-        F1.v[b + c] = BV[c] + h1 * BA[c];
+        F1.u[b + c] = fmaf(s1, BA[c], fmaf(d1, BV[c], BU[c]));
+        F1.v[b + c] = fmaf(h1, BA[c], BV[c]);
       }
     }
   }
@@ -378,8 +380,8 @@ __device__ __forceinline__ void newmark_point(size_t g, const PointState& P, rea
       F0.a[b + c] = A[c];
       F0.v[b + c] = V[c];
     } else {
-      F0.u[b + c] = U[c] + d0 * V[c] + s0 * A[c];
-      F0.v[b + c] = V[c] + h0 * A[c];
+      F0.u[b + c] = fmaf(s0, A[c], fmaf(d0, V[c], U[c]));
+      F0.v[b + c] = fmaf(h0, A[c], V[c]);
     }
   }
 }

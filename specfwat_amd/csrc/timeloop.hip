@@ -131,14 +131,14 @@ __global__ void k_assemble(int npts, const int* __restrict__ glist, const int* _
     } else if (c < nloc + nfp + nsp) {
       const int p = c - nloc - nfp;
       const realw* S = sarr + 3 * (size_t)p;
-      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f0x += S[0] * w; f0y += S[1] * w; f0z += S[2] * w; }
-      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f1x += S[0] * w; f1y += S[1] * w; f1z += S[2] * w; }
+      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f0x = fmaf(S[0], w, f0x); f0y = fmaf(S[1], w, f0y); f0z = fmaf(S[2], w, f0z); }
+      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f1x = fmaf(S[0], w, f1x); f1y = fmaf(S[1], w, f1y); f1z = fmaf(S[2], w, f1z); }
     } else {
       const int p = c - nloc - nfp - nsp;
       const realw h = rech[p];
       const size_t r = 3 * (size_t)(p / 125);
-      if (F0.adj) { f0x += h * F0.adj[r]; f0y += h * F0.adj[r + 1]; f0z += h * F0.adj[r + 2]; }
-      if (TWO && F1.adj) { f1x += h * F1.adj[r]; f1y += h * F1.adj[r + 1]; f1z += h * F1.adj[r + 2]; }
+      if (F0.adj) { f0x = fmaf(h, F0.adj[r], f0x); f0y = fmaf(h, F0.adj[r + 1], f0y); f0z = fmaf(h, F0.adj[r + 2], f0z); }
+      if (TWO && F1.adj) { f1x = fmaf(h, F1.adj[r], f1x); f1y = fmaf(h, F1.adj[r + 1], f1y); f1z = fmaf(h, F1.adj[r + 2], f1z); }
     }
   }
   newmark_point<TWO>((size_t)g, P, f0x, f0y, f0z, f1x, f1y, f1z, F0, F1, LAST, rho_kl, rho_part, flush != 0,
@@ -167,14 +167,14 @@ __device__ __forceinline__ void assemble_tail(int q, int q1, const int* __restri
     } else if (c < nloc + nfp + nsp) {
       const int p = c - nloc - nfp;
       const realw* S = sarr + 3 * (size_t)p;
-      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f[0] += S[0] * w; f[1] += S[1] * w; f[2] += S[2] * w; }
-      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f[3] += S[0] * w; f[4] += S[1] * w; f[5] += S[2] * w; }
+      if (F0.stf) { const realw w = (realw)F0.stf[p / 125]; f[0] = fmaf(S[0], w, f[0]); f[1] = fmaf(S[1], w, f[1]); f[2] = fmaf(S[2], w, f[2]); }
+      if (TWO && F1.stf) { const realw w = (realw)F1.stf[p / 125]; f[3] = fmaf(S[0], w, f[3]); f[4] = fmaf(S[1], w, f[4]); f[5] = fmaf(S[2], w, f[5]); }
     } else {
       const int p = c - nloc - nfp - nsp;
       const realw h = rech[p];
       const size_t r = 3 * (size_t)(p / 125);
-      if (F0.adj) { f[0] += h * F0.adj[r]; f[1] += h * F0.adj[r + 1]; f[2] += h * F0.adj[r + 2]; }
-      if (TWO && F1.adj) { f[3] += h * F1.adj[r]; f[4] += h * F1.adj[r + 1]; f[5] += h * F1.adj[r + 2]; }
+      if (F0.adj) { f[0] = fmaf(h, F0.adj[r], f[0]); f[1] = fmaf(h, F0.adj[r + 1], f[1]); f[2] = fmaf(h, F0.adj[r + 2], f[2]); }
+      if (TWO && F1.adj) { f[3] = fmaf(h, F1.adj[r], f[3]); f[4] = fmaf(h, F1.adj[r + 1], f[4]); f[5] = fmaf(h, F1.adj[r + 2], f[5]); }
     }
   }
 }
