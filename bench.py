@@ -37,11 +37,12 @@ KERNEL_BLOCK = 16
 def element_kernel_name(tuning, nirr):
     """Name of the element-pass kernel the chosen launch shape runs: the general
     pass on meshes with irregular elements or regular_path 0, else the all-regular
-    pass (1), its pipelined form (2) or the pipelined form with ping-pong LDS
-    buffers (3)."""
+    pass (1), its pipelined form (2), the pipelined form with ping-pong LDS
+    buffers (3) or that form with non-temporal streams (4)."""
     if nirr > 0 or tuning.get("regular_path", 1) == 0:
         return "k_forces"
-    return {1: "k_forces_reg", 2: "k_forces_reg_pipe", 3: "k_forces_reg_pp"}[tuning["regular_path"]]
+    return {1: "k_forces_reg", 2: "k_forces_reg_pipe", 3: "k_forces_reg_pp",
+            4: "k_forces_reg_pp_nt"}[tuning["regular_path"]]
 
 
 def kernel_acc_bytes(kb):
@@ -84,7 +85,8 @@ def parse():
     p.add_argument("--graph", action="store_true",
                    help="c1/c2: capture each run's step loop into one HIP graph (no per-kernel timing)")
     p.add_argument("--asm-lanes", type=int, default=0,
-                   help="points per lane of the gather assembly (1 or 2; 0: the autotuner's choice)")
+                   help="points per lane of the gather assembly (1 or 2; 0: 2 for the forward pass, "
+                        "1 for the adjoint pass)")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--force-irregular", action="store_true",
                    help="c1/c2: every element stores its inverse Jacobian per point (the general-hexahedron "

@@ -154,7 +154,8 @@ void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);
 /* element-pass launch shape (autotuner knobs): regular_path 1 = folded-table kernel on
  * all-regular meshes, 2 = its software-pipelined form (next element's gathers in flight during
  * the current one's contractions; isotropic), 3 = the pipelined form with two barriers per
- * element (both fields through ping-pong LDS buffers); blocks_per_cu_* = persistent workgroups per CU,
+ * element (both fields through ping-pong LDS buffers), 4 = form 3 with non-temporal hints on the
+ * streamed indices, moduli and kernel partial sums; blocks_per_cu_* = persistent workgroups per CU,
  * 0 = one per element */
 void fwat_set_tuning_(long *Mesh_pointer, int *regular_path, int *blocks_per_cu_fwd, int *blocks_per_cu_adj);
 /* Element lattice of the mesh: element (ix,iy,iz) = ix + nex*(iy + ney*iz).  Checked against
@@ -179,8 +180,9 @@ void fwat_set_slabs_(long *Mesh_pointer, int *elems_per_slab);
  * adjoint loop before they are added into the fp64 accumulators (default 16; 1 = every step in
  * fp64).  The rho kernel (summed in the assembly pass) uses the same blocks. */
 void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
-/* points per lane of the gather assembly, 1 (default) or 2: twice the loads in flight per lane, the
- * same per-point sums (launch-shape knob; no counterpart in the reference) */
+/* points per lane of the gather assembly: 1, 2 (twice the loads in flight per lane; the same
+ * per-point sums) or 0 (default: 2 for the forward one-field pass, 1 for the adjoint two-field
+ * pass); launch-shape knob, no counterpart in the reference */
 void fwat_set_assembly_lanes_(long *Mesh_pointer, int *points_per_lane);
 /* HIP graph capture of the whole-run loop (fused gather path): on = 1 makes every
  * fwat_iterate_time_ call capture its step loop from the simulation stream into one graph and

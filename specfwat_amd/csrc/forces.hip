@@ -638,8 +638,11 @@ __device__ __forceinline__ void forces_reg_pipe_body(const ForceArgs& A) {
 // an element's writes never wait for the previous element's readers (those
 // finished before the previous element's second barrier).  Arithmetic and
 // results are forces_reg_pipe_body's.
-template <int NF, int KMODE, int SP>
+template <int NF, int KMODE, int SP, bool NT>
 __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
+  // NT: the streamed per-point arrays (indices, moduli, kernel partial sums) are read and
+  // written with non-temporal hints, leaving the caches to the displacement gathers
+  const auto ld = [](const auto* p) { if constexpr (NT) return __builtin_nontemporal_load(p); else return *p; };
   __shared__ realw s_tab[128];
   __shared__ realw s_u[2][NF][3][128];
   __shared__ realw s_s[2][NF][6][128];
@@ -674,10 +677,10 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
   const bool flush = A.kflush != 0;
   const auto load_moduli = [&](size_t Lg, int e, realw& kl, realw& ml) {
     if (SP == SEM_STATIC_FP32) {
-      kl = A.kappa[Lg];
-      ml = A.mu[Lg];
+      kl = ld(&A.kappa[Lg]);
+      ml = ld(&A.mu[Lg]);
     } else {
-      const uint32_t pk = reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg];
+      const uint32_t pk = ld(&reinterpret_cast<const uint32_t*>(A.kmu_lp)[Lg]);
       const float2 kb = reinterpret_cast<const float2*>(A.kmu_base)[e];
       kl = kb.x + lp_to_f<SP>((uint16_t)(pk & 0xffffu)) * A.scale_k;
       ml = kb.y + lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
@@ -690,7 +693,7 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
   if (le < e_end) {
     const int e = A.elem_begin + le;
     const size_t Lg = (size_t)e * 125 + tt;
-    const int ig = A.ibool[Lg];
+    const int ig = ld(&A.ibool[Lg]);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       const realw* U = f == 0 ? A.u0 : A.u1;
@@ -698,7 +701,7 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
       for (int c = 0; c < 3; ++c) un[f][c] = U[3 * (size_t)ig + c];
     }
     load_moduli(Lg, e, kn, mn);
-    if (le + per_xcd < e_end) ig_next = A.ibool[(size_t)(e + per_xcd) * 125 + tt];
+    if (le + per_xcd < e_end) ig_next = ld(&A.ibool[(size_t)(e + per_xcd) * 125 + tt]);
   }
   int b = 0;
   for (; le < e_end; le += per_xcd, b ^= 1) {
@@ -722,12 +725,12 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
         for (int c = 0; c < 3; ++c) un[f][c] = U[3 * (size_t)ig_next + c];
       }
       load_moduli((size_t)e1 * 125 + tt, e1, kn, mn);
-      if (le + 2 * per_xcd < e_end) ig_next = A.ibool[(size_t)(e1 + per_xcd) * 125 + tt];
+      if (le + 2 * per_xcd < e_end) ig_next = ld(&A.ibool[(size_t)(e1 + per_xcd) * 125 + tt]);
     }
     float pm = 0.0f, pk = 0.0f;
     if (KMODE == 2 && A.mu_part && act) {
-      pm = A.mu_part[Lg];
-      pk = A.kappa_part[Lg];
+      pm = ld(&A.mu_part[Lg]);
+      pk = ld(&A.kappa_part[Lg]);
     }
     __syncthreads();  // displacements of both fields in buffer b
     realw ea[6];
@@ -760,8 +763,16 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
           } else if (flush) {
             A.mu_kl[Lg] += A.dt * (double)(pm + prod);
             A.kappa_kl[Lg] += A.dt * (double)(pk + kx);
-            A.mu_part[Lg] = 0.0f;
-            A.kappa_part[Lg] = 0.0f;
+            if constexpr (NT) {
+              __builtin_nontemporal_store(0.0f, &A.mu_part[Lg]);
+              __builtin_nontemporal_store(0.0f, &A.kappa_part[Lg]);
+            } else {
+              A.mu_part[Lg] = 0.0f;
+              A.kappa_part[Lg] = 0.0f;
+            }
+          } else if constexpr (NT) {
+            __builtin_nontemporal_store(pm + prod, &A.mu_part[Lg]);
+            __builtin_nontemporal_store(pk + kx, &A.kappa_part[Lg]);
           } else {
             A.mu_part[Lg] = pm + prod;
             A.kappa_part[Lg] = pk + kx;
@@ -797,10 +808,10 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
   }
 }
 
-template <int NF, int KMODE, int SP>
+template <int NF, int KMODE, int SP, bool NT = false>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : 5))) void k_forces_reg_pp(
     ForceArgs A) {
-  forces_reg_pp_body<NF, KMODE, SP>(A);
+  forces_reg_pp_body<NF, KMODE, SP, NT>(A);
 }
 
 // Two fields hold 6 more registers of look-ahead: 4 waves per SIMD (128 VGPRs)
@@ -836,6 +847,7 @@ void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, int varian
   if constexpr (ANISO) SEM_LAUNCH((k_forces_reg_aniso<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
   else if (variant == 2) SEM_LAUNCH((k_forces_reg_pipe<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
   else if (variant == 3) SEM_LAUNCH((k_forces_reg_pp<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else if (variant == 4) SEM_LAUNCH((k_forces_reg_pp<NF, KMODE, SP, true>), dim3(nb), dim3(128), 0, s, a);
   else SEM_LAUNCH((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
 }
 

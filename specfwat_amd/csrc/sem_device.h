@@ -196,7 +196,7 @@ struct SemMesh {
   realw* d_floc = nullptr;      // [nspec][125][3]
   realw* d_floc_b = nullptr;    // second field of the adjoint pass
   realw* d_face_scratch = nullptr;  // [nfaces*25][3] absorbing term of a non-stored field
-  int asm_points_per_lane = 1;  // gather assembly: 1 or 2 points per lane
+  int asm_points_per_lane = 0;  // gather assembly: 1 or 2 points per lane, 0 = by pass (see stage_assemble)
   int* d_asm_off = nullptr;     // [nglob+1]
   int* d_asm_idx = nullptr;     // [ncontrib]
   size_t n_contrib = 0;

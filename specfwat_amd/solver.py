@@ -130,7 +130,8 @@ class ElasticRun:
                       int(brick_blocks_per_cu))
 
     def set_assembly_lanes(self, points_per_lane):
-        """Points per lane of the gather assembly (1 or 2; the same sums)."""
+        """Points per lane of the gather assembly (1 or 2; the same sums), 0 for
+        the default (2 for the one-field pass, 1 for the two-field pass)."""
         _lib.call("fwat_set_assembly_lanes_", self.h, int(points_per_lane))
 
     def set_graph(self, on=True):
