@@ -71,6 +71,9 @@ def parse():
     p.add_argument("--c5-plain-events", type=int, default=2)
     p.add_argument("--c4-nstep-scale", type=float, default=1.0,
                    help="NSTEP of the c4 events is U[4000,12000] times this factor")
+    p.add_argument("--c4-mesh", default="c4", choices=["c4", "c1"],
+                   help="mesh of the c4 event set: the 64x64x32 box (default) or the 16x16x8 box, "
+                        "where every launch is short and batching amortises it")
     p.add_argument("--c4-store", type=int, default=1, choices=[0, 1],
                    help="Stacey store of the batches: 0 fp32, 1 fp16 with a scale per (step, event) record")
     p.add_argument("--c4-plain-events", type=int, default=4,
@@ -430,7 +433,7 @@ def batched(a):
     from specfwat_amd.sources import Receivers, Sources, surface_grid
     from specfwat_amd.tune import tune_element_passes
 
-    w = workload("c4")
+    w = workload(a.c4_mesh)
     t0 = time.time()
     mesh = BoxMesh(w["nex"], w["ney"], w["nez"], w["lx"], w["ly"], w["lz"])
     true_model = perturbed_model(mesh, seed=2024)
@@ -586,7 +589,8 @@ def batched(a):
         "vs_baseline": None,
         "dtype": "f32" if sp == 0 else ("f32+bf16-statics" if sp == 1 else "f32+fp16-statics"),
         "data": "synthetic",
-        "config": {"workload": (f"C4 batched: isotropic box {w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, "
+        "config": {"workload": (f"{'C4' if a.c4_mesh == 'c4' else 'C4 event set on the C1 mesh'} batched: "
+                                f"isotropic box {w['nex']}x{w['ney']}x{w['nez']} el, {nglob} GLL pts, "
                                 f"{len(srcs)} events (depth U[2,18] km, random MT, f0 U[0.5,1.5] Hz), "
                                 f"NSTEP U[4000,12000] x {a.c4_nstep_scale} = {min(nsteps)}..{max(nsteps)}, "
                                 f"{rec.nrec} receivers, dt={dt}; fwd + misfit + adjoint/reconstruction + "

@@ -17,7 +17,7 @@ from .solver import ElasticRun
 # software-pipelined form, 3 the pipelined form with two barriers per element (isotropic);
 # 0 workgroups per CU = one workgroup per element
 CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32), (2, 6), (2, 8), (2, 12), (2, 16),
-              (3, 6), (3, 8), (3, 12), (3, 16), (4, 8), (4, 12)]
+              (3, 6), (3, 8), (3, 12), (3, 16)]
 BRICKS = [(2, 2, 2), (4, 2, 2), (4, 4, 2)]
 SLABS = [6, 12, 24]  # slabs per mesh
 
