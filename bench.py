@@ -717,7 +717,7 @@ def aniso_smooth(a):
     path on a sample of the events; then Gaussian PDE smoothing (sigma_h 5 km,
     sigma_v 2 km) of all 21 kernels of one event in one device run."""
     from specfwat_amd import _lib
-    from specfwat_amd.batch import EventBatch, batch_device_bytes, plan_batches
+    from specfwat_amd.batch import EventBatch, batch_device_bytes, plan_batches, tune_batch
     from specfwat_amd.mesh import BoxMesh
     from specfwat_amd.misfit import event_adjoint_sources
     from specfwat_amd.model import hti_model, perturbed_model
@@ -748,7 +748,13 @@ def aniso_smooth(a):
     # the anisotropic batch adds the 21 fp64 kernels and their fp32 partial sums per event
     per_aniso = lambda e: batch_device_bytes(mesh, e, K, a.c4_store) + e * mesh.nspec * 125 * 21 * 12
     fits = [e for e in (1, 2, 4, 8) if per_aniso(e) <= 0.85 * free_b and e <= len(srcs)]
-    E = a.events_per_batch or max(fits)
+    bpc, ttable = 8, []
+    if a.events_per_batch:
+        E = a.events_per_batch
+    else:
+        (E, bpc), ttable = tune_batch(mesh, smooth_model, srcs, rec, dt, sizes=tuple(fits), cij=cij_smooth)
+        log("batch tuner (events, blocks/CU, fwd ms/event-step, adj ms/event-step): "
+            + ", ".join(f"({e},{b},{x:.3f},{y:.3f})" for e, b, x, y in ttable) + f" -> E={E} bpc={bpc}")
     groups = plan_batches(nsteps, E)
     log(f"c5: {E} events per batch ({len(groups)} batches; sizes fitting the HBM: {fits})")
 
@@ -769,6 +775,7 @@ def aniso_smooth(a):
     for gi, g in enumerate(groups):
         b = EventBatch(mesh, *smooth_model, [srcs[i] for i in g], rec, dt, [nsteps[i] for i in g], cij=cij_smooth)
         b.set_store_precision(a.c4_store)
+        b.set_tuning(bpc)
         _lib.call("fwat_synchronize_", b.run.h)
         ts = time.time()
         b.forward(timed=True)
@@ -870,7 +877,7 @@ def aniso_smooth(a):
                                 f"{E} per launch, NSTEP={K}, dt={dt}, {rec.nrec} receivers, rho + 21 c_ij kernels in "
                                 "fp64; then PDE smoothing of all 21 kernels of one event, sigma_h 5 km, sigma_v 2 km "
                                 "(diagonal regular-element map)"),
-                   "events_per_batch": E,
+                   "events_per_batch": E, "blocks_per_cu": bpc, "tuner": [list(r) for r in ttable],
                    "stacey_store": "fp16 + scale per (step, event)" if a.c4_store else "fp32"},
         "batched_vs_plain": {"plain_value": pvalue, "speedup": bvalue / pvalue, "plain_events_timed": len(check),
                              "kernel_rel_l2": kerr, "plain_element_pass_tuning": tuning},

@@ -160,11 +160,11 @@ def batch_device_bytes(mesh, nevents, nmax, store_precision=0):
 
 
 def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_cu=(4, 8, 16), nstep=12,
-               static_precision=0, hbm_budget=None):
+               static_precision=0, hbm_budget=None, cij=None):
     """Events per launch and persistent workgroups per CU of the batched
-    element pass, from short timed runs on the real mesh.  The figure of
-    merit is the element + assembly time per event-step of a forward and an
-    adjoint step.  Returns ((events, blocks_per_cu), table)."""
+    element pass, from short timed runs on the real mesh (anisotropic with
+    cij).  The figure of merit is the element + assembly time per event-step
+    of a forward and an adjoint step.  Returns ((events, blocks_per_cu), table)."""
     table = []
     for e in sizes:
         if e > len(sources):
@@ -172,7 +172,7 @@ def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_
         if hbm_budget is not None and batch_device_bytes(mesh, e, nstep) > hbm_budget:
             continue
         b = EventBatch(mesh, *model, sources[:e], receivers, dt, [nstep] * e,
-                       static_precision=static_precision)
+                       static_precision=static_precision, cij=cij)
         b.forward()  # warm-up; fills the Stacey rows the adjoint replays
         b.set_adjoint_sources(np.zeros((e, nstep, b.nrec, 3), F32))
         for bpc in blocks_per_cu:
