@@ -42,7 +42,7 @@ def element_kernel_name(tuning, nirr):
     if nirr > 0 or tuning.get("regular_path", 1) == 0:
         return "k_forces"
     return {1: "k_forces_reg", 2: "k_forces_reg_pipe", 3: "k_forces_reg_pp",
-            4: "k_forces_reg_pp_nt"}[tuning["regular_path"]]
+            4: "k_forces_reg_pp_nt", 5: "k_forces_reg_pp"}[tuning["regular_path"]]
 
 
 def kernel_acc_bytes(kb):
@@ -90,6 +90,8 @@ def parse():
     p.add_argument("--asm-lanes", type=int, default=0,
                    help="points per lane of the gather assembly (1 or 2; 0: 2 for the forward pass, "
                         "1 for the adjoint pass)")
+    p.add_argument("--asm-block", type=int, default=0,
+                   help="lanes per workgroup of the gather assembly (64..1024; 0: the library's 256)")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--force-irregular", action="store_true",
                    help="c1/c2: every element stores its inverse Jacobian per point (the general-hexahedron "
@@ -98,9 +100,10 @@ def parse():
     p.add_argument("--fk", type=int, default=0, metavar="R",
                    help="c2: inject a synthetic incident field (FK coupling) through the Stacey faces of the "
                         "forward run, one sample every R steps, the series resident in HBM")
-    p.add_argument("--tuning", default="", metavar="R,F,A",
+    p.add_argument("--tuning", default="", metavar="R,F,A[,I]",
                    help="fixed launch shape instead of the autotuner: regular_path, forward and adjoint "
-                        "workgroups per CU (e.g. 1,16,8), as the autotuner picks them")
+                        "workgroups per CU (e.g. 1,16,8), as the autotuner picks them, and whether the "
+                        "pipelined passes finish the element-interior points (1, the default, or 0)")
     p.add_argument("--window-start", type=int, default=-1, metavar="S0",
                    help="the timed event covers steps S0+1..S0+K, the state at S0 from an untimed forward run "
                         "(default: 0 for K >= 2000, else the first P arrival at the nearest receiver + 50 steps)")
@@ -194,8 +197,9 @@ def main():
     fused = not a.unfused
     tuning = {"regular_path": 1, "blocks_per_cu_fwd": 0, "blocks_per_cu_adj": 8}
     if a.tuning:
-        r, bf, ba = (int(v) for v in a.tuning.split(","))
-        tuning = {"regular_path": r, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba}
+        vals = [int(v) for v in a.tuning.split(",")]
+        tuning = {"regular_path": vals[0], "blocks_per_cu_fwd": vals[1], "blocks_per_cu_adj": vals[2],
+                  "interior": vals[3] if len(vals) > 3 else 1}
     elif fused and not a.no_tune:
         from specfwat_amd.tune import tune_element_passes
         tuning, table = tune_element_passes(mesh, smooth_model, src, rec, dt,
@@ -242,6 +246,9 @@ def main():
         if a.asm_lanes:
             f.set_assembly_lanes(a.asm_lanes)
             ad.set_assembly_lanes(a.asm_lanes)
+        if a.asm_block:
+            f.set_assembly_block(a.asm_block)
+            ad.set_assembly_block(a.asm_block)
         if window and state_smooth is not None:
             f.set_fields(*state_smooth)
         if a.fk:
@@ -289,6 +296,11 @@ def main():
     f_asm, a_asm = f.assembly_time(), ad.assembly_time()
     brick = tuning.get("brick") if fused else None
     binfo = ad.brick_info() if brick else None
+    # points the gather assembles and element-interior points the element passes finish
+    n_sh_f, n_in_f, act_f, _ = f.interior_info()
+    n_sh_a, n_in_a, _, act_a = ad.interior_info()
+    n_in_f = n_in_f if (fused and act_f) else 0
+    n_in_a = n_in_a if (fused and act_a) else 0
     k = ad.kernels()
     f.close(); ad.close()
     if not all(np.isfinite(v).all() for v in k.values()):
@@ -328,19 +340,27 @@ def main():
         # two-field element pass: ibool 4 + kappa,mu 8 (+ 40 irregular Jacobian) per local
         # point, mu/kappa accumulators 2 x kacc, element-local forces written for both fields
         # 24; displacement gather of both fields 12 + 12 per unique point
+        # With element-interior points finished by the element pass (n_in of them, one
+        # element-local slot each): their element-local forces are not written; the pass
+        # reads their rmass_inv 12 and velocities (12 per field), writes u and v (24 per
+        # field) and, two fields, the rho kernel kacc; the assembly covers the other
+        # nglob - n_in points, their contributions and element-local slots only.
         ename = element_kernel_name(tuning, nirr)
+        ngs = nglob - n_in_a
         kinds = {
-            f"{ename}<2,iso,fused-kernels>": ((28.0 + kmu + 2 * kacc) * nloc + 40.0 * nirr + 24.0 * nglob,
-                                              adj_ms, adj_n),
+            f"{ename}<2,iso,fused-kernels>": ((28.0 + kmu + 2 * kacc) * nloc + 40.0 * nirr + 24.0 * nglob
+                                              + (-24.0 + 12.0 + 72.0 + kacc) * n_in_a, adj_ms, adj_n),
             # two-field assembly: CSR offsets 4 per point + indices 4 per contribution,
             # element-local forces of both fields 24 per local point, Stacey terms of both
             # fields 24 per face point, rmass_inv 12, u,v of both fields read 48 + write 48,
             # rho kernel kacc per unique point
-            "k_assemble<two-field>": (4.0 * nglob + 4.0 * ncontrib + 24.0 * nloc + 24.0 * nfp
-                                      + 12.0 * nglob + 96.0 * nglob + kacc * nglob, adj_asm_ms, adj_asm_n),
+            "k_assemble<two-field>": (4.0 * ngs + 4.0 * (ncontrib - n_in_a) + 24.0 * (nloc - n_in_a) + 24.0 * nfp
+                                      + 12.0 * ngs + 96.0 * ngs + kacc * ngs, adj_asm_ms, adj_asm_n),
         }
-        fwd_bytes = (16.0 + kmu) * nloc + 40.0 * nirr + 12.0 * nglob
-        fwd_asm_bytes = 4.0 * nglob + 4.0 * ncontrib + 12.0 * nloc + 12.0 * nfp + 12.0 * nglob + 48.0 * nglob
+        ngf = nglob - n_in_f
+        fwd_bytes = (16.0 + kmu) * nloc + 40.0 * nirr + 12.0 * nglob + (-12.0 + 12.0 + 36.0) * n_in_f
+        fwd_asm_bytes = (4.0 * ngf + 4.0 * (ncontrib - n_in_f) + 12.0 * (nloc - n_in_f) + 12.0 * nfp
+                         + 12.0 * ngf + 48.0 * ngf)
     else:
         # SPECFEM stage order: one field per launch, atomic scatter, strain store
         kinds = {"k_forces<1,iso,strain-store,atomic>": (12.0 * nloc + 24.0 * nloc + 40.0 * nirr + 36.0 * nglob,

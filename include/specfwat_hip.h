@@ -155,9 +155,26 @@ void fwat_move_stacey_store_(long *Mesh_pointer_from, long *Mesh_pointer_to);
  * all-regular meshes, 2 = its software-pipelined form (next element's gathers in flight during
  * the current one's contractions; isotropic), 3 = the pipelined form with two barriers per
  * element (both fields through ping-pong LDS buffers), 4 = form 3 with non-temporal hints on the
- * streamed indices, moduli and kernel partial sums; blocks_per_cu_* = persistent workgroups per CU,
- * 0 = one per element */
+ * streamed indices, moduli and kernel partial sums, 5 = form 3 visiting the elements of a lattice
+ * mesh (fwat_set_lattice_) in column blocks, layer by layer (element order elsewhere);
+ * blocks_per_cu_* = persistent workgroups per CU, 0 = one per element */
 void fwat_set_tuning_(long *Mesh_pointer, int *regular_path, int *blocks_per_cu_fwd, int *blocks_per_cu_adj);
+/* The column-block element order regular_path 5 uses for a nex x ney x nez lattice and per_xcd
+ * workgroups per XCD: order[nex*ney*nez], 0-based elements in traversal order.  Host only; no
+ * counterpart in the reference (the reference's element loop visits elements in order). */
+void fwat_column_block_order_(int *nex, int *ney, int *nez, int *per_xcd, int *order);
+/* Element-interior points finished by the element pass of the fused loops (on = 1, the default;
+ * the pipelined passes 3 and 5 on isotropic all-regular meshes without slabs or bricks): points
+ * that one element-local slot alone refers to (no Stacey face, source or receiver term) are
+ * numbered after all others inside the library, the element pass applies their Newmark update
+ * and density-kernel term, and the gather assembly covers the others only.  Results are those
+ * of the gather path bit for bit.  No counterpart in the reference (its assembly is the
+ * element loop's scatter-add, prepare_fwd.f90:364 -> iterate_time). */
+void fwat_set_interior_(long *Mesh_pointer, int *on);
+/* Library numbering of a prepared mesh: points assembled by the gather (n_shared) and
+ * element-interior points (n_interior); whether a forward / two-field fused pass with the
+ * current launch shape finishes the interior points in the element pass. */
+void fwat_interior_info_(long *Mesh_pointer, int *n_shared, int *n_interior, int *active_fwd, int *active_adj);
 /* Element lattice of the mesh: element (ix,iy,iz) = ix + nex*(iy + ney*iz).  Checked against
  * ibool (shared faces, point multiplicities); enables fwat_set_brick_.  No counterpart in the
  * reference (SPECFEM meshes carry no lattice; this box mesher's do). */
@@ -184,6 +201,10 @@ void fwat_set_kernel_block_(long *Mesh_pointer, int *nsteps);
  * per-point sums) or 0 (default: 2 for the forward one-field pass, 1 for the adjoint two-field
  * pass); launch-shape knob, no counterpart in the reference */
 void fwat_set_assembly_lanes_(long *Mesh_pointer, int *points_per_lane);
+/* lanes per workgroup of the gather assembly (64..1024, a multiple of 64; default 256): larger
+ * workgroups keep longer runs of consecutive points on one XCD; launch-shape knob, no
+ * counterpart in the reference */
+void fwat_set_assembly_block_(long *Mesh_pointer, int *lanes);
 /* HIP graph capture of the whole-run loop (fused gather path): on = 1 makes every
  * fwat_iterate_time_ call capture its step loop from the simulation stream into one graph and
  * launch it as a unit (the GPU then runs the steps' kernels back to back, independent of the

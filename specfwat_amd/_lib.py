@@ -59,10 +59,14 @@ SIGNATURES = {
     "fwat_set_static_precision_": "Li",
     "fwat_set_tuning_": "Liii",
     "fwat_set_lattice_": "Liii",
+    "fwat_column_block_order_": "iiiiI",
+    "fwat_set_interior_": "Li",
+    "fwat_interior_info_": "LIIII",
     "fwat_set_brick_": "Liiii",
     "fwat_brick_info_": "Lill",
     "fwat_set_kernel_block_": "Li",
     "fwat_set_assembly_lanes_": "Li",
+    "fwat_set_assembly_block_": "Li",
     "fwat_set_graph_": "Li",
     "fwat_assembly_csr_": "iiIiIIiIiIiIiiII",
     "fwat_set_slabs_": "Li",

@@ -47,6 +47,9 @@ struct ForceArgs {
   const realw* cij;
   const realw* tables;
   const int* elem_list;
+  // pipelined regular pass in column blocks (regular_path 5): columns of the element lattice
+  // (ncol = nex * ney) and elements per column; col_nez = 0: element order
+  int col_ncol, col_nez;
   int elem_begin, elem_count;
   int nblocks_padded;
   realw xix_regular, jacobian_regular;
@@ -66,6 +69,17 @@ struct ForceArgs {
   float* cij_part;
   int kflush;         // this step closes a partial-sum block
   double dt;
+  // interior-point Newmark (pipelined pass of a fused loop): points >= n_shared have
+  // this element's term only and are finished here (newmark_point); every lane's
+  // element term still goes to floc
+  int n_shared;
+  int last;
+  const realw* rm;
+  AsmField F0, F1;
+  double* rho_kl;
+  float* rho_part;
+  double* hess_kl;
+  double dtk;
 };
 
 __device__ __forceinline__ float bf16_to_f(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
@@ -638,7 +652,7 @@ __device__ __forceinline__ void forces_reg_pipe_body(const ForceArgs& A) {
 // an element's writes never wait for the previous element's readers (those
 // finished before the previous element's second barrier).  Arithmetic and
 // results are forces_reg_pipe_body's.
-template <int NF, int KMODE, int SP, bool NT>
+template <int NF, int KMODE, int SP, bool NT, bool INT>
 __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
   // NT: the streamed per-point arrays (indices, moduli, kernel partial sums) are read and
   // written with non-temporal hints, leaving the caches to the displacement gathers
@@ -686,26 +700,47 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
       ml = kb.y + lp_to_f<SP>((uint16_t)(pk >> 16)) * A.scale_m;
     }
   };
+  // element at traversal index i (uniform over the workgroup): element order, or the column
+  // blocks of sem_column_block_order (XCD x's share [x*chunk, (x+1)*chunk) holds whole
+  // columns; groups of per_xcd columns are visited layer by layer), by integer arithmetic
+  const int cz = A.col_nez;
+  const int c_first = cz ? (x * chunk) / cz : 0;
+  const int c_count = cz ? (e_end - x * chunk) / cz : 0;
+  const auto elem_at = [&](int i) {
+    if (!cz) return A.elem_begin + i;
+    const int t = i - x * chunk;
+    const int g = t / (per_xcd * cz);
+    const int r = t - g * per_xcd * cz;
+    const int pg = min(per_xcd, c_count - g * per_xcd);
+    const int z = r / pg;
+    return c_first + g * per_xcd + (r - z * pg) + A.col_ncol * z;
+  };
   int le = x * chunk + slot;
   realw un[NF][3];
   realw kn = 0, mn = 0;
-  int ig_next = 0;
+  int ig_next = 0, ig_cur = 0;
+  int e_cur = 0, e_nxt = 0;
   if (le < e_end) {
-    const int e = A.elem_begin + le;
-    const size_t Lg = (size_t)e * 125 + tt;
+    e_cur = elem_at(le);
+    const size_t Lg = (size_t)e_cur * 125 + tt;
     const int ig = ld(&A.ibool[Lg]);
+    ig_cur = ig;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       const realw* U = f == 0 ? A.u0 : A.u1;
 #pragma unroll
       for (int c = 0; c < 3; ++c) un[f][c] = U[3 * (size_t)ig + c];
     }
-    load_moduli(Lg, e, kn, mn);
-    if (le + per_xcd < e_end) ig_next = ld(&A.ibool[(size_t)(e + per_xcd) * 125 + tt]);
+    load_moduli(Lg, e_cur, kn, mn);
+    if (le + per_xcd < e_end) {
+      e_nxt = elem_at(le + per_xcd);
+      ig_next = ld(&A.ibool[(size_t)e_nxt * 125 + tt]);
+    }
   }
   int b = 0;
   for (; le < e_end; le += per_xcd, b ^= 1) {
-    const int e = A.elem_begin + le;
+    const int e = e_cur;
+    const int ig = ig_cur;  // this lane's point of element e
     const size_t Lg = (size_t)e * 125 + tt;
     if (act) {
 #pragma unroll
@@ -717,7 +752,7 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
     }
     const realw kl = kn, ml = mn;
     if (le + per_xcd < e_end) {
-      const int e1 = e + per_xcd;
+      const int e1 = e_nxt;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const realw* U = f == 0 ? A.u0 : A.u1;
@@ -725,13 +760,19 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
         for (int c = 0; c < 3; ++c) un[f][c] = U[3 * (size_t)ig_next + c];
       }
       load_moduli((size_t)e1 * 125 + tt, e1, kn, mn);
-      if (le + 2 * per_xcd < e_end) ig_next = ld(&A.ibool[(size_t)(e1 + per_xcd) * 125 + tt]);
+      e_cur = e1;
+      ig_cur = ig_next;
+      if (le + 2 * per_xcd < e_end) {
+        e_nxt = elem_at(le + 2 * per_xcd);
+        ig_next = ld(&A.ibool[(size_t)e_nxt * 125 + tt]);
+      }
     }
     float pm = 0.0f, pk = 0.0f;
     if (KMODE == 2 && A.mu_part && act) {
       pm = ld(&A.mu_part[Lg]);
       pk = ld(&A.kappa_part[Lg]);
     }
+    const bool inner = INT && act && ig >= A.n_shared;
     __syncthreads();  // displacements of both fields in buffer b
     realw ea[6];
 #pragma unroll
@@ -787,7 +828,21 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
         s_s[b][f][3][t] = ml * exy2; s_s[b][f][4][t] = ml * exz2; s_s[b][f][5][t] = ml * eyz2;
       }
     }
+    // an interior point's inverse mass, velocities and density-kernel partial sum,
+    // loaded now so that they arrive during the divergence contraction
+    PointState P;
+    if (INT && inner) {
+      const size_t pb = 3 * (size_t)ig;
+      P.part = (NF == 2 && A.rho_part) ? A.rho_part[ig] : 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        P.r[c] = A.rm[pb + c];
+        P.v0[c] = A.F0.v[pb + c];
+        if (NF == 2) P.v1[c] = A.F1.v[pb + c];
+      }
+    }
     __syncthreads();  // stresses of both fields in buffer b
+    realw ff[NF][3];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       realw fx = 0, fy = 0, fz = 0;
@@ -798,6 +853,9 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
         fx += s_s[b][f][3][Lb] * hwJ[l]; fy += s_s[b][f][1][Lb] * hwJ[l]; fz += s_s[b][f][5][Lb] * hwJ[l];
         fx += s_s[b][f][4][Lc] * hwK[l]; fy += s_s[b][f][5][Lc] * hwK[l]; fz += s_s[b][f][2][Lc] * hwK[l];
       }
+      ff[f][0] = fx; ff[f][1] = fy; ff[f][2] = fz;
+      // every lane stores its element-local forces (whole rows: an element row with
+      // gaps costs more to write than the interior points' 12 bytes save)
       if (act) {
         realw* F = (f == 0 ? A.floc0 : A.floc1) + 3 * Lg;
         F[0] = fx;
@@ -805,13 +863,30 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
         F[2] = fz;
       }
     }
+    if (INT && inner) {
+      // the point's one element term, summed as the gather sums a row (0 + term),
+      // then the assembly's Newmark finish (same expressions, same rounding)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        P.u0[c] = s_u[b][0][c][t];
+        if (NF == 2) P.u1[c] = s_u[b][NF - 1][c][t];
+      }
+      const realw z = 0.0f;
+      if constexpr (NF == 2)
+        newmark_point<true>((size_t)ig, P, z + ff[0][0], z + ff[0][1], z + ff[0][2], z + ff[NF - 1][0],
+                            z + ff[NF - 1][1], z + ff[NF - 1][2], A.F0, A.F1, A.last != 0, A.rho_kl, A.rho_part,
+                            flush, A.hess_kl, A.dtk);
+      else
+        newmark_point<false>((size_t)ig, P, z + ff[0][0], z + ff[0][1], z + ff[0][2], z, z, z, A.F0, A.F1,
+                             A.last != 0, nullptr, nullptr, true, nullptr, A.dtk);
+    }
   }
 }
 
-template <int NF, int KMODE, int SP, bool NT = false>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : 5))) void k_forces_reg_pp(
+template <int NF, int KMODE, int SP, bool NT = false, bool INT = false>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : (INT ? 6 : 5)))) void k_forces_reg_pp(
     ForceArgs A) {
-  forces_reg_pp_body<NF, KMODE, SP, NT>(A);
+  forces_reg_pp_body<NF, KMODE, SP, NT, INT>(A);
 }
 
 // Two fields hold 6 more registers of look-ahead: 4 waves per SIMD (128 VGPRs)
@@ -835,17 +910,23 @@ __global__ __launch_bounds__(128) void k_forces_reg_aniso(ForceArgs A) {
   forces_reg_body<NF, ANISO, KMODE, SP>(A);
 }
 
-template <int NF, bool ANISO, int KMODE, int SP>
-void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, int variant = 1) {
+// Workgroups of a regular-pass launch: a multiple of 8 (one share per XCD);
+// blocks_per_cu = 0: one workgroup per element.
+int reg_grid(int elem_count, int blocks_per_cu) {
   int dev = 0, ncu = 0;
   HIP_CHECK(hipGetDevice(&dev));
   HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  // a multiple of 8 (one share per XCD); blocks_per_cu = 0: one workgroup per element
-  const int all = (a.elem_count + 7) & ~7;
-  int nb = blocks_per_cu > 0 ? std::min(all, (ncu * blocks_per_cu + 7) & ~7) : all;
-  nb = std::max(8, nb);
+  const int all = (elem_count + 7) & ~7;
+  const int nb = blocks_per_cu > 0 ? std::min(all, (ncu * blocks_per_cu + 7) & ~7) : all;
+  return std::max(8, nb);
+}
+
+template <int NF, bool ANISO, int KMODE, int SP>
+void launch_reg(const ForceArgs& a, hipStream_t s, int blocks_per_cu, int variant = 1) {
+  const int nb = reg_grid(a.elem_count, blocks_per_cu);
   if constexpr (ANISO) SEM_LAUNCH((k_forces_reg_aniso<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
   else if (variant == 2) SEM_LAUNCH((k_forces_reg_pipe<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
+  else if (variant == 3 && a.rm) SEM_LAUNCH((k_forces_reg_pp<NF, KMODE, SP, false, true>), dim3(nb), dim3(128), 0, s, a);
   else if (variant == 3) SEM_LAUNCH((k_forces_reg_pp<NF, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
   else if (variant == 4) SEM_LAUNCH((k_forces_reg_pp<NF, KMODE, SP, true>), dim3(nb), dim3(128), 0, s, a);
   else SEM_LAUNCH((k_forces_reg<NF, ANISO, KMODE, SP>), dim3(nb), dim3(128), 0, s, a);
@@ -862,8 +943,69 @@ void launch_t(const ForceArgs& a, hipStream_t s) {
 
 }  // namespace
 
+// Column-block traversal of a lattice mesh (element (ix,iy,iz) = ix + nex*(iy + ney*iz))
+// for the pipelined regular pass with P workgroups per XCD.  The pass gives XCD x the
+// traversal indices [x*chunk, (x+1)*chunk) and its P workgroups take P consecutive
+// indices per iteration.  Elements are listed column by column (a column = the nez
+// elements above one (ix,iy)), so each XCD owns whole columns; inside an XCD's share
+// groups of P consecutive columns are visited layer by layer.  One iteration then
+// covers a patch of P columns at one depth and the next one the patch above it: the
+// face an element shares with the element below was read one iteration earlier, when
+// it is still in the XCD's L2, where the element-order traversal reaches it
+// nex*ney elements later.  Every element appears once.  When chunk is not a multiple
+// of nez the shares would split columns, and the order is the element order.  The
+// pass computes the same map by integer arithmetic (forces_reg_pp_body, elem_at).
+std::vector<int> sem_column_block_order(int nex, int ney, int nez, int per_xcd) {
+  const int ncol = nex * ney, n = ncol * nez;
+  const int chunk = (n + 7) >> 3;
+  if (chunk % nez != 0) {  // the shares would split columns: element order (as the pass keeps it)
+    std::vector<int> id(n);
+    for (int e = 0; e < n; ++e) id[e] = e;
+    return id;
+  }
+  std::vector<int> cm(n);  // column-major list
+  for (int c = 0, q = 0; c < ncol; ++c)
+    for (int z = 0; z < nez; ++z) cm[q++] = c + ncol * z;
+  std::vector<int> order(n);
+  const int P = std::max(1, per_xcd);
+  for (int x = 0; x < 8; ++x) {
+    const int b = std::min(n, x * chunk), end = std::min(n, (x + 1) * chunk);
+    if (b >= end) continue;
+    // columns of this share in list order, and each element's group / depth / column
+    const int c0 = cm[b] % ncol;
+    std::vector<long long> key(end - b);
+    std::vector<int> pos(end - b);
+    for (int q = b; q < end; ++q) {
+      const int e = cm[q], c = e % ncol, z = e / ncol;
+      const long long g = (c - c0) / P;
+      key[q - b] = (g * nez + z) * (long long)ncol + c;
+      pos[q - b] = q;
+    }
+    std::stable_sort(pos.begin(), pos.end(), [&](int u, int v) { return key[u - b] < key[v - b]; });
+    for (int q = b; q < end; ++q) order[q] = cm[pos[q - b]];
+  }
+  return order;
+}
+
+extern "C" {
+// The column-block element order of the pipelined regular pass (regular_path 5)
+// for a nex x ney x nez lattice and per_xcd workgroups per XCD, 0-based
+// elements in traversal order (host only; no counterpart in the reference).
+void fwat_column_block_order_(int* nex, int* ney, int* nez, int* per_xcd, int* order) {
+  if (*nex <= 0 || *ney <= 0 || *nez <= 0 || *per_xcd <= 0) SEM_FAIL("fwat_column_block_order_: bad arguments");
+  const std::vector<int> o = sem_column_block_order(*nex, *ney, *nez, *per_xcd);
+  std::copy(o.begin(), o.end(), order);
+}
+}  // extern "C"
+
+bool sem_interior_active(const SemMesh* m, int nfields) {
+  return m->interior_newmark && m->n_shared < m->nglob && m->nspec_irregular == 0 && !m->anisotropy &&
+         (m->regular_fast_path == 3 || m->regular_fast_path == 5) && m->nslabs == 0 &&
+         !sem_brick_active(m, nfields) && m->d_floc != nullptr;
+}
+
 void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin, int elem_count,
-                       realw deltat, const int* elem_list, bool gather) {
+                       realw deltat, const int* elem_list, bool gather, const SemInterior* in) {
   if (elem_count <= 0) return;
   ForceArgs a{};
   a.ibool = m->d_ibool;
@@ -926,8 +1068,34 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   if (gather && m->nspec_irregular == 0 && m->regular_fast_path && kernel_mode != 1) {
     // regular_fast_path 2: the software-pipelined isotropic pass, 3: the same with two
     // barriers per element (ping-pong LDS buffers)
-    const int pipe = m->regular_fast_path;
+    int pipe = m->regular_fast_path;
     const int bpc = nfields == 2 ? m->blocks_per_cu_adj : m->blocks_per_cu_fwd;
+    if (pipe == 5) {
+      // 5: the ping-pong pass in column-block order on a lattice mesh over the whole mesh,
+      // when every XCD's share is whole columns (anything else keeps element order)
+      pipe = 3;
+      const int chunk = (elem_count + 7) >> 3;
+      if (!an && m->lat_nex > 0 && elem_begin == 0 && elem_count == m->nspec && chunk % m->lat_nez == 0) {
+        a.col_ncol = m->lat_nex * m->lat_ney;
+        a.col_nez = m->lat_nez;
+      }
+    }
+    if (in && pipe == 3 && !an && kernel_mode != 1 && elem_begin == 0 && elem_count == m->nspec &&
+        sem_interior_active(m, nfields)) {
+      // the pass finishes the points [n_shared, nglob) (Newmark, density kernel)
+      a.n_shared = m->n_shared;
+      a.last = in->last ? 1 : 0;
+      a.rm = m->d_rmass_inv;
+      a.F0 = in->F0;
+      a.F1 = in->F1;
+      a.dtk = in->dtk;
+      if (nfields == 2) {
+        a.rho_kl = m->d_rho_kl_glob;
+        a.rho_part = m->kernel_block > 1 ? m->d_rho_part : nullptr;
+        a.hess_kl = m->d_hess_kl_glob;
+        if (!a.rho_kl) SEM_FAIL("two-field pass needs the density kernel accumulator");
+      }
+    }
     if (nfields == 2) {
       if (an ? !m->d_cijkl_kl : (!m->d_mu_kl || !m->d_kappa_kl))
         SEM_FAIL("two-field force pass needs the %s kernel accumulators", an ? "cijkl" : "mu/kappa");

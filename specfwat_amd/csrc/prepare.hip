@@ -165,6 +165,46 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
     if (g < 0 || g >= m->nglob) SEM_FAIL("ibool(%zu) = %d outside 1..NGLOB_AB=%d", L, g + 1, m->nglob);
     ib[L] = g;
   }
+  {
+    // library numbering: a point is "interior" when exactly one element-local slot
+    // refers to it and no Stacey face, source or receiver term does
+    std::vector<uint8_t> touch(m->nglob, 0);  // element-local slots, saturating at 2
+    for (size_t L = 0; L < nloc; ++L) touch[ib[L]] = (uint8_t)std::min(2, touch[ib[L]] + 1);
+    const int nf = *ABSORBING_CONDITIONS ? *h_num_abs_boundary_faces : 0;
+    for (int f = 0; f < nf; ++f) {
+      const int e = h_abs_boundary_ispec[f] - 1;
+      if (e < 0 || e >= m->nspec) SEM_FAIL("abs_boundary_ispec(%d) = %d out of range", f + 1, e + 1);
+      for (int q = 0; q < 25; ++q) {
+        const size_t fq = (size_t)f * 25 + q;
+        const int i = h_abs_boundary_ijk[fq * 3] - 1, j = h_abs_boundary_ijk[fq * 3 + 1] - 1,
+                  k = h_abs_boundary_ijk[fq * 3 + 2] - 1;
+        if (i < 0 || i > 4 || j < 0 || j > 4 || k < 0 || k > 4) SEM_FAIL("abs_boundary_ijk out of range");
+        touch[ib[(size_t)e * 125 + i + 5 * j + 25 * k]] = 2;
+      }
+    }
+    const auto whole_element = [&](int e) {
+      if (e < 0 || e >= m->nspec) return;  // checked with a message below
+      for (int L = 0; L < 125; ++L) touch[ib[(size_t)e * 125 + L]] = 2;
+    };
+    for (int q = 0; q < *NSOURCES; ++q) whole_element(h_ispec_selected_source[q] - 1);
+    for (int q = 0; q < *nrec; ++q) whole_element(h_ispec_selected_rec[q] - 1);
+    int n_sh = 0;
+    std::vector<int> perm(m->nglob, -1);
+    for (int g = 0; g < m->nglob; ++g)
+      if (touch[g] != 1) perm[g] = n_sh++;
+    int k = n_sh;
+    for (size_t L = 0; L < nloc; ++L)
+      if (touch[ib[L]] == 1) perm[ib[L]] = k++;
+    if (k != m->nglob) SEM_FAIL("point numbering: %d of %d points placed", k, m->nglob);
+    m->n_shared = n_sh;
+    if (n_sh < m->nglob) {
+      uint64_t h = 1469598103934665603ull;  // FNV-1a over the numbering
+      for (int g = 0; g < m->nglob; ++g) { h ^= (uint32_t)perm[g]; h *= 1099511628211ull; }
+      m->perm_hash = h;
+      for (size_t L = 0; L < nloc; ++L) ib[L] = perm[ib[L]];
+      m->h_perm.swap(perm);
+    }
+  }
   m->d_ibool = dev_upload(ib.data(), nloc, s);
   std::vector<int> irr(h_irregular_element_number, h_irregular_element_number + m->nspec);
   for (int e = 0; e < m->nspec; ++e)
@@ -285,9 +325,10 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   const size_t nloc = (size_t)m->nspec * 125, n3 = (size_t)m->nglob * 3;
   std::vector<realw> rm(n3);
   for (int g = 0; g < m->nglob; ++g) {
-    rm[3 * (size_t)g] = rmassx[g];
-    rm[3 * (size_t)g + 1] = rmassy[g];
-    rm[3 * (size_t)g + 2] = rmassz[g];
+    const size_t p = 3 * (size_t)sem_point(m, g);
+    rm[p] = rmassx[g];
+    rm[p + 1] = rmassy[g];
+    rm[p + 2] = rmassz[g];
   }
   m->d_rmass_inv = dev_upload(rm.data(), n3, s);
   m->anisotropy = *ANISOTROPY != 0;
@@ -548,47 +589,77 @@ void fwat_assembly_csr_(int* NSPEC_AB, int* NGLOB_AB, int* h_ibool, int* num_abs
   std::memcpy(idx, i.data(), i.size() * sizeof(int));
 }
 
-// transfers of the forward / adjoint field (Fortran displ(3,NGLOB_AB) layout)
+}  // extern "C"
+
+// transfers of the forward / adjoint field (Fortran displ(3,NGLOB_AB) layout, the
+// caller's point numbering; the device holds the library numbering, sem_point)
+void sem_fields_to_device(SemMesh* m, int size, realw* const host[3], realw* const dev[3]) {
+  if (size != 3 * m->nglob) SEM_FAIL("field transfer: size %d != 3*NGLOB_AB", size);
+  const size_t n = (size_t)size;
+  std::vector<realw> buf;
+  for (int q = 0; q < 3; ++q) {
+    const realw* src = host[q];
+    if (!m->h_perm.empty()) {
+      buf.resize(n);
+      for (int g = 0; g < m->nglob; ++g) {
+        const size_t p = 3 * (size_t)m->h_perm[g];
+        buf[p] = src[3 * (size_t)g]; buf[p + 1] = src[3 * (size_t)g + 1]; buf[p + 2] = src[3 * (size_t)g + 2];
+      }
+      src = buf.data();
+    }
+    HIP_CHECK(hipMemcpyAsync(dev[q], src, n * sizeof(realw), hipMemcpyHostToDevice, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+  }
+}
+
+void sem_fields_from_device(SemMesh* m, int size, realw* const host[3], realw* const dev[3]) {
+  if (size != 3 * m->nglob) SEM_FAIL("field transfer: size %d != 3*NGLOB_AB", size);
+  const size_t n = (size_t)size;
+  std::vector<realw> buf(m->h_perm.empty() ? 0 : n);
+  for (int q = 0; q < 3; ++q) {
+    realw* dst = m->h_perm.empty() ? host[q] : buf.data();
+    HIP_CHECK(hipMemcpyAsync(dst, dev[q], n * sizeof(realw), hipMemcpyDeviceToHost, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    if (!m->h_perm.empty())
+      for (int g = 0; g < m->nglob; ++g) {
+        const size_t p = 3 * (size_t)m->h_perm[g];
+        host[q][3 * (size_t)g] = buf[p]; host[q][3 * (size_t)g + 1] = buf[p + 1]; host[q][3 * (size_t)g + 2] = buf[p + 2];
+      }
+  }
+}
+
+extern "C" {
+
 void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
   SemMesh* m = sem_mesh(Mesh_pointer);
-  const size_t n = (size_t)*size * sizeof(realw);
-  HIP_CHECK(hipMemcpyAsync(m->d_displ, displ, n, hipMemcpyHostToDevice, m->stream));
-  HIP_CHECK(hipMemcpyAsync(m->d_veloc, veloc, n, hipMemcpyHostToDevice, m->stream));
-  HIP_CHECK(hipMemcpyAsync(m->d_accel, accel, n, hipMemcpyHostToDevice, m->stream));
-  HIP_CHECK(hipStreamSynchronize(m->stream));
+  realw* const h[3] = {displ, veloc, accel};
+  realw* const d[3] = {m->d_displ, m->d_veloc, m->d_accel};
+  sem_fields_to_device(m, *size, h, d);
 }
 
 void transfer_fields_el_from_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
   SemMesh* m = sem_mesh(Mesh_pointer);
-  const size_t n = (size_t)*size * sizeof(realw);
-  HIP_CHECK(hipMemcpyAsync(displ, m->d_displ, n, hipMemcpyDeviceToHost, m->stream));
-  HIP_CHECK(hipMemcpyAsync(veloc, m->d_veloc, n, hipMemcpyDeviceToHost, m->stream));
-  HIP_CHECK(hipMemcpyAsync(accel, m->d_accel, n, hipMemcpyDeviceToHost, m->stream));
-  HIP_CHECK(hipStreamSynchronize(m->stream));
+  realw* const h[3] = {displ, veloc, accel};
+  realw* const d[3] = {m->d_displ, m->d_veloc, m->d_accel};
+  sem_fields_from_device(m, *size, h, d);
 }
 
 void transfer_b_fields_to_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (!m->d_b_displ) SEM_FAIL("transfer_b_fields_to_device_: adjoint fields not prepared");
-  const size_t n = (size_t)*size * sizeof(realw);
-  HIP_CHECK(hipMemcpyAsync(m->d_b_displ, b_displ, n, hipMemcpyHostToDevice, m->stream));
-  HIP_CHECK(hipMemcpyAsync(m->d_b_veloc, b_veloc, n, hipMemcpyHostToDevice, m->stream));
-  HIP_CHECK(hipMemcpyAsync(m->d_b_accel, b_accel, n, hipMemcpyHostToDevice, m->stream));
-  HIP_CHECK(hipStreamSynchronize(m->stream));
+  realw* const h[3] = {b_displ, b_veloc, b_accel};
+  realw* const d[3] = {m->d_b_displ, m->d_b_veloc, m->d_b_accel};
+  sem_fields_to_device(m, *size, h, d);
 }
 
 void transfer_b_fields_from_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (!m->d_b_displ) SEM_FAIL("transfer_b_fields_from_device_: adjoint fields not prepared");
-  const size_t n = (size_t)*size * sizeof(realw);
-  HIP_CHECK(hipMemcpyAsync(b_displ, m->d_b_displ, n, hipMemcpyDeviceToHost, m->stream));
-  HIP_CHECK(hipMemcpyAsync(b_veloc, m->d_b_veloc, n, hipMemcpyDeviceToHost, m->stream));
-  HIP_CHECK(hipMemcpyAsync(b_accel, m->d_b_accel, n, hipMemcpyDeviceToHost, m->stream));
-  HIP_CHECK(hipStreamSynchronize(m->stream));
+  realw* const h[3] = {b_displ, b_veloc, b_accel};
+  realw* const d[3] = {m->d_b_displ, m->d_b_veloc, m->d_b_accel};
+  sem_fields_from_device(m, *size, h, d);
 }
 
-// Stacey boundary store of a forward run: (NDIM,NGLLSQUARE,num_abs_boundary_faces,NSTEP)
-// to host (SpecFWAT writes it to absorb_field.bin, specfem_api.f90:359-361) and back.
 void transfer_stacey_store_from_device_(long* Mesh_pointer, realw* h_store) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (!m->d_stacey_store) SEM_FAIL("no Stacey store allocated");

@@ -117,6 +117,15 @@ struct SemMesh {
   hipStream_t stream = nullptr;
 
   // geometry / indexing
+  // Library point numbering (prepare_constants_device_): points touched by more than one
+  // element-local slot or by a Stacey face, source or receiver term come first, in the
+  // caller's order, [0, n_shared); points touched by one element-local slot only follow,
+  // element by element.  h_perm[caller point] = library point (empty: the caller's
+  // numbering, every point shared); perm_hash identifies the numbering.
+  int n_shared = 0;
+  std::vector<int> h_perm;
+  uint64_t perm_hash = 0;
+  int interior_newmark = 1;  // fused loops: the element pass finishes points [n_shared, nglob)
   int* d_ibool = nullptr;
   int* d_irregular = nullptr;  // [nspec], 0 regular else 1-based
   realw* d_jinv10 = nullptr;
@@ -197,6 +206,7 @@ struct SemMesh {
   realw* d_floc_b = nullptr;    // second field of the adjoint pass
   realw* d_face_scratch = nullptr;  // [nfaces*25][3] absorbing term of a non-stored field
   int asm_points_per_lane = 0;  // gather assembly: 1 or 2 points per lane, 0 = by pass (see stage_assemble)
+  int asm_block = 256;          // gather assembly: lanes per workgroup (64..1024)
   int* d_asm_off = nullptr;     // [nglob+1]
   int* d_asm_idx = nullptr;     // [ncontrib]
   size_t n_contrib = 0;
@@ -234,6 +244,9 @@ struct SemMesh {
   double last_asm_ms = 0.0;
   int last_asm_launches = 0;
 };
+
+// library number of the caller's point g (0-based)
+inline int sem_point(const SemMesh* m, int g) { return m->h_perm.empty() ? g : m->h_perm[g]; }
 
 inline SemMesh* sem_mesh(long* Mesh_pointer) {
   SemMesh* m = reinterpret_cast<SemMesh*>(*Mesh_pointer);
@@ -332,6 +345,12 @@ __device__ __forceinline__ PointState load_point(size_t g, const realw* __restri
   return P;
 }
 
+__device__ __forceinline__ void store3(realw* p, realw x, realw y, realw z) {
+  p[0] = x;
+  p[1] = y;
+  p[2] = z;
+}
+
 template <bool TWO>
 __device__ __forceinline__ void newmark_point(size_t g, const PointState& P, realw f0x, realw f0y, realw f0z,
                                               realw f1x, realw f1y, realw f1z, const AsmField& F0,
@@ -363,26 +382,23 @@ __device__ __forceinline__ void newmark_point(size_t g, const PointState& P, rea
       rho_part[g] = P.part + x;
     }
     if (hess_kl) hess_kl[g] = fma(dtk, (double)fmaf(A[2], BA[2], fmaf(A[1], BA[1], A[0] * BA[0])), hess_kl[g]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (last) {
-        F1.a[b + c] = BA[c];
-        F1.v[b + c] = BV[c];
-      } else {
-        F1.u[b + c] = fmaf(s1, BA[c], fmaf(d1, BV[c], BU[c]));
-        F1.v[b + c] = fmaf(h1, BA[c], BV[c]);
-      }
+    // each point's three components are stored together (one 12-byte store per array)
+    if (last) {
+      store3(F1.a + b, BA[0], BA[1], BA[2]);
+      store3(F1.v + b, BV[0], BV[1], BV[2]);
+    } else {
+      store3(F1.u + b, fmaf(s1, BA[0], fmaf(d1, BV[0], BU[0])), fmaf(s1, BA[1], fmaf(d1, BV[1], BU[1])),
+             fmaf(s1, BA[2], fmaf(d1, BV[2], BU[2])));
+      store3(F1.v + b, fmaf(h1, BA[0], BV[0]), fmaf(h1, BA[1], BV[1]), fmaf(h1, BA[2], BV[2]));
     }
   }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (last) {
-      F0.a[b + c] = A[c];
-      F0.v[b + c] = V[c];
-    } else {
-      F0.u[b + c] = fmaf(s0, A[c], fmaf(d0, V[c], U[c]));
-      F0.v[b + c] = fmaf(h0, A[c], V[c]);
-    }
+  if (last) {
+    store3(F0.a + b, A[0], A[1], A[2]);
+    store3(F0.v + b, V[0], V[1], V[2]);
+  } else {
+    store3(F0.u + b, fmaf(s0, A[0], fmaf(d0, V[0], U[0])), fmaf(s0, A[1], fmaf(d0, V[1], U[1])),
+           fmaf(s0, A[2], fmaf(d0, V[2], U[2])));
+    store3(F0.v + b, fmaf(h0, A[0], V[0]), fmaf(h0, A[1], V[1]), fmaf(h0, A[2], V[2]));
   }
 }
 
@@ -403,9 +419,27 @@ __device__ __forceinline__ void newmark_point(size_t g, const realw* __restrict_
 void sem_build_assembly_csr(int nglob, int nspec, const std::vector<int>& owner, const std::vector<int>& visit,
                             std::vector<int>& off, std::vector<int>& idx);
 
+// Newmark finish of the points [n_shared, nglob) inside the element pass of a fused
+// loop (each such point has one element-local term, so its force is complete there):
+// the fields and time steps of the pass, this step's last-step flag and the kernel step.
+struct SemInterior {
+  AsmField F0, F1;
+  bool last;
+  double dtk;
+};
+
 // internal launchers shared between the C-ABI files
+// (in: the element pass also finishes the element-interior points when
+// sem_interior_active(m, nfields); the assembly then covers [0, n_shared) only)
 void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin, int elem_count,
-                       realw deltat, const int* elem_list, bool gather);
+                       realw deltat, const int* elem_list, bool gather, const SemInterior* in = nullptr);
+// true when a whole-mesh fused element pass of nfields fields finishes the element-interior points
+bool sem_interior_active(const SemMesh* m, int nfields);
+// host <-> device copies of three point fields (u, v, a) of size 3 * nglob in the
+// caller's numbering (the device arrays hold the library numbering)
+void sem_fields_to_device(SemMesh* m, int size, realw* const host[3], realw* const dev[3]);
+void sem_fields_from_device(SemMesh* m, int size, realw* const host[3], realw* const dev[3]);
+
 void sem_zero_kernels(SemMesh* m);
 // brick-fused pass (brick.hip): true when the mesh handle runs it
 bool sem_brick_active(const SemMesh* m, int nfields);

@@ -559,21 +559,23 @@ void stage_stacey_face(SemMesh* m, const realw* v, realw* out, int it = 0) {
                      m->d_face_coef, v, out, fk_step(m, it));
 }
 
-// Assembly + Newmark of points 0..nglob-1, or of slab `slab`'s point list.
+// Assembly + Newmark of points 0..nglob-1 (0..n_shared-1 when the element pass
+// finished the others: interior), or of slab `slab`'s point list.
 void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const AsmField& F1, double dtk,
-                    int slab = -1) {
+                    int slab = -1, bool interior = false) {
   const int nfp = m->absorbing ? m->nfaces * 25 : 0;
   float* rho_part = m->kernel_block > 1 ? m->d_rho_part : nullptr;
   const int nloc = m->nspec * 125, nsp = m->nsources * 125;
   const int* glist = slab >= 0 ? m->d_slab_g + m->h_slab_goff[slab] : nullptr;
-  const int npts = slab >= 0 ? m->h_slab_goff[slab + 1] - m->h_slab_goff[slab] : m->nglob;
+  const int npts = slab >= 0 ? m->h_slab_goff[slab + 1] - m->h_slab_goff[slab] : (interior ? m->n_shared : m->nglob);
   if (npts == 0) return;
-  const dim3 bl(TPB);
+  const int tpb = m->asm_block;
+  const dim3 bl(tpb);
   // 0 (default): two points per lane for the one-field pass, one for the two-field pass
   // (measured on the C2 mesh: 0.656 vs 0.668 ms one-field, 1.32 vs 1.21 ms two-field)
   const int ppl = m->asm_points_per_lane ? m->asm_points_per_lane : (two ? 1 : 2);
   if (ppl == 2) {
-    const dim3 g2((unsigned)((npts + 2 * TPB - 1) / (2 * TPB)));
+    const dim3 g2((unsigned)((npts + 2 * tpb - 1) / (2 * tpb)));
 #define ASM2(L, T, RK, RP, FL, HK)                                                                              \
   SEM_LAUNCH((k_assemble2<L, T>), g2, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp, nsp, \
              m->d_sourcearrays, m->d_rec_h, m->d_rmass_inv, F0, F1, RK, RP, FL, HK, dtk)
@@ -587,7 +589,7 @@ void stage_assemble(SemMesh* m, bool last, bool two, const AsmField& F0, const A
 #undef ASM2
     return;
   }
-  const dim3 gr(nblk(npts));
+  const dim3 gr(nblk(npts, tpb));
   if (two) {
     if (last)
       SEM_LAUNCH((k_assemble<true, true>), gr, bl, 0, m->stream, npts, glist, m->d_asm_off, m->d_asm_idx, nloc, nfp,
@@ -698,9 +700,10 @@ struct LoopGraph {
   }
 };
 
-void stage_forces_timed(SemMesh* m, int nfields, int kmode, realw dt, ForceTimer* tm, bool gather) {
+void stage_forces_timed(SemMesh* m, int nfields, int kmode, realw dt, ForceTimer* tm, bool gather,
+                        const SemInterior* in = nullptr) {
   if (tm) tm->open(m->stream);
-  sem_launch_forces(m, nfields, kmode, 0, m->nspec, dt, nullptr, gather);
+  sem_launch_forces(m, nfields, kmode, 0, m->nspec, dt, nullptr, gather, in);
   if (tm) tm->close(m->stream);
 }
 
@@ -946,6 +949,17 @@ void fwat_copy_forward_to_backward_(long* Mesh_pointer_from, long* Mesh_pointer_
   if (a->nglob != b->nglob || !b->d_b_displ) SEM_FAIL("fwat_copy_forward_to_backward_: incompatible handles");
   const size_t n = (size_t)a->nglob * 3 * sizeof(realw);
   HIP_CHECK(hipStreamSynchronize(a->stream));
+  if (a->n_shared != b->n_shared || a->perm_hash != b->perm_hash || a->h_perm.size() != b->h_perm.size()) {
+    // the handles number their points differently (other source or receiver elements):
+    // through the host, in the caller's numbering
+    std::vector<realw> u(3 * (size_t)a->nglob), v(u.size()), w(u.size());
+    realw* const h[3] = {u.data(), v.data(), w.data()};
+    realw* const da[3] = {a->d_displ, a->d_veloc, a->d_accel};
+    realw* const db[3] = {b->d_b_displ, b->d_b_veloc, b->d_b_accel};
+    sem_fields_from_device(a, 3 * a->nglob, h, da);
+    sem_fields_to_device(b, 3 * b->nglob, h, db);
+    return;
+  }
   HIP_CHECK(hipMemcpyAsync(b->d_b_displ, a->d_displ, n, hipMemcpyDeviceToDevice, b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_b_veloc, a->d_veloc, n, hipMemcpyDeviceToDevice, b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_b_accel, a->d_accel, n, hipMemcpyDeviceToDevice, b->stream));
@@ -1043,9 +1057,14 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       const int nfp = m->nfaces * 25;
       LoopGraph lg(m, graph);
       stage_predictor(m, false, dt);  // u_1, v_1/2 from the initial state
+      // the element pass finishes the element-interior points (their Newmark update
+      // needs nothing but their own element's term); the assembly covers the others
+      const bool inner = sem_interior_active(m, 1);
       for (int it = 1; it <= N; ++it) {
         step_begin(it);
-        stage_forces_timed(m, 1, 0, dt, tm, true);
+        const SemInterior in{AsmField{nullptr, nullptr, nullptr, nullptr, m->d_displ, m->d_veloc, m->d_accel, dt},
+                             AsmField{}, it == N, 0.0};
+        stage_forces_timed(m, 1, 0, dt, tm, true, inner ? &in : nullptr);
         realw* face = nullptr;
         if (m->absorbing && m->nfaces) {
           face = smode ? m->d_stacey_store + (size_t)(it - 1) * nfp * 3 : m->d_face_scratch;
@@ -1055,7 +1074,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         AsmField F0{m->d_floc, face, m->nsources ? m->d_stf + (size_t)(it - 1) * m->nsources : nullptr,
                     nullptr, m->d_displ, m->d_veloc, m->d_accel, dt};
         if (ta) ta->open(s);
-        stage_assemble(m, it == N, false, F0, F0, 0.0);
+        stage_assemble(m, it == N, false, F0, F0, 0.0, -1, inner);
         if (ta) ta->close(s);
       }
       lg.launch();
@@ -1151,11 +1170,15 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
       const int nfp = m->nfaces * 25;
       LoopGraph lg(m, graph);
       stage_predictor(m, false, dt);  // adjoint field (from rest)
+      const bool inner = sem_interior_active(m, 2);
       for (int it = 1; it <= N; ++it) {
         step_begin(it);
         const int rec0 = N - it;  // 0-based record / source row of forward step N + 1 - it
         m->kernel_flush = closes_block(it);
-        stage_forces_timed(m, 2, 2, dt, tm, true);
+        const SemInterior in{AsmField{nullptr, nullptr, nullptr, nullptr, m->d_displ, m->d_veloc, m->d_accel, dt},
+                             AsmField{nullptr, nullptr, nullptr, nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt},
+                             it == N, dtk};
+        stage_forces_timed(m, 2, 2, dt, tm, true, inner ? &in : nullptr);
         realw* face_a = nullptr;
         if (m->absorbing && m->nfaces) {
           face_a = m->d_face_scratch;
@@ -1167,7 +1190,7 @@ void fwat_iterate_time_(long* Mesh_pointer, int* SIMULATION_TYPE, realw* DT, int
         AsmField F1{m->d_floc_b, face_b, m->nsources ? m->d_stf + (size_t)rec0 * m->nsources : nullptr,
                     nullptr, m->d_b_displ, m->d_b_veloc, m->d_b_accel, -dt};
         if (ta) ta->open(s);
-        stage_assemble(m, it == N, true, F0, F1, dtk);
+        stage_assemble(m, it == N, true, F0, F1, dtk, -1, inner);
         if (ta) ta->close(s);
       }
       lg.launch();
@@ -1291,10 +1314,40 @@ void fwat_copy_probe_unrolled_(long* nbytes, int* reps, int* unroll, int* blocks
 // Points per lane of the gather assembly: 1 (k_assemble), 2 (k_assemble2, twice
 // the loads in flight per lane; the same sums) or 0 (default: 2 for the one-field
 // pass, 1 for the two-field pass)
+// Lanes per workgroup of the gather assembly launches (a multiple of 64 in 64..1024;
+// default 256).  Consecutive workgroups go to the XCDs in turn, so larger workgroups keep
+// longer runs of consecutive points (and the element rows they share) on one XCD.
+void fwat_set_assembly_block_(long* Mesh_pointer, int* lanes) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  if (*lanes < 64 || *lanes > 1024 || *lanes % 64) SEM_FAIL("fwat_set_assembly_block_: 64..1024, a multiple of 64");
+  m->asm_block = *lanes;
+}
+
 void fwat_set_assembly_lanes_(long* Mesh_pointer, int* points_per_lane) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (*points_per_lane < 0 || *points_per_lane > 2) SEM_FAIL("fwat_set_assembly_lanes_: 0, 1 or 2 points per lane");
   m->asm_points_per_lane = *points_per_lane;
+}
+
+// Element-interior points (one element-local term, no face, source or receiver
+// term; numbered after the shared points by prepare_constants_device_) finished by
+// the element pass of the fused loops: 1 = on (default; pipelined passes 3 and 5 on
+// isotropic all-regular meshes, without slabs or bricks), 0 = every point through
+// the gather assembly.  Results are the same bit for bit.
+void fwat_set_interior_(long* Mesh_pointer, int* on) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  m->interior_newmark = *on != 0;
+}
+
+// Points the gather assembly of a fused loop covers, and the element-interior points
+// the element pass finishes when fwat_set_interior_ is on (for a forward and a
+// two-field pass with the current launch shape).
+void fwat_interior_info_(long* Mesh_pointer, int* n_shared, int* n_interior, int* active_fwd, int* active_adj) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  *n_shared = m->n_shared;
+  *n_interior = m->nglob - m->n_shared;
+  *active_fwd = sem_interior_active(m, 1) ? 1 : 0;
+  *active_adj = sem_interior_active(m, 2) ? 1 : 0;
 }
 
 // HIP graph capture of the whole-run loop (fused gather path): 1 = every
@@ -1309,14 +1362,15 @@ void fwat_set_graph_(long* Mesh_pointer, int* on) {
 // specfwat_amd/tune.py): regular_path 1 = folded-table kernel on all-regular
 // meshes, 2 = its software-pipelined form, 3 = the pipelined form with two
 // barriers per element, 4 = form 3 with non-temporal hints on the streamed
-// arrays (isotropic; anisotropic passes keep 1), 0 = general
+// arrays, 5 = form 3 visiting the elements of a lattice mesh in column blocks
+// (isotropic; anisotropic passes keep 1), 0 = general
 // kernel; blocks_per_cu_* = persistent workgroups per CU
 // for the one-field and two-field passes, 0 = one workgroup per element.
 void fwat_set_tuning_(long* Mesh_pointer, int* regular_path, int* blocks_per_cu_fwd, int* blocks_per_cu_adj) {
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (*blocks_per_cu_fwd < 0 || *blocks_per_cu_adj < 0 || *blocks_per_cu_fwd > 64 || *blocks_per_cu_adj > 64)
     SEM_FAIL("fwat_set_tuning_: blocks per CU must be in 0..64");
-  if (*regular_path < 0 || *regular_path > 4) SEM_FAIL("fwat_set_tuning_: regular_path must be 0 to 4");
+  if (*regular_path < 0 || *regular_path > 5) SEM_FAIL("fwat_set_tuning_: regular_path must be 0 to 5");
   m->regular_fast_path = *regular_path;
   m->blocks_per_cu_fwd = *blocks_per_cu_fwd;
   m->blocks_per_cu_adj = *blocks_per_cu_adj;

@@ -117,11 +117,13 @@ class ElasticRun:
             _lib.call("fwat_set_stf_", self.h, self.stf, self.nstep)
 
     def set_tuning(self, regular_path=1, blocks_per_cu_fwd=0, blocks_per_cu_adj=8, brick=(0, 0, 0),
-                   brick_blocks_per_cu=0, slab_elems=0):
+                   brick_blocks_per_cu=0, slab_elems=0, interior=True):
         """Element-pass launch shape; brick=(bx, by, bz) switches the brick-fused
         element + assembly pass on (lattice meshes of regular elements only);
-        slab_elems > 0 runs element pass + assembly slab after slab."""
+        slab_elems > 0 runs element pass + assembly slab after slab; interior: the
+        pipelined passes finish the element-interior points (set_interior)."""
         _lib.call("fwat_set_tuning_", self.h, int(regular_path), int(blocks_per_cu_fwd), int(blocks_per_cu_adj))
+        _lib.call("fwat_set_interior_", self.h, int(interior))
         _lib.call("fwat_set_slabs_", self.h, int(slab_elems))
         if brick[0] and not self.lattice:
             raise ValueError("the brick pass needs a lattice mesh of regular elements")
@@ -133,6 +135,22 @@ class ElasticRun:
         """Points per lane of the gather assembly (1 or 2; the same sums), 0 for
         the default (2 for the one-field pass, 1 for the two-field pass)."""
         _lib.call("fwat_set_assembly_lanes_", self.h, int(points_per_lane))
+
+    def set_assembly_block(self, lanes):
+        """Lanes per workgroup of the gather assembly (64..1024, default 256)."""
+        _lib.call("fwat_set_assembly_block_", self.h, int(lanes))
+
+    def set_interior(self, on=True):
+        """Element-interior points finished by the element pass of the fused loops
+        (pipelined passes 3 and 5; the same results bit for bit) or assembled by the gather."""
+        _lib.call("fwat_set_interior_", self.h, int(on))
+
+    def interior_info(self):
+        """(points the gather assembles, element-interior points, forward pass finishes them,
+        two-field pass finishes them) with the current launch shape."""
+        out = [np.zeros(1, np.int32) for _ in range(4)]
+        _lib.call("fwat_interior_info_", self.h, *out)
+        return tuple(int(o[0]) for o in out)
 
     def set_graph(self, on=True):
         """Capture every whole-run loop of this handle into one HIP graph and launch

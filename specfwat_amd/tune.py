@@ -14,18 +14,20 @@ import numpy as np
 from .solver import ElasticRun
 
 # (regular_path, workgroups per CU): 0 general kernel, 1 folded-table kernel, 2 its
-# software-pipelined form, 3 the pipelined form with two barriers per element (isotropic);
+# software-pipelined form, 3 the pipelined form with two barriers per element (isotropic),
+# 5 form 3 visiting a lattice mesh's elements in column blocks (form 3 elsewhere);
 # 0 workgroups per CU = one workgroup per element
 CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32), (2, 6), (2, 8), (2, 12), (2, 16),
-              (3, 6), (3, 8), (3, 12), (3, 16)]
+              (3, 6), (3, 8), (3, 12), (3, 16), (5, 6), (5, 8), (5, 12), (5, 16)]
 BRICKS = [(2, 2, 2), (4, 2, 2), (4, 4, 2)]
 SLABS = [6, 12, 24]  # slabs per mesh
+INTERIOR_BPC = [6, 8, 10, 12]  # workgroups per CU tried with the element-interior points in the pipelined pass
 
 
 def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidates=CANDIDATES,
-                        static_precision=0, bricks=BRICKS, slabs=SLABS):
-    """Returns ({'regular_path', 'blocks_per_cu_fwd', 'blocks_per_cu_adj'[, 'brick']}, table);
-    the dict is ElasticRun.set_tuning's keyword arguments."""
+                        static_precision=0, bricks=BRICKS, slabs=SLABS, interior_bpc=INTERIOR_BPC):
+    """Returns ({'regular_path', 'blocks_per_cu_fwd', 'blocks_per_cu_adj', 'interior'[, 'brick'
+    | 'slab_elems']}, table); the dict is ElasticRun.set_tuning's keyword arguments."""
     rho, vp, vs = model
     f = ElasticRun(mesh, rho, vp, vs, sources, receivers, dt, nstep, 1, True,
                    static_precision=static_precision)
@@ -36,8 +38,9 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
     f.iterate_time()  # warm-up, also fills the Stacey store for the adjoint runs
     a.take_forward(f)
     for reg, bpc in candidates:
-        f.set_tuning(reg, bpc, bpc)
-        a.set_tuning(reg, bpc, bpc)
+        # element passes alone, every point through the gather (interior points below)
+        f.set_tuning(reg, bpc, bpc, interior=False)
+        a.set_tuning(reg, bpc, bpc, interior=False)
         f.iterate_time(timed=True)
         fms, fn = f.force_time()
         a.iterate_time(timed=True)
@@ -48,18 +51,36 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
     reg = min(regs, key=lambda g: min(r[2] for r in table if r[0] == g) + min(r[3] for r in table if r[0] == g))
     bf = min((r for r in table if r[0] == reg), key=lambda r: r[2])[1]
     ba = min((r for r in table if r[0] == reg), key=lambda r: r[3])[1]
-    best = {"regular_path": reg, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba}
+    best = {"regular_path": reg, "blocks_per_cu_fwd": bf, "blocks_per_cu_adj": ba, "interior": False}
+
+    def step_ms(**kw):
+        # element + assembly time per step of the forward and the adjoint pass
+        f.set_tuning(**kw)
+        a.set_tuning(**kw)
+        f.iterate_time(timed=True)
+        fe, fa = f.force_time(), f.assembly_time()
+        a.iterate_time(timed=True)
+        ae, aa = a.force_time(), a.assembly_time()
+        return (fe[0] + fa[0]) / max(fe[1], 1), (ae[0] + aa[0]) / max(ae[1], 1)
+
+    # the pipelined passes (3, 5) can finish the element-interior points themselves: the
+    # element pass grows, the gather shrinks; whole element + assembly time decides
+    if reg in (3, 5):
+        off = step_ms(**best)
+        table.append(("interior-off", best["blocks_per_cu_fwd"], off[0], off[1]))
+        # the pass with interior points holds more state: its own workgroup counts
+        for bpc in interior_bpc:
+            on = step_ms(**dict(best, blocks_per_cu_fwd=bpc, blocks_per_cu_adj=bpc, interior=True))
+            table.append(("interior-on", bpc, on[0], on[1]))
+        # one interior switch for both passes: the total of the best forward and adjoint shapes
+        on_total = min(r[2] for r in table if r[0] == "interior-on") + min(r[3] for r in table if r[0] == "interior-on")
+        if on_total < off[0] + off[1]:
+            best["interior"] = True
+            best["blocks_per_cu_fwd"] = min((r for r in table if r[0] == "interior-on"), key=lambda r: r[2])[1]
+            best["blocks_per_cu_adj"] = min((r for r in table if r[0] == "interior-on"), key=lambda r: r[3])[1]
     # brick-fused pass (lattice meshes): element + assembly time per step against the
     # element + assembly time of the best gather shapes
     if bricks and f.lattice:
-        def step_ms(**kw):
-            f.set_tuning(**kw)
-            a.set_tuning(**kw)
-            f.iterate_time(timed=True)
-            fe, fa = f.force_time(), f.assembly_time()
-            a.iterate_time(timed=True)
-            ae, aa = a.force_time(), a.assembly_time()
-            return (fe[0] + fa[0]) / max(fe[1], 1), (ae[0] + aa[0]) / max(ae[1], 1)
         base_f, base_a = step_ms(**best)
         table.append(("gather", best["blocks_per_cu_fwd"], base_f, base_a))
         top = (base_f + base_a, None)
@@ -73,8 +94,9 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
                 top = (tf + ta, br)
         if top[1] is not None:
             best = dict(best, brick=top[1])
-    # element slabs (any mesh): element pass + assembly slab after slab
-    if slabs and "brick" not in best:
+    # element slabs (any mesh): element pass + assembly slab after slab (every point
+    # through the gather)
+    if slabs and "brick" not in best and not best["interior"]:
         def step_ms_s(nsl, **kw):
             # element + assembly time per step; with slabs every step has nsl launches of each
             f.set_tuning(**kw)

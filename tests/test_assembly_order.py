@@ -123,3 +123,40 @@ def test_oracle_ordered_assembly_matches_colour_assembly():
     assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-12
     with pytest.raises(ValueError):
         orc.Oracle(*args, elem_order=np.zeros(mesh.nspec, np.int32), **kw)
+
+
+@pytest.mark.parametrize("dims,per_xcd", [((96, 96, 48), 384), ((96, 96, 48), 256), ((16, 16, 8), 32),
+                                          ((16, 16, 8), 96), ((7, 5, 3), 4)])
+def test_column_block_element_order(dims, per_xcd):
+    """The element order of the column-block traversal (regular_path 5): a permutation of
+    the elements; XCD x's share [x*chunk, (x+1)*chunk) holds whole columns when chunk is a
+    multiple of nez; every run of per_xcd consecutive indices inside a full group of
+    per_xcd columns lies at one depth, one depth above the run before it, over the same
+    columns."""
+    nex, ney, nez = dims
+    n = nex * ney * nez
+    ncol = nex * ney
+    order = np.zeros(n, np.int32)
+    _lib.call("fwat_column_block_order_", nex, ney, nez, per_xcd, order)
+    assert np.array_equal(np.sort(order), np.arange(n))
+    chunk = (n + 7) // 8
+    if chunk % nez:  # shares would split columns: element order
+        assert np.array_equal(order, np.arange(n))
+    col, depth = order % ncol, order // ncol
+    for x in range(8):
+        b, e = min(n, x * chunk), min(n, (x + 1) * chunk)
+        if b >= e:
+            continue
+        share = slice(b, e)
+        if chunk % nez == 0:
+            cols, counts = np.unique(col[share], return_counts=True)
+            assert np.all(counts == nez)  # whole columns
+        # full groups: per_xcd columns visited layer by layer
+        cols_in_order = list(dict.fromkeys((order[b:e] % ncol).tolist()))
+        ngroups_full = len(cols_in_order) // per_xcd if chunk % nez == 0 else 0
+        for g in range(ngroups_full):
+            g0 = b + g * per_xcd * nez
+            for z in range(nez):
+                run = slice(g0 + z * per_xcd, g0 + (z + 1) * per_xcd)
+                assert np.all(depth[run] == z)
+                assert np.array_equal(col[run], col[g0:g0 + per_xcd])

@@ -255,7 +255,8 @@ def test_element_pass_launch_shapes_agree():
     ko = o.adjoint(adj)
     m = c["mesh"]
     results = []
-    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8), (2, 0), (2, 1), (2, 8), (3, 0), (3, 1), (3, 8), (4, 1), (4, 8)):
+    for reg, bpc in ((0, 0), (1, 0), (1, 2), (1, 8), (2, 0), (2, 1), (2, 8), (3, 0), (3, 1), (3, 8), (4, 1), (4, 8),
+                     (5, 0), (5, 1), (5, 8)):
         print(f"launch shape: regular_path {reg}, {bpc} workgroups per CU", flush=True)
         src = Sources(m, c["sxyz"], c["mt"], c["f0"])
         rec = Receivers(m, c["rxyz"])
@@ -294,7 +295,7 @@ def test_pipelined_pass_with_several_elements_per_workgroup():
     assert np.abs(obs).max() > 1e-5 and np.abs(ko["mu_kl"]).max() > 1e-20  # far above FP32 underflow
     m = c["mesh"]
     out = []
-    for reg, bpc in ((1, 1), (2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3), (4, 2), (4, 3)):
+    for reg, bpc in ((1, 1), (2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3), (4, 2), (4, 3), (5, 1), (5, 2), (5, 3)):
         print(f"launch shape: regular_path {reg}, {bpc} workgroups per CU", flush=True)
         src = Sources(m, c["sxyz"], c["mt"], c["f0"])
         rec = Receivers(m, c["rxyz"])
@@ -318,6 +319,12 @@ def test_pipelined_pass_with_several_elements_per_workgroup():
         assert rel(syn, out[0][0]) < 1e-6
         for n in k:
             assert rel(k[n], out[0][1][n]) < 1e-6, n
+    # the column-block traversal (5) computes every element as the element-order pass (3) does:
+    # bit for bit at the same launch shape
+    for q3, q5 in ((4, 9), (5, 10), (6, 11)):
+        assert np.array_equal(out[q3][0], out[q5][0])
+        for n in out[q3][1]:
+            assert np.array_equal(out[q3][1][n], out[q5][1][n]), n
 
 
 @pytest.mark.parametrize("aniso", [False, True])
@@ -387,21 +394,24 @@ def test_element_slabs_are_bitwise_the_single_pass(kind):
 def test_two_points_per_lane_assembly_is_bitwise_the_one_point_pass(kind):
     """The assembly with two points per lane (k_assemble2) sums every point's
     row in the same order as k_assemble: seismograms and kernels identical,
-    also for a mesh whose point count is not a multiple of the block."""
+    also for a mesh whose point count is not a multiple of the block, and with
+    64 to 1,024 lanes per assembly workgroup."""
     c = make_case(seed=31, nstep=70, perturb=0.12 if kind == "irregular" else 0.0)
     m = c["mesh"]
     o = oracle_of(c, 64)
     adj = (0.5 * o.forward(save=True)).astype(np.float32)
     out = []
-    for lanes in (1, 2, 0):
+    for lanes, block in ((1, 256), (2, 256), (0, 256), (1, 1024), (2, 512), (0, 64)):
         src = Sources(m, c["sxyz"], c["mt"], c["f0"])
         rec = Receivers(m, c["rxyz"])
         f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
         f.set_assembly_lanes(lanes)
+        f.set_assembly_block(block)
         f.iterate_time()
         syn = f.seismograms()
         a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
         a.set_assembly_lanes(lanes)
+        a.set_assembly_block(block)
         a.take_forward(f)
         f.close()
         a.set_adjoint_sources(adj)
@@ -413,3 +423,89 @@ def test_two_points_per_lane_assembly_is_bitwise_the_one_point_pass(kind):
         assert np.array_equal(syn, out[0][0])
         for n in k:
             assert np.array_equal(k[n], out[0][1][n]), n
+
+
+def _interior_run(c, reg, bpc, interior, adj, rec_fwd=None):
+    """Forward + adjoint with the given launch shape and interior switch; the forward
+    handle may have other receivers than the adjoint one (rec_fwd)."""
+    m = c["mesh"]
+    src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+    rec = Receivers(m, c["rxyz"])
+    f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec_fwd or rec, c["dt"], c["nstep"], 1, True)
+    f.set_tuning(reg, bpc, bpc)
+    f.set_interior(interior)
+    info_f = f.interior_info()
+    f.iterate_time()
+    syn = f.seismograms()
+    fin = f.fields()
+    a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False, approximate_hess=True)
+    a.set_tuning(reg, bpc, bpc)
+    a.set_interior(interior)
+    info_a = a.interior_info()
+    a.take_forward(f)
+    f.close()
+    a.set_adjoint_sources(adj)
+    a.iterate_time()
+    k = a.kernels()
+    h = np.zeros((m.nspec, 125))
+    _lib.call("transfer_kernels_hess_el_tohost_dp_", a.h, h, m.nspec)
+    k["hess_kl"] = h
+    out = dict(syn=syn, fin=fin, k=k, b=a.b_fields(), adjf=a.fields(), info_f=info_f, info_a=info_a)
+    a.close()
+    return out
+
+
+def test_interior_points_finished_by_the_element_pass_are_bitwise_the_gather():
+    """The pipelined passes (3, and 5 in column-block order) finish the element-interior
+    points themselves (Newmark update, density kernel, Hessian) and the gather assembly
+    covers the shared points only: seismograms, final forward fields, kernels and the
+    final adjoint and reconstructed fields equal the all-gather path bit for bit, on a
+    2,048-element mesh with several elements per workgroup, and match the FP64 oracle."""
+    c = make_case(nex=16, ney=16, nez=8, seed=37, nstep=300, nrec=6)
+    o = oracle_of(c, 64)
+    obs = o.forward(save=True)
+    adj = (0.5 * obs).astype(np.float32)
+    ko = o.adjoint(adj)
+    m = c["mesh"]
+    for reg, bpc, mode in ((3, 1, 1), (3, 2, 1), (5, 2, 1), (5, 3, 1)):
+        print(f"interior points: regular_path {reg}, {bpc} workgroups per CU, mode {mode}", flush=True)
+        on = _interior_run(c, reg, bpc, mode, adj)
+        off = _interior_run(c, reg, bpc, 0, adj)
+        n_sh, n_in, act_f, act_a = on["info_f"]
+        assert act_f == 1 and on["info_a"][3] == 1
+        assert off["info_f"][2] == 0 and off["info_a"][3] == 0
+        assert n_sh + n_in == m.nglob
+        # 27 interior points per element except in the source and receiver elements
+        assert n_in >= 27 * (m.nspec - 1 - 6)
+        assert np.array_equal(on["syn"], off["syn"])
+        for q in range(3):
+            assert np.array_equal(on["fin"][q], off["fin"][q]), ("forward field", q)
+            assert np.array_equal(on["b"][q], off["b"][q]), ("reconstructed field", q)
+            assert np.array_equal(on["adjf"][q], off["adjf"][q]), ("adjoint field", q)
+        for n in on["k"]:
+            assert np.array_equal(on["k"][n], off["k"][n]), n
+        assert rel(on["syn"], obs) < SEIS_TOL
+        for n in ("rho_kl", "mu_kl", "kappa_kl"):
+            assert rel(on["k"][n], ko[n]) < KERN_TOL, (reg, bpc, n)
+
+
+def test_handles_with_other_receivers_hand_over_the_forward_state():
+    """A forward handle with other receiver elements numbers its points otherwise than
+    the adjoint handle; the in-HBM hand-off then goes through the caller's numbering and
+    the adjoint run is bit for bit the one whose forward handle had the same receivers."""
+    c = make_case(nex=8, ney=8, nez=4, seed=41, nstep=160, nrec=4)
+    m = c["mesh"]
+    rng = np.random.default_rng(5)
+    other = Receivers(m, np.stack([rng.uniform(0.1, 0.9, 3) * m.lx, rng.uniform(0.1, 0.9, 3) * m.ly,
+                                   np.zeros(3)], axis=1))
+    adj = np.asarray(np.random.default_rng(6).standard_normal((c["nstep"], 4, 3)) * 1e-3, np.float32)
+    same = _interior_run(c, 3, 1, True, adj)
+    diff = _interior_run(c, 3, 1, True, adj, rec_fwd=other)
+    # receiver elements {214, 206, 221} against {252, 230, 221, 234}: other numberings
+    assert diff["info_f"][0] != diff["info_a"][0]
+    assert same["info_f"][0] == same["info_a"][0]
+    for q in range(3):
+        assert np.array_equal(same["fin"][q], diff["fin"][q])
+        assert np.array_equal(same["b"][q], diff["b"][q])
+    for n in same["k"]:
+        assert np.array_equal(same["k"][n], diff["k"][n]), n
