@@ -426,6 +426,8 @@ def main():
                       "fwd_force_avg_ms": fwd_avg_ms,
                       "fwd_force_GBps": fwd_bytes / (fwd_avg_ms * 1e-3) / 1e9 if fwd_avg_ms > 0 else None,
                       "measurement_device_ms": misfit_mod.last_time_ms(),
+                      "points_assembled": {"forward": nglob - n_in_f, "adjoint": nglob - n_in_a,
+                                           "element_interior_finished_in_element_pass": n_in_a},
                       "fwd_assembly_avg_ms": fwd_asm_ms / max(fwd_asm_n, 1),
                       "fwd_assembly_GBps": (fwd_asm_bytes / (fwd_asm_ms / fwd_asm_n * 1e-3) / 1e9
                                             if fwd_asm_n and fwd_asm_ms > 0 else None)},

@@ -89,6 +89,19 @@ module specfwat_hip
       real(c_float) :: DT
     end subroutine
 
+    subroutine fwat_set_tuning(Mesh_pointer, regular_path, blocks_per_cu_fwd, blocks_per_cu_adj) &
+        bind(C, name="fwat_set_tuning_")
+      import :: c_long, c_int
+      integer(c_long) :: Mesh_pointer
+      integer(c_int) :: regular_path, blocks_per_cu_fwd, blocks_per_cu_adj
+    end subroutine
+
+    subroutine fwat_set_interior(Mesh_pointer, on) bind(C, name="fwat_set_interior_")
+      import :: c_long, c_int
+      integer(c_long) :: Mesh_pointer
+      integer(c_int) :: on
+    end subroutine
+
     subroutine fwat_copy_forward_to_backward(Mesh_from, Mesh_to) bind(C, name="fwat_copy_forward_to_backward_")
       import :: c_long
       integer(c_long) :: Mesh_from, Mesh_to
