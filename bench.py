@@ -301,6 +301,11 @@ def main():
     n_sh_a, n_in_a, _, act_a = ad.interior_info()
     n_in_f = n_in_f if (fused and act_f) else 0
     n_in_a = n_in_a if (fused and act_a) else 0
+    # of those, the ones at an element's 27 inner slots (the rest: inner points of a
+    # free-surface element's top face), whose element-local forces the pass does not store
+    n_special = len(set(src.ispec.tolist()) | set(rec.ispec.tolist()))
+    n_slot_f = min(n_in_f, 27 * (mesh.nspec - n_special))
+    n_slot_a = min(n_in_a, 27 * (mesh.nspec - n_special))
     k = ad.kernels()
     f.close(); ad.close()
     if not all(np.isfinite(v).all() for v in k.values()):
@@ -341,15 +346,16 @@ def main():
         # point, mu/kappa accumulators 2 x kacc, element-local forces written for both fields
         # 24; displacement gather of both fields 12 + 12 per unique point
         # With element-interior points finished by the element pass (n_in of them, one
-        # element-local slot each): their element-local forces are not written; the pass
-        # reads their rmass_inv 12 and velocities (12 per field), writes u and v (24 per
-        # field) and, two fields, the rho kernel kacc; the assembly covers the other
-        # nglob - n_in points, their contributions and element-local slots only.
+        # element-local slot each): the pass reads their rmass_inv 12 and velocities (12
+        # per field), writes u and v (24 per field) and, two fields, the rho kernel kacc,
+        # and stores no element-local forces for the n_slot of them at an element's inner
+        # slots; the assembly covers the other nglob - n_in points, their contributions
+        # and element-local slots only.
         ename = element_kernel_name(tuning, nirr)
         ngs = nglob - n_in_a
         kinds = {
             f"{ename}<2,iso,fused-kernels>": ((28.0 + kmu + 2 * kacc) * nloc + 40.0 * nirr + 24.0 * nglob
-                                              + (-24.0 + 12.0 + 72.0 + kacc) * n_in_a, adj_ms, adj_n),
+                                              + (12.0 + 72.0 + kacc) * n_in_a - 24.0 * n_slot_a, adj_ms, adj_n),
             # two-field assembly: CSR offsets 4 per point + indices 4 per contribution,
             # element-local forces of both fields 24 per local point, Stacey terms of both
             # fields 24 per face point, rmass_inv 12, u,v of both fields read 48 + write 48,
@@ -358,7 +364,7 @@ def main():
                                       + 12.0 * ngs + 96.0 * ngs + kacc * ngs, adj_asm_ms, adj_asm_n),
         }
         ngf = nglob - n_in_f
-        fwd_bytes = (16.0 + kmu) * nloc + 40.0 * nirr + 12.0 * nglob + (-12.0 + 12.0 + 36.0) * n_in_f
+        fwd_bytes = (16.0 + kmu) * nloc + 40.0 * nirr + 12.0 * nglob + (12.0 + 36.0) * n_in_f - 12.0 * n_slot_f
         fwd_asm_bytes = (4.0 * ngf + 4.0 * (ncontrib - n_in_f) + 12.0 * (nloc - n_in_f) + 12.0 * nfp
                          + 12.0 * ngf + 48.0 * ngf)
     else:

@@ -26,7 +26,7 @@
 //         same pass (requires NF == 2)
 //   SP    storage precision of kappa/mu/Jacobian: fp32, bf16, fp16 (scaled)
 //   ASM   0 = fp32 atomic add into accel (SPECFEM's scatter), 1 = write the
-//         element-local forces to floc[ispec][L][3]; a per-point gather over
+//         element-local forces to floc[ispec][sem_floc_slot(L)][3]; a per-point gather over
 //         a static CSR then assembles them in a fixed order (no atomics)
 #include "sem_device.h"
 
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
         unsafeAtomicAdd(Acc + 3 * (size_t)iglob + 1, -fy);
         unsafeAtomicAdd(Acc + 3 * (size_t)iglob + 2, -fz);
       } else {
-        realw* F = Floc + 3 * Lg;
+        realw* F = Floc + 3 * (Lg - tt + sem_floc_slot(tt));
         F[0] = -fx;
         F[1] = -fy;
         F[2] = -fz;
@@ -467,7 +467,7 @@ __device__ __forceinline__ void forces_reg_body(const ForceArgs& A) {
         fx += s_s[4][Lc] * hwK[l]; fy += s_s[5][Lc] * hwK[l]; fz += s_s[2][Lc] * hwK[l];
       }
       if (act) {
-        realw* F = Floc + 3 * Lg;
+        realw* F = Floc + 3 * (Lg - tt + sem_floc_slot(tt));
         F[0] = fx;
         F[1] = fy;
         F[2] = fz;
@@ -637,7 +637,7 @@ __device__ __forceinline__ void forces_reg_pipe_body(const ForceArgs& A) {
         fx += s_s[4][Lc] * hwK[l]; fy += s_s[5][Lc] * hwK[l]; fz += s_s[2][Lc] * hwK[l];
       }
       if (act) {
-        realw* F = Floc + 3 * Lg;
+        realw* F = Floc + 3 * (Lg - tt + sem_floc_slot(tt));
         F[0] = fx;
         F[1] = fy;
         F[2] = fz;
@@ -854,10 +854,13 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
         fx += s_s[b][f][4][Lc] * hwK[l]; fy += s_s[b][f][5][Lc] * hwK[l]; fz += s_s[b][f][2][Lc] * hwK[l];
       }
       ff[f][0] = fx; ff[f][1] = fy; ff[f][2] = fz;
-      // every lane stores its element-local forces (whole rows: an element row with
-      // gaps costs more to write than the interior points' 12 bytes save)
-      if (act) {
-        realw* F = (f == 0 ? A.floc0 : A.floc1) + 3 * Lg;
+      // element-local forces; a point this pass finishes needs none, and the 27 inner
+      // slots sit at the end of the row (sem_floc_slot), so leaving them out leaves the
+      // rest of the row whole (the inner points of a free-surface element's top face
+      // lie mid-row and are stored anyway)
+      const int fs = sem_floc_slot(tt);
+      if (act && !(INT && inner && fs >= 98)) {
+        realw* F = (f == 0 ? A.floc0 : A.floc1) + 3 * (Lg - tt + fs);
         F[0] = fx;
         F[1] = fy;
         F[2] = fz;
@@ -884,7 +887,7 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
 }
 
 template <int NF, int KMODE, int SP, bool NT = false, bool INT = false>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : (INT ? 6 : 5)))) void k_forces_reg_pp(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NF == 2 ? 4 : 5))) void k_forces_reg_pp(
     ForceArgs A) {
   forces_reg_pp_body<NF, KMODE, SP, NT, INT>(A);
 }

@@ -74,12 +74,18 @@ std::vector<int> contribution_owners(SemMesh* m) {
   return owner;
 }
 
-void upload_assembly_csr(SemMesh* m, const std::vector<int>& off, const std::vector<int>& idx) {
+// The device CSR addresses element-local terms by their place in the element rows of
+// floc: q = ispec * 125 + L becomes ispec * 125 + sem_floc_slot(L).
+void upload_assembly_csr(SemMesh* m, const std::vector<int>& off, std::vector<int> idx) {
+  const int nloc = m->nspec * 125;
+  for (int& q : idx)
+    if (q < nloc) q = q - q % 125 + sem_floc_slot(q % 125);
   dev_free(m->d_asm_off);
   dev_free(m->d_asm_idx);
   m->n_contrib = idx.size();
   m->d_asm_off = dev_upload(off.data(), off.size(), m->stream);
   m->d_asm_idx = dev_upload(idx.data(), idx.size(), m->stream);
+  HIP_CHECK(hipStreamSynchronize(m->stream));
 }
 
 }  // namespace

@@ -245,6 +245,27 @@ struct SemMesh {
   int last_asm_launches = 0;
 };
 
+// Position of local point L = i + 5j + 25k in an element's row of element-local forces
+// (floc[ispec][slot][3]).  A gather reads a row in pieces, each at the time the point
+// numbering reaches the points that piece belongs to, so the row is grouped by piece: the
+// k = 0 face (25 slots, shared with the element below), the j = 0 face above it (20, with
+// the element in front), the i = 0 face of the rest (16, with the element to the left),
+// then the element's other points (i, j, k >= 1): the 37 on its k, j or i = 4 faces, and
+// last the 27 inner ones (i, j, k in 1..3), which the gather never reads when the element
+// pass finishes them.  A bijection of 0..124.
+__host__ __device__ inline int sem_floc_slot(int L) {
+  const int i = L % 5, j = (L / 5) % 5, k = L / 25;
+  if (k == 0) return i + 5 * j;
+  if (j == 0) return 25 + i + 5 * (k - 1);
+  if (i == 0) return 45 + (j - 1) + 4 * (k - 1);
+  if (i <= 3 && j <= 3 && k <= 3) return 98 + (i - 1) + 3 * (j - 1) + 9 * (k - 1);
+  // rank among the (i, j, k >= 1) points not inner, in L order: own-index minus the inner
+  // points before it
+  const int o = (i - 1) + 4 * (j - 1) + 16 * (k - 1);
+  const int inner_before = 9 * (k - 1 < 3 ? k - 1 : 3) + (k <= 3 ? 3 * (j - 1 < 3 ? j - 1 : 3) + (j <= 3 ? (i - 1 < 3 ? i - 1 : 3) : 0) : 0);
+  return 61 + o - inner_before;
+}
+
 // library number of the caller's point g (0-based)
 inline int sem_point(const SemMesh* m, int g) { return m->h_perm.empty() ? g : m->h_perm[g]; }
 
