@@ -7,7 +7,7 @@
 // per-event arithmetic is the single-event fused path's; batching changes the
 // data layout and what is read once for all events:
 //   fields u, v, a              [nglob][E][3]      (events innermost)
-//   element-local forces        [nloc][E][3]
+//   element-local forces        [nloc][E][3] (element rows grouped by reader, sem_floc_slot)
 //   Stacey store                [NSTEP_max][nfp][E][3]  fp32, or fp16 with one scale per
 //                                                        (record, event) (fwat_batch_set_store_precision_)
 //   rho kernel                  [nglob][E] fp64;  mu, kappa kernels [E][nloc] fp64
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
           out[3 * ev + 1] = s_u[f][ev][1][t];
           out[3 * ev + 2] = s_u[f][ev][2][t];
         }
-        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + Lg * (3 * E), out);
+        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + (Lg - tt + sem_floc_slot(tt)) * (3 * E), out);
       }
     }
   }
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(128) void k_batch_elem_aniso(BatchElemArgs A) {
           out[3 * ev + 1] = s_u[f][ev][1][t];
           out[3 * ev + 2] = s_u[f][ev][2][t];
         }
-        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + Lg * (3 * E), out);
+        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + (Lg - tt + sem_floc_slot(tt)) * (3 * E), out);
       }
     }
   }
@@ -886,6 +886,9 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
     for (int e = 0; e < m->nspec; ++e) visit[e] = e;
   }
   sem_build_assembly_csr(m->nglob, m->nspec, owner, visit, off, idx);
+  // element-local terms addressed by their place in the grouped element rows (sem_floc_slot)
+  for (int& q : idx)
+    if ((size_t)q < nloc) q = q - q % 125 + sem_floc_slot(q % 125);
   B->d_off = dev_upload(off.data(), off.size(), s);
   B->d_idx = dev_upload(idx.data(), idx.size(), s);
   B->d_sarr = dev_upload(h_sourcearrays, (size_t)E * 375, s);
