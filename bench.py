@@ -92,6 +92,8 @@ def parse():
                         "1 for the adjoint pass)")
     p.add_argument("--asm-block", type=int, default=0,
                    help="lanes per workgroup of the gather assembly (64..1024; 0: the library's 256)")
+    p.add_argument("--floc-order", action="store_true",
+                   help="element-local force rows in local-point order instead of grouped by reader")
     p.add_argument("--static-precision", type=int, default=0, help="0 fp32, 1 bf16, 2 fp16")
     p.add_argument("--force-irregular", action="store_true",
                    help="c1/c2: every element stores its inverse Jacobian per point (the general-hexahedron "
@@ -249,6 +251,9 @@ def main():
         if a.asm_block:
             f.set_assembly_block(a.asm_block)
             ad.set_assembly_block(a.asm_block)
+        if a.floc_order:
+            f.set_floc_layout(False)
+            ad.set_floc_layout(False)
         if window and state_smooth is not None:
             f.set_fields(*state_smooth)
         if a.fk:
@@ -391,6 +396,7 @@ def main():
             if line.startswith("hbm_bytes_per_launch " + kname):
                 traffic = float(line.split("=")[1])
     fwd_avg_ms = fwd_ms / max(fwd_n, 1)
+    step_bytes = nsl * (sum(v[0] for v in kinds.values()) + fwd_bytes + fwd_asm_bytes)
     result = {
         "metric": "GLL-pt-steps/s/event fwd+adj",
         "value": value,
@@ -424,7 +430,10 @@ def main():
                    **({"fk_injection": {"every_steps": a.fk}} if a.fk else {})},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn},
+                     "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": avg_ms, "launches": kn,
+                     # the whole step: the four passes' algorithmic bytes over the wall time per step
+                     "step": {"algorithmic_bytes": step_bytes, "achieved": step_bytes / (wall / K) / 1e9,
+                              "frac": step_bytes / (wall / K) / 1e9 / HBM_PEAK_GBS}},
         "breakdown": {"wall_s": wall, "forward_s": tf - ts, "adjoint_s": te - tf,
                       "kernels": {k: {"avg_ms": v[1] / max(v[2], 1),
                                       "GBps": v[0] / (v[1] / max(v[2], 1) * 1e-3) / 1e9 if v[1] > 0 else None}

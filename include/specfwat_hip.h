@@ -205,6 +205,11 @@ void fwat_set_assembly_lanes_(long *Mesh_pointer, int *points_per_lane);
  * workgroups keep longer runs of consecutive points on one XCD; launch-shape knob, no
  * counterpart in the reference */
 void fwat_set_assembly_block_(long *Mesh_pointer, int *lanes);
+/* layout of the element-local force rows of the gather path: 1 = grouped by the neighbour that
+ * reads each piece (k = 0, j = 0, i = 0 faces, the element's other points, its inner points last;
+ * the default), 0 = local-point order; the same results bit for bit; launch-shape knob, no
+ * counterpart in the reference */
+void fwat_set_floc_layout_(long *Mesh_pointer, int *grouped);
 /* HIP graph capture of the whole-run loop (fused gather path): on = 1 makes every
  * fwat_iterate_time_ call capture its step loop from the simulation stream into one graph and
  * launch it as a unit (the GPU then runs the steps' kernels back to back, independent of the

@@ -67,6 +67,7 @@ SIGNATURES = {
     "fwat_set_kernel_block_": "Li",
     "fwat_set_assembly_lanes_": "Li",
     "fwat_set_assembly_block_": "Li",
+    "fwat_set_floc_layout_": "Li",
     "fwat_set_graph_": "Li",
     "fwat_assembly_csr_": "iiIiIIiIiIiIiiII",
     "fwat_set_slabs_": "Li",

@@ -50,6 +50,7 @@ struct ForceArgs {
   // pipelined regular pass in column blocks (regular_path 5): columns of the element lattice
   // (ncol = nex * ney) and elements per column; col_nez = 0: element order
   int col_ncol, col_nez;
+  int grouped;  // element-local force rows grouped by reader (sem_floc_slot), else in local-point order
   int elem_begin, elem_count;
   int nblocks_padded;
   realw xix_regular, jacobian_regular;
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(128) void k_forces(ForceArgs A) {
         unsafeAtomicAdd(Acc + 3 * (size_t)iglob + 1, -fy);
         unsafeAtomicAdd(Acc + 3 * (size_t)iglob + 2, -fz);
       } else {
-        realw* F = Floc + 3 * (Lg - tt + sem_floc_slot(tt));
+        realw* F = Floc + 3 * (Lg - tt + (A.grouped ? sem_floc_slot(tt) : tt));
         F[0] = -fx;
         F[1] = -fy;
         F[2] = -fz;
@@ -467,7 +468,7 @@ __device__ __forceinline__ void forces_reg_body(const ForceArgs& A) {
         fx += s_s[4][Lc] * hwK[l]; fy += s_s[5][Lc] * hwK[l]; fz += s_s[2][Lc] * hwK[l];
       }
       if (act) {
-        realw* F = Floc + 3 * (Lg - tt + sem_floc_slot(tt));
+        realw* F = Floc + 3 * (Lg - tt + (A.grouped ? sem_floc_slot(tt) : tt));
         F[0] = fx;
         F[1] = fy;
         F[2] = fz;
@@ -637,7 +638,7 @@ __device__ __forceinline__ void forces_reg_pipe_body(const ForceArgs& A) {
         fx += s_s[4][Lc] * hwK[l]; fy += s_s[5][Lc] * hwK[l]; fz += s_s[2][Lc] * hwK[l];
       }
       if (act) {
-        realw* F = Floc + 3 * (Lg - tt + sem_floc_slot(tt));
+        realw* F = Floc + 3 * (Lg - tt + (A.grouped ? sem_floc_slot(tt) : tt));
         F[0] = fx;
         F[1] = fy;
         F[2] = fz;
@@ -858,8 +859,8 @@ __device__ __forceinline__ void forces_reg_pp_body(const ForceArgs& A) {
       // slots sit at the end of the row (sem_floc_slot), so leaving them out leaves the
       // rest of the row whole (the inner points of a free-surface element's top face
       // lie mid-row and are stored anyway)
-      const int fs = sem_floc_slot(tt);
-      if (act && !(INT && inner && fs >= 98)) {
+      const int fs = A.grouped ? sem_floc_slot(tt) : tt;
+      if (act && !(INT && inner && A.grouped && fs >= 98)) {
         realw* F = (f == 0 ? A.floc0 : A.floc1) + 3 * (Lg - tt + fs);
         F[0] = fx;
         F[1] = fy;
@@ -1023,6 +1024,7 @@ void sem_launch_forces(SemMesh* m, int nfields, int kernel_mode, int elem_begin,
   a.cij = m->d_cij;
   a.tables = m->d_tables;
   a.elem_list = elem_list;
+  a.grouped = m->floc_grouped;
   a.elem_begin = elem_begin;
   a.elem_count = elem_count;
   a.nblocks_padded = xcd_pad(elem_count);

@@ -75,11 +75,12 @@ std::vector<int> contribution_owners(SemMesh* m) {
 }
 
 // The device CSR addresses element-local terms by their place in the element rows of
-// floc: q = ispec * 125 + L becomes ispec * 125 + sem_floc_slot(L).
+// floc: with grouped rows q = ispec * 125 + L becomes ispec * 125 + sem_floc_slot(L).
 void upload_assembly_csr(SemMesh* m, const std::vector<int>& off, std::vector<int> idx) {
   const int nloc = m->nspec * 125;
-  for (int& q : idx)
-    if (q < nloc) q = q - q % 125 + sem_floc_slot(q % 125);
+  if (m->floc_grouped)
+    for (int& q : idx)
+      if (q < nloc) q = q - q % 125 + sem_floc_slot(q % 125);
   dev_free(m->d_asm_off);
   dev_free(m->d_asm_idx);
   m->n_contrib = idx.size();
@@ -449,6 +450,24 @@ void prepare_sim2_or_3_const_device_(long* Mesh_pointer, int* nadj_rec_local, in
   SemMesh* m = sem_mesh(Mesh_pointer);
   if (*nadj_rec_local != m->nrec) SEM_FAIL("nadj_rec_local (%d) must equal nrec (%d)", *nadj_rec_local, m->nrec);
   (void)NTSTEP_BETWEEN_READ_ADJSRC;
+}
+
+// Layout of the element-local force rows of the gather path: 1 = grouped by reader
+// (sem_floc_slot, the default), 0 = local-point order.  Rebuilds the device CSR; results
+// are the same bit for bit.  No counterpart in the reference.
+void fwat_set_floc_layout_(long* Mesh_pointer, int* grouped) {
+  SemMesh* m = sem_mesh(Mesh_pointer);
+  const int g = *grouped != 0;
+  if (g == m->floc_grouped) return;
+  m->floc_grouped = g;
+  std::vector<int> visit = m->h_visit, off, idx;
+  if (visit.empty()) {
+    visit.resize(m->nspec);
+    for (int e = 0; e < m->nspec; ++e) visit[e] = e;
+  }
+  HIP_CHECK(hipStreamSynchronize(m->stream));
+  sem_build_assembly_csr(m->nglob, m->nspec, contribution_owners(m), visit, off, idx);
+  upload_assembly_csr(m, off, idx);
 }
 
 // Store the static material/geometry arrays in BF16 (precision=1) or FP16

@@ -207,6 +207,7 @@ struct SemMesh {
   realw* d_face_scratch = nullptr;  // [nfaces*25][3] absorbing term of a non-stored field
   int asm_points_per_lane = 0;  // gather assembly: 1 or 2 points per lane, 0 = by pass (see stage_assemble)
   int asm_block = 256;          // gather assembly: lanes per workgroup (64..1024)
+  int floc_grouped = 1;         // floc rows grouped by reader (sem_floc_slot), 0: local-point order
   int* d_asm_off = nullptr;     // [nglob+1]
   int* d_asm_idx = nullptr;     // [ncontrib]
   size_t n_contrib = 0;

@@ -140,6 +140,10 @@ class ElasticRun:
         """Lanes per workgroup of the gather assembly (64..1024, default 256)."""
         _lib.call("fwat_set_assembly_block_", self.h, int(lanes))
 
+    def set_floc_layout(self, grouped=True):
+        """Element-local force rows grouped by reader (default) or in local-point order."""
+        _lib.call("fwat_set_floc_layout_", self.h, int(bool(grouped)))
+
     def set_interior(self, on=True):
         """Element-interior points finished by the element pass of the fused loops
         (pipelined passes 3 and 5; the same results bit for bit) or assembled by the gather."""

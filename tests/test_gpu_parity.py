@@ -509,3 +509,38 @@ def test_handles_with_other_receivers_hand_over_the_forward_state():
         assert np.array_equal(same["b"][q], diff["b"][q])
     for n in same["k"]:
         assert np.array_equal(same["k"][n], diff["k"][n]), n
+
+
+@pytest.mark.parametrize("reg,interior", [(1, 0), (3, 0), (3, 1)])
+def test_element_row_layouts_are_bitwise_the_same(reg, interior):
+    """Element-local force rows grouped by reader (the default) and in local-point order
+    give the same seismograms, kernels and final fields bit for bit, with and without the
+    element-interior points in the element pass."""
+    c = make_case(nex=8, ney=8, nez=4, seed=43, nstep=120, nrec=4)
+    m = c["mesh"]
+    adj = np.asarray(np.random.default_rng(7).standard_normal((c["nstep"], 4, 3)) * 1e-3, np.float32)
+    out = []
+    for grouped in (True, False):
+        print(f"row layout grouped={grouped}, regular_path {reg}, interior {interior}", flush=True)
+        src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+        rec = Receivers(m, c["rxyz"])
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
+        f.set_tuning(reg, 2, 2, interior=interior)
+        f.set_floc_layout(grouped)
+        f.iterate_time()
+        syn = f.seismograms()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
+        a.set_tuning(reg, 2, 2, interior=interior)
+        a.set_floc_layout(grouped)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj)
+        a.iterate_time()
+        out.append((syn, a.kernels(), a.b_fields()))
+        a.close()
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        for n in out[0][1]:
+            assert np.array_equal(out[0][1][n], o[1][n]), n
+        for q in range(3):
+            assert np.array_equal(out[0][2][q], o[2][q])
