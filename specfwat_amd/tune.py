@@ -2,12 +2,15 @@
 
 Times short forward and adjoint runs on the real mesh for a set of
 candidate shapes (general kernel, folded-table kernel with one workgroup
-per element, persistent workgroups per CU, its software-pipelined form) and
-returns the fastest; on
-lattice meshes it then compares the brick-fused element + assembly pass
-(brick shapes 2x2x2, 4x2x2, 4x4x2) against the best gather shapes.  The
-caller applies the result with ElasticRun.set_tuning.  The measurement is
-the live HIP-event time of the passes on the simulation stream.
+per element, persistent workgroups per CU, its software-pipelined forms,
+element order or column blocks) and returns the fastest; for the pipelined
+passes it then compares element pass + assembly with and without the
+element-interior points finished in the element pass (at several workgroup
+counts); on lattice meshes it compares the brick-fused element + assembly
+pass (brick shapes 2x2x2, 4x2x2, 4x4x2) against the best gather shapes, and
+on any mesh element slabs.  The caller applies the result with
+ElasticRun.set_tuning.  The measurement is the live HIP-event time of the
+passes on the simulation stream.
 """
 import numpy as np
 

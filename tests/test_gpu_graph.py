@@ -13,11 +13,14 @@ from test_gpu_parity import KERN_TOL, SEIS_TOL, make_case, oracle_of, rel
 pytestmark = pytest.mark.gpu
 
 
-def event(c, graph, adj):
+def event(c, graph, adj, tuning=None):
     m = c["mesh"]
     src, rec = Sources(m, c["sxyz"], c["mt"], c["f0"]), Receivers(m, c["rxyz"])
     f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True)
     a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False)
+    if tuning:
+        f.set_tuning(**tuning)
+        a.set_tuning(**tuning)
     f.set_graph(graph)
     a.set_graph(graph)
     f.iterate_time()
@@ -49,3 +52,18 @@ def test_graph_runs_equal_eager_runs():
     assert rel(s1, obs) < SEIS_TOL
     for n in ("rho_kl", "mu_kl", "kappa_kl"):
         assert rel(k1[n], ko[n]) < KERN_TOL, n
+
+
+def test_graph_runs_with_interior_points_equal_eager_runs():
+    """The same with the pipelined pass finishing the element-interior points (a regular
+    mesh, regular_path 3): captured runs equal eager runs bit for bit, twice."""
+    c = make_case(nex=8, ney=8, nez=4, seed=47, nstep=96)
+    adj = np.asarray(np.random.default_rng(9).standard_normal((c["nstep"], 5, 3)) * 1e-3, np.float32)
+    tun = dict(regular_path=3, blocks_per_cu_fwd=2, blocks_per_cu_adj=2, interior=True)
+    s0, k0 = event(c, False, adj, tun)
+    for rep in range(2):
+        print(f"graph run {rep} (interior points)", flush=True)
+        s1, k1 = event(c, True, adj, tun)
+        assert np.array_equal(s1, s0), rep
+        for n in k0:
+            assert np.array_equal(k1[n], k0[n]), (rep, n)
