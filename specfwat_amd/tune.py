@@ -24,7 +24,7 @@ CANDIDATES = [(0, 0), (1, 0), (1, 8), (1, 10), (1, 16), (1, 20), (1, 32), (2, 6)
               (3, 6), (3, 8), (3, 12), (3, 16), (5, 6), (5, 8), (5, 12), (5, 16)]
 BRICKS = [(2, 2, 2), (4, 2, 2), (4, 4, 2)]
 SLABS = [6, 12, 24]  # slabs per mesh
-INTERIOR_BPC = [6, 8, 10, 12]  # workgroups per CU tried with the element-interior points in the pipelined pass
+INTERIOR_BPC = [6, 7, 8, 9, 10, 12]  # workgroups per CU tried with the element-interior points in the pipelined pass
 
 
 def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidates=CANDIDATES,

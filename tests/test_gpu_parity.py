@@ -544,3 +544,33 @@ def test_element_row_layouts_are_bitwise_the_same(reg, interior):
             assert np.array_equal(out[0][1][n], o[1][n]), n
         for q in range(3):
             assert np.array_equal(out[0][2][q], o[2][q])
+
+
+@pytest.mark.parametrize("sp", [1, 2])
+def test_interior_points_with_reduced_precision_statics(sp):
+    """Element-interior points finished by the pipelined pass with bf16 (1) or fp16-scaled
+    (2) statics: bit for bit the all-gather path with the same statics."""
+    c = make_case(nex=8, ney=8, nez=4, seed=53, nstep=100, nrec=4)
+    m = c["mesh"]
+    adj = np.asarray(np.random.default_rng(11).standard_normal((c["nstep"], 4, 3)) * 1e-3, np.float32)
+    out = []
+    for interior in (True, False):
+        print(f"static precision {sp}, interior {interior}", flush=True)
+        src = Sources(m, c["sxyz"], c["mt"], c["f0"])
+        rec = Receivers(m, c["rxyz"])
+        f = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 1, True, static_precision=sp)
+        f.set_tuning(3, 2, 2, interior=interior)
+        assert f.interior_info()[2] == int(interior)
+        f.iterate_time()
+        syn = f.seismograms()
+        a = ElasticRun(m, c["rho"], c["vp"], c["vs"], src, rec, c["dt"], c["nstep"], 3, False, static_precision=sp)
+        a.set_tuning(3, 2, 2, interior=interior)
+        a.take_forward(f)
+        f.close()
+        a.set_adjoint_sources(adj)
+        a.iterate_time()
+        out.append((syn, a.kernels()))
+        a.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for n in out[0][1]:
+        assert np.array_equal(out[0][1][n], out[1][1][n]), n
