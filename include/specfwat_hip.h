@@ -210,6 +210,9 @@ void fwat_set_assembly_block_(long *Mesh_pointer, int *lanes);
  * the default), 0 = local-point order; the same results bit for bit; launch-shape knob, no
  * counterpart in the reference */
 void fwat_set_floc_layout_(long *Mesh_pointer, int *grouped);
+/* the grouped row's slot of each local point L = i + 5j + 25k (slot[125]); host only, no
+ * counterpart in the reference */
+void fwat_floc_row_layout_(int *slot);
 /* HIP graph capture of the whole-run loop (fused gather path): on = 1 makes every
  * fwat_iterate_time_ call capture its step loop from the simulation stream into one graph and
  * launch it as a unit (the GPU then runs the steps' kernels back to back, independent of the

@@ -68,6 +68,7 @@ SIGNATURES = {
     "fwat_set_assembly_lanes_": "Li",
     "fwat_set_assembly_block_": "Li",
     "fwat_set_floc_layout_": "Li",
+    "fwat_floc_row_layout_": "I",
     "fwat_set_graph_": "Li",
     "fwat_assembly_csr_": "iiIiIIiIiIiIiiII",
     "fwat_set_slabs_": "Li",

@@ -452,6 +452,12 @@ void prepare_sim2_or_3_const_device_(long* Mesh_pointer, int* nadj_rec_local, in
   (void)NTSTEP_BETWEEN_READ_ADJSRC;
 }
 
+// Place of each local point L = i + 5j + 25k in the grouped element row
+// (sem_floc_slot): slot[L] for L = 0..124.  Host only; no counterpart in the reference.
+void fwat_floc_row_layout_(int* slot) {
+  for (int L = 0; L < 125; ++L) slot[L] = sem_floc_slot(L);
+}
+
 // Layout of the element-local force rows of the gather path: 1 = grouped by reader
 // (sem_floc_slot, the default), 0 = local-point order.  Rebuilds the device CSR; results
 // are the same bit for bit.  No counterpart in the reference.

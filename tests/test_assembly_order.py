@@ -160,3 +160,23 @@ def test_column_block_element_order(dims, per_xcd):
                 run = slice(g0 + z * per_xcd, g0 + (z + 1) * per_xcd)
                 assert np.all(depth[run] == z)
                 assert np.array_equal(col[run], col[g0:g0 + per_xcd])
+
+
+def test_grouped_element_row_layout():
+    """The grouped element row (sem_floc_slot): a bijection of the 125 local points whose
+    pieces are contiguous and in the order k = 0 face (25), j = 0 face above it (20), i = 0
+    face of the rest (16), the element's other non-inner points (37), its inner points
+    (27), each piece in local-point order."""
+    slot = np.zeros(125, np.int32)
+    _lib.call("fwat_floc_row_layout_", slot)
+    assert np.array_equal(np.sort(slot), np.arange(125))
+    L = np.arange(125)
+    i, j, k = L % 5, (L // 5) % 5, L // 25
+    inner = (i >= 1) & (i <= 3) & (j >= 1) & (j <= 3) & (k >= 1) & (k <= 3)
+    pieces = [k == 0, (k > 0) & (j == 0), (k > 0) & (j > 0) & (i == 0),
+              (k > 0) & (j > 0) & (i > 0) & ~inner, inner]
+    start = 0
+    for piece, size in zip(pieces, (25, 20, 16, 37, 27)):
+        assert piece.sum() == size
+        assert np.array_equal(slot[L[piece]], np.arange(start, start + size))  # contiguous, in L order
+        start += size
