@@ -67,20 +67,22 @@ def tune_element_passes(mesh, model, sources, receivers, dt, nstep=20, candidate
         return (fe[0] + fa[0]) / max(fe[1], 1), (ae[0] + aa[0]) / max(ae[1], 1)
 
     # the pipelined passes (3, 5) can finish the element-interior points themselves: the
-    # element pass grows, the gather shrinks; whole element + assembly time decides
-    if reg in (3, 5):
+    # element pass grows, the gather shrinks, so whole element + assembly time decides,
+    # against the best shape above whatever its regular_path
+    pipe = [g for g in (3, 5) if g in regs] if mesh.nspec_irregular == 0 else []
+    if pipe:
+        ireg = min(pipe, key=lambda g: min(r[2] for r in table if r[0] == g) + min(r[3] for r in table if r[0] == g))
         off = step_ms(**best)
         table.append(("interior-off", best["blocks_per_cu_fwd"], off[0], off[1]))
         # the pass with interior points holds more state: its own workgroup counts
         for bpc in interior_bpc:
-            on = step_ms(**dict(best, blocks_per_cu_fwd=bpc, blocks_per_cu_adj=bpc, interior=True))
+            on = step_ms(regular_path=ireg, blocks_per_cu_fwd=bpc, blocks_per_cu_adj=bpc, interior=True)
             table.append(("interior-on", bpc, on[0], on[1]))
         # one interior switch for both passes: the total of the best forward and adjoint shapes
-        on_total = min(r[2] for r in table if r[0] == "interior-on") + min(r[3] for r in table if r[0] == "interior-on")
-        if on_total < off[0] + off[1]:
-            best["interior"] = True
-            best["blocks_per_cu_fwd"] = min((r for r in table if r[0] == "interior-on"), key=lambda r: r[2])[1]
-            best["blocks_per_cu_adj"] = min((r for r in table if r[0] == "interior-on"), key=lambda r: r[3])[1]
+        ons = [r for r in table if r[0] == "interior-on"]
+        if min(r[2] for r in ons) + min(r[3] for r in ons) < off[0] + off[1]:
+            best = {"regular_path": ireg, "blocks_per_cu_fwd": min(ons, key=lambda r: r[2])[1],
+                    "blocks_per_cu_adj": min(ons, key=lambda r: r[3])[1], "interior": True}
     # brick-fused pass (lattice meshes): element + assembly time per step against the
     # element + assembly time of the best gather shapes
     if bricks and f.lattice:
