@@ -80,6 +80,9 @@ def parse():
                    help="events also run one at a time on the plain single-event path (its rate per "
                         "event-step is taken from these)")
     p.add_argument("--events-per-batch", type=int, default=0, help="0: autotuned")
+    p.add_argument("--batch-interior", type=int, default=0, choices=[0, 1],
+                   help="isotropic batches: 1 element-interior points finished in the element pass, "
+                        "0 every point in the assembly")
     p.add_argument("--hbm-query", type=int, default=1, help="c4: size batches from the free HBM (torch)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-steps", type=int, default=None,
@@ -498,7 +501,8 @@ def batched(a):
     if a.events_per_batch:
         E, bpc, ttable = a.events_per_batch, 8, []
     else:
-        (E, bpc), ttable = tune_batch(mesh, smooth_model, srcs, rec, dt, static_precision=sp, sizes=tuple(fits))
+        (E, bpc), ttable = tune_batch(mesh, smooth_model, srcs, rec, dt, static_precision=sp, sizes=tuple(fits),
+                                      interior=bool(a.batch_interior))
         log("batch tuner (events, blocks/CU, fwd ms/event-step, adj ms/event-step): "
             + ", ".join(f"({e},{b},{x:.3f},{y:.3f})" for e, b, x, y in ttable) + f" -> E={E} bpc={bpc}")
     log(f"batch sizes fitting {budget / 1e9:.0f} GB with the {'fp16' if a.c4_store else 'fp32'} store: {fits}")
@@ -521,6 +525,10 @@ def batched(a):
                        static_precision=sp)
         b.set_tuning(bpc)
         b.set_store_precision(a.c4_store)
+        b.set_interior(a.batch_interior)
+        # points the element pass finishes (the source elements' own, at most 27 per
+        # event, stay in the assembly and are counted with the others here)
+        n_fin[0] = b.run.interior_info()[1] if a.batch_interior else 0
         _lib.call("fwat_synchronize_", b.run.h)
         ts = time.time()
         b.forward(timed=True)
@@ -538,6 +546,7 @@ def batched(a):
         b.close()
         return wall, syn, kern, fwd_t, adj_t
 
+    n_fin = [0]
     bwall, bsyn, bkern, times = 0.0, {}, {}, {}
     check = set(groups[0][:2])  # the two longest events: kernels compared with the plain path
     for gi, g in enumerate(groups):
@@ -596,17 +605,23 @@ def batched(a):
     ncontrib = nloc + nfp + 125 * (e + rec.nrec)
     kmu = 8.0 if sp == 0 else 4.0
     kacc = kernel_acc_bytes(KERNEL_BLOCK)  # the batch handle's partial-sum blocks (library default)
+    n_in = n_fin[0]
+    n_sh = nglob - n_in
     kinds = {
         # statics once (ibool 4 + kappa,mu) per local point; per event: both fields gathered
-        # 24 per unique point, forces of both fields written 24 and the mu/kappa accumulators
-        # 2 x kacc per local point
-        f"k_batch_elem<{e},2>": ((4.0 + kmu) * nloc + e * (24.0 * nglob + (24.0 + 2 * kacc) * nloc),
+        # 24 per unique point, forces of both fields written 24 per local point that the
+        # assembly reads and the mu/kappa accumulators 2 x kacc per local point; for the
+        # points finished here rmass_inv 12 once and per event v of both fields read 24,
+        # u, v of both fields written 48 and the rho kernel kacc
+        f"k_batch_elem<{e},2>": ((4.0 + kmu) * nloc + 12.0 * n_in
+                                 + e * (24.0 * nglob + 2 * kacc * nloc + 24.0 * (nloc - n_in)
+                                        + (72.0 + kacc) * n_in),
                                  atimes[0], atimes[1]),
-        # CSR 4 per point + 4 per contribution, rmass_inv 12 per point once; per event: forces
-        # of both fields 24 per local point, Stacey terms 24 per face point, u,v of both fields
-        # read + written 96 and the rho kernel kacc per unique point
-        f"k_batch_asm<{e},two-field>": (16.0 * nglob + 4.0 * ncontrib
-                                        + e * (24.0 * nloc + 24.0 * nfp + (96.0 + kacc) * nglob),
+        # over the points it assembles: CSR 4 per point + 4 per contribution, rmass_inv 12 per
+        # point once; per event: forces of both fields 24 per element term, Stacey terms 24 per
+        # face point, u,v of both fields read + written 96 and the rho kernel kacc per point
+        f"k_batch_asm<{e},two-field>": (16.0 * n_sh + 4.0 * (ncontrib - n_in)
+                                        + e * (24.0 * (nloc - n_in) + 24.0 * nfp + (96.0 + kacc) * n_sh),
                                         atimes[2], atimes[3]),
     }
     kname = max(kinds, key=lambda k: kinds[k][1])
@@ -633,6 +648,7 @@ def batched(a):
                                 f"{rec.nrec} receivers, dt={dt}; fwd + misfit + adjoint/reconstruction + "
                                 "rho/mu/kappa kernels per event"),
                    "events_per_batch": E, "blocks_per_cu": bpc, "batches": [len(g) for g in groups],
+                   "element_interior_points_finished_in_element_pass": n_in,
                    "tuner": [list(r) for r in ttable]},
         "batched_vs_plain": {"plain_value": pvalue, "speedup": bvalue / pvalue,
                              "plain_events_timed": len(plain),

@@ -246,6 +246,12 @@ void fwat_set_static_precision_(long *Mesh_pointer, int *precision);
 void fwat_batch_create_(long *Mesh_pointer, long *Batch_pointer, int *nevents, int *nstep, realw *DT,
                         realw *h_sourcearrays, int *h_ispec_selected_source, double *h_stf);
 void fwat_batch_set_tuning_(long *Batch_pointer, int *blocks_per_cu);
+/* 1: an isotropic batch's element pass finishes the element-interior points (one element term,
+ * no source of the batch in their element: Newmark of every active event and the density kernel)
+ * and the assembly covers the other points; 0 (default): every point in the assembly.  Bitwise
+ * the same results.  No reference counterpart (the reference runs one event at a time,
+ * fwat_fwd_measure_adj.f90:43-51). */
+void fwat_batch_set_interior_(long *Batch_pointer, int *on);
 /* Stacey store of the batch's next forward run: 0 fp32 (12 bytes per face point, event and step),
  * 1 fp16 with one scale per (step, event) record (6 bytes): the record's largest value maps to
  * 16384, so each stored value keeps 11 significant bits; halves the reconstruction store that

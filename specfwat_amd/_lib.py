@@ -74,6 +74,7 @@ SIGNATURES = {
     "fwat_set_slabs_": "Li",
     "fwat_batch_create_": "LLiIfFID",
     "fwat_batch_set_tuning_": "Li",
+    "fwat_batch_set_interior_": "Li",
     "fwat_batch_set_store_precision_": "Li",
     "fwat_batch_forward_": "Li",
     "fwat_batch_set_adjoint_sources_": "LF",

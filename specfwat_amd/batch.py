@@ -53,6 +53,11 @@ class EventBatch:
     def set_tuning(self, blocks_per_cu=8):
         _lib.call("fwat_batch_set_tuning_", self.h, int(blocks_per_cu))
 
+    def set_interior(self, on=True):
+        """Element-interior points finished in the element pass (isotropic batches;
+        default off; the results are the same bits either way)."""
+        _lib.call("fwat_batch_set_interior_", self.h, int(bool(on)))
+
     def set_store_precision(self, precision):
         """Stacey store of the next forward run: 0 fp32, 1 fp16 with one scale per
         step and event (half the HBM; the reconstruction replays the rounded terms)."""
@@ -159,12 +164,13 @@ def batch_device_bytes(mesh, nevents, nmax, store_precision=0):
             + store + nfp * e * 12 + 4 * (ng + nloc + nfp) + 8 * e * nmax)
 
 
-def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_cu=(4, 8, 16), nstep=12,
-               static_precision=0, hbm_budget=None, cij=None):
+def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_cu=(4, 6, 8, 12, 16), nstep=12,
+               static_precision=0, hbm_budget=None, cij=None, interior=False):
     """Events per launch and persistent workgroups per CU of the batched
     element pass, from short timed runs on the real mesh (anisotropic with
-    cij).  The figure of merit is the element + assembly time per event-step
-    of a forward and an adjoint step.  Returns ((events, blocks_per_cu), table)."""
+    cij; interior: EventBatch.set_interior).  The figure of merit is the
+    element + assembly time per event-step of a forward and an adjoint step.
+    Returns ((events, blocks_per_cu), table)."""
     table = []
     for e in sizes:
         if e > len(sources):
@@ -173,6 +179,7 @@ def tune_batch(mesh, model, sources, receivers, dt, sizes=(2, 4, 8), blocks_per_
             continue
         b = EventBatch(mesh, *model, sources[:e], receivers, dt, [nstep] * e,
                        static_precision=static_precision, cij=cij)
+        b.set_interior(interior)
         b.forward()  # warm-up; fills the Stacey rows the adjoint replays
         b.set_adjoint_sources(np.zeros((e, nstep, b.nrec, 3), F32))
         for bpc in blocks_per_cu:

@@ -48,8 +48,8 @@ struct BatchElemArgs {
   int nspec;
   realw xix_regular, jacobian_regular;
   float scale_k, scale_m;
-  const realw* u0;   // [nglob][E][3]
-  const realw* u1;
+  realw* u0;         // [nglob][E][3]
+  realw* u1;
   realw* floc0;      // [nloc][E][3]
   realw* floc1;
   double* mu_kl;     // [E][nloc]
@@ -62,6 +62,18 @@ struct BatchElemArgs {
   const realw* cij;  // [nspec][21][125] (anisotropic)
   double* cijkl_kl;  // [E][nspec][21][125]
   float* cij_part;   // [E][nspec][21][125] fp32 partial sums
+  // element-interior points (k_batch_elem<..., true>): a point >= n_shared of an
+  // element that holds none of the batch's sources has this element's term only,
+  // so the pass finishes it (Newmark of every active event, density kernel)
+  int n_shared;
+  int n_src;
+  int src_el[EMAX];  // the batch's source elements (their points are gathered)
+  int last_mask;     // bit e set: this is event e's last step of the run
+  const realw* rm;   // [nglob][3] inverse mass
+  realw *v0, *a0, *v1, *a1;  // [nglob][E][3] (field 1 runs backwards in time)
+  double* rho_kl;    // [nglob][E]
+  float* rho_part;   // [nglob][E]
+  realw fdt;         // field time step
 };
 
 __device__ __forceinline__ float bf16f(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
@@ -107,14 +119,77 @@ __device__ __forceinline__ void store_pt(realw* __restrict__ p, const realw (&x)
   }
 }
 
+// Newmark finish of event ev at a point whose only term is one element's: the
+// element pass holds the point's forces f0, f1 and displacements u0, u1; its inverse
+// mass r, velocities v0, v1 and density-kernel partial sum were loaded ahead.  Same
+// expressions and rounding as k_batch_asm (the gather sums such a row as 0 + term).
+template <int E, bool TWO>
+__device__ __forceinline__ void batch_point_newmark(const BatchElemArgs& A, int g, int ev, const realw (&r)[3],
+                                                    const realw (&f0)[3], const realw (&f1)[3],
+                                                    const realw (&u0)[3], const realw (&u1)[3],
+                                                    const realw (&v0)[3], const realw (&v1)[3], float part) {
+  const size_t o = (size_t)g * (3 * E) + 3 * ev;
+  const bool last = (A.last_mask >> ev) & 1;
+  const realw z = 0.0f, d0 = A.fdt, h0 = 0.5f * A.fdt, s0 = 0.5f * A.fdt * A.fdt;
+  realw Ac[3], V[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Ac[c] = (z + f0[c]) * r[c];
+    V[c] = fmaf(h0, Ac[c], v0[c]);
+  }
+  if (TWO) {
+    const realw d1 = -A.fdt, h1 = 0.5f * d1, s1 = 0.5f * d1 * d1;
+    realw BA[3], BV[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      BA[c] = (z + f1[c]) * r[c];
+      BV[c] = fmaf(h1, BA[c], v1[c]);
+    }
+    // kernel_acc with the partial sum loaded ahead
+    const float x = fmaf(Ac[2], u1[2], fmaf(Ac[1], u1[1], Ac[0] * u1[0]));
+    const size_t q = (size_t)g * E + ev;
+    if (!A.rho_part) {
+      A.rho_kl[q] = fma(A.dt, (double)x, A.rho_kl[q]);
+    } else if (A.kflush) {
+      A.rho_kl[q] = fma(A.dt, (double)(part + x), A.rho_kl[q]);
+      A.rho_part[q] = 0.0f;
+    } else {
+      A.rho_part[q] = part + x;
+    }
+    if (last) {
+      store3(A.a1 + o, BA[0], BA[1], BA[2]);
+      store3(A.v1 + o, BV[0], BV[1], BV[2]);
+    } else {
+      store3(A.u1 + o, fmaf(s1, BA[0], fmaf(d1, BV[0], u1[0])), fmaf(s1, BA[1], fmaf(d1, BV[1], u1[1])),
+             fmaf(s1, BA[2], fmaf(d1, BV[2], u1[2])));
+      store3(A.v1 + o, fmaf(h1, BA[0], BV[0]), fmaf(h1, BA[1], BV[1]), fmaf(h1, BA[2], BV[2]));
+    }
+  }
+  if (last) {
+    store3(A.a0 + o, Ac[0], Ac[1], Ac[2]);
+    store3(A.v0 + o, V[0], V[1], V[2]);
+  } else {
+    store3(A.u0 + o, fmaf(s0, Ac[0], fmaf(d0, V[0], u0[0])), fmaf(s0, Ac[1], fmaf(d0, V[1], u0[1])),
+           fmaf(s0, Ac[2], fmaf(d0, V[2], u0[2])));
+    store3(A.v0 + o, fmaf(h0, Ac[0], V[0]), fmaf(h0, Ac[1], V[1]), fmaf(h0, Ac[2], V[2]));
+  }
+}
+
 // Element pass of E events (NF = 1 forward, NF = 2 adjoint + reconstruction
 // with in-pass mu/kappa kernels), regular iso meshes, folded tables,
 // persistent XCD-chunked schedule.  The fields of all events are loaded
 // together (3E contiguous floats per lane), then the events are processed one
 // after the other out of LDS; each event's forces overwrite its own LDS slot
 // and are stored together at the end (3E contiguous floats per lane).
-template <int E, int NF, int SP>
-__global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
+// INT: the element-interior points are finished here (batch_point_newmark) and
+// their rows, the last 27 of each element row, are not stored.
+// (the waves-per-SIMD floor is what the pass without the interior points reaches,
+// so that their Newmark tail does not cost occupancy)
+constexpr int batch_elem_waves(int E, int NF) { return NF == 2 ? (E >= 8 ? 3 : 4) : (E >= 8 ? 5 : E == 4 ? 6 : 7); }
+
+template <int E, int NF, int SP, bool INT>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(INT ? 1 : batch_elem_waves(E, NF))))
+void k_batch_elem(BatchElemArgs A) {
   __shared__ realw s_tab[128];
   __shared__ realw s_u[NF][E][3][128];
   __shared__ realw s_s[NF][6][128];
@@ -151,6 +226,9 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
   for (int e = x * chunk + slot; e < e_end; e += per_xcd) {
     const size_t Lg = (size_t)e * 125 + tt;
     const int iglob = A.ibool[Lg];
+    bool src = false;  // uniform over the workgroup
+    if (INT)
+      for (int k = 0; k < A.n_src; ++k) src |= e == A.src_el[k];
     realw kl, ml;
     if (SP == SEM_STATIC_FP32) {
       kl = A.kappa[Lg];
@@ -177,10 +255,29 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
         }
       }
     }
+    const bool inner = INT && act && !src && iglob >= A.n_shared;
+    realw rmi[3] = {0.0f, 0.0f, 0.0f};
+    if (INT && inner) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) rmi[c] = A.rm[3 * (size_t)iglob + c];
+    }
     __syncthreads();
 #pragma unroll 1
     for (int ev = 0; ev < E; ++ev) {
       if (!((mask >> ev) & 1)) continue;  // uniform over the grid
+      // an interior point's velocities and density-kernel partial sum for this event,
+      // loaded now so that they arrive during the contractions
+      realw pv0[3] = {0.0f, 0.0f, 0.0f}, pv1[3] = {0.0f, 0.0f, 0.0f};
+      float ppart = 0.0f;
+      if (INT && inner) {
+        const size_t o = (size_t)iglob * (3 * E) + 3 * ev;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          pv0[c] = A.v0[o + c];
+          if (NF == 2) pv1[c] = A.v1[o + c];
+        }
+        if (NF == 2 && A.rho_part) ppart = A.rho_part[(size_t)iglob * E + ev];
+      }
       realw ea0 = 0, ea1 = 0, ea2 = 0, ea3 = 0, ea4 = 0, ea5 = 0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -214,6 +311,7 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
         s_s[f][3][t] = ml * exy2; s_s[f][4][t] = ml * exz2; s_s[f][5][t] = ml * eyz2;
       }
       __syncthreads();  // stresses complete; every lane is done reading s_u[*][ev]
+      realw ff[NF][3];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         realw fx = 0, fy = 0, fz = 0;
@@ -224,11 +322,25 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
           fx += s_s[f][3][Lb] * hwJ[l]; fy += s_s[f][1][Lb] * hwJ[l]; fz += s_s[f][5][Lb] * hwJ[l];
           fx += s_s[f][4][Lc] * hwK[l]; fy += s_s[f][5][Lc] * hwK[l]; fz += s_s[f][2][Lc] * hwK[l];
         }
-        s_u[f][ev][0][t] = fx; s_u[f][ev][1][t] = fy; s_u[f][ev][2][t] = fz;  // own slot only
+        ff[f][0] = fx; ff[f][1] = fy; ff[f][2] = fz;
+      }
+      if (INT && inner) {
+        // own slot: this event's displacements, before the forces replace them
+        const realw u0[3] = {s_u[0][ev][0][t], s_u[0][ev][1][t], s_u[0][ev][2][t]};
+        const realw u1[3] = {s_u[NF - 1][ev][0][t], s_u[NF - 1][ev][1][t], s_u[NF - 1][ev][2][t]};
+        batch_point_newmark<E, NF == 2>(A, iglob, ev, rmi, ff[0], ff[NF - 1], u0, u1, pv0, pv1, ppart);
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        s_u[f][ev][0][t] = ff[f][0]; s_u[f][ev][1][t] = ff[f][1]; s_u[f][ev][2][t] = ff[f][2];  // own slot only
       }
       __syncthreads();  // s_s free for the next event
     }
-    if (act) {
+    const int fs = sem_floc_slot(tt);
+    // the 27 inner slots sit at the end of the element row, so leaving the rows of the
+    // points finished here out leaves the rest whole (a free-surface element's top
+    // face points that have no other term lie mid-row and are stored anyway)
+    if (act && !(inner && fs >= 98)) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         realw out[3 * E];
@@ -238,7 +350,7 @@ __global__ __launch_bounds__(128) void k_batch_elem(BatchElemArgs A) {
           out[3 * ev + 1] = s_u[f][ev][1][t];
           out[3 * ev + 2] = s_u[f][ev][2][t];
         }
-        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + (Lg - tt + sem_floc_slot(tt)) * (3 * E), out);
+        store_pt<E>((f == 0 ? A.floc0 : A.floc1) + (Lg - tt + fs) * (3 * E), out);
       }
     }
   }
@@ -404,6 +516,9 @@ struct BatchAsmArgs {
   const realw* adj;    // [E][NSTEP_max][nrec][3]
   realw dt;
   double dtk;
+  int n_main;          // points [0, n_main) (all of them, or the shared ones when the
+  const int* extra;    // element pass finishes the rest), then the n_extra listed ones
+  int n_extra;
 };
 
 // Gather assembly + Newmark, one lane per (point, event).  TWO: adjoint field
@@ -412,9 +527,12 @@ struct BatchAsmArgs {
 // (event = p / 125), receiver points; summed in that fixed order.
 template <int E, bool TWO>
 __global__ void k_batch_asm(BatchAsmArgs A) {
-  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (size_t)A.nglob * E) return;
-  const int g = (int)(q / E), ev = (int)(q % E);
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nm = (size_t)A.n_main * E;
+  if (l >= nm + (size_t)A.n_extra * E) return;
+  const int g = l < nm ? (int)(l / E) : A.extra[(l - nm) / E];
+  const int ev = (int)(l % E);  // nm is a multiple of E
+  const size_t q = (size_t)g * E + ev;
   const int ne = A.nstep[ev];
   const bool active = TWO ? (A.it > A.nmax - ne) : (A.it <= ne);
   if (!active) return;
@@ -470,6 +588,8 @@ __global__ void k_batch_asm(BatchAsmArgs A) {
       f0x += h * R[0]; f0y += h * R[1]; f0z += h * R[2];
     }
   }
+  // explicit fmas: the element pass finishes its points with the same expressions
+  // (batch_point_newmark), so both give the same bits whatever the compiler contracts
   const size_t b = 3 * (size_t)g;
   const realw r0 = A.rm[b], r1 = A.rm[b + 1], r2 = A.rm[b + 2];
   const realw dt = A.dt, h0 = 0.5f * dt, s0 = 0.5f * dt * dt;
@@ -477,23 +597,25 @@ __global__ void k_batch_asm(BatchAsmArgs A) {
   const realw Ac[3] = {f0x * r0, f0y * r1, f0z * r2};
   realw V[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) V[c] = A.v0[o + c] + h0 * Ac[c];
+  for (int c = 0; c < 3; ++c) V[c] = fmaf(h0, Ac[c], A.v0[o + c]);
   if (TWO) {
+    const realw d1 = -dt, h1 = 0.5f * d1, s1 = 0.5f * d1 * d1;
     const realw BA[3] = {f1x * r0, f1y * r1, f1z * r2};
     realw BV[3], BU[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { BV[c] = A.v1[o + c] - h0 * BA[c]; BU[c] = A.u1[o + c]; }
-    kernel_acc(A.rho_kl, A.rho_part, q, Ac[0] * BU[0] + Ac[1] * BU[1] + Ac[2] * BU[2], A.dtk, A.kflush != 0);
+    for (int c = 0; c < 3; ++c) { BV[c] = fmaf(h1, BA[c], A.v1[o + c]); BU[c] = A.u1[o + c]; }
+    const float x = fmaf(Ac[2], BU[2], fmaf(Ac[1], BU[1], Ac[0] * BU[0]));
+    kernel_acc(A.rho_kl, A.rho_part, q, x, A.dtk, A.kflush != 0);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if (last) { A.a1[o + c] = BA[c]; A.v1[o + c] = BV[c]; }
-      else { A.u1[o + c] = BU[c] - dt * BV[c] + s0 * BA[c]; A.v1[o + c] = BV[c] - h0 * BA[c]; }
+      else { A.u1[o + c] = fmaf(s1, BA[c], fmaf(d1, BV[c], BU[c])); A.v1[o + c] = fmaf(h1, BA[c], BV[c]); }
     }
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     if (last) { A.a0[o + c] = Ac[c]; A.v0[o + c] = V[c]; }
-    else { A.u0[o + c] = A.u0[o + c] + dt * V[c] + s0 * Ac[c]; A.v0[o + c] = V[c] + h0 * Ac[c]; }
+    else { A.u0[o + c] = fmaf(s0, Ac[c], fmaf(dt, V[c], A.u0[o + c])); A.v0[o + c] = fmaf(h0, Ac[c], V[c]); }
   }
 }
 
@@ -658,7 +780,17 @@ struct SemBatch {
   double elem_ms = 0, asm_ms = 0;
   int elem_n = 0, asm_n = 0;
   int blocks_per_cu = 8;
+  int interior = 0;                                   // element pass finishes the interior points
+  std::vector<int> src_el;                            // distinct source elements of the batch
+  int* d_extra = nullptr;                             // interior points of those elements (gathered)
+  int n_extra = 0;
 };
+
+// true when the element pass finishes the points [n_shared, nglob) outside the
+// source elements (isotropic batches on a mesh numbered with interior points)
+bool batch_interior(const SemBatch* B) {
+  return B->interior && !B->aniso && B->m->n_shared < B->m->nglob;
+}
 
 SemBatch* batch_of(long* p) {
   SemBatch* b = reinterpret_cast<SemBatch*>(*p);
@@ -667,7 +799,7 @@ SemBatch* batch_of(long* p) {
 }
 
 template <int E>
-void launch_elem(SemBatch* B, int nf, const int* active) {
+void launch_elem(SemBatch* B, int nf, const int* active, const int* last) {
   SemMesh* m = B->m;
   BatchElemArgs A{};
   A.ibool = m->d_ibool; A.kappa = m->d_kappa; A.mu = m->d_mu;
@@ -680,15 +812,30 @@ void launch_elem(SemBatch* B, int nf, const int* active) {
   A.mu_kl = B->mu_kl; A.kappa_kl = B->kappa_kl; A.dt = (double)B->dt;
   A.mu_part = B->mu_part; A.kappa_part = B->kappa_part; A.kflush = B->kflush;
   A.cij = m->d_cij; A.cijkl_kl = B->cijkl_kl; A.cij_part = B->cij_part;
-  A.active_mask = 0;
-  for (int e = 0; e < E; ++e) A.active_mask |= active[e] ? 1 << e : 0;
+  A.active_mask = A.last_mask = 0;
+  for (int e = 0; e < E; ++e) {
+    A.active_mask |= active[e] ? 1 << e : 0;
+    A.last_mask |= last[e] ? 1 << e : 0;
+  }
+  const bool in = batch_interior(B);
+  if (in) {
+    A.n_shared = m->n_shared;
+    A.n_src = (int)B->src_el.size();
+    for (int k = 0; k < A.n_src; ++k) A.src_el[k] = B->src_el[k];
+    A.rm = m->d_rmass_inv;
+    A.v0 = B->v; A.a0 = B->a; A.v1 = B->bv; A.a1 = B->ba;
+    A.rho_kl = B->rho_kl; A.rho_part = B->rho_part;
+    A.fdt = B->dt;
+  }
   int dev = 0, ncu = 0;
   HIP_CHECK(hipGetDevice(&dev));
   HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int nb = std::max(8, std::min((m->nspec + 7) & ~7, (ncu * B->blocks_per_cu + 7) & ~7));
   const int sp = m->static_precision;
   hipStream_t s = m->stream;
-#define LB(NF, SP) SEM_LAUNCH((k_batch_elem<E, NF, SP>), dim3(nb), dim3(128), 0, s, A)
+#define LB(NF, SP)                                                                \
+  if (in) SEM_LAUNCH((k_batch_elem<E, NF, SP, true>), dim3(nb), dim3(128), 0, s, A);  \
+  else SEM_LAUNCH((k_batch_elem<E, NF, SP, false>), dim3(nb), dim3(128), 0, s, A)
   if (B->aniso) {
     if (nf == 2) SEM_LAUNCH((k_batch_elem_aniso<E, 2>), dim3(nb), dim3(128), 0, s, A);
     else SEM_LAUNCH((k_batch_elem_aniso<E, 1>), dim3(nb), dim3(128), 0, s, A);
@@ -705,12 +852,12 @@ void launch_elem(SemBatch* B, int nf, const int* active) {
   HIP_CHECK(hipGetLastError());
 }
 
-void elem_pass(SemBatch* B, int nf, const int* active) {
+void elem_pass(SemBatch* B, int nf, const int* active, const int* last) {
   switch (B->E) {
-    case 1: launch_elem<1>(B, nf, active); break;
-    case 2: launch_elem<2>(B, nf, active); break;
-    case 4: launch_elem<4>(B, nf, active); break;
-    case 8: launch_elem<8>(B, nf, active); break;
+    case 1: launch_elem<1>(B, nf, active, last); break;
+    case 2: launch_elem<2>(B, nf, active, last); break;
+    case 4: launch_elem<4>(B, nf, active, last); break;
+    case 8: launch_elem<8>(B, nf, active, last); break;
     default: SEM_FAIL("batch size %d not supported (1, 2, 4, 8)", B->E);
   }
 }
@@ -726,6 +873,10 @@ void asm_pass(SemBatch* B, bool two, int it) {
   A.rho_kl = B->rho_kl; A.nstep = B->d_nstep; A.stf = B->d_stf; A.adj = B->adj;
   A.rho_part = B->rho_part; A.kflush = B->kflush;
   A.dt = B->dt; A.dtk = (double)B->dt;
+  const bool in = batch_interior(B);
+  A.n_main = in ? m->n_shared : m->nglob;
+  A.extra = in ? B->d_extra : nullptr;
+  A.n_extra = in ? B->n_extra : 0;
   const size_t row = B->nfp * B->E * 3;
   if (two) {
     A.face0 = B->face_scratch;
@@ -733,7 +884,7 @@ void asm_pass(SemBatch* B, bool two, int it) {
   } else {
     A.face0 = B->store_precision ? B->face_scratch : B->store + (size_t)(it - 1) * row;
   }
-  const dim3 gr(nblk((size_t)m->nglob * B->E)), bl(TPB);
+  const dim3 gr(nblk(((size_t)A.n_main + A.n_extra) * B->E)), bl(TPB);
   hipStream_t s = m->stream;
 #define LA(EE)                                                           \
   if (two) SEM_LAUNCH((k_batch_asm<EE, true>), gr, bl, 0, s, A);       \
@@ -879,6 +1030,18 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
     for (int L = 0; L < 125; ++L) owner[nloc + nfp + (size_t)e * 125 + L] = ib[(size_t)el * 125 + L];
   }
   for (size_t q = 0; q < ri.size(); ++q) owner[nloc + nfp + nsp + q] = ri[q];
+  // the source elements' points have terms beyond their element's: the gather
+  // assembles the ones the element pass would otherwise finish
+  for (int e = 0; e < E; ++e) {
+    const int el = h_ispec_selected_source[e] - 1;
+    if (std::find(B->src_el.begin(), B->src_el.end(), el) == B->src_el.end()) B->src_el.push_back(el);
+  }
+  std::vector<int> extra;
+  for (int el : B->src_el)
+    for (int L = 0; L < 125; ++L)
+      if (ib[(size_t)el * 125 + L] >= m->n_shared) extra.push_back(ib[(size_t)el * 125 + L]);
+  B->n_extra = (int)extra.size();
+  if (B->n_extra) B->d_extra = dev_upload(extra.data(), extra.size(), s);
   // every point's element terms in the mesh's phase-loop order, as the single-event path
   std::vector<int> visit = m->h_visit, off, idx;
   if (visit.empty()) {
@@ -906,6 +1069,17 @@ void fwat_batch_create_(long* Mesh_pointer, long* Batch_pointer, int* nevents, i
   for (realw* p : {B->u, B->v, B->a}) hipLaunchKernelGGL(k_fill, dim3(nblk(nf3)), dim3(TPB), 0, s, p, nf3);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// 1: isotropic batches finish the element-interior points (one element term, no
+// source) in the element pass and assemble the rest; 0 (default): every point in
+// the assembly.  Same results either way; on 64x64x32 elements with 2 and 4 events
+// per launch the adjoint pass is slower with 1 (each event's 12-byte point pieces
+// are strided by the event rows), the forward pass the same.
+void fwat_batch_set_interior_(long* Batch_pointer, int* on) {
+  SemBatch* B = batch_of(Batch_pointer);
+  if (*on != 0 && *on != 1) SEM_FAIL("interior must be 0 or 1");
+  B->interior = *on;
 }
 
 void fwat_batch_set_tuning_(long* Batch_pointer, int* blocks_per_cu) {
@@ -942,11 +1116,14 @@ void fwat_batch_forward_(long* Batch_pointer, int* timed) {
   B->elem_ms = B->asm_ms = 0;
   B->elem_n = B->asm_n = 0;
   PassTimer* tm = *timed ? new PassTimer(N) : nullptr;
-  int active[EMAX] = {0};
+  int active[EMAX] = {0}, last[EMAX] = {0};
   for (int it = 1; it <= N; ++it) {
-    for (int e = 0; e < E; ++e) active[e] = it <= B->nstep[e];
+    for (int e = 0; e < E; ++e) {
+      active[e] = it <= B->nstep[e];
+      last[e] = it == B->nstep[e];
+    }
     if (tm) tm->mark(it, 0, s);
-    elem_pass(B, 1, active);
+    elem_pass(B, 1, active, last);
     if (tm) tm->mark(it, 1, s);
     stacey_pass(B, it, false);
     if (m->nrec)
@@ -1030,12 +1207,15 @@ void fwat_batch_adjoint_(long* Batch_pointer, int* timed) {
   B->elem_ms = B->asm_ms = 0;
   B->elem_n = B->asm_n = 0;
   PassTimer* tm = *timed ? new PassTimer(N) : nullptr;
-  int active[EMAX] = {0};
+  int active[EMAX] = {0}, last[EMAX] = {0};
   for (int it = 1; it <= N; ++it) {
-    for (int e = 0; e < E; ++e) active[e] = it > N - B->nstep[e];
+    for (int e = 0; e < E; ++e) {
+      active[e] = it > N - B->nstep[e];
+      last[e] = it == N;
+    }
     B->kflush = kb <= 1 || it % kb == 0 || it == N;  // every event is active at it == N
     if (tm) tm->mark(it, 0, s);
-    elem_pass(B, 2, active);
+    elem_pass(B, 2, active, last);
     if (tm) tm->mark(it, 1, s);
     stacey_pass(B, it, true);
     if (tm) tm->mark(it, 2, s);
@@ -1129,6 +1309,7 @@ void fwat_batch_destroy_(long* Batch_pointer) {
   SemBatch* B = reinterpret_cast<SemBatch*>(*Batch_pointer);
   if (!B) return;
   HIP_CHECK(hipStreamSynchronize(B->m->stream));
+  dev_free(B->d_extra);
   dev_free(B->d_nstep); dev_free(B->d_off); dev_free(B->d_idx); dev_free(B->d_sarr); dev_free(B->d_stf);
   dev_free(B->u); dev_free(B->v); dev_free(B->a); dev_free(B->bu); dev_free(B->bv); dev_free(B->ba);
   dev_free(B->floc0); dev_free(B->floc1); dev_free(B->face_scratch); dev_free(B->store);

@@ -65,11 +65,13 @@ def single_event(c, e, src, rec):
     return syn, k
 
 
-def run_batch(c, srcs, rec, blocks_per_cu=None):
+def run_batch(c, srcs, rec, blocks_per_cu=None, interior=None):
     nsteps = [ev["nstep"] for ev in c["events"]]
     b = EventBatch(c["mesh"], c["rho"], c["vp"], c["vs"], srcs, rec, c["dt"], nsteps)
     if blocks_per_cu is not None:
         b.set_tuning(blocks_per_cu)
+    if interior is not None:
+        b.set_interior(interior)
     b.forward()
     syn = b.seismograms()
     adj = np.zeros_like(syn)
@@ -103,6 +105,29 @@ def test_batch_matches_single_event_runs(nsteps):
         assert es < 1e-5
         for q, v in ek.items():
             assert v < 1e-4, q
+
+
+@pytest.mark.parametrize("nsteps", [[70, 55], [70, 45, 90, 60], [80, 60, 100, 70, 90, 50, 75, 95]])
+def test_batch_interior_points_finished_by_the_element_pass_are_bitwise_the_gather(nsteps):
+    # the element pass finishes the points with one element term (Newmark of each
+    # active event, the last-step rows, the density kernel) in the gather's
+    # expressions; the source elements' points stay in the gather.  Events 0 and 1
+    # share their source element (one element listed once).
+    c = make_batch_case(len(nsteps), nsteps, seed=20 + len(nsteps), nex=5, ney=5, nez=4)
+    c["events"][1]["sxyz"] = [list(np.asarray(c["events"][0]["sxyz"][0]) + [1.0, -1.0, 1.0])]
+    m = c["mesh"]
+    rec = Receivers(m, c["rxyz"])
+    srcs = [Sources(m, ev["sxyz"], ev["mt"], ev["f0"]) for ev in c["events"]]
+    assert srcs[0].ispec[0] == srcs[1].ispec[0]
+    probe = ElasticRun(m, c["rho"], c["vp"], c["vs"], None, rec, c["dt"], 10, 1, False)
+    n_shared, n_interior = probe.interior_info()[:2]
+    probe.close()
+    assert n_interior > 0 and n_shared + n_interior == m.nglob
+    on = run_batch(c, srcs, rec, interior=True)
+    off = run_batch(c, srcs, rec, interior=False)
+    assert np.any(on[0]) and np.array_equal(on[0], off[0])
+    for q in ("rho_kl", "mu_kl", "kappa_kl"):
+        assert np.any(on[1][q]) and np.array_equal(on[1][q], off[1][q]), q
 
 
 def test_batch_event_matches_oracle():
